@@ -1,0 +1,92 @@
+"""ctypes binding of libgjw.so (the HIP kernels and their C ABI, include/gjw.h).
+
+The library is built in-tree by ``gerjoii_amd/csrc/Makefile`` (see
+``gerjoii_amd.build``).  Loading fails loudly when it is missing: there is no
+CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgjw.so")
+
+GJW_OK = 0
+GJW_PATH_BATCHED, GJW_PATH_PLAIN = 0, 1
+GJW_OPT_PATH, GJW_OPT_CKPT_EVERY, GJW_OPT_TILE_L, GJW_OPT_SNAPSHOT = 1, 2, 3, 4
+GJW_SNAP_F32, GJW_SNAP_BF16, GJW_SNAP_F16 = 0, 1, 2
+GJW_MISFIT_L2 = 0
+GJW_GRAD_EXACT, GJW_GRAD_REF = 0, 1
+
+# every symbol include/gjw.h declares
+EXPORTS = ["gjw_create", "gjw_destroy", "gjw_last_error", "gjw_version", "gjw_survey_create",
+           "gjw_survey_destroy", "gjw_set_model", "gjw_set_shots", "gjw_set_option",
+           "gjw_forward", "gjw_misfit", "gjw_gradient", "gjw_fwi_gradient",
+           "gjw_differentiate_cube", "gjw_last_timing"]
+
+
+class GjwGrid(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("pml", C.c_int32), ("air", C.c_int32),
+                ("nt", C.c_int32), ("pml_order", C.c_int32), ("dx", C.c_double),
+                ("dt", C.c_double), ("eps0", C.c_double), ("mu0", C.c_double),
+                ("R0", C.c_double)]
+
+
+class GjwError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load libgjw.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GjwError(f"{LIB_PATH} is missing: build it with gerjoii_amd.build() "
+                       "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    i32, i64 = C.c_int32, C.c_int64
+    lib.gjw_last_error.restype = C.c_char_p
+    lib.gjw_version.restype = C.c_char_p
+    lib.gjw_create.argtypes = [i32, C.POINTER(vp)]
+    lib.gjw_destroy.argtypes = [vp]
+    lib.gjw_survey_create.argtypes = [vp, C.POINTER(GjwGrid), C.POINTER(vp)]
+    lib.gjw_survey_destroy.argtypes = [vp]
+    lib.gjw_set_model.argtypes = [vp, vp, vp]
+    lib.gjw_set_shots.argtypes = [vp, i32, vp, vp, vp, vp]
+    lib.gjw_set_option.argtypes = [vp, i32, i64]
+    lib.gjw_forward.argtypes = [vp, vp, vp, i32]
+    lib.gjw_misfit.argtypes = [vp, vp, i32, vp, vp]
+    lib.gjw_gradient.argtypes = [vp, vp, i32, vp, vp, vp, vp]
+    lib.gjw_fwi_gradient.argtypes = [vp, vp, vp, vp, vp]
+    lib.gjw_differentiate_cube.argtypes = [vp, vp, i32, i32, i32, C.c_double]
+    lib.gjw_last_timing.argtypes = [vp, vp, vp]
+    for name in EXPORTS:
+        getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
+    lib.gjw_last_error.restype = C.c_char_p
+    lib.gjw_version.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(rc, what=""):
+    if rc != GJW_OK:
+        msg = load().gjw_last_error().decode(errors="replace")
+        raise GjwError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def ptr(x):
+    """Address of a numpy array or torch tensor (host or device), or None."""
+    if x is None:
+        return None
+    if hasattr(x, "data_ptr"):
+        return C.c_void_p(x.data_ptr())
+    if hasattr(x, "ctypes"):
+        return C.c_void_p(x.ctypes.data)
+    raise TypeError(type(x))

@@ -1,0 +1,1038 @@
+// gjw_api.hip -- host driver and C ABI of libgjw (declared in include/gjw.h).
+//
+// Owns the per-survey device memory, builds the coefficient arrays from the
+// model (src/w/w_exp2pml.m, w_cpml.m, w_coeff.m restated in FP64 on the host),
+// and sequences the kernels of gjw_kernels.hip:
+//   forward:  nt x k_fwd_step (all shots per launch), checkpoint every K steps
+//   gradient: per segment (last to first) restore checkpoint, recompute K
+//             steps storing Ez frames, K x k_adj_step, flush FP32 partial
+//             sums of the correlation into the FP64 gradient.
+// The plain path runs the same kernels one shot at a time and stores the
+// whole domain wavefield, as src/w/w_solve.m:76,102 does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gjw.h"
+#include "gjw_internal.h"
+
+namespace gjw {
+// kernels (gjw_kernels.hip)
+__global__ void k_fwd_step(DevGrid, DevShots, int, const float*, const float*, const float*,
+                           float*, float*, float*, float*, float*, int, float*, float*,
+                           long long);
+__global__ void k_adj_step(DevGrid, DevShots, int, const float*, const float*, const float*,
+                           const float*, const float*, float*, float*, float*, float*, float*,
+                           int, const float*, const float*, const float*, long long, float*,
+                           long long, float*);
+__global__ void k_fill(float*, long long, float);
+__global__ void k_fill_d(double*, long long, double);
+__global__ void k_inject_last(DevGrid, int, int, int, const int*, const int*, const int*,
+                              const float*, int, float*);
+__global__ void k_gather(const float*, long long, const int*, int, int, float*);
+__global__ void k_scatter(const float*, long long, const int*, int, int, float*);
+__global__ void k_domain_frame(DevGrid, int, const float*, float*, long long);
+__global__ void k_flush(int, int, int, float*, long long, const double*, const double*,
+                        double*, double*);
+__global__ void k_resid_l2(const float*, const float*, float*, long long, double*);
+__global__ void k_reduce_d(const double*, int, double, double*);
+__global__ void k_dcube(float*, long long, int, double);
+}  // namespace gjw
+
+using namespace gjw;
+
+// --------------------------------------------------------------------------
+// errors
+// --------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? GJW_ERR_MEM : GJW_ERR_HIP, "%s: %s (%s:%d)", \
+                  #expr, hipGetErrorString(e_), __FILE__, __LINE__);                   \
+  } while (0)
+
+#define TRY(expr)            \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != GJW_OK) return r_; \
+  } while (0)
+
+// --------------------------------------------------------------------------
+// objects
+// --------------------------------------------------------------------------
+struct gjw_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+template <class T>
+struct DBuf {  // device buffer, grows on demand
+  T* p = nullptr;
+  size_t n = 0;
+  int ensure(size_t cnt) {
+    if (cnt <= n && p) return GJW_OK;
+    if (p) {
+      HIPCHK(hipFree(p));
+      p = nullptr;
+      n = 0;
+    }
+    if (cnt == 0) cnt = 1;
+    HIPCHK(hipMalloc(&p, cnt * sizeof(T)));
+    n = cnt;
+    return GJW_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct gjw_survey {
+  gjw_ctx* ctx = nullptr;
+  gjw_grid gr{};
+  int NX = 0, NY = 0, P = 0, LD = 0, di0 = 0, dj0 = 0;
+  long long FS = 0;
+  bool have_model = false, have_shots = false;
+  // options
+  int path = GJW_PATH_BATCHED, ck_every = 0, tile_L = 0, snap = GJW_SNAP_F32;
+  int tuned_L = 0;
+  // model, padded (NX x NY, column-major, FP64)
+  std::vector<double> pe, ps;
+  float bx0 = 0.f, by0 = 0.f;
+  DBuf<float> coef;  // ce, ch, cxe, cxh, cye, cyh, hax, hbx, hay, hby (10*FS)
+  DBuf<double> fe, fs;
+  std::vector<int> pml_idx;
+  DBuf<int> d_pml;
+  // shots
+  int ns = 0, ntr = 0;
+  std::vector<int> si, sj, tr_shot, tr_row, tr_col;
+  std::vector<long long> rec_ptr;
+  std::vector<float> sc;
+  DBuf<int> d_si, d_sj, d_cptr, d_crow, d_cid, d_trs, d_trr, d_trc;
+  DBuf<float> d_sc, d_wav;
+  // state
+  DBuf<float> fld;   // forward: E0,Hx0,Hy0,E1,Hx1,Hy1,X,Y  (8 * ns * FS)
+  DBuf<float> adjf;  // adjoint: M0,Nx0,Ny0,X0,Y0,M1,Nx1,Ny1,X1,Y1 (10 * ns * FS)
+  DBuf<float> tr, adj, obs;
+  DBuf<float> ck;    // checkpoints
+  DBuf<float> frames;
+  DBuf<float> acc;
+  DBuf<double> ge, gsg, part, Jd, gw;
+  int ck_K = 0, ck_n = 0, kept = 0;  // kept: 1 batched checkpoints, 2 plain wavefield
+  int cur = 0;                       // which forward buffer holds the current state
+  bool have_adj = false;
+  double last_ms = 0, last_cells = 0;
+};
+
+static inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
+
+static int host_or_dev_copy(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return GJW_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, st));
+  return GJW_OK;
+}
+
+// --------------------------------------------------------------------------
+// context
+// --------------------------------------------------------------------------
+extern "C" const char* gjw_last_error(void) { return g_err.c_str(); }
+extern "C" const char* gjw_version(void) { return "gjw 0.1 (gfx950)"; }
+
+extern "C" int gjw_create(int32_t device, gjw_ctx** out) {
+  if (!out) return fail(GJW_ERR_ARG, "gjw_create: out is NULL");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return fail(GJW_ERR_ARG, "gjw_create: device %d out of range (%d devices)", device, n);
+  HIPCHK(hipSetDevice(device));
+  gjw_ctx* c = new gjw_ctx;
+  c->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&c->ev0));
+  HIPCHK(hipEventCreate(&c->ev1));
+  *out = c;
+  return GJW_OK;
+}
+
+extern "C" int gjw_destroy(gjw_ctx* c) {
+  if (!c) return GJW_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  delete c;
+  return GJW_OK;
+}
+
+// --------------------------------------------------------------------------
+// survey
+// --------------------------------------------------------------------------
+extern "C" int gjw_survey_create(gjw_ctx* c, const gjw_grid* g, gjw_survey** out) {
+  if (!c || !g || !out) return fail(GJW_ERR_ARG, "gjw_survey_create: NULL argument");
+  if (g->m < 1 || g->n < 1 || g->pml < 1 || g->air < 0 || g->nt < 3)
+    return fail(GJW_ERR_ARG, "gjw_survey_create: bad sizes m=%d n=%d pml=%d air=%d nt=%d", g->m,
+                g->n, g->pml, g->air, g->nt);
+  if (!(g->dx > 0) || !(g->dt > 0) || !(g->eps0 > 0) || !(g->mu0 > 0) || !(g->R0 > 0) ||
+      !(g->R0 < 1) || g->pml_order < 0)
+    return fail(GJW_ERR_ARG, "gjw_survey_create: bad physics (dx, dt, eps0, mu0, R0, order)");
+  gjw_survey* s = new gjw_survey;
+  s->ctx = c;
+  s->gr = *g;
+  s->P = g->pml;
+  s->NX = g->m + 2 * g->pml + g->air;  // w_geom.m:136
+  s->NY = g->n + 2 * g->pml;           // w_geom.m:135
+  s->LD = (int)cdiv(s->NX + 1, 64) * 64;
+  s->FS = (long long)s->LD * (s->NY + 1);
+  s->di0 = s->P + g->air;  // pis + air, 0-based (w_geom.m:168)
+  s->dj0 = s->P;           // pjs, 0-based (w_geom.m:169)
+  *out = s;
+  return GJW_OK;
+}
+
+extern "C" int gjw_survey_destroy(gjw_survey* s) {
+  if (!s) return GJW_OK;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  s->coef.release(); s->fe.release(); s->fs.release(); s->d_pml.release();
+  s->d_si.release(); s->d_sj.release(); s->d_cptr.release(); s->d_crow.release();
+  s->d_cid.release(); s->d_trs.release(); s->d_trr.release(); s->d_trc.release();
+  s->d_sc.release(); s->d_wav.release(); s->fld.release(); s->adjf.release();
+  s->tr.release(); s->adj.release(); s->obs.release(); s->ck.release();
+  s->frames.release(); s->acc.release(); s->ge.release(); s->gsg.release();
+  s->part.release(); s->Jd.release(); s->gw.release();
+  delete s;
+  return GJW_OK;
+}
+
+extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
+  if (!s) return fail(GJW_ERR_ARG, "gjw_set_option: NULL survey");
+  switch (key) {
+    case GJW_OPT_PATH:
+      if (v != GJW_PATH_BATCHED && v != GJW_PATH_PLAIN)
+        return fail(GJW_ERR_ARG, "GJW_OPT_PATH: bad value %lld", (long long)v);
+      s->path = (int)v;
+      break;
+    case GJW_OPT_CKPT_EVERY:
+      if (v < 0 || v > s->gr.nt) return fail(GJW_ERR_ARG, "GJW_OPT_CKPT_EVERY out of range");
+      s->ck_every = (int)v;
+      break;
+    case GJW_OPT_TILE_L:
+      if (v < 0 || v > 4096) return fail(GJW_ERR_ARG, "GJW_OPT_TILE_L out of range");
+      s->tile_L = (int)v;
+      break;
+    case GJW_OPT_SNAPSHOT:
+      if (v != GJW_SNAP_F32) return fail(GJW_ERR_ARG, "GJW_OPT_SNAPSHOT: only F32 is built");
+      s->snap = (int)v;
+      break;
+    default:
+      return fail(GJW_ERR_ARG, "gjw_set_option: unknown key %d", key);
+  }
+  s->kept = 0;
+  return GJW_OK;
+}
+
+// w_exp2pml.m:17-37 (air rows, edge replication of eps into the PML, zero
+// sigma outside the model), then w_cpml.m / w_coeff.m in FP64.
+extern "C" int gjw_set_model(gjw_survey* s, const double* eps_r, const double* sigma) {
+  if (!s || !eps_r || !sigma) return fail(GJW_ERR_ARG, "gjw_set_model: NULL argument");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  const int m = s->gr.m, n = s->gr.n, P = s->P, air = s->gr.air, NX = s->NX, NY = s->NY;
+  std::vector<double> em((size_t)m * n), sm((size_t)m * n);
+  // inputs may live on the device
+  HIPCHK(hipMemcpy(em.data(), eps_r, em.size() * sizeof(double), hipMemcpyDefault));
+  HIPCHK(hipMemcpy(sm.data(), sigma, sm.size() * sizeof(double), hipMemcpyDefault));
+  for (size_t k = 0; k < em.size(); ++k)
+    if (!(em[k] > 0) || !std::isfinite(em[k]) || !std::isfinite(sm[k]) || sm[k] < 0)
+      return fail(GJW_ERR_ARG, "gjw_set_model: eps must be > 0 and sigma >= 0 (cell %zu)", k);
+  s->pe.assign((size_t)NX * NY, 0.0);
+  s->ps.assign((size_t)NX * NY, 0.0);
+  for (int j = 0; j < NY; ++j) {
+    const int jm = std::min(std::max(j - P, 0), n - 1);
+    const bool mcol = (j - P >= 0 && j - P < n);
+    for (int i = 0; i < NX; ++i) {
+      const int r = std::min(std::max(i - P, 0), air + m - 1);  // row of [air; model]
+      double e = 1.0, sg = 0.0;
+      if (r >= air) {
+        e = em[(size_t)(r - air) + (size_t)jm * m];
+        sg = sm[(size_t)(r - air) + (size_t)jm * m];
+      }
+      const bool mrow = (i - P >= 0 && i - P < air + m);
+      if (!(mcol && mrow)) sg = 0.0;
+      s->pe[(size_t)i + (size_t)j * NX] = e;
+      s->ps[(size_t)i + (size_t)j * NX] = sg;
+    }
+  }
+  const double e0 = s->gr.eps0, u0 = s->gr.mu0, dt = s->gr.dt, dx = s->gr.dx, dy = dx;
+  const int ord = s->gr.pml_order;
+  const double cl = 1.0 / std::sqrt(u0 * e0);
+  const double smax = -(ord + 1) * e0 * cl * std::log(s->gr.R0) / (2.0 * dx * P);  // w_cpml.m:47
+  const long long FS = s->FS, LD = s->LD;
+  std::vector<float> h((size_t)10 * FS, 0.f);
+  float *ce = h.data(), *ch = ce + FS, *cxe = ch + FS, *cxh = cxe + FS, *cye = cxh + FS,
+        *cyh = cye + FS, *hax = cyh + FS, *hbx = hax + FS, *hay = hbx + FS, *hby = hay + FS;
+  auto EPS = [&](int i, int j) { return s->pe[(size_t)i + (size_t)j * NX]; };
+  auto SIG = [&](int i, int j) { return s->ps[(size_t)i + (size_t)j * NX]; };
+  // profile of strip cell a (1 = outermost on the n-sides), electric / magnetic
+  auto prof_n = [&](int a, double off) { return std::pow(((P - a + 1) - off) / P, ord); };
+  auto prof_p = [&](int a, double off) { return std::pow((a - off) / P, ord); };
+  s->pml_idx.clear();
+  for (int j = 0; j <= NY; ++j)
+    for (int i = 0; i <= NX; ++i) {
+      const long long k = (long long)j * LD + i;
+      const int kd = ekind(NX, NY, P, i, j);
+      if (kd == KIND_INNER) {
+        const double D = 2.0 * EPS(i, j) * e0 + dt * SIG(i, j);
+        ce[k] = (float)((2.0 * EPS(i, j) * e0 - dt * SIG(i, j)) / D);
+        ch[k] = (float)((2.0 * dt / dx) / D);
+      } else if (kd == KIND_PML) {
+        s->pml_idx.push_back((int)k);
+        const double ee = EPS(i, j);
+        // Ezx (w_cpml.m:60-61, 93-96, 133-134, 174-175)
+        if (i <= P) {
+          const double se = smax * prof_n(i, 0.75);
+          cxe[k] = (float)((2 * e0 - dt * se) / (2 * e0 + dt * se));
+          cxh[k] = (float)((2 * dt / dx) / (2 * e0 + dt * se));
+        } else if (i >= NX - P) {
+          const double se = smax * prof_p(i - NX + P + 1, 0.75);
+          cxe[k] = (float)((2 * e0 * ee - dt * se) / (2 * e0 * ee + dt * se));
+          cxh[k] = (float)((2 * dt / dx) / (2 * e0 * ee + dt * se));
+        } else {
+          cxe[k] = 1.f;
+          cxh[k] = (float)(dt / (dx * e0 * ee));
+        }
+        // Ezy (w_cpml.m:63-64, 103-104, 141-144, 182-185)
+        if (j <= P) {
+          const double se = smax * (dx / dy) * prof_n(j, 0.75);
+          cye[k] = (float)((2 * e0 * ee - dt * se) / (2 * e0 * ee + dt * se));
+          cyh[k] = (float)(-(2 * dt / dy) / (2 * e0 * ee + dt * se));
+        } else if (j >= NY - P) {
+          const double se = smax * (dx / dy) * prof_p(j - NY + P + 1, 0.75);
+          cye[k] = (float)((2 * e0 * ee - dt * se) / (2 * e0 * ee + dt * se));
+          cyh[k] = (float)(-(2 * dt / dy) / (2 * e0 * ee + dt * se));
+        } else if (i <= P) {
+          cye[k] = 1.f;
+          cyh[k] = (float)(-dt / (dy * e0));
+        } else {
+          cye[k] = 1.f;
+          cyh[k] = (float)(-dt / (dy * e0 * ee));
+        }
+      }
+      // Hy strips: rows 0..P-1 (xn, vacuum) and NX-P..NX-1 (xp), columns 1..NY-1
+      if (i <= NX - 1 && j >= 1 && j <= NY - 1 && (i <= P - 1 || i >= NX - P)) {
+        double sm_;
+        if (i <= P - 1)
+          sm_ = (u0 / e0) * smax * prof_n(i + 1, 0.25);
+        else
+          sm_ = (u0 / e0) * smax * prof_p(i - NX + P + 1, 0.25) / EPS(i, j);
+        hay[k] = (float)((2 * u0 - dt * sm_) / (2 * u0 + dt * sm_));
+        hby[k] = (float)((2 * dt / dx) / (2 * u0 + dt * sm_));
+      }
+      // Hx strips: columns 0..P-1 (yn) and NY-P..NY-1 (yp), rows 1..NX-1
+      if (j <= NY - 1 && i >= 1 && i <= NX - 1 && (j <= P - 1 || j >= NY - P)) {
+        double sm_;
+        if (j <= P - 1)
+          sm_ = (u0 / e0) * smax * (dx / dy) * prof_n(j + 1, 0.25) / EPS(i, j);
+        else
+          sm_ = (u0 / e0) * smax * (dx / dy) * prof_p(j - NY + P + 1, 0.25) / EPS(i, j);
+        hax[k] = (float)((2 * u0 - dt * sm_) / (2 * u0 + dt * sm_));
+        hbx[k] = (float)(-(2 * dt / dy) / (2 * u0 + dt * sm_));
+      }
+    }
+  s->bx0 = (float)(-(2 * dt / dy) / (2 * u0));  // Chxez
+  s->by0 = (float)((2 * dt / dx) / (2 * u0));   // Chyez
+  // gradient scale factors on the domain: d/d eps = -2 eps0 / D, d/d sigma = -dt / D
+  std::vector<double> hfe((size_t)m * n, 0.0), hfs((size_t)m * n, 0.0);
+  for (int b = 0; b < n; ++b)
+    for (int a = 0; a < m; ++a) {
+      const int i = s->di0 + a, j = s->dj0 + b;
+      if (ekind(NX, NY, P, i, j) != KIND_INNER) continue;
+      const double D = 2.0 * EPS(i, j) * e0 + dt * SIG(i, j);
+      hfe[(size_t)a + (size_t)b * m] = -2.0 * e0 / D;
+      hfs[(size_t)a + (size_t)b * m] = -dt / D;
+    }
+  hipStream_t st = s->ctx->stream;
+  TRY(s->coef.ensure(h.size()));
+  HIPCHK(hipMemcpyAsync(s->coef.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  TRY(s->fe.ensure(hfe.size()));
+  TRY(s->fs.ensure(hfs.size()));
+  HIPCHK(hipMemcpyAsync(s->fe.p, hfe.data(), hfe.size() * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(s->fs.p, hfs.data(), hfs.size() * 8, hipMemcpyHostToDevice, st));
+  TRY(s->d_pml.ensure(s->pml_idx.size()));
+  HIPCHK(hipMemcpyAsync(s->d_pml.p, s->pml_idx.data(), s->pml_idx.size() * sizeof(int),
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  s->have_model = true;
+  s->kept = 0;
+  if (s->have_shots) {  // source coefficients depend on the model
+    for (int q = 0; q < s->ns; ++q) {
+      const int i = s->si[q], j = s->sj[q];
+      s->sc[q] = ekind(NX, NY, P, i, j) == KIND_INNER
+                     ? (float)(-(2.0 * dt) / (2.0 * EPS(i, j) * e0 + dt * SIG(i, j)))
+                     : 0.f;
+    }
+    HIPCHK(hipMemcpy(s->d_sc.p, s->sc.data(), s->ns * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return GJW_OK;
+}
+
+extern "C" int gjw_set_shots(gjw_survey* s, int32_t ns, const int32_t* src, const float* wavelets,
+                             const int64_t* rec_ptr, const int32_t* rec_lin) {
+  if (!s || !src || !wavelets || !rec_ptr || (ns > 0 && !rec_lin))
+    return fail(GJW_ERR_ARG, "gjw_set_shots: NULL argument");
+  if (ns < 1) return fail(GJW_ERR_ARG, "gjw_set_shots: ns must be >= 1");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  const int NX = s->NX, NY = s->NY, P = s->P, nt = s->gr.nt;
+  std::vector<int32_t> hs(2 * (size_t)ns);
+  std::vector<int64_t> hp(ns + 1);
+  HIPCHK(hipMemcpy(hs.data(), src, hs.size() * 4, hipMemcpyDefault));
+  HIPCHK(hipMemcpy(hp.data(), rec_ptr, hp.size() * 8, hipMemcpyDefault));
+  if (hp[0] != 0) return fail(GJW_ERR_ARG, "gjw_set_shots: rec_ptr[0] must be 0");
+  for (int q = 0; q < ns; ++q)
+    if (hp[q + 1] < hp[q]) return fail(GJW_ERR_ARG, "gjw_set_shots: rec_ptr not monotone");
+  const long long ntr = hp[ns];
+  if (ntr > (1LL << 30)) return fail(GJW_ERR_ARG, "gjw_set_shots: too many receivers");
+  std::vector<int32_t> hl(std::max<long long>(ntr, 1));
+  if (ntr) HIPCHK(hipMemcpy(hl.data(), rec_lin, ntr * 4, hipMemcpyDefault));
+  s->ns = ns;
+  s->ntr = (int)ntr;
+  s->si.resize(ns);
+  s->sj.resize(ns);
+  s->sc.assign(ns, 0.f);
+  for (int q = 0; q < ns; ++q) {
+    const int i = hs[2 * q] - 1, j = hs[2 * q + 1] - 1;  // [iz, ix] 1-based
+    if (i < 1 || i > NX - 1 || j < 1 || j > NY - 1)
+      return fail(GJW_ERR_ARG,
+                  "gjw_set_shots: source %d at [%d,%d] must lie in rows 2..%d, cols 2..%d",
+                  q + 1, i + 1, j + 1, NX, NY);
+    s->si[q] = i;
+    s->sj[q] = j;
+  }
+  s->rec_ptr.assign(hp.begin(), hp.end());
+  s->tr_shot.resize(ntr);
+  s->tr_row.resize(ntr);
+  s->tr_col.resize(ntr);
+  const long long NE = (long long)(NX + 1) * (NY + 1);
+  for (int q = 0; q < ns; ++q)
+    for (long long t = hp[q]; t < hp[q + 1]; ++t) {
+      const long long lin = (long long)hl[t] - 1;  // w_M.m:20 sub2ind([nx+1,ny+1],rz,rx)
+      if (lin < 0 || lin >= NE)
+        return fail(GJW_ERR_ARG, "gjw_set_shots: receiver index %d out of range 1..%lld",
+                    hl[t], NE);
+      s->tr_shot[t] = q;
+      s->tr_row[t] = (int)(lin % (NX + 1));
+      s->tr_col[t] = (int)(lin / (NX + 1));
+    }
+  // column CSR per shot
+  std::vector<int> cptr((size_t)ns * (NY + 2), 0), crow(ntr), cid(ntr);
+  {
+    std::vector<long long> order(ntr);
+    for (long long t = 0; t < ntr; ++t) order[t] = t;
+    std::stable_sort(order.begin(), order.end(), [&](long long a, long long b) {
+      if (s->tr_shot[a] != s->tr_shot[b]) return s->tr_shot[a] < s->tr_shot[b];
+      return s->tr_col[a] < s->tr_col[b];
+    });
+    std::vector<int> cnt((size_t)ns * (NY + 1), 0);
+    for (long long t = 0; t < ntr; ++t) cnt[(size_t)s->tr_shot[t] * (NY + 1) + s->tr_col[t]]++;
+    int run = 0;
+    for (int q = 0; q < ns; ++q) {
+      for (int j = 0; j <= NY; ++j) {
+        cptr[(size_t)q * (NY + 2) + j] = run;
+        run += cnt[(size_t)q * (NY + 1) + j];
+      }
+      cptr[(size_t)q * (NY + 2) + NY + 1] = run;
+    }
+    for (long long t = 0; t < ntr; ++t) {
+      crow[t] = s->tr_row[order[t]];
+      cid[t] = (int)order[t];
+    }
+  }
+  hipStream_t st = s->ctx->stream;
+  TRY(s->d_si.ensure(ns));
+  TRY(s->d_sj.ensure(ns));
+  TRY(s->d_sc.ensure(ns));
+  TRY(s->d_wav.ensure((size_t)ns * nt));
+  TRY(s->d_cptr.ensure(cptr.size()));
+  TRY(s->d_crow.ensure(ntr));
+  TRY(s->d_cid.ensure(ntr));
+  TRY(s->d_trs.ensure(ntr));
+  TRY(s->d_trr.ensure(ntr));
+  TRY(s->d_trc.ensure(ntr));
+  HIPCHK(hipMemcpy(s->d_si.p, s->si.data(), ns * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->d_sj.p, s->sj.data(), ns * 4, hipMemcpyHostToDevice));
+  TRY(host_or_dev_copy(s->d_wav.p, wavelets, (size_t)ns * nt * 4, st));
+  HIPCHK(hipMemcpy(s->d_cptr.p, cptr.data(), cptr.size() * 4, hipMemcpyHostToDevice));
+  if (ntr) {
+    HIPCHK(hipMemcpy(s->d_crow.p, crow.data(), ntr * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->d_cid.p, cid.data(), ntr * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->d_trs.p, s->tr_shot.data(), ntr * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->d_trr.p, s->tr_row.data(), ntr * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->d_trc.p, s->tr_col.data(), ntr * 4, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  s->have_shots = true;
+  s->kept = 0;
+  s->have_adj = false;
+  if (s->have_model) {
+    for (int q = 0; q < ns; ++q) {
+      const int i = s->si[q], j = s->sj[q];
+      const double ee = s->pe[(size_t)i + (size_t)j * NX], sg = s->ps[(size_t)i + (size_t)j * NX];
+      // w_coeff.m:43-44; a source inside the strips is overwritten (w_solve.m:92-94)
+      s->sc[q] = ekind(NX, NY, P, i, j) == KIND_INNER
+                     ? (float)(-(2.0 * s->gr.dt) / (2.0 * ee * s->gr.eps0 + s->gr.dt * sg))
+                     : 0.f;
+    }
+  }
+  HIPCHK(hipMemcpy(s->d_sc.p, s->sc.data(), ns * 4, hipMemcpyHostToDevice));
+  return GJW_OK;
+}
+
+// --------------------------------------------------------------------------
+// launch helpers
+// --------------------------------------------------------------------------
+static DevGrid dev_grid(const gjw_survey* s) {
+  DevGrid g;
+  g.NX = s->NX; g.NY = s->NY; g.P = s->P; g.LD = s->LD; g.FS = s->FS;
+  g.bx0 = s->bx0; g.by0 = s->by0;
+  const float* c = s->coef.p;
+  const long long FS = s->FS;
+  g.ce = c; g.ch = c + FS; g.cxe = c + 2 * FS; g.cxh = c + 3 * FS; g.cye = c + 4 * FS;
+  g.cyh = c + 5 * FS; g.hax = c + 6 * FS; g.hbx = c + 7 * FS; g.hay = c + 8 * FS;
+  g.hby = c + 9 * FS;
+  g.di0 = s->di0; g.dj0 = s->dj0; g.m = s->gr.m; g.n = s->gr.n;
+  return g;
+}
+
+// shots [q0, q0+cnt) of the survey
+static DevShots dev_shots(const gjw_survey* s, int q0) {
+  DevShots d;
+  d.nt = s->gr.nt;
+  d.si = s->d_si.p + q0;
+  d.sj = s->d_sj.p + q0;
+  d.sc = s->d_sc.p + q0;
+  d.wav = s->d_wav.p + (long long)q0 * s->gr.nt;
+  d.cptr = s->d_cptr.p + (long long)q0 * (s->NY + 2);
+  d.crow = s->d_crow.p;
+  d.cid = s->d_cid.p;
+  return d;
+}
+
+static int launch_fill(float* p, long long n, float v, hipStream_t st) {
+  if (n <= 0) return GJW_OK;
+  const int blocks = (int)std::min<long long>(cdiv(n, 256), 65536);
+  hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, st, p, n, v);
+  HIPCHK(hipGetLastError());
+  return GJW_OK;
+}
+
+static int launch_fill_d(double* p, long long n, double v, hipStream_t st) {
+  if (n <= 0) return GJW_OK;
+  const int blocks = (int)std::min<long long>(cdiv(n, 256), 65536);
+  hipLaunchKernelGGL(k_fill_d, dim3(blocks), dim3(256), 0, st, p, n, v);
+  HIPCHK(hipGetLastError());
+  return GJW_OK;
+}
+
+static inline dim3 step_grid(const gjw_survey* s, int L, int nshots) {
+  return dim3((unsigned)cdiv(s->NX + 1, 64), (unsigned)cdiv(cdiv(s->NY + 1, L), 4),
+              (unsigned)nshots);
+}
+
+struct FwdBufs {
+  float *E[2], *Hx[2], *Hy[2], *X, *Y;
+};
+
+static FwdBufs fwd_bufs(gjw_survey* s, int nsh) {
+  FwdBufs b;
+  const long long sz = (long long)nsh * s->FS;
+  float* p = s->fld.p;
+  b.E[0] = p; b.Hx[0] = p + sz; b.Hy[0] = p + 2 * sz;
+  b.E[1] = p + 3 * sz; b.Hx[1] = p + 4 * sz; b.Hy[1] = p + 5 * sz;
+  b.X = p + 6 * sz; b.Y = p + 7 * sz;
+  return b;
+}
+
+struct AdjBufs {
+  float *M[2], *Nx[2], *Ny[2], *X[2], *Y[2];
+};
+
+static AdjBufs adj_bufs(gjw_survey* s, int nsh) {
+  AdjBufs b;
+  const long long sz = (long long)nsh * s->FS;
+  float* p = s->adjf.p;
+  for (int t = 0; t < 2; ++t) {
+    b.M[t] = p + (5 * t + 0) * sz;
+    b.Nx[t] = p + (5 * t + 1) * sz;
+    b.Ny[t] = p + (5 * t + 2) * sz;
+    b.X[t] = p + (5 * t + 3) * sz;
+    b.Y[t] = p + (5 * t + 4) * sz;
+  }
+  return b;
+}
+
+static int fwd_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, int L,
+                      FwdBufs& b, int cur, int it, float* tr, float* frame, long long fstride) {
+  hipLaunchKernelGGL(k_fwd_step, step_grid(s, L, nsh), dim3(64, 4), 0, s->ctx->stream, g, sh, L,
+                     b.E[cur], b.Hx[cur], b.Hy[cur], b.E[cur ^ 1], b.Hx[cur ^ 1], b.Hy[cur ^ 1],
+                     b.X, b.Y, it, tr, frame, fstride);
+  HIPCHK(hipGetLastError());
+  return GJW_OK;
+}
+
+static int adj_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, int L,
+                      AdjBufs& b, int cur, int it, const float* adj, const float* F1,
+                      const float* F0, long long fstride, float* acc, long long astride,
+                      float* gw) {
+  hipLaunchKernelGGL(k_adj_step, step_grid(s, L, nsh), dim3(64, 4), 0, s->ctx->stream, g, sh, L,
+                     b.M[cur], b.Nx[cur], b.Ny[cur], b.X[cur], b.Y[cur], b.M[cur ^ 1],
+                     b.Nx[cur ^ 1], b.Ny[cur ^ 1], b.X[cur ^ 1], b.Y[cur ^ 1], it, adj, F1, F0,
+                     fstride, acc, astride, gw);
+  HIPCHK(hipGetLastError());
+  return GJW_OK;
+}
+
+static int check_ready(gjw_survey* s, const char* who) {
+  if (!s) return fail(GJW_ERR_ARG, "%s: NULL survey", who);
+  if (!s->have_model) return fail(GJW_ERR_STATE, "%s: gjw_set_model has not been called", who);
+  if (!s->have_shots) return fail(GJW_ERR_STATE, "%s: gjw_set_shots has not been called", who);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  return GJW_OK;
+}
+
+// Tile width L (columns per wavefront): time a few steps of each candidate on
+// zeroed batch state and keep the fastest.
+static int autotune_L(gjw_survey* s, int nsh) {
+  if (s->tile_L > 0) {
+    s->tuned_L = s->tile_L;
+    return GJW_OK;
+  }
+  if (s->tuned_L > 0) return GJW_OK;
+  const int cand[] = {8, 16, 32, 64};
+  hipStream_t st = s->ctx->stream;
+  DevGrid g = dev_grid(s);
+  DevShots sh = dev_shots(s, 0);
+  FwdBufs b = fwd_bufs(s, nsh);
+  float best = 1e30f;
+  int bestL = 16;
+  const int reps = 6;
+  for (int L : cand) {
+    int cur = 0;
+    TRY(fwd_launch(s, g, sh, nsh, L, b, cur, 0, nullptr, nullptr, 0));  // warm
+    HIPCHK(hipEventRecord(s->ctx->ev0, st));
+    for (int r = 0; r < reps; ++r) {
+      TRY(fwd_launch(s, g, sh, nsh, L, b, cur, 0, nullptr, nullptr, 0));
+      cur ^= 1;
+    }
+    HIPCHK(hipEventRecord(s->ctx->ev1, st));
+    HIPCHK(hipEventSynchronize(s->ctx->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
+    if (ms < best) {
+      best = ms;
+      bestL = L;
+    }
+  }
+  s->tuned_L = bestL;
+  return GJW_OK;
+}
+
+static int ensure_fwd_state(gjw_survey* s, int nsh) {
+  TRY(s->fld.ensure((size_t)8 * nsh * s->FS));
+  return GJW_OK;
+}
+
+static int zero_fwd_state(gjw_survey* s, int nsh) {
+  return launch_fill(s->fld.p, (long long)8 * nsh * s->FS, 0.f, s->ctx->stream);
+}
+
+// checkpoint layout: [ck][E, Hx, Hy: ns*FS each][X, Y: ns*npml each]
+static long long ck_stride(const gjw_survey* s) {
+  return 3LL * s->ns * s->FS + 2LL * s->ns * (long long)s->pml_idx.size();
+}
+
+static int ck_save(gjw_survey* s, FwdBufs& b, int cur, int c) {
+  hipStream_t st = s->ctx->stream;
+  const long long sz = (long long)s->ns * s->FS, npml = (long long)s->pml_idx.size();
+  float* dst = s->ck.p + (long long)c * ck_stride(s);
+  HIPCHK(hipMemcpyAsync(dst, b.E[cur], sz * 4, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(dst + sz, b.Hx[cur], sz * 4, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(dst + 2 * sz, b.Hy[cur], sz * 4, hipMemcpyDeviceToDevice, st));
+  const long long nn = npml * s->ns;
+  if (nn > 0) {
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)cdiv(nn, 256)), dim3(256), 0, st, b.X, s->FS,
+                       s->d_pml.p, (int)npml, s->ns, dst + 3 * sz);
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)cdiv(nn, 256)), dim3(256), 0, st, b.Y, s->FS,
+                       s->d_pml.p, (int)npml, s->ns, dst + 3 * sz + nn);
+    HIPCHK(hipGetLastError());
+  }
+  return GJW_OK;
+}
+
+static int ck_restore(gjw_survey* s, FwdBufs& b, int cur, int c) {
+  hipStream_t st = s->ctx->stream;
+  const long long sz = (long long)s->ns * s->FS, npml = (long long)s->pml_idx.size();
+  const float* src = s->ck.p + (long long)c * ck_stride(s);
+  HIPCHK(hipMemcpyAsync(b.E[cur], src, sz * 4, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(b.Hx[cur], src + sz, sz * 4, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(b.Hy[cur], src + 2 * sz, sz * 4, hipMemcpyDeviceToDevice, st));
+  const long long nn = npml * s->ns;
+  if (nn > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)cdiv(nn, 256)), dim3(256), 0, st,
+                       src + 3 * sz, s->FS, s->d_pml.p, (int)npml, s->ns, b.X);
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)cdiv(nn, 256)), dim3(256), 0, st,
+                       src + 3 * sz + nn, s->FS, s->d_pml.p, (int)npml, s->ns, b.Y);
+    HIPCHK(hipGetLastError());
+  }
+  return GJW_OK;
+}
+
+static int ck_interval(const gjw_survey* s) {
+  if (s->ck_every > 0) return s->ck_every;
+  return std::max(1, (int)std::ceil(std::sqrt((double)s->gr.nt)));
+}
+
+// --------------------------------------------------------------------------
+// forward
+// --------------------------------------------------------------------------
+static int forward_batched(gjw_survey* s, int keep) {
+  const int ns = s->ns, nt = s->gr.nt;
+  hipStream_t st = s->ctx->stream;
+  TRY(ensure_fwd_state(s, ns));
+  TRY(s->tr.ensure(std::max<long long>((long long)s->ntr * nt, 1)));
+  TRY(zero_fwd_state(s, ns));
+  TRY(autotune_L(s, ns));
+  TRY(zero_fwd_state(s, ns));
+  if (keep) {
+    s->ck_K = ck_interval(s);
+    s->ck_n = (int)cdiv(nt, s->ck_K);
+    TRY(s->ck.ensure((size_t)s->ck_n * ck_stride(s)));
+  }
+  DevGrid g = dev_grid(s);
+  DevShots sh = dev_shots(s, 0);
+  FwdBufs b = fwd_bufs(s, ns);
+  int cur = 0;
+  HIPCHK(hipEventRecord(s->ctx->ev0, st));
+  for (int it = 0; it < nt; ++it) {
+    if (keep && it % s->ck_K == 0) TRY(ck_save(s, b, cur, it / s->ck_K));
+    TRY(fwd_launch(s, g, sh, ns, s->tuned_L, b, cur, it, s->tr.p, nullptr, 0));
+    cur ^= 1;
+  }
+  HIPCHK(hipEventRecord(s->ctx->ev1, st));
+  s->cur = cur;
+  s->kept = keep ? 1 : 0;
+  s->last_cells = (double)(s->NX + 1) * (s->NY + 1) * nt * ns;
+  return GJW_OK;
+}
+
+// plain: one shot at a time, whole domain wavefield kept (w_solve.m:76,102)
+static int forward_plain(gjw_survey* s, int keep) {
+  const int ns = s->ns, nt = s->gr.nt;
+  hipStream_t st = s->ctx->stream;
+  const long long per = (long long)s->gr.m * s->gr.n;
+  TRY(ensure_fwd_state(s, 1));
+  TRY(s->tr.ensure(std::max<long long>((long long)s->ntr * nt, 1)));
+  if (keep) TRY(s->frames.ensure((size_t)ns * (nt + 1) * per));
+  const int L = s->tile_L > 0 ? s->tile_L : 16;
+  DevGrid g = dev_grid(s);
+  FwdBufs b = fwd_bufs(s, 1);
+  HIPCHK(hipEventRecord(s->ctx->ev0, st));
+  for (int q = 0; q < ns; ++q) {
+    DevShots sh = dev_shots(s, q);
+    TRY(zero_fwd_state(s, 1));
+    float* fr = keep ? s->frames.p + (long long)q * (nt + 1) * per : nullptr;
+    if (fr) TRY(launch_fill(fr, per, 0.f, st));  // frame 0 = Ez(S_0) = 0
+    int cur = 0;
+    for (int it = 0; it < nt; ++it) {
+      TRY(fwd_launch(s, g, sh, 1, L, b, cur, it, s->tr.p, fr ? fr + (long long)(it + 1) * per : nullptr,
+                     0));
+      cur ^= 1;
+    }
+  }
+  HIPCHK(hipEventRecord(s->ctx->ev1, st));
+  s->kept = keep ? 2 : 0;
+  s->last_cells = (double)(s->NX + 1) * (s->NY + 1) * nt * ns;
+  return GJW_OK;
+}
+
+static int finish_timing(gjw_survey* s) {
+  HIPCHK(hipEventSynchronize(s->ctx->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
+  s->last_ms = ms;
+  return GJW_OK;
+}
+
+extern "C" int gjw_forward(gjw_survey* s, float* traces, float* jz, int32_t keep) {
+  TRY(check_ready(s, "gjw_forward"));
+  if (jz) return fail(GJW_ERR_ARG, "gjw_forward: jz output is not built yet; pass NULL");
+  s->have_adj = false;
+  TRY(s->path == GJW_PATH_PLAIN ? forward_plain(s, keep) : forward_batched(s, keep));
+  TRY(finish_timing(s));
+  if (traces) {
+    HIPCHK(hipMemcpyAsync(traces, s->tr.p, (size_t)s->ntr * s->gr.nt * 4, hipMemcpyDefault,
+                          s->ctx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return GJW_OK;
+}
+
+// --------------------------------------------------------------------------
+// misfit
+// --------------------------------------------------------------------------
+static int misfit_dev(gjw_survey* s, const float* d_obs, int32_t kind) {
+  if (kind != GJW_MISFIT_L2) return fail(GJW_ERR_ARG, "gjw_misfit: unsupported kind %d", kind);
+  hipStream_t st = s->ctx->stream;
+  const long long n = (long long)s->ntr * s->gr.nt;
+  TRY(s->adj.ensure(std::max<long long>(n, 1)));
+  TRY(s->obs.ensure(std::max<long long>(n, 1)));
+  TRY(s->part.ensure(1024));
+  TRY(s->Jd.ensure(1));
+  TRY(host_or_dev_copy(s->obs.p, d_obs, n * 4, st));
+  const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(n, 256), 1), 1024);
+  hipLaunchKernelGGL(k_resid_l2, dim3(blocks), dim3(256), 0, st, s->tr.p, s->obs.p, s->adj.p, n,
+                     s->part.p);
+  hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, blocks, 0.5, s->Jd.p);
+  HIPCHK(hipGetLastError());
+  s->have_adj = true;
+  return GJW_OK;
+}
+
+extern "C" int gjw_misfit(gjw_survey* s, const float* d_obs, int32_t kind, float* e_out,
+                          double* J) {
+  TRY(check_ready(s, "gjw_misfit"));
+  if (!d_obs) return fail(GJW_ERR_ARG, "gjw_misfit: d_obs is NULL");
+  if (!s->tr.p) return fail(GJW_ERR_STATE, "gjw_misfit: run gjw_forward first");
+  TRY(misfit_dev(s, d_obs, kind));
+  hipStream_t st = s->ctx->stream;
+  if (e_out)
+    HIPCHK(hipMemcpyAsync(e_out, s->adj.p, (size_t)s->ntr * s->gr.nt * 4, hipMemcpyDefault, st));
+  if (J) HIPCHK(hipMemcpyAsync(J, s->Jd.p, 8, hipMemcpyDefault, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return GJW_OK;
+}
+
+// --------------------------------------------------------------------------
+// gradient
+// --------------------------------------------------------------------------
+static int adj_init(gjw_survey* s, int nsh, int q0, const float* adj) {
+  hipStream_t st = s->ctx->stream;
+  TRY(s->adjf.ensure((size_t)10 * nsh * s->FS));
+  TRY(launch_fill(s->adjf.p, (long long)10 * nsh * s->FS, 0.f, st));
+  // residual of the last step at the receivers of shots [q0, q0+nsh)
+  const long long t0 = s->rec_ptr[q0], t1 = s->rec_ptr[q0 + nsh];
+  if (t1 > t0) {
+    DevGrid g = dev_grid(s);
+    AdjBufs b = adj_bufs(s, nsh);
+    hipLaunchKernelGGL(k_inject_last, dim3((unsigned)cdiv(t1 - t0, 256)), dim3(256), 0, st, g,
+                       s->gr.nt, q0, (int)(t1 - t0), s->d_trs.p + t0, s->d_trr.p + t0,
+                       s->d_trc.p + t0, adj + t0 * s->gr.nt, s->gr.nt - 1, b.M[0]);
+    HIPCHK(hipGetLastError());
+  }
+  return GJW_OK;
+}
+
+static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
+  const int ns = s->ns, nt = s->gr.nt, K = s->ck_K;
+  const long long per = (long long)s->gr.m * s->gr.n;
+  hipStream_t st = s->ctx->stream;
+  TRY(s->frames.ensure((size_t)ns * (K + 1) * per));
+  TRY(s->acc.ensure((size_t)2 * ns * per));
+  TRY(launch_fill(s->acc.p, 2LL * ns * per, 0.f, st));
+  TRY(adj_init(s, ns, 0, adj));
+  DevGrid g = dev_grid(s);
+  DevShots sh = dev_shots(s, 0);
+  FwdBufs fb = fwd_bufs(s, ns);
+  AdjBufs ab = adj_bufs(s, ns);
+  const long long fstride = (long long)(K + 1) * per;  // frames per shot
+  int acur = 0;
+  for (int c = s->ck_n - 1; c >= 0; --c) {
+    const int k0 = c * K, k1 = std::min(k0 + K, nt);
+    int cur = 0;
+    TRY(ck_restore(s, fb, cur, c));
+    hipLaunchKernelGGL(k_domain_frame, dim3((unsigned)cdiv(per * ns, 256)), dim3(256), 0, st, g, ns,
+                       fb.E[cur], s->frames.p, fstride);
+    HIPCHK(hipGetLastError());
+    for (int it = k0; it < k1; ++it) {
+      TRY(fwd_launch(s, g, sh, ns, s->tuned_L, fb, cur, it, nullptr,
+                     s->frames.p + (long long)(it + 1 - k0) * per, fstride));
+      cur ^= 1;
+    }
+    for (int it = k1 - 1; it >= k0; --it) {
+      TRY(adj_launch(s, g, sh, ns, s->tuned_L, ab, acur, it, adj,
+                     s->frames.p + (long long)(it + 1 - k0) * per,
+                     s->frames.p + (long long)(it - k0) * per, fstride, s->acc.p, per, gw));
+      acur ^= 1;
+    }
+    hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
+                       s->gr.n, ns, s->acc.p, per, s->fe.p, s->fs.p, s->ge.p, s->gsg.p);
+    HIPCHK(hipGetLastError());
+  }
+  return GJW_OK;
+}
+
+static int grad_plain_exact(gjw_survey* s, const float* adj, float* gw) {
+  const int ns = s->ns, nt = s->gr.nt;
+  const long long per = (long long)s->gr.m * s->gr.n;
+  hipStream_t st = s->ctx->stream;
+  const int L = s->tile_L > 0 ? s->tile_L : 16;
+  const int K = ck_interval(s);
+  TRY(s->acc.ensure((size_t)2 * per));
+  TRY(launch_fill(s->acc.p, 2LL * per, 0.f, st));
+  DevGrid g = dev_grid(s);
+  for (int q = 0; q < ns; ++q) {
+    DevShots sh = dev_shots(s, q);
+    TRY(adj_init(s, 1, q, adj));
+    AdjBufs ab = adj_bufs(s, 1);
+    const float* u = s->frames.p + (long long)q * (nt + 1) * per;
+    int acur = 0;
+    for (int it = nt - 1; it >= 0; --it) {
+      TRY(adj_launch(s, g, sh, 1, L, ab, acur, it, adj, u + (long long)(it + 1) * per,
+                     u + (long long)it * per, 0, s->acc.p, 0,
+                     gw ? gw + (long long)q * nt : nullptr));
+      acur ^= 1;
+      if (it % K == 0) {
+        hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
+                           s->gr.n, 1, s->acc.p, per, s->fe.p, s->fs.p, s->ge.p, s->gsg.p);
+        HIPCHK(hipGetLastError());
+      }
+    }
+  }
+  return GJW_OK;
+}
+
+static int grad_run(gjw_survey* s, const float* adj_in, int32_t mode, double* g_eps,
+                    double* g_sig, double* g_w) {
+  if (mode != GJW_GRAD_EXACT)
+    return fail(GJW_ERR_ARG, "gjw_gradient: mode %d is not built yet", mode);
+  if (!s->kept) return fail(GJW_ERR_STATE, "gjw_gradient: run gjw_forward(keep=1) first");
+  hipStream_t st = s->ctx->stream;
+  const int ns = s->ns, nt = s->gr.nt;
+  const long long per = (long long)s->gr.m * s->gr.n, n = (long long)s->ntr * nt;
+  const float* adj = nullptr;
+  if (adj_in) {
+    TRY(s->adj.ensure(std::max<long long>(n, 1)));
+    TRY(host_or_dev_copy(s->adj.p, adj_in, n * 4, st));
+    adj = s->adj.p;
+  } else {
+    if (!s->have_adj) return fail(GJW_ERR_STATE, "gjw_gradient: no adjoint source (call gjw_misfit)");
+    adj = s->adj.p;
+  }
+  TRY(s->ge.ensure(per));
+  TRY(s->gsg.ensure(per));
+  TRY(launch_fill_d(s->ge.p, per, 0.0, st));
+  TRY(launch_fill_d(s->gsg.p, per, 0.0, st));
+  float* gwf = nullptr;
+  DBuf<float> gwtmp;
+  if (g_w) {
+    TRY(gwtmp.ensure((size_t)ns * nt));
+    TRY(launch_fill(gwtmp.p, (long long)ns * nt, 0.f, st));
+    gwf = gwtmp.p;
+  }
+  HIPCHK(hipEventRecord(s->ctx->ev0, st));
+  int rc = s->kept == 1 ? grad_batched(s, adj, gwf) : grad_plain_exact(s, adj, gwf);
+  if (rc != GJW_OK) {
+    gwtmp.release();
+    return rc;
+  }
+  HIPCHK(hipEventRecord(s->ctx->ev1, st));
+  TRY(finish_timing(s));
+  s->last_cells = (double)(s->NX + 1) * (s->NY + 1) * nt * ns * (s->kept == 1 ? 2 : 1);
+  if (g_eps) HIPCHK(hipMemcpyAsync(g_eps, s->ge.p, per * 8, hipMemcpyDefault, st));
+  if (g_sig) HIPCHK(hipMemcpyAsync(g_sig, s->gsg.p, per * 8, hipMemcpyDefault, st));
+  if (g_w) {
+    std::vector<float> h((size_t)ns * nt);
+    HIPCHK(hipMemcpyAsync(h.data(), gwf, h.size() * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<double> hd(h.begin(), h.end());
+    HIPCHK(hipMemcpy(g_w, hd.data(), hd.size() * 8, hipMemcpyDefault));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  gwtmp.release();
+  return GJW_OK;
+}
+
+extern "C" int gjw_gradient(gjw_survey* s, const float* adj, int32_t mode, double* g_eps,
+                            double* g_sig, double* g_w, float* a_max) {
+  TRY(check_ready(s, "gjw_gradient"));
+  if (a_max) return fail(GJW_ERR_ARG, "gjw_gradient: a_max is only produced in REF mode");
+  return grad_run(s, adj, mode, g_eps, g_sig, g_w);
+}
+
+extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps, double* g_sig,
+                                double* J) {
+  TRY(check_ready(s, "gjw_fwi_gradient"));
+  if (!d_obs) return fail(GJW_ERR_ARG, "gjw_fwi_gradient: d_obs is NULL");
+  hipStream_t st = s->ctx->stream;
+  HIPCHK(hipEventRecord(s->ctx->ev0, st));
+  hipEvent_t e0;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventRecord(e0, st));
+  int rc = s->path == GJW_PATH_PLAIN ? forward_plain(s, 1) : forward_batched(s, 1);
+  if (rc == GJW_OK) rc = misfit_dev(s, d_obs, GJW_MISFIT_L2);
+  if (rc == GJW_OK) rc = grad_run(s, nullptr, GJW_GRAD_EXACT, g_eps, g_sig, nullptr);
+  if (rc != GJW_OK) {
+    (void)hipEventDestroy(e0);
+    return rc;
+  }
+  HIPCHK(hipEventSynchronize(s->ctx->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, s->ctx->ev1));
+  HIPCHK(hipEventDestroy(e0));
+  s->last_ms = ms;
+  s->last_cells = (double)(s->NX + 1) * (s->NY + 1) * s->gr.nt * s->ns *
+                  (s->path == GJW_PATH_PLAIN ? 2 : 3);
+  if (J) HIPCHK(hipMemcpy(J, s->Jd.p, 8, hipMemcpyDefault));
+  return GJW_OK;
+}
+
+extern "C" int gjw_last_timing(gjw_survey* s, double* ms, double* cells) {
+  if (!s) return fail(GJW_ERR_ARG, "gjw_last_timing: NULL survey");
+  if (ms) *ms = s->last_ms;
+  if (cells) *cells = s->last_cells;
+  return GJW_OK;
+}
+
+extern "C" int gjw_differentiate_cube(gjw_ctx* c, float* v, int32_t nz, int32_t nx, int32_t nt,
+                                      double dt) {
+  if (!c || !v) return fail(GJW_ERR_ARG, "gjw_differentiate_cube: NULL argument");
+  if (nz < 1 || nx < 1 || nt < 3) return fail(GJW_ERR_ARG, "gjw_differentiate_cube: bad sizes");
+  HIPCHK(hipSetDevice(c->device));
+  hipPointerAttribute_t at;
+  const long long fr = (long long)nz * nx, n = fr * nt;
+  hipError_t pe = hipPointerGetAttributes(&at, v);
+  const bool on_dev = (pe == hipSuccess && at.type == hipMemoryTypeDevice);
+  (void)hipGetLastError();
+  float* d = v;
+  if (!on_dev) HIPCHK(hipMalloc(&d, n * 4));
+  if (!on_dev) HIPCHK(hipMemcpyAsync(d, v, n * 4, hipMemcpyDefault, c->stream));
+  hipLaunchKernelGGL(k_dcube, dim3((unsigned)cdiv(fr, 256)), dim3(256), 0, c->stream, d, fr, nt,
+                     dt);
+  HIPCHK(hipGetLastError());
+  if (!on_dev) {
+    HIPCHK(hipMemcpyAsync(v, d, n * 4, hipMemcpyDefault, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(d));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return GJW_OK;
+}

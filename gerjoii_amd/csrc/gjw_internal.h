@@ -1,0 +1,55 @@
+// gjw_internal.h -- device-side data layout shared by the kernels and the
+// host driver of libgjw.  Not part of the public C ABI (include/gjw.h).
+//
+// Index convention on the device: 0-based, depth i fastest.  Ez lives on
+// rows 0..NX and columns 0..NY (MATLAB's (NX+1) x (NY+1), w_solve.m:70-72);
+// Hx uses rows 0..NX, columns 0..NY-1; Hy rows 0..NX-1, columns 0..NY.  All
+// three share one pitched layout: element (i, j) sits at j*LD + i, with LD a
+// multiple of 64 so that a 64-lane wavefront reads one aligned 256-byte row
+// segment.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gjw {
+
+// Per-survey constants (shared by every shot of the survey).
+struct DevGrid {
+  int NX, NY, P, LD;
+  long long FS;            // floats per field per shot = LD*(NY+1)
+  float bx0, by0;          // inner Chxez, Chyez (w_coeff.m:34-40; Chxh=Chyh=1)
+  const float* ce;         // inner Ceze   (LD layout)
+  const float* ch;         // inner Cezhy  (Cezhx = -Cezhy since dy = dx)
+  const float* cxe;        // split-field Ezx/Ezy coefficients at PML Ez cells
+  const float* cxh;
+  const float* cye;
+  const float* cyh;
+  const float* hax;        // Hx strip cells (columns 0..P-1, NY-P..NY-1)
+  const float* hbx;
+  const float* hay;        // Hy strip cells (rows 0..P-1, NX-P..NX-1)
+  const float* hby;
+  // stored domain window (w_geom.m:168-172): rows di0..di0+m-1, cols dj0..dj0+n-1
+  int di0, dj0, m, n;
+};
+
+// Per-shot geometry of one survey.
+struct DevShots {
+  int nt;
+  const int* si;     // source row (0-based Ez index), per shot
+  const int* sj;     // source column
+  const float* sc;   // Cezj of the shot, 0 when the source cell is overwritten
+  const float* wav;  // wavelets [shot][nt]
+  const int* cptr;   // receivers by column: [shot][NY+2] offsets into crow/cid
+  const int* crow;   // receiver row
+  const int* cid;    // global trace index; trace t occupies tr[t*nt .. t*nt+nt)
+};
+
+enum { KIND_RING = 0, KIND_INNER = 1, KIND_PML = 2 };
+
+__host__ __device__ inline int ekind(int NX, int NY, int P, int i, int j) {
+  if (i <= 0 || i >= NX || j <= 0 || j >= NY) return KIND_RING;
+  if (i <= P || i >= NX - P || j <= P || j >= NY - P) return KIND_PML;
+  return KIND_INNER;
+}
+
+}  // namespace gjw

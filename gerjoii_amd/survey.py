@@ -1,0 +1,257 @@
+"""Host-side API over libgjw: one ``Survey`` = one padded grid, its model and
+its shots, resident on one GPU.
+
+Mirrors the reference's per-shot pipeline (src/w/w_update_e.m:50-98:
+w_load -> w_fwd -> w_bozdag/w_E -> w_grad) for all shots of a survey at once.
+All arrays use MATLAB's column-major convention: model arrays are (m, n)
+depth-first, a shot's traces are (nt, nr) (``gerjoii_.w.d_2d``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+
+EPS0 = 8.854187817e-12  # parame_.w.eps_0
+MU0 = 4e-7 * np.pi      # parame_.w.mu_0
+C0 = 1.0 / np.sqrt(MU0 * EPS0)
+
+
+@dataclass
+class Grid:
+    """Padded-grid geometry (src/w/w_geom.m:78-172; pml_w == pml_d)."""
+    m: int           # model rows (depth)
+    n: int           # model columns
+    pml: int
+    air: int
+    nt: int
+    dx: float
+    dt: float
+    eps0: float = EPS0
+    mu0: float = MU0
+    R0: float = 1e-8
+    pml_order: int = 2
+
+    @property
+    def nx(self):  # finite_.w.nx (depth, with PML and air)
+        return self.m + 2 * self.pml + self.air
+
+    @property
+    def ny(self):  # finite_.w.ny
+        return self.n + 2 * self.pml
+
+    @property
+    def pis(self):
+        return self.pml + 1
+
+    @property
+    def pjs(self):
+        return self.pml + 1
+
+    @property
+    def pie(self):
+        return self.nx - self.pml + 1
+
+    @property
+    def pje(self):
+        return self.ny - self.pml + 1
+
+    def cells(self):
+        """Padded Ez cells, (nx+1)*(ny+1): the unit of one cell-update."""
+        return (self.nx + 1) * (self.ny + 1)
+
+    def c_struct(self):
+        return N.GjwGrid(self.m, self.n, self.pml, self.air, self.nt, self.pml_order,
+                         self.dx, self.dt, self.eps0, self.mu0, self.R0)
+
+
+def dt_cfl(dx, eps_min=1.0, courant=0.9):
+    """w_geom.m:120-122: dt = 0.9 / (v_max sqrt(dx^-2 + dy^-2)), dy = dx."""
+    vmax = C0 / np.sqrt(eps_min)
+    return courant / (vmax * np.sqrt(2.0 / dx**2))
+
+
+def w_wavelet(t, ampli, tau):
+    """w_wavelet.m:11-32: -(t - t0) exp(-(t - t0)^2 / tau^2), tau = 1/(pi fo),
+    t0 = 5 tau, scaled to max-abs ``ampli``."""
+    tau = tau / np.pi
+    t0 = 5 * tau
+    w = -(t - t0) * np.exp(-(((t - t0) ** 2) / (tau**2)))
+    return ampli * (w / np.max(np.abs(w)))
+
+
+def src_index(g: Grid, iz0, ix0):
+    """w_load.m:31-32: source [iz, ix] on the padded Ez grid (1-based) from
+    model offsets (iz0 rows below the top of the model, ix0 columns)."""
+    return (g.pis + g.air + int(iz0), g.pjs + int(ix0))
+
+
+def rec_Mw(g: Grid, rz0, rx0):
+    """w_load.m:34-35 + w_M.m:20: receivers [rx, rz] -> linear Mw (1-based)."""
+    rz = g.pis + g.air + np.asarray(rz0, dtype=np.int64)
+    rx = g.pjs + np.asarray(rx0, dtype=np.int64)
+    return (rz - 1) + (rx - 1) * (g.nx + 1) + 1
+
+
+class Context:
+    """One GPU (gjw_create); shared by the surveys on it."""
+
+    def __init__(self, device=0):
+        lib = N.load()
+        h = C.c_void_p()
+        N.check(lib.gjw_create(int(device), C.byref(h)), "gjw_create")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            N.load().gjw_destroy(self.h)
+            self.h = None
+
+    def differentiate_cube(self, v, dt):
+        """differentiate_cube.m on the GPU; v is an (m, n, nt) float32 array,
+        column-major (numpy F-order), modified in place."""
+        assert v.dtype == np.float32
+        nz, nx, nt = v.shape
+        if isinstance(v, np.ndarray):
+            assert v.flags.f_contiguous
+        N.check(N.load().gjw_differentiate_cube(self.h, N.ptr(v), nz, nx, nt, float(dt)),
+                "gjw_differentiate_cube")
+        return v
+
+
+_default_ctx = {}
+
+
+def context(device=0):
+    if device not in _default_ctx:
+        _default_ctx[device] = Context(device)
+    return _default_ctx[device]
+
+
+def _sync_torch(*xs):
+    for x in xs:
+        if x is not None and hasattr(x, "is_cuda") and x.is_cuda:
+            import torch
+            torch.cuda.synchronize(x.device)
+            return
+
+
+class Survey:
+    """A survey resident on one GPU: grid + model + shots + device state."""
+
+    def __init__(self, grid: Grid, ctx: Context | None = None):
+        self.grid = grid
+        self.ctx = ctx or context(0)
+        h = C.c_void_p()
+        gs = grid.c_struct()
+        N.check(N.load().gjw_survey_create(self.ctx.h, C.byref(gs), C.byref(h)),
+                "gjw_survey_create")
+        self.h = h
+        self.rec_ptr = None
+
+    def close(self):
+        if self.h:
+            N.load().gjw_survey_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- setup --------------------------------------------------------------
+    def set_option(self, key, value):
+        N.check(N.load().gjw_set_option(self.h, int(key), int(value)), "gjw_set_option")
+
+    def set_path(self, plain: bool):
+        self.set_option(N.GJW_OPT_PATH, N.GJW_PATH_PLAIN if plain else N.GJW_PATH_BATCHED)
+
+    def set_model(self, eps_r, sigma):
+        e = np.asfortranarray(eps_r, dtype=np.float64)
+        s = np.asfortranarray(sigma, dtype=np.float64)
+        assert e.shape == (self.grid.m, self.grid.n) and s.shape == e.shape
+        N.check(N.load().gjw_set_model(self.h, N.ptr(e), N.ptr(s)), "gjw_set_model")
+
+    def set_shots(self, src, wavelets, receivers):
+        """src: (ns, 2) 1-based [iz, ix]; wavelets: (ns, nt); receivers: list of
+        1-based Mw arrays (one per shot)."""
+        src = np.ascontiguousarray(np.asarray(src, dtype=np.int32).reshape(-1, 2))
+        ns = src.shape[0]
+        wv = np.ascontiguousarray(np.asarray(wavelets, dtype=np.float32).reshape(ns, self.grid.nt))
+        recs = [np.asarray(r, dtype=np.int32).ravel() for r in receivers]
+        assert len(recs) == ns
+        rp = np.zeros(ns + 1, dtype=np.int64)
+        rp[1:] = np.cumsum([len(r) for r in recs])
+        rl = np.ascontiguousarray(np.concatenate(recs) if rp[-1] else np.zeros(1, np.int32))
+        N.check(N.load().gjw_set_shots(self.h, ns, N.ptr(src), N.ptr(wv), N.ptr(rp), N.ptr(rl)),
+                "gjw_set_shots")
+        self.rec_ptr = rp
+        self.ns = ns
+
+    @property
+    def ntraces(self):
+        return int(self.rec_ptr[-1])
+
+    def traces_shape(self):
+        return (self.ntraces, self.grid.nt)
+
+    # -- compute ------------------------------------------------------------
+    def forward(self, keep=False, out=None):
+        """Traces of all shots, (ntraces, nt) float32: row t is trace t
+        (shot-major), i.e. shot s's d_2d is out[rec_ptr[s]:rec_ptr[s+1]].T."""
+        if out is None:
+            out = np.zeros(self.traces_shape(), dtype=np.float32)
+        N.check(N.load().gjw_forward(self.h, N.ptr(out), None, int(bool(keep))), "gjw_forward")
+        return out
+
+    def misfit(self, d_obs, e_out=None):
+        """L2 adjoint source e = d - d_o and J = 0.5 e'e (FP64) of the last forward."""
+        _sync_torch(d_obs, e_out)
+        if e_out is None:
+            e_out = np.zeros(self.traces_shape(), dtype=np.float32)
+        J = C.c_double(0)
+        d = d_obs if hasattr(d_obs, "data_ptr") else np.ascontiguousarray(d_obs, dtype=np.float32)
+        N.check(N.load().gjw_misfit(self.h, N.ptr(d), N.GJW_MISFIT_L2, N.ptr(e_out), C.byref(J)),
+                "gjw_misfit")
+        return e_out, J.value
+
+    def gradient(self, adj=None, mode=N.GJW_GRAD_EXACT, want_gw=False, g_eps=None, g_sig=None):
+        """FP64 gradients (m, n) w.r.t. eps_r and sigma (and optionally the
+        wavelet gradient (ns, nt)) for adjoint source ``adj`` (traces layout)."""
+        m, n = self.grid.m, self.grid.n
+        if g_eps is None:
+            g_eps = np.zeros((m, n), dtype=np.float64, order="F")
+        if g_sig is None:
+            g_sig = np.zeros((m, n), dtype=np.float64, order="F")
+        gw = np.zeros((self.ns, self.grid.nt), dtype=np.float64) if want_gw else None
+        if adj is not None and not hasattr(adj, "data_ptr"):
+            adj = np.ascontiguousarray(adj, dtype=np.float32)
+        _sync_torch(adj, g_eps, g_sig)
+        N.check(N.load().gjw_gradient(self.h, N.ptr(adj), int(mode), N.ptr(g_eps), N.ptr(g_sig),
+                                      N.ptr(gw), None), "gjw_gradient")
+        return (g_eps, g_sig, gw) if want_gw else (g_eps, g_sig)
+
+    def fwi_gradient(self, d_obs, g_eps=None, g_sig=None):
+        """Forward + L2 misfit + exact gradient, all on the device."""
+        m, n = self.grid.m, self.grid.n
+        if g_eps is None:
+            g_eps = np.zeros((m, n), dtype=np.float64, order="F")
+        if g_sig is None:
+            g_sig = np.zeros((m, n), dtype=np.float64, order="F")
+        if not hasattr(d_obs, "data_ptr"):
+            d_obs = np.ascontiguousarray(d_obs, dtype=np.float32)
+        _sync_torch(d_obs, g_eps, g_sig)
+        J = C.c_double(0)
+        N.check(N.load().gjw_fwi_gradient(self.h, N.ptr(d_obs), N.ptr(g_eps), N.ptr(g_sig),
+                                          C.byref(J)), "gjw_fwi_gradient")
+        return g_eps, g_sig, J.value
+
+    def last_timing(self):
+        ms, cells = C.c_double(0), C.c_double(0)
+        N.check(N.load().gjw_last_timing(self.h, C.byref(ms), C.byref(cells)), "gjw_last_timing")
+        return ms.value, cells.value

@@ -1,0 +1,146 @@
+/*
+ * gjw.h -- C ABI of libgjw: MI355X (gfx950) radar FDTD forward solve and
+ * adjoint gradient for gerjoii's src/w path.
+ *
+ * The entry points replace the numerics behind these MATLAB functions
+ * (reference: diegozain/gerjoii, paths relative to its root):
+ *   [finite_,gerjoii_] = w_fwd(geome_,parame_,finite_,gerjoii_)   src/w/w_fwd.m:1
+ *   [finite_,gerjoii_] = w_fwd_(geome_,parame_,finite_,gerjoii_,is) src/w/w_fwd_.m:1
+ *   gerjoii_ = w_solve(geome_,parame_,finite_,gerjoii_)             src/w/w_solve.m:1
+ *   [g_w,a_max] = w_grad(u_w,geome_,finite_,gerjoii_,parame_)      src/w/w_grad.m:1
+ *   [e,d_e,d_oe] = w_bozdag(d_o,d,obj_fnc)                         src/w/w_bozdag.m:1
+ *   Ew = w_E(d_o,d,Nw,obj_fnc)                                     src/w/w_E.m:1
+ *   v = differentiate_cube(v,dt)                  src/shared/w-dc/differentiate_cube.m:1
+ * A MEX gateway (or ctypes / torch) fills a gjw_grid from geome_/parame_/
+ * finite_, passes gerjoii_.w.{s,M,wvlet_} as the shot geometry, and reads
+ * traces and gradients back.
+ *
+ * Conventions
+ *  - Arrays are MATLAB column-major (depth fastest).  Model arrays are m x n
+ *    (rows = depth, parame_.w.epsilon / .sigma); traces of one shot are
+ *    nt x nr (gerjoii_.w.d_2d); shots are concatenated in shot order.
+ *  - Indices are MATLAB 1-based: a source is [iz, ix] on the padded Ez grid
+ *    (gerjoii_.w.s, src/w/w_load.m:31-32); a receiver is the linear index Mw
+ *    into the (nx+1) x (ny+1) Ez grid (src/w/w_M.m:20).
+ *  - Every pointer argument may be host memory or device memory of the
+ *    context's GPU; the library tells them apart itself.
+ *  - Every function returns GJW_OK (0) or a negative code; gjw_last_error()
+ *    returns a per-thread message describing the last failure.
+ *  - One context per process and GPU; calls on one context are serialised by
+ *    the caller (MATLAB calls MEX on one thread).
+ */
+#ifndef GJW_H
+#define GJW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GJW_OK 0
+#define GJW_ERR_ARG -1     /* invalid argument (size, index, option)     */
+#define GJW_ERR_HIP -2     /* HIP runtime error                          */
+#define GJW_ERR_STATE -3   /* call out of order (e.g. gradient before forward) */
+#define GJW_ERR_MEM -4     /* device memory exhausted                    */
+
+typedef struct gjw_ctx gjw_ctx;
+typedef struct gjw_survey gjw_survey;
+
+/* Geometry and physics of one padded grid.  Field names follow the MATLAB
+ * structs (src/w/w_geom.m:78-172, src/joint/wdc_geom.m:43-125). */
+typedef struct {
+  int32_t m;         /* geome_.m: model rows (depth, no PML, no air)      */
+  int32_t n;         /* geome_.n: model columns                           */
+  int32_t pml;       /* finite_.w.pml_w == finite_.w.pml_d                */
+  int32_t air;       /* parame_.w.air: air rows above the model           */
+  int32_t nt;        /* geome_.w.nt                                       */
+  int32_t pml_order; /* finite_.w.pml_order (2)                           */
+  double dx;         /* geome_.dx == geome_.dy  [m]                       */
+  double dt;         /* geome_.w.dt [s]                                   */
+  double eps0;       /* parame_.w.eps_0                                   */
+  double mu0;        /* parame_.w.mu_0                                    */
+  double R0;         /* finite_.w.R_0 (1e-8)                              */
+} gjw_grid;
+
+/* ---- context ---------------------------------------------------------- */
+int gjw_create(int32_t device, gjw_ctx **out);
+int gjw_destroy(gjw_ctx *ctx);
+const char *gjw_last_error(void);
+/* Library version string. */
+const char *gjw_version(void);
+
+/* ---- survey: one grid + its model + its shots -------------------------- */
+int gjw_survey_create(gjw_ctx *ctx, const gjw_grid *grid, gjw_survey **out);
+int gjw_survey_destroy(gjw_survey *s);
+
+/* Model (parame_.w.epsilon, parame_.w.sigma), m x n doubles.  Pads with air
+ * and PML as src/w/w_exp2pml.m:17-37 and builds the coefficients of
+ * src/w/w_cpml.m:44-185 and src/w/w_coeff.m:28-55 in FP64, stored as FP32. */
+int gjw_set_model(gjw_survey *s, const double *eps_r, const double *sigma);
+
+/* Shot geometry.  src: 2 x ns int32 ([iz; ix] per shot, 1-based);
+ * wavelets: nt x ns float (gerjoii_.w.wvlet_ per shot);
+ * rec_ptr: ns+1 int64 offsets into rec_lin; rec_lin: 1-based Mw per shot. */
+int gjw_set_shots(gjw_survey *s, int32_t ns, const int32_t *src, const float *wavelets,
+                  const int64_t *rec_ptr, const int32_t *rec_lin);
+
+/* Options (gjw_set_option keys). */
+#define GJW_OPT_PATH 1       /* GJW_PATH_BATCHED (default) | GJW_PATH_PLAIN        */
+#define GJW_OPT_CKPT_EVERY 2 /* checkpoint interval in steps; 0 = ceil(sqrt(nt))   */
+#define GJW_OPT_TILE_L 3     /* columns marched per wavefront; 0 = autotuned        */
+#define GJW_OPT_SNAPSHOT 4   /* GJW_SNAP_F32 (default) | GJW_SNAP_BF16 | GJW_SNAP_F16 */
+#define GJW_PATH_BATCHED 0   /* all shots per launch, checkpoint + recompute        */
+#define GJW_PATH_PLAIN 1     /* one shot at a time, full wavefield stored (as w_solve.m) */
+#define GJW_SNAP_F32 0
+#define GJW_SNAP_BF16 1
+#define GJW_SNAP_F16 2
+int gjw_set_option(gjw_survey *s, int32_t key, int64_t value);
+
+/* Forward solve of every shot (src/w/w_solve.m:80-118).
+ * traces: sum_s nt*nr_s floats (gerjoii_.w.d_2d per shot, concatenated);
+ * jz: nt x ns floats or NULL (gerjoii_.w.Jy, the field at the source).
+ * keep != 0 keeps the forward state (checkpoints, or the stored wavefield on
+ * the plain path) for a following gjw_gradient. */
+int gjw_forward(gjw_survey *s, float *traces, float *jz, int32_t keep);
+
+/* Misfit and adjoint source of the last forward (src/w/w_bozdag.m:13-55,
+ * src/w/w_E.m:7-16) against observed traces d_obs (same layout as traces).
+ * kind: GJW_MISFIT_L2.  e_out (optional) receives the adjoint source; J
+ * (optional) receives 0.5 * e'e in FP64.  The adjoint source stays on the
+ * device for gjw_gradient. */
+#define GJW_MISFIT_L2 0
+int gjw_misfit(gjw_survey *s, const float *d_obs, int32_t kind, float *e_out, double *J);
+
+/* Gradient of the misfit w.r.t. the model (m x n doubles, FP64 accumulation
+ * over time and shots).
+ * mode GJW_GRAD_EXACT: the exact discrete adjoint of the forward solve with
+ *   adjoint source adj (traces layout; NULL = the one left by gjw_misfit).
+ *   Strip (PML) coefficients are held fixed, i.e. the gradient covers every
+ *   model cell whose Ez is updated by the inner Yee formula.
+ * mode GJW_GRAD_REF: the reference's w_grad.m semantics (negated sigma,
+ *   residual written into Ez at the receivers, correlation with u for sigma
+ *   and with d_t u for eps); plain path only; a_max (m x n float, optional)
+ *   receives |a| of the last step (w_grad.m:110-112).
+ * g_w (optional, nt x ns doubles, EXACT only): dJ/d(wavelet). */
+#define GJW_GRAD_EXACT 0
+#define GJW_GRAD_REF 1
+int gjw_gradient(gjw_survey *s, const float *adj, int32_t mode, double *g_eps, double *g_sig,
+                 double *g_w, float *a_max);
+
+/* Forward + L2 misfit + exact gradient in one call, everything on the device
+ * (what w_update_e.m / w_update_s.m need per iteration, for all shots). */
+int gjw_fwi_gradient(gjw_survey *s, const float *d_obs, double *g_eps, double *g_sig, double *J);
+
+/* Time derivative of an m x n x nt cube in place, second order
+ * (src/shared/w-dc/differentiate_cube.m:23-51). */
+int gjw_differentiate_cube(gjw_ctx *ctx, float *v, int32_t nz, int32_t nx, int32_t nt, double dt);
+
+/* Device-time of the last gjw_forward / gjw_gradient / gjw_fwi_gradient
+ * call and the number of cell-updates it executed. */
+int gjw_last_timing(gjw_survey *s, double *ms, double *cell_updates);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GJW_H */

@@ -1,0 +1,401 @@
+"""CPU oracle for gerjoii's radar FDTD hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
+leg may import this module, and only as the checker.  The product package
+``gerjoii_amd`` never imports it.
+
+Two layers:
+
+* ctypes bindings to ``libgjw_oracle.so`` (built from ``gjw_oracle.c`` by
+  ``oracle/Makefile``): padding (w_exp2pml.m), coefficients (w_cpml.m,
+  w_coeff.m), the forward solve (w_solve.m), the reference gradient
+  (w_grad.m), differentiate_cube.m and the exact discrete adjoint.
+* numpy restatements of the small host-side pieces: geometry (w_geom.m),
+  the source wavelet (w_wavelet.m), index maps (w_load.m, w_M.m), the
+  misfits and adjoint sources (w_bozdag.m, instant_.m, w_E.m).
+
+Arrays follow the MATLAB layout: column-major (Fortran order), depth first.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgjw_oracle.so")
+
+EPS0 = 8.854187817e-12  # parame_.w.eps_0 as set in the project configs
+MU0 = 4e-7 * np.pi      # parame_.w.mu_0
+
+
+class _Dims(C.Structure):
+    _fields_ = [("m", C.c_int), ("n", C.c_int), ("P", C.c_int), ("air", C.c_int),
+                ("NX", C.c_int), ("NY", C.c_int), ("pis", C.c_int), ("pie", C.c_int),
+                ("pjs", C.c_int), ("pje", C.c_int)]
+
+
+class _Phys(C.Structure):
+    _fields_ = [("dx", C.c_double), ("dt", C.c_double), ("eps0", C.c_double),
+                ("mu0", C.c_double), ("R0", C.c_double), ("order", C.c_int),
+                ("nt", C.c_int)]
+
+
+_lib = None
+
+
+def build():
+    """Compile the oracle library (gcc; no GPU)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+    return _lib
+
+
+@dataclass
+class Grid:
+    """Geometry of one padded grid (w_geom.m:135-172, 1-based indices)."""
+    m: int            # model rows (depth)
+    n: int            # model columns (horizontal)
+    P: int            # pml_w == pml_d
+    air: int          # air rows above the model
+    dx: float
+    dt: float
+    nt: int
+    eps0: float = EPS0
+    mu0: float = MU0
+    R0: float = 1e-8  # w_geom.m:160
+    order: int = 2    # w_geom.m:159
+
+    @property
+    def NX(self):
+        return self.m + 2 * self.P + self.air
+
+    @property
+    def NY(self):
+        return self.n + 2 * self.P
+
+    @property
+    def pis(self):
+        return self.P + 1
+
+    @property
+    def pie(self):
+        return self.NX - self.P + 1
+
+    @property
+    def pjs(self):
+        return self.P + 1
+
+    @property
+    def pje(self):
+        return self.NY - self.P + 1
+
+    @property
+    def domain_i(self):
+        """1-based rows of the stored domain, w_geom.m:168."""
+        return np.arange(self.pis + self.air, self.pie)
+
+    @property
+    def domain_j(self):
+        """1-based columns of the stored domain, w_geom.m:169."""
+        return np.arange(self.pjs, self.pje)
+
+    def _c(self):
+        d = _Dims()
+        lib().orc_dims_make(self.m, self.n, self.P, self.air, C.byref(d))
+        p = _Phys(self.dx, self.dt, self.eps0, self.mu0, self.R0, self.order, self.nt)
+        return d, p
+
+
+def dt_cfl(dx, eps_min=1.0, courant=0.9):
+    """w_geom.m:99-122: dt = 0.9 / (v_max sqrt(1/dx^2 + 1/dy^2)), dy = dx."""
+    c = 1.0 / np.sqrt(MU0 * EPS0)
+    vmax = c / np.sqrt(eps_min)
+    return courant * 1.0 / (vmax * np.sqrt(1.0 / dx**2 + 1.0 / dx**2))
+
+
+def w_wavelet(t, ampli, tau):
+    """w_wavelet.m:11-32 (t in seconds; tau = 1/fo before the /pi)."""
+    tau = tau / np.pi
+    to = 5 * tau
+    w = -(t - to) * np.exp(-(((t - to) ** 2) / (tau**2)))
+    return ampli * (w / np.max(np.abs(w)))
+
+
+def sub2ind_Mw(NX, NY, rz, rx):
+    """w_M.m:20: Mw = sub2ind([nx+1, ny+1], rz, rx) (1-based)."""
+    rz = np.asarray(rz, dtype=np.int64)
+    rx = np.asarray(rx, dtype=np.int64)
+    return (rz - 1) + (rx - 1) * (NX + 1) + 1
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _real(dtype):
+    dtype = np.dtype(dtype)
+    if dtype == np.float64:
+        return "_d", C.c_double
+    if dtype == np.float32:
+        return "_f", C.c_float
+    raise TypeError(dtype)
+
+
+def exp2pml(g: Grid, eps_model, sig_model, dtype=np.float64):
+    """w_exp2pml.m: pad materials with air and PML (NX x NY, column-major)."""
+    sfx, ct = _real(dtype)
+    em = np.asfortranarray(eps_model, dtype=np.float64)
+    sm = np.asfortranarray(sig_model, dtype=np.float64)
+    assert em.shape == (g.m, g.n) and sm.shape == (g.m, g.n)
+    eo = np.zeros((g.NX, g.NY), dtype=dtype, order="F")
+    so = np.zeros((g.NX, g.NY), dtype=dtype, order="F")
+    getattr(lib(), "orc_exp2pml" + sfx)(g.m, g.n, g.P, g.air, _ptr(em, C.c_double),
+                                        _ptr(sm, C.c_double), _ptr(eo, ct), _ptr(so, ct))
+    return eo, so
+
+
+COEFF_NAMES = ["Ceze", "Cezhy", "Cezhx", "Chyh_xn", "Chyez_xn", "Cezxe_xn", "Cezxhy_xn",
+               "Chyh_xp", "Chyez_xp", "Cezxe_xp", "Cezxhy_xp", "Cezyhx_xp", "Chxh_yn",
+               "Chxez_yn", "Cezxhy_yn", "Cezye_yn", "Cezyhx_yn", "Chxh_yp", "Chxez_yp",
+               "Cezxhy_yp", "Cezye_yp", "Cezyhx_yp"]
+SCALAR_NAMES = ["Chxh", "Chxez", "Chyh", "Chyez", "Cezj", "Cezye_xn", "Cezyhx_xn",
+                "Cezye_xp", "Cezxe_yn", "Cezxe_yp"]
+
+
+def coeff_shape(g: Grid, name):
+    NX, NY, P = g.NX, g.NY, g.P
+    if name in ("Ceze", "Cezhy", "Cezhx"):
+        return (NX, NY)
+    if name.endswith("_xn") or name.endswith("_xp"):
+        return (P, NY - 1 - 2 * P) if name == "Cezyhx_xp" else (P, NY - 1)
+    if name in ("Cezxhy_yn", "Cezxhy_yp"):
+        return (NX - 1 - 2 * P, P)
+    return (NX - 1, P)
+
+
+def coeffs(g: Grid, eps_pad, sig_pad, src, dtype=np.float64):
+    """w_cpml.m + w_coeff.m: dict of coefficient arrays (reference shapes)."""
+    sfx, ct = _real(dtype)
+    d, p = g._c()
+    ep = np.asfortranarray(eps_pad, dtype=dtype)
+    sp = np.asfortranarray(sig_pad, dtype=dtype)
+    out = {}
+    scal = np.zeros(10)
+    for k, name in enumerate(COEFF_NAMES):
+        arr = np.zeros(coeff_shape(g, name), dtype=dtype, order="F")
+        rc = getattr(lib(), "orc_coeff_export" + sfx)(
+            C.byref(d), C.byref(p), _ptr(ep, ct), _ptr(sp, ct), int(src[0]), int(src[1]),
+            k, _ptr(arr, ct), _ptr(scal, C.c_double))
+        assert rc == 0, rc
+        out[name] = arr
+    for k, name in enumerate(SCALAR_NAMES):
+        out[name] = scal[k]
+    return out
+
+
+def forward(g: Grid, eps_pad, sig_pad, src, wvlet, Mw, store_u=False, dtype=np.float64,
+            percell=False):
+    """w_solve.m: returns (d [nt x nr], Jz [nt], u [m x n x nt] or None)."""
+    sfx, ct = _real(dtype)
+    d_, p = g._c()
+    ep = np.asfortranarray(eps_pad, dtype=dtype)
+    sp = np.asfortranarray(sig_pad, dtype=dtype)
+    w = np.ascontiguousarray(wvlet, dtype=dtype)
+    mw = np.ascontiguousarray(Mw, dtype=np.int32)
+    nr = mw.size
+    d = np.zeros((g.nt, nr), dtype=dtype, order="F")
+    if percell:
+        rc = getattr(lib(), "orc_forward_percell" + sfx)(
+            C.byref(d_), C.byref(p), _ptr(ep, ct), _ptr(sp, ct), int(src[0]), int(src[1]),
+            _ptr(w, ct), nr, _ptr(mw, C.c_int), _ptr(d, ct))
+        assert rc == 0, rc
+        return d
+    jz = np.zeros(g.nt, dtype=dtype)
+    u = np.zeros((g.m, g.n, g.nt), dtype=dtype, order="F") if store_u else None
+    rc = getattr(lib(), "orc_forward" + sfx)(
+        C.byref(d_), C.byref(p), _ptr(ep, ct), _ptr(sp, ct), int(src[0]), int(src[1]),
+        _ptr(w, ct), nr, _ptr(mw, C.c_int), _ptr(d, ct), _ptr(jz, ct),
+        _ptr(u, ct) if store_u else None)
+    assert rc == 0, rc
+    return d, jz, u
+
+
+def differentiate_cube(v, dt):
+    """differentiate_cube.m (returns a new array; the reference works in place)."""
+    v = np.array(v, order="F", copy=True)
+    sfx, ct = _real(v.dtype)
+    nz, nx, nt = v.shape
+    rc = getattr(lib(), "orc_differentiate_cube" + sfx)(_ptr(v, ct), nz, nx, nt, C.c_double(dt))
+    assert rc == 0, rc
+    return v
+
+
+def grad_ref(g: Grid, eps_pad, sig_pad, src, Mw, e, u, dtype=np.float64):
+    """w_grad.m: returns (g_w [m x n], a_max [m x n])."""
+    sfx, ct = _real(dtype)
+    d_, p = g._c()
+    ep = np.asfortranarray(eps_pad, dtype=dtype)
+    sp = np.asfortranarray(sig_pad, dtype=dtype)
+    mw = np.ascontiguousarray(Mw, dtype=np.int32)
+    ee = np.asfortranarray(e, dtype=dtype)
+    uu = np.asfortranarray(u, dtype=dtype)
+    gw = np.zeros((g.m, g.n), dtype=dtype, order="F")
+    am = np.zeros((g.m, g.n), dtype=dtype, order="F")
+    rc = getattr(lib(), "orc_grad_ref" + sfx)(
+        C.byref(d_), C.byref(p), _ptr(ep, ct), _ptr(sp, ct), int(src[0]), int(src[1]),
+        mw.size, _ptr(mw, C.c_int), _ptr(ee, ct), _ptr(uu, ct), _ptr(gw, ct), _ptr(am, ct))
+    assert rc == 0, rc
+    return gw, am
+
+
+def grad_exact(g: Grid, eps_pad, sig_pad, src, wvlet, Mw, e, K=None, pml_terms=True,
+               dtype=np.float64):
+    """Exact discrete adjoint of w_solve.m.
+
+    Returns (g_eps_pad, g_sig_pad, g_w): dJ/d eps_r_z, dJ/d sigma_e_z (NX x NY)
+    and dJ/d wavelet (nt), for dJ/dd = e (nt x nr)."""
+    sfx, ct = _real(dtype)
+    d_, p = g._c()
+    if K is None:
+        K = max(1, int(np.sqrt(g.nt)))
+    ep = np.asfortranarray(eps_pad, dtype=dtype)
+    sp = np.asfortranarray(sig_pad, dtype=dtype)
+    w = np.ascontiguousarray(wvlet, dtype=dtype)
+    mw = np.ascontiguousarray(Mw, dtype=np.int32)
+    ee = np.asfortranarray(e, dtype=dtype)
+    ge = np.zeros((g.NX, g.NY), order="F")
+    gs = np.zeros((g.NX, g.NY), order="F")
+    gw = np.zeros(g.nt)
+    rc = getattr(lib(), "orc_grad_exact" + sfx)(
+        C.byref(d_), C.byref(p), _ptr(ep, ct), _ptr(sp, ct), int(src[0]), int(src[1]),
+        _ptr(w, ct), mw.size, _ptr(mw, C.c_int), _ptr(ee, ct), int(K), int(bool(pml_terms)),
+        _ptr(ge, C.c_double), _ptr(gs, C.c_double), _ptr(gw, C.c_double))
+    assert rc == 0, rc
+    return ge, gs, gw
+
+
+def fold_pad(g: Grid, g_eps_pad, g_sig_pad):
+    """Chain rule of w_exp2pml.m: padded-grid gradient -> m x n model gradient."""
+    d_, _ = g._c()
+    ge = np.asfortranarray(g_eps_pad, dtype=np.float64)
+    gs = np.asfortranarray(g_sig_pad, dtype=np.float64)
+    oe = np.zeros((g.m, g.n), order="F")
+    os_ = np.zeros((g.m, g.n), order="F")
+    lib().orc_fold_pad(C.byref(d_), _ptr(ge, C.c_double), _ptr(gs, C.c_double),
+                       _ptr(oe, C.c_double), _ptr(os_, C.c_double))
+    return oe, os_
+
+
+def survey_grad_f32(g: Grid, eps_pad, sig_pad, srcs, wvlets, Mws, e_all, K):
+    """CPU baseline: float oracle, OpenMP over shots; returns (g_eps_pad,
+    g_sig_pad, threads)."""
+    d_, p = g._c()
+    ep = np.asfortranarray(eps_pad, dtype=np.float32)
+    sp = np.asfortranarray(sig_pad, dtype=np.float32)
+    ns = len(srcs)
+    src = np.ascontiguousarray(np.asarray(srcs, dtype=np.int32).reshape(ns, 2))
+    wv = np.ascontiguousarray(np.asarray(wvlets, dtype=np.float32).reshape(ns, g.nt))
+    mw = np.ascontiguousarray(np.asarray(Mws, dtype=np.int32))
+    nr = mw.shape[1]
+    ea = np.ascontiguousarray(np.asarray(e_all, dtype=np.float32))  # ns x nr x nt
+    ge = np.zeros((g.NX, g.NY), order="F")
+    gs = np.zeros((g.NX, g.NY), order="F")
+    thr = C.c_int(0)
+    rc = lib().orc_survey_grad_f(C.byref(d_), C.byref(p), _ptr(ep, C.c_float), _ptr(sp, C.c_float),
+                                 ns, _ptr(src, C.c_int), _ptr(wv, C.c_float), nr,
+                                 _ptr(mw, C.c_int), _ptr(ea, C.c_float), int(K),
+                                 _ptr(ge, C.c_double), _ptr(gs, C.c_double), C.byref(thr))
+    assert rc == 0, rc
+    return ge, gs, thr.value
+
+
+# --------------------------------------------------------------------------
+# misfits and adjoint sources: w_bozdag.m, instant_.m, w_E.m
+# --------------------------------------------------------------------------
+
+def hilbert_analytic(x, axis=0):
+    """MATLAB ``hilbert``: analytic signal by FFT of length N along ``axis``:
+    X * h with h = [1, 2 .. 2, (1 if N even), 0 ..], then IFFT."""
+    x = np.asarray(x, dtype=np.float64)
+    N = x.shape[axis]
+    X = np.fft.fft(x, axis=axis)
+    h = np.zeros(N)
+    if N % 2 == 0:
+        h[0] = h[N // 2] = 1
+        h[1:N // 2] = 2
+    else:
+        h[0] = 1
+        h[1:(N + 1) // 2] = 2
+    shape = [1] * x.ndim
+    shape[axis] = N
+    return np.fft.ifft(X * h.reshape(shape), axis=axis)
+
+
+def instant_(u):
+    """instant_.m:14-20: (envelope, phase, Hilbert transform), column-wise."""
+    a = hilbert_analytic(u, axis=0)
+    return np.abs(a), np.angle(a), np.imag(a)
+
+
+def w_bozdag(d_o, d, obj_fnc="L2"):
+    """w_bozdag.m:13-55: returns (e, d_e, d_oe)."""
+    d_o = np.asarray(d_o, dtype=np.float64)
+    d = np.asarray(d, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if obj_fnc in ("L2", "FREQstp"):
+            return d - d_o, d, d_o
+        if obj_fnc == "env_L2":
+            d_e, _, d_H = instant_(d)
+            d_oe = instant_(d_o)[0]
+            dedoe = (d_e - d_oe) / d_e
+            e__ = np.imag(hilbert_analytic(dedoe * d_H, axis=0))
+            e = d * dedoe - e__
+            e[np.isnan(e)] = 0
+            return e, d_e, d_oe
+        if obj_fnc == "phase_L2":
+            d_e, d_, d_H = instant_(d)
+            d_oe = instant_(d_o)[1]
+            d_do_ = (d_ - d_oe) / (d_e**2)
+            e__ = np.imag(hilbert_analytic(d_do_ * d, axis=0))
+            e = -d_do_ + e__
+            return e, d_, d_oe
+        if obj_fnc == "env_L22":
+            d_e, _, d_H = instant_(d)
+            d_oe = instant_(d_o)[0]
+            dedoe = d_e**2 - d_oe**2
+            e__ = np.imag(hilbert_analytic(dedoe * d_H, axis=0))
+            return d * dedoe - e__, d_e, d_oe
+        if obj_fnc == "env_LOG":
+            d_e, _, d_H = instant_(d)
+            d_oe = instant_(d_o)[0]
+            dedoe = np.real(np.log((d_oe / d_e).astype(complex)))
+            e__ = np.imag(hilbert_analytic(dedoe * (d_H / (d_e**2)), axis=0))
+            e = -dedoe * (d / (d_e**2)) + e__
+            e[np.isnan(e)] = 0
+            return e, d_e, d_oe
+    raise ValueError(obj_fnc)
+
+
+def w_E(d_o, d, Nw=None, obj_fnc="L2"):
+    """w_E.m:7-16: 0.5 e^T N e in double (N = identity when None)."""
+    d = np.asarray(d, dtype=np.float64).ravel(order="F")
+    d_o = np.asarray(d_o, dtype=np.float64).ravel(order="F")
+    if obj_fnc == "env_L22":
+        d, d_o = d**2, d_o**2
+    elif obj_fnc == "env_LOG":
+        d, d_o = np.log(d), np.log(d_o)
+    e = d - d_o
+    if Nw is None:
+        return 0.5 * float(e @ e)
+    return 0.5 * float(e @ (Nw @ e))

@@ -1,0 +1,178 @@
+"""GPU parity: libgjw (HIP, FP32) against the FP64 CPU oracle, through the C ABI.
+
+Tolerances (written per test): traces relative L2 <= 1e-4; gradients relative
+L2 <= 1e-3 and cosine >= 0.9999 against the FP64 exact adjoint.
+"""
+import numpy as np
+import pytest
+
+from gerjoii_amd import configs
+from gerjoii_amd import survey as S
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def cosine(a, b):
+    a, b = np.ravel(a), np.ravel(b)
+    return float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+
+
+def oracle_grid(O, g):
+    return O.Grid(g.m, g.n, g.pml, g.air, g.dx, g.dt, g.nt, eps0=g.eps0, mu0=g.mu0, R0=g.R0,
+                  order=g.pml_order)
+
+
+def oracle_forward(O, case, eps, sig):
+    g = oracle_grid(O, case.grid)
+    ep, sp = O.exp2pml(g, eps, sig)
+    out = []
+    for q in range(len(case.src)):
+        d, _, _ = O.forward(g, ep, sp, case.src[q], case.wavelets[q], case.receivers[q])
+        out.append(d.T)  # (nr, nt), traces layout
+    return np.concatenate(out, axis=0)
+
+
+def oracle_grad(O, case, eps, sig, adj, rec_ptr):
+    g = oracle_grid(O, case.grid)
+    ep, sp = O.exp2pml(g, eps, sig)
+    ge = np.zeros((g.NX, g.NY))
+    gs = np.zeros((g.NX, g.NY))
+    gws = []
+    for q in range(len(case.src)):
+        e = adj[rec_ptr[q]:rec_ptr[q + 1]].T.astype(np.float64)
+        a, b, gw = O.grad_exact(g, ep, sp, case.src[q], case.wavelets[q], case.receivers[q], e,
+                                pml_terms=False)
+        ge += a
+        gs += b
+        gws.append(gw)
+    fe, fs = O.fold_pad(g, ge, gs)
+    return fe, fs, np.array(gws)
+
+
+def make_survey(case, ctx, plain=False, ck=0, L=0):
+    sv = S.Survey(case.grid, ctx)
+    sv.set_model(case.eps0, case.sig0)
+    sv.set_shots(case.src, case.wavelets, case.receivers)
+    sv.set_path(plain)
+    if ck:
+        sv.set_option(2, ck)
+    if L:
+        sv.set_option(3, L)
+    return sv
+
+
+@pytest.mark.parametrize("air", [0, 4])
+def test_forward_small(oracle_mod, gpu_ctx, air):
+    case = configs.small(air=air)
+    print(f"forward small air={air} grid={case.grid.nx}x{case.grid.ny} nt={case.grid.nt}", flush=True)
+    sv = make_survey(case, gpu_ctx)
+    d = sv.forward()
+    ref = oracle_forward(oracle_mod, case, case.eps0, case.sig0)
+    err = rel_l2(d, ref)
+    print("trace rel L2", err)
+    assert np.abs(ref).max() > 0
+    assert err <= 1e-4
+
+
+@pytest.mark.parametrize("L", [8, 16, 64])
+def test_forward_tiles_agree(gpu_ctx, L):
+    """Every tile width computes the same traces (fast and general paths)."""
+    case = configs.small(m=150, n=140, P=12, nt=200)
+    base = make_survey(case, gpu_ctx, L=4).forward()
+    d = make_survey(case, gpu_ctx, L=L).forward()
+    assert rel_l2(d, base) <= 1e-6
+
+
+def test_forward_plain_equals_batched(gpu_ctx):
+    case = configs.small()
+    a = make_survey(case, gpu_ctx).forward()
+    b = make_survey(case, gpu_ctx, plain=True).forward()
+    assert rel_l2(a, b) <= 1e-6
+
+
+def test_forward_c1(oracle_mod, gpu_ctx):
+    case = configs.c1()
+    print("forward C1", flush=True)
+    sv = make_survey(case, gpu_ctx)
+    d = sv.forward()
+    ref = oracle_forward(oracle_mod, case, case.eps0, case.sig0)
+    err = rel_l2(d, ref)
+    print("C1 trace rel L2", err)
+    assert err <= 1e-4
+
+
+@pytest.mark.parametrize("plain", [False, True])
+@pytest.mark.parametrize("air", [0, 3])
+def test_gradient_small(oracle_mod, gpu_ctx, plain, air):
+    case = configs.small(air=air, nt=240)
+    print(f"gradient small plain={plain} air={air}", flush=True)
+    sv = make_survey(case, gpu_ctx, plain=plain, ck=7)
+    sv.forward(keep=True)
+    rng = np.random.default_rng(3)
+    adj = rng.standard_normal(sv.traces_shape()).astype(np.float32)
+    ge, gs, gw = sv.gradient(adj, want_gw=True)
+    fe, fs, fw = oracle_grad(oracle_mod, case, case.eps0, case.sig0, adj, sv.rec_ptr)
+    print("eps", rel_l2(ge, fe), cosine(ge, fe), "sig", rel_l2(gs, fs), cosine(gs, fs),
+          "w", rel_l2(gw, fw))
+    assert rel_l2(ge, fe) <= 1e-3 and cosine(ge, fe) >= 0.9999
+    assert rel_l2(gs, fs) <= 1e-3 and cosine(gs, fs) >= 0.9999
+    assert rel_l2(gw, fw) <= 1e-3
+
+
+def test_dot_product_gpu(gpu_ctx):
+    """<F w, e> = <w, F^T e> for the wavelet -> traces map, in FP32 on the GPU."""
+    case = configs.small(nt=300)
+    sv = make_survey(case, gpu_ctx, ck=17)
+    d = sv.forward(keep=True).astype(np.float64)
+    rng = np.random.default_rng(5)
+    adj = rng.standard_normal(d.shape).astype(np.float32)
+    _, _, gw = sv.gradient(adj, want_gw=True)
+    lhs = float((d * adj).sum())
+    rhs = float((case.wavelets.astype(np.float32).astype(np.float64) * gw).sum())
+    print("dot", lhs, rhs)
+    assert abs(lhs - rhs) <= 1e-4 * abs(lhs)
+
+
+def test_fwi_gradient_l2(oracle_mod, gpu_ctx):
+    """Forward + misfit + gradient in one call equals the three-call sequence
+    and the oracle's gradient of the L2 misfit."""
+    case = configs.small(nt=240, ns=2)
+    sv = make_survey(case, gpu_ctx, ck=9)
+    svt = make_survey(case, gpu_ctx)
+    svt.set_model(case.eps_true, case.sig_true)
+    dobs = svt.forward()
+    ge, gs, J = sv.fwi_gradient(dobs)
+    d = sv.forward(keep=True)
+    e, J2 = sv.misfit(dobs)
+    np.testing.assert_allclose(e, d - dobs, rtol=0, atol=1e-6 * np.abs(d).max())
+    assert abs(J - J2) <= 1e-9 * abs(J) and abs(J - 0.5 * float((e.astype(np.float64) ** 2).sum())) <= 1e-6 * J
+    fe, fs, _ = oracle_grad(oracle_mod, case, case.eps0, case.sig0, e, sv.rec_ptr)
+    assert rel_l2(ge, fe) <= 1e-3 and cosine(ge, fe) >= 0.9999
+    assert rel_l2(gs, fs) <= 1e-3
+
+
+def test_differentiate_cube_gpu(oracle_mod, gpu_ctx):
+    rng = np.random.default_rng(2)
+    for nt in (7, 8):
+        u = np.asfortranarray(rng.standard_normal((5, 4, nt)).astype(np.float32))
+        ref = oracle_mod.differentiate_cube(u.astype(np.float64), 0.1)
+        v = u.copy(order="F")
+        gpu_ctx.differentiate_cube(v, 0.1)
+        assert rel_l2(v, ref) <= 1e-6
+
+
+def test_errors_are_reported(gpu_ctx):
+    from gerjoii_amd import _native as N
+    case = configs.small()
+    sv = S.Survey(case.grid, gpu_ctx)
+    with pytest.raises(N.GjwError, match="set_model"):
+        sv.forward()
+    sv.set_model(case.eps0, case.sig0)
+    bad = case.src.copy()
+    bad[0] = (1, 5)  # on the never-updated outer ring
+    with pytest.raises(N.GjwError, match="source"):
+        sv.set_shots(bad, case.wavelets, case.receivers)

@@ -1,0 +1,251 @@
+"""CPU tests that pin the oracle (no GPU).
+
+The reference ships no tests, fixtures or golden vectors for this path, so
+each oracle function is pinned by an independent check:
+* the slice-literal numpy reading of w_solve.m / w_grad.m / w_cpml.m /
+  w_coeff.m / w_exp2pml.m (bit for bit in FP64);
+* closed forms (coefficients, PML profile, wavelet);
+* mathematics: linearity and mirror symmetry of the solver, the energy decay
+  of a lossy medium, the dot-product test and central finite differences for
+  the exact adjoint, numpy.gradient(edge_order=2) for differentiate_cube.m,
+  scipy.signal.hilbert for instant_.m;
+* the committed golden fixtures (tests/golden/), produced by this oracle and
+  re-checked here so that a later change to it is caught.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import literal_np as L
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def O(oracle_mod):
+    return oracle_mod
+
+
+def tiny(O, m=8, n=12, P=3, air=2, nt=40, seed=0):
+    rng = np.random.default_rng(seed)
+    dx = 0.05
+    g = O.Grid(m, n, P, air, dx, O.dt_cfl(dx), nt)
+    eps = rng.uniform(4, 16, (m, n))
+    sig = rng.uniform(0, 1e-2, (m, n))
+    w = O.w_wavelet(np.arange(nt) * g.dt, 1e3, 1 / 300e6)
+    s = (P + air + 2, P + 4)
+    Mw = O.sub2ind_Mw(g.NX, g.NY, np.full(5, P + air + 2), P + 1 + np.arange(5) * 2)
+    return g, eps, sig, w, s, Mw
+
+
+def test_exp2pml_literal(O):
+    g, eps, sig, *_ = tiny(O)
+    a = O.exp2pml(g, eps, sig)
+    b = L.w_exp2pml(eps, sig, g.P, g.air)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_coefficients_literal_and_closed_form(O):
+    g, eps, sig, w, s, Mw = tiny(O)
+    ep, sp = O.exp2pml(g, eps, sig)
+    k = O.coeffs(g, ep, sp, s)
+    k2 = L.w_cpml(ep, g.NX, g.NY, g.P, g.dx, g.dt, g.eps0, g.mu0, g.R0, g.order)
+    k2.update(L.w_coeff(ep, sp, s, g.dx, g.dt, g.eps0, g.mu0))
+    for name in O.COEFF_NAMES:
+        assert np.array_equal(k[name], k2[name]), name
+    # closed forms: lossless cells give Ceze = 1; Cezhy = dt / (eps dx) there
+    D = 2 * ep * g.eps0 + g.dt * sp
+    np.testing.assert_allclose(k["Cezhy"], 2 * g.dt / g.dx / D, rtol=1e-15)
+    np.testing.assert_allclose(k["Ceze"][sp == 0], 1.0, rtol=0, atol=0)
+    assert k["Chxh"] == 1.0 and k["Chyh"] == 1.0
+    np.testing.assert_allclose(k["Chyez"], g.dt / (g.mu0 * g.dx), rtol=1e-15)
+    # PML profile: sigma_max from R0 and a quadratic grading, outermost cell largest
+    c = 1 / np.sqrt(g.mu0 * g.eps0)
+    smax = -3 * g.eps0 * c * np.log(g.R0) / (2 * g.dx * g.P)
+    se = smax * ((np.arange(g.P, 0, -1) - 0.75) / g.P) ** 2
+    np.testing.assert_allclose(k["Cezxe_xn"][:, 0], (2 * g.eps0 - g.dt * se) / (2 * g.eps0 + g.dt * se),
+                               rtol=1e-14)
+    assert np.all(np.diff(k["Cezxe_xn"][:, 0]) > 0)  # decay weakest next to the interior
+
+
+def test_forward_literal_bitwise(O):
+    g, eps, sig, w, s, Mw = tiny(O)
+    ep, sp = O.exp2pml(g, eps, sig)
+    d, jz, u = O.forward(g, ep, sp, s, w, Mw, store_u=True)
+    d2, jz2, u2 = L.w_solve(ep, sp, s, w, Mw, g.P, g.air, g.dx, g.dt, g.eps0, g.mu0)
+    assert np.abs(d).max() > 0
+    assert np.array_equal(d, d2) and np.array_equal(u, u2) and np.array_equal(jz, jz2)
+    # the per-cell form used by the adjoint and by the GPU kernels
+    assert np.array_equal(O.forward(g, ep, sp, s, w, Mw, percell=True), d)
+
+
+def test_grad_ref_literal_bitwise(O):
+    g, eps, sig, w, s, Mw = tiny(O)
+    ep, sp = O.exp2pml(g, eps, sig)
+    _, _, u = O.forward(g, ep, sp, s, w, Mw, store_u=True)
+    e = np.random.default_rng(1).standard_normal((g.nt, len(Mw)))
+    du = O.differentiate_cube(u, g.dt)
+    a = O.grad_ref(g, ep, sp, s, Mw, e, du)
+    b = L.w_grad(du, e, ep, sp, s, Mw, g.P, g.air, g.dx, g.dt, g.eps0, g.mu0)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_forward_linear_and_mirror(O):
+    g, eps, sig, w, s, Mw = tiny(O, m=10, n=15, P=3, air=0, nt=60)
+    ep, sp = O.exp2pml(g, eps, sig)
+    d1 = O.forward(g, ep, sp, s, w, Mw)[0]
+    d2 = O.forward(g, ep, sp, s, 2.5 * w, Mw)[0]
+    np.testing.assert_allclose(d2, 2.5 * d1, rtol=1e-12, atol=1e-12 * np.abs(d1).max())
+    # homogeneous medium: mirroring source and receivers across the grid's
+    # vertical axis (Ez column j <-> NY+2-j) mirrors the traces; this checks
+    # the left and right strips against each other
+    ep, sp = O.exp2pml(g, np.full_like(eps, 7.0), np.full_like(sig, 4e-3))
+    d1 = O.forward(g, ep, sp, s, w, Mw)[0]
+    sm = (s[0], g.NY + 2 - s[1])
+    rz = (np.asarray(Mw) - 1) % (g.NX + 1) + 1
+    rx = (np.asarray(Mw) - 1) // (g.NX + 1) + 1
+    dm = O.forward(g, ep, sp, sm, w, O.sub2ind_Mw(g.NX, g.NY, rz, g.NY + 2 - rx))[0]
+    np.testing.assert_allclose(dm, d1, rtol=1e-9, atol=1e-9 * np.abs(d1).max())
+
+
+def test_lossy_decay(O):
+    """With sigma > 0 the late-time field energy is below the lossless one."""
+    g, eps, sig, w, s, Mw = tiny(O, m=20, n=20, P=4, air=0, nt=200)
+    ep, sp0 = O.exp2pml(g, eps, 0 * sig)
+    _, _, u0 = O.forward(g, ep, sp0, s, w, Mw, store_u=True)
+    ep, sp1 = O.exp2pml(g, eps, 0 * sig + 0.05)
+    _, _, u1 = O.forward(g, ep, sp1, s, w, Mw, store_u=True)
+    assert (u1[..., -50:] ** 2).sum() < 0.5 * (u0[..., -50:] ** 2).sum()
+
+
+def test_differentiate_cube_numpy_gradient(O):
+    rng = np.random.default_rng(4)
+    for nt in (3, 4, 9, 10):
+        u = rng.standard_normal((3, 5, nt))
+        np.testing.assert_allclose(O.differentiate_cube(u, 0.3),
+                                   np.gradient(u, 0.3, axis=2, edge_order=2), rtol=1e-13, atol=1e-13)
+
+
+def exact_setup(O, air=2):
+    rng = np.random.default_rng(11)
+    m, n, P, nt = 9, 13, 4, 110
+    dx = 0.05
+    g = O.Grid(m, n, P, air, dx, O.dt_cfl(dx), nt)
+    eps = rng.uniform(4, 16, (m, n))
+    sig = rng.uniform(0, 3e-2, (m, n))
+    w = O.w_wavelet(np.arange(nt) * g.dt, 1e3, 1 / 300e6)
+    s = (P + air + 3, P + 5)
+    rz = np.array([P + air + 2, P + air + 2, P + air + 5, 2, P + air + 1, g.NX])
+    rx = np.array([P + 2, P + 9, P + 12, P + 6, P + 1, 5])
+    Mw = O.sub2ind_Mw(g.NX, g.NY, rz, rx)
+    return g, eps, sig, w, s, Mw
+
+
+def test_exact_adjoint_dot_product(O):
+    g, eps, sig, w, s, Mw = exact_setup(O)
+    ep, sp = O.exp2pml(g, eps, sig)
+    d = O.forward(g, ep, sp, s, w, Mw)[0]
+    e = np.random.default_rng(2).standard_normal(d.shape)
+    for K in (1, 7, g.nt):
+        _, _, gw = O.grad_exact(g, ep, sp, s, w, Mw, e, K=K)
+        lhs, rhs = (d * e).sum(), (w * gw).sum()
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (K, lhs, rhs)
+
+
+@pytest.mark.parametrize("air", [0, 2])
+def test_exact_gradient_finite_differences(O, air):
+    g, eps, sig, w, s, Mw = exact_setup(O, air)
+    ep, sp = O.exp2pml(g, eps, sig)
+    d = O.forward(g, ep, sp, s, w, Mw)[0]
+    e = np.random.default_rng(3).standard_normal(d.shape)
+    ge, gs, _ = O.grad_exact(g, ep, sp, s, w, Mw, e, K=9)
+    gem, gsm = O.fold_pad(g, ge, gs)
+
+    def J(E, Sg):
+        a, b = O.exp2pml(g, E, Sg)
+        return float((O.forward(g, a, b, s, w, Mw)[0] * e).sum())
+
+    cells = [(0, 0), (0, 5), (g.m - 1, g.n - 1), (3, 0), (5, 7), (g.m - 1, 3), (2, g.n - 1),
+             (s[0] - g.P - air - 1, s[1] - g.P - 1)]
+    for a, b in cells:
+        h = 1e-6 * eps[a, b]
+        Ep, Em = eps.copy(), eps.copy()
+        Ep[a, b] += h
+        Em[a, b] -= h
+        fd = (J(Ep, sig) - J(Em, sig)) / (2 * h)
+        assert abs(fd - gem[a, b]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(gem).max()), (a, b, fd, gem[a, b])
+        hs = 1e-6
+        Sp, Sm = sig.copy(), sig.copy()
+        Sp[a, b] += hs
+        Sm[a, b] -= hs
+        fds = (J(eps, Sp) - J(eps, Sm)) / (2 * hs)
+        assert abs(fds - gsm[a, b]) <= 1e-5 * max(abs(fds), 1e-3 * np.abs(gsm).max()), (a, b, fds, gsm[a, b])
+
+
+def test_exact_without_strip_terms_matches_interior_fd(O):
+    """pml_terms=False (strip coefficients held fixed, what the GPU computes)
+    equals the full gradient away from the model edges and the first column."""
+    g, eps, sig, w, s, Mw = exact_setup(O, 2)
+    ep, sp = O.exp2pml(g, eps, sig)
+    e = np.random.default_rng(3).standard_normal((g.nt, len(Mw)))
+    full = O.fold_pad(g, *O.grad_exact(g, ep, sp, s, w, Mw, e, K=9)[:2])
+    part = O.fold_pad(g, *O.grad_exact(g, ep, sp, s, w, Mw, e, K=9, pml_terms=False)[:2])
+    np.testing.assert_allclose(part[0][1:-1, 1:-1], full[0][1:-1, 1:-1], rtol=1e-12, atol=1e-12 * np.abs(full[0]).max())
+    np.testing.assert_allclose(part[1], full[1], rtol=1e-12, atol=0)
+
+
+def test_wavelet_closed_form(O):
+    f0 = 100e6
+    t = np.linspace(0, 60e-9, 601)
+    w = O.w_wavelet(t, 1e3, 1 / f0)
+    tau = 1 / (np.pi * f0)
+    assert np.isclose(np.abs(w).max(), 1e3)
+    # derivative-of-Gaussian: zero crossing at t0 = 5 tau, extrema at t0 +- tau/sqrt(2)
+    k = np.argmax(w)
+    assert abs(t[k] - (5 * tau - tau / np.sqrt(2))) <= (t[1] - t[0])
+    assert abs(np.interp(5 * tau, t, w)) < 1e-4 * 1e3
+
+
+def test_hilbert_matches_scipy(O):
+    from scipy.signal import hilbert
+    x = np.random.default_rng(0).standard_normal((33, 4))
+    for N in (33, 32):
+        np.testing.assert_allclose(O.hilbert_analytic(x[:N], axis=0), hilbert(x[:N], axis=0),
+                                   rtol=1e-12, atol=1e-12)
+
+
+def test_misfits(O):
+    rng = np.random.default_rng(9)
+    t = np.linspace(0, 1, 128)[:, None]
+    d = np.sin(40 * t + rng.uniform(0, 1, (1, 3))) * np.exp(-((t - 0.5) / 0.2) ** 2)
+    do = 0.9 * np.roll(d, 2, axis=0)
+    e, _, _ = O.w_bozdag(do, d, "L2")
+    np.testing.assert_array_equal(e, d - do)
+    assert np.isclose(O.w_E(do, d), 0.5 * ((d - do) ** 2).sum())
+    # env_L2 adjoint source = gradient of 0.5 ||A(d) - A(d_o)||^2 (Bozdag et al. 2011)
+    def Jenv(dd):
+        a = np.abs(O.hilbert_analytic(dd, axis=0))
+        b = np.abs(O.hilbert_analytic(do, axis=0))
+        return 0.5 * ((a - b) ** 2).sum()
+    e, _, _ = O.w_bozdag(do, d, "env_L2")
+    k = (60, 1)
+    h = 1e-6
+    dp, dm = d.copy(), d.copy()
+    dp[k] += h
+    dm[k] -= h
+    fd = (Jenv(dp) - Jenv(dm)) / (2 * h)
+    assert abs(fd - e[k]) <= 1e-5 * max(abs(fd), 1e-6)
+
+
+def test_golden_fixtures(O):
+    path = os.path.join(HERE, "golden", "tiny_forward_grad.npz")
+    z = np.load(path, allow_pickle=False)
+    g = O.Grid(int(z["m"]), int(z["n"]), int(z["P"]), int(z["air"]), float(z["dx"]), float(z["dt"]),
+               int(z["nt"]))
+    ep, sp = O.exp2pml(g, z["eps"], z["sig"])
+    d = O.forward(g, ep, sp, tuple(z["src"]), z["w"], z["Mw"])[0]
+    np.testing.assert_allclose(d, z["d"], rtol=1e-12, atol=1e-12 * np.abs(z["d"]).max())
+    ge, gs, gw = O.grad_exact(g, ep, sp, tuple(z["src"]), z["w"], z["Mw"], z["e"], K=5)
+    np.testing.assert_allclose(ge, z["g_eps_pad"], rtol=1e-10, atol=1e-12 * np.abs(z["g_eps_pad"]).max())
+    np.testing.assert_allclose(gs, z["g_sig_pad"], rtol=1e-10, atol=1e-12 * np.abs(z["g_sig_pad"]).max())
