@@ -1,0 +1,242 @@
+#!/usr/bin/env python3
+"""FWI shot-gradients/s on one MI355X: batched vs plain, at <= 1e-3 gradient
+error; Gcell-updates/s; roofline of the dominant kernel; CPU baseline.
+
+One step = one full FWI gradient of the survey: forward of every shot (with
+checkpoints), L2 misfit against observed traces, exact adjoint with
+recomputation, FP64 gradient reduced over time and shots.  Inputs (model,
+observed traces, geometry) are resident in HBM before the timed region;
+observed traces come from a forward solve of the true model (synthetic data).
+
+Default workload: 2000 x 600 model, 40-cell PML, nt = 8000, 64 shots at 0.3 m
+spacing, 400 receivers per shot, FP32 wavefields, FP64 gradient accumulation.
+
+Prints one JSON line (see the keys below).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
+BYTES_PER_CELL_UPDATE = 32  # Ez, Hx, Hy read+write + 2 coefficient reads (FP32)
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=64)
+    ap.add_argument("--nt", type=int, default=8000)
+    ap.add_argument("--plain-shots", type=int, default=2)
+    ap.add_argument("--cpu-shots", type=int, default=16)
+    ap.add_argument("--cpu-nt", type=int, default=160)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--ckpt", type=int, default=0)
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="only run three 1-GiB device copies (calibrates PMC byte counters)")
+    args = ap.parse_args()
+    if args.probe_copy:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe_copy_bytes": a.numel() * 4}))
+        return
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only: the survey's shots are "
+                          "batched inside one device; use --gpus 1"}))
+        sys.exit(2)
+
+    import torch
+    from gerjoii_amd import configs
+    from gerjoii_amd import survey as S
+    from gerjoii_amd import _native as N
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    case = configs.c3(ns=args.shots, nt=args.nt)
+    g = case.grid
+    cells = g.cells()
+    ns = len(case.src)
+    log(f"workload C3: model {g.n}x{g.m}, padded Ez {g.ny + 1}x{g.nx + 1} = {cells} cells, "
+        f"nt={g.nt}, shots={ns}")
+
+    ctx = S.context(0)
+    sv = S.Survey(g, ctx)
+    sv.set_shots(case.src, case.wavelets, case.receivers)
+    if args.tile:
+        sv.set_option(N.GJW_OPT_TILE_L, args.tile)
+    if args.ckpt:
+        sv.set_option(N.GJW_OPT_CKPT_EVERY, args.ckpt)
+    # observed data: true model, kept in HBM
+    sv.set_model(case.eps_true, case.sig_true)
+    d_obs = torch.zeros(sv.traces_shape(), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    sv.forward(out=d_obs)
+    log("observed traces ready; tile L =", sv.tile())
+    sv.set_model(case.eps0, case.sig0)
+    g_eps = torch.zeros((g.n, g.m), dtype=torch.float64, device=dev)  # (m, n) column-major
+    g_sig = torch.zeros_like(g_eps)
+
+    for w in range(args.warmup):
+        sv.fwi_gradient(d_obs, g_eps, g_sig)
+        log(f"warmup {w + 1}/{args.warmup} done")
+    sv.set_option(N.GJW_OPT_KTIMING, 16)
+    sv.kernel_stats(0, reset=True)
+    sv.kernel_stats(1, reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = 0.0
+    for k in range(args.steps):
+        _, _, J = sv.fwi_gradient(d_obs, g_eps, g_sig)
+        log(f"step {k + 1}/{args.steps} J={J:.6e}")
+    torch.cuda.synchronize()
+    dt_s = time.perf_counter() - t0
+    sv.set_option(N.GJW_OPT_KTIMING, 0)
+    fwd_ms, fwd_n = sv.kernel_stats(0)
+    adj_ms, adj_n = sv.kernel_stats(1)
+    ms_step = dt_s * 1e3 / args.steps
+    shot_grads = ns * args.steps / dt_s
+    cell_updates = 3.0 * cells * g.nt * ns * args.steps  # forward, recompute, adjoint
+    gcells = cell_updates / dt_s / 1e9
+    ge_b = g_eps.cpu().numpy().T.copy()
+    tile_L = sv.tile()
+    sv.close()
+    del sv
+    torch.cuda.empty_cache()
+
+    # per-kernel roofline: algorithmic bytes per launch / live average duration
+    bytes_launch = BYTES_PER_CELL_UPDATE * cells * ns
+    kern = {
+        "k_fwd_step": {"avg_ms": fwd_ms, "samples": fwd_n, "launches_per_step": 2 * g.nt},
+        "k_adj_step": {"avg_ms": adj_ms, "samples": adj_n, "launches_per_step": g.nt},
+    }
+    for k, v in kern.items():
+        v["achieved_GBs"] = bytes_launch / (v["avg_ms"] * 1e-3) / 1e9 if v["avg_ms"] > 0 else None
+        v["share_of_step"] = v["avg_ms"] * v["launches_per_step"] / ms_step if ms_step > 0 else None
+    dom = max(kern, key=lambda k: kern[k]["avg_ms"] * kern[k]["launches_per_step"])
+    traffic = _traffic_from_profiles(dom, cells, ns)
+    roof = {"kernel": dom, "bound": "hbm", "achieved": kern[dom]["achieved_GBs"],
+            "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": (kern[dom]["achieved_GBs"] or 0) / PEAK_HBM_GBS,
+            "traffic": traffic, "bytes_per_launch_algorithmic": bytes_launch,
+            "kernels": kern}
+
+    # plain single-shot path on a subset of the same shots
+    plain = None
+    if not args.no_plain and args.plain_shots > 0:
+        q = args.plain_shots
+        sp = S.Survey(g, ctx)
+        sp.set_shots(case.src[:q], case.wavelets[:q], case.receivers[:q])
+        sp.set_path(True)
+        sp.set_model(case.eps0, case.sig0)
+        rows = int(sp.rec_ptr[-1])
+        dsub = d_obs[:rows].contiguous()
+        ge_p = torch.zeros_like(g_eps)
+        gs_p = torch.zeros_like(g_sig)
+        sp.fwi_gradient(dsub, ge_p, gs_p)  # warm
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sp.fwi_gradient(dsub, ge_p, gs_p)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t1
+        plain = {"shot_gradients_per_s": q / tp, "shots_timed": q, "s": tp,
+                 "gcell_updates_per_s": 2.0 * cells * g.nt * q / tp / 1e9,
+                 "path": "one shot per launch, whole domain wavefield stored (no checkpoints)"}
+        sp.close()
+        del sp
+        torch.cuda.empty_cache()
+        log(f"plain: {plain['shot_gradients_per_s']:.3f} shot-gradients/s")
+
+    cpu = None if args.no_cpu else _cpu_baseline(case, args)
+
+    out = {
+        "metric": "FWI shot-gradients/s, batched vs plain, at <=1e-3 grad err; Gcell-updates/s",
+        "value": shot_grads,
+        "unit": "shot-gradients/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded layered model with topography; observed traces from the true model)",
+        "config": {"workload": "batched FWI gradient, 2000x600 model, 40-cell PML, nt=%d, %d shots, "
+                               "400 receivers/shot, checkpoint + recompute, FP64 accumulation" % (g.nt, ns),
+                   "grid_cells": cells, "nt": g.nt, "shots": ns, "tile_L": tile_L,
+                   "parallelism": "single GPU, all shots in each launch"},
+        "gcell_updates_per_s": gcells,
+        "misfit_J": J,
+        "plain": plain,
+        "batched_vs_plain": (shot_grads / plain["shot_gradients_per_s"]) if plain else None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def _traffic_from_profiles(kernel, cells, ns):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if one
+    exists for this kernel and workload (profiles/<kernel>_C3_pmc.txt, line
+    'bytes_per_launch: <float>')."""
+    p = os.path.join(ROOT, "profiles", f"{kernel}_C3_pmc.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("bytes_per_launch:"):
+            try:
+                return float(line.split(":")[1])
+            except ValueError:
+                return None
+    return None
+
+
+def _cpu_baseline(case, args):
+    """The FP32 CPU oracle (forward + exact adjoint, one shot per OpenMP
+    thread) on a bounded sample of the same workload: the same grid and
+    shots, truncated to --cpu-nt steps, extrapolated per step to nt."""
+    from oracle import oracle as O
+    g = case.grid
+    q = min(args.cpu_shots, len(case.src))
+    nt_s = min(args.cpu_nt, g.nt)
+    og = O.Grid(g.m, g.n, g.pml, g.air, g.dx, g.dt, nt_s)
+    ep, sp = O.exp2pml(og, case.eps0, case.sig0)
+    rng = np.random.default_rng(0)
+    e = rng.standard_normal((q, len(case.receivers[0]), nt_s)).astype(np.float32)
+    K = max(1, int(np.ceil(np.sqrt(nt_s))))
+    t0 = time.perf_counter()
+    _, _, thr = O.survey_grad_f32(og, ep, sp, case.src[:q], case.wavelets[:q, :nt_s],
+                                  np.array(case.receivers[:q]), e, K)
+    dt_s = time.perf_counter() - t0
+    per_shot_full = dt_s / q * (g.nt / nt_s)
+    cells = g.cells()
+    log(f"cpu baseline: {q} shots x {nt_s} steps in {dt_s:.1f} s on {thr} threads")
+    return {"value": 1.0 / per_shot_full,
+            "unit": "shot-gradients/s", "cores": thr, "kind": "port",
+            "gcell_updates_per_s": 3.0 * cells * nt_s * q / dt_s / 1e9,
+            "sample": f"{q} shots of the workload, first {nt_s} of {g.nt} steps (forward, checkpointed "
+                      f"recompute, exact adjoint), float oracle, extrapolated linearly in nt"}
+
+
+if __name__ == "__main__":
+    main()

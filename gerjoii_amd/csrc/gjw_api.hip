@@ -39,7 +39,10 @@ __global__ void k_gather(const float*, long long, const int*, int, int, float*);
 __global__ void k_scatter(const float*, long long, const int*, int, int, float*);
 __global__ void k_domain_frame(DevGrid, int, const float*, float*, long long);
 __global__ void k_flush(int, int, int, float*, long long, const double*, const double*,
-                        double*, double*);
+                        double*, double*, double);
+__global__ void k_ref_overwrite(DevGrid, int, int, int, const int*, const int*, const float*, int,
+                                float*);
+__global__ void k_ref_corr(DevGrid, const float*, const float*, int, int, double, float*, float*);
 __global__ void k_resid_l2(const float*, const float*, float*, long long, double*);
 __global__ void k_reduce_d(const double*, int, double, double*);
 __global__ void k_dcube(float*, long long, int, double);
@@ -121,11 +124,13 @@ struct gjw_survey {
   std::vector<double> pe, ps;
   float bx0 = 0.f, by0 = 0.f;
   DBuf<float> coef;  // ce, ch, cxe, cxh, cye, cyh, hax, hbx, hay, hby (10*FS)
+  DBuf<float> coef_ref;  // ce, ch with sigma negated (w_grad.m:26-27)
+  DBuf<float> zero_sc;   // per-shot zero source coefficients
   DBuf<double> fe, fs;
   std::vector<int> pml_idx;
   DBuf<int> d_pml;
   // shots
-  int ns = 0, ntr = 0;
+  int ns = 0, ntr = 0, rrow_min = 1 << 30, rrow_max = -1;
   std::vector<int> si, sj, tr_shot, tr_row, tr_col;
   std::vector<long long> rec_ptr;
   std::vector<float> sc;
@@ -143,6 +148,13 @@ struct gjw_survey {
   int cur = 0;                       // which forward buffer holds the current state
   bool have_adj = false;
   double last_ms = 0, last_cells = 0;
+  // sampled per-kernel timing (GJW_OPT_KTIMING): event pairs around every
+  // ksample-th launch of k_fwd_step / k_adj_step, resolved after each call
+  int ksample = 0;
+  std::vector<hipEvent_t> kev_pool;
+  std::vector<std::pair<int, int>> kev_used;  // (kernel id, index of start event)
+  double kt_ms[2] = {0, 0};
+  long long kt_n[2] = {0, 0};
 };
 
 static inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
@@ -214,7 +226,8 @@ extern "C" int gjw_survey_destroy(gjw_survey* s) {
   if (!s) return GJW_OK;
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
-  s->coef.release(); s->fe.release(); s->fs.release(); s->d_pml.release();
+  for (auto e : s->kev_pool) (void)hipEventDestroy(e);
+  s->coef.release(); s->coef_ref.release(); s->zero_sc.release(); s->fe.release(); s->fs.release(); s->d_pml.release();
   s->d_si.release(); s->d_sj.release(); s->d_cptr.release(); s->d_crow.release();
   s->d_cid.release(); s->d_trs.release(); s->d_trr.release(); s->d_trc.release();
   s->d_sc.release(); s->d_wav.release(); s->fld.release(); s->adjf.release();
@@ -241,6 +254,10 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       if (v < 0 || v > 4096) return fail(GJW_ERR_ARG, "GJW_OPT_TILE_L out of range");
       s->tile_L = (int)v;
       break;
+    case GJW_OPT_KTIMING:
+      if (v < 0) return fail(GJW_ERR_ARG, "GJW_OPT_KTIMING must be >= 0");
+      s->ksample = (int)v;
+      return GJW_OK;
     case GJW_OPT_SNAPSHOT:
       if (v != GJW_SNAP_F32) return fail(GJW_ERR_ARG, "GJW_OPT_SNAPSHOT: only F32 is built");
       s->snap = (int)v;
@@ -289,7 +306,8 @@ extern "C" int gjw_set_model(gjw_survey* s, const double* eps_r, const double* s
   const double smax = -(ord + 1) * e0 * cl * std::log(s->gr.R0) / (2.0 * dx * P);  // w_cpml.m:47
   const long long FS = s->FS, LD = s->LD;
   std::vector<float> h((size_t)10 * FS, 0.f);
-  float *ce = h.data(), *ch = ce + FS, *cxe = ch + FS, *cxh = cxe + FS, *cye = cxh + FS,
+  // (Ceze, Cezhy) interleaved per cell in the first 2*FS floats
+  float *cc = h.data(), *cxe = cc + 2 * FS, *cxh = cxe + FS, *cye = cxh + FS,
         *cyh = cye + FS, *hax = cyh + FS, *hbx = hax + FS, *hay = hbx + FS, *hby = hay + FS;
   auto EPS = [&](int i, int j) { return s->pe[(size_t)i + (size_t)j * NX]; };
   auto SIG = [&](int i, int j) { return s->ps[(size_t)i + (size_t)j * NX]; };
@@ -303,8 +321,8 @@ extern "C" int gjw_set_model(gjw_survey* s, const double* eps_r, const double* s
       const int kd = ekind(NX, NY, P, i, j);
       if (kd == KIND_INNER) {
         const double D = 2.0 * EPS(i, j) * e0 + dt * SIG(i, j);
-        ce[k] = (float)((2.0 * EPS(i, j) * e0 - dt * SIG(i, j)) / D);
-        ch[k] = (float)((2.0 * dt / dx) / D);
+        cc[2 * k] = (float)((2.0 * EPS(i, j) * e0 - dt * SIG(i, j)) / D);
+        cc[2 * k + 1] = (float)((2.0 * dt / dx) / D);
       } else if (kd == KIND_PML) {
         s->pml_idx.push_back((int)k);
         const double ee = EPS(i, j);
@@ -429,6 +447,8 @@ extern "C" int gjw_set_shots(gjw_survey* s, int32_t ns, const int32_t* src, cons
     s->sj[q] = j;
   }
   s->rec_ptr.assign(hp.begin(), hp.end());
+  s->rrow_min = 1 << 30;
+  s->rrow_max = -1;
   s->tr_shot.resize(ntr);
   s->tr_row.resize(ntr);
   s->tr_col.resize(ntr);
@@ -442,6 +462,8 @@ extern "C" int gjw_set_shots(gjw_survey* s, int32_t ns, const int32_t* src, cons
       s->tr_shot[t] = q;
       s->tr_row[t] = (int)(lin % (NX + 1));
       s->tr_col[t] = (int)(lin / (NX + 1));
+      s->rrow_min = std::min(s->rrow_min, s->tr_row[t]);
+      s->rrow_max = std::max(s->rrow_max, s->tr_row[t]);
     }
   // column CSR per shot
   std::vector<int> cptr((size_t)ns * (NY + 2), 0), crow(ntr), cid(ntr);
@@ -516,7 +538,7 @@ static DevGrid dev_grid(const gjw_survey* s) {
   g.bx0 = s->bx0; g.by0 = s->by0;
   const float* c = s->coef.p;
   const long long FS = s->FS;
-  g.ce = c; g.ch = c + FS; g.cxe = c + 2 * FS; g.cxh = c + 3 * FS; g.cye = c + 4 * FS;
+  g.cc = reinterpret_cast<const float2*>(c); g.cxe = c + 2 * FS; g.cxh = c + 3 * FS; g.cye = c + 4 * FS;
   g.cyh = c + 5 * FS; g.hax = c + 6 * FS; g.hbx = c + 7 * FS; g.hay = c + 8 * FS;
   g.hby = c + 9 * FS;
   g.di0 = s->di0; g.dj0 = s->dj0; g.m = s->gr.m; g.n = s->gr.n;
@@ -534,6 +556,8 @@ static DevShots dev_shots(const gjw_survey* s, int q0) {
   d.cptr = s->d_cptr.p + (long long)q0 * (s->NY + 2);
   d.crow = s->d_crow.p;
   d.cid = s->d_cid.p;
+  d.rmin = s->rrow_min;
+  d.rmax = s->rrow_max;
   return d;
 }
 
@@ -590,25 +614,63 @@ static AdjBufs adj_bufs(gjw_survey* s, int nsh) {
   return b;
 }
 
+static int kev_begin(gjw_survey* s, int which, int it, int* slot) {
+  *slot = -1;
+  if (s->ksample <= 0 || it % s->ksample != 0) return GJW_OK;
+  const size_t need = 2 * (s->kev_used.size() + 1);
+  while (s->kev_pool.size() < need) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    s->kev_pool.push_back(e);
+  }
+  *slot = (int)(2 * s->kev_used.size());
+  s->kev_used.push_back({which, *slot});
+  HIPCHK(hipEventRecord(s->kev_pool[*slot], s->ctx->stream));
+  return GJW_OK;
+}
+
+static int kev_end(gjw_survey* s, int slot) {
+  if (slot >= 0) HIPCHK(hipEventRecord(s->kev_pool[slot + 1], s->ctx->stream));
+  return GJW_OK;
+}
+
+// fold the sampled event pairs of the last call into the per-kernel averages
+static int kev_resolve(gjw_survey* s) {
+  if (s->kev_used.empty()) return GJW_OK;
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  for (auto& u : s->kev_used) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, s->kev_pool[u.second], s->kev_pool[u.second + 1]));
+    s->kt_ms[u.first] += ms;
+    s->kt_n[u.first] += 1;
+  }
+  s->kev_used.clear();
+  return GJW_OK;
+}
+
 static int fwd_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, int L,
                       FwdBufs& b, int cur, int it, float* tr, float* frame, long long fstride) {
+  int slot;
+  TRY(kev_begin(s, 0, it, &slot));
   hipLaunchKernelGGL(k_fwd_step, step_grid(s, L, nsh), dim3(64, 4), 0, s->ctx->stream, g, sh, L,
                      b.E[cur], b.Hx[cur], b.Hy[cur], b.E[cur ^ 1], b.Hx[cur ^ 1], b.Hy[cur ^ 1],
                      b.X, b.Y, it, tr, frame, fstride);
   HIPCHK(hipGetLastError());
-  return GJW_OK;
+  return kev_end(s, slot);
 }
 
 static int adj_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, int L,
                       AdjBufs& b, int cur, int it, const float* adj, const float* F1,
                       const float* F0, long long fstride, float* acc, long long astride,
                       float* gw) {
+  int slot;
+  TRY(kev_begin(s, 1, it, &slot));
   hipLaunchKernelGGL(k_adj_step, step_grid(s, L, nsh), dim3(64, 4), 0, s->ctx->stream, g, sh, L,
                      b.M[cur], b.Nx[cur], b.Ny[cur], b.X[cur], b.Y[cur], b.M[cur ^ 1],
                      b.Nx[cur ^ 1], b.Ny[cur ^ 1], b.X[cur ^ 1], b.Y[cur ^ 1], it, adj, F1, F0,
                      fstride, acc, astride, gw);
   HIPCHK(hipGetLastError());
-  return GJW_OK;
+  return kev_end(s, slot);
 }
 
 static int check_ready(gjw_survey* s, const char* who) {
@@ -635,6 +697,8 @@ static int autotune_L(gjw_survey* s, int nsh) {
   float best = 1e30f;
   int bestL = 16;
   const int reps = 6;
+  const int ks = s->ksample;
+  s->ksample = 0;
   for (int L : cand) {
     int cur = 0;
     TRY(fwd_launch(s, g, sh, nsh, L, b, cur, 0, nullptr, nullptr, 0));  // warm
@@ -653,6 +717,7 @@ static int autotune_L(gjw_survey* s, int nsh) {
     }
   }
   s->tuned_L = bestL;
+  s->ksample = ks;
   return GJW_OK;
 }
 
@@ -744,37 +809,43 @@ static int forward_batched(gjw_survey* s, int keep) {
   return GJW_OK;
 }
 
-// plain: one shot at a time, whole domain wavefield kept (w_solve.m:76,102)
+// plain: one shot at a time, whole domain wavefield kept (w_solve.m:76,102).
+// frames (nt+1 domain frames of the shot) may be null when nothing is kept.
+static int forward_plain_shot(gjw_survey* s, int q, float* fr) {
+  const int nt = s->gr.nt;
+  const long long per = (long long)s->gr.m * s->gr.n;
+  const int L = s->tile_L > 0 ? s->tile_L : 16;
+  DevGrid g = dev_grid(s);
+  FwdBufs b = fwd_bufs(s, 1);
+  DevShots sh = dev_shots(s, q);
+  TRY(zero_fwd_state(s, 1));
+  if (fr) TRY(launch_fill(fr, per, 0.f, s->ctx->stream));  // frame 0 = Ez(S_0) = 0
+  int cur = 0;
+  for (int it = 0; it < nt; ++it) {
+    TRY(fwd_launch(s, g, sh, 1, L, b, cur, it, s->tr.p, fr ? fr + (long long)(it + 1) * per : nullptr,
+                   0));
+    cur ^= 1;
+  }
+  return GJW_OK;
+}
+
 static int forward_plain(gjw_survey* s, int keep) {
   const int ns = s->ns, nt = s->gr.nt;
-  hipStream_t st = s->ctx->stream;
   const long long per = (long long)s->gr.m * s->gr.n;
   TRY(ensure_fwd_state(s, 1));
   TRY(s->tr.ensure(std::max<long long>((long long)s->ntr * nt, 1)));
   if (keep) TRY(s->frames.ensure((size_t)ns * (nt + 1) * per));
-  const int L = s->tile_L > 0 ? s->tile_L : 16;
-  DevGrid g = dev_grid(s);
-  FwdBufs b = fwd_bufs(s, 1);
-  HIPCHK(hipEventRecord(s->ctx->ev0, st));
-  for (int q = 0; q < ns; ++q) {
-    DevShots sh = dev_shots(s, q);
-    TRY(zero_fwd_state(s, 1));
-    float* fr = keep ? s->frames.p + (long long)q * (nt + 1) * per : nullptr;
-    if (fr) TRY(launch_fill(fr, per, 0.f, st));  // frame 0 = Ez(S_0) = 0
-    int cur = 0;
-    for (int it = 0; it < nt; ++it) {
-      TRY(fwd_launch(s, g, sh, 1, L, b, cur, it, s->tr.p, fr ? fr + (long long)(it + 1) * per : nullptr,
-                     0));
-      cur ^= 1;
-    }
-  }
-  HIPCHK(hipEventRecord(s->ctx->ev1, st));
+  HIPCHK(hipEventRecord(s->ctx->ev0, s->ctx->stream));
+  for (int q = 0; q < ns; ++q)
+    TRY(forward_plain_shot(s, q, keep ? s->frames.p + (long long)q * (nt + 1) * per : nullptr));
+  HIPCHK(hipEventRecord(s->ctx->ev1, s->ctx->stream));
   s->kept = keep ? 2 : 0;
   s->last_cells = (double)(s->NX + 1) * (s->NY + 1) * nt * ns;
   return GJW_OK;
 }
 
 static int finish_timing(gjw_survey* s) {
+  TRY(kev_resolve(s));
   HIPCHK(hipEventSynchronize(s->ctx->ev1));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
@@ -884,35 +955,139 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
       acur ^= 1;
     }
     hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
-                       s->gr.n, ns, s->acc.p, per, s->fe.p, s->fs.p, s->ge.p, s->gsg.p);
+                       s->gr.n, ns, s->acc.p, per, s->fe.p, s->fs.p, s->ge.p, s->gsg.p, 1.0);
     HIPCHK(hipGetLastError());
   }
   return GJW_OK;
 }
 
-static int grad_plain_exact(gjw_survey* s, const float* adj, float* gw) {
-  const int ns = s->ns, nt = s->gr.nt;
+static int grad_plain_shot(gjw_survey* s, int q, const float* adj, const float* u, float* gw) {
+  const int nt = s->gr.nt;
   const long long per = (long long)s->gr.m * s->gr.n;
   hipStream_t st = s->ctx->stream;
   const int L = s->tile_L > 0 ? s->tile_L : 16;
   const int K = ck_interval(s);
+  DevGrid g = dev_grid(s);
+  DevShots sh = dev_shots(s, q);
+  TRY(adj_init(s, 1, q, adj));
+  AdjBufs ab = adj_bufs(s, 1);
+  int acur = 0;
+  for (int it = nt - 1; it >= 0; --it) {
+    TRY(adj_launch(s, g, sh, 1, L, ab, acur, it, adj, u + (long long)(it + 1) * per,
+                   u + (long long)it * per, 0, s->acc.p, 0, gw ? gw + (long long)q * nt : nullptr));
+    acur ^= 1;
+    if (it % K == 0) {
+      hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
+                         s->gr.n, 1, s->acc.p, per, s->fe.p, s->fs.p, s->ge.p, s->gsg.p, 1.0);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  return GJW_OK;
+}
+
+static int grad_plain_exact(gjw_survey* s, const float* adj, float* gw) {
+  const long long per = (long long)s->gr.m * s->gr.n;
+  TRY(s->acc.ensure((size_t)2 * per));
+  TRY(launch_fill(s->acc.p, 2LL * per, 0.f, s->ctx->stream));
+  for (int q = 0; q < s->ns; ++q)
+    TRY(grad_plain_shot(s, q, adj, s->frames.p + (long long)q * (s->gr.nt + 1) * per, gw));
+  return GJW_OK;
+}
+
+// plain FWI gradient: per shot, forward storing the whole domain wavefield,
+// L2 residual of that shot, adjoint; J summed over shots at the end.
+static int fwi_plain(gjw_survey* s, const float* d_obs) {
+  const int ns = s->ns, nt = s->gr.nt;
+  const long long per = (long long)s->gr.m * s->gr.n, n = (long long)s->ntr * nt;
+  hipStream_t st = s->ctx->stream;
+  const int B = 64;  // residual blocks per shot
+  TRY(ensure_fwd_state(s, 1));
+  TRY(s->tr.ensure(std::max<long long>(n, 1)));
+  TRY(s->adj.ensure(std::max<long long>(n, 1)));
+  TRY(s->obs.ensure(std::max<long long>(n, 1)));
+  TRY(s->part.ensure((size_t)ns * B));
+  TRY(s->Jd.ensure(1));
+  TRY(s->frames.ensure((size_t)(nt + 1) * per));
+  TRY(s->acc.ensure((size_t)2 * per));
+  TRY(s->ge.ensure(per));
+  TRY(s->gsg.ensure(per));
+  TRY(host_or_dev_copy(s->obs.p, d_obs, n * 4, st));
+  TRY(launch_fill(s->acc.p, 2LL * per, 0.f, st));
+  TRY(launch_fill_d(s->ge.p, per, 0.0, st));
+  TRY(launch_fill_d(s->gsg.p, per, 0.0, st));
+  TRY(launch_fill_d(s->part.p, (long long)ns * B, 0.0, st));
+  for (int q = 0; q < ns; ++q) {
+    TRY(forward_plain_shot(s, q, s->frames.p));
+    const long long t0 = s->rec_ptr[q] * nt, cnt = (s->rec_ptr[q + 1] - s->rec_ptr[q]) * nt;
+    if (cnt > 0) {
+      hipLaunchKernelGGL(k_resid_l2, dim3(B), dim3(256), 0, st, s->tr.p + t0, s->obs.p + t0,
+                         s->adj.p + t0, cnt, s->part.p + (long long)q * B);
+      HIPCHK(hipGetLastError());
+    }
+    TRY(grad_plain_shot(s, q, s->adj.p, s->frames.p, nullptr));
+  }
+  hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, ns * B, 0.5, s->Jd.p);
+  HIPCHK(hipGetLastError());
+  s->have_adj = true;
+  s->kept = 0;
+  return GJW_OK;
+}
+
+// Reference gradient (w_grad.m:20-115) on the plain path's stored wavefield:
+// per shot, a forward-type sweep with negated conductivity in the inner
+// coefficients, the residual written into Ez at the receivers before each
+// step, no source, and the correlation g -= u(nt-it) .* a, g *= dt.
+static int grad_ref(gjw_survey* s, const float* adj, float* amax) {
+  const int ns = s->ns, nt = s->gr.nt, NX = s->NX, NY = s->NY, P = s->P;
+  const long long per = (long long)s->gr.m * s->gr.n, FS = s->FS;
+  hipStream_t st = s->ctx->stream;
+  const int L = s->tile_L > 0 ? s->tile_L : 16;
+  const int K = ck_interval(s);
+  if (!s->coef_ref.p) {  // inner Ceze / Cezhy with -sigma
+    std::vector<float> h((size_t)2 * FS, 0.f);
+    const double e0 = s->gr.eps0, dt = s->gr.dt, dx = s->gr.dx;
+    for (int j = 0; j <= NY; ++j)
+      for (int i = 0; i <= NX; ++i) {
+        if (ekind(NX, NY, P, i, j) != KIND_INNER) continue;
+        const double ee = s->pe[(size_t)i + (size_t)j * NX], sg = -s->ps[(size_t)i + (size_t)j * NX];
+        const double D = 2.0 * ee * e0 + dt * sg;
+        h[2 * ((size_t)j * s->LD + i)] = (float)((2.0 * ee * e0 - dt * sg) / D);
+        h[2 * ((size_t)j * s->LD + i) + 1] = (float)((2.0 * dt / dx) / D);
+      }
+    TRY(s->coef_ref.ensure(h.size()));
+    HIPCHK(hipMemcpy(s->coef_ref.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  }
+  TRY(s->zero_sc.ensure(ns));
+  TRY(launch_fill(s->zero_sc.p, ns, 0.f, st));
+  DevGrid g = dev_grid(s);
+  g.cc = reinterpret_cast<const float2*>(s->coef_ref.p);
   TRY(s->acc.ensure((size_t)2 * per));
   TRY(launch_fill(s->acc.p, 2LL * per, 0.f, st));
-  DevGrid g = dev_grid(s);
+  TRY(ensure_fwd_state(s, 1));
+  FwdBufs b = fwd_bufs(s, 1);
   for (int q = 0; q < ns; ++q) {
     DevShots sh = dev_shots(s, q);
-    TRY(adj_init(s, 1, q, adj));
-    AdjBufs ab = adj_bufs(s, 1);
-    const float* u = s->frames.p + (long long)q * (nt + 1) * per;
-    int acur = 0;
-    for (int it = nt - 1; it >= 0; --it) {
-      TRY(adj_launch(s, g, sh, 1, L, ab, acur, it, adj, u + (long long)(it + 1) * per,
-                     u + (long long)it * per, 0, s->acc.p, 0,
-                     gw ? gw + (long long)q * nt : nullptr));
-      acur ^= 1;
-      if (it % K == 0) {
+    sh.sc = s->zero_sc.p + q;
+    TRY(zero_fwd_state(s, 1));
+    const float* U = s->frames.p + (long long)q * (nt + 1) * per;
+    const int t0 = (int)s->rec_ptr[q], t1 = (int)s->rec_ptr[q + 1];
+    int cur = 0;
+    for (int it = 0; it < nt; ++it) {
+      if (t1 > t0) {
+        hipLaunchKernelGGL(k_ref_overwrite, dim3(1), dim3(64), 0, st, g, nt, t0, t1, s->d_trr.p,
+                           s->d_trc.p, adj, nt - 1 - it, b.E[cur]);
+        HIPCHK(hipGetLastError());
+      }
+      TRY(fwd_launch(s, g, sh, 1, L, b, cur, it, nullptr, nullptr, 0));
+      cur ^= 1;
+      hipLaunchKernelGGL(k_ref_corr, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, g, b.E[cur],
+                         U, nt - it, nt, s->gr.dt, s->acc.p,
+                         (amax && it == nt - 1 && q == ns - 1) ? amax : nullptr);
+      HIPCHK(hipGetLastError());
+      if (it % K == K - 1 || it == nt - 1) {
         hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
-                           s->gr.n, 1, s->acc.p, per, s->fe.p, s->fs.p, s->ge.p, s->gsg.p);
+                           s->gr.n, 1, s->acc.p, per, nullptr, nullptr, s->ge.p, s->gsg.p,
+                           -s->gr.dt);
         HIPCHK(hipGetLastError());
       }
     }
@@ -921,10 +1096,14 @@ static int grad_plain_exact(gjw_survey* s, const float* adj, float* gw) {
 }
 
 static int grad_run(gjw_survey* s, const float* adj_in, int32_t mode, double* g_eps,
-                    double* g_sig, double* g_w) {
-  if (mode != GJW_GRAD_EXACT)
-    return fail(GJW_ERR_ARG, "gjw_gradient: mode %d is not built yet", mode);
+                    double* g_sig, double* g_w, float* a_max = nullptr) {
+  if (mode != GJW_GRAD_EXACT && mode != GJW_GRAD_REF)
+    return fail(GJW_ERR_ARG, "gjw_gradient: unknown mode %d", mode);
   if (!s->kept) return fail(GJW_ERR_STATE, "gjw_gradient: run gjw_forward(keep=1) first");
+  if (mode == GJW_GRAD_REF && s->kept != 2)
+    return fail(GJW_ERR_STATE,
+                "gjw_gradient: REF mode needs the stored wavefield of the plain path "
+                "(GJW_OPT_PATH = GJW_PATH_PLAIN, then gjw_forward(keep=1))");
   hipStream_t st = s->ctx->stream;
   const int ns = s->ns, nt = s->gr.nt;
   const long long per = (long long)s->gr.m * s->gr.n, n = (long long)s->ntr * nt;
@@ -949,7 +1128,11 @@ static int grad_run(gjw_survey* s, const float* adj_in, int32_t mode, double* g_
     gwf = gwtmp.p;
   }
   HIPCHK(hipEventRecord(s->ctx->ev0, st));
-  int rc = s->kept == 1 ? grad_batched(s, adj, gwf) : grad_plain_exact(s, adj, gwf);
+  DBuf<float> amtmp;
+  if (a_max) TRY(amtmp.ensure(per));
+  int rc = mode == GJW_GRAD_REF ? grad_ref(s, adj, a_max ? amtmp.p : nullptr)
+           : s->kept == 1       ? grad_batched(s, adj, gwf)
+                                : grad_plain_exact(s, adj, gwf);
   if (rc != GJW_OK) {
     gwtmp.release();
     return rc;
@@ -966,16 +1149,21 @@ static int grad_run(gjw_survey* s, const float* adj_in, int32_t mode, double* g_
     std::vector<double> hd(h.begin(), h.end());
     HIPCHK(hipMemcpy(g_w, hd.data(), hd.size() * 8, hipMemcpyDefault));
   }
+  if (a_max) HIPCHK(hipMemcpyAsync(a_max, amtmp.p, per * 4, hipMemcpyDefault, st));
   HIPCHK(hipStreamSynchronize(st));
   gwtmp.release();
+  amtmp.release();
   return GJW_OK;
 }
 
 extern "C" int gjw_gradient(gjw_survey* s, const float* adj, int32_t mode, double* g_eps,
                             double* g_sig, double* g_w, float* a_max) {
   TRY(check_ready(s, "gjw_gradient"));
-  if (a_max) return fail(GJW_ERR_ARG, "gjw_gradient: a_max is only produced in REF mode");
-  return grad_run(s, adj, mode, g_eps, g_sig, g_w);
+  if (a_max && mode != GJW_GRAD_REF)
+    return fail(GJW_ERR_ARG, "gjw_gradient: a_max is only produced in REF mode");
+  if (g_w && mode != GJW_GRAD_EXACT)
+    return fail(GJW_ERR_ARG, "gjw_gradient: g_w is only produced in EXACT mode");
+  return grad_run(s, adj, mode, g_eps, g_sig, g_w, a_max);
 }
 
 extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps, double* g_sig,
@@ -983,18 +1171,31 @@ extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps
   TRY(check_ready(s, "gjw_fwi_gradient"));
   if (!d_obs) return fail(GJW_ERR_ARG, "gjw_fwi_gradient: d_obs is NULL");
   hipStream_t st = s->ctx->stream;
-  HIPCHK(hipEventRecord(s->ctx->ev0, st));
+  const long long per = (long long)s->gr.m * s->gr.n;
   hipEvent_t e0;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventRecord(e0, st));
-  int rc = s->path == GJW_PATH_PLAIN ? forward_plain(s, 1) : forward_batched(s, 1);
-  if (rc == GJW_OK) rc = misfit_dev(s, d_obs, GJW_MISFIT_L2);
-  if (rc == GJW_OK) rc = grad_run(s, nullptr, GJW_GRAD_EXACT, g_eps, g_sig, nullptr);
+  int rc;
+  if (s->path == GJW_PATH_PLAIN) {
+    rc = fwi_plain(s, d_obs);
+    if (rc == GJW_OK) {
+      if (hipEventRecord(s->ctx->ev1, st) != hipSuccess) rc = fail(GJW_ERR_HIP, "event record");
+      if (rc == GJW_OK) rc = kev_resolve(s);
+      if (rc == GJW_OK && g_eps)
+        rc = host_or_dev_copy(g_eps, s->ge.p, per * 8, st);
+      if (rc == GJW_OK && g_sig)
+        rc = host_or_dev_copy(g_sig, s->gsg.p, per * 8, st);
+    }
+  } else {
+    rc = forward_batched(s, 1);
+    if (rc == GJW_OK) rc = misfit_dev(s, d_obs, GJW_MISFIT_L2);
+    if (rc == GJW_OK) rc = grad_run(s, nullptr, GJW_GRAD_EXACT, g_eps, g_sig, nullptr);
+  }
   if (rc != GJW_OK) {
     (void)hipEventDestroy(e0);
     return rc;
   }
-  HIPCHK(hipEventSynchronize(s->ctx->ev1));
+  HIPCHK(hipStreamSynchronize(st));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0, s->ctx->ev1));
   HIPCHK(hipEventDestroy(e0));
@@ -1002,6 +1203,25 @@ extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps
   s->last_cells = (double)(s->NX + 1) * (s->NY + 1) * s->gr.nt * s->ns *
                   (s->path == GJW_PATH_PLAIN ? 2 : 3);
   if (J) HIPCHK(hipMemcpy(J, s->Jd.p, 8, hipMemcpyDefault));
+  return GJW_OK;
+}
+
+extern "C" int gjw_kernel_stats(gjw_survey* s, int32_t which, double* avg_ms, int64_t* count,
+                                int32_t reset) {
+  if (!s || which < 0 || which > 1) return fail(GJW_ERR_ARG, "gjw_kernel_stats: bad argument");
+  TRY(kev_resolve(s));
+  if (avg_ms) *avg_ms = s->kt_n[which] ? s->kt_ms[which] / s->kt_n[which] : 0.0;
+  if (count) *count = s->kt_n[which];
+  if (reset) {
+    s->kt_ms[which] = 0;
+    s->kt_n[which] = 0;
+  }
+  return GJW_OK;
+}
+
+extern "C" int gjw_tile(gjw_survey* s, int32_t* L) {
+  if (!s || !L) return fail(GJW_ERR_ARG, "gjw_tile: NULL argument");
+  *L = s->tuned_L;
   return GJW_OK;
 }
 

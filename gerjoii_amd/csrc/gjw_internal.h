@@ -18,8 +18,7 @@ struct DevGrid {
   int NX, NY, P, LD;
   long long FS;            // floats per field per shot = LD*(NY+1)
   float bx0, by0;          // inner Chxez, Chyez (w_coeff.m:34-40; Chxh=Chyh=1)
-  const float* ce;         // inner Ceze   (LD layout)
-  const float* ch;         // inner Cezhy  (Cezhx = -Cezhy since dy = dx)
+  const float2* cc;        // inner (Ceze, Cezhy) per Ez cell (Cezhx = -Cezhy, dy = dx)
   const float* cxe;        // split-field Ezx/Ezy coefficients at PML Ez cells
   const float* cxh;
   const float* cye;
@@ -42,6 +41,7 @@ struct DevShots {
   const int* cptr;   // receivers by column: [shot][NY+2] offsets into crow/cid
   const int* crow;   // receiver row
   const int* cid;    // global trace index; trace t occupies tr[t*nt .. t*nt+nt)
+  int rmin, rmax;    // receiver rows over all shots (waves outside skip the lookup)
 };
 
 enum { KIND_RING = 0, KIND_INNER = 1, KIND_PML = 2 };

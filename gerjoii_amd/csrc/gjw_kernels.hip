@@ -70,6 +70,50 @@ __device__ __forceinline__ float hx_new(const DevGrid& g, int i, int j, float hx
 // ==========================================================================
 // forward: fused step
 // ==========================================================================
+// Columns per software-pipelined chunk: the loads of U columns are issued
+// together before any of them is used, so each wavefront keeps ~5U 256-byte
+// requests in flight (one column alone leaves the kernel latency-bound).
+constexpr int U = 4;
+
+// One column of the general (strip-aware) update; used for every column the
+// unrolled interior chunk does not cover.
+__device__ __forceinline__ void fwd_column(const DevGrid& g, int lane, int i, int j, bool ok,
+                                           const float* __restrict__ E0,
+                                           const float* __restrict__ Hx0,
+                                           const float* __restrict__ Hy0, float* __restrict__ X,
+                                           float* __restrict__ Y, float& e_c, float& hx_prev,
+                                           float& en, float& hxn, float& hyn) {
+  const int NX = g.NX, NY = g.NY, P = g.P;
+  const long long LD = g.LD;
+  const long long k = (long long)j * LD + i;
+  const float e_n = (ok && j < NY) ? E0[k + LD] : 0.f;
+  const float hx = (ok && j < NY) ? Hx0[k] : 0.f;
+  const float hy = (ok && i < NX) ? Hy0[k] : 0.f;
+  float e_dn = shfl_dn1(e_c);
+  if (lane == 63) e_dn = (i + 1 <= NX) ? E0[k + 1] : 0.f;
+  hyn = (ok && i < NX) ? hy_new(g, i, j, hy, e_c, e_dn) : 0.f;
+  hxn = (ok && j < NY) ? hx_new(g, i, j, hx, e_c, e_n) : 0.f;
+  float hy_up = shfl_up1(hyn);
+  if (lane == 0) hy_up = (ok && i >= 1) ? hy_new(g, i - 1, j, Hy0[k - 1], E0[k - 1], e_c) : 0.f;
+  en = 0.f;
+  if (ok) {
+    const int kd = ekind(NX, NY, P, i, j);
+    const float dhy = hyn - hy_up, dhx = hxn - hx_prev;
+    if (kd == KIND_INNER) {
+      const float2 c = g.cc[k];
+      en = c.x * e_c + c.y * (dhy - dhx);
+    } else if (kd == KIND_PML) {
+      // split field Ez = Ezx + Ezy (w_solve.m:232-267)
+      const float x = g.cxe[k] * X[k] + g.cxh[k] * dhy;
+      const float y = g.cye[k] * Y[k] + g.cyh[k] * dhx;
+      X[k] = x;
+      Y[k] = y;
+      en = x + y;
+    }
+  }
+  e_c = e_n;
+}
+
 __global__ void __launch_bounds__(256)
 k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
            const float* __restrict__ Hx0, const float* __restrict__ Hy0, float* __restrict__ E1,
@@ -93,6 +137,8 @@ k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
   const int* cp = s.cptr + (long long)shot * (NY + 2);
   float* fr = frame ? frame + (long long)shot * fstride : nullptr;
   const bool fastrows = (i0 >= P + 1) && (i0 + 63 <= NX - P - 1);
+  const float bx0 = g.bx0, by0 = g.by0;
+  const bool has_rec = tr && i0 <= s.rmax && i0 + 63 >= s.rmin;
 
   float e_c = ok ? E0[(long long)jb * LD + i] : 0.f;
   float hx_prev = 0.f;  // Hx'(i, j-1)
@@ -100,50 +146,15 @@ k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
     const long long km = (long long)(jb - 1) * LD + i;
     hx_prev = hx_new(g, i, jb - 1, Hx0[km], E0[km], e_c);
   }
-  for (int j = jb; j < je; ++j) {
-    const long long k = (long long)j * LD + i;
-    const float e_n = (ok && j < NY) ? E0[k + LD] : 0.f;
-    const float hx = (ok && j < NY) ? Hx0[k] : 0.f;
-    const float hy = (ok && i < NX) ? Hy0[k] : 0.f;
-    float e_dn = shfl_dn1(e_c);
-    if (lane == 63) e_dn = (i + 1 <= NX) ? E0[k + 1] : 0.f;
-    float hyn, hxn, en;
-    const bool fast = fastrows && j >= P + 1 && j <= NY - P - 1;
-    if (fast) {
-      // interior: mu_r = 1, sigma_m = 0 => Chxh = Chyh = 1 (w_coeff.m:34-40)
-      hyn = hy + g.by0 * (e_dn - e_c);
-      hxn = hx + g.bx0 * (e_n - e_c);
-      float hy_up = shfl_up1(hyn);
-      if (lane == 0) hy_up = Hy0[k - 1] + g.by0 * (e_c - E0[k - 1]);
-      en = g.ce[k] * e_c + g.ch[k] * ((hyn - hy_up) - (hxn - hx_prev));
-    } else {
-      hyn = (ok && i < NX) ? hy_new(g, i, j, hy, e_c, e_dn) : 0.f;
-      hxn = (ok && j < NY) ? hx_new(g, i, j, hx, e_c, e_n) : 0.f;
-      float hy_up = shfl_up1(hyn);
-      if (lane == 0) hy_up = (ok && i >= 1) ? hy_new(g, i - 1, j, Hy0[k - 1], E0[k - 1], e_c) : 0.f;
-      en = 0.f;
-      if (ok) {
-        const int kd = ekind(NX, NY, P, i, j);
-        const float dhy = hyn - hy_up, dhx = hxn - hx_prev;
-        if (kd == KIND_INNER) {
-          en = g.ce[k] * e_c + g.ch[k] * (dhy - dhx);
-        } else if (kd == KIND_PML) {
-          // split field Ez = Ezx + Ezy (w_solve.m:232-267)
-          const float x = g.cxe[k] * X[k] + g.cxh[k] * dhy;
-          const float y = g.cye[k] * Y[k] + g.cyh[k] * dhx;
-          X[k] = x;
-          Y[k] = y;
-          en = x + y;
-        }
-      }
-    }
-    // soft source (w_solve.m:92); sc is 0 when the strip overwrites the cell
-    if (j == sj && i == si) en += wsrc;
+  // per-column epilogue: source, stores, receivers, domain frame
+  auto emit = [&](int j, float en, float hxn, float hyn) {
+    if (j == sj && i == si) en += wsrc;  // soft source (w_solve.m:92)
     if (ok) {
+      const long long k = (long long)j * LD + i;
       E1[k] = en;
       if (j < NY) Hx1[k] = hxn;
       if (i < NX) Hy1[k] = hyn;
-      if (tr) {  // receivers of this column (w_solve.m:100)
+      if (has_rec) {  // receivers of this column (w_solve.m:100)
         const int qa = cp[j], qb = cp[j + 1];
         for (int q = qa; q < qb; ++q)
           if (s.crow[q] == i) tr[(long long)s.cid[q] * s.nt + it] = en;
@@ -153,8 +164,54 @@ k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
         if (a >= 0 && a < g.m && b >= 0 && b < g.n) fr[(long long)b * g.m + a] = en;
       }
     }
-    e_c = e_n;
-    hx_prev = hxn;
+  };
+  int j = jb;
+  while (j < je) {
+    if (fastrows && j >= P + 1 && j + U - 1 <= NY - P - 1 && j + U <= je) {
+      // interior chunk: mu_r = 1, sigma_m = 0 => Chxh = Chyh = 1 (w_coeff.m:34-40)
+      const long long k = (long long)j * LD + i;
+      float en_[U], hx_[U], hy_[U], h63[U], hu0[U], eu0[U];
+      float2 c_[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long long kk = k + u * LD;
+        en_[u] = E0[kk + LD];
+        hx_[u] = Hx0[kk];
+        hy_[u] = Hy0[kk];
+        c_[u] = g.cc[kk];
+      }
+      if (lane == 63) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) h63[u] = E0[k + u * LD + 1];
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          hu0[u] = Hy0[k + u * LD - 1];
+          eu0[u] = E0[k + u * LD - 1];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        float e_dn = shfl_dn1(e_c);
+        if (lane == 63) e_dn = h63[u];
+        const float hyn = hy_[u] + by0 * (e_dn - e_c);
+        const float hxn = hx_[u] + bx0 * (en_[u] - e_c);
+        float hy_up = shfl_up1(hyn);
+        if (lane == 0) hy_up = hu0[u] + by0 * (e_c - eu0[u]);
+        const float en = c_[u].x * e_c + c_[u].y * ((hyn - hy_up) - (hxn - hx_prev));
+        emit(j + u, en, hxn, hyn);
+        e_c = en_[u];
+        hx_prev = hxn;
+      }
+      j += U;
+    } else {
+      float en, hxn, hyn;
+      fwd_column(g, lane, i, j, ok, E0, Hx0, Hy0, X, Y, e_c, hx_prev, en, hxn, hyn);
+      emit(j, en, hxn, hyn);
+      hx_prev = hxn;
+      ++j;
+    }
   }
 }
 
@@ -170,7 +227,7 @@ __device__ __forceinline__ void w_of(const DevGrid& g, int i, int j, long long k
                                      const float* __restrict__ Y0, float& wy, float& wx) {
   const int kd = ekind(g.NX, g.NY, g.P, i, j);
   if (kd == KIND_INNER) {
-    const float c = g.ch[k];
+    const float c = g.cc[k].y;
     wy = c * me;
     wx = -c * me;
   } else if (kd == KIND_PML) {
@@ -214,6 +271,10 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
   const float* f0 = F0 ? F0 + (long long)shot * fstride : nullptr;
   float2* ac = acc ? reinterpret_cast<float2*>(acc) + (long long)shot * astride : nullptr;
   const bool fastrows = (i0 >= P + 2) && (i0 + 64 <= NX - P - 1);
+  const float bx0 = g.bx0, by0 = g.by0;
+  const int ai = i - g.di0;  // domain row of this lane
+  const bool arow = ai >= 0 && ai < g.m;
+  const bool has_rec = adj && it >= 1 && i0 <= s.rmax && i0 + 63 >= s.rmin;
 
   float me_c = 0.f, wy_c = 0.f, wx_c = 0.f;
   if (ok) {
@@ -229,27 +290,105 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
     hx_coef(g, i, jb - 1, a, b);
     px_prev = b * (a * Nx0[km] + wx_m - wx_c);
   }
-  for (int j = jb; j < je; ++j) {
+  // per-column epilogue: residual injection, correlation, wavelet gradient, stores
+  // pre: the correlation operands (v1, v0, t) were loaded by the caller when
+  // pre_ok; otherwise they are loaded here.
+  auto emit = [&](int j, float mn, float me, float nxv, float nyv, bool inner, bool pre_ok,
+                  float v1, float v0, float2 t) {
+    if (!ok) return;
     const long long k = (long long)j * LD + i;
-    const bool fast = fastrows && j >= P + 2 && j <= NY - P - 2;
-    float me_n = 0.f, wy_n = 0.f, wx_n = 0.f;
-    float nyv = 0.f, nxv = 0.f, py, px, py_up, mn;
-    if (fast) {
-      // every cell touched here is inner: W = (c M, -c M), A = 1, B = (by0, bx0)
-      me_n = M0[k + LD];
-      const float cn = g.ch[k + LD];
-      wy_n = cn * me_n;
-      wx_n = -cn * me_n;
-      float wy_dn = shfl_dn1(wy_c);
-      if (lane == 63) wy_dn = g.ch[k + 1] * M0[k + 1];
-      nyv = Ny0[k] + wy_c - wy_dn;
-      nxv = Nx0[k] + wx_c - wx_n;
-      py = g.by0 * nyv;
-      px = g.bx0 * nxv;
-      py_up = shfl_up1(py);
-      if (lane == 0) py_up = g.by0 * (Ny0[k - 1] + g.ch[k - 1] * M0[k - 1] - wy_c);
-      mn = g.ce[k] * me_c + py_up - py + px_prev - px;
+    if (has_rec) {  // residual of step it-1 enters dJ/dEz(S_it)
+      const int qa = cp[j], qb = cp[j + 1];
+      for (int q = qa; q < qb; ++q)
+        if (s.crow[q] == i) mn += adj[(long long)s.cid[q] * s.nt + (it - 1)];
+    }
+    if (ac && inner && arow) {
+      const int b = j - g.dj0;
+      if (b >= 0 && b < g.n) {
+        const long long fk = (long long)b * g.m + ai;
+        if (!pre_ok) {
+          v1 = f1[fk];
+          v0 = f0 ? f0[fk] : 0.f;
+          t = ac[fk];
+        }
+        t.x += me * (v1 - v0);  // -> eps gradient
+        t.y += me * (v1 + v0);  // -> sigma gradient
+        ac[fk] = t;
+      }
+    }
+    if (gw && i == si && j == sj) gw[(long long)shot * s.nt + it] = sc * me;
+    M1[k] = mn;
+    if (i < NX) Ny1[k] = nyv;
+    if (j < NY) Nx1[k] = nxv;
+  };
+  int j = jb;
+  while (j < je) {
+    if (fastrows && j >= P + 2 && j + U - 1 <= NY - P - 2 && j + U <= je) {
+      // interior chunk: W = (c M, -c M), A = 1, B = (by0, bx0) for every cell touched
+      const long long k = (long long)j * LD + i;
+      float mn_[U], nx_[U], ny_[U], m63[U], c63[U], nu0[U], mu0[U], cu0[U];
+      float2 cn_[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long long kk = k + u * LD;
+        mn_[u] = M0[kk + LD];
+        cn_[u] = g.cc[kk + LD];
+        nx_[u] = Nx0[kk];
+        ny_[u] = Ny0[kk];
+      }
+      if (lane == 63) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          m63[u] = M0[k + u * LD + 1];
+          c63[u] = g.cc[k + u * LD + 1].y;
+        }
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          nu0[u] = Ny0[k + u * LD - 1];
+          mu0[u] = M0[k + u * LD - 1];
+          cu0[u] = g.cc[k + u * LD - 1].y;
+        }
+      }
+      // correlation operands of the chunk's columns (all inside the domain)
+      float v1_[U], v0_[U];
+      float2 t_[U];
+      const bool pre = ac && arow;
+      if (pre) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const long long fk = (long long)(j + u - g.dj0) * g.m + ai;
+          v1_[u] = f1[fk];
+          v0_[u] = f0 ? f0[fk] : 0.f;
+          t_[u] = ac[fk];
+        }
+      }
+      float ce_c = g.cc[k].x;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float wy_n = cn_[u].y * mn_[u];
+        const float wx_n = -wy_n;
+        float wy_dn = shfl_dn1(wy_c);
+        if (lane == 63) wy_dn = c63[u] * m63[u];
+        const float nyv = ny_[u] + wy_c - wy_dn;
+        const float nxv = nx_[u] + wx_c - wx_n;
+        const float py = by0 * nyv, px = bx0 * nxv;
+        float py_up = shfl_up1(py);
+        if (lane == 0) py_up = by0 * (nu0[u] + cu0[u] * mu0[u] - wy_c);
+        const float mn = ce_c * me_c + py_up - py + px_prev - px;
+        emit(j + u, mn, me_c, nxv, nyv, true, pre, pre ? v1_[u] : 0.f, pre ? v0_[u] : 0.f,
+             pre ? t_[u] : make_float2(0.f, 0.f));
+        me_c = mn_[u];
+        wy_c = wy_n;
+        wx_c = wx_n;
+        ce_c = cn_[u].x;
+        px_prev = px;
+      }
+      j += U;
     } else {
+      const long long k = (long long)j * LD + i;
+      float me_n = 0.f, wy_n = 0.f, wx_n = 0.f;
       if (ok && j < NY) {
         me_n = M0[k + LD];
         w_of(g, i, j + 1, k + LD, me_n, X0, Y0, wy_n, wx_n);
@@ -263,8 +402,7 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
           wy_dn = a_;
         }
       }
-      py = 0.f;
-      px = 0.f;
+      float nyv = 0.f, nxv = 0.f, py = 0.f, px = 0.f;
       if (ok && i < NX) {
         float a, b;
         hy_coef(g, i, j, a, b);
@@ -277,7 +415,7 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
         nxv = a * Nx0[k] + wx_c - wx_n;
         px = b * nxv;
       }
-      py_up = shfl_up1(py);
+      float py_up = shfl_up1(py);
       if (lane == 0) {
         py_up = 0.f;
         if (ok && i >= 1) {
@@ -287,43 +425,25 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
           py_up = b * (a * Ny0[k - 1] + wyu - wy_c);
         }
       }
-      mn = 0.f;
+      float mn = 0.f;
+      bool inner = false;
       if (ok) {
         const int kd = ekind(NX, NY, P, i, j);
-        const float base = (kd == KIND_INNER) ? g.ce[k] * me_c : (kd == KIND_PML ? 0.f : me_c);
+        inner = kd == KIND_INNER;
+        const float base = inner ? g.cc[k].x * me_c : (kd == KIND_PML ? 0.f : me_c);
         mn = base + py_up - py + px_prev - px;
         if (kd == KIND_PML) {
           X1[k] = g.cxe[k] * (X0[k] + me_c);
           Y1[k] = g.cye[k] * (Y0[k] + me_c);
         }
       }
+      emit(j, mn, me_c, nxv, nyv, inner, false, 0.f, 0.f, make_float2(0.f, 0.f));
+      me_c = me_n;
+      wy_c = wy_n;
+      wx_c = wx_n;
+      px_prev = px;
+      ++j;
     }
-    if (ok) {
-      if (adj && it >= 1) {  // residual of step it-1 enters dJ/dEz(S_it)
-        const int qa = cp[j], qb = cp[j + 1];
-        for (int q = qa; q < qb; ++q)
-          if (s.crow[q] == i) mn += adj[(long long)s.cid[q] * s.nt + (it - 1)];
-      }
-      if (ac && ekind(NX, NY, P, i, j) == KIND_INNER) {
-        const int a = i - g.di0, b = j - g.dj0;
-        if (a >= 0 && a < g.m && b >= 0 && b < g.n) {
-          const long long fk = (long long)b * g.m + a;
-          const float v1 = f1[fk], v0 = f0 ? f0[fk] : 0.f;
-          float2 t = ac[fk];
-          t.x += me_c * (v1 - v0);  // -> eps gradient
-          t.y += me_c * (v1 + v0);  // -> sigma gradient
-          ac[fk] = t;
-        }
-      }
-      if (gw && i == si && j == sj) gw[(long long)shot * s.nt + it] = sc * me_c;
-      M1[k] = mn;
-      if (i < NX) Ny1[k] = nyv;
-      if (j < NY) Nx1[k] = nxv;
-    }
-    me_c = me_n;
-    wy_c = wy_n;
-    wx_c = wx_n;
-    px_prev = px;
   }
 }
 
@@ -388,7 +508,7 @@ __global__ void k_domain_frame(DevGrid g, int ns, const float* __restrict__ E,
 // in shot order; acc is cleared for the next segment.
 __global__ void k_flush(int m, int n, int ns, float* __restrict__ acc, long long astride,
                         const double* __restrict__ fe, const double* __restrict__ fs,
-                        double* __restrict__ ge, double* __restrict__ gs) {
+                        double* __restrict__ ge, double* __restrict__ gs, double scale) {
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const long long per = (long long)m * n;
   if (t >= per) return;
@@ -400,8 +520,8 @@ __global__ void k_flush(int m, int n, int ns, float* __restrict__ acc, long long
     ss += (double)v.y;
     a2[(long long)sh * astride + t] = make_float2(0.f, 0.f);
   }
-  ge[t] += fe[t] * se;
-  gs[t] += fs[t] * ss;
+  ge[t] += (fe ? fe[t] : scale) * se;
+  gs[t] += (fs ? fs[t] : scale) * ss;
 }
 
 // L2 residual e = d - d_o (w_bozdag.m:13-15) and per-block partial sums of
@@ -437,6 +557,47 @@ __global__ void k_reduce_d(const double* __restrict__ part, int n, double scale,
     __syncthreads();
   }
   if (threadIdx.x == 0) *out = scale * sh[0];
+}
+
+// Reference gradient (w_grad.m:72): the residual of step it_r overwrites Ez
+// at every receiver of the shot before the step.  One thread walks the
+// receivers in order, so a repeated receiver keeps the last value, as
+// MATLAB's Ez(Mw) = e assignment does.
+__global__ void k_ref_overwrite(DevGrid g, int nt, int t0, int t1, const int* __restrict__ tr_row,
+                                const int* __restrict__ tr_col, const float* __restrict__ adj,
+                                int it_r, float* __restrict__ E) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int t = t0; t < t1; ++t)
+    E[(long long)tr_col[t] * g.LD + tr_row[t]] = adj[(long long)t * nt + it_r];
+}
+
+// Reference correlation (w_grad.m:86-87) for one step: with a = Ez on the
+// domain after the step and u the stored forward frame k (1-based step),
+//   acc.x += (d_t u)_k * a   (differentiate_cube.m:23-51 stencil)
+//   acc.y += u_k * a
+// U points at frame 0 (Ez(S_0)); frames 1..nt are the forward steps.
+__global__ void k_ref_corr(DevGrid g, const float* __restrict__ E, const float* __restrict__ U,
+                           int k, int nt, double dt, float* __restrict__ acc,
+                           float* __restrict__ amax) {
+  const long long per = (long long)g.m * g.n;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= per) return;
+  const int b = (int)(t / g.m), a = (int)(t % g.m);
+  const float av = E[(long long)(b + g.dj0) * g.LD + (a + g.di0)];
+  auto u = [&](int f) { return (double)U[(long long)f * per + t]; };
+  double du;
+  if (k == 1)
+    du = (-1.5 * u(1) + 2.0 * u(2) - 0.5 * u(3)) / dt;
+  else if (k == nt)
+    du = (1.5 * u(nt) - 2.0 * u(nt - 1) + 0.5 * u(nt - 2)) / dt;
+  else
+    du = (u(k + 1) - u(k - 1)) / (2.0 * dt);
+  float2* a2 = reinterpret_cast<float2*>(acc);
+  float2 v = a2[t];
+  v.x += (float)du * av;
+  v.y += (float)u(k) * av;
+  a2[t] = v;
+  if (amax) amax[t] = fabsf(av);
 }
 
 // In-place second-order time derivative of an nz*nx x nt cube, one thread per

@@ -172,6 +172,10 @@ class Survey:
         self.set_option(N.GJW_OPT_PATH, N.GJW_PATH_PLAIN if plain else N.GJW_PATH_BATCHED)
 
     def set_model(self, eps_r, sigma):
+        if hasattr(eps_r, "detach"):
+            eps_r = eps_r.detach().cpu().numpy()
+        if hasattr(sigma, "detach"):
+            sigma = sigma.detach().cpu().numpy()
         e = np.asfortranarray(eps_r, dtype=np.float64)
         s = np.asfortranarray(sigma, dtype=np.float64)
         assert e.shape == (self.grid.m, self.grid.n) and s.shape == e.shape
@@ -195,7 +199,7 @@ class Survey:
 
     @property
     def ntraces(self):
-        return int(self.rec_ptr[-1])
+        return int(self.rec_ptr[-1]) if self.rec_ptr is not None else 0
 
     def traces_shape(self):
         return (self.ntraces, self.grid.nt)
@@ -204,8 +208,9 @@ class Survey:
     def forward(self, keep=False, out=None):
         """Traces of all shots, (ntraces, nt) float32: row t is trace t
         (shot-major), i.e. shot s's d_2d is out[rec_ptr[s]:rec_ptr[s+1]].T."""
-        if out is None:
+        if out is None and self.rec_ptr is not None:
             out = np.zeros(self.traces_shape(), dtype=np.float32)
+        _sync_torch(out)
         N.check(N.load().gjw_forward(self.h, N.ptr(out), None, int(bool(keep))), "gjw_forward")
         return out
 
@@ -250,6 +255,18 @@ class Survey:
         N.check(N.load().gjw_fwi_gradient(self.h, N.ptr(d_obs), N.ptr(g_eps), N.ptr(g_sig),
                                           C.byref(J)), "gjw_fwi_gradient")
         return g_eps, g_sig, J.value
+
+    def kernel_stats(self, which, reset=False):
+        """(average ms, count) of the sampled forward (0) / adjoint (1) step kernels."""
+        ms, n = C.c_double(0), C.c_int64(0)
+        N.check(N.load().gjw_kernel_stats(self.h, int(which), C.byref(ms), C.byref(n), int(reset)),
+                "gjw_kernel_stats")
+        return ms.value, n.value
+
+    def tile(self):
+        L = C.c_int32(0)
+        N.check(N.load().gjw_tile(self.h, C.byref(L)), "gjw_tile")
+        return L.value
 
     def last_timing(self):
         ms, cells = C.c_double(0), C.c_double(0)

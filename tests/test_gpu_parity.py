@@ -176,3 +176,76 @@ def test_errors_are_reported(gpu_ctx):
     bad[0] = (1, 5)  # on the never-updated outer ring
     with pytest.raises(N.GjwError, match="source"):
         sv.set_shots(bad, case.wavelets, case.receivers)
+
+
+def test_gradient_ref_mode(oracle_mod, gpu_ctx):
+    """w_grad.m semantics (negated sigma, receivers overwritten, correlation
+    with d_t u for eps and u for sigma) against the FP64 restatement."""
+    O = oracle_mod
+    case = configs.small(nt=200, ns=2)
+    case.sig0[:] = np.clip(case.sig0, 0, 5e-3)  # keep the -sigma growth modest
+    sv = make_survey(case, gpu_ctx, plain=True)
+    sv.forward(keep=True)
+    rng = np.random.default_rng(8)
+    adj = rng.standard_normal(sv.traces_shape()).astype(np.float32)
+    from gerjoii_amd import _native as N
+    import ctypes as C
+    ge = np.zeros((case.grid.m, case.grid.n), order="F")
+    gs = np.zeros_like(ge)
+    am = np.zeros((case.grid.m, case.grid.n), dtype=np.float32, order="F")
+    N.check(N.load().gjw_gradient(sv.h, N.ptr(adj), N.GJW_GRAD_REF, N.ptr(ge), N.ptr(gs), None,
+                                  N.ptr(am)), "gjw_gradient REF")
+    g = oracle_grid(O, case.grid)
+    ep, sp = O.exp2pml(g, case.eps0, case.sig0)
+    re = np.zeros_like(ge)
+    rs = np.zeros_like(ge)
+    for q in range(len(case.src)):
+        _, _, u = O.forward(g, ep, sp, case.src[q], case.wavelets[q], case.receivers[q], store_u=True)
+        e = adj[sv.rec_ptr[q]:sv.rec_ptr[q + 1]].T.astype(np.float64)
+        a, amax = O.grad_ref(g, ep, sp, case.src[q], case.receivers[q], e, O.differentiate_cube(u, g.dt))
+        b, _ = O.grad_ref(g, ep, sp, case.src[q], case.receivers[q], e, u)
+        re += a
+        rs += b
+    print("REF eps", rel_l2(ge, re), cosine(ge, re), "sig", rel_l2(gs, rs), "amax", rel_l2(am, amax))
+    assert rel_l2(ge, re) <= 1e-3 and cosine(ge, re) >= 0.9999
+    assert rel_l2(gs, rs) <= 1e-3 and cosine(gs, rs) >= 0.9999
+    assert rel_l2(am, amax) <= 1e-3
+
+
+def test_autograd_function(oracle_mod, gpu_ctx):
+    """torch.autograd.Function backward == gjw_gradient == FP64 oracle, and
+    central finite differences through the FP32 GPU forward agree on the
+    strongest cells within FP32 noise."""
+    import torch
+    from gerjoii_amd.autograd import traces
+    case = configs.small(nt=240, ns=2)
+    sv = make_survey(case, gpu_ctx, ck=11)
+    sv_true = make_survey(case, gpu_ctx)
+    sv_true.set_model(case.eps_true, case.sig_true)
+    dobs = torch.tensor(sv_true.forward(), device="cuda")
+    eps = torch.tensor(case.eps0, dtype=torch.float64, requires_grad=True)
+    sig = torch.tensor(case.sig0, dtype=torch.float64, requires_grad=True)
+    d = traces(eps, sig, sv)
+    loss = 0.5 * ((d.double() - dobs.double()) ** 2).sum()
+    loss.backward()
+    e = (d - dobs).detach().cpu().numpy()
+    fe, fs, _ = oracle_grad(oracle_mod, case, case.eps0, case.sig0, e, sv.rec_ptr)
+    assert rel_l2(eps.grad.numpy(), fe) <= 1e-3 and rel_l2(sig.grad.numpy(), fs) <= 1e-3
+    # finite differences (FP32 forward, FP64 misfit) on the 4 strongest interior cells
+    ge = eps.grad.numpy()
+    inner = np.zeros_like(ge, dtype=bool)
+    inner[2:-2, 2:-2] = True
+    idx = np.argsort(np.where(inner, np.abs(ge), 0).ravel())[-4:]
+    for k in idx:
+        a, b = np.unravel_index(k, ge.shape)
+        h = 2e-3 * case.eps0[a, b]
+        vals = []
+        for sgn in (1, -1):
+            E = case.eps0.copy()
+            E[a, b] += sgn * h
+            sv.set_model(E, case.sig0)
+            dd = sv.forward().astype(np.float64)
+            vals.append(0.5 * ((dd - dobs.cpu().numpy()) ** 2).sum())
+        fd = (vals[0] - vals[1]) / (2 * h)
+        print("fd", (a, b), fd, ge[a, b])
+        assert abs(fd - ge[a, b]) <= 0.05 * abs(ge[a, b])
