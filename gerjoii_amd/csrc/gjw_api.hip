@@ -38,7 +38,7 @@ __global__ void k_inject_last(DevGrid, int, int, int, const int*, const int*, co
 __global__ void k_gather(const float*, long long, const int*, int, int, float*);
 __global__ void k_scatter(const float*, long long, const int*, int, int, float*);
 __global__ void k_domain_frame(DevGrid, int, const float*, float*, long long);
-__global__ void k_flush(int, int, int, float*, long long, const double*, const double*,
+__global__ void k_flush(int, int, int, int, int, float*, long long, const double*, const double*,
                         double*, double*, double);
 __global__ void k_ref_overwrite(DevGrid, int, int, int, const int*, const int*, const float*, int,
                                 float*);
@@ -577,9 +577,9 @@ static int launch_fill_d(double* p, long long n, double v, hipStream_t st) {
   return GJW_OK;
 }
 
+// 1-D launch; the kernels decode (row tile, column strip, shot) themselves
 static inline dim3 step_grid(const gjw_survey* s, int L, int nshots) {
-  return dim3((unsigned)cdiv(s->NX + 1, 64), (unsigned)cdiv(cdiv(s->NY + 1, L), 4),
-              (unsigned)nshots);
+  return dim3((unsigned)(cdiv(s->NX + 1, 64) * cdiv(cdiv(s->NY + 1, L), 4) * nshots));
 }
 
 struct FwdBufs {
@@ -813,16 +813,16 @@ static int forward_batched(gjw_survey* s, int keep) {
 // frames (nt+1 domain frames of the shot) may be null when nothing is kept.
 static int forward_plain_shot(gjw_survey* s, int q, float* fr) {
   const int nt = s->gr.nt;
-  const long long per = (long long)s->gr.m * s->gr.n;
+  const long long FD = (long long)s->LD * s->gr.n;  // one domain frame, field pitch
   const int L = s->tile_L > 0 ? s->tile_L : 16;
   DevGrid g = dev_grid(s);
   FwdBufs b = fwd_bufs(s, 1);
   DevShots sh = dev_shots(s, q);
   TRY(zero_fwd_state(s, 1));
-  if (fr) TRY(launch_fill(fr, per, 0.f, s->ctx->stream));  // frame 0 = Ez(S_0) = 0
+  if (fr) TRY(launch_fill(fr, FD, 0.f, s->ctx->stream));  // frame 0 = Ez(S_0) = 0
   int cur = 0;
   for (int it = 0; it < nt; ++it) {
-    TRY(fwd_launch(s, g, sh, 1, L, b, cur, it, s->tr.p, fr ? fr + (long long)(it + 1) * per : nullptr,
+    TRY(fwd_launch(s, g, sh, 1, L, b, cur, it, s->tr.p, fr ? fr + (long long)(it + 1) * FD : nullptr,
                    0));
     cur ^= 1;
   }
@@ -831,13 +831,13 @@ static int forward_plain_shot(gjw_survey* s, int q, float* fr) {
 
 static int forward_plain(gjw_survey* s, int keep) {
   const int ns = s->ns, nt = s->gr.nt;
-  const long long per = (long long)s->gr.m * s->gr.n;
+  const long long FD = (long long)s->LD * s->gr.n;
   TRY(ensure_fwd_state(s, 1));
   TRY(s->tr.ensure(std::max<long long>((long long)s->ntr * nt, 1)));
-  if (keep) TRY(s->frames.ensure((size_t)ns * (nt + 1) * per));
+  if (keep) TRY(s->frames.ensure((size_t)ns * (nt + 1) * FD));
   HIPCHK(hipEventRecord(s->ctx->ev0, s->ctx->stream));
   for (int q = 0; q < ns; ++q)
-    TRY(forward_plain_shot(s, q, keep ? s->frames.p + (long long)q * (nt + 1) * per : nullptr));
+    TRY(forward_plain_shot(s, q, keep ? s->frames.p + (long long)q * (nt + 1) * FD : nullptr));
   HIPCHK(hipEventRecord(s->ctx->ev1, s->ctx->stream));
   s->kept = keep ? 2 : 0;
   s->last_cells = (double)(s->NX + 1) * (s->NY + 1) * nt * ns;
@@ -924,17 +924,17 @@ static int adj_init(gjw_survey* s, int nsh, int q0, const float* adj) {
 
 static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
   const int ns = s->ns, nt = s->gr.nt, K = s->ck_K;
-  const long long per = (long long)s->gr.m * s->gr.n;
+  const long long per = (long long)s->gr.m * s->gr.n, FD = (long long)s->LD * s->gr.n;
   hipStream_t st = s->ctx->stream;
-  TRY(s->frames.ensure((size_t)ns * (K + 1) * per));
-  TRY(s->acc.ensure((size_t)2 * ns * per));
-  TRY(launch_fill(s->acc.p, 2LL * ns * per, 0.f, st));
+  TRY(s->frames.ensure((size_t)ns * (K + 1) * FD));
+  TRY(s->acc.ensure((size_t)2 * ns * FD));
+  TRY(launch_fill(s->acc.p, 2LL * ns * FD, 0.f, st));
   TRY(adj_init(s, ns, 0, adj));
   DevGrid g = dev_grid(s);
   DevShots sh = dev_shots(s, 0);
   FwdBufs fb = fwd_bufs(s, ns);
   AdjBufs ab = adj_bufs(s, ns);
-  const long long fstride = (long long)(K + 1) * per;  // frames per shot
+  const long long fstride = (long long)(K + 1) * FD;  // frames per shot
   int acur = 0;
   for (int c = s->ck_n - 1; c >= 0; --c) {
     const int k0 = c * K, k1 = std::min(k0 + K, nt);
@@ -945,17 +945,17 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
     HIPCHK(hipGetLastError());
     for (int it = k0; it < k1; ++it) {
       TRY(fwd_launch(s, g, sh, ns, s->tuned_L, fb, cur, it, nullptr,
-                     s->frames.p + (long long)(it + 1 - k0) * per, fstride));
+                     s->frames.p + (long long)(it + 1 - k0) * FD, fstride));
       cur ^= 1;
     }
     for (int it = k1 - 1; it >= k0; --it) {
       TRY(adj_launch(s, g, sh, ns, s->tuned_L, ab, acur, it, adj,
-                     s->frames.p + (long long)(it + 1 - k0) * per,
-                     s->frames.p + (long long)(it - k0) * per, fstride, s->acc.p, per, gw));
+                     s->frames.p + (long long)(it + 1 - k0) * FD,
+                     s->frames.p + (long long)(it - k0) * FD, fstride, s->acc.p, FD, gw));
       acur ^= 1;
     }
     hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
-                       s->gr.n, ns, s->acc.p, per, s->fe.p, s->fs.p, s->ge.p, s->gsg.p, 1.0);
+                       s->gr.n, s->LD, s->di0, ns, s->acc.p, FD, s->fe.p, s->fs.p, s->ge.p, s->gsg.p, 1.0);
     HIPCHK(hipGetLastError());
   }
   return GJW_OK;
@@ -963,7 +963,7 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
 
 static int grad_plain_shot(gjw_survey* s, int q, const float* adj, const float* u, float* gw) {
   const int nt = s->gr.nt;
-  const long long per = (long long)s->gr.m * s->gr.n;
+  const long long per = (long long)s->gr.m * s->gr.n, FD = (long long)s->LD * s->gr.n;
   hipStream_t st = s->ctx->stream;
   const int L = s->tile_L > 0 ? s->tile_L : 16;
   const int K = ck_interval(s);
@@ -973,12 +973,12 @@ static int grad_plain_shot(gjw_survey* s, int q, const float* adj, const float* 
   AdjBufs ab = adj_bufs(s, 1);
   int acur = 0;
   for (int it = nt - 1; it >= 0; --it) {
-    TRY(adj_launch(s, g, sh, 1, L, ab, acur, it, adj, u + (long long)(it + 1) * per,
-                   u + (long long)it * per, 0, s->acc.p, 0, gw ? gw + (long long)q * nt : nullptr));
+    TRY(adj_launch(s, g, sh, 1, L, ab, acur, it, adj, u + (long long)(it + 1) * FD,
+                   u + (long long)it * FD, 0, s->acc.p, 0, gw ? gw + (long long)q * nt : nullptr));
     acur ^= 1;
     if (it % K == 0) {
       hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
-                         s->gr.n, 1, s->acc.p, per, s->fe.p, s->fs.p, s->ge.p, s->gsg.p, 1.0);
+                         s->gr.n, s->LD, s->di0, 1, s->acc.p, FD, s->fe.p, s->fs.p, s->ge.p, s->gsg.p, 1.0);
       HIPCHK(hipGetLastError());
     }
   }
@@ -986,11 +986,11 @@ static int grad_plain_shot(gjw_survey* s, int q, const float* adj, const float* 
 }
 
 static int grad_plain_exact(gjw_survey* s, const float* adj, float* gw) {
-  const long long per = (long long)s->gr.m * s->gr.n;
-  TRY(s->acc.ensure((size_t)2 * per));
-  TRY(launch_fill(s->acc.p, 2LL * per, 0.f, s->ctx->stream));
+  const long long FD = (long long)s->LD * s->gr.n;
+  TRY(s->acc.ensure((size_t)2 * FD));
+  TRY(launch_fill(s->acc.p, 2LL * FD, 0.f, s->ctx->stream));
   for (int q = 0; q < s->ns; ++q)
-    TRY(grad_plain_shot(s, q, adj, s->frames.p + (long long)q * (s->gr.nt + 1) * per, gw));
+    TRY(grad_plain_shot(s, q, adj, s->frames.p + (long long)q * (s->gr.nt + 1) * FD, gw));
   return GJW_OK;
 }
 
@@ -999,6 +999,7 @@ static int grad_plain_exact(gjw_survey* s, const float* adj, float* gw) {
 static int fwi_plain(gjw_survey* s, const float* d_obs) {
   const int ns = s->ns, nt = s->gr.nt;
   const long long per = (long long)s->gr.m * s->gr.n, n = (long long)s->ntr * nt;
+  const long long FD = (long long)s->LD * s->gr.n;
   hipStream_t st = s->ctx->stream;
   const int B = 64;  // residual blocks per shot
   TRY(ensure_fwd_state(s, 1));
@@ -1007,12 +1008,12 @@ static int fwi_plain(gjw_survey* s, const float* d_obs) {
   TRY(s->obs.ensure(std::max<long long>(n, 1)));
   TRY(s->part.ensure((size_t)ns * B));
   TRY(s->Jd.ensure(1));
-  TRY(s->frames.ensure((size_t)(nt + 1) * per));
-  TRY(s->acc.ensure((size_t)2 * per));
+  TRY(s->frames.ensure((size_t)(nt + 1) * FD));
+  TRY(s->acc.ensure((size_t)2 * FD));
   TRY(s->ge.ensure(per));
   TRY(s->gsg.ensure(per));
   TRY(host_or_dev_copy(s->obs.p, d_obs, n * 4, st));
-  TRY(launch_fill(s->acc.p, 2LL * per, 0.f, st));
+  TRY(launch_fill(s->acc.p, 2LL * FD, 0.f, st));
   TRY(launch_fill_d(s->ge.p, per, 0.0, st));
   TRY(launch_fill_d(s->gsg.p, per, 0.0, st));
   TRY(launch_fill_d(s->part.p, (long long)ns * B, 0.0, st));
@@ -1039,7 +1040,7 @@ static int fwi_plain(gjw_survey* s, const float* d_obs) {
 // step, no source, and the correlation g -= u(nt-it) .* a, g *= dt.
 static int grad_ref(gjw_survey* s, const float* adj, float* amax) {
   const int ns = s->ns, nt = s->gr.nt, NX = s->NX, NY = s->NY, P = s->P;
-  const long long per = (long long)s->gr.m * s->gr.n, FS = s->FS;
+  const long long per = (long long)s->gr.m * s->gr.n, FS = s->FS, FD = (long long)s->LD * s->gr.n;
   hipStream_t st = s->ctx->stream;
   const int L = s->tile_L > 0 ? s->tile_L : 16;
   const int K = ck_interval(s);
@@ -1061,15 +1062,15 @@ static int grad_ref(gjw_survey* s, const float* adj, float* amax) {
   TRY(launch_fill(s->zero_sc.p, ns, 0.f, st));
   DevGrid g = dev_grid(s);
   g.cc = reinterpret_cast<const float2*>(s->coef_ref.p);
-  TRY(s->acc.ensure((size_t)2 * per));
-  TRY(launch_fill(s->acc.p, 2LL * per, 0.f, st));
+  TRY(s->acc.ensure((size_t)2 * FD));
+  TRY(launch_fill(s->acc.p, 2LL * FD, 0.f, st));
   TRY(ensure_fwd_state(s, 1));
   FwdBufs b = fwd_bufs(s, 1);
   for (int q = 0; q < ns; ++q) {
     DevShots sh = dev_shots(s, q);
     sh.sc = s->zero_sc.p + q;
     TRY(zero_fwd_state(s, 1));
-    const float* U = s->frames.p + (long long)q * (nt + 1) * per;
+    const float* U = s->frames.p + (long long)q * (nt + 1) * FD;
     const int t0 = (int)s->rec_ptr[q], t1 = (int)s->rec_ptr[q + 1];
     int cur = 0;
     for (int it = 0; it < nt; ++it) {
@@ -1086,7 +1087,7 @@ static int grad_ref(gjw_survey* s, const float* adj, float* amax) {
       HIPCHK(hipGetLastError());
       if (it % K == K - 1 || it == nt - 1) {
         hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
-                           s->gr.n, 1, s->acc.p, per, nullptr, nullptr, s->ge.p, s->gsg.p,
+                           s->gr.n, s->LD, s->di0, 1, s->acc.p, FD, nullptr, nullptr, s->ge.p, s->gsg.p,
                            -s->gr.dt);
         HIPCHK(hipGetLastError());
       }

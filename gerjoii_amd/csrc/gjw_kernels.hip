@@ -67,6 +67,30 @@ __device__ __forceinline__ float hx_new(const DevGrid& g, int i, int j, float hx
   return a * hx + b * (e_jp1 - e_j);
 }
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
+// XCDs (each with its own L2), so launch-order neighbours land on different
+// L2s.  The launch is 1-D; block b is remapped to logical tile
+// t = (b mod 8) * (B/8) + b/8, which gives each XCD a contiguous run of
+// logical tiles.  Logical tiles enumerate row tiles fastest, then column
+// strips, then shots, so the rows and columns a wave reads as halo were
+// fetched by a wave on the same XCD.
+struct TileId {
+  int rowtile, strip, shot;
+};
+
+__device__ __forceinline__ TileId tile_of_block(int NX, int NY, int L) {
+  const unsigned B = gridDim.x, b = blockIdx.x;
+  const unsigned per = B / 8u;
+  const unsigned t = (b < per * 8u) ? (b % 8u) * per + b / 8u : b;
+  const unsigned gx = (unsigned)((NX + 1 + 63) / 64);
+  const unsigned gy = (unsigned)(((NY + 1 + L - 1) / L + 3) / 4);
+  TileId id;
+  id.rowtile = (int)(t % gx);
+  id.strip = (int)((t / gx) % gy);
+  id.shot = (int)(t / (gx * gy));
+  return id;
+}
+
 // ==========================================================================
 // forward: fused step
 // ==========================================================================
@@ -121,10 +145,11 @@ k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
            float* __restrict__ Y, int it, float* __restrict__ tr, float* __restrict__ frame,
            long long fstride) {
   const int lane = threadIdx.x;
-  const int shot = blockIdx.z;
-  const int i0 = blockIdx.x * 64;
-  const int jb = (blockIdx.y * blockDim.y + threadIdx.y) * L;
   const int NX = g.NX, NY = g.NY, P = g.P;
+  const TileId tid = tile_of_block(NX, NY, L);
+  const int shot = tid.shot;
+  const int i0 = tid.rowtile * 64;
+  const int jb = (tid.strip * blockDim.y + threadIdx.y) * L;
   const long long LD = g.LD;
   if (jb > NY) return;
   const int je = min(jb + L, NY + 1);
@@ -159,9 +184,9 @@ k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
         for (int q = qa; q < qb; ++q)
           if (s.crow[q] == i) tr[(long long)s.cid[q] * s.nt + it] = en;
       }
-      if (fr) {
+      if (fr) {  // domain frames keep the field pitch: cell (i, j) at (j - dj0) * LD + i
         const int a = i - g.di0, b = j - g.dj0;
-        if (a >= 0 && a < g.m && b >= 0 && b < g.n) fr[(long long)b * g.m + a] = en;
+        if (a >= 0 && a < g.m && b >= 0 && b < g.n) fr[(long long)b * LD + i] = en;
       }
     }
   };
@@ -252,10 +277,11 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
            const float* __restrict__ F1, const float* __restrict__ F0, long long fstride,
            float* __restrict__ acc, long long astride, float* __restrict__ gw) {
   const int lane = threadIdx.x;
-  const int shot = blockIdx.z;
-  const int i0 = blockIdx.x * 64;
-  const int jb = (blockIdx.y * blockDim.y + threadIdx.y) * L;
   const int NX = g.NX, NY = g.NY, P = g.P;
+  const TileId tid = tile_of_block(NX, NY, L);
+  const int shot = tid.shot;
+  const int i0 = tid.rowtile * 64;
+  const int jb = (tid.strip * blockDim.y + threadIdx.y) * L;
   const long long LD = g.LD;
   if (jb > NY) return;
   const int je = min(jb + L, NY + 1);
@@ -305,7 +331,7 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
     if (ac && inner && arow) {
       const int b = j - g.dj0;
       if (b >= 0 && b < g.n) {
-        const long long fk = (long long)b * g.m + ai;
+        const long long fk = (long long)b * LD + i;
         if (!pre_ok) {
           v1 = f1[fk];
           v0 = f0 ? f0[fk] : 0.f;
@@ -358,7 +384,7 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
       if (pre) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const long long fk = (long long)(j + u - g.dj0) * g.m + ai;
+          const long long fk = (long long)(j + u - g.dj0) * LD + i;
           v1_[u] = f1[fk];
           v0_[u] = f0 ? f0[fk] : 0.f;
           t_[u] = ac[fk];
@@ -500,25 +526,27 @@ __global__ void k_domain_frame(DevGrid g, int ns, const float* __restrict__ E,
   const int sh = (int)(t / per);
   const long long r = t % per;
   const int b = (int)(r / g.m), a = (int)(r % g.m);
-  frame[(long long)sh * fstride + r] =
+  frame[(long long)sh * fstride + (long long)b * g.LD + (a + g.di0)] =
       E[(long long)sh * g.FS + (long long)(b + g.dj0) * g.LD + (a + g.di0)];
 }
 
 // acc (float2 per domain cell per shot) -> FP64 gradient, summed over shots
 // in shot order; acc is cleared for the next segment.
-__global__ void k_flush(int m, int n, int ns, float* __restrict__ acc, long long astride,
-                        const double* __restrict__ fe, const double* __restrict__ fs,
-                        double* __restrict__ ge, double* __restrict__ gs, double scale) {
+__global__ void k_flush(int m, int n, int LD, int di0, int ns, float* __restrict__ acc,
+                        long long astride, const double* __restrict__ fe,
+                        const double* __restrict__ fs, double* __restrict__ ge,
+                        double* __restrict__ gs, double scale) {
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const long long per = (long long)m * n;
   if (t >= per) return;
+  const long long ka = (t / m) * LD + (t % m) + di0;  // partials use the field pitch
   float2* a2 = reinterpret_cast<float2*>(acc);
   double se = 0.0, ss = 0.0;
   for (int sh = 0; sh < ns; ++sh) {
-    float2 v = a2[(long long)sh * astride + t];
+    float2 v = a2[(long long)sh * astride + ka];
     se += (double)v.x;
     ss += (double)v.y;
-    a2[(long long)sh * astride + t] = make_float2(0.f, 0.f);
+    a2[(long long)sh * astride + ka] = make_float2(0.f, 0.f);
   }
   ge[t] += (fe ? fe[t] : scale) * se;
   gs[t] += (fs ? fs[t] : scale) * ss;
@@ -583,8 +611,9 @@ __global__ void k_ref_corr(DevGrid g, const float* __restrict__ E, const float* 
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t >= per) return;
   const int b = (int)(t / g.m), a = (int)(t % g.m);
+  const long long FD = (long long)g.LD * g.n, ka = (long long)b * g.LD + a + g.di0;
   const float av = E[(long long)(b + g.dj0) * g.LD + (a + g.di0)];
-  auto u = [&](int f) { return (double)U[(long long)f * per + t]; };
+  auto u = [&](int f) { return (double)U[(long long)f * FD + ka]; };
   double du;
   if (k == 1)
     du = (-1.5 * u(1) + 2.0 * u(2) - 0.5 * u(3)) / dt;
@@ -593,10 +622,10 @@ __global__ void k_ref_corr(DevGrid g, const float* __restrict__ E, const float* 
   else
     du = (u(k + 1) - u(k - 1)) / (2.0 * dt);
   float2* a2 = reinterpret_cast<float2*>(acc);
-  float2 v = a2[t];
+  float2 v = a2[ka];
   v.x += (float)du * av;
   v.y += (float)u(k) * av;
-  a2[t] = v;
+  a2[ka] = v;
   if (amax) amax[t] = fabsf(av);
 }
 
