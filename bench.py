@@ -100,8 +100,8 @@ def main():
         sv.fwi_gradient(d_obs, g_eps, g_sig)
         log(f"warmup {w + 1}/{args.warmup} done")
     sv.set_option(N.GJW_OPT_KTIMING, 16)
-    sv.kernel_stats(0, reset=True)
-    sv.kernel_stats(1, reset=True)
+    for kk in range(3):
+        sv.kernel_stats(kk, reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     J = 0.0
@@ -111,8 +111,9 @@ def main():
     torch.cuda.synchronize()
     dt_s = time.perf_counter() - t0
     sv.set_option(N.GJW_OPT_KTIMING, 0)
-    fwd_ms, fwd_n = sv.kernel_stats(0)
-    adj_ms, adj_n = sv.kernel_stats(1)
+    fwd1_ms, fwd1_n, fwd1_l = sv.kernel_stats(0)
+    adj_ms, adj_n, adj_l = sv.kernel_stats(1)
+    fwd2_ms, fwd2_n, fwd2_l = sv.kernel_stats(2)
     ms_step = dt_s * 1e3 / args.steps
     shot_grads = ns * args.steps / dt_s
     cell_updates = 3.0 * cells * g.nt * ns * args.steps  # forward, recompute, adjoint
@@ -125,19 +126,27 @@ def main():
 
     # per-kernel roofline: algorithmic bytes per launch / live average duration
     bytes_launch = BYTES_PER_CELL_UPDATE * cells * ns
+    # launches per gradient: the forward and recompute sweeps run two time steps
+    # per k_fwd_step2 launch (one-step k_fwd_step only for odd segment tails)
     kern = {
-        "k_fwd_step": {"avg_ms": fwd_ms, "samples": fwd_n, "launches_per_step": 2 * g.nt},
-        "k_adj_step": {"avg_ms": adj_ms, "samples": adj_n, "launches_per_step": g.nt},
+        "k_fwd_step2": {"avg_ms": fwd2_ms, "samples": fwd2_n, "steps_per_launch": 2,
+                        "launches_per_step": fwd2_l / args.steps},
+        "k_fwd_step": {"avg_ms": fwd1_ms, "samples": fwd1_n, "steps_per_launch": 1,
+                       "launches_per_step": fwd1_l / args.steps},
+        "k_adj_step": {"avg_ms": adj_ms, "samples": adj_n, "steps_per_launch": 1,
+                       "launches_per_step": adj_l / args.steps},
     }
     for k, v in kern.items():
-        v["achieved_GBs"] = bytes_launch / (v["avg_ms"] * 1e-3) / 1e9 if v["avg_ms"] > 0 else None
+        spl = v["steps_per_launch"]
+        v["achieved_GBs"] = (spl * bytes_launch / (v["avg_ms"] * 1e-3) / 1e9) if v["avg_ms"] > 0 else None
         v["share_of_step"] = v["avg_ms"] * v["launches_per_step"] / ms_step if ms_step > 0 else None
     dom = max(kern, key=lambda k: kern[k]["avg_ms"] * kern[k]["launches_per_step"])
     traffic = _traffic_from_profiles(dom, cells, ns)
     roof = {"kernel": dom, "bound": "hbm", "achieved": kern[dom]["achieved_GBs"],
             "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": (kern[dom]["achieved_GBs"] or 0) / PEAK_HBM_GBS,
-            "traffic": traffic, "bytes_per_launch_algorithmic": bytes_launch,
+            "traffic": traffic,
+            "bytes_per_launch_algorithmic": bytes_launch * kern[dom]["steps_per_launch"],
             "kernels": kern}
 
     # plain single-shot path on a subset of the same shots

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libgjw.so")
 GJW_OK = 0
 GJW_PATH_BATCHED, GJW_PATH_PLAIN = 0, 1
 GJW_OPT_PATH, GJW_OPT_CKPT_EVERY, GJW_OPT_TILE_L, GJW_OPT_SNAPSHOT, GJW_OPT_KTIMING = 1, 2, 3, 4, 5
+GJW_OPT_FUSE = 6
 GJW_SNAP_F32, GJW_SNAP_BF16, GJW_SNAP_F16 = 0, 1, 2
 GJW_MISFIT_L2 = 0
 GJW_GRAD_EXACT, GJW_GRAD_REF = 0, 1
@@ -66,7 +67,7 @@ def load():
     lib.gjw_fwi_gradient.argtypes = [vp, vp, vp, vp, vp]
     lib.gjw_differentiate_cube.argtypes = [vp, vp, i32, i32, i32, C.c_double]
     lib.gjw_last_timing.argtypes = [vp, vp, vp]
-    lib.gjw_kernel_stats.argtypes = [vp, i32, vp, vp, i32]
+    lib.gjw_kernel_stats.argtypes = [vp, i32, vp, vp, vp, i32]
     lib.gjw_tile.argtypes = [vp, vp]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int

@@ -31,6 +31,20 @@ __global__ void k_adj_step(DevGrid, DevShots, int, const float*, const float*, c
                            const float*, const float*, float*, float*, float*, float*, float*,
                            int, const float*, const float*, const float*, long long, float*,
                            long long, float*);
+template <bool FRAMES>
+__global__ void k_fwd_step2(DevGrid, DevShots, int, const float*, const float*, const float*,
+                            const float*, const float*, float*, float*, float*, float*, float*, int,
+                            float*, float*, float*, long long, float*);
+extern template __global__ void k_fwd_step2<false>(DevGrid, DevShots, int, const float*,
+                                                   const float*, const float*, const float*,
+                                                   const float*, float*, float*, float*, float*,
+                                                   float*, int, float*, float*, float*, long long,
+                                                   float*);
+extern template __global__ void k_fwd_step2<true>(DevGrid, DevShots, int, const float*,
+                                                  const float*, const float*, const float*,
+                                                  const float*, float*, float*, float*, float*,
+                                                  float*, int, float*, float*, float*, long long,
+                                                  float*);
 __global__ void k_fill(float*, long long, float);
 __global__ void k_fill_d(double*, long long, double);
 __global__ void k_inject_last(DevGrid, int, int, int, const int*, const int*, const int*,
@@ -118,14 +132,15 @@ struct gjw_survey {
   long long FS = 0;
   bool have_model = false, have_shots = false;
   // options
-  int path = GJW_PATH_BATCHED, ck_every = 0, tile_L = 0, snap = GJW_SNAP_F32;
-  int tuned_L = 0;
+  int path = GJW_PATH_BATCHED, ck_every = 0, tile_L = 0, snap = GJW_SNAP_F32, fuse = 2;
+  int tuned_L = 0, tuned_L1 = 0;  // two-step and one-step kernels
   // model, padded (NX x NY, column-major, FP64)
   std::vector<double> pe, ps;
   float bx0 = 0.f, by0 = 0.f;
   DBuf<float> coef;  // ce, ch, cxe, cxh, cye, cyh, hax, hbx, hay, hby (10*FS)
   DBuf<float> coef_ref;  // ce, ch with sigma negated (w_grad.m:26-27)
   DBuf<float> zero_sc;   // per-shot zero source coefficients
+  DBuf<float> trash;     // write target of lanes that own no cell (see k_fwd_step2)
   DBuf<double> fe, fs;
   std::vector<int> pml_idx;
   DBuf<int> d_pml;
@@ -153,8 +168,9 @@ struct gjw_survey {
   int ksample = 0;
   std::vector<hipEvent_t> kev_pool;
   std::vector<std::pair<int, int>> kev_used;  // (kernel id, index of start event)
-  double kt_ms[2] = {0, 0};
-  long long kt_n[2] = {0, 0};
+  double kt_ms[3] = {0, 0, 0};  // k_fwd_step, k_adj_step, k_fwd_step2
+  long long kt_n[3] = {0, 0, 0};
+  long long kl_n[3] = {0, 0, 0};  // all launches while timing is on
 };
 
 static inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
@@ -227,7 +243,8 @@ extern "C" int gjw_survey_destroy(gjw_survey* s) {
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
   for (auto e : s->kev_pool) (void)hipEventDestroy(e);
-  s->coef.release(); s->coef_ref.release(); s->zero_sc.release(); s->fe.release(); s->fs.release(); s->d_pml.release();
+  s->coef.release(); s->coef_ref.release(); s->zero_sc.release(); s->trash.release();
+  s->fe.release(); s->fs.release(); s->d_pml.release();
   s->d_si.release(); s->d_sj.release(); s->d_cptr.release(); s->d_crow.release();
   s->d_cid.release(); s->d_trs.release(); s->d_trr.release(); s->d_trc.release();
   s->d_sc.release(); s->d_wav.release(); s->fld.release(); s->adjf.release();
@@ -253,6 +270,11 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
     case GJW_OPT_TILE_L:
       if (v < 0 || v > 4096) return fail(GJW_ERR_ARG, "GJW_OPT_TILE_L out of range");
       s->tile_L = (int)v;
+      break;
+    case GJW_OPT_FUSE:
+      if (v != 1 && v != 2) return fail(GJW_ERR_ARG, "GJW_OPT_FUSE must be 1 or 2");
+      s->fuse = (int)v;
+      s->tuned_L = 0;
       break;
     case GJW_OPT_KTIMING:
       if (v < 0) return fail(GJW_ERR_ARG, "GJW_OPT_KTIMING must be >= 0");
@@ -583,7 +605,7 @@ static inline dim3 step_grid(const gjw_survey* s, int L, int nshots) {
 }
 
 struct FwdBufs {
-  float *E[2], *Hx[2], *Hy[2], *X, *Y;
+  float *E[2], *Hx[2], *Hy[2], *X[2], *Y[2];
 };
 
 static FwdBufs fwd_bufs(gjw_survey* s, int nsh) {
@@ -592,7 +614,7 @@ static FwdBufs fwd_bufs(gjw_survey* s, int nsh) {
   float* p = s->fld.p;
   b.E[0] = p; b.Hx[0] = p + sz; b.Hy[0] = p + 2 * sz;
   b.E[1] = p + 3 * sz; b.Hx[1] = p + 4 * sz; b.Hy[1] = p + 5 * sz;
-  b.X = p + 6 * sz; b.Y = p + 7 * sz;
+  b.X[0] = p + 6 * sz; b.Y[0] = p + 7 * sz; b.X[1] = p + 8 * sz; b.Y[1] = p + 9 * sz;
   return b;
 }
 
@@ -616,7 +638,9 @@ static AdjBufs adj_bufs(gjw_survey* s, int nsh) {
 
 static int kev_begin(gjw_survey* s, int which, int it, int* slot) {
   *slot = -1;
-  if (s->ksample <= 0 || it % s->ksample != 0) return GJW_OK;
+  if (s->ksample <= 0) return GJW_OK;
+  s->kl_n[which] += 1;
+  if (it % s->ksample > 1 || (it % s->ksample == 1 && which != 2)) return GJW_OK;
   const size_t need = 2 * (s->kev_used.size() + 1);
   while (s->kev_pool.size() < need) {
     hipEvent_t e;
@@ -649,14 +673,60 @@ static int kev_resolve(gjw_survey* s) {
 }
 
 static int fwd_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, int L,
-                      FwdBufs& b, int cur, int it, float* tr, float* frame, long long fstride) {
+                      FwdBufs& b, int cur, int it, float* tr, float* frame, long long fstride,
+                      int xc = 0) {
   int slot;
   TRY(kev_begin(s, 0, it, &slot));
   hipLaunchKernelGGL(k_fwd_step, step_grid(s, L, nsh), dim3(64, 4), 0, s->ctx->stream, g, sh, L,
                      b.E[cur], b.Hx[cur], b.Hy[cur], b.E[cur ^ 1], b.Hx[cur ^ 1], b.Hy[cur ^ 1],
-                     b.X, b.Y, it, tr, frame, fstride);
+                     b.X[xc], b.Y[xc], it, tr, frame, fstride);
   HIPCHK(hipGetLastError());
   return kev_end(s, slot);
+}
+
+// two steps per launch; grid of 60-row overlapped tiles
+static inline dim3 step2_grid(const gjw_survey* s, int L, int nshots) {
+  return dim3((unsigned)(cdiv(s->NX + 1, 60) * cdiv(cdiv(s->NY + 1, L), 4) * nshots));
+}
+
+static int fwd2_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, int L,
+                       FwdBufs& b, int cur, int xc, int it, float* tr, float* fr1, float* fr2,
+                       long long fstride) {
+  int slot;
+  TRY(s->trash.ensure(256));
+  TRY(kev_begin(s, 2, it, &slot));
+  if (fr1)
+    hipLaunchKernelGGL(k_fwd_step2<true>, step2_grid(s, L, nsh), dim3(64, 4), 0, s->ctx->stream,
+                       g, sh, L, b.E[cur], b.Hx[cur], b.Hy[cur], b.X[xc], b.Y[xc], b.E[cur ^ 1],
+                       b.Hx[cur ^ 1], b.Hy[cur ^ 1], b.X[xc ^ 1], b.Y[xc ^ 1], it, tr, fr1, fr2,
+                       fstride, s->trash.p);
+  else
+    hipLaunchKernelGGL(k_fwd_step2<false>, step2_grid(s, L, nsh), dim3(64, 4), 0, s->ctx->stream,
+                       g, sh, L, b.E[cur], b.Hx[cur], b.Hy[cur], b.X[xc], b.Y[xc], b.E[cur ^ 1],
+                       b.Hx[cur ^ 1], b.Hy[cur ^ 1], b.X[xc ^ 1], b.Y[xc ^ 1], it, tr, fr1, fr2,
+                       fstride, s->trash.p);
+  HIPCHK(hipGetLastError());
+  return kev_end(s, slot);
+}
+
+// Advance the batched forward state from step it by up to `limit` steps (one
+// or two per launch); frame (optional) receives Ez(S_{it+1}) and, for a
+// two-step launch, Ez(S_{it+2}) at frame + fstep.  Returns the steps taken.
+static int fwd_advance(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, FwdBufs& b,
+                       int& cur, int& xc, int it, int limit, float* tr, float* frame,
+                       long long fstep, long long fstride, int* taken) {
+  if (s->fuse == 2 && limit >= 2) {
+    TRY(fwd2_launch(s, g, sh, nsh, s->tuned_L, b, cur, xc, it, tr, frame,
+                    frame ? frame + fstep : nullptr, fstride));
+    cur ^= 1;
+    xc ^= 1;
+    *taken = 2;
+  } else {
+    TRY(fwd_launch(s, g, sh, nsh, s->tuned_L1, b, cur, it, tr, frame, fstride, xc));
+    cur ^= 1;
+    *taken = 1;
+  }
+  return GJW_OK;
 }
 
 static int adj_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, int L,
@@ -681,53 +751,61 @@ static int check_ready(gjw_survey* s, const char* who) {
   return GJW_OK;
 }
 
-// Tile width L (columns per wavefront): time a few steps of each candidate on
-// zeroed batch state and keep the fastest.
+// Tile width L (columns per wavefront): time a few launches of each candidate
+// on zeroed batch state and keep the fastest, for the two-step kernel and for
+// the one-step kernel (odd tails, plain path).
 static int autotune_L(gjw_survey* s, int nsh) {
   if (s->tile_L > 0) {
-    s->tuned_L = s->tile_L;
+    s->tuned_L = s->tuned_L1 = s->tile_L;
     return GJW_OK;
   }
-  if (s->tuned_L > 0) return GJW_OK;
-  const int cand[] = {8, 16, 32, 64};
+  if (s->tuned_L > 0 && s->tuned_L1 > 0) return GJW_OK;
   hipStream_t st = s->ctx->stream;
   DevGrid g = dev_grid(s);
   DevShots sh = dev_shots(s, 0);
   FwdBufs b = fwd_bufs(s, nsh);
-  float best = 1e30f;
-  int bestL = 16;
-  const int reps = 6;
   const int ks = s->ksample;
   s->ksample = 0;
-  for (int L : cand) {
-    int cur = 0;
-    TRY(fwd_launch(s, g, sh, nsh, L, b, cur, 0, nullptr, nullptr, 0));  // warm
-    HIPCHK(hipEventRecord(s->ctx->ev0, st));
-    for (int r = 0; r < reps; ++r) {
-      TRY(fwd_launch(s, g, sh, nsh, L, b, cur, 0, nullptr, nullptr, 0));
-      cur ^= 1;
+  const int reps = 4;
+  for (int which = 1; which <= 2; ++which) {
+    const int c1[] = {8, 16, 32, 64}, c2[] = {16, 32, 64, 128};
+    const int* cand = which == 1 ? c1 : c2;
+    float best = 1e30f;
+    int bestL = which == 1 ? 16 : 32;
+    for (int ci = 0; ci < 4; ++ci) {
+      const int L = cand[ci];
+      int cur = 0, xc = 0;
+      for (int r = -1; r < reps; ++r) {
+        if (r == 0) HIPCHK(hipEventRecord(s->ctx->ev0, st));
+        if (which == 1)
+          TRY(fwd_launch(s, g, sh, nsh, L, b, cur, 0, nullptr, nullptr, 0, xc));
+        else
+          TRY(fwd2_launch(s, g, sh, nsh, L, b, cur, xc, 0, nullptr, nullptr, nullptr, 0));
+        cur ^= 1;
+        if (which == 2) xc ^= 1;
+      }
+      HIPCHK(hipEventRecord(s->ctx->ev1, st));
+      HIPCHK(hipEventSynchronize(s->ctx->ev1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
+      if (ms < best) {
+        best = ms;
+        bestL = L;
+      }
     }
-    HIPCHK(hipEventRecord(s->ctx->ev1, st));
-    HIPCHK(hipEventSynchronize(s->ctx->ev1));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
-    if (ms < best) {
-      best = ms;
-      bestL = L;
-    }
+    (which == 1 ? s->tuned_L1 : s->tuned_L) = bestL;
   }
-  s->tuned_L = bestL;
   s->ksample = ks;
   return GJW_OK;
 }
 
 static int ensure_fwd_state(gjw_survey* s, int nsh) {
-  TRY(s->fld.ensure((size_t)8 * nsh * s->FS));
+  TRY(s->fld.ensure((size_t)10 * nsh * s->FS));
   return GJW_OK;
 }
 
 static int zero_fwd_state(gjw_survey* s, int nsh) {
-  return launch_fill(s->fld.p, (long long)8 * nsh * s->FS, 0.f, s->ctx->stream);
+  return launch_fill(s->fld.p, (long long)10 * nsh * s->FS, 0.f, s->ctx->stream);
 }
 
 // checkpoint layout: [ck][E, Hx, Hy: ns*FS each][X, Y: ns*npml each]
@@ -735,7 +813,7 @@ static long long ck_stride(const gjw_survey* s) {
   return 3LL * s->ns * s->FS + 2LL * s->ns * (long long)s->pml_idx.size();
 }
 
-static int ck_save(gjw_survey* s, FwdBufs& b, int cur, int c) {
+static int ck_save(gjw_survey* s, FwdBufs& b, int cur, int xc, int c) {
   hipStream_t st = s->ctx->stream;
   const long long sz = (long long)s->ns * s->FS, npml = (long long)s->pml_idx.size();
   float* dst = s->ck.p + (long long)c * ck_stride(s);
@@ -744,16 +822,16 @@ static int ck_save(gjw_survey* s, FwdBufs& b, int cur, int c) {
   HIPCHK(hipMemcpyAsync(dst + 2 * sz, b.Hy[cur], sz * 4, hipMemcpyDeviceToDevice, st));
   const long long nn = npml * s->ns;
   if (nn > 0) {
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)cdiv(nn, 256)), dim3(256), 0, st, b.X, s->FS,
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)cdiv(nn, 256)), dim3(256), 0, st, b.X[xc], s->FS,
                        s->d_pml.p, (int)npml, s->ns, dst + 3 * sz);
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)cdiv(nn, 256)), dim3(256), 0, st, b.Y, s->FS,
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)cdiv(nn, 256)), dim3(256), 0, st, b.Y[xc], s->FS,
                        s->d_pml.p, (int)npml, s->ns, dst + 3 * sz + nn);
     HIPCHK(hipGetLastError());
   }
   return GJW_OK;
 }
 
-static int ck_restore(gjw_survey* s, FwdBufs& b, int cur, int c) {
+static int ck_restore(gjw_survey* s, FwdBufs& b, int cur, int xc, int c) {
   hipStream_t st = s->ctx->stream;
   const long long sz = (long long)s->ns * s->FS, npml = (long long)s->pml_idx.size();
   const float* src = s->ck.p + (long long)c * ck_stride(s);
@@ -763,9 +841,9 @@ static int ck_restore(gjw_survey* s, FwdBufs& b, int cur, int c) {
   const long long nn = npml * s->ns;
   if (nn > 0) {
     hipLaunchKernelGGL(k_scatter, dim3((unsigned)cdiv(nn, 256)), dim3(256), 0, st,
-                       src + 3 * sz, s->FS, s->d_pml.p, (int)npml, s->ns, b.X);
+                       src + 3 * sz, s->FS, s->d_pml.p, (int)npml, s->ns, b.X[xc]);
     hipLaunchKernelGGL(k_scatter, dim3((unsigned)cdiv(nn, 256)), dim3(256), 0, st,
-                       src + 3 * sz + nn, s->FS, s->d_pml.p, (int)npml, s->ns, b.Y);
+                       src + 3 * sz + nn, s->FS, s->d_pml.p, (int)npml, s->ns, b.Y[xc]);
     HIPCHK(hipGetLastError());
   }
   return GJW_OK;
@@ -795,12 +873,16 @@ static int forward_batched(gjw_survey* s, int keep) {
   DevGrid g = dev_grid(s);
   DevShots sh = dev_shots(s, 0);
   FwdBufs b = fwd_bufs(s, ns);
-  int cur = 0;
+  int cur = 0, xc = 0;
   HIPCHK(hipEventRecord(s->ctx->ev0, st));
-  for (int it = 0; it < nt; ++it) {
-    if (keep && it % s->ck_K == 0) TRY(ck_save(s, b, cur, it / s->ck_K));
-    TRY(fwd_launch(s, g, sh, ns, s->tuned_L, b, cur, it, s->tr.p, nullptr, 0));
-    cur ^= 1;
+  for (int it = 0; it < nt;) {
+    if (keep && it % s->ck_K == 0) TRY(ck_save(s, b, cur, xc, it / s->ck_K));
+    // never step across a checkpoint boundary
+    int limit = nt - it;
+    if (keep) limit = std::min(limit, s->ck_K - it % s->ck_K);
+    int taken = 0;
+    TRY(fwd_advance(s, g, sh, ns, b, cur, xc, it, limit, s->tr.p, nullptr, 0, 0, &taken));
+    it += taken;
   }
   HIPCHK(hipEventRecord(s->ctx->ev1, st));
   s->cur = cur;
@@ -938,18 +1020,19 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
   int acur = 0;
   for (int c = s->ck_n - 1; c >= 0; --c) {
     const int k0 = c * K, k1 = std::min(k0 + K, nt);
-    int cur = 0;
-    TRY(ck_restore(s, fb, cur, c));
+    int cur = 0, xc = 0;
+    TRY(ck_restore(s, fb, cur, xc, c));
     hipLaunchKernelGGL(k_domain_frame, dim3((unsigned)cdiv(per * ns, 256)), dim3(256), 0, st, g, ns,
                        fb.E[cur], s->frames.p, fstride);
     HIPCHK(hipGetLastError());
-    for (int it = k0; it < k1; ++it) {
-      TRY(fwd_launch(s, g, sh, ns, s->tuned_L, fb, cur, it, nullptr,
-                     s->frames.p + (long long)(it + 1 - k0) * FD, fstride));
-      cur ^= 1;
+    for (int it = k0; it < k1;) {
+      int taken = 0;
+      TRY(fwd_advance(s, g, sh, ns, fb, cur, xc, it, k1 - it, nullptr,
+                      s->frames.p + (long long)(it + 1 - k0) * FD, FD, fstride, &taken));
+      it += taken;
     }
     for (int it = k1 - 1; it >= k0; --it) {
-      TRY(adj_launch(s, g, sh, ns, s->tuned_L, ab, acur, it, adj,
+      TRY(adj_launch(s, g, sh, ns, s->tuned_L1, ab, acur, it, adj,
                      s->frames.p + (long long)(it + 1 - k0) * FD,
                      s->frames.p + (long long)(it - k0) * FD, fstride, s->acc.p, FD, gw));
       acur ^= 1;
@@ -1208,14 +1291,16 @@ extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps
 }
 
 extern "C" int gjw_kernel_stats(gjw_survey* s, int32_t which, double* avg_ms, int64_t* count,
-                                int32_t reset) {
-  if (!s || which < 0 || which > 1) return fail(GJW_ERR_ARG, "gjw_kernel_stats: bad argument");
+                                int64_t* launches, int32_t reset) {
+  if (!s || which < 0 || which > 2) return fail(GJW_ERR_ARG, "gjw_kernel_stats: bad argument");
   TRY(kev_resolve(s));
   if (avg_ms) *avg_ms = s->kt_n[which] ? s->kt_ms[which] / s->kt_n[which] : 0.0;
   if (count) *count = s->kt_n[which];
+  if (launches) *launches = s->kl_n[which];
   if (reset) {
     s->kt_ms[which] = 0;
     s->kt_n[which] = 0;
+    s->kl_n[which] = 0;
   }
   return GJW_OK;
 }

@@ -19,8 +19,34 @@
 
 namespace gjw {
 
-__device__ __forceinline__ float shfl_dn1(float v) { return __shfl_down(v, 1, 64); }
-__device__ __forceinline__ float shfl_up1(float v) { return __shfl_up(v, 1, 64); }
+// Neighbour-lane reads through DPP wave shifts (a VALU modifier, no LDS
+// round trip): shfl_dn1 gives lane l the value of lane l+1 (wave_shl:1),
+// shfl_up1 the value of lane l-1 (wave_shr:1).  The lane without a source
+// (63, resp. 0) keeps its own value; callers overwrite it.
+__device__ __forceinline__ float shfl_dn1(float v) {
+  const int x = __float_as_int(v);
+  return __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float shfl_up1(float v) {
+  const int x = __float_as_int(v);
+  return __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x138, 0xF, 0xF, false));
+}
+
+// Raw buffer stores: a lane that has nothing to write passes an offset past
+// the end of the buffer, which the hardware drops.  The store stays a single
+// unconditional instruction, so the compiler can count the vector-memory
+// operations in flight exactly (a store under a branch makes it wait for all
+// of them, including loads issued for the next chunk).
+constexpr unsigned kOOB = 0x7ffffff0u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_of(float* p, long long nfloats) {
+  return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)(nfloats * 4), 0x00020000);
+}
+__device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsigned off_floats,
+                                       bool on) {
+  // the builtin takes the 32 data bits as an integer: pass the bits, not a converted value
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(on ? off_floats * 4u : kOOB),
+                                        0, 0);
+}
 
 // --------------------------------------------------------------------------
 // per-cell coefficient selection (w_solve.m:144-147 inner, :170-186 strips)
@@ -78,11 +104,11 @@ struct TileId {
   int rowtile, strip, shot;
 };
 
-__device__ __forceinline__ TileId tile_of_block(int NX, int NY, int L) {
+__device__ __forceinline__ TileId tile_of_block(int NX, int NY, int L, int rows = 64) {
   const unsigned B = gridDim.x, b = blockIdx.x;
   const unsigned per = B / 8u;
   const unsigned t = (b < per * 8u) ? (b % 8u) * per + b / 8u : b;
-  const unsigned gx = (unsigned)((NX + 1 + 63) / 64);
+  const unsigned gx = (unsigned)((NX + 1 + rows - 1) / rows);
   const unsigned gy = (unsigned)(((NY + 1 + L - 1) / L + 3) / 4);
   TileId id;
   id.rowtile = (int)(t % gx);
@@ -149,7 +175,9 @@ k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
   const TileId tid = tile_of_block(NX, NY, L);
   const int shot = tid.shot;
   const int i0 = tid.rowtile * 64;
-  const int jb = (tid.strip * blockDim.y + threadIdx.y) * L;
+  // threadIdx.y is constant across a 64-lane wavefront: make the strip start
+  // (and everything derived from it) scalar so loop control stays uniform
+  const int jb = __builtin_amdgcn_readfirstlane((tid.strip * blockDim.y + threadIdx.y) * L);
   const long long LD = g.LD;
   if (jb > NY) return;
   const int je = min(jb + L, NY + 1);
@@ -241,6 +269,377 @@ k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
 }
 
 // ==========================================================================
+// forward: two fused steps per pass (temporal blocking)
+// ==========================================================================
+// One launch advances every shot from step n to n+2.  A wavefront's 64 lanes
+// hold rows r0..r0+63 with r0 = 60*t - 2: lanes 2..61 own their rows, lanes
+// 0-1 and 62-63 recompute the neighbours' rows that the second step needs,
+// so no lane loads a halo of its own.  Along the march, step 2 runs one
+// column behind step 1 (column j for step 1, j-1 for step 2), which is all
+// the second step needs in that direction.  Each field value is read and
+// written once per two steps.  Split-field aux X/Y ping-pong (X0 -> X2) so
+// that halo lanes can read the start-of-pass values of cells other waves own.
+constexpr int ROWS2 = 60;
+
+// One Yee step of lane row i at column j with strip-aware coefficients.
+// e_c, e_n: E_old(i, j), E_old(i, j+1); hx, hy: H_old(i, j); hxp: Hx_new(i, j-1);
+// x, y: split-field aux of the cell (old in, new out, PML cells only).
+__device__ __forceinline__ void yee_general(const DevGrid& g, int i, int j, bool okrow, float e_c,
+                                            float e_n, float hx, float hy, float hxp, float& x,
+                                            float& y, float& hxn, float& hyn, float& en) {
+  const int NX = g.NX, NY = g.NY, P = g.P;
+  const float e_dn = shfl_dn1(e_c);
+  const bool ok = okrow && j >= 0 && j <= NY;
+  hyn = (ok && i < NX) ? hy_new(g, i, j, hy, e_c, e_dn) : 0.f;
+  hxn = (ok && j < NY) ? hx_new(g, i, j, hx, e_c, e_n) : 0.f;
+  const float hy_up = shfl_up1(hyn);
+  en = 0.f;
+  if (ok) {
+    const int kd = ekind(NX, NY, P, i, j);
+    const float dhy = hyn - hy_up, dhx = hxn - hxp;
+    if (kd == KIND_INNER) {
+      const float2 c = g.cc[(long long)j * g.LD + i];
+      en = c.x * e_c + c.y * (dhy - dhx);
+    } else if (kd == KIND_PML) {
+      const long long k = (long long)j * g.LD + i;
+      x = g.cxe[k] * x + g.cxh[k] * dhy;
+      y = g.cye[k] * y + g.cyh[k] * dhx;
+      en = x + y;
+    }
+  }
+}
+
+// Masked raw buffer load: a lane with on == false reads past the end of the
+// buffer and gets 0 without touching memory (no branch, see bstore).
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned off_floats, bool on) {
+  return __uint_as_float(
+      __builtin_amdgcn_raw_buffer_load_b32(r, (int)(on ? off_floats * 4u : kOOB), 0, 0));
+}
+
+// FRAMES: the recompute sweep stores Ez on the domain after both steps.
+//
+// Column handling.  Step-1 columns jb-1 .. je are computed (step 2 runs one
+// column behind; the first two produce only what the strip's first output
+// column needs).  Interior columns (both steps' columns inside
+// [P+1, NY-P-1]) go through U-column chunks whose memory operations are all
+// unconditional raw buffer loads/stores with per-lane masks, so the compiler
+// waits only for what each chunk needs:
+//   * waves whose 64 rows are all inner rows take the scalar-coefficient path;
+//   * the other waves (top and bottom PML rows) take a row-class path: every
+//     lane knows whether its row is inner, in the top strip (coefficients
+//     depend on the row only and sit in registers) or in the bottom strip
+//     (coefficients depend on eps(i, j) and are loaded per cell).
+// The left/right PML columns and the grid's edge columns use yee_general.
+template <bool FRAMES>
+__global__ void __launch_bounds__(256)
+k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
+            const float* __restrict__ Hx0, const float* __restrict__ Hy0,
+            const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ E2,
+            float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
+            float* __restrict__ Y2, int it, float* __restrict__ tr, float* __restrict__ frame1,
+            float* __restrict__ frame2, long long fstride, float* __restrict__ trash) {
+  const int lane = threadIdx.x;
+  const int NX = g.NX, NY = g.NY, P = g.P;
+  const long long LD = g.LD;
+  const TileId tid = tile_of_block(NX, NY, L, ROWS2);
+  const int shot = tid.shot;
+  const int r0 = tid.rowtile * ROWS2 - 2;
+  // threadIdx.y is constant across a 64-lane wavefront: make the strip start
+  // (and everything derived from it) scalar so loop control stays uniform
+  const int jb = __builtin_amdgcn_readfirstlane((tid.strip * blockDim.y + threadIdx.y) * L);
+  if (jb > NY) return;
+  const int je = min(jb + L, NY + 1);
+  const int i = r0 + lane;
+  const bool okrow = i >= 0 && i <= NX;
+  const bool own = okrow && lane >= 2 && lane <= 61;
+  const long long so = (long long)shot * g.FS;
+  E0 += so; Hx0 += so; Hy0 += so; X0 += so; Y0 += so;
+  E2 += so; Hx2 += so; Hy2 += so; X2 += so; Y2 += so;
+  const int si = s.si[shot], sj = s.sj[shot];
+  const float ws1 = s.sc[shot] * s.wav[(long long)shot * s.nt + it];
+  const float ws2 = s.sc[shot] * s.wav[(long long)shot * s.nt + it + 1];
+  const int* cp = s.cptr + (long long)shot * (NY + 2);
+  float* fr1 = FRAMES ? frame1 + (long long)shot * fstride : nullptr;
+  float* fr2 = FRAMES ? frame2 + (long long)shot * fstride : nullptr;
+  const bool has_rec = tr && r0 + 2 <= s.rmax && r0 + 61 >= s.rmin;
+  const bool inner_rows = (r0 >= P + 1) && (r0 + 63 <= NX - P - 1);
+  const bool dom_row = i - g.di0 >= 0 && i - g.di0 < g.m;
+  auto dom_col = [&](int c) { return c - g.dj0 >= 0 && c - g.dj0 < g.n; };
+  const float bx0 = g.bx0, by0 = g.by0;
+  const long long ic = okrow ? i : 0;  // address row for loads of valid lanes
+  const long long FD = (long long)LD * g.n;
+
+  const __amdgpu_buffer_rsrc_t rE0 = rsrc_of(const_cast<float*>(E0), g.FS),
+                               rHx0 = rsrc_of(const_cast<float*>(Hx0), g.FS),
+                               rHy0 = rsrc_of(const_cast<float*>(Hy0), g.FS),
+                               rX0 = rsrc_of(const_cast<float*>(X0), g.FS),
+                               rY0 = rsrc_of(const_cast<float*>(Y0), g.FS),
+                               rE2 = rsrc_of(E2, g.FS), rHx2 = rsrc_of(Hx2, g.FS),
+                               rHy2 = rsrc_of(Hy2, g.FS), rX2 = rsrc_of(X2, g.FS),
+                               rY2 = rsrc_of(Y2, g.FS),
+                               rCC = rsrc_of(const_cast<float*>(reinterpret_cast<const float*>(g.cc)), 2 * g.FS),
+                               rCXE = rsrc_of(const_cast<float*>(g.cxe), g.FS),
+                               rCXH = rsrc_of(const_cast<float*>(g.cxh), g.FS),
+                               rCYH = rsrc_of(const_cast<float*>(g.cyh), g.FS),
+                               rHAY = rsrc_of(const_cast<float*>(g.hay), g.FS),
+                               rHBY = rsrc_of(const_cast<float*>(g.hby), g.FS);
+
+  // row classes (valid in interior columns)
+  const bool e_inner = okrow && i >= P + 1 && i <= NX - P - 1;
+  const bool e_xn = okrow && i >= 1 && i <= P;
+  const bool e_xp = okrow && i >= NX - P && i <= NX - 1;
+  const bool e_pml = e_xn || e_xp;
+  const bool h_inner = okrow && i >= P && i <= NX - P - 1;
+  const bool h_xn = okrow && i >= 0 && i <= P - 1;
+  const bool h_xp = okrow && i >= NX - P && i <= NX - 1;
+  const bool h_any = h_inner || h_xn || h_xp;  // Hy exists (i <= NX-1)
+  // top-strip coefficients depend on the row only: read them once (column P+1)
+  float hay_r = 1.f, hby_r = 0.f, cxe_r = 1.f, cxh_r = 0.f, cyh_r = 0.f;
+  if (!inner_rows) {
+    const unsigned kr = (unsigned)((P + 1) * LD + ic);
+    hay_r = h_xn ? bload(rHAY, kr, true) : 1.f;
+    hby_r = h_xn ? bload(rHBY, kr, true) : 0.f;
+    cxe_r = e_xn ? bload(rCXE, kr, true) : 1.f;
+    cxh_r = e_xn ? bload(rCXH, kr, true) : 0.f;
+    cyh_r = e_xn ? bload(rCYH, kr, true) : 0.f;
+  }
+  // Hy coefficients of rows whose Hy update does not depend on the column
+  const float hay_row = h_xn ? hay_r : 1.f;
+  const float hby_row = h_xn ? hby_r : (h_inner ? by0 : 0.f);
+
+  auto ldE = [&](int j) { return (okrow && j >= 0 && j <= NY) ? E0[(long long)j * LD + ic] : 0.f; };
+  auto ldHx = [&](int j) { return (okrow && j >= 0 && j < NY) ? Hx0[(long long)j * LD + ic] : 0.f; };
+  auto ldHy = [&](int j) { return (okrow && i < NX && j >= 0 && j <= NY) ? Hy0[(long long)j * LD + ic] : 0.f; };
+  auto is_pml = [&](int j) { return okrow && j >= 0 && j <= NY && ekind(NX, NY, P, i, j) == KIND_PML; };
+  auto record = [&](int j, float e, int step, float* fr) {
+    if (has_rec) {
+      const int qa = cp[j], qb = cp[j + 1];
+      for (int q = qa; q < qb; ++q)
+        if (s.crow[q] == i) tr[(long long)s.cid[q] * s.nt + step] = e;
+    }
+    if (fr) {
+      const int a = i - g.di0, b = j - g.dj0;
+      if (a >= 0 && a < g.m && b >= 0 && b < g.n) fr[(long long)b * LD + i] = e;
+    }
+  };
+
+  // ---- prologue: step 1 starts at column jb-1; it needs Hx^{n+1}(jb-2)
+  const int j1 = jb - 1;
+  float e0_c = ldE(j1);  // E^n(i, j)
+  float hx1_prev = 0.f;  // Hx^{n+1}(i, j-1)
+  if (j1 - 1 >= 0 && okrow) hx1_prev = hx_new(g, i, j1 - 1, ldHx(j1 - 1), ldE(j1 - 1), e0_c);
+  // step-2 carries (column j-1 of step 1): E, Hx, Hy, X, Y at n+1; Hx^{n+2}(j-2);
+  // and the column's coefficients
+  float e1_prev = 0.f, hx1_c2 = 0.f, hy1_c2 = 0.f, x1_c2 = 0.f, y1_c2 = 0.f, hx2_prev = 0.f;
+  float2 cc_c2 = make_float2(0.f, 0.f);
+  float cxe_c2 = 1.f, cxh_c2 = 0.f, cyh_c2 = 0.f, hay_c2 = 1.f, hby_c2 = 0.f;
+
+  // both steps' columns of a chunk at jj are interior; jj <= je keeps it useful
+  auto chunkable = [&](int jj) {
+    return jj - 1 >= P + 1 && jj + U - 1 <= NY - P - 1 && jj <= je;
+  };
+
+  // ---- one general column (strips, edges, grid boundary)
+  auto general_col = [&](int j) {
+    const bool jin = j >= 0 && j <= NY;
+    const float e0_n = ldE(j + 1);
+    float x1 = 0.f, y1 = 0.f;
+    if (is_pml(j)) {
+      x1 = X0[(long long)j * LD + ic];
+      y1 = Y0[(long long)j * LD + ic];
+    }
+    float hx1, hy1, e1;
+    yee_general(g, i, j, okrow, e0_c, e0_n, ldHx(j), ldHy(j), hx1_prev, x1, y1, hx1, hy1, e1);
+    if (jin && j == sj && i == si) e1 += ws1;
+    if (own && jin && j >= jb && j < je) record(j, e1, it, fr1);
+    const int c2 = j - 1;
+    float hx2 = 0.f;
+    if (c2 >= jb - 1) {
+      // Hx^{n+2}(c2) always (the next column's step 2 needs it); the rest only for owned columns
+      float x2 = x1_c2, y2 = y1_c2, hy2, e2;
+      yee_general(g, i, c2, okrow, e1_prev, e1, hx1_c2, hy1_c2, hx2_prev, x2, y2, hx2, hy2, e2);
+      if (c2 >= 0 && c2 <= NY && c2 == sj && i == si) e2 += ws2;
+      if (own && c2 >= jb && c2 < je) {
+        const long long k2 = (long long)c2 * LD + i;
+        E2[k2] = e2;
+        if (c2 < NY) Hx2[k2] = hx2;
+        if (i < NX) Hy2[k2] = hy2;
+        if (is_pml(c2)) {
+          X2[k2] = x2;
+          Y2[k2] = y2;
+        }
+        record(c2, e2, it + 1, fr2);
+      }
+    }
+    e0_c = e0_n;
+    hx1_prev = hx1;
+    e1_prev = e1;
+    hx1_c2 = hx1;
+    hy1_c2 = hy1;
+    x1_c2 = x1;
+    y1_c2 = y1;
+    hx2_prev = hx2;
+    const long long kj = (long long)(jin ? j : 0) * LD + ic;
+    cc_c2 = jin ? g.cc[kj] : make_float2(0.f, 0.f);
+    const bool xp_cell = jin && j >= 1 && j <= NY - 1;
+    cxe_c2 = (e_xp && xp_cell) ? g.cxe[kj] : cxe_r;
+    cxh_c2 = (e_xp && xp_cell) ? g.cxh[kj] : cxh_r;
+    cyh_c2 = (e_xp && xp_cell) ? g.cyh[kj] : cyh_r;
+    hay_c2 = (h_xp && xp_cell) ? g.hay[kj] : hay_row;
+    hby_c2 = (h_xp && xp_cell) ? g.hby[kj] : hby_row;
+  };
+
+  // ---- interior chunk, all rows inner: scalar H coefficients, (Ceze, Cezhy) loads
+  auto inner_chunk = [&](int jj) {
+    float en_[U], hx_[U], hy_[U];
+    float2 c_[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned kk = (unsigned)((jj + u) * LD + i);
+      en_[u] = bload(rE0, kk + (unsigned)LD, true);
+      hx_[u] = bload(rHx0, kk, true);
+      hy_[u] = bload(rHy0, kk, true);
+      c_[u] = make_float2(bload(rCC, 2 * kk, true), bload(rCC, 2 * kk + 1, true));
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c1 = jj + u, c2 = c1 - 1;
+      // step 1 at c1
+      const float e_dn = shfl_dn1(e0_c);
+      const float hy1 = hy_[u] + by0 * (e_dn - e0_c);
+      const float hx1 = hx_[u] + bx0 * (en_[u] - e0_c);
+      const float hy1u = shfl_up1(hy1);
+      float e1 = c_[u].x * e0_c + c_[u].y * ((hy1 - hy1u) - (hx1 - hx1_prev));
+      e1 += (c1 == sj && i == si) ? ws1 : 0.f;
+      // step 2 at c2
+      const float e_dn2 = shfl_dn1(e1_prev);
+      const float hy2 = hy1_c2 + by0 * (e_dn2 - e1_prev);
+      const float hx2 = hx1_c2 + bx0 * (e1 - e1_prev);
+      const float hy2u = shfl_up1(hy2);
+      float e2 = cc_c2.x * e1_prev + cc_c2.y * ((hy2 - hy2u) - (hx2 - hx2_prev));
+      e2 += (c2 == sj && i == si) ? ws2 : 0.f;
+      const bool o2 = own && c2 >= jb && c2 < je;
+      const unsigned k2 = (unsigned)(c2 * LD + i);
+      bstore(e2, rE2, k2, o2);
+      bstore(hx2, rHx2, k2, o2);
+      bstore(hy2, rHy2, k2, o2);
+      if (FRAMES) {
+        bstore(e1, rsrc_of(fr1, FD), (unsigned)((c1 - g.dj0) * LD + i),
+               own && dom_row && dom_col(c1) && c1 >= jb && c1 < je);
+        bstore(e2, rsrc_of(fr2, FD), (unsigned)((c2 - g.dj0) * LD + i), o2 && dom_row && dom_col(c2));
+      }
+      e0_c = en_[u];
+      hx1_prev = hx1;
+      e1_prev = e1;
+      hx1_c2 = hx1;
+      hy1_c2 = hy1;
+      hx2_prev = hx2;
+      cc_c2 = c_[u];
+    }
+  };
+
+  // ---- interior chunk, rows of every class (top / bottom PML rows present)
+  auto class_chunk = [&](int jj) {
+    float en_[U], hx_[U], hy_[U], x_[U], y_[U], cxe_[U], cxh_[U], cyh_[U], hay_[U], hby_[U];
+    float2 c_[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned kk = (unsigned)((jj + u) * LD + ic);
+      en_[u] = bload(rE0, kk + (unsigned)LD, okrow);
+      hx_[u] = bload(rHx0, kk, okrow);
+      hy_[u] = bload(rHy0, kk, h_any);
+      c_[u] = make_float2(bload(rCC, 2 * kk, e_inner), bload(rCC, 2 * kk + 1, e_inner));
+      x_[u] = bload(rX0, kk, e_pml);
+      y_[u] = bload(rY0, kk, e_pml);
+      cxe_[u] = e_xp ? bload(rCXE, kk, true) : cxe_r;
+      cxh_[u] = e_xp ? bload(rCXH, kk, true) : cxh_r;
+      cyh_[u] = e_xp ? bload(rCYH, kk, true) : cyh_r;
+      hay_[u] = h_xp ? bload(rHAY, kk, true) : hay_row;
+      hby_[u] = h_xp ? bload(rHBY, kk, true) : hby_row;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c1 = jj + u, c2 = c1 - 1;
+      // step 1 at c1
+      const float e_dn = shfl_dn1(e0_c);
+      const float hy1 = h_any ? hay_[u] * hy_[u] + hby_[u] * (e_dn - e0_c) : 0.f;
+      const float hx1 = okrow ? hx_[u] + bx0 * (en_[u] - e0_c) : 0.f;
+      const float hy1u = shfl_up1(hy1);
+      const float dhy1 = hy1 - hy1u, dhx1 = hx1 - hx1_prev;
+      const float x1 = cxe_[u] * x_[u] + cxh_[u] * dhy1;
+      const float y1 = y_[u] + cyh_[u] * dhx1;
+      // ring rows (0, NX) are never updated: they keep their value
+      float e1 = e_inner ? c_[u].x * e0_c + c_[u].y * (dhy1 - dhx1) : (e_pml ? x1 + y1 : e0_c);
+      e1 += (c1 == sj && i == si) ? ws1 : 0.f;
+      // step 2 at c2
+      const float e_dn2 = shfl_dn1(e1_prev);
+      const float hy2 = h_any ? hay_c2 * hy1_c2 + hby_c2 * (e_dn2 - e1_prev) : 0.f;
+      const float hx2 = okrow ? hx1_c2 + bx0 * (e1 - e1_prev) : 0.f;
+      const float hy2u = shfl_up1(hy2);
+      const float dhy2 = hy2 - hy2u, dhx2 = hx2 - hx2_prev;
+      const float x2 = cxe_c2 * x1_c2 + cxh_c2 * dhy2;
+      const float y2 = y1_c2 + cyh_c2 * dhx2;
+      float e2 = e_inner ? cc_c2.x * e1_prev + cc_c2.y * (dhy2 - dhx2) : (e_pml ? x2 + y2 : e1_prev);
+      e2 += (c2 == sj && i == si) ? ws2 : 0.f;
+      const bool o2 = own && c2 >= jb && c2 < je;
+      const unsigned k2 = (unsigned)(c2 * LD + ic);
+      bstore(e2, rE2, k2, o2);
+      bstore(hx2, rHx2, k2, o2);
+      bstore(hy2, rHy2, k2, o2 && h_any);
+      bstore(x2, rX2, k2, o2 && e_pml);
+      bstore(y2, rY2, k2, o2 && e_pml);
+      if (has_rec) {
+        if (own && c1 >= jb && c1 < je) record(c1, e1, it, nullptr);
+        if (o2) record(c2, e2, it + 1, nullptr);
+      }
+      if (FRAMES) {
+        bstore(e1, rsrc_of(fr1, FD), (unsigned)((c1 - g.dj0) * LD + ic),
+               own && dom_row && dom_col(c1) && c1 >= jb && c1 < je);
+        bstore(e2, rsrc_of(fr2, FD), (unsigned)((c2 - g.dj0) * LD + ic), o2 && dom_row && dom_col(c2));
+      }
+      e0_c = en_[u];
+      hx1_prev = hx1;
+      e1_prev = e1;
+      hx1_c2 = hx1;
+      hy1_c2 = hy1;
+      x1_c2 = x1;
+      y1_c2 = y1;
+      hx2_prev = hx2;
+      cc_c2 = c_[u];
+      cxe_c2 = cxe_[u];
+      cxh_c2 = cxh_[u];
+      cyh_c2 = cyh_[u];
+      hay_c2 = hay_[u];
+      hby_c2 = hby_[u];
+    }
+  };
+
+  int j = j1;  // step 1 at column j, step 2 at column j-1
+  const bool use_inner = inner_rows && !has_rec;
+  while (j <= je) {
+    if (chunkable(j)) {
+      if (use_inner)
+        inner_chunk(j);
+      else
+        class_chunk(j);
+      j += U;
+    } else {
+      general_col(j);
+      ++j;
+    }
+  }
+}
+
+template __global__ void k_fwd_step2<false>(DevGrid, DevShots, int, const float*, const float*,
+                                            const float*, const float*, const float*, float*,
+                                            float*, float*, float*, float*, int, float*, float*,
+                                            float*, long long, float*);
+template __global__ void k_fwd_step2<true>(DevGrid, DevShots, int, const float*, const float*,
+                                           const float*, const float*, const float*, float*,
+                                           float*, float*, float*, float*, int, float*, float*,
+                                           float*, long long, float*);
+
+// ==========================================================================
 // adjoint: one step of the transpose of k_fwd_step
 // ==========================================================================
 // Adjoint state per shot: M (adjoint of Ez), Nx/Ny (adjoint of the updated
@@ -281,7 +680,9 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
   const TileId tid = tile_of_block(NX, NY, L);
   const int shot = tid.shot;
   const int i0 = tid.rowtile * 64;
-  const int jb = (tid.strip * blockDim.y + threadIdx.y) * L;
+  // threadIdx.y is constant across a 64-lane wavefront: make the strip start
+  // (and everything derived from it) scalar so loop control stays uniform
+  const int jb = __builtin_amdgcn_readfirstlane((tid.strip * blockDim.y + threadIdx.y) * L);
   const long long LD = g.LD;
   if (jb > NY) return;
   const int je = min(jb + L, NY + 1);

@@ -257,11 +257,12 @@ class Survey:
         return g_eps, g_sig, J.value
 
     def kernel_stats(self, which, reset=False):
-        """(average ms, count) of the sampled forward (0) / adjoint (1) step kernels."""
-        ms, n = C.c_double(0), C.c_int64(0)
-        N.check(N.load().gjw_kernel_stats(self.h, int(which), C.byref(ms), C.byref(n), int(reset)),
-                "gjw_kernel_stats")
-        return ms.value, n.value
+        """(average ms, samples, launches) of the one-step forward (0), adjoint
+        (1) and two-step forward (2) kernels while GJW_OPT_KTIMING is on."""
+        ms, n, nl = C.c_double(0), C.c_int64(0), C.c_int64(0)
+        N.check(N.load().gjw_kernel_stats(self.h, int(which), C.byref(ms), C.byref(n), C.byref(nl),
+                                          int(reset)), "gjw_kernel_stats")
+        return ms.value, n.value, nl.value
 
     def tile(self):
         L = C.c_int32(0)

@@ -91,6 +91,7 @@ int gjw_set_shots(gjw_survey *s, int32_t ns, const int32_t *src, const float *wa
 #define GJW_OPT_TILE_L 3     /* columns marched per wavefront; 0 = autotuned        */
 #define GJW_OPT_SNAPSHOT 4   /* GJW_SNAP_F32 (default) | GJW_SNAP_BF16 | GJW_SNAP_F16 */
 #define GJW_OPT_KTIMING 5    /* n > 0: time every n-th step kernel with HIP events  */
+#define GJW_OPT_FUSE 6       /* time steps per forward launch on the batched path: 2 (default) or 1 */
 #define GJW_PATH_BATCHED 0   /* all shots per launch, checkpoint + recompute        */
 #define GJW_PATH_PLAIN 1     /* one shot at a time, full wavefield stored (as w_solve.m) */
 #define GJW_SNAP_F32 0
@@ -141,10 +142,12 @@ int gjw_differentiate_cube(gjw_ctx *ctx, float *v, int32_t nz, int32_t nx, int32
  * call and the number of cell-updates it executed. */
 int gjw_last_timing(gjw_survey *s, double *ms, double *cell_updates);
 
-/* Average device duration (ms) and count of the step kernels timed so far
- * under GJW_OPT_KTIMING.  which: 0 = forward step, 1 = adjoint step.  Has no
+/* Average device duration (ms) and count of the sampled step kernels, and the
+ * number of launches of that kernel, while GJW_OPT_KTIMING is on.  which: 0 = one-step forward, 1 = adjoint step,
+ * 2 = two-step forward.  Has no
  * counterpart in the reference (measurement only). */
-int gjw_kernel_stats(gjw_survey *s, int32_t which, double *avg_ms, int64_t *count, int32_t reset);
+int gjw_kernel_stats(gjw_survey *s, int32_t which, double *avg_ms, int64_t *count,
+                     int64_t *launches, int32_t reset);
 
 /* Columns per wavefront chosen by the autotuner (0 before the first forward). */
 int gjw_tile(gjw_survey *s, int32_t *L);

@@ -249,3 +249,60 @@ def test_autograd_function(oracle_mod, gpu_ctx):
         fd = (vals[0] - vals[1]) / (2 * h)
         print("fd", (a, b), fd, ge[a, b])
         assert abs(fd - ge[a, b]) <= 0.05 * abs(ge[a, b])
+
+
+@pytest.mark.parametrize("ck", [7, 10])
+def test_two_step_kernel_matches_one_step(gpu_ctx, ck):
+    """The temporally blocked forward (two steps per launch, overlapped row
+    tiles) equals the one-step kernel: traces and gradient, odd nt, odd and
+    even checkpoint intervals, several tile widths."""
+    from gerjoii_amd import _native as N
+    case = configs.small(m=130, n=150, P=12, nt=241, ns=2)
+    rng = np.random.default_rng(4)
+    ref = make_survey(case, gpu_ctx, ck=ck)
+    ref.set_option(N.GJW_OPT_FUSE, 1)
+    d1 = ref.forward(keep=True)
+    adj = rng.standard_normal(d1.shape).astype(np.float32)
+    g1 = ref.gradient(adj)
+    for L in (5, 16, 64):
+        sv = make_survey(case, gpu_ctx, ck=ck, L=L)
+        sv.set_option(N.GJW_OPT_FUSE, 2)
+        d2 = sv.forward(keep=True)
+        g2 = sv.gradient(adj)
+        print("L", L, rel_l2(d2, d1), rel_l2(g2[0], g1[0]), rel_l2(g2[1], g1[1]))
+        assert rel_l2(d2, d1) <= 1e-6
+        assert rel_l2(g2[0], g1[0]) <= 1e-5 and rel_l2(g2[1], g1[1]) <= 1e-5
+
+
+def _borehole_case(nt=420):
+    """Deep source and a vertical receiver array through the whole model depth,
+    so that every row tile's interior update reaches a receiver within nt."""
+    case = configs.small(m=170, n=130, P=12, nt=nt, ns=2)
+    g = case.grid
+    src = np.array([S.src_index(g, 60, 40), S.src_index(g, 110, 90)], dtype=np.int32)
+    rz = np.arange(2, g.m - 2, 4)
+    rec = S.rec_Mw(g, rz, np.full(rz.size, 70))
+    case.src = src
+    case.receivers = [rec.copy(), rec.copy()]
+    return case
+
+
+def test_borehole_forward_and_gradient(oracle_mod, gpu_ctx):
+    """Interior (fast-path) tiles against the FP64 oracle, one- and two-step."""
+    from gerjoii_amd import _native as N
+    case = _borehole_case()
+    ref = oracle_forward(oracle_mod, case, case.eps0, case.sig0)
+    rng = np.random.default_rng(6)
+    adj = rng.standard_normal(ref.shape).astype(np.float32)
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in case.receivers])])
+    fe, fs, _ = oracle_grad(oracle_mod, case, case.eps0, case.sig0, adj, rp)
+    for fuse in (1, 2):
+        for L in (8, 32):
+            sv = make_survey(case, gpu_ctx, ck=13, L=L)
+            sv.set_option(N.GJW_OPT_FUSE, fuse)
+            d = sv.forward(keep=True)
+            ge, gs = sv.gradient(adj)
+            print("fuse", fuse, "L", L, "traces", rel_l2(d, ref), "eps", rel_l2(ge, fe), "sig", rel_l2(gs, fs))
+            assert rel_l2(d, ref) <= 1e-4
+            assert rel_l2(ge, fe) <= 1e-3 and cosine(ge, fe) >= 0.9999
+            assert rel_l2(gs, fs) <= 1e-3
