@@ -100,7 +100,7 @@ def main():
         sv.fwi_gradient(d_obs, g_eps, g_sig)
         log(f"warmup {w + 1}/{args.warmup} done")
     sv.set_option(N.GJW_OPT_KTIMING, 16)
-    for kk in range(3):
+    for kk in range(4):
         sv.kernel_stats(kk, reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -114,6 +114,7 @@ def main():
     fwd1_ms, fwd1_n, fwd1_l = sv.kernel_stats(0)
     adj_ms, adj_n, adj_l = sv.kernel_stats(1)
     fwd2_ms, fwd2_n, fwd2_l = sv.kernel_stats(2)
+    adj2_ms, adj2_n, adj2_l = sv.kernel_stats(3)
     ms_step = dt_s * 1e3 / args.steps
     shot_grads = ns * args.steps / dt_s
     cell_updates = 3.0 * cells * g.nt * ns * args.steps  # forward, recompute, adjoint
@@ -126,8 +127,8 @@ def main():
 
     # per-kernel roofline: algorithmic bytes per launch / live average duration
     bytes_launch = BYTES_PER_CELL_UPDATE * cells * ns
-    # launches per gradient: the forward and recompute sweeps run two time steps
-    # per k_fwd_step2 launch (one-step k_fwd_step only for odd segment tails)
+    # launches per gradient: every sweep runs two time steps per launch
+    # (k_fwd_step2, k_adj_step2); the one-step kernels only take odd segment tails
     kern = {
         "k_fwd_step2": {"avg_ms": fwd2_ms, "samples": fwd2_n, "steps_per_launch": 2,
                         "launches_per_step": fwd2_l / args.steps},
@@ -135,6 +136,8 @@ def main():
                        "launches_per_step": fwd1_l / args.steps},
         "k_adj_step": {"avg_ms": adj_ms, "samples": adj_n, "steps_per_launch": 1,
                        "launches_per_step": adj_l / args.steps},
+        "k_adj_step2": {"avg_ms": adj2_ms, "samples": adj2_n, "steps_per_launch": 2,
+                        "launches_per_step": adj2_l / args.steps},
     }
     for k, v in kern.items():
         spl = v["steps_per_launch"]

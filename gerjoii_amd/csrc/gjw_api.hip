@@ -31,6 +31,10 @@ __global__ void k_adj_step(DevGrid, DevShots, int, const float*, const float*, c
                            const float*, const float*, float*, float*, float*, float*, float*,
                            int, const float*, const float*, const float*, long long, float*,
                            long long, float*);
+__global__ void k_adj_step2(DevGrid, DevShots, int, const float*, const float*, const float*,
+                            const float*, const float*, float*, float*, float*, float*, float*,
+                            int, const float*, const float*, const float*, const float*,
+                            long long, float*, long long, float*);
 template <bool FRAMES>
 __global__ void k_fwd_step2(DevGrid, DevShots, int, const float*, const float*, const float*,
                             const float*, const float*, float*, float*, float*, float*, float*, int,
@@ -168,9 +172,9 @@ struct gjw_survey {
   int ksample = 0;
   std::vector<hipEvent_t> kev_pool;
   std::vector<std::pair<int, int>> kev_used;  // (kernel id, index of start event)
-  double kt_ms[3] = {0, 0, 0};  // k_fwd_step, k_adj_step, k_fwd_step2
-  long long kt_n[3] = {0, 0, 0};
-  long long kl_n[3] = {0, 0, 0};  // all launches while timing is on
+  double kt_ms[4] = {0, 0, 0, 0};  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2
+  long long kt_n[4] = {0, 0, 0, 0};
+  long long kl_n[4] = {0, 0, 0, 0};  // all launches while timing is on
 };
 
 static inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
@@ -640,7 +644,10 @@ static int kev_begin(gjw_survey* s, int which, int it, int* slot) {
   *slot = -1;
   if (s->ksample <= 0) return GJW_OK;
   s->kl_n[which] += 1;
-  if (it % s->ksample > 1 || (it % s->ksample == 1 && which != 2)) return GJW_OK;
+  // two-step kernels advance `it` by 2: accept either residue so that one
+  // launch in every ksample steps is timed
+  const bool two = which == 2 || which == 3;
+  if (it % s->ksample > 1 || (it % s->ksample == 1 && !two)) return GJW_OK;
   const size_t need = 2 * (s->kev_used.size() + 1);
   while (s->kev_pool.size() < need) {
     hipEvent_t e;
@@ -739,6 +746,21 @@ static int adj_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int n
                      b.M[cur], b.Nx[cur], b.Ny[cur], b.X[cur], b.Y[cur], b.M[cur ^ 1],
                      b.Nx[cur ^ 1], b.Ny[cur ^ 1], b.X[cur ^ 1], b.Y[cur ^ 1], it, adj, F1, F0,
                      fstride, acc, astride, gw);
+  HIPCHK(hipGetLastError());
+  return kev_end(s, slot);
+}
+
+// transposes forward steps it and it-1 (it >= 1) in one launch
+static int adj2_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, int L,
+                       AdjBufs& b, int cur, int it, const float* adj, const float* Fa,
+                       const float* Fb, const float* Fc, long long fstride, float* acc,
+                       long long astride, float* gw) {
+  int slot;
+  TRY(kev_begin(s, 3, it, &slot));
+  hipLaunchKernelGGL(k_adj_step2, step2_grid(s, L, nsh), dim3(64, 4), 0, s->ctx->stream, g, sh,
+                     L, b.M[cur], b.Nx[cur], b.Ny[cur], b.X[cur], b.Y[cur], b.M[cur ^ 1],
+                     b.Nx[cur ^ 1], b.Ny[cur ^ 1], b.X[cur ^ 1], b.Y[cur ^ 1], it, adj, Fa, Fb,
+                     Fc, fstride, acc, astride, gw);
   HIPCHK(hipGetLastError());
   return kev_end(s, slot);
 }
@@ -1031,10 +1053,17 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
                       s->frames.p + (long long)(it + 1 - k0) * FD, FD, fstride, &taken));
       it += taken;
     }
-    for (int it = k1 - 1; it >= k0; --it) {
-      TRY(adj_launch(s, g, sh, ns, s->tuned_L1, ab, acur, it, adj,
-                     s->frames.p + (long long)(it + 1 - k0) * FD,
-                     s->frames.p + (long long)(it - k0) * FD, fstride, s->acc.p, FD, gw));
+    for (int it = k1 - 1; it >= k0;) {
+      const float* F = s->frames.p + (long long)(it - k0) * FD;  // Ez(S_it)
+      if (s->fuse == 2 && it - 1 >= k0) {
+        TRY(adj2_launch(s, g, sh, ns, s->tuned_L, ab, acur, it, adj, F + FD, F, F - FD, fstride,
+                        s->acc.p, FD, gw));
+        it -= 2;
+      } else {
+        TRY(adj_launch(s, g, sh, ns, s->tuned_L1, ab, acur, it, adj, F + FD, F, fstride,
+                       s->acc.p, FD, gw));
+        it -= 1;
+      }
       acur ^= 1;
     }
     hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
@@ -1292,7 +1321,7 @@ extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps
 
 extern "C" int gjw_kernel_stats(gjw_survey* s, int32_t which, double* avg_ms, int64_t* count,
                                 int64_t* launches, int32_t reset) {
-  if (!s || which < 0 || which > 2) return fail(GJW_ERR_ARG, "gjw_kernel_stats: bad argument");
+  if (!s || which < 0 || which > 3) return fail(GJW_ERR_ARG, "gjw_kernel_stats: bad argument");
   TRY(kev_resolve(s));
   if (avg_ms) *avg_ms = s->kt_n[which] ? s->kt_ms[which] / s->kt_n[which] : 0.0;
   if (count) *count = s->kt_n[which];

@@ -875,6 +875,485 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
 }
 
 // ==========================================================================
+// adjoint: two fused steps per pass (temporal blocking)
+// ==========================================================================
+// One launch transposes forward steps it and it-1 for every shot, with the
+// tiling of k_fwd_step2: 64-lane waves over rows r0 = 60*t - 2 .. r0+63, lanes
+// 2..61 own their rows, step 2 one column behind step 1.  The correlation
+// partials of a cell are read and written once per pass:
+//   acc.x += me1 (F[it+1] - F[it]) + me2 (F[it] - F[it-1])
+//   acc.y += me1 (F[it+1] + F[it]) + me2 (F[it] + F[it-1])
+// with me1 = M(S_{it+1}) (input) and me2 = M(S_it) (after step 1 and its
+// residual injection).  Requires it >= 1.
+
+// W of an Ez cell from its values (the weights its adjoint hands to Hy, Hx).
+__device__ __forceinline__ void w_cell(const DevGrid& g, int i, int j, bool okrow, float me,
+                                       float x, float y, float& wy, float& wx) {
+  wy = 0.f;
+  wx = 0.f;
+  if (!okrow || j < 0 || j > g.NY) return;
+  const int kd = ekind(g.NX, g.NY, g.P, i, j);
+  const long long k = (long long)j * g.LD + i;
+  if (kd == KIND_INNER) {
+    const float c = g.cc[k].y;
+    wy = c * me;
+    wx = -c * me;
+  } else if (kd == KIND_PML) {
+    wy = g.cxh[k] * (x + me);
+    wx = g.cyh[k] * (y + me);
+  }
+}
+
+// One adjoint step of lane row i at column j with strip-aware coefficients.
+// me, (wy_c, wx_c): M(i, j) and its W before the step; wx_n: W_x(i, j+1);
+// ny, nx: Ny(i, j), Nx(i, j) before the step; x, y: adjoint split-field aux
+// (in/out, PML cells); pxp: BX*Nx'(i, j-1).  Out: mn = M(i, j) after the
+// step (no residual), nyv / nxv the new Ny / Nx, px = BX*nxv.
+__device__ __forceinline__ void adj_general(const DevGrid& g, int i, int j, bool okrow, float me,
+                                            float wy_c, float wx_c, float wx_n, float ny,
+                                            float nx, float& x, float& y, float pxp, float& mn,
+                                            float& nyv, float& nxv, float& px) {
+  const int NX = g.NX, NY = g.NY, P = g.P;
+  const bool ok = okrow && j >= 0 && j <= NY;
+  const float wy_dn = shfl_dn1(wy_c);
+  float py = 0.f;
+  nyv = 0.f;
+  nxv = 0.f;
+  px = 0.f;
+  if (ok && i < NX) {
+    float a, b;
+    hy_coef(g, i, j, a, b);
+    nyv = a * ny + wy_c - wy_dn;
+    py = b * nyv;
+  }
+  if (ok && j < NY) {
+    float a, b;
+    hx_coef(g, i, j, a, b);
+    nxv = a * nx + wx_c - wx_n;
+    px = b * nxv;
+  }
+  const float py_up = shfl_up1(py);
+  mn = 0.f;
+  if (ok) {
+    const int kd = ekind(NX, NY, P, i, j);
+    const long long k = (long long)j * g.LD + i;
+    const float base = kd == KIND_INNER ? g.cc[k].x * me : (kd == KIND_PML ? 0.f : me);
+    mn = base + py_up - py + pxp - px;
+    if (kd == KIND_PML) {
+      x = g.cxe[k] * (x + me);
+      y = g.cye[k] * (y + me);
+    }
+  }
+}
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 bload2(__amdgpu_buffer_rsrc_t r, unsigned off_floats, bool on) {
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(on ? off_floats * 4u : kOOB), 0, 0);
+  return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+__device__ __forceinline__ void bstore2(float2 v, __amdgpu_buffer_rsrc_t r, unsigned off_floats,
+                                        bool on) {
+  u32x2 w;
+  w.x = __float_as_uint(v.x);
+  w.y = __float_as_uint(v.y);
+  __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)(on ? off_floats * 4u : kOOB), 0, 0);
+}
+
+// it: the later of the two forward steps transposed (it >= 1).  M0 holds
+// dJ/dEz(S_{it+1}) with the residual of step it already added; M2 receives
+// dJ/dEz(S_{it-1}) with the residual of step it-2 added (when it >= 2).
+// Fa, Fb, Fc: forward frames Ez(S_{it+1}), Ez(S_it), Ez(S_{it-1}) on the domain.
+__global__ void __launch_bounds__(256)
+k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
+            const float* __restrict__ Nx0, const float* __restrict__ Ny0,
+            const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ M2,
+            float* __restrict__ Nx2, float* __restrict__ Ny2, float* __restrict__ X2,
+            float* __restrict__ Y2, int it, const float* __restrict__ adj,
+            const float* __restrict__ Fa, const float* __restrict__ Fb,
+            const float* __restrict__ Fc, long long fstride, float* __restrict__ acc,
+            long long astride, float* __restrict__ gw) {
+  const int lane = threadIdx.x;
+  const int NX = g.NX, NY = g.NY, P = g.P;
+  const long long LD = g.LD;
+  const TileId tid = tile_of_block(NX, NY, L, ROWS2);
+  const int shot = tid.shot;
+  const int r0 = tid.rowtile * ROWS2 - 2;
+  const int jb = __builtin_amdgcn_readfirstlane((tid.strip * blockDim.y + threadIdx.y) * L);
+  if (jb > NY) return;
+  const int je = min(jb + L, NY + 1);
+  const int i = r0 + lane;
+  const bool okrow = i >= 0 && i <= NX;
+  const bool own = okrow && lane >= 2 && lane <= 61;
+  const long long so = (long long)shot * g.FS;
+  M0 += so; Nx0 += so; Ny0 += so; X0 += so; Y0 += so;
+  M2 += so; Nx2 += so; Ny2 += so; X2 += so; Y2 += so;
+  const int si = s.si[shot], sj = s.sj[shot];
+  const float sc = s.sc[shot];
+  const int* cp = s.cptr + (long long)shot * (NY + 2);
+  const long long FD = (long long)LD * g.n;
+  const long long fo = (long long)shot * fstride;
+  float* ac = acc ? acc + 2 * (long long)shot * astride : nullptr;
+  // residual injection reaches halo lanes too: step 2 reads their step-1 values
+  const bool has_rec = adj && r0 <= s.rmax && r0 + 63 >= s.rmin;
+  const bool inner_rows = (r0 >= P + 1) && (r0 + 63 <= NX - P - 1);
+  const bool dom_row = i - g.di0 >= 0 && i - g.di0 < g.m;
+  auto dom_col = [&](int c) { return c - g.dj0 >= 0 && c - g.dj0 < g.n; };
+  const float bx0 = g.bx0, by0 = g.by0;
+  const long long ic = okrow ? i : 0;
+
+  const __amdgpu_buffer_rsrc_t rM0 = rsrc_of(const_cast<float*>(M0), g.FS),
+                               rNx0 = rsrc_of(const_cast<float*>(Nx0), g.FS),
+                               rNy0 = rsrc_of(const_cast<float*>(Ny0), g.FS),
+                               rX0 = rsrc_of(const_cast<float*>(X0), g.FS),
+                               rY0 = rsrc_of(const_cast<float*>(Y0), g.FS),
+                               rM2 = rsrc_of(M2, g.FS), rNx2 = rsrc_of(Nx2, g.FS),
+                               rNy2 = rsrc_of(Ny2, g.FS), rX2 = rsrc_of(X2, g.FS),
+                               rY2 = rsrc_of(Y2, g.FS),
+                               rCC = rsrc_of(const_cast<float*>(reinterpret_cast<const float*>(g.cc)), 2 * g.FS),
+                               rCXE = rsrc_of(const_cast<float*>(g.cxe), g.FS),
+                               rCXH = rsrc_of(const_cast<float*>(g.cxh), g.FS),
+                               rCYH = rsrc_of(const_cast<float*>(g.cyh), g.FS),
+                               rHAY = rsrc_of(const_cast<float*>(g.hay), g.FS),
+                               rHBY = rsrc_of(const_cast<float*>(g.hby), g.FS),
+                               rFa = rsrc_of(const_cast<float*>(Fa + fo), FD),
+                               rFb = rsrc_of(const_cast<float*>(Fb + fo), FD),
+                               rFc = rsrc_of(const_cast<float*>(Fc + fo), FD),
+                               rAC = rsrc_of(ac, ac ? 2 * FD : 0),
+                               rGW = rsrc_of(gw ? gw + (long long)shot * s.nt : nullptr,
+                                             gw ? s.nt : 0);
+
+  // row classes (valid in interior columns)
+  const bool e_inner = okrow && i >= P + 1 && i <= NX - P - 1;
+  const bool e_xn = okrow && i >= 1 && i <= P;
+  const bool e_xp = okrow && i >= NX - P && i <= NX - 1;
+  const bool e_pml = e_xn || e_xp;
+  const bool h_inner = okrow && i >= P && i <= NX - P - 1;
+  const bool h_xn = okrow && i >= 0 && i <= P - 1;
+  const bool h_xp = okrow && i >= NX - P && i <= NX - 1;
+  const bool h_any = h_inner || h_xn || h_xp;
+  float hay_r = 1.f, hby_r = 0.f, cxe_r = 1.f, cxh_r = 0.f, cyh_r = 0.f;
+  if (!inner_rows) {
+    const unsigned kr = (unsigned)((P + 1) * LD + ic);
+    hay_r = h_xn ? bload(rHAY, kr, true) : 1.f;
+    hby_r = h_xn ? bload(rHBY, kr, true) : 0.f;
+    cxe_r = e_xn ? bload(rCXE, kr, true) : 1.f;
+    cxh_r = e_xn ? bload(rCXH, kr, true) : 0.f;
+    cyh_r = e_xn ? bload(rCYH, kr, true) : 0.f;
+  }
+  const float hay_row = h_xn ? hay_r : 1.f;
+  const float hby_row = h_xn ? hby_r : (h_inner ? by0 : 0.f);
+
+  auto ldM = [&](int j) { return (okrow && j >= 0 && j <= NY) ? M0[(long long)j * LD + ic] : 0.f; };
+  auto ldNx = [&](int j) { return (okrow && j >= 0 && j < NY) ? Nx0[(long long)j * LD + ic] : 0.f; };
+  auto ldNy = [&](int j) { return (okrow && i < NX && j >= 0 && j <= NY) ? Ny0[(long long)j * LD + ic] : 0.f; };
+  auto is_pml = [&](int j) { return okrow && j >= 0 && j <= NY && ekind(NX, NY, P, i, j) == KIND_PML; };
+  auto ldX = [&](int j) { return is_pml(j) ? X0[(long long)j * LD + ic] : 0.f; };
+  auto ldY = [&](int j) { return is_pml(j) ? Y0[(long long)j * LD + ic] : 0.f; };
+  // data residual of forward step `step` at the receivers of column c
+  auto inject = [&](int c, float& mn, int step) {
+    if (has_rec && step >= 0 && c >= 0 && c <= NY) {
+      const int qa = cp[c], qb = cp[c + 1];
+      for (int q = qa; q < qb; ++q)
+        if (s.crow[q] == i) mn += adj[(long long)s.cid[q] * s.nt + step];
+    }
+  };
+  // class-path coefficients of column j (interior columns): the split-field
+  // and inner coefficients the W and aux updates of column j use
+  auto coef_col = [&](int j, float& ce, float& cy, float& cxe, float& cxh, float& cyh) {
+    const bool jin = j >= 1 && j <= NY - 1;
+    const long long k = (long long)(jin ? j : 0) * LD + ic;
+    const float2 c = (jin && e_inner) ? g.cc[k] : make_float2(0.f, 0.f);
+    ce = c.x;
+    cy = c.y;
+    cxe = (jin && e_xp) ? g.cxe[k] : cxe_r;
+    cxh = (jin && e_xp) ? g.cxh[k] : cxh_r;
+    cyh = (jin && e_xp) ? g.cyh[k] : cyh_r;
+  };
+
+  // ---- state at the start of step-1 column j (step 2 at j-1):
+  // step 1: M0, W0, X0, Y0 at j; px1 at j-1; coefficients of column j
+  // step 2: M after step 1 at j-1 and its W; Ny, Nx, X, Y after step 1 at
+  //         j-1; px2 at j-2; M0 at j-1 (correlation); coefficients of j-1
+  const int j1 = jb - 1;
+  float me1_c = ldM(j1), x0_c = ldX(j1), y0_c = ldY(j1), wy1_c, wx1_c;
+  w_cell(g, i, j1, okrow, me1_c, x0_c, y0_c, wy1_c, wx1_c);
+  float px1_prev = 0.f;
+  if (j1 - 1 >= 0 && j1 - 1 < NY && okrow) {
+    float wyp, wxp, a, b;
+    w_cell(g, i, j1 - 1, okrow, ldM(j1 - 1), ldX(j1 - 1), ldY(j1 - 1), wyp, wxp);
+    hx_coef(g, i, j1 - 1, a, b);
+    px1_prev = b * (a * ldNx(j1 - 1) + wxp - wx1_c);
+  }
+  float me2_c = 0.f, wy2_c = 0.f, wx2_c = 0.f, nyv1_c2 = 0.f, nxv1_c2 = 0.f, x1_c2 = 0.f,
+        y1_c2 = 0.f, px2_prev = 0.f, me1_c2 = 0.f;
+  float ce1_c, cy1_c, cxe_1c, cxh_1c, cyh_1c;
+  coef_col(j1, ce1_c, cy1_c, cxe_1c, cxh_1c, cyh_1c);
+  float ce2_c = 0.f, cxe_2c = 1.f, hay_2c = 1.f, hby_2c = 0.f;
+
+  auto chunkable = [&](int jj) { return jj - 1 >= P + 1 && jj + U <= NY - P - 1 && jj <= je; };
+
+  auto correlate = [&](float2& t, float fa, float fb, float fc) {
+    t.x = (t.x + me1_c2 * (fa - fb)) + me2_c * (fb - fc);
+    t.y = (t.y + me1_c2 * (fa + fb)) + me2_c * (fb + fc);
+  };
+
+  // ---- one general column (strips, edges, grid boundary)
+  auto general_col = [&](int j) {
+    const bool jin = j >= 0 && j <= NY;
+    const float me1_n = ldM(j + 1), x0_n = ldX(j + 1), y0_n = ldY(j + 1);
+    float wy1_n, wx1_n;
+    w_cell(g, i, j + 1, okrow, me1_n, x0_n, y0_n, wy1_n, wx1_n);
+    float x1 = x0_c, y1 = y0_c, mn1, nyv1, nxv1, px1;
+    adj_general(g, i, j, okrow, me1_c, wy1_c, wx1_c, wx1_n, ldNy(j), ldNx(j), x1, y1, px1_prev,
+                mn1, nyv1, nxv1, px1);
+    inject(j, mn1, it - 1);
+    float wy2_n, wx2_n;
+    w_cell(g, i, j, okrow, mn1, x1, y1, wy2_n, wx2_n);
+    const int c2 = j - 1;
+    float px2 = 0.f;
+    if (c2 >= jb - 1) {
+      float x2 = x1_c2, y2 = y1_c2, mn2, nyv2, nxv2;
+      adj_general(g, i, c2, okrow, me2_c, wy2_c, wx2_c, wx2_n, nyv1_c2, nxv1_c2, x2, y2,
+                  px2_prev, mn2, nyv2, nxv2, px2);
+      inject(c2, mn2, it - 2);
+      if (own && c2 >= jb && c2 < je) {
+        const long long k2 = (long long)c2 * LD + i;
+        M2[k2] = mn2;
+        if (c2 < NY) Nx2[k2] = nxv2;
+        if (i < NX) Ny2[k2] = nyv2;
+        const int kd = ekind(NX, NY, P, i, c2);
+        if (kd == KIND_PML) {
+          X2[k2] = x2;
+          Y2[k2] = y2;
+        }
+        if (ac && kd == KIND_INNER && dom_row && dom_col(c2)) {
+          const long long fk = (long long)(c2 - g.dj0) * LD + i;
+          float2* t2 = reinterpret_cast<float2*>(ac) + fk;
+          float2 t = *t2;
+          correlate(t, Fa[fo + fk], Fb[fo + fk], Fc[fo + fk]);
+          *t2 = t;
+        }
+        if (gw && i == si && c2 == sj) {
+          gw[(long long)shot * s.nt + it] = sc * me1_c2;
+          gw[(long long)shot * s.nt + it - 1] = sc * me2_c;
+        }
+      }
+    }
+    // hand over to the next column (and to a chunk that may follow)
+    me1_c2 = me1_c;
+    me1_c = me1_n;
+    wy1_c = wy1_n;
+    wx1_c = wx1_n;
+    x0_c = x0_n;
+    y0_c = y0_n;
+    px1_prev = px1;
+    me2_c = mn1;
+    wy2_c = wy2_n;
+    wx2_c = wx2_n;
+    nyv1_c2 = nyv1;
+    nxv1_c2 = nxv1;
+    x1_c2 = x1;
+    y1_c2 = y1;
+    px2_prev = px2;
+    ce2_c = ce1_c;
+    cxe_2c = cxe_1c;
+    const bool xp_cell = jin && j >= 1 && j <= NY - 1;
+    const long long kj = (long long)(jin ? j : 0) * LD + ic;
+    hay_2c = (h_xp && xp_cell) ? g.hay[kj] : hay_row;
+    hby_2c = (h_xp && xp_cell) ? g.hby[kj] : hby_row;
+    coef_col(j + 1, ce1_c, cy1_c, cxe_1c, cxh_1c, cyh_1c);
+  };
+
+  // correlation / gradient operands of step-2 column c2 (owned domain cells)
+  auto corr_on = [&](int c2) { return own && dom_row && c2 >= jb && c2 < je && dom_col(c2); };
+
+  // ---- interior chunk, all rows inner, no receivers
+  auto inner_chunk = [&](int jj) {
+    float mL[U], nx_[U], ny_[U], fa_[U], fb_[U], fc_[U];
+    float2 cL[U], t_[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned kk = (unsigned)((jj + u) * LD + i);
+      mL[u] = bload(rM0, kk + (unsigned)LD, true);
+      cL[u] = bload2(rCC, 2 * (kk + (unsigned)LD), true);
+      nx_[u] = bload(rNx0, kk, true);
+      ny_[u] = bload(rNy0, kk, true);
+      const int c2 = jj + u - 1;
+      const bool on = ac && corr_on(c2);
+      const unsigned fk = (unsigned)((c2 - g.dj0) * LD + i);
+      fa_[u] = bload(rFa, fk, on);
+      fb_[u] = bload(rFb, fk, on);
+      fc_[u] = bload(rFc, fk, on);
+      t_[u] = bload2(rAC, 2 * fk, on);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c1 = jj + u, c2 = c1 - 1;
+      // step 1 at c1
+      const float wy1_n = cL[u].y * mL[u];
+      const float wx1_n = -cL[u].y * mL[u];
+      const float nyv1 = ny_[u] + wy1_c - shfl_dn1(wy1_c);
+      const float nxv1 = nx_[u] + wx1_c - wx1_n;
+      const float py1 = by0 * nyv1, px1 = bx0 * nxv1;
+      const float mn1 = ce1_c * me1_c + shfl_up1(py1) - py1 + px1_prev - px1;
+      const float wy2_n = cy1_c * mn1;
+      const float wx2_n = -cy1_c * mn1;
+      // step 2 at c2
+      const float nyv2 = nyv1_c2 + wy2_c - shfl_dn1(wy2_c);
+      const float nxv2 = nxv1_c2 + wx2_c - wx2_n;
+      const float py2 = by0 * nyv2, px2 = bx0 * nxv2;
+      const float mn2 = ce2_c * me2_c + shfl_up1(py2) - py2 + px2_prev - px2;
+      const bool o2 = own && c2 >= jb && c2 < je;
+      const unsigned k2 = (unsigned)(c2 * LD + i);
+      bstore(mn2, rM2, k2, o2);
+      bstore(nxv2, rNx2, k2, o2);
+      bstore(nyv2, rNy2, k2, o2);
+      {
+        float2 t = t_[u];
+        correlate(t, fa_[u], fb_[u], fc_[u]);
+        bstore2(t, rAC, 2 * (unsigned)((c2 - g.dj0) * LD + i), ac && corr_on(c2));
+      }
+      const bool at_src = o2 && i == si && c2 == sj;
+      bstore(sc * me1_c2, rGW, (unsigned)it, at_src);
+      bstore(sc * me2_c, rGW, (unsigned)(it - 1), at_src);
+      me1_c2 = me1_c;
+      me1_c = mL[u];
+      wy1_c = wy1_n;
+      wx1_c = wx1_n;
+      px1_prev = px1;
+      me2_c = mn1;
+      wy2_c = wy2_n;
+      wx2_c = wx2_n;
+      nyv1_c2 = nyv1;
+      nxv1_c2 = nxv1;
+      px2_prev = px2;
+      ce2_c = ce1_c;
+      ce1_c = cL[u].x;
+      cy1_c = cL[u].y;
+    }
+    x0_c = y0_c = x1_c2 = y1_c2 = 0.f;
+    cxe_2c = 1.f;
+    hay_2c = 1.f;
+    hby_2c = by0;
+    cxe_1c = 1.f;
+    cxh_1c = cyh_1c = 0.f;
+  };
+
+  // ---- interior chunk, rows of every class (top / bottom PML rows, receivers)
+  auto class_chunk = [&](int jj) {
+    float mL[U], xL[U], yL[U], cxeL[U], cxhL[U], cyhL[U], nx_[U], ny_[U], hayA[U], hbyA[U],
+        fa_[U], fb_[U], fc_[U];
+    float2 cL[U], t_[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned kk = (unsigned)((jj + u) * LD + ic), kn = kk + (unsigned)LD;
+      mL[u] = bload(rM0, kn, okrow);
+      cL[u] = bload2(rCC, 2 * kn, e_inner);
+      xL[u] = bload(rX0, kn, e_pml);
+      yL[u] = bload(rY0, kn, e_pml);
+      cxeL[u] = e_xp ? bload(rCXE, kn, true) : cxe_r;
+      cxhL[u] = e_xp ? bload(rCXH, kn, true) : cxh_r;
+      cyhL[u] = e_xp ? bload(rCYH, kn, true) : cyh_r;
+      nx_[u] = bload(rNx0, kk, okrow);
+      ny_[u] = bload(rNy0, kk, h_any);
+      hayA[u] = h_xp ? bload(rHAY, kk, true) : hay_row;
+      hbyA[u] = h_xp ? bload(rHBY, kk, true) : hby_row;
+      const int c2 = jj + u - 1;
+      const bool on = ac && e_inner && corr_on(c2);
+      const unsigned fk = (unsigned)((c2 - g.dj0) * LD + ic);
+      fa_[u] = bload(rFa, fk, on);
+      fb_[u] = bload(rFb, fk, on);
+      fc_[u] = bload(rFc, fk, on);
+      t_[u] = bload2(rAC, 2 * fk, on);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c1 = jj + u, c2 = c1 - 1;
+      // step 1 at c1
+      const float wy1_n = e_inner ? cL[u].y * mL[u] : (e_pml ? cxhL[u] * (xL[u] + mL[u]) : 0.f);
+      const float wx1_n = e_inner ? -cL[u].y * mL[u] : (e_pml ? cyhL[u] * (yL[u] + mL[u]) : 0.f);
+      const float wdn1 = shfl_dn1(wy1_c);
+      const float nyv1 = h_any ? hayA[u] * ny_[u] + wy1_c - wdn1 : 0.f;
+      const float nxv1 = okrow ? nx_[u] + wx1_c - wx1_n : 0.f;
+      const float py1 = hbyA[u] * nyv1, px1 = bx0 * nxv1;
+      const float pup1 = shfl_up1(py1);
+      const float base1 = e_inner ? ce1_c * me1_c : (e_pml ? 0.f : me1_c);
+      float mn1 = base1 + pup1 - py1 + px1_prev - px1;
+      const float x1 = e_pml ? cxe_1c * (x0_c + me1_c) : 0.f;
+      const float y1 = e_pml ? y0_c + me1_c : 0.f;  // cye = 1 off the left/right strips
+      inject(c1, mn1, it - 1);
+      const float wy2_n = e_inner ? cy1_c * mn1 : (e_pml ? cxh_1c * (x1 + mn1) : 0.f);
+      const float wx2_n = e_inner ? -cy1_c * mn1 : (e_pml ? cyh_1c * (y1 + mn1) : 0.f);
+      // step 2 at c2
+      const float wdn2 = shfl_dn1(wy2_c);
+      const float nyv2 = h_any ? hay_2c * nyv1_c2 + wy2_c - wdn2 : 0.f;
+      const float nxv2 = okrow ? nxv1_c2 + wx2_c - wx2_n : 0.f;
+      const float py2 = hby_2c * nyv2, px2 = bx0 * nxv2;
+      const float pup2 = shfl_up1(py2);
+      const float base2 = e_inner ? ce2_c * me2_c : (e_pml ? 0.f : me2_c);
+      float mn2 = base2 + pup2 - py2 + px2_prev - px2;
+      const float x2 = e_pml ? cxe_2c * (x1_c2 + me2_c) : 0.f;
+      const float y2 = e_pml ? y1_c2 + me2_c : 0.f;
+      inject(c2, mn2, it - 2);
+      const bool o2 = own && c2 >= jb && c2 < je;
+      const unsigned k2 = (unsigned)(c2 * LD + ic);
+      bstore(mn2, rM2, k2, o2);
+      bstore(nxv2, rNx2, k2, o2);
+      bstore(nyv2, rNy2, k2, o2 && h_any);
+      bstore(x2, rX2, k2, o2 && e_pml);
+      bstore(y2, rY2, k2, o2 && e_pml);
+      {
+        float2 t = t_[u];
+        correlate(t, fa_[u], fb_[u], fc_[u]);
+        bstore2(t, rAC, 2 * (unsigned)((c2 - g.dj0) * LD + ic), ac && e_inner && corr_on(c2));
+      }
+      const bool at_src = o2 && i == si && c2 == sj;
+      bstore(sc * me1_c2, rGW, (unsigned)it, at_src);
+      bstore(sc * me2_c, rGW, (unsigned)(it - 1), at_src);
+      me1_c2 = me1_c;
+      me1_c = mL[u];
+      wy1_c = wy1_n;
+      wx1_c = wx1_n;
+      x0_c = xL[u];
+      y0_c = yL[u];
+      px1_prev = px1;
+      me2_c = mn1;
+      wy2_c = wy2_n;
+      wx2_c = wx2_n;
+      nyv1_c2 = nyv1;
+      nxv1_c2 = nxv1;
+      x1_c2 = x1;
+      y1_c2 = y1;
+      px2_prev = px2;
+      ce2_c = ce1_c;
+      cxe_2c = cxe_1c;
+      hay_2c = hayA[u];
+      hby_2c = hbyA[u];
+      ce1_c = cL[u].x;
+      cy1_c = cL[u].y;
+      cxe_1c = cxeL[u];
+      cxh_1c = cxhL[u];
+      cyh_1c = cyhL[u];
+    }
+  };
+
+  int j = j1;
+  const bool use_inner = inner_rows && !has_rec;
+  while (j <= je) {
+    if (chunkable(j)) {
+      if (use_inner)
+        inner_chunk(j);
+      else
+        class_chunk(j);
+      j += U;
+    } else {
+      general_col(j);
+      ++j;
+    }
+  }
+}
+
+// ==========================================================================
 // small kernels
 // ==========================================================================
 __global__ void k_fill(float* p, long long n, float v) {

@@ -257,8 +257,9 @@ class Survey:
         return g_eps, g_sig, J.value
 
     def kernel_stats(self, which, reset=False):
-        """(average ms, samples, launches) of the one-step forward (0), adjoint
-        (1) and two-step forward (2) kernels while GJW_OPT_KTIMING is on."""
+        """(average ms, samples, launches) of the one-step forward (0), one-step
+        adjoint (1), two-step forward (2) and two-step adjoint (3) kernels while
+        GJW_OPT_KTIMING is on."""
         ms, n, nl = C.c_double(0), C.c_int64(0), C.c_int64(0)
         N.check(N.load().gjw_kernel_stats(self.h, int(which), C.byref(ms), C.byref(n), C.byref(nl),
                                           int(reset)), "gjw_kernel_stats")

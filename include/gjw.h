@@ -143,8 +143,8 @@ int gjw_differentiate_cube(gjw_ctx *ctx, float *v, int32_t nz, int32_t nx, int32
 int gjw_last_timing(gjw_survey *s, double *ms, double *cell_updates);
 
 /* Average device duration (ms) and count of the sampled step kernels, and the
- * number of launches of that kernel, while GJW_OPT_KTIMING is on.  which: 0 = one-step forward, 1 = adjoint step,
- * 2 = two-step forward.  Has no
+ * number of launches of that kernel, while GJW_OPT_KTIMING is on.  which: 0 = one-step forward, 1 = one-step adjoint,
+ * 2 = two-step forward, 3 = two-step adjoint.  Has no
  * counterpart in the reference (measurement only). */
 int gjw_kernel_stats(gjw_survey *s, int32_t which, double *avg_ms, int64_t *count,
                      int64_t *launches, int32_t reset);
