@@ -141,7 +141,7 @@ struct gjw_survey {
   // model, padded (NX x NY, column-major, FP64)
   std::vector<double> pe, ps;
   float bx0 = 0.f, by0 = 0.f;
-  DBuf<float> coef;  // ce, ch, cxe, cxh, cye, cyh, hax, hbx, hay, hby (10*FS)
+  DBuf<float> coef;  // cc (2), cxe, cxh, cye, cyh, hax, hbx, hay, hby, uc0 (4), uc1 (4), uc2
   DBuf<float> coef_ref;  // ce, ch with sigma negated (w_grad.m:26-27)
   DBuf<float> zero_sc;   // per-shot zero source coefficients
   DBuf<float> trash;     // write target of lanes that own no cell (see k_fwd_step2)
@@ -331,7 +331,7 @@ extern "C" int gjw_set_model(gjw_survey* s, const double* eps_r, const double* s
   const double cl = 1.0 / std::sqrt(u0 * e0);
   const double smax = -(ord + 1) * e0 * cl * std::log(s->gr.R0) / (2.0 * dx * P);  // w_cpml.m:47
   const long long FS = s->FS, LD = s->LD;
-  std::vector<float> h((size_t)10 * FS, 0.f);
+  std::vector<float> h((size_t)kCoefFloats * FS, 0.f);
   // (Ceze, Cezhy) interleaved per cell in the first 2*FS floats
   float *cc = h.data(), *cxe = cc + 2 * FS, *cxh = cxe + FS, *cye = cxh + FS,
         *cyh = cye + FS, *hax = cyh + FS, *hbx = hax + FS, *hay = hbx + FS, *hby = hay + FS;
@@ -405,6 +405,49 @@ extern "C" int gjw_set_model(gjw_survey* s, const double* eps_r, const double* s
     }
   s->bx0 = (float)(-(2 * dt / dy) / (2 * u0));  // Chxez
   s->by0 = (float)((2 * dt / dx) / (2 * u0));   // Chyez
+  // Per-cell table of every update coefficient, for the kernels' edge and
+  // strip columns: uc0 = (ce, cxe, cxh, cye), uc1 = (cyh, AX, BX, AY), uc2 = BY
+  // with  Ez' = ce Ez + (cxe X + cxh dHy) + (cye Y + cyh dHx)  and the split
+  // parts stored at PML cells only (inner: ce = Ceze, cxh = -cyh = Cezhy,
+  // cxe = cye = 0; ring: ce = 1, rest 0), Hx' = AX Hx + BX dEz/dj,
+  // Hy' = AY Hy + BY dEz/di ((1, 0) where a component is not updated).
+  {
+    float *uc0 = h.data() + 10 * FS, *uc1 = uc0 + 4 * FS, *uc2 = uc1 + 4 * FS;
+    for (int j = 0; j <= NY; ++j)
+      for (int i = 0; i <= NX; ++i) {
+        const long long k = (long long)j * LD + i;
+        float ce = 1.f, a = 0.f, b = 0.f, c = 0.f, d = 0.f;
+        const int kd = ekind(NX, NY, P, i, j);
+        if (kd == KIND_INNER) {
+          ce = cc[2 * k];
+          b = cc[2 * k + 1];
+          d = -cc[2 * k + 1];
+        } else if (kd == KIND_PML) {
+          ce = 0.f;
+          a = cxe[k]; b = cxh[k]; c = cye[k]; d = cyh[k];
+        }
+        float ax = 1.f, bx = 0.f, ay = 1.f, by = 0.f;
+        if (j <= NY - 1) {
+          if (j >= P && j <= NY - P - 1) {
+            bx = s->bx0;
+          } else if (i >= 1 && i <= NX - 1) {
+            ax = hax[k];
+            bx = hbx[k];
+          }
+        }
+        if (i <= NX - 1) {
+          if (i >= P && i <= NX - P - 1) {
+            by = s->by0;
+          } else if (j >= 1 && j <= NY - 1) {
+            ay = hay[k];
+            by = hby[k];
+          }
+        }
+        uc0[4 * k] = ce; uc0[4 * k + 1] = a; uc0[4 * k + 2] = b; uc0[4 * k + 3] = c;
+        uc1[4 * k] = d; uc1[4 * k + 1] = ax; uc1[4 * k + 2] = bx; uc1[4 * k + 3] = ay;
+        uc2[k] = by;
+      }
+  }
   // gradient scale factors on the domain: d/d eps = -2 eps0 / D, d/d sigma = -dt / D
   std::vector<double> hfe((size_t)m * n, 0.0), hfs((size_t)m * n, 0.0);
   for (int b = 0; b < n; ++b)
@@ -567,6 +610,9 @@ static DevGrid dev_grid(const gjw_survey* s) {
   g.cc = reinterpret_cast<const float2*>(c); g.cxe = c + 2 * FS; g.cxh = c + 3 * FS; g.cye = c + 4 * FS;
   g.cyh = c + 5 * FS; g.hax = c + 6 * FS; g.hbx = c + 7 * FS; g.hay = c + 8 * FS;
   g.hby = c + 9 * FS;
+  g.uc0 = reinterpret_cast<const float4*>(c + 10 * FS);
+  g.uc1 = reinterpret_cast<const float4*>(c + 14 * FS);
+  g.uc2 = c + 18 * FS;
   g.di0 = s->di0; g.dj0 = s->dj0; g.m = s->gr.m; g.n = s->gr.n;
   return g;
 }

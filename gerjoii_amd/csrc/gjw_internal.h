@@ -27,6 +27,11 @@ struct DevGrid {
   const float* hbx;
   const float* hay;        // Hy strip cells (rows 0..P-1, NX-P..NX-1)
   const float* hby;
+  // every coefficient per cell (edge and strip columns of the two-step kernels):
+  // uc0 = (ce, cxe, cxh, cye), uc1 = (cyh, AX, BX, AY), uc2 = BY
+  const float4* uc0;
+  const float4* uc1;
+  const float* uc2;
   // stored domain window (w_geom.m:168-172): rows di0..di0+m-1, cols dj0..dj0+n-1
   int di0, dj0, m, n;
 };
@@ -45,6 +50,9 @@ struct DevShots {
 };
 
 enum { KIND_RING = 0, KIND_INNER = 1, KIND_PML = 2 };
+
+// floats per cell slot of the coefficient buffer (cc 2, split 4, H strips 4, table 9)
+constexpr int kCoefFloats = 19;
 
 __host__ __device__ inline int ekind(int NX, int NY, int P, int i, int j) {
   if (i <= 0 || i >= NX || j <= 0 || j >= NY) return KIND_RING;

@@ -97,9 +97,13 @@ __device__ __forceinline__ float hx_new(const DevGrid& g, int i, int j, float hx
 // XCDs (each with its own L2), so launch-order neighbours land on different
 // L2s.  The launch is 1-D; block b is remapped to logical tile
 // t = (b mod 8) * (B/8) + b/8, which gives each XCD a contiguous run of
-// logical tiles.  Logical tiles enumerate row tiles fastest, then column
-// strips, then shots, so the rows and columns a wave reads as halo were
-// fetched by a wave on the same XCD.
+// logical tiles.  Logical tiles enumerate, fastest first: the shots of a
+// group of kShotGroup shots, row tiles, column strips, shot groups.  The
+// shots of a group run the same (row tile, strip) back to back on one XCD, so
+// they share its coefficient lines in L2; the rows and columns a wave reads
+// as halo were fetched by a wave on the same XCD a few tiles earlier.
+constexpr int kShotGroup = 8;
+
 struct TileId {
   int rowtile, strip, shot;
 };
@@ -110,10 +114,15 @@ __device__ __forceinline__ TileId tile_of_block(int NX, int NY, int L, int rows 
   const unsigned t = (b < per * 8u) ? (b % 8u) * per + b / 8u : b;
   const unsigned gx = (unsigned)((NX + 1 + rows - 1) / rows);
   const unsigned gy = (unsigned)(((NY + 1 + L - 1) / L + 3) / 4);
+  const unsigned ns = B / (gx * gy);
+  const unsigned span = (unsigned)kShotGroup * gx * gy;  // tiles per full shot group
+  const unsigned grp = t / span, rem = t - grp * span;
+  const unsigned gc = min((unsigned)kShotGroup, ns - grp * (unsigned)kShotGroup);
+  const unsigned rest = rem / gc;
   TileId id;
-  id.rowtile = (int)(t % gx);
-  id.strip = (int)((t / gx) % gy);
-  id.shot = (int)(t / (gx * gy));
+  id.shot = (int)(grp * (unsigned)kShotGroup + rem % gc);
+  id.rowtile = (int)(rest % gx);
+  id.strip = (int)(rest / gx);
   return id;
 }
 
@@ -281,39 +290,18 @@ k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
 // that halo lanes can read the start-of-pass values of cells other waves own.
 constexpr int ROWS2 = 60;
 
-// One Yee step of lane row i at column j with strip-aware coefficients.
-// e_c, e_n: E_old(i, j), E_old(i, j+1); hx, hy: H_old(i, j); hxp: Hx_new(i, j-1);
-// x, y: split-field aux of the cell (old in, new out, PML cells only).
-__device__ __forceinline__ void yee_general(const DevGrid& g, int i, int j, bool okrow, float e_c,
-                                            float e_n, float hx, float hy, float hxp, float& x,
-                                            float& y, float& hxn, float& hyn, float& en) {
-  const int NX = g.NX, NY = g.NY, P = g.P;
-  const float e_dn = shfl_dn1(e_c);
-  const bool ok = okrow && j >= 0 && j <= NY;
-  hyn = (ok && i < NX) ? hy_new(g, i, j, hy, e_c, e_dn) : 0.f;
-  hxn = (ok && j < NY) ? hx_new(g, i, j, hx, e_c, e_n) : 0.f;
-  const float hy_up = shfl_up1(hyn);
-  en = 0.f;
-  if (ok) {
-    const int kd = ekind(NX, NY, P, i, j);
-    const float dhy = hyn - hy_up, dhx = hxn - hxp;
-    if (kd == KIND_INNER) {
-      const float2 c = g.cc[(long long)j * g.LD + i];
-      en = c.x * e_c + c.y * (dhy - dhx);
-    } else if (kd == KIND_PML) {
-      const long long k = (long long)j * g.LD + i;
-      x = g.cxe[k] * x + g.cxh[k] * dhy;
-      y = g.cye[k] * y + g.cyh[k] * dhx;
-      en = x + y;
-    }
-  }
-}
-
 // Masked raw buffer load: a lane with on == false reads past the end of the
 // buffer and gets 0 without touching memory (no branch, see bstore).
 __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned off_floats, bool on) {
   return __uint_as_float(
       __builtin_amdgcn_raw_buffer_load_b32(r, (int)(on ? off_floats * 4u : kOOB), 0, 0));
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned off_floats, bool on) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(on ? off_floats * 4u : kOOB), 0, 0);
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
+                     __uint_as_float(v.w));
 }
 
 // FRAMES: the recompute sweep stores Ez on the domain after both steps.
@@ -329,7 +317,8 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned off_fl
 //     lane knows whether its row is inner, in the top strip (coefficients
 //     depend on the row only and sit in registers) or in the bottom strip
 //     (coefficients depend on eps(i, j) and are loaded per cell).
-// The left/right PML columns and the grid's edge columns use yee_general.
+// The left/right PML columns, the grid's edge columns and the chunks that
+// straddle them use a per-cell coefficient table (univ_chunk).
 template <bool FRAMES>
 __global__ void __launch_bounds__(256)
 k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
@@ -382,7 +371,10 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
                                rCXH = rsrc_of(const_cast<float*>(g.cxh), g.FS),
                                rCYH = rsrc_of(const_cast<float*>(g.cyh), g.FS),
                                rHAY = rsrc_of(const_cast<float*>(g.hay), g.FS),
-                               rHBY = rsrc_of(const_cast<float*>(g.hby), g.FS);
+                               rHBY = rsrc_of(const_cast<float*>(g.hby), g.FS),
+                               rU0 = rsrc_of(const_cast<float*>(&g.uc0->x), 4 * g.FS),
+                               rU1 = rsrc_of(const_cast<float*>(&g.uc1->x), 4 * g.FS),
+                               rU2 = rsrc_of(const_cast<float*>(g.uc2), g.FS);
 
   // row classes (valid in interior columns)
   const bool e_inner = okrow && i >= P + 1 && i <= NX - P - 1;
@@ -393,6 +385,11 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
   const bool h_xn = okrow && i >= 0 && i <= P - 1;
   const bool h_xp = okrow && i >= NX - P && i <= NX - 1;
   const bool h_any = h_inner || h_xn || h_xp;  // Hy exists (i <= NX-1)
+  const bool notring = okrow && i >= 1 && i <= NX - 1;
+  // split-field (PML) Ez cell in column c
+  auto pml_cell = [&](int c) {
+    return notring && c >= 1 && c <= NY - 1 && (e_pml || c <= P || c >= NY - P);
+  };
   // top-strip coefficients depend on the row only: read them once (column P+1)
   float hay_r = 1.f, hby_r = 0.f, cxe_r = 1.f, cxh_r = 0.f, cyh_r = 0.f;
   if (!inner_rows) {
@@ -409,18 +406,11 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
 
   auto ldE = [&](int j) { return (okrow && j >= 0 && j <= NY) ? E0[(long long)j * LD + ic] : 0.f; };
   auto ldHx = [&](int j) { return (okrow && j >= 0 && j < NY) ? Hx0[(long long)j * LD + ic] : 0.f; };
-  auto ldHy = [&](int j) { return (okrow && i < NX && j >= 0 && j <= NY) ? Hy0[(long long)j * LD + ic] : 0.f; };
-  auto is_pml = [&](int j) { return okrow && j >= 0 && j <= NY && ekind(NX, NY, P, i, j) == KIND_PML; };
-  auto record = [&](int j, float e, int step, float* fr) {
-    if (has_rec) {
-      const int qa = cp[j], qb = cp[j + 1];
-      for (int q = qa; q < qb; ++q)
-        if (s.crow[q] == i) tr[(long long)s.cid[q] * s.nt + step] = e;
-    }
-    if (fr) {
-      const int a = i - g.di0, b = j - g.dj0;
-      if (a >= 0 && a < g.m && b >= 0 && b < g.n) fr[(long long)b * LD + i] = e;
-    }
+  // receiver samples of column j (0 <= j <= NY)
+  auto record = [&](int j, float e, int step) {
+    const int qa = cp[j], qb = cp[j + 1];
+    for (int q = qa; q < qb; ++q)
+      if (s.crow[q] == i) tr[(long long)s.cid[q] * s.nt + step] = e;
   };
 
   // ---- prologue: step 1 starts at column jb-1; it needs Hx^{n+1}(jb-2)
@@ -434,59 +424,96 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
   float2 cc_c2 = make_float2(0.f, 0.f);
   float cxe_c2 = 1.f, cxh_c2 = 0.f, cyh_c2 = 0.f, hay_c2 = 1.f, hby_c2 = 0.f;
 
-  // both steps' columns of a chunk at jj are interior; jj <= je keeps it useful
-  auto chunkable = [&](int jj) {
-    return jj - 1 >= P + 1 && jj + U - 1 <= NY - P - 1 && jj <= je;
-  };
+  // both steps' columns of a chunk at jj are interior
+  auto chunkable = [&](int jj) { return jj - 1 >= P + 1 && jj + U - 1 <= NY - P - 1; };
 
-  // ---- one general column (strips, edges, grid boundary)
-  auto general_col = [&](int j) {
-    const bool jin = j >= 0 && j <= NY;
-    const float e0_n = ldE(j + 1);
-    float x1 = 0.f, y1 = 0.f;
-    if (is_pml(j)) {
-      x1 = X0[(long long)j * LD + ic];
-      y1 = Y0[(long long)j * LD + ic];
+  // ---- chunk of any columns (left/right strips, grid edges, strip ends):
+  // every coefficient per cell from the table, Ez' = ce Ez + X' + Y'
+  auto univ_chunk = [&](int jj) {
+    float en_[U], hx_[U], hy_[U], x_[U], y_[U], kc_[U];
+    float4 ka_[U], kb_[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = jj + u;
+      const bool cin = okrow && c >= 0 && c <= NY;
+      const unsigned kk = (unsigned)(c * LD + ic);
+      en_[u] = bload(rE0, kk + (unsigned)LD, okrow && c + 1 >= 0 && c + 1 <= NY);
+      hx_[u] = bload(rHx0, kk, cin && c < NY);
+      hy_[u] = bload(rHy0, kk, cin && i < NX);
+      const bool pm = pml_cell(c);
+      x_[u] = bload(rX0, kk, pm);
+      y_[u] = bload(rY0, kk, pm);
+      ka_[u] = bload4(rU0, 4 * kk, cin);
+      kb_[u] = bload4(rU1, 4 * kk, cin);
+      kc_[u] = bload(rU2, kk, cin);
     }
-    float hx1, hy1, e1;
-    yee_general(g, i, j, okrow, e0_c, e0_n, ldHx(j), ldHy(j), hx1_prev, x1, y1, hx1, hy1, e1);
-    if (jin && j == sj && i == si) e1 += ws1;
-    if (own && jin && j >= jb && j < je) record(j, e1, it, fr1);
-    const int c2 = j - 1;
-    float hx2 = 0.f;
-    if (c2 >= jb - 1) {
-      // Hx^{n+2}(c2) always (the next column's step 2 needs it); the rest only for owned columns
-      float x2 = x1_c2, y2 = y1_c2, hy2, e2;
-      yee_general(g, i, c2, okrow, e1_prev, e1, hx1_c2, hy1_c2, hx2_prev, x2, y2, hx2, hy2, e2);
-      if (c2 >= 0 && c2 <= NY && c2 == sj && i == si) e2 += ws2;
-      if (own && c2 >= jb && c2 < je) {
-        const long long k2 = (long long)c2 * LD + i;
-        E2[k2] = e2;
-        if (c2 < NY) Hx2[k2] = hx2;
-        if (i < NX) Hy2[k2] = hy2;
-        if (is_pml(c2)) {
-          X2[k2] = x2;
-          Y2[k2] = y2;
-        }
-        record(c2, e2, it + 1, fr2);
+    float4 ka2, kb2;
+    float kc2;
+    {
+      const int c = jj - 1;
+      const bool cin = okrow && c >= 0 && c <= NY;
+      const unsigned kk = (unsigned)(c * LD + ic);
+      ka2 = bload4(rU0, 4 * kk, cin);
+      kb2 = bload4(rU1, 4 * kk, cin);
+      kc2 = bload(rU2, kk, cin);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c1 = jj + u, c2 = c1 - 1;
+      // step 1 at c1: (ce, cxe, cxh, cye) = ka, (cyh, AX, BX, AY) = kb, BY = kc
+      const float e_dn = shfl_dn1(e0_c);
+      const float hy1 = kb_[u].w * hy_[u] + kc_[u] * (e_dn - e0_c);
+      const float hx1 = kb_[u].y * hx_[u] + kb_[u].z * (en_[u] - e0_c);
+      const float dhy1 = hy1 - shfl_up1(hy1), dhx1 = hx1 - hx1_prev;
+      const float x1 = ka_[u].y * x_[u] + ka_[u].z * dhy1;
+      const float y1 = ka_[u].w * y_[u] + kb_[u].x * dhx1;
+      float e1 = ka_[u].x * e0_c + x1 + y1;
+      e1 += (c1 == sj && i == si) ? ws1 : 0.f;
+      // step 2 at c2
+      const float e_dn2 = shfl_dn1(e1_prev);
+      const float hy2 = kb2.w * hy1_c2 + kc2 * (e_dn2 - e1_prev);
+      const float hx2 = kb2.y * hx1_c2 + kb2.z * (e1 - e1_prev);
+      const float dhy2 = hy2 - shfl_up1(hy2), dhx2 = hx2 - hx2_prev;
+      const float x2 = ka2.y * x1_c2 + ka2.z * dhy2;
+      const float y2 = ka2.w * y1_c2 + kb2.x * dhx2;
+      float e2 = ka2.x * e1_prev + x2 + y2;
+      e2 += (c2 == sj && i == si) ? ws2 : 0.f;
+      const bool o2 = own && c2 >= jb && c2 < je;
+      const bool pm2 = pml_cell(c2);
+      const unsigned k2 = (unsigned)(c2 * LD + ic);
+      bstore(e2, rE2, k2, o2);
+      bstore(hx2, rHx2, k2, o2 && c2 < NY);
+      bstore(hy2, rHy2, k2, o2 && h_any);
+      bstore(x2, rX2, k2, o2 && pm2);
+      bstore(y2, rY2, k2, o2 && pm2);
+      if (has_rec) {
+        if (own && c1 >= jb && c1 < je) record(c1, e1, it);
+        if (o2) record(c2, e2, it + 1);
       }
+      if (FRAMES) {
+        bstore(e1, rsrc_of(fr1, FD), (unsigned)((c1 - g.dj0) * LD + ic),
+               own && dom_row && dom_col(c1) && c1 >= jb && c1 < je);
+        bstore(e2, rsrc_of(fr2, FD), (unsigned)((c2 - g.dj0) * LD + ic), o2 && dom_row && dom_col(c2));
+      }
+      e0_c = en_[u];
+      hx1_prev = hx1;
+      e1_prev = e1;
+      hx1_c2 = hx1;
+      hy1_c2 = hy1;
+      x1_c2 = x1;
+      y1_c2 = y1;
+      hx2_prev = hx2;
+      ka2 = ka_[u];
+      kb2 = kb_[u];
+      kc2 = kc_[u];
     }
-    e0_c = e0_n;
-    hx1_prev = hx1;
-    e1_prev = e1;
-    hx1_c2 = hx1;
-    hy1_c2 = hy1;
-    x1_c2 = x1;
-    y1_c2 = y1;
-    hx2_prev = hx2;
-    const long long kj = (long long)(jin ? j : 0) * LD + ic;
-    cc_c2 = jin ? g.cc[kj] : make_float2(0.f, 0.f);
-    const bool xp_cell = jin && j >= 1 && j <= NY - 1;
-    cxe_c2 = (e_xp && xp_cell) ? g.cxe[kj] : cxe_r;
-    cxh_c2 = (e_xp && xp_cell) ? g.cxh[kj] : cxh_r;
-    cyh_c2 = (e_xp && xp_cell) ? g.cyh[kj] : cyh_r;
-    hay_c2 = (h_xp && xp_cell) ? g.hay[kj] : hay_row;
-    hby_c2 = (h_xp && xp_cell) ? g.hby[kj] : hby_row;
+    // coefficients of the next step-2 column, in the interior chunks' form
+    cc_c2 = make_float2(ka2.x, ka2.z);
+    cxe_c2 = ka2.y;
+    cxh_c2 = ka2.z;
+    cyh_c2 = kb2.x;
+    hay_c2 = kb2.w;
+    hby_c2 = kc2;
   };
 
   // ---- interior chunk, all rows inner: scalar H coefficients, (Ceze, Cezhy) loads
@@ -536,6 +563,7 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
       hx2_prev = hx2;
       cc_c2 = c_[u];
     }
+    x1_c2 = y1_c2 = 0.f;  // no split-field cells in these rows
   };
 
   // ---- interior chunk, rows of every class (top / bottom PML rows present)
@@ -589,8 +617,8 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
       bstore(x2, rX2, k2, o2 && e_pml);
       bstore(y2, rY2, k2, o2 && e_pml);
       if (has_rec) {
-        if (own && c1 >= jb && c1 < je) record(c1, e1, it, nullptr);
-        if (o2) record(c2, e2, it + 1, nullptr);
+        if (own && c1 >= jb && c1 < je) record(c1, e1, it);
+        if (o2) record(c2, e2, it + 1);
       }
       if (FRAMES) {
         bstore(e1, rsrc_of(fr1, FD), (unsigned)((c1 - g.dj0) * LD + ic),
@@ -614,19 +642,15 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
     }
   };
 
-  int j = j1;  // step 1 at column j, step 2 at column j-1
+  // step 1 at columns j .. j+U-1, step 2 one column behind
   const bool use_inner = inner_rows && !has_rec;
-  while (j <= je) {
-    if (chunkable(j)) {
-      if (use_inner)
-        inner_chunk(j);
-      else
-        class_chunk(j);
-      j += U;
-    } else {
-      general_col(j);
-      ++j;
-    }
+  for (int j = j1; j <= je; j += U) {
+    if (!chunkable(j))
+      univ_chunk(j);
+    else if (use_inner)
+      inner_chunk(j);
+    else
+      class_chunk(j);
   }
 }
 
@@ -904,48 +928,6 @@ __device__ __forceinline__ void w_cell(const DevGrid& g, int i, int j, bool okro
   }
 }
 
-// One adjoint step of lane row i at column j with strip-aware coefficients.
-// me, (wy_c, wx_c): M(i, j) and its W before the step; wx_n: W_x(i, j+1);
-// ny, nx: Ny(i, j), Nx(i, j) before the step; x, y: adjoint split-field aux
-// (in/out, PML cells); pxp: BX*Nx'(i, j-1).  Out: mn = M(i, j) after the
-// step (no residual), nyv / nxv the new Ny / Nx, px = BX*nxv.
-__device__ __forceinline__ void adj_general(const DevGrid& g, int i, int j, bool okrow, float me,
-                                            float wy_c, float wx_c, float wx_n, float ny,
-                                            float nx, float& x, float& y, float pxp, float& mn,
-                                            float& nyv, float& nxv, float& px) {
-  const int NX = g.NX, NY = g.NY, P = g.P;
-  const bool ok = okrow && j >= 0 && j <= NY;
-  const float wy_dn = shfl_dn1(wy_c);
-  float py = 0.f;
-  nyv = 0.f;
-  nxv = 0.f;
-  px = 0.f;
-  if (ok && i < NX) {
-    float a, b;
-    hy_coef(g, i, j, a, b);
-    nyv = a * ny + wy_c - wy_dn;
-    py = b * nyv;
-  }
-  if (ok && j < NY) {
-    float a, b;
-    hx_coef(g, i, j, a, b);
-    nxv = a * nx + wx_c - wx_n;
-    px = b * nxv;
-  }
-  const float py_up = shfl_up1(py);
-  mn = 0.f;
-  if (ok) {
-    const int kd = ekind(NX, NY, P, i, j);
-    const long long k = (long long)j * g.LD + i;
-    const float base = kd == KIND_INNER ? g.cc[k].x * me : (kd == KIND_PML ? 0.f : me);
-    mn = base + py_up - py + pxp - px;
-    if (kd == KIND_PML) {
-      x = g.cxe[k] * (x + me);
-      y = g.cye[k] * (y + me);
-    }
-  }
-}
-
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float2 bload2(__amdgpu_buffer_rsrc_t r, unsigned off_floats, bool on) {
   const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(on ? off_floats * 4u : kOOB), 0, 0);
@@ -1020,7 +1002,10 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
                                rFc = rsrc_of(const_cast<float*>(Fc + fo), FD),
                                rAC = rsrc_of(ac, ac ? 2 * FD : 0),
                                rGW = rsrc_of(gw ? gw + (long long)shot * s.nt : nullptr,
-                                             gw ? s.nt : 0);
+                                             gw ? s.nt : 0),
+                               rU0 = rsrc_of(const_cast<float*>(&g.uc0->x), 4 * g.FS),
+                               rU1 = rsrc_of(const_cast<float*>(&g.uc1->x), 4 * g.FS),
+                               rU2 = rsrc_of(const_cast<float*>(g.uc2), g.FS);
 
   // row classes (valid in interior columns)
   const bool e_inner = okrow && i >= P + 1 && i <= NX - P - 1;
@@ -1031,6 +1016,10 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
   const bool h_xn = okrow && i >= 0 && i <= P - 1;
   const bool h_xp = okrow && i >= NX - P && i <= NX - 1;
   const bool h_any = h_inner || h_xn || h_xp;
+  const bool notring = okrow && i >= 1 && i <= NX - 1;
+  auto pml_cell = [&](int c) {
+    return notring && c >= 1 && c <= NY - 1 && (e_pml || c <= P || c >= NY - P);
+  };
   float hay_r = 1.f, hby_r = 0.f, cxe_r = 1.f, cxh_r = 0.f, cyh_r = 0.f;
   if (!inner_rows) {
     const unsigned kr = (unsigned)((P + 1) * LD + ic);
@@ -1045,10 +1034,8 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
 
   auto ldM = [&](int j) { return (okrow && j >= 0 && j <= NY) ? M0[(long long)j * LD + ic] : 0.f; };
   auto ldNx = [&](int j) { return (okrow && j >= 0 && j < NY) ? Nx0[(long long)j * LD + ic] : 0.f; };
-  auto ldNy = [&](int j) { return (okrow && i < NX && j >= 0 && j <= NY) ? Ny0[(long long)j * LD + ic] : 0.f; };
-  auto is_pml = [&](int j) { return okrow && j >= 0 && j <= NY && ekind(NX, NY, P, i, j) == KIND_PML; };
-  auto ldX = [&](int j) { return is_pml(j) ? X0[(long long)j * LD + ic] : 0.f; };
-  auto ldY = [&](int j) { return is_pml(j) ? Y0[(long long)j * LD + ic] : 0.f; };
+  auto ldX = [&](int j) { return pml_cell(j) ? X0[(long long)j * LD + ic] : 0.f; };
+  auto ldY = [&](int j) { return pml_cell(j) ? Y0[(long long)j * LD + ic] : 0.f; };
   // data residual of forward step `step` at the receivers of column c
   auto inject = [&](int c, float& mn, int step) {
     if (has_rec && step >= 0 && c >= 0 && c <= NY) {
@@ -1090,78 +1077,11 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
   coef_col(j1, ce1_c, cy1_c, cxe_1c, cxh_1c, cyh_1c);
   float ce2_c = 0.f, cxe_2c = 1.f, hay_2c = 1.f, hby_2c = 0.f;
 
-  auto chunkable = [&](int jj) { return jj - 1 >= P + 1 && jj + U <= NY - P - 1 && jj <= je; };
+  auto chunkable = [&](int jj) { return jj - 1 >= P + 1 && jj + U <= NY - P - 1; };
 
   auto correlate = [&](float2& t, float fa, float fb, float fc) {
     t.x = (t.x + me1_c2 * (fa - fb)) + me2_c * (fb - fc);
     t.y = (t.y + me1_c2 * (fa + fb)) + me2_c * (fb + fc);
-  };
-
-  // ---- one general column (strips, edges, grid boundary)
-  auto general_col = [&](int j) {
-    const bool jin = j >= 0 && j <= NY;
-    const float me1_n = ldM(j + 1), x0_n = ldX(j + 1), y0_n = ldY(j + 1);
-    float wy1_n, wx1_n;
-    w_cell(g, i, j + 1, okrow, me1_n, x0_n, y0_n, wy1_n, wx1_n);
-    float x1 = x0_c, y1 = y0_c, mn1, nyv1, nxv1, px1;
-    adj_general(g, i, j, okrow, me1_c, wy1_c, wx1_c, wx1_n, ldNy(j), ldNx(j), x1, y1, px1_prev,
-                mn1, nyv1, nxv1, px1);
-    inject(j, mn1, it - 1);
-    float wy2_n, wx2_n;
-    w_cell(g, i, j, okrow, mn1, x1, y1, wy2_n, wx2_n);
-    const int c2 = j - 1;
-    float px2 = 0.f;
-    if (c2 >= jb - 1) {
-      float x2 = x1_c2, y2 = y1_c2, mn2, nyv2, nxv2;
-      adj_general(g, i, c2, okrow, me2_c, wy2_c, wx2_c, wx2_n, nyv1_c2, nxv1_c2, x2, y2,
-                  px2_prev, mn2, nyv2, nxv2, px2);
-      inject(c2, mn2, it - 2);
-      if (own && c2 >= jb && c2 < je) {
-        const long long k2 = (long long)c2 * LD + i;
-        M2[k2] = mn2;
-        if (c2 < NY) Nx2[k2] = nxv2;
-        if (i < NX) Ny2[k2] = nyv2;
-        const int kd = ekind(NX, NY, P, i, c2);
-        if (kd == KIND_PML) {
-          X2[k2] = x2;
-          Y2[k2] = y2;
-        }
-        if (ac && kd == KIND_INNER && dom_row && dom_col(c2)) {
-          const long long fk = (long long)(c2 - g.dj0) * LD + i;
-          float2* t2 = reinterpret_cast<float2*>(ac) + fk;
-          float2 t = *t2;
-          correlate(t, Fa[fo + fk], Fb[fo + fk], Fc[fo + fk]);
-          *t2 = t;
-        }
-        if (gw && i == si && c2 == sj) {
-          gw[(long long)shot * s.nt + it] = sc * me1_c2;
-          gw[(long long)shot * s.nt + it - 1] = sc * me2_c;
-        }
-      }
-    }
-    // hand over to the next column (and to a chunk that may follow)
-    me1_c2 = me1_c;
-    me1_c = me1_n;
-    wy1_c = wy1_n;
-    wx1_c = wx1_n;
-    x0_c = x0_n;
-    y0_c = y0_n;
-    px1_prev = px1;
-    me2_c = mn1;
-    wy2_c = wy2_n;
-    wx2_c = wx2_n;
-    nyv1_c2 = nyv1;
-    nxv1_c2 = nxv1;
-    x1_c2 = x1;
-    y1_c2 = y1;
-    px2_prev = px2;
-    ce2_c = ce1_c;
-    cxe_2c = cxe_1c;
-    const bool xp_cell = jin && j >= 1 && j <= NY - 1;
-    const long long kj = (long long)(jin ? j : 0) * LD + ic;
-    hay_2c = (h_xp && xp_cell) ? g.hay[kj] : hay_row;
-    hby_2c = (h_xp && xp_cell) ? g.hby[kj] : hby_row;
-    coef_col(j + 1, ce1_c, cy1_c, cxe_1c, cxh_1c, cyh_1c);
   };
 
   // correlation / gradient operands of step-2 column c2 (owned domain cells)
@@ -1337,19 +1257,132 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
     }
   };
 
-  int j = j1;
-  const bool use_inner = inner_rows && !has_rec;
-  while (j <= je) {
-    if (chunkable(j)) {
-      if (use_inner)
-        inner_chunk(j);
-      else
-        class_chunk(j);
-      j += U;
-    } else {
-      general_col(j);
-      ++j;
+  // ---- chunk of any columns (left/right strips, grid edges, strip ends):
+  // every coefficient per cell from the table:
+  //   W = (cxh (X + M), cyh (Y + M)),  M' = ce M + d(BY Ny')/di + d(BX Nx')/dj,
+  //   X' = cxe (X + M), Y' = cye (Y + M)
+  auto univ_chunk = [&](int jj) {
+    float mL[U], xL[U], yL[U], kcL[U], nx_[U], ny_[U], fa_[U], fb_[U], fc_[U];
+    float4 kaL[U], kbL[U];
+    float2 t_[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = jj + u, cn = c + 1;
+      const unsigned kk = (unsigned)(c * LD + ic), kn = kk + (unsigned)LD;
+      const bool nin = okrow && cn >= 0 && cn <= NY;
+      mL[u] = bload(rM0, kn, nin);
+      const bool pmn = pml_cell(cn);
+      xL[u] = bload(rX0, kn, pmn);
+      yL[u] = bload(rY0, kn, pmn);
+      kaL[u] = bload4(rU0, 4 * kn, nin);
+      kbL[u] = bload4(rU1, 4 * kn, nin);
+      kcL[u] = bload(rU2, kn, nin);
+      nx_[u] = bload(rNx0, kk, okrow && c >= 0 && c < NY);
+      ny_[u] = bload(rNy0, kk, okrow && c >= 0 && c <= NY && i < NX);
+      const int c2 = c - 1;
+      const bool on = ac && e_inner && c2 >= P + 1 && c2 <= NY - P - 1 && corr_on(c2);
+      const unsigned fk = (unsigned)((c2 - g.dj0) * LD + ic);
+      fa_[u] = bload(rFa, fk, on);
+      fb_[u] = bload(rFb, fk, on);
+      fc_[u] = bload(rFc, fk, on);
+      t_[u] = bload2(rAC, 2 * fk, on);
     }
+    float4 ka1, kb1, ka2, kb2;
+    float kc1, kc2;
+    {
+      const bool in1 = okrow && jj >= 0 && jj <= NY, in2 = okrow && jj - 1 >= 0 && jj - 1 <= NY;
+      const unsigned k1 = (unsigned)(jj * LD + ic), k2 = (unsigned)((jj - 1) * LD + ic);
+      ka1 = bload4(rU0, 4 * k1, in1);
+      kb1 = bload4(rU1, 4 * k1, in1);
+      kc1 = bload(rU2, k1, in1);
+      ka2 = bload4(rU0, 4 * k2, in2);
+      kb2 = bload4(rU1, 4 * k2, in2);
+      kc2 = bload(rU2, k2, in2);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c1 = jj + u, c2 = c1 - 1;
+      // W at c1+1 before step 1: (ce, cxe, cxh, cye) = ka, (cyh, AX, BX, AY) = kb, BY = kc
+      const float wy1_n = kaL[u].z * (xL[u] + mL[u]);
+      const float wx1_n = kbL[u].x * (yL[u] + mL[u]);
+      // step 1 at c1
+      const float nyv1 = kb1.w * ny_[u] + wy1_c - shfl_dn1(wy1_c);
+      const float nxv1 = kb1.y * nx_[u] + wx1_c - wx1_n;
+      const float py1 = kc1 * nyv1, px1 = kb1.z * nxv1;
+      float mn1 = ka1.x * me1_c + shfl_up1(py1) - py1 + px1_prev - px1;
+      const float x1 = ka1.y * (x0_c + me1_c);
+      const float y1 = ka1.w * (y0_c + me1_c);
+      inject(c1, mn1, it - 1);
+      const float wy2_n = ka1.z * (x1 + mn1);
+      const float wx2_n = kb1.x * (y1 + mn1);
+      // step 2 at c2
+      const float nyv2 = kb2.w * nyv1_c2 + wy2_c - shfl_dn1(wy2_c);
+      const float nxv2 = kb2.y * nxv1_c2 + wx2_c - wx2_n;
+      const float py2 = kc2 * nyv2, px2 = kb2.z * nxv2;
+      float mn2 = ka2.x * me2_c + shfl_up1(py2) - py2 + px2_prev - px2;
+      const float x2 = ka2.y * (x1_c2 + me2_c);
+      const float y2 = ka2.w * (y1_c2 + me2_c);
+      inject(c2, mn2, it - 2);
+      const bool o2 = own && c2 >= jb && c2 < je;
+      const bool pm2 = pml_cell(c2);
+      const unsigned k2 = (unsigned)(c2 * LD + ic);
+      bstore(mn2, rM2, k2, o2);
+      bstore(nxv2, rNx2, k2, o2 && c2 < NY);
+      bstore(nyv2, rNy2, k2, o2 && h_any);
+      bstore(x2, rX2, k2, o2 && pm2);
+      bstore(y2, rY2, k2, o2 && pm2);
+      {
+        float2 t = t_[u];
+        correlate(t, fa_[u], fb_[u], fc_[u]);
+        bstore2(t, rAC, 2 * (unsigned)((c2 - g.dj0) * LD + ic),
+                ac && e_inner && c2 >= P + 1 && c2 <= NY - P - 1 && corr_on(c2));
+      }
+      const bool at_src = o2 && i == si && c2 == sj;
+      bstore(sc * me1_c2, rGW, (unsigned)it, at_src);
+      bstore(sc * me2_c, rGW, (unsigned)(it - 1), at_src);
+      me1_c2 = me1_c;
+      me1_c = mL[u];
+      wy1_c = wy1_n;
+      wx1_c = wx1_n;
+      x0_c = xL[u];
+      y0_c = yL[u];
+      px1_prev = px1;
+      me2_c = mn1;
+      wy2_c = wy2_n;
+      wx2_c = wx2_n;
+      nyv1_c2 = nyv1;
+      nxv1_c2 = nxv1;
+      x1_c2 = x1;
+      y1_c2 = y1;
+      px2_prev = px2;
+      ka2 = ka1;
+      kb2 = kb1;
+      kc2 = kc1;
+      ka1 = kaL[u];
+      kb1 = kbL[u];
+      kc1 = kcL[u];
+    }
+    // coefficients of the next columns, in the interior chunks' form
+    ce1_c = ka1.x;
+    cy1_c = ka1.z;
+    cxe_1c = ka1.y;
+    cxh_1c = ka1.z;
+    cyh_1c = kb1.x;
+    ce2_c = ka2.x;
+    cxe_2c = ka2.y;
+    hay_2c = kb2.w;
+    hby_2c = kc2;
+  };
+
+  // step 1 at columns j .. j+U-1, step 2 one column behind
+  const bool use_inner = inner_rows && !has_rec;
+  for (int j = j1; j <= je; j += U) {
+    if (!chunkable(j))
+      univ_chunk(j);
+    else if (use_inner)
+      inner_chunk(j);
+    else
+      class_chunk(j);
   }
 }
 
