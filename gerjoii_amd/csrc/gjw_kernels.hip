@@ -297,6 +297,19 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned off_fl
       __builtin_amdgcn_raw_buffer_load_b32(r, (int)(on ? off_floats * 4u : kOOB), 0, 0));
 }
 
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 bload2(__amdgpu_buffer_rsrc_t r, unsigned off_floats, bool on) {
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(on ? off_floats * 4u : kOOB), 0, 0);
+  return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+__device__ __forceinline__ void bstore2(float2 v, __amdgpu_buffer_rsrc_t r, unsigned off_floats,
+                                        bool on) {
+  u32x2 w;
+  w.x = __float_as_uint(v.x);
+  w.y = __float_as_uint(v.y);
+  __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)(on ? off_floats * 4u : kOOB), 0, 0);
+}
+
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned off_floats, bool on) {
   const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(on ? off_floats * 4u : kOOB), 0, 0);
@@ -517,26 +530,37 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
   };
 
   // ---- interior chunk, all rows inner: scalar H coefficients, (Ceze, Cezhy) loads
-  auto inner_chunk = [&](int jj) {
-    float en_[U], hx_[U], hy_[U];
-    float2 c_[U];
+  // Loads of one interior chunk (all rows inner).  The interior run is
+  // software-pipelined: the next chunk's loads are issued before the current
+  // chunk is computed, so a wave always has a chunk of requests in flight.
+  // Every load is issued unconditionally (past the end of the run it reads
+  // out of bounds, which returns 0 without a memory access) so the compiler
+  // can count outstanding requests exactly.
+  struct InnerBuf {
+    float en[U], hx[U], hy[U];
+    float2 c[U];
+  };
+  auto inner_load = [&](int jj, InnerBuf& b, bool on) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const unsigned kk = (unsigned)((jj + u) * LD + i);
-      en_[u] = bload(rE0, kk + (unsigned)LD, true);
-      hx_[u] = bload(rHx0, kk, true);
-      hy_[u] = bload(rHy0, kk, true);
-      c_[u] = make_float2(bload(rCC, 2 * kk, true), bload(rCC, 2 * kk + 1, true));
+      b.en[u] = bload(rE0, kk + (unsigned)LD, on);
+      b.hx[u] = bload(rHx0, kk, on);
+      b.hy[u] = bload(rHy0, kk, on);
+      b.c[u] = bload2(rCC, 2 * kk, on);
     }
+  };
+  // ---- interior chunk, all rows inner: scalar H coefficients, (Ceze, Cezhy) loads
+  auto inner_compute = [&](int jj, const InnerBuf& b) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int c1 = jj + u, c2 = c1 - 1;
       // step 1 at c1
       const float e_dn = shfl_dn1(e0_c);
-      const float hy1 = hy_[u] + by0 * (e_dn - e0_c);
-      const float hx1 = hx_[u] + bx0 * (en_[u] - e0_c);
+      const float hy1 = b.hy[u] + by0 * (e_dn - e0_c);
+      const float hx1 = b.hx[u] + bx0 * (b.en[u] - e0_c);
       const float hy1u = shfl_up1(hy1);
-      float e1 = c_[u].x * e0_c + c_[u].y * ((hy1 - hy1u) - (hx1 - hx1_prev));
+      float e1 = b.c[u].x * e0_c + b.c[u].y * ((hy1 - hy1u) - (hx1 - hx1_prev));
       e1 += (c1 == sj && i == si) ? ws1 : 0.f;
       // step 2 at c2
       const float e_dn2 = shfl_dn1(e1_prev);
@@ -555,13 +579,13 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
                own && dom_row && dom_col(c1) && c1 >= jb && c1 < je);
         bstore(e2, rsrc_of(fr2, FD), (unsigned)((c2 - g.dj0) * LD + i), o2 && dom_row && dom_col(c2));
       }
-      e0_c = en_[u];
+      e0_c = b.en[u];
       hx1_prev = hx1;
       e1_prev = e1;
       hx1_c2 = hx1;
       hy1_c2 = hy1;
       hx2_prev = hx2;
-      cc_c2 = c_[u];
+      cc_c2 = b.c[u];
     }
     x1_c2 = y1_c2 = 0.f;  // no split-field cells in these rows
   };
@@ -642,16 +666,29 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
     }
   };
 
-  // step 1 at columns j .. j+U-1, step 2 one column behind
+  // Step 1 at columns j .. j+U-1, step 2 one column behind.  Chunk starts
+  // j1 + kU split into a strip/edge prefix, an interior run and a suffix.
   const bool use_inner = inner_rows && !has_rec;
-  for (int j = j1; j <= je; j += U) {
-    if (!chunkable(j))
-      univ_chunk(j);
-    else if (use_inner)
-      inner_chunk(j);
-    else
-      class_chunk(j);
+  int j = j1;
+  for (; j <= je && !chunkable(j); j += U) univ_chunk(j);
+  int jz = j;  // end of the interior run
+  while (jz <= je && chunkable(jz)) jz += U;
+  if (use_inner) {
+    InnerBuf A, B;
+    inner_load(j, A, j < jz);
+    while (j < jz) {
+      inner_load(j + U, B, j + U < jz);
+      inner_compute(j, A);
+      j += U;
+      if (j >= jz) break;
+      inner_load(j + U, A, j + U < jz);
+      inner_compute(j, B);
+      j += U;
+    }
+  } else {
+    for (; j < jz; j += U) class_chunk(j);
   }
+  for (; j <= je; j += U) univ_chunk(j);
 }
 
 template __global__ void k_fwd_step2<false>(DevGrid, DevShots, int, const float*, const float*,
@@ -926,19 +963,6 @@ __device__ __forceinline__ void w_cell(const DevGrid& g, int i, int j, bool okro
     wy = g.cxh[k] * (x + me);
     wx = g.cyh[k] * (y + me);
   }
-}
-
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ float2 bload2(__amdgpu_buffer_rsrc_t r, unsigned off_floats, bool on) {
-  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(on ? off_floats * 4u : kOOB), 0, 0);
-  return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
-}
-__device__ __forceinline__ void bstore2(float2 v, __amdgpu_buffer_rsrc_t r, unsigned off_floats,
-                                        bool on) {
-  u32x2 w;
-  w.x = __float_as_uint(v.x);
-  w.y = __float_as_uint(v.y);
-  __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)(on ? off_floats * 4u : kOOB), 0, 0);
 }
 
 // it: the later of the two forward steps transposed (it >= 1).  M0 holds
