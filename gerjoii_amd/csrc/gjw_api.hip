@@ -54,6 +54,10 @@ __global__ void k_fill_d(double*, long long, double);
 __global__ void k_inject_last(DevGrid, int, int, int, const int*, const int*, const int*,
                               const float*, int, float*);
 __global__ void k_gather(const float*, long long, const int*, int, int, float*);
+__global__ void k_pack16(const float*, long long, const int*, long long, int, int, int*,
+                         unsigned short*);
+__global__ void k_unpack16(const unsigned short*, long long, int, const int*, int, const int*,
+                           long long, float*);
 __global__ void k_scatter(const float*, long long, const int*, int, int, float*);
 __global__ void k_domain_frame(DevGrid, int, const float*, float*, long long);
 __global__ void k_flush(int, int, int, int, int, float*, long long, const double*, const double*,
@@ -159,7 +163,7 @@ struct gjw_survey {
   DBuf<float> fld;   // forward: E0,Hx0,Hy0,E1,Hx1,Hy1,X,Y  (8 * ns * FS)
   DBuf<float> adjf;  // adjoint: M0,Nx0,Ny0,X0,Y0,M1,Nx1,Ny1,X1,Y1 (10 * ns * FS)
   DBuf<float> tr, adj, obs;
-  DBuf<float> ck;    // checkpoints
+  DBuf<float> ck;    // checkpoints (FP32, or 16-bit planes + shifts; see ck_stride)
   DBuf<float> frames;
   DBuf<float> acc;
   DBuf<double> ge, gsg, part, Jd, gw;
@@ -285,7 +289,10 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       s->ksample = (int)v;
       return GJW_OK;
     case GJW_OPT_SNAPSHOT:
-      if (v != GJW_SNAP_F32) return fail(GJW_ERR_ARG, "GJW_OPT_SNAPSHOT: only F32 is built");
+      if (v != GJW_SNAP_F32 && v != GJW_SNAP_BF16 && v != GJW_SNAP_F16)
+        return fail(GJW_ERR_ARG, "GJW_OPT_SNAPSHOT: F32, BF16 or F16");
+      if (v != GJW_SNAP_F32 && s->LD > 2048)
+        return fail(GJW_ERR_ARG, "GJW_OPT_SNAPSHOT: 16-bit snapshots need NX+1 <= 2048 rows");
       s->snap = (int)v;
       break;
     default:
@@ -876,14 +883,60 @@ static int zero_fwd_state(gjw_survey* s, int nsh) {
   return launch_fill(s->fld.p, (long long)10 * nsh * s->FS, 0.f, s->ctx->stream);
 }
 
-// checkpoint layout: [ck][E, Hx, Hy: ns*FS each][X, Y: ns*npml each]
+// Checkpoint layout, one slot per checkpoint:
+//   FP32:   [E, Hx, Hy: ns*FS floats each][X, Y: ns*npml floats each]
+//   16-bit: the same five planes as BF16/FP16 values, each plane padded to
+//           256 bytes, then per plane the power-of-two shifts (int32) of its
+//           scale blocks (one grid column of E/Hx/Hy; 256 split-field cells)
+static inline long long al256(long long b) { return (b + 255) / 256 * 256; }
+
+static long long ck_plane_cnt(const gjw_survey* s, int p) {  // values per shot
+  return p < 3 ? s->FS : (long long)s->pml_idx.size();
+}
+static int ck_plane_B(const gjw_survey* s, int p) { return p < 3 ? s->LD : 256; }
+static long long ck_plane_nb(const gjw_survey* s, int p) {  // scale blocks per shot
+  return cdiv(ck_plane_cnt(s, p), ck_plane_B(s, p));
+}
+
+// slot size in floats (the buffer is typed float)
 static long long ck_stride(const gjw_survey* s) {
-  return 3LL * s->ns * s->FS + 2LL * s->ns * (long long)s->pml_idx.size();
+  if (s->snap == GJW_SNAP_F32)
+    return 3LL * s->ns * s->FS + 2LL * s->ns * (long long)s->pml_idx.size();
+  long long bytes = 0;
+  for (int p = 0; p < 5; ++p)
+    bytes += al256(2LL * s->ns * ck_plane_cnt(s, p)) + al256(4LL * s->ns * ck_plane_nb(s, p));
+  return bytes / 4;
+}
+
+// 16-bit slot: plane pointers and their shift tables
+static void ck16_layout(const gjw_survey* s, int c, unsigned short* pl[5], int* sh[5]) {
+  char* base = reinterpret_cast<char*>(s->ck.p + (long long)c * ck_stride(s));
+  for (int p = 0; p < 5; ++p) {
+    pl[p] = reinterpret_cast<unsigned short*>(base);
+    base += al256(2LL * s->ns * ck_plane_cnt(s, p));
+    sh[p] = reinterpret_cast<int*>(base);
+    base += al256(4LL * s->ns * ck_plane_nb(s, p));
+  }
 }
 
 static int ck_save(gjw_survey* s, FwdBufs& b, int cur, int xc, int c) {
   hipStream_t st = s->ctx->stream;
   const long long sz = (long long)s->ns * s->FS, npml = (long long)s->pml_idx.size();
+  if (s->snap != GJW_SNAP_F32) {
+    unsigned short* pl[5];
+    int* sh[5];
+    ck16_layout(s, c, pl, sh);
+    const float* src[5] = {b.E[cur], b.Hx[cur], b.Hy[cur], b.X[xc], b.Y[xc]};
+    for (int p = 0; p < 5; ++p) {
+      const long long cnt = ck_plane_cnt(s, p);
+      if (cnt == 0) continue;
+      hipLaunchKernelGGL(k_pack16, dim3((unsigned)ck_plane_nb(s, p), (unsigned)s->ns), dim3(256), 0,
+                         st, src[p], s->FS, p < 3 ? nullptr : s->d_pml.p, cnt, ck_plane_B(s, p),
+                         s->snap, sh[p], pl[p]);
+      HIPCHK(hipGetLastError());
+    }
+    return GJW_OK;
+  }
   float* dst = s->ck.p + (long long)c * ck_stride(s);
   HIPCHK(hipMemcpyAsync(dst, b.E[cur], sz * 4, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(dst + sz, b.Hx[cur], sz * 4, hipMemcpyDeviceToDevice, st));
@@ -902,6 +955,21 @@ static int ck_save(gjw_survey* s, FwdBufs& b, int cur, int xc, int c) {
 static int ck_restore(gjw_survey* s, FwdBufs& b, int cur, int xc, int c) {
   hipStream_t st = s->ctx->stream;
   const long long sz = (long long)s->ns * s->FS, npml = (long long)s->pml_idx.size();
+  if (s->snap != GJW_SNAP_F32) {
+    unsigned short* pl[5];
+    int* sh[5];
+    ck16_layout(s, c, pl, sh);
+    float* dst[5] = {b.E[cur], b.Hx[cur], b.Hy[cur], b.X[xc], b.Y[xc]};
+    for (int p = 0; p < 5; ++p) {
+      const long long cnt = ck_plane_cnt(s, p);
+      if (cnt == 0) continue;
+      hipLaunchKernelGGL(k_unpack16, dim3((unsigned)ck_plane_nb(s, p), (unsigned)s->ns), dim3(256),
+                         0, st, pl[p], cnt, ck_plane_B(s, p), sh[p], s->snap,
+                         p < 3 ? nullptr : s->d_pml.p, s->FS, dst[p]);
+      HIPCHK(hipGetLastError());
+    }
+    return GJW_OK;
+  }
   const float* src = s->ck.p + (long long)c * ck_stride(s);
   HIPCHK(hipMemcpyAsync(b.E[cur], src, sz * 4, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipMemcpyAsync(b.Hx[cur], src + sz, sz * 4, hipMemcpyDeviceToDevice, st));

@@ -1454,6 +1454,86 @@ __global__ void k_scatter(const float* __restrict__ src, long long FS, const int
   dst[(long long)sh * FS + idx[q]] = src[t];
 }
 
+// --------------------------------------------------------------------------
+// 16-bit checkpoint snapshots with per-block scaling
+// --------------------------------------------------------------------------
+// A snapshot plane is one field of every shot: either the whole pitched field
+// (idx == null, cnt = FS values per shot, scale blocks of B = LD values: one
+// grid column) or its split-field cells (idx = PML cell list, cnt entries per
+// shot, blocks of B = 256 list entries).  Each block is scaled by a power of
+// two 2^sh chosen from its max |v| so that its largest value lands just below
+// 2^15 (FP16) or in [1, 2) (BF16); power-of-two scaling is exact, so the only
+// rounding is the round-to-nearest-even conversion.  One workgroup per block.
+constexpr int kSnapMaxPer = 8;  // values per thread: B <= 256 * kSnapMaxPer
+
+__device__ __forceinline__ int snap_shift(float a, int mode) {
+  if (!(a > 0.f)) return 0;
+  int e;
+  frexpf(a, &e);  // a = f * 2^e, 0.5 <= f < 1
+  return mode == 2 ? 15 - e : 1 - e;
+}
+
+__device__ __forceinline__ unsigned short to16(float v, int mode) {
+  if (mode == 2) return __builtin_bit_cast(unsigned short, (_Float16)v);  // RNE
+  unsigned u = __float_as_uint(v);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // bf16 round to nearest even
+  return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float from16(unsigned short q, int mode) {
+  if (mode == 2) return (float)__builtin_bit_cast(_Float16, q);
+  return __uint_as_float((unsigned)q << 16);
+}
+
+// grid (blocks per shot, shots), 256 threads; mode 1 = BF16, 2 = FP16;
+// shifts[shot * nb + block] receives sh
+__global__ void __launch_bounds__(256)
+k_pack16(const float* __restrict__ src, long long FS, const int* __restrict__ idx, long long cnt,
+         int B, int mode, int* __restrict__ shifts, unsigned short* __restrict__ dst) {
+  const int shot = blockIdx.y, blk = blockIdx.x;
+  const long long t0 = (long long)blk * B;
+  const int nb = gridDim.x;
+  const float* p = src + (long long)shot * FS;
+  float v[kSnapMaxPer];
+  float m = 0.f;
+#pragma unroll
+  for (int r = 0; r < kSnapMaxPer; ++r) {
+    const int o = r * 256 + (int)threadIdx.x;
+    const long long t = t0 + o;
+    v[r] = (o < B && t < cnt) ? p[idx ? idx[t] : t] : 0.f;
+    m = fmaxf(m, fabsf(v[r]));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  __shared__ float part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+  const int sh = snap_shift(m, mode);
+  if (threadIdx.x == 0) shifts[(long long)shot * nb + blk] = sh;
+  unsigned short* q = dst + (long long)shot * cnt;
+#pragma unroll
+  for (int r = 0; r < kSnapMaxPer; ++r) {
+    const int o = r * 256 + (int)threadIdx.x;
+    const long long t = t0 + o;
+    if (o < B && t < cnt) q[t] = to16(ldexpf(v[r], sh), mode);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_unpack16(const unsigned short* __restrict__ src, long long cnt, int B,
+           const int* __restrict__ shifts, int mode, const int* __restrict__ idx, long long FS,
+           float* __restrict__ dst) {
+  const int shot = blockIdx.y, blk = blockIdx.x;
+  const long long t0 = (long long)blk * B;
+  const int sh = shifts[(long long)shot * gridDim.x + blk];
+  const unsigned short* q = src + (long long)shot * cnt;
+  float* p = dst + (long long)shot * FS;
+  for (int o = threadIdx.x; o < B; o += 256) {
+    const long long t = t0 + o;
+    if (t < cnt) p[idx ? idx[t] : t] = ldexpf(from16(q[t], mode), -sh);
+  }
+}
+
 // Ez(S) on the domain -> frame (used for the frame of a restored checkpoint)
 __global__ void k_domain_frame(DevGrid g, int ns, const float* __restrict__ E,
                                float* __restrict__ frame, long long fstride) {

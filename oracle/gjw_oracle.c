@@ -46,6 +46,35 @@ int orc_dims_make(int m, int n, int P, int air, orc_dims *g) {
   return 0;
 }
 
+/* 16-bit checkpoint snapshots with per-snapshot scaling (mode 1 = BF16,
+ * 2 = FP16), the format of the device checkpoints: a plane is scaled by 2^sh,
+ * sh from its max |v| (largest value just below 2^15 for FP16, in [1, 2) for
+ * BF16), rounded to nearest even at 8 (BF16) or 11 (FP16) significant bits
+ * with the format's subnormal step (2^-133, 2^-24), and scaled back.  Only
+ * the rounding changes a value: the power-of-two scaling is exact. */
+static int snap_shift(double amax, int mode) {
+  if (!(amax > 0)) return 0;
+  int e;
+  frexp(amax, &e); /* amax = f * 2^e, 0.5 <= f < 1 */
+  return mode == 2 ? 15 - e : 1 - e;
+}
+
+static double snap_round(double v, int sh, int mode) {
+  const double x = ldexp(v, sh);
+  if (x == 0 || !isfinite(x)) return v;
+  int e;
+  frexp(fabs(x), &e); /* |x| in [2^(e-1), 2^e) */
+  const int bits = mode == 2 ? 11 : 8, qmin = mode == 2 ? -24 : -133;
+  int qe = (e - 1) - (bits - 1);
+  if (qe < qmin) qe = qmin;
+  const double q = ldexp(1.0, qe);
+  return ldexp(nearbyint(x / q) * q, -sh);
+}
+
+/* exported for the tests of the rounding itself */
+int orc_snap_shift(double amax, int mode) { return snap_shift(amax, mode); }
+double orc_snap_round(double v, int sh, int mode) { return snap_round(v, sh, mode); }
+
 #define REAL double
 #define SFX _d
 #include "gjw_oracle_impl.h"
@@ -111,7 +140,7 @@ int orc_survey_grad_f(const orc_dims *g, const orc_phys *ph, const float *eps,
       int rc = orc_grad_exact_f(g, ph, eps, sig, src[2 * s], src[2 * s + 1],
                                 wv + (size_t)s * ph->nt, nr, Mw + (size_t)s * nr,
                                 e_all + (size_t)s * ph->nt * nr, K, 0,
-                                ge + (size_t)s * npad, gs + (size_t)s * npad, NULL);
+                                ge + (size_t)s * npad, gs + (size_t)s * npad, NULL, 0);
       if (rc) fail = rc;
     }
   }

@@ -818,15 +818,53 @@ static REAL *FN(aux_ptr)(TY(orc_state) * s, int id) {
   }
 }
 
+/* Round a checkpoint state as a 16-bit snapshot (snap_round), with one
+ * scale per block: a grid column of Ez, Hx or Hy, or 256 consecutive
+ * split-field cells (column-major cell order) of the Ezx / Ezy values the
+ * assembly reads (the other aux entries never reach Ez). */
+static void FN(snap_state)(const orc_dims *g, const TY(orc_cellmap) * q,
+                           TY(orc_state) * s, int mode) {
+  const int NX = g->NX, NY = g->NY;
+  const size_t nE = (size_t)(NX + 1) * (NY + 1);
+  REAL *pl[3] = {s->Ez, s->Hx, s->Hy};
+  const size_t rows[3] = {(size_t)NX + 1, (size_t)NX + 1, (size_t)NX};
+  const int cols[3] = {NY + 1, NY, NY + 1};
+  for (int p = 0; p < 3; ++p)
+    for (int j = 0; j < cols[p]; ++j) {
+      REAL *c = pl[p] + (size_t)j * rows[p];
+      double amax = 0;
+      for (size_t i = 0; i < rows[p]; ++i) amax = fmax(amax, fabs((double)c[i]));
+      const int sh = snap_shift(amax, mode);
+      for (size_t i = 0; i < rows[p]; ++i) c[i] = (REAL)snap_round((double)c[i], sh, mode);
+    }
+  REAL **cell = (REAL **)malloc(nE * sizeof(REAL *));
+  if (!cell) return;
+  for (int w = 0; w < 2; ++w) {
+    size_t n = 0;
+    for (size_t k = 0; k < nE; ++k)
+      if (q->kind[k] == 2)
+        cell[n++] = FN(aux_ptr)(s, w ? q->yid[k] : q->xid[k]) + (w ? q->yoff[k] : q->xoff[k]);
+    for (size_t b0 = 0; b0 < n; b0 += 256) {
+      const size_t b1 = b0 + 256 < n ? b0 + 256 : n;
+      double amax = 0;
+      for (size_t t = b0; t < b1; ++t) amax = fmax(amax, fabs((double)*cell[t]));
+      const int sh = snap_shift(amax, mode);
+      for (size_t t = b0; t < b1; ++t) *cell[t] = (REAL)snap_round((double)*cell[t], sh, mode);
+    }
+  }
+  free(cell);
+}
+
 /* g_eps_pad, g_sig_pad: NX x NY; g_w: nt (dJ/dw, the source-wavelet
  * gradient); any may be NULL.  pml_terms = 0 drops the strip-coefficient
  * eps terms (the strips then act as a fixed absorber). K = recompute
- * segment length (checkpoint interval). */
+ * segment length (checkpoint interval).  snap = 1 / 2 stores the checkpoints
+ * as BF16 / FP16 snapshots (snap_state), 0 keeps them exact. */
 int FN(orc_grad_exact)(const orc_dims *g, const orc_phys *ph, const REAL *eps,
                        const REAL *sig, int si, int sj, const REAL *wvlet,
                        int nr, const int *Mw, const REAL *e, int K,
                        int pml_terms, double *g_eps_pad, double *g_sig_pad,
-                       double *g_w) {
+                       double *g_w, int snap) {
   const int NX = g->NX, NY = g->NY, nt = ph->nt;
   const size_t le = (size_t)NX + 1, lh = (size_t)NX;
   const size_t nE = le * (NY + 1), nHx = le * NY, nHy = lh * (NY + 1);
@@ -857,7 +895,14 @@ int FN(orc_grad_exact)(const orc_dims *g, const orc_phys *ph, const REAL *eps,
   if (g_w) memset(g_w, 0, sizeof(double) * nt);
   /* forward sweep, keeping checkpoints */
   for (int it = 0; it < nt; ++it) {
-    if (it % K == 0) memcpy(ck + (size_t)(it / K) * SL, s0.buf, SL * sizeof(REAL));
+    if (it % K == 0) {
+      memcpy(ck + (size_t)(it / K) * SL, s0.buf, SL * sizeof(REAL));
+      if (snap) {
+        TY(orc_state) cv;
+        FN(state_rebase)(&s0, ck + (size_t)(it / K) * SL, &cv);
+        FN(snap_state)(g, &q, &cv, snap);
+      }
+    }
     FN(step_literal)(g, &c, &s0, si, sj, wvlet[it], 1);
   }
   const int sk = (int)IE(si, sj);

@@ -260,11 +260,13 @@ def grad_ref(g: Grid, eps_pad, sig_pad, src, Mw, e, u, dtype=np.float64):
 
 
 def grad_exact(g: Grid, eps_pad, sig_pad, src, wvlet, Mw, e, K=None, pml_terms=True,
-               dtype=np.float64):
+               dtype=np.float64, snap=0):
     """Exact discrete adjoint of w_solve.m.
 
     Returns (g_eps_pad, g_sig_pad, g_w): dJ/d eps_r_z, dJ/d sigma_e_z (NX x NY)
-    and dJ/d wavelet (nt), for dJ/dd = e (nt x nr)."""
+    and dJ/d wavelet (nt), for dJ/dd = e (nt x nr).  snap = 1 / 2 stores the
+    checkpoints as BF16 / FP16 snapshots with per-plane power-of-two scaling
+    (the device checkpoint format), 0 keeps them exact."""
     sfx, ct = _real(dtype)
     d_, p = g._c()
     if K is None:
@@ -280,9 +282,25 @@ def grad_exact(g: Grid, eps_pad, sig_pad, src, wvlet, Mw, e, K=None, pml_terms=T
     rc = getattr(lib(), "orc_grad_exact" + sfx)(
         C.byref(d_), C.byref(p), _ptr(ep, ct), _ptr(sp, ct), int(src[0]), int(src[1]),
         _ptr(w, ct), mw.size, _ptr(mw, C.c_int), _ptr(ee, ct), int(K), int(bool(pml_terms)),
-        _ptr(ge, C.c_double), _ptr(gs, C.c_double), _ptr(gw, C.c_double))
+        _ptr(ge, C.c_double), _ptr(gs, C.c_double), _ptr(gw, C.c_double), int(snap))
     assert rc == 0, rc
     return ge, gs, gw
+
+
+def snap_round(v, mode, amax=None):
+    """Values ``v`` stored as one 16-bit snapshot plane (mode 1 = BF16,
+    2 = FP16) and read back: power-of-two scaling from max |v|, round to
+    nearest even, scale back (the C oracle's snap_round / snap_shift)."""
+    L = lib()
+    L.orc_snap_round.restype = C.c_double
+    L.orc_snap_round.argtypes = [C.c_double, C.c_int, C.c_int]
+    L.orc_snap_shift.restype = C.c_int
+    L.orc_snap_shift.argtypes = [C.c_double, C.c_int]
+    v = np.asarray(v, dtype=np.float64)
+    a = float(np.abs(v).max()) if amax is None else float(amax)
+    sh = L.orc_snap_shift(a, int(mode))
+    out = np.array([L.orc_snap_round(float(x), sh, int(mode)) for x in v.ravel()])
+    return out.reshape(v.shape), sh
 
 
 def fold_pad(g: Grid, g_eps_pad, g_sig_pad):

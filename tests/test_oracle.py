@@ -249,3 +249,51 @@ def test_golden_fixtures(O):
     ge, gs, gw = O.grad_exact(g, ep, sp, tuple(z["src"]), z["w"], z["Mw"], z["e"], K=5)
     np.testing.assert_allclose(ge, z["g_eps_pad"], rtol=1e-10, atol=1e-12 * np.abs(z["g_eps_pad"]).max())
     np.testing.assert_allclose(gs, z["g_sig_pad"], rtol=1e-10, atol=1e-12 * np.abs(z["g_sig_pad"]).max())
+
+
+# --------------------------------------------------------------------------
+# 16-bit checkpoint snapshots
+# --------------------------------------------------------------------------
+def test_snapshot_rounding_matches_library_conversions(O):
+    """The snapshot rounding equals numpy's float16 and torch's bfloat16
+    round-to-nearest-even conversions of the power-of-two-scaled values, and
+    the scale puts the plane's largest value just below 2^15 (FP16) or in
+    [1, 2) (BF16)."""
+    import torch
+    rng = np.random.default_rng(11)
+    for scale in (1e-7, 3.0, 2.5e4):
+        v = (rng.standard_normal(4000) * scale).astype(np.float32).astype(np.float64)
+        v[:8] = [0.0, -0.0, 5e-12 * scale, -3e-9 * scale, 1e-30, v[8], -v[9], v[10]]
+        a = np.abs(v).max()
+        r16, sh16 = O.snap_round(v, 2)
+        assert 2.0 ** 14 <= a * 2.0 ** sh16 < 2.0 ** 15
+        ref16 = np.ldexp(np.ldexp(v, sh16).astype(np.float32).astype(np.float16).astype(np.float64), -sh16)
+        np.testing.assert_array_equal(r16, ref16)
+        rb, shb = O.snap_round(v, 1)
+        assert 1.0 <= a * 2.0 ** shb < 2.0
+        xb = torch.from_numpy(np.ldexp(v, shb).astype(np.float32)).to(torch.bfloat16).to(torch.float64)
+        np.testing.assert_array_equal(rb, np.ldexp(xb.numpy(), -shb))
+    # ties go to even: 1 + 2^-8 (BF16) and 1 + 2^-11 (FP16) round down, 1 + 3*2^-8 up
+    t = np.array([1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, 1.5])
+    rb, _ = O.snap_round(t, 1, amax=1.5)
+    np.testing.assert_array_equal(rb, [1.0, 1.0 + 4 * 2.0 ** -8, 1.5])
+    t = np.array([1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11])
+    r16, sh = O.snap_round(t, 2, amax=1.5)
+    assert sh == 14
+    np.testing.assert_array_equal(r16, [1.0, 1.0 + 4 * 2.0 ** -11])
+
+
+def test_oracle_snapshot_gradient(O):
+    """Checkpoints stored as FP16 / BF16 snapshots perturb the exact gradient
+    by about the formats' rounding (2^-12, 2^-9 relative), and the rounding
+    does reach the recomputed segments (the gradient changes)."""
+    g, eps, sig, w, s, Mw = exact_setup(O)
+    ep, sp = O.exp2pml(g, eps, sig)
+    e = np.random.default_rng(5).standard_normal((g.nt, Mw.size))
+    ref = O.grad_exact(g, ep, sp, s, w, Mw, e, K=13, pml_terms=False)
+    for mode, tol in ((2, 1e-3), (1, 1e-2)):
+        got = O.grad_exact(g, ep, sp, s, w, Mw, e, K=13, pml_terms=False, snap=mode)
+        for a, b in zip(got[:2], ref[:2]):
+            err = np.linalg.norm(a - b) / np.linalg.norm(b)
+            print("snap", mode, "rel err", err)
+            assert 0 < err <= tol, (mode, err)
