@@ -16,6 +16,7 @@ GJW_OK = 0
 GJW_PATH_BATCHED, GJW_PATH_PLAIN = 0, 1
 GJW_OPT_PATH, GJW_OPT_CKPT_EVERY, GJW_OPT_TILE_L, GJW_OPT_SNAPSHOT, GJW_OPT_KTIMING = 1, 2, 3, 4, 5
 GJW_OPT_FUSE = 6
+GJW_OPT_GRAPH = 7
 GJW_SNAP_F32, GJW_SNAP_BF16, GJW_SNAP_F16 = 0, 1, 2
 GJW_MISFIT_L2 = 0
 GJW_GRAD_EXACT, GJW_GRAD_REF = 0, 1

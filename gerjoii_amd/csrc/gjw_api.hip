@@ -163,6 +163,8 @@ struct gjw_survey {
   DBuf<float> fld;   // forward: E0,Hx0,Hy0,E1,Hx1,Hy1,X,Y  (8 * ns * FS)
   DBuf<float> adjf;  // adjoint: M0,Nx0,Ny0,X0,Y0,M1,Nx1,Ny1,X1,Y1 (10 * ns * FS)
   DBuf<float> tr, adj, obs;
+  DBuf<float> jzb;   // source recorder [shot][nt] (gjw_forward with jz)
+  bool want_jz = false;
   DBuf<float> ck;    // checkpoints (FP32, or 16-bit planes + shifts; see ck_stride)
   DBuf<float> frames;
   DBuf<float> acc;
@@ -174,6 +176,11 @@ struct gjw_survey {
   // sampled per-kernel timing (GJW_OPT_KTIMING): event pairs around every
   // ksample-th launch of k_fwd_step / k_adj_step, resolved after each call
   int ksample = 0;
+  // GJW_OPT_GRAPH: the captured batched FWI gradient and the buffers it uses
+  int graph_on = 0, graph_calls = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uintptr_t> graph_key;
   std::vector<hipEvent_t> kev_pool;
   std::vector<std::pair<int, int>> kev_used;  // (kernel id, index of start event)
   double kt_ms[4] = {0, 0, 0, 0};  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2
@@ -251,12 +258,14 @@ extern "C" int gjw_survey_destroy(gjw_survey* s) {
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
   for (auto e : s->kev_pool) (void)hipEventDestroy(e);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
   s->coef.release(); s->coef_ref.release(); s->zero_sc.release(); s->trash.release();
   s->fe.release(); s->fs.release(); s->d_pml.release();
   s->d_si.release(); s->d_sj.release(); s->d_cptr.release(); s->d_crow.release();
   s->d_cid.release(); s->d_trs.release(); s->d_trr.release(); s->d_trc.release();
   s->d_sc.release(); s->d_wav.release(); s->fld.release(); s->adjf.release();
-  s->tr.release(); s->adj.release(); s->obs.release(); s->ck.release();
+  s->tr.release(); s->adj.release(); s->obs.release(); s->ck.release(); s->jzb.release();
   s->frames.release(); s->acc.release(); s->ge.release(); s->gsg.release();
   s->part.release(); s->Jd.release(); s->gw.release();
   delete s;
@@ -279,6 +288,11 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       if (v < 0 || v > 4096) return fail(GJW_ERR_ARG, "GJW_OPT_TILE_L out of range");
       s->tile_L = (int)v;
       break;
+    case GJW_OPT_GRAPH:
+      if (v != 0 && v != 1) return fail(GJW_ERR_ARG, "GJW_OPT_GRAPH must be 0 or 1");
+      s->graph_on = (int)v;
+      s->graph_calls = 0;
+      return GJW_OK;
     case GJW_OPT_FUSE:
       if (v != 1 && v != 2) return fail(GJW_ERR_ARG, "GJW_OPT_FUSE must be 1 or 2");
       s->fuse = (int)v;
@@ -624,8 +638,9 @@ static DevGrid dev_grid(const gjw_survey* s) {
   return g;
 }
 
-// shots [q0, q0+cnt) of the survey
-static DevShots dev_shots(const gjw_survey* s, int q0) {
+// shots [q0, q0+cnt) of the survey; jz: record the source cell (forward sweeps
+// of gjw_forward that asked for it)
+static DevShots dev_shots(const gjw_survey* s, int q0, bool jz = false) {
   DevShots d;
   d.nt = s->gr.nt;
   d.si = s->d_si.p + q0;
@@ -637,6 +652,7 @@ static DevShots dev_shots(const gjw_survey* s, int q0) {
   d.cid = s->d_cid.p;
   d.rmin = s->rrow_min;
   d.rmax = s->rrow_max;
+  d.jz = jz ? s->jzb.p + (long long)q0 * s->gr.nt : nullptr;
   return d;
 }
 
@@ -1007,7 +1023,7 @@ static int forward_batched(gjw_survey* s, int keep) {
     TRY(s->ck.ensure((size_t)s->ck_n * ck_stride(s)));
   }
   DevGrid g = dev_grid(s);
-  DevShots sh = dev_shots(s, 0);
+  DevShots sh = dev_shots(s, 0, s->want_jz);
   FwdBufs b = fwd_bufs(s, ns);
   int cur = 0, xc = 0;
   HIPCHK(hipEventRecord(s->ctx->ev0, st));
@@ -1035,7 +1051,7 @@ static int forward_plain_shot(gjw_survey* s, int q, float* fr) {
   const int L = s->tile_L > 0 ? s->tile_L : 16;
   DevGrid g = dev_grid(s);
   FwdBufs b = fwd_bufs(s, 1);
-  DevShots sh = dev_shots(s, q);
+  DevShots sh = dev_shots(s, q, s->want_jz);
   TRY(zero_fwd_state(s, 1));
   if (fr) TRY(launch_fill(fr, FD, 0.f, s->ctx->stream));  // frame 0 = Ez(S_0) = 0
   int cur = 0;
@@ -1073,14 +1089,20 @@ static int finish_timing(gjw_survey* s) {
 
 extern "C" int gjw_forward(gjw_survey* s, float* traces, float* jz, int32_t keep) {
   TRY(check_ready(s, "gjw_forward"));
-  if (jz) return fail(GJW_ERR_ARG, "gjw_forward: jz output is not built yet; pass NULL");
   s->have_adj = false;
-  TRY(s->path == GJW_PATH_PLAIN ? forward_plain(s, keep) : forward_batched(s, keep));
+  s->want_jz = jz != nullptr;
+  if (jz) TRY(s->jzb.ensure((size_t)s->ns * s->gr.nt));
+  const int rc = s->path == GJW_PATH_PLAIN ? forward_plain(s, keep) : forward_batched(s, keep);
+  s->want_jz = false;
+  TRY(rc);
   TRY(finish_timing(s));
   if (traces) {
     HIPCHK(hipMemcpyAsync(traces, s->tr.p, (size_t)s->ntr * s->gr.nt * 4, hipMemcpyDefault,
                           s->ctx->stream));
   }
+  if (jz)
+    HIPCHK(hipMemcpyAsync(jz, s->jzb.p, (size_t)s->ns * s->gr.nt * 4, hipMemcpyDefault,
+                          s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
   return GJW_OK;
 }
@@ -1393,6 +1415,110 @@ extern "C" int gjw_gradient(gjw_survey* s, const float* adj, int32_t mode, doubl
   return grad_run(s, adj, mode, g_eps, g_sig, g_w, a_max);
 }
 
+// Batched FWI gradient in two parts.  fwi_batched_prepare allocates every
+// buffer and runs the autotuner (host synchronisation); fwi_batched_enqueue
+// then only enqueues kernels and device copies on the survey stream (forward
+// sweep with checkpoints, L2 residual and misfit, adjoint sweep with
+// recomputation), so it can be captured as one hipGraph.  The observed traces
+// are expected in s->obs.
+static int fwi_batched_prepare(gjw_survey* s) {
+  const int ns = s->ns, nt = s->gr.nt;
+  const long long n = (long long)s->ntr * nt, per = (long long)s->gr.m * s->gr.n;
+  const long long FD = (long long)s->LD * s->gr.n;
+  TRY(ensure_fwd_state(s, ns));
+  TRY(s->tr.ensure(std::max<long long>(n, 1)));
+  TRY(zero_fwd_state(s, ns));
+  TRY(autotune_L(s, ns));
+  s->ck_K = ck_interval(s);
+  s->ck_n = (int)cdiv(nt, s->ck_K);
+  TRY(s->ck.ensure((size_t)s->ck_n * ck_stride(s)));
+  TRY(s->adj.ensure(std::max<long long>(n, 1)));
+  TRY(s->obs.ensure(std::max<long long>(n, 1)));
+  TRY(s->part.ensure(1024));
+  TRY(s->Jd.ensure(1));
+  TRY(s->ge.ensure(per));
+  TRY(s->gsg.ensure(per));
+  TRY(s->frames.ensure((size_t)ns * (s->ck_K + 1) * FD));
+  TRY(s->acc.ensure((size_t)2 * ns * FD));
+  TRY(s->adjf.ensure((size_t)10 * ns * s->FS));
+  return GJW_OK;
+}
+
+static int fwi_batched_enqueue(gjw_survey* s) {
+  const int ns = s->ns, nt = s->gr.nt;
+  const long long n = (long long)s->ntr * nt, per = (long long)s->gr.m * s->gr.n;
+  hipStream_t st = s->ctx->stream;
+  DevGrid g = dev_grid(s);
+  DevShots sh = dev_shots(s, 0);
+  FwdBufs b = fwd_bufs(s, ns);
+  TRY(zero_fwd_state(s, ns));
+  int cur = 0, xc = 0;
+  for (int it = 0; it < nt;) {
+    if (it % s->ck_K == 0) TRY(ck_save(s, b, cur, xc, it / s->ck_K));
+    int taken = 0;
+    TRY(fwd_advance(s, g, sh, ns, b, cur, xc, it, std::min(nt - it, s->ck_K - it % s->ck_K),
+                    s->tr.p, nullptr, 0, 0, &taken));
+    it += taken;
+  }
+  s->cur = cur;
+  const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(n, 256), 1), 1024);
+  hipLaunchKernelGGL(k_resid_l2, dim3(blocks), dim3(256), 0, st, s->tr.p, s->obs.p, s->adj.p, n,
+                     s->part.p);
+  hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, blocks, 0.5, s->Jd.p);
+  HIPCHK(hipGetLastError());
+  TRY(launch_fill_d(s->ge.p, per, 0.0, st));
+  TRY(launch_fill_d(s->gsg.p, per, 0.0, st));
+  return grad_batched(s, s->adj.p, nullptr);
+}
+
+// every device address and launch parameter the enqueued sequence bakes in
+static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
+  auto u = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
+  auto fbits = [](float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return (uintptr_t)b;
+  };
+  return {u(s->fld.p),    u(s->tr.p),    u(s->ck.p),     u(s->adj.p),   u(s->obs.p),
+          u(s->part.p),   u(s->Jd.p),    u(s->ge.p),     u(s->gsg.p),   u(s->frames.p),
+          u(s->acc.p),    u(s->adjf.p),  u(s->coef.p),   u(s->fe.p),    u(s->fs.p),
+          u(s->d_pml.p),  u(s->d_si.p),  u(s->d_sj.p),   u(s->d_sc.p),  u(s->d_wav.p),
+          u(s->d_cptr.p), u(s->d_crow.p), u(s->d_cid.p), u(s->d_trs.p), u(s->d_trr.p),
+          u(s->d_trc.p),  u(s->trash.p), (uintptr_t)s->ns, (uintptr_t)s->ntr,
+          (uintptr_t)s->gr.nt, (uintptr_t)s->tuned_L, (uintptr_t)s->tuned_L1,
+          (uintptr_t)s->ck_K, (uintptr_t)s->fuse, (uintptr_t)s->snap,
+          (uintptr_t)s->rrow_min, (uintptr_t)s->rrow_max, (uintptr_t)s->pml_idx.size(),
+          fbits(s->bx0), fbits(s->by0)};
+}
+
+static int fwi_graph_run(gjw_survey* s) {
+  hipStream_t st = s->ctx->stream;
+  const std::vector<uintptr_t> key = fwi_graph_key(s);
+  if (!s->gexec || key != s->graph_key) {
+    if (s->gexec) HIPCHK(hipGraphExecDestroy(s->gexec));
+    if (s->graph) HIPCHK(hipGraphDestroy(s->graph));
+    s->gexec = nullptr;
+    s->graph = nullptr;
+    const int ks = s->ksample;
+    s->ksample = 0;  // no timing events inside the graph
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    const int rc = fwi_batched_enqueue(s);
+    hipGraph_t gr = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &gr);
+    s->ksample = ks;
+    if (rc != GJW_OK) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    HIPCHK(ec);
+    s->graph = gr;
+    HIPCHK(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
+    s->graph_key = key;
+  }
+  HIPCHK(hipGraphLaunch(s->gexec, st));
+  return GJW_OK;
+}
+
 extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps, double* g_sig,
                                 double* J) {
   TRY(check_ready(s, "gjw_fwi_gradient"));
@@ -1414,9 +1540,21 @@ extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps
         rc = host_or_dev_copy(g_sig, s->gsg.p, per * 8, st);
     }
   } else {
-    rc = forward_batched(s, 1);
-    if (rc == GJW_OK) rc = misfit_dev(s, d_obs, GJW_MISFIT_L2);
-    if (rc == GJW_OK) rc = grad_run(s, nullptr, GJW_GRAD_EXACT, g_eps, g_sig, nullptr);
+    rc = fwi_batched_prepare(s);
+    if (rc == GJW_OK) rc = host_or_dev_copy(s->obs.p, d_obs, (long long)s->ntr * s->gr.nt * 4, st);
+    // the first call runs eagerly (and settles the autotuner); with
+    // GJW_OPT_GRAPH later calls replay the captured sequence
+    const bool use_graph = s->graph_on && s->graph_calls++ > 0;
+    if (rc == GJW_OK) rc = use_graph ? fwi_graph_run(s) : fwi_batched_enqueue(s);
+    if (rc == GJW_OK && hipEventRecord(s->ctx->ev1, st) != hipSuccess)
+      rc = fail(GJW_ERR_HIP, "event record");
+    if (rc == GJW_OK) rc = kev_resolve(s);
+    if (rc == GJW_OK) {
+      s->kept = 1;
+      s->have_adj = true;
+      if (g_eps) rc = host_or_dev_copy(g_eps, s->ge.p, per * 8, st);
+      if (rc == GJW_OK && g_sig) rc = host_or_dev_copy(g_sig, s->gsg.p, per * 8, st);
+    }
   }
   if (rc != GJW_OK) {
     (void)hipEventDestroy(e0);

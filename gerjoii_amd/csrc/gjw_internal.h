@@ -47,6 +47,7 @@ struct DevShots {
   const int* crow;   // receiver row
   const int* cid;    // global trace index; trace t occupies tr[t*nt .. t*nt+nt)
   int rmin, rmax;    // receiver rows over all shots (waves outside skip the lookup)
+  float* jz;         // source recorder Ez(source) per step, [shot][nt], or null
 };
 
 enum { KIND_RING = 0, KIND_INNER = 1, KIND_PML = 2 };

@@ -210,7 +210,10 @@ k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
   }
   // per-column epilogue: source, stores, receivers, domain frame
   auto emit = [&](int j, float en, float hxn, float hyn) {
-    if (j == sj && i == si) en += wsrc;  // soft source (w_solve.m:92)
+    if (j == sj && i == si) {
+      en += wsrc;                                       // soft source (w_solve.m:92)
+      if (s.jz) s.jz[(long long)shot * s.nt + it] = en;  // source recorder (w_solve.m:104)
+    }
     if (ok) {
       const long long k = (long long)j * LD + i;
       E1[k] = en;
@@ -370,6 +373,9 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
   const float bx0 = g.bx0, by0 = g.by0;
   const long long ic = okrow ? i : 0;  // address row for loads of valid lanes
   const long long FD = (long long)LD * g.n;
+  // source recorder (w_solve.m:104): Ez at the source cell after each step
+  const __amdgpu_buffer_rsrc_t rJZ =
+      rsrc_of(s.jz ? s.jz + (long long)shot * s.nt : nullptr, s.jz ? s.nt : 0);
 
   const __amdgpu_buffer_rsrc_t rE0 = rsrc_of(const_cast<float*>(E0), g.FS),
                                rHx0 = rsrc_of(const_cast<float*>(Hx0), g.FS),
@@ -403,22 +409,18 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
   auto pml_cell = [&](int c) {
     return notring && c >= 1 && c <= NY - 1 && (e_pml || c <= P || c >= NY - P);
   };
-  // top-strip coefficients depend on the row only: read them once (column P+1)
-  float hay_r = 1.f, hby_r = 0.f, cxe_r = 1.f, cxh_r = 0.f, cyh_r = 0.f;
-  if (!inner_rows) {
-    const unsigned kr = (unsigned)((P + 1) * LD + ic);
-    hay_r = h_xn ? bload(rHAY, kr, true) : 1.f;
-    hby_r = h_xn ? bload(rHBY, kr, true) : 0.f;
-    cxe_r = e_xn ? bload(rCXE, kr, true) : 1.f;
-    cxh_r = e_xn ? bload(rCXH, kr, true) : 0.f;
-    cyh_r = e_xn ? bload(rCYH, kr, true) : 0.f;
-  }
+  // top-strip coefficients depend on the row only: read them once
+  // (from the coefficient table at column P+1, loaded without a branch)
+  const bool rows_on = okrow && !inner_rows;
+  const unsigned kr = (unsigned)((P + 1) * LD + ic);
+  const float4 kr0 = bload4(rU0, 4 * kr, rows_on), kr1 = bload4(rU1, 4 * kr, rows_on);
+  const float kr2 = bload(rU2, kr, rows_on);
+  const float cxe_r = rows_on ? kr0.y : 1.f, cxh_r = rows_on ? kr0.z : 0.f;
+  const float cyh_r = rows_on ? kr1.x : 0.f;
   // Hy coefficients of rows whose Hy update does not depend on the column
-  const float hay_row = h_xn ? hay_r : 1.f;
-  const float hby_row = h_xn ? hby_r : (h_inner ? by0 : 0.f);
+  const float hay_row = rows_on ? kr1.w : 1.f;
+  const float hby_row = rows_on ? kr2 : by0;
 
-  auto ldE = [&](int j) { return (okrow && j >= 0 && j <= NY) ? E0[(long long)j * LD + ic] : 0.f; };
-  auto ldHx = [&](int j) { return (okrow && j >= 0 && j < NY) ? Hx0[(long long)j * LD + ic] : 0.f; };
   // receiver samples of column j (0 <= j <= NY)
   auto record = [&](int j, float e, int step) {
     const int qa = cp[j], qb = cp[j + 1];
@@ -426,11 +428,18 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
       if (s.crow[q] == i) tr[(long long)s.cid[q] * s.nt + step] = e;
   };
 
-  // ---- prologue: step 1 starts at column jb-1; it needs Hx^{n+1}(jb-2)
+  // ---- prologue: step 1 starts at column jb-1; it needs Hx^{n+1}(jb-2).
+  // Branch-free loads, so they overlap with the first chunk's.
   const int j1 = jb - 1;
-  float e0_c = ldE(j1);  // E^n(i, j)
-  float hx1_prev = 0.f;  // Hx^{n+1}(i, j-1)
-  if (j1 - 1 >= 0 && okrow) hx1_prev = hx_new(g, i, j1 - 1, ldHx(j1 - 1), ldE(j1 - 1), e0_c);
+  float e0_c, hx1_prev;  // E^n(i, j), Hx^{n+1}(i, j-1)
+  {
+    const unsigned ka = (unsigned)(j1 * LD + ic), kb = ka - (unsigned)LD;
+    const bool ina = okrow && j1 >= 0 && j1 <= NY, inb = okrow && j1 - 1 >= 0 && j1 - 1 <= NY;
+    e0_c = bload(rE0, ka, ina);
+    const float eb = bload(rE0, kb, inb), hxb = bload(rHx0, kb, inb && j1 - 1 < NY);
+    const float4 kbb = bload4(rU1, 4 * kb, inb);  // (cyh, AX, BX, AY) of column j1-1
+    hx1_prev = kbb.y * hxb + kbb.z * (e0_c - eb);
+  }
   // step-2 carries (column j-1 of step 1): E, Hx, Hy, X, Y at n+1; Hx^{n+2}(j-2);
   // and the column's coefficients
   float e1_prev = 0.f, hx1_c2 = 0.f, hy1_c2 = 0.f, x1_c2 = 0.f, y1_c2 = 0.f, hx2_prev = 0.f;
@@ -491,6 +500,8 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
       const float y2 = ka2.w * y1_c2 + kb2.x * dhx2;
       float e2 = ka2.x * e1_prev + x2 + y2;
       e2 += (c2 == sj && i == si) ? ws2 : 0.f;
+      bstore(e1, rJZ, (unsigned)it, own && i == si && c1 == sj && c1 >= jb && c1 < je);
+      bstore(e2, rJZ, (unsigned)(it + 1), own && i == si && c2 == sj && c2 >= jb && c2 < je);
       const bool o2 = own && c2 >= jb && c2 < je;
       const bool pm2 = pml_cell(c2);
       const unsigned k2 = (unsigned)(c2 * LD + ic);
@@ -569,6 +580,8 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
       const float hy2u = shfl_up1(hy2);
       float e2 = cc_c2.x * e1_prev + cc_c2.y * ((hy2 - hy2u) - (hx2 - hx2_prev));
       e2 += (c2 == sj && i == si) ? ws2 : 0.f;
+      bstore(e1, rJZ, (unsigned)it, own && i == si && c1 == sj && c1 >= jb && c1 < je);
+      bstore(e2, rJZ, (unsigned)(it + 1), own && i == si && c2 == sj && c2 >= jb && c2 < je);
       const bool o2 = own && c2 >= jb && c2 < je;
       const unsigned k2 = (unsigned)(c2 * LD + i);
       bstore(e2, rE2, k2, o2);
@@ -603,11 +616,11 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
       c_[u] = make_float2(bload(rCC, 2 * kk, e_inner), bload(rCC, 2 * kk + 1, e_inner));
       x_[u] = bload(rX0, kk, e_pml);
       y_[u] = bload(rY0, kk, e_pml);
-      cxe_[u] = e_xp ? bload(rCXE, kk, true) : cxe_r;
-      cxh_[u] = e_xp ? bload(rCXH, kk, true) : cxh_r;
-      cyh_[u] = e_xp ? bload(rCYH, kk, true) : cyh_r;
-      hay_[u] = h_xp ? bload(rHAY, kk, true) : hay_row;
-      hby_[u] = h_xp ? bload(rHBY, kk, true) : hby_row;
+      cxe_[u] = e_xp ? bload(rCXE, kk, e_xp) : cxe_r;
+      cxh_[u] = e_xp ? bload(rCXH, kk, e_xp) : cxh_r;
+      cyh_[u] = e_xp ? bload(rCYH, kk, e_xp) : cyh_r;
+      hay_[u] = h_xp ? bload(rHAY, kk, h_xp) : hay_row;
+      hby_[u] = h_xp ? bload(rHBY, kk, h_xp) : hby_row;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -633,6 +646,8 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
       const float y2 = y1_c2 + cyh_c2 * dhx2;
       float e2 = e_inner ? cc_c2.x * e1_prev + cc_c2.y * (dhy2 - dhx2) : (e_pml ? x2 + y2 : e1_prev);
       e2 += (c2 == sj && i == si) ? ws2 : 0.f;
+      bstore(e1, rJZ, (unsigned)it, own && i == si && c1 == sj && c1 >= jb && c1 < je);
+      bstore(e2, rJZ, (unsigned)(it + 1), own && i == si && c2 == sj && c2 >= jb && c2 < je);
       const bool o2 = own && c2 >= jb && c2 < je;
       const unsigned k2 = (unsigned)(c2 * LD + ic);
       bstore(e2, rE2, k2, o2);
@@ -947,23 +962,6 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
 // with me1 = M(S_{it+1}) (input) and me2 = M(S_it) (after step 1 and its
 // residual injection).  Requires it >= 1.
 
-// W of an Ez cell from its values (the weights its adjoint hands to Hy, Hx).
-__device__ __forceinline__ void w_cell(const DevGrid& g, int i, int j, bool okrow, float me,
-                                       float x, float y, float& wy, float& wx) {
-  wy = 0.f;
-  wx = 0.f;
-  if (!okrow || j < 0 || j > g.NY) return;
-  const int kd = ekind(g.NX, g.NY, g.P, i, j);
-  const long long k = (long long)j * g.LD + i;
-  if (kd == KIND_INNER) {
-    const float c = g.cc[k].y;
-    wy = c * me;
-    wx = -c * me;
-  } else if (kd == KIND_PML) {
-    wy = g.cxh[k] * (x + me);
-    wx = g.cyh[k] * (y + me);
-  }
-}
 
 // it: the later of the two forward steps transposed (it >= 1).  M0 holds
 // dJ/dEz(S_{it+1}) with the residual of step it already added; M2 receives
@@ -1044,22 +1042,17 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
   auto pml_cell = [&](int c) {
     return notring && c >= 1 && c <= NY - 1 && (e_pml || c <= P || c >= NY - P);
   };
-  float hay_r = 1.f, hby_r = 0.f, cxe_r = 1.f, cxh_r = 0.f, cyh_r = 0.f;
-  if (!inner_rows) {
-    const unsigned kr = (unsigned)((P + 1) * LD + ic);
-    hay_r = h_xn ? bload(rHAY, kr, true) : 1.f;
-    hby_r = h_xn ? bload(rHBY, kr, true) : 0.f;
-    cxe_r = e_xn ? bload(rCXE, kr, true) : 1.f;
-    cxh_r = e_xn ? bload(rCXH, kr, true) : 0.f;
-    cyh_r = e_xn ? bload(rCYH, kr, true) : 0.f;
-  }
-  const float hay_row = h_xn ? hay_r : 1.f;
-  const float hby_row = h_xn ? hby_r : (h_inner ? by0 : 0.f);
+  // (from the coefficient table at column P+1, loaded without a branch)
+  const bool rows_on = okrow && !inner_rows;
+  const unsigned kr = (unsigned)((P + 1) * LD + ic);
+  const float4 kr0 = bload4(rU0, 4 * kr, rows_on), kr1 = bload4(rU1, 4 * kr, rows_on);
+  const float kr2 = bload(rU2, kr, rows_on);
+  const float cxe_r = rows_on ? kr0.y : 1.f, cxh_r = rows_on ? kr0.z : 0.f;
+  const float cyh_r = rows_on ? kr1.x : 0.f;
+  // Hy coefficients of rows whose Hy update does not depend on the column
+  const float hay_row = rows_on ? kr1.w : 1.f;
+  const float hby_row = rows_on ? kr2 : by0;
 
-  auto ldM = [&](int j) { return (okrow && j >= 0 && j <= NY) ? M0[(long long)j * LD + ic] : 0.f; };
-  auto ldNx = [&](int j) { return (okrow && j >= 0 && j < NY) ? Nx0[(long long)j * LD + ic] : 0.f; };
-  auto ldX = [&](int j) { return pml_cell(j) ? X0[(long long)j * LD + ic] : 0.f; };
-  auto ldY = [&](int j) { return pml_cell(j) ? Y0[(long long)j * LD + ic] : 0.f; };
   // data residual of forward step `step` at the receivers of column c
   auto inject = [&](int c, float& mn, int step) {
     if (has_rec && step >= 0 && c >= 0 && c <= NY) {
@@ -1068,37 +1061,39 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
         if (s.crow[q] == i) mn += adj[(long long)s.cid[q] * s.nt + step];
     }
   };
-  // class-path coefficients of column j (interior columns): the split-field
-  // and inner coefficients the W and aux updates of column j use
-  auto coef_col = [&](int j, float& ce, float& cy, float& cxe, float& cxh, float& cyh) {
-    const bool jin = j >= 1 && j <= NY - 1;
-    const long long k = (long long)(jin ? j : 0) * LD + ic;
-    const float2 c = (jin && e_inner) ? g.cc[k] : make_float2(0.f, 0.f);
-    ce = c.x;
-    cy = c.y;
-    cxe = (jin && e_xp) ? g.cxe[k] : cxe_r;
-    cxh = (jin && e_xp) ? g.cxh[k] : cxh_r;
-    cyh = (jin && e_xp) ? g.cyh[k] : cyh_r;
-  };
+
 
   // ---- state at the start of step-1 column j (step 2 at j-1):
   // step 1: M0, W0, X0, Y0 at j; px1 at j-1; coefficients of column j
   // step 2: M after step 1 at j-1 and its W; Ny, Nx, X, Y after step 1 at
   //         j-1; px2 at j-2; M0 at j-1 (correlation); coefficients of j-1
+  // The prologue loads are branch-free so that they overlap with the first
+  // chunk's.
   const int j1 = jb - 1;
-  float me1_c = ldM(j1), x0_c = ldX(j1), y0_c = ldY(j1), wy1_c, wx1_c;
-  w_cell(g, i, j1, okrow, me1_c, x0_c, y0_c, wy1_c, wx1_c);
-  float px1_prev = 0.f;
-  if (j1 - 1 >= 0 && j1 - 1 < NY && okrow) {
-    float wyp, wxp, a, b;
-    w_cell(g, i, j1 - 1, okrow, ldM(j1 - 1), ldX(j1 - 1), ldY(j1 - 1), wyp, wxp);
-    hx_coef(g, i, j1 - 1, a, b);
-    px1_prev = b * (a * ldNx(j1 - 1) + wxp - wx1_c);
+  float me1_c, x0_c, y0_c, wy1_c, wx1_c, px1_prev;
+  float ce1_c, cy1_c, cxe_1c, cxh_1c, cyh_1c;
+  {
+    const unsigned ka = (unsigned)(j1 * LD + ic), kb = ka - (unsigned)LD;
+    const bool ina = okrow && j1 >= 0 && j1 <= NY, inb = okrow && j1 - 1 >= 0 && j1 - 1 <= NY;
+    const bool pa = pml_cell(j1), pb = pml_cell(j1 - 1);
+    me1_c = bload(rM0, ka, ina);
+    x0_c = bload(rX0, ka, pa);
+    y0_c = bload(rY0, ka, pa);
+    const float4 a0 = bload4(rU0, 4 * ka, ina), a1 = bload4(rU1, 4 * ka, ina);
+    const float mb = bload(rM0, kb, inb), yb = bload(rY0, kb, pb);
+    const float nxb = bload(rNx0, kb, inb && j1 - 1 < NY);
+    const float4 b1 = bload4(rU1, 4 * kb, inb);  // (cyh, AX, BX, AY) of column j1-1
+    wy1_c = a0.z * (x0_c + me1_c);
+    wx1_c = a1.x * (y0_c + me1_c);
+    px1_prev = b1.z * (b1.y * nxb + b1.x * (yb + mb) - wx1_c);
+    ce1_c = a0.x;
+    cy1_c = a0.z;
+    cxe_1c = a0.y;
+    cxh_1c = a0.z;
+    cyh_1c = a1.x;
   }
   float me2_c = 0.f, wy2_c = 0.f, wx2_c = 0.f, nyv1_c2 = 0.f, nxv1_c2 = 0.f, x1_c2 = 0.f,
         y1_c2 = 0.f, px2_prev = 0.f, me1_c2 = 0.f;
-  float ce1_c, cy1_c, cxe_1c, cxh_1c, cyh_1c;
-  coef_col(j1, ce1_c, cy1_c, cxe_1c, cxh_1c, cyh_1c);
   float ce2_c = 0.f, cxe_2c = 1.f, hay_2c = 1.f, hby_2c = 0.f;
 
   auto chunkable = [&](int jj) { return jj - 1 >= P + 1 && jj + U <= NY - P - 1; };
@@ -1195,13 +1190,13 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
       cL[u] = bload2(rCC, 2 * kn, e_inner);
       xL[u] = bload(rX0, kn, e_pml);
       yL[u] = bload(rY0, kn, e_pml);
-      cxeL[u] = e_xp ? bload(rCXE, kn, true) : cxe_r;
-      cxhL[u] = e_xp ? bload(rCXH, kn, true) : cxh_r;
-      cyhL[u] = e_xp ? bload(rCYH, kn, true) : cyh_r;
+      cxeL[u] = e_xp ? bload(rCXE, kn, e_xp) : cxe_r;
+      cxhL[u] = e_xp ? bload(rCXH, kn, e_xp) : cxh_r;
+      cyhL[u] = e_xp ? bload(rCYH, kn, e_xp) : cyh_r;
       nx_[u] = bload(rNx0, kk, okrow);
       ny_[u] = bload(rNy0, kk, h_any);
-      hayA[u] = h_xp ? bload(rHAY, kk, true) : hay_row;
-      hbyA[u] = h_xp ? bload(rHBY, kk, true) : hby_row;
+      hayA[u] = h_xp ? bload(rHAY, kk, h_xp) : hay_row;
+      hbyA[u] = h_xp ? bload(rHBY, kk, h_xp) : hby_row;
       const int c2 = jj + u - 1;
       const bool on = ac && e_inner && corr_on(c2);
       const unsigned fk = (unsigned)((c2 - g.dj0) * LD + ic);

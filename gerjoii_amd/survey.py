@@ -205,14 +205,21 @@ class Survey:
         return (self.ntraces, self.grid.nt)
 
     # -- compute ------------------------------------------------------------
-    def forward(self, keep=False, out=None):
+    def forward(self, keep=False, out=None, jz=None):
         """Traces of all shots, (ntraces, nt) float32: row t is trace t
-        (shot-major), i.e. shot s's d_2d is out[rec_ptr[s]:rec_ptr[s+1]].T."""
+        (shot-major), i.e. shot s's d_2d is out[rec_ptr[s]:rec_ptr[s+1]].T.
+        jz=True (or an (ns, nt) float32 array / cuda tensor) also records the
+        field at each shot's source cell per step (w_solve.m:104, gerjoii_.w.Jy)
+        and returns (traces, jz)."""
         if out is None and self.rec_ptr is not None:
             out = np.zeros(self.traces_shape(), dtype=np.float32)
-        _sync_torch(out)
-        N.check(N.load().gjw_forward(self.h, N.ptr(out), None, int(bool(keep))), "gjw_forward")
-        return out
+        if jz is True:
+            jz = np.zeros((len(self.rec_ptr) - 1 if self.rec_ptr is not None else 0, self.grid.nt),
+                          dtype=np.float32)
+        _sync_torch(out, jz)
+        N.check(N.load().gjw_forward(self.h, N.ptr(out), N.ptr(jz) if jz is not None else None,
+                                     int(bool(keep))), "gjw_forward")
+        return out if jz is None else (out, jz)
 
     def misfit(self, d_obs, e_out=None):
         """L2 adjoint source e = d - d_o and J = 0.5 e'e (FP64) of the last forward."""

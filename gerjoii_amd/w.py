@@ -57,10 +57,11 @@ def w_fwd(geome_, parame_, finite_, gerjoii_, keep_wavefield=True):
     sv.set_shots(np.asarray(gw["s"]).reshape(1, 2), np.asarray(gw["wvlet_"]).reshape(1, -1),
                  [np.asarray(gw["M"]).ravel()])
     sv.set_path(True)
-    d = sv.forward(keep=keep_wavefield)
+    d, jz = sv.forward(keep=keep_wavefield, jz=True)
     gerjoii_ = dict(gerjoii_)
     gerjoii_["w"] = dict(gw)
     gerjoii_["w"]["d_2d"] = d.T.copy()                    # nt x nr
+    gerjoii_["w"]["Jy"] = jz[0].copy()                     # source recorder (w_solve.m:104,124)
     gerjoii_["w"]["u_2d"] = WavefieldHandle(sv) if keep_wavefield else None
     obj = gw.get("obj_FNC", "L2")
     d_o = parame_.get("natu", {}).get("w", {}).get("d_2d")

@@ -92,6 +92,9 @@ int gjw_set_shots(gjw_survey *s, int32_t ns, const int32_t *src, const float *wa
 #define GJW_OPT_SNAPSHOT 4   /* GJW_SNAP_F32 (default) | GJW_SNAP_BF16 | GJW_SNAP_F16 */
 #define GJW_OPT_KTIMING 5    /* n > 0: time every n-th step kernel with HIP events  */
 #define GJW_OPT_FUSE 6       /* time steps per forward launch on the batched path: 2 (default) or 1 */
+#define GJW_OPT_GRAPH 7      /* 1: gjw_fwi_gradient (batched path) replays its whole launch
+                                sequence as one captured hipGraph (captured on the second call
+                                and again whenever a buffer or option it uses changes) */
 #define GJW_PATH_BATCHED 0   /* all shots per launch, checkpoint + recompute        */
 #define GJW_PATH_PLAIN 1     /* one shot at a time, full wavefield stored (as w_solve.m) */
 #define GJW_SNAP_F32 0

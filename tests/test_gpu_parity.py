@@ -333,3 +333,55 @@ def test_snapshot_checkpoints(oracle_mod, gpu_ctx, mode):
     assert rel_l2(ge, fe) > 0
     bound, cos_min = (1e-2, 0.9999) if mode == 2 else (5e-2, 0.999)
     assert rel_l2(ge, fe) <= bound and rel_l2(gs, fs) <= bound and cosine(ge, fe) >= cos_min
+
+
+def test_source_recorder(oracle_mod, gpu_ctx):
+    """gjw_forward's jz output equals the oracle's Jz (w_solve.m:104): the
+    field at the source cell after each step, for sources in PML-row tiles and
+    in interior tiles, one- and two-step kernels and the plain path."""
+    from gerjoii_amd import _native as N
+    for case in (configs.small(air=3), _borehole_case(nt=200)):
+        g = oracle_grid(oracle_mod, case.grid)
+        ep, sp = oracle_mod.exp2pml(g, case.eps0, case.sig0)
+        ref = np.stack([oracle_mod.forward(g, ep, sp, case.src[q], case.wavelets[q],
+                                           case.receivers[q])[1] for q in range(len(case.src))])
+        for plain, fuse in ((False, 2), (False, 1), (True, 1)):
+            sv = make_survey(case, gpu_ctx, plain=plain)
+            sv.set_option(N.GJW_OPT_FUSE, fuse)
+            d, jz = sv.forward(jz=True)
+            print("jz plain", plain, "fuse", fuse, rel_l2(jz, ref))
+            assert jz.shape == ref.shape and np.abs(ref).max() > 0
+            assert rel_l2(jz, ref) <= 1e-4
+
+
+@pytest.mark.parametrize("snap", [0, 1])
+def test_fwi_gradient_graph_replays(gpu_ctx, snap):
+    """GJW_OPT_GRAPH: every replay of the captured batched FWI gradient equals
+    the eager result bit for bit, with eager work and synchronisations between
+    the replays, and replays follow a change of the model (values, not
+    addresses, change)."""
+    import torch
+    from gerjoii_amd import _native as N
+    case = configs.small(nt=150, ns=3)
+    svt = make_survey(case, gpu_ctx)
+    svt.set_model(case.eps_true, case.sig_true)
+    dobs = torch.from_numpy(svt.forward()).cuda()
+    eager = make_survey(case, gpu_ctx, ck=7)
+    graph = make_survey(case, gpu_ctx, ck=7)
+    for sv in (eager, graph):
+        sv.set_option(N.GJW_OPT_SNAPSHOT, snap)
+    graph.set_option(N.GJW_OPT_GRAPH, 1)
+    models = [(case.eps0, case.sig0), (case.eps0 * 1.05, case.sig0 * 0.9)]
+    for eps, sig in models:
+        eager.set_model(eps, sig)
+        graph.set_model(eps, sig)
+        ge0, gs0, J0 = eager.fwi_gradient(dobs)
+        for rep in range(4):
+            ge, gs, J = graph.fwi_gradient(dobs)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(ge, ge0)
+            np.testing.assert_array_equal(gs, gs0)
+            assert J == J0
+            svt.forward()  # eager work on another survey between replays
+            torch.cuda.synchronize()
+        print("graph replays ok, J", J0)
