@@ -17,14 +17,17 @@ GJW_PATH_BATCHED, GJW_PATH_PLAIN = 0, 1
 GJW_OPT_PATH, GJW_OPT_CKPT_EVERY, GJW_OPT_TILE_L, GJW_OPT_SNAPSHOT, GJW_OPT_KTIMING = 1, 2, 3, 4, 5
 GJW_OPT_FUSE = 6
 GJW_OPT_GRAPH = 7
+GJW_OPT_MISFIT = 8
 GJW_SNAP_F32, GJW_SNAP_BF16, GJW_SNAP_F16 = 0, 1, 2
 GJW_MISFIT_L2 = 0
+GJW_MISFIT_ENV_L2, GJW_MISFIT_PHASE_L2, GJW_MISFIT_ENV_L22, GJW_MISFIT_ENV_LOG = 1, 2, 3, 4
+MISFIT_KINDS = {"L2": 0, "FREQstp": 0, "env_L2": 1, "phase_L2": 2, "env_L22": 3, "env_LOG": 4}
 GJW_GRAD_EXACT, GJW_GRAD_REF = 0, 1
 
 # every symbol include/gjw.h declares
 EXPORTS = ["gjw_create", "gjw_destroy", "gjw_last_error", "gjw_version", "gjw_survey_create",
            "gjw_survey_destroy", "gjw_set_model", "gjw_set_shots", "gjw_set_option",
-           "gjw_forward", "gjw_misfit", "gjw_gradient", "gjw_fwi_gradient",
+           "gjw_forward", "gjw_misfit", "gjw_bozdag", "gjw_gradient", "gjw_fwi_gradient",
            "gjw_differentiate_cube", "gjw_last_timing", "gjw_kernel_stats", "gjw_tile"]
 
 
@@ -64,6 +67,7 @@ def load():
     lib.gjw_set_option.argtypes = [vp, i32, i64]
     lib.gjw_forward.argtypes = [vp, vp, vp, i32]
     lib.gjw_misfit.argtypes = [vp, vp, i32, vp, vp]
+    lib.gjw_bozdag.argtypes = [vp, vp, vp, i32, vp, vp]
     lib.gjw_gradient.argtypes = [vp, vp, i32, vp, vp, vp, vp]
     lib.gjw_fwi_gradient.argtypes = [vp, vp, vp, vp, vp]
     lib.gjw_differentiate_cube.argtypes = [vp, vp, i32, i32, i32, C.c_double]

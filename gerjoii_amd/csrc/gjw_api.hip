@@ -19,6 +19,8 @@
 #include <string>
 #include <vector>
 
+#include <hipfft/hipfft.h>
+
 #include "../../include/gjw.h"
 #include "gjw_internal.h"
 
@@ -66,6 +68,11 @@ __global__ void k_ref_overwrite(DevGrid, int, int, int, const int*, const int*, 
                                 float*);
 __global__ void k_ref_corr(DevGrid, const float*, const float*, int, int, double, float*, float*);
 __global__ void k_resid_l2(const float*, const float*, float*, long long, double*);
+__global__ void k_analytic_spec(const float2*, float2*, int, long long);
+__global__ void k_obs_instant(const float2*, long long, int, int, float*);
+__global__ void k_misfit_stage1(const float*, const float2*, const float*, long long, int, int,
+                                float*, float*, double*);
+__global__ void k_misfit_stage2(const float*, const float2*, long long, int, int, float*);
 __global__ void k_reduce_d(const double*, int, double, double*);
 __global__ void k_dcube(float*, long long, int, double);
 }  // namespace gjw
@@ -176,6 +183,12 @@ struct gjw_survey {
   // sampled per-kernel timing (GJW_OPT_KTIMING): event pairs around every
   // ksample-th launch of k_fwd_step / k_adj_step, resolved after each call
   int ksample = 0;
+  // Hilbert-based misfits: FFT plans over all traces and their buffers
+  int misfit_kind = GJW_MISFIT_L2;  // misfit of gjw_fwi_gradient (GJW_OPT_MISFIT)
+  hipfftHandle fft_r2c = 0, fft_c2c = 0;
+  int fft_nt = 0, fft_ntr = 0;
+  DBuf<float2> fhalf, fana;
+  DBuf<float> fin, fov, fP;
   // GJW_OPT_GRAPH: the captured batched FWI gradient and the buffers it uses
   int graph_on = 0, graph_calls = 0;
   hipGraph_t graph = nullptr;
@@ -258,6 +271,9 @@ extern "C" int gjw_survey_destroy(gjw_survey* s) {
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
   for (auto e : s->kev_pool) (void)hipEventDestroy(e);
+  if (s->fft_r2c) (void)hipfftDestroy(s->fft_r2c);
+  if (s->fft_c2c) (void)hipfftDestroy(s->fft_c2c);
+  s->fhalf.release(); s->fana.release(); s->fin.release(); s->fov.release(); s->fP.release();
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   s->coef.release(); s->coef_ref.release(); s->zero_sc.release(); s->trash.release();
@@ -288,6 +304,11 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       if (v < 0 || v > 4096) return fail(GJW_ERR_ARG, "GJW_OPT_TILE_L out of range");
       s->tile_L = (int)v;
       break;
+    case GJW_OPT_MISFIT:
+      if (v < GJW_MISFIT_L2 || v > GJW_MISFIT_ENV_LOG)
+        return fail(GJW_ERR_ARG, "GJW_OPT_MISFIT: unknown misfit %lld", (long long)v);
+      s->misfit_kind = (int)v;
+      return GJW_OK;
     case GJW_OPT_GRAPH:
       if (v != 0 && v != 1) return fail(GJW_ERR_ARG, "GJW_OPT_GRAPH must be 0 or 1");
       s->graph_on = (int)v;
@@ -1110,8 +1131,85 @@ extern "C" int gjw_forward(gjw_survey* s, float* traces, float* jz, int32_t keep
 // --------------------------------------------------------------------------
 // misfit
 // --------------------------------------------------------------------------
+#define FFTCHK(x)                                                             \
+  do {                                                                        \
+    hipfftResult r_ = (x);                                                    \
+    if (r_ != HIPFFT_SUCCESS) return fail(GJW_ERR_HIP, "hipfft %d at %s:%d", (int)r_, __FILE__, __LINE__); \
+  } while (0)
+
+// analytic signal (unnormalised: x nt) of the ntr traces in s->fin, into s->fana
+static int hilbert_dev(gjw_survey* s) {
+  const int nt = s->gr.nt, ntr = (int)s->ntr;
+  hipStream_t st = s->ctx->stream;
+  if (!s->fft_r2c || s->fft_nt != nt || s->fft_ntr != ntr) {
+    if (s->fft_r2c) FFTCHK(hipfftDestroy(s->fft_r2c));
+    if (s->fft_c2c) FFTCHK(hipfftDestroy(s->fft_c2c));
+    s->fft_r2c = s->fft_c2c = 0;
+    int n1 = nt;
+    FFTCHK(hipfftPlanMany(&s->fft_r2c, 1, &n1, nullptr, 1, nt, nullptr, 1, nt / 2 + 1,
+                          HIPFFT_R2C, ntr));
+    FFTCHK(hipfftPlanMany(&s->fft_c2c, 1, &n1, nullptr, 1, nt, nullptr, 1, nt, HIPFFT_C2C, ntr));
+    s->fft_nt = nt;
+    s->fft_ntr = ntr;
+  }
+  FFTCHK(hipfftSetStream(s->fft_r2c, st));
+  FFTCHK(hipfftSetStream(s->fft_c2c, st));
+  const long long n = (long long)ntr * nt;
+  FFTCHK(hipfftExecR2C(s->fft_r2c, reinterpret_cast<hipfftReal*>(s->fin.p),
+                       reinterpret_cast<hipfftComplex*>(s->fhalf.p)));
+  const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(n, 256), 1), 8192);
+  hipLaunchKernelGGL(k_analytic_spec, dim3(blocks), dim3(256), 0, st, s->fhalf.p, s->fana.p, nt,
+                     (long long)ntr);
+  HIPCHK(hipGetLastError());
+  FFTCHK(hipfftExecC2C(s->fft_c2c, reinterpret_cast<hipfftComplex*>(s->fana.p),
+                       reinterpret_cast<hipfftComplex*>(s->fana.p), HIPFFT_BACKWARD));
+  return GJW_OK;
+}
+
+// Adjoint source and misfit of the last forward against s->obs (w_bozdag.m,
+// w_E.m): L2 directly; the envelope / phase kinds through three Hilbert
+// transforms (observed, synthetic, and the adjoint-source term).
+static int misfit_enqueue(gjw_survey* s, int32_t kind) {
+  hipStream_t st = s->ctx->stream;
+  const int nt = s->gr.nt;
+  const long long n = (long long)s->ntr * nt;
+  if (kind == GJW_MISFIT_L2) {
+    const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(n, 256), 1), 1024);
+    hipLaunchKernelGGL(k_resid_l2, dim3(blocks), dim3(256), 0, st, s->tr.p, s->obs.p, s->adj.p, n,
+                       s->part.p);
+    hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, blocks, 0.5, s->Jd.p);
+    HIPCHK(hipGetLastError());
+    return GJW_OK;
+  }
+  if (n == 0) return fail(GJW_ERR_STATE, "misfit: no traces");
+  TRY(s->fin.ensure(n));
+  TRY(s->fov.ensure(n));
+  TRY(s->fP.ensure(n));
+  TRY(s->fana.ensure(n));
+  TRY(s->fhalf.ensure((size_t)s->ntr * (nt / 2 + 1)));
+  const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(n, 256), 1), 1024);
+  // observed envelope or phase
+  HIPCHK(hipMemcpyAsync(s->fin.p, s->obs.p, n * 4, hipMemcpyDeviceToDevice, st));
+  TRY(hilbert_dev(s));
+  hipLaunchKernelGGL(k_obs_instant, dim3(blocks), dim3(256), 0, st, s->fana.p, n, nt, kind,
+                     s->fov.p);
+  // synthetic: misfit terms, q and P
+  HIPCHK(hipMemcpyAsync(s->fin.p, s->tr.p, n * 4, hipMemcpyDeviceToDevice, st));
+  TRY(hilbert_dev(s));
+  hipLaunchKernelGGL(k_misfit_stage1, dim3(blocks), dim3(256), 0, st, s->tr.p, s->fana.p,
+                     s->fov.p, n, nt, kind, s->fin.p, s->fP.p, s->part.p);
+  hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, blocks, 1.0, s->Jd.p);
+  // e = P -+ imag(hilbert(q))
+  TRY(hilbert_dev(s));
+  hipLaunchKernelGGL(k_misfit_stage2, dim3(blocks), dim3(256), 0, st, s->fP.p, s->fana.p, n, nt,
+                     kind, s->adj.p);
+  HIPCHK(hipGetLastError());
+  return GJW_OK;
+}
+
 static int misfit_dev(gjw_survey* s, const float* d_obs, int32_t kind) {
-  if (kind != GJW_MISFIT_L2) return fail(GJW_ERR_ARG, "gjw_misfit: unsupported kind %d", kind);
+  if (kind < GJW_MISFIT_L2 || kind > GJW_MISFIT_ENV_LOG)
+    return fail(GJW_ERR_ARG, "gjw_misfit: unknown kind %d", kind);
   hipStream_t st = s->ctx->stream;
   const long long n = (long long)s->ntr * s->gr.nt;
   TRY(s->adj.ensure(std::max<long long>(n, 1)));
@@ -1119,13 +1217,19 @@ static int misfit_dev(gjw_survey* s, const float* d_obs, int32_t kind) {
   TRY(s->part.ensure(1024));
   TRY(s->Jd.ensure(1));
   TRY(host_or_dev_copy(s->obs.p, d_obs, n * 4, st));
-  const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(n, 256), 1), 1024);
-  hipLaunchKernelGGL(k_resid_l2, dim3(blocks), dim3(256), 0, st, s->tr.p, s->obs.p, s->adj.p, n,
-                     s->part.p);
-  hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, blocks, 0.5, s->Jd.p);
-  HIPCHK(hipGetLastError());
+  TRY(misfit_enqueue(s, kind));
   s->have_adj = true;
   return GJW_OK;
+}
+
+extern "C" int gjw_bozdag(gjw_survey* s, const float* d_obs, const float* d, int32_t kind,
+                          float* e_out, double* J) {
+  TRY(check_ready(s, "gjw_bozdag"));
+  if (!d_obs || !d) return fail(GJW_ERR_ARG, "gjw_bozdag: NULL traces");
+  const long long n = (long long)s->ntr * s->gr.nt;
+  TRY(s->tr.ensure(std::max<long long>(n, 1)));
+  TRY(host_or_dev_copy(s->tr.p, d, n * 4, s->ctx->stream));
+  return gjw_misfit(s, d_obs, kind, e_out, J);
 }
 
 extern "C" int gjw_misfit(gjw_survey* s, const float* d_obs, int32_t kind, float* e_out,
@@ -1245,6 +1349,8 @@ static int grad_plain_exact(gjw_survey* s, const float* adj, float* gw) {
 // plain FWI gradient: per shot, forward storing the whole domain wavefield,
 // L2 residual of that shot, adjoint; J summed over shots at the end.
 static int fwi_plain(gjw_survey* s, const float* d_obs) {
+  if (s->misfit_kind != GJW_MISFIT_L2)
+    return fail(GJW_ERR_ARG, "gjw_fwi_gradient: the plain path computes the L2 misfit only");
   const int ns = s->ns, nt = s->gr.nt;
   const long long per = (long long)s->gr.m * s->gr.n, n = (long long)s->ntr * nt;
   const long long FD = (long long)s->LD * s->gr.n;
@@ -1446,7 +1552,7 @@ static int fwi_batched_prepare(gjw_survey* s) {
 
 static int fwi_batched_enqueue(gjw_survey* s) {
   const int ns = s->ns, nt = s->gr.nt;
-  const long long n = (long long)s->ntr * nt, per = (long long)s->gr.m * s->gr.n;
+  const long long per = (long long)s->gr.m * s->gr.n;
   hipStream_t st = s->ctx->stream;
   DevGrid g = dev_grid(s);
   DevShots sh = dev_shots(s, 0);
@@ -1461,11 +1567,7 @@ static int fwi_batched_enqueue(gjw_survey* s) {
     it += taken;
   }
   s->cur = cur;
-  const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(n, 256), 1), 1024);
-  hipLaunchKernelGGL(k_resid_l2, dim3(blocks), dim3(256), 0, st, s->tr.p, s->obs.p, s->adj.p, n,
-                     s->part.p);
-  hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, blocks, 0.5, s->Jd.p);
-  HIPCHK(hipGetLastError());
+  TRY(misfit_enqueue(s, s->misfit_kind));
   TRY(launch_fill_d(s->ge.p, per, 0.0, st));
   TRY(launch_fill_d(s->gsg.p, per, 0.0, st));
   return grad_batched(s, s->adj.p, nullptr);
@@ -1544,7 +1646,9 @@ extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps
     if (rc == GJW_OK) rc = host_or_dev_copy(s->obs.p, d_obs, (long long)s->ntr * s->gr.nt * 4, st);
     // the first call runs eagerly (and settles the autotuner); with
     // GJW_OPT_GRAPH later calls replay the captured sequence
-    const bool use_graph = s->graph_on && s->graph_calls++ > 0;
+    // (the Hilbert-based misfits run their FFTs eagerly: no graph for them)
+    const bool use_graph =
+        s->graph_on && s->misfit_kind == GJW_MISFIT_L2 && s->graph_calls++ > 0;
     if (rc == GJW_OK) rc = use_graph ? fwi_graph_run(s) : fwi_batched_enqueue(s);
     if (rc == GJW_OK && hipEventRecord(s->ctx->ev1, st) != hipSuccess)
       rc = fail(GJW_ERR_HIP, "event record");

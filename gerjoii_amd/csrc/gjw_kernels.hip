@@ -1529,6 +1529,110 @@ k_unpack16(const unsigned short* __restrict__ src, long long cnt, int B,
   }
 }
 
+// --------------------------------------------------------------------------
+// Hilbert-based misfits (w_bozdag.m:13-55, instant_.m:14-20, w_E.m:7-16)
+// --------------------------------------------------------------------------
+// Traces are rows of nt samples.  MATLAB's hilbert(x) along time: FFT of
+// length nt, weights h = [1, 2 .. 2, (1 if nt even), 0 ..], inverse FFT.  The
+// forward real-to-complex FFT gives the nt/2+1 non-negative frequencies; this
+// kernel writes the full weighted spectrum for the complex inverse FFT (the
+// inverse is unnormalised: the analytic signal is the result / nt).
+__global__ void k_analytic_spec(const float2* __restrict__ half, float2* __restrict__ full, int nt,
+                                long long ntr) {
+  const int nh = nt / 2 + 1;
+  const long long n = ntr * nt;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long r = t / nt;
+    const int k = (int)(t - r * nt);
+    float2 v = make_float2(0.f, 0.f);
+    if (k < nh) {
+      const float2 x = half[r * nh + k];
+      const float w = (k == 0 || (nt % 2 == 0 && k == nt / 2)) ? 1.f : 2.f;
+      v = make_float2(w * x.x, w * x.y);
+    }
+    full[t] = v;
+  }
+}
+
+// envelope (kind 1, 3, 4) or instantaneous phase (kind 2) of the observed
+// traces: ov = |A| or angle(A), A = analytic signal (ana / nt)
+__global__ void k_obs_instant(const float2* __restrict__ ana, long long n, int nt, int kind,
+                             float* __restrict__ ov) {
+  const float inv = 1.f / (float)nt;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const float re = ana[t].x * inv, im = ana[t].y * inv;
+    ov[t] = kind == 2 ? atan2f(im, re) : sqrtf(re * re + im * im);
+  }
+}
+
+// Stage 1 per sample of the synthetic traces d with analytic signal A:
+// the misfit term (FP64 block partials of 0.5 * r^2), q (the signal whose
+// Hilbert transform enters the adjoint source) and P (the rest):
+//   env_L2   r = de - oe,        f = r / de,       q = f dH,       P = d f
+//   phase_L2 r = ph - oph,       f = r / de^2,     q = f d,        P = -f
+//   env_L22  r = de^2 - oe^2,    f = r,            q = f dH,       P = d f
+//   env_LOG  r = log de - log oe, f = log(oe / de), q = f dH / de^2, P = -f d / de^2
+__global__ void k_misfit_stage1(const float* __restrict__ d, const float2* __restrict__ ana,
+                                const float* __restrict__ ov, long long n, int nt, int kind,
+                                float* __restrict__ q, float* __restrict__ P,
+                                double* __restrict__ part) {
+  const float inv = 1.f / (float)nt;
+  double acc = 0.0;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const float re = ana[t].x * inv, dH = ana[t].y * inv, x = d[t], o = ov[t];
+    const float de = sqrtf(re * re + dH * dH);
+    float f, qq, pp;
+    double r;
+    if (kind == 1) {
+      f = (de - o) / de;
+      qq = f * dH;
+      pp = x * f;
+      r = (double)de - (double)o;
+    } else if (kind == 2) {
+      const float ph = atan2f(dH, re);
+      f = (ph - o) / (de * de);
+      qq = f * x;
+      pp = -f;
+      r = (double)ph - (double)o;
+    } else if (kind == 3) {
+      f = de * de - o * o;
+      qq = f * dH;
+      pp = x * f;
+      r = (double)de * de - (double)o * o;
+    } else {
+      f = logf(o / de);
+      qq = f * (dH / (de * de));
+      pp = -f * (x / (de * de));
+      r = log((double)de) - log((double)o);
+    }
+    q[t] = qq;
+    P[t] = pp;
+    acc += 0.5 * r * r;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  __shared__ double ps[4];
+  if ((threadIdx.x & 63) == 0) ps[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (ps[0] + ps[1]) + (ps[2] + ps[3]);
+}
+
+// Stage 2: e = P -+ imag(hilbert(q)); NaN -> 0 for env_L2 and env_LOG
+// (w_bozdag.m:25,54)
+__global__ void k_misfit_stage2(const float* __restrict__ P, const float2* __restrict__ anaq,
+                                long long n, int nt, int kind, float* __restrict__ e) {
+  const float inv = 1.f / (float)nt;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const float h = anaq[t].y * inv;
+    float v = (kind == 1 || kind == 3) ? P[t] - h : P[t] + h;
+    if ((kind == 1 || kind == 4) && v != v) v = 0.f;
+    e[t] = v;
+  }
+}
+
 // Ez(S) on the domain -> frame (used for the frame of a restored checkpoint)
 __global__ void k_domain_frame(DevGrid g, int ns, const float* __restrict__ E,
                                float* __restrict__ frame, long long fstride) {

@@ -221,16 +221,35 @@ class Survey:
                                      int(bool(keep))), "gjw_forward")
         return out if jz is None else (out, jz)
 
-    def misfit(self, d_obs, e_out=None):
-        """L2 adjoint source e = d - d_o and J = 0.5 e'e (FP64) of the last forward."""
+    def misfit(self, d_obs, e_out=None, kind="L2"):
+        """Adjoint source and misfit (FP64) of the last forward against d_obs
+        (w_bozdag.m / w_E.m); kind is the reference's obj_FNC: 'L2',
+        'env_L2', 'phase_L2', 'env_L22' or 'env_LOG'."""
         _sync_torch(d_obs, e_out)
         if e_out is None:
             e_out = np.zeros(self.traces_shape(), dtype=np.float32)
         J = C.c_double(0)
         d = d_obs if hasattr(d_obs, "data_ptr") else np.ascontiguousarray(d_obs, dtype=np.float32)
-        N.check(N.load().gjw_misfit(self.h, N.ptr(d), N.GJW_MISFIT_L2, N.ptr(e_out), C.byref(J)),
-                "gjw_misfit")
+        N.check(N.load().gjw_misfit(self.h, N.ptr(d), N.MISFIT_KINDS[kind], N.ptr(e_out),
+                                    C.byref(J)), "gjw_misfit")
         return e_out, J.value
+
+    def bozdag(self, d_obs, d, kind="L2", e_out=None):
+        """w_bozdag.m + w_E.m for caller-given synthetic traces d (they replace
+        the survey's traces): returns (e, J)."""
+        _sync_torch(d_obs, d, e_out)
+        if e_out is None:
+            e_out = np.zeros(self.traces_shape(), dtype=np.float32)
+        J = C.c_double(0)
+        a = d_obs if hasattr(d_obs, "data_ptr") else np.ascontiguousarray(d_obs, dtype=np.float32)
+        b = d if hasattr(d, "data_ptr") else np.ascontiguousarray(d, dtype=np.float32)
+        N.check(N.load().gjw_bozdag(self.h, N.ptr(a), N.ptr(b), N.MISFIT_KINDS[kind], N.ptr(e_out),
+                                    C.byref(J)), "gjw_bozdag")
+        return e_out, J.value
+
+    def set_misfit(self, kind):
+        """Misfit of fwi_gradient (batched path): an obj_FNC name as in misfit()."""
+        self.set_option(N.GJW_OPT_MISFIT, N.MISFIT_KINDS[kind])
 
     def gradient(self, adj=None, mode=N.GJW_GRAD_EXACT, want_gw=False, g_eps=None, g_sig=None):
         """FP64 gradients (m, n) w.r.t. eps_r and sigma (and optionally the

@@ -92,6 +92,7 @@ int gjw_set_shots(gjw_survey *s, int32_t ns, const int32_t *src, const float *wa
 #define GJW_OPT_SNAPSHOT 4   /* GJW_SNAP_F32 (default) | GJW_SNAP_BF16 | GJW_SNAP_F16 */
 #define GJW_OPT_KTIMING 5    /* n > 0: time every n-th step kernel with HIP events  */
 #define GJW_OPT_FUSE 6       /* time steps per forward launch on the batched path: 2 (default) or 1 */
+#define GJW_OPT_MISFIT 8     /* misfit of gjw_fwi_gradient: GJW_MISFIT_* (batched path) */
 #define GJW_OPT_GRAPH 7      /* 1: gjw_fwi_gradient (batched path) replays its whole launch
                                 sequence as one captured hipGraph (captured on the second call
                                 and again whenever a buffer or option it uses changes) */
@@ -110,12 +111,28 @@ int gjw_set_option(gjw_survey *s, int32_t key, int64_t value);
 int gjw_forward(gjw_survey *s, float *traces, float *jz, int32_t keep);
 
 /* Misfit and adjoint source of the last forward (src/w/w_bozdag.m:13-55,
- * src/w/w_E.m:7-16) against observed traces d_obs (same layout as traces).
- * kind: GJW_MISFIT_L2.  e_out (optional) receives the adjoint source; J
- * (optional) receives 0.5 * e'e in FP64.  The adjoint source stays on the
- * device for gjw_gradient. */
+ * src/w/w_E.m:7-16, instant_.m:14-20) against observed traces d_obs (same
+ * layout as traces).  kind: the reference's obj_FNC
+ *   GJW_MISFIT_L2        'L2'        e = d - d_o,                J = 0.5|d - d_o|^2
+ *   GJW_MISFIT_ENV_L2    'env_L2'    envelopes,                  J = 0.5|A(d) - A(d_o)|^2
+ *   GJW_MISFIT_PHASE_L2  'phase_L2'  instantaneous phases,       J = 0.5|phi(d) - phi(d_o)|^2
+ *   GJW_MISFIT_ENV_L22   'env_L22'   squared envelopes,          J = 0.5|A(d)^2 - A(d_o)^2|^2
+ *   GJW_MISFIT_ENV_LOG   'env_LOG'   log envelopes,              J = 0.5|log A(d) - log A(d_o)|^2
+ * (Hilbert transforms per trace by FFT of length nt in FP32, as MATLAB's
+ * hilbert).  e_out (optional) receives the adjoint source; J (optional) the
+ * misfit in FP64.  The adjoint source stays on the device for gjw_gradient. */
 #define GJW_MISFIT_L2 0
+#define GJW_MISFIT_ENV_L2 1
+#define GJW_MISFIT_PHASE_L2 2
+#define GJW_MISFIT_ENV_L22 3
+#define GJW_MISFIT_ENV_LOG 4
 int gjw_misfit(gjw_survey *s, const float *d_obs, int32_t kind, float *e_out, double *J);
+
+/* [e, ~, ~] = w_bozdag(d_o, d, obj_fnc) and J = w_E(...) (src/w/w_bozdag.m:1,
+ * src/w/w_E.m:1) for synthetic traces d given by the caller instead of the
+ * last forward (same layout; they replace the survey's traces). */
+int gjw_bozdag(gjw_survey *s, const float *d_obs, const float *d, int32_t kind, float *e_out,
+               double *J);
 
 /* Gradient of the misfit w.r.t. the model (m x n doubles, FP64 accumulation
  * over time and shots).

@@ -385,3 +385,62 @@ def test_fwi_gradient_graph_replays(gpu_ctx, snap):
             svt.forward()  # eager work on another survey between replays
             torch.cuda.synchronize()
         print("graph replays ok, J", J0)
+
+
+@pytest.mark.parametrize("nt", [240, 241])
+def test_hilbert_misfits(oracle_mod, gpu_ctx, nt):
+    """Envelope / phase misfits on the device (FP32 FFTs) against the oracle's
+    w_bozdag.m / w_E.m in FP64, even and odd nt; then the FWI gradient with
+    the env_L2 misfit against the oracle's gradient for the oracle's e."""
+    case = configs.small(nt=nt, ns=2)
+    svt = make_survey(case, gpu_ctx)
+    svt.set_model(case.eps_true, case.sig_true)
+    dobs = svt.forward()
+    sv = make_survey(case, gpu_ctx, ck=11)
+    d = sv.forward(keep=True)
+    rp = sv.rec_ptr
+    # samples whose envelopes (synthetic and observed) are above 1e-3 of the
+    # trace's largest: the phase and log-envelope sources divide by |A(d)|^2 and
+    # take log |A|, so below that floor FP32 rounding dominates them
+    env = np.abs(oracle_mod.hilbert_analytic(d.T.astype(np.float64))).T
+    envo = np.abs(oracle_mod.hilbert_analytic(dobs.T.astype(np.float64))).T
+    good = (env >= 1e-3 * env.max(axis=1, keepdims=True)) & \
+           (envo >= 1e-3 * envo.max(axis=1, keepdims=True))
+    for kind in ("env_L2", "phase_L2", "env_L22", "env_LOG"):
+        e, J = sv.misfit(dobs, kind=kind)
+        es, Js = [], 0.0
+        for q in range(len(case.src)):
+            a, b = rp[q], rp[q + 1]
+            eq, de, doe = oracle_mod.w_bozdag(dobs[a:b].T, d[a:b].T, kind)
+            es.append(eq.T)
+            Js += oracle_mod.w_E(doe, de, obj_fnc=kind)
+        eref = np.concatenate(es, axis=0)
+        eg = rel_l2(e[good], eref[good])
+        print(kind, "e rel L2", rel_l2(e, eref), "above the floor", eg, "J", J, Js)
+        assert np.isfinite(e).all() and np.abs(eref).max() > 0
+        if kind in ("env_L2", "env_L22"):
+            assert rel_l2(e, eref) <= 1e-4 and abs(J - Js) <= 1e-5 * abs(Js)
+        elif kind == "phase_L2":  # (the phase below the floor also enters J)
+            assert eg <= 1e-3 and abs(J - Js) <= 1e-4 * abs(Js)
+    # env_LOG (log |A| and 1/|A|^2) on traces with a noise floor, through
+    # gjw_bozdag with caller-given synthetic traces
+    rng = np.random.default_rng(3)
+    dn = (d + 2e-3 * np.abs(d).max() * rng.standard_normal(d.shape)).astype(np.float32)
+    on = (dobs + 2e-3 * np.abs(dobs).max() * rng.standard_normal(d.shape)).astype(np.float32)
+    for kind in ("env_LOG", "phase_L2", "env_L2"):
+        e, J = sv.bozdag(on, dn, kind=kind)
+        es, Js = [], 0.0
+        for q in range(len(case.src)):
+            a, b = rp[q], rp[q + 1]
+            eq, de, doe = oracle_mod.w_bozdag(on[a:b].T, dn[a:b].T, kind)
+            es.append(eq.T)
+            Js += oracle_mod.w_E(doe, de, obj_fnc=kind)
+        eref = np.concatenate(es, axis=0)
+        print(kind, "noisy traces: e rel L2", rel_l2(e, eref), "J", J, Js)
+        assert rel_l2(e, eref) <= 1e-3 and abs(J - Js) <= 1e-5 * abs(Js)
+    sv.set_misfit("env_L2")
+    ge, gs, J = sv.fwi_gradient(dobs)
+    d = sv.forward(keep=True)
+    e, _ = sv.misfit(dobs, kind="env_L2")
+    fe, fs, _ = oracle_grad(oracle_mod, case, case.eps0, case.sig0, e, rp, K=11)
+    assert rel_l2(ge, fe) <= 1e-3 and rel_l2(gs, fs) <= 1e-3 and cosine(ge, fe) >= 0.9999
