@@ -25,4 +25,4 @@ def build(verbose=False):
     return os.path.join(HERE, "libgjw.so")
 
 
-from .survey import Context, Grid, Survey, context, dt_cfl, w_wavelet  # noqa: E402,F401
+from .survey import Batch, Context, Grid, Survey, context, dt_cfl, w_wavelet  # noqa: E402,F401

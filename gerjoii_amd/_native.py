@@ -28,7 +28,9 @@ GJW_GRAD_EXACT, GJW_GRAD_REF = 0, 1
 EXPORTS = ["gjw_create", "gjw_destroy", "gjw_last_error", "gjw_version", "gjw_survey_create",
            "gjw_survey_destroy", "gjw_set_model", "gjw_set_shots", "gjw_set_option",
            "gjw_forward", "gjw_misfit", "gjw_bozdag", "gjw_gradient", "gjw_fwi_gradient",
-           "gjw_differentiate_cube", "gjw_last_timing", "gjw_kernel_stats", "gjw_tile"]
+           "gjw_differentiate_cube", "gjw_last_timing", "gjw_kernel_stats", "gjw_tile",
+           "gjw_batch_create", "gjw_batch_destroy", "gjw_batch_fwi_gradient",
+           "gjw_batch_last_timing"]
 
 
 class GjwGrid(C.Structure):
@@ -68,6 +70,10 @@ def load():
     lib.gjw_forward.argtypes = [vp, vp, vp, i32]
     lib.gjw_misfit.argtypes = [vp, vp, i32, vp, vp]
     lib.gjw_bozdag.argtypes = [vp, vp, vp, i32, vp, vp]
+    lib.gjw_batch_create.argtypes = [vp, i32, vp, vp]
+    lib.gjw_batch_destroy.argtypes = [vp]
+    lib.gjw_batch_fwi_gradient.argtypes = [vp, vp, vp, vp, vp]
+    lib.gjw_batch_last_timing.argtypes = [vp, vp, vp]
     lib.gjw_gradient.argtypes = [vp, vp, i32, vp, vp, vp, vp]
     lib.gjw_fwi_gradient.argtypes = [vp, vp, vp, vp, vp]
     lib.gjw_differentiate_cube.argtypes = [vp, vp, i32, i32, i32, C.c_double]

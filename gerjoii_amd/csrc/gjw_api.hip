@@ -51,6 +51,13 @@ extern template __global__ void k_fwd_step2<true>(DevGrid, DevShots, int, const 
                                                   const float*, float*, float*, float*, float*,
                                                   float*, int, float*, float*, float*, long long,
                                                   float*);
+__global__ void k_fwd_step_multi(const DevSurvey*, MultiArgs);
+template <bool FRAMES>
+__global__ void k_fwd_step2_multi(const DevSurvey*, MultiArgs);
+extern template __global__ void k_fwd_step2_multi<false>(const DevSurvey*, MultiArgs);
+extern template __global__ void k_fwd_step2_multi<true>(const DevSurvey*, MultiArgs);
+__global__ void k_adj_step_multi(const DevSurvey*, MultiArgs);
+__global__ void k_adj_step2_multi(const DevSurvey*, MultiArgs);
 __global__ void k_fill(float*, long long, float);
 __global__ void k_fill_d(double*, long long, double);
 __global__ void k_inject_last(DevGrid, int, int, int, const int*, const int*, const int*,
@@ -148,6 +155,7 @@ struct gjw_survey {
   bool have_model = false, have_shots = false;
   // options
   int path = GJW_PATH_BATCHED, ck_every = 0, tile_L = 0, snap = GJW_SNAP_F32, fuse = 2;
+  int ck_batch = 0;  // checkpoint interval imposed by gjw_batch_fwi_gradient
   int tuned_L = 0, tuned_L1 = 0;  // two-step and one-step kernels
   // model, padded (NX x NY, column-major, FP64)
   std::vector<double> pe, ps;
@@ -1023,6 +1031,7 @@ static int ck_restore(gjw_survey* s, FwdBufs& b, int cur, int xc, int c) {
 }
 
 static int ck_interval(const gjw_survey* s) {
+  if (s->ck_batch > 0) return s->ck_batch;  // common interval of a survey batch
   if (s->ck_every > 0) return s->ck_every;
   return std::max(1, (int)std::ceil(std::sqrt((double)s->gr.nt)));
 }
@@ -1672,6 +1681,271 @@ extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps
   s->last_cells = (double)(s->NX + 1) * (s->NY + 1) * s->gr.nt * s->ns *
                   (s->path == GJW_PATH_PLAIN ? 2 : 3);
   if (J) HIPCHK(hipMemcpy(J, s->Jd.p, 8, hipMemcpyDefault));
+  return GJW_OK;
+}
+
+// --------------------------------------------------------------------------
+// several surveys in shared launches
+// --------------------------------------------------------------------------
+// A batch runs the FWI gradients of up to kMaxSurveys surveys (each with its
+// own grid, model, shots, trace length and misfit) with one launch per time
+// step (or pair of steps) covering the tiles of every survey still active.
+// All surveys start at step 0 and share one checkpoint interval K; a launch
+// fuses two steps only where no survey starts or ends between them.
+struct gjw_batch {
+  gjw_ctx* ctx = nullptr;
+  std::vector<gjw_survey*> sv;
+  DBuf<DevSurvey> dsv;
+  double last_ms = 0, last_cells = 0;
+};
+
+extern "C" int gjw_batch_create(gjw_ctx* ctx, int32_t n, gjw_survey* const* surveys,
+                                gjw_batch** out) {
+  if (!ctx || !surveys || !out) return fail(GJW_ERR_ARG, "gjw_batch_create: NULL argument");
+  if (n < 1 || n > kMaxSurveys)
+    return fail(GJW_ERR_ARG, "gjw_batch_create: 1 to %d surveys", kMaxSurveys);
+  for (int k = 0; k < n; ++k) {
+    if (!surveys[k] || surveys[k]->ctx != ctx)
+      return fail(GJW_ERR_ARG, "gjw_batch_create: survey %d is NULL or of another context", k);
+    for (int q = 0; q < k; ++q)
+      if (surveys[q] == surveys[k]) return fail(GJW_ERR_ARG, "gjw_batch_create: survey repeated");
+  }
+  gjw_batch* b = new gjw_batch;
+  b->ctx = ctx;
+  b->sv.assign(surveys, surveys + n);
+  *out = b;
+  return GJW_OK;
+}
+
+extern "C" int gjw_batch_destroy(gjw_batch* b) {
+  if (!b) return GJW_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  b->dsv.release();
+  delete b;
+  return GJW_OK;
+}
+
+static DevSurvey dev_survey(gjw_survey* s) {
+  DevSurvey d;
+  d.g = dev_grid(s);
+  d.s = dev_shots(s, 0);
+  d.ns = s->ns;
+  d.fsz = (long long)s->ns * s->FS;
+  d.fld = s->fld.p;
+  d.adjf = s->adjf.p;
+  d.tr = s->tr.p;
+  d.adj = s->adj.p;
+  d.frames = s->frames.p;
+  d.FD = (long long)s->LD * s->gr.n;
+  d.fstride = (long long)(s->ck_K + 1) * d.FD;
+  d.acc = s->acc.p;
+  return d;
+}
+
+// one multi-survey launch over surveys ids (steps: 1 or 2; adj: adjoint kernels)
+static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bool adj,
+                        bool frames, const int* it, const int* cur, const int* xc,
+                        const int* fslot, int rec) {
+  MultiArgs a{};
+  a.n = (int)ids.size();
+  a.rec = rec;
+  int blk = 0;
+  const int rows = steps == 2 ? 60 : 64;
+  for (int k = 0; k < a.n; ++k) {
+    gjw_survey* s = b->sv[ids[k]];
+    const int L = steps == 2 ? s->tuned_L : s->tuned_L1;
+    a.blk0[k] = blk;
+    a.sid[k] = ids[k];
+    a.it[k] = it[ids[k]];
+    a.L[k] = L;
+    a.cur[k] = cur[ids[k]];
+    a.xc[k] = xc ? xc[ids[k]] : 0;
+    a.fslot[k] = fslot ? fslot[ids[k]] : -1;
+    const long long nb = cdiv(s->NX + 1, rows) * cdiv(cdiv(s->NY + 1, L), 4) * s->ns;
+    blk += (int)(cdiv(nb, 8) * 8);
+  }
+  a.blk0[a.n] = blk;
+  if (blk == 0) return GJW_OK;
+  hipStream_t st = b->ctx->stream;
+  const dim3 grid((unsigned)blk), block(64, 4);
+  if (!adj && steps == 1)
+    hipLaunchKernelGGL(k_fwd_step_multi, grid, block, 0, st, b->dsv.p, a);
+  else if (!adj && frames)
+    hipLaunchKernelGGL(k_fwd_step2_multi<true>, grid, block, 0, st, b->dsv.p, a);
+  else if (!adj)
+    hipLaunchKernelGGL(k_fwd_step2_multi<false>, grid, block, 0, st, b->dsv.p, a);
+  else if (steps == 1)
+    hipLaunchKernelGGL(k_adj_step_multi, grid, block, 0, st, b->dsv.p, a);
+  else
+    hipLaunchKernelGGL(k_adj_step2_multi, grid, block, 0, st, b->dsv.p, a);
+  HIPCHK(hipGetLastError());
+  return GJW_OK;
+}
+
+extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
+                                      double* const* g_eps, double* const* g_sig, double* J) {
+  if (!b || !d_obs) return fail(GJW_ERR_ARG, "gjw_batch_fwi_gradient: NULL argument");
+  const int S = (int)b->sv.size();
+  HIPCHK(hipSetDevice(b->ctx->device));
+  hipStream_t st = b->ctx->stream;
+  int T = 0, fuse = 2;
+  for (int k = 0; k < S; ++k) {
+    gjw_survey* s = b->sv[k];
+    TRY(check_ready(s, "gjw_batch_fwi_gradient"));
+    if (!d_obs[k]) return fail(GJW_ERR_ARG, "gjw_batch_fwi_gradient: d_obs[%d] is NULL", k);
+    if (s->path != GJW_PATH_BATCHED)
+      return fail(GJW_ERR_ARG, "gjw_batch_fwi_gradient: survey %d is on the plain path", k);
+    T = std::max(T, s->gr.nt);
+    fuse = std::min(fuse, s->fuse);
+  }
+  int K = 0;
+  for (int k = 0; k < S; ++k) K = std::max(K, b->sv[k]->ck_every);
+  if (K <= 0) K = std::max(1, (int)std::ceil(std::sqrt((double)T)));
+  int nt[kMaxSurveys], cur[kMaxSurveys], xc[kMaxSurveys], itv[kMaxSurveys], fsl[kMaxSurveys];
+  for (int k = 0; k < S; ++k) {
+    gjw_survey* s = b->sv[k];
+    s->ck_batch = K;
+    const int rc = fwi_batched_prepare(s);
+    s->ck_batch = 0;
+    TRY(rc);
+    s->ck_K = K;
+    s->ck_n = (int)cdiv(s->gr.nt, K);
+    TRY(host_or_dev_copy(s->obs.p, d_obs[k], (long long)s->ntr * s->gr.nt * 4, st));
+    nt[k] = s->gr.nt;
+  }
+  std::vector<DevSurvey> hs(S);
+  for (int k = 0; k < S; ++k) hs[k] = dev_survey(b->sv[k]);
+  TRY(b->dsv.ensure(S));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpy(b->dsv.p, hs.data(), S * sizeof(DevSurvey), hipMemcpyHostToDevice));
+  HIPCHK(hipEventRecord(b->ctx->ev0, st));
+  // ---- forward sweep with checkpoints
+  for (int k = 0; k < S; ++k) {
+    TRY(zero_fwd_state(b->sv[k], b->sv[k]->ns));
+    cur[k] = xc[k] = 0;
+  }
+  for (int tau = 0; tau < T;) {
+    std::vector<int> act;
+    bool two = fuse == 2 && K - tau % K >= 2;
+    for (int k = 0; k < S; ++k)
+      if (tau < nt[k]) {
+        act.push_back(k);
+        gjw_survey* s = b->sv[k];
+        if (tau % K == 0) {
+          FwdBufs fb = fwd_bufs(s, s->ns);
+          TRY(ck_save(s, fb, cur[k], xc[k], tau / K));
+        }
+        if (nt[k] - tau < 2) two = false;
+        itv[k] = tau;
+      }
+    TRY(batch_launch(b, act, two ? 2 : 1, false, false, itv, cur, xc, nullptr, 1));
+    for (int k : act) {
+      cur[k] ^= 1;
+      if (two) xc[k] ^= 1;
+    }
+    tau += two ? 2 : 1;
+  }
+  // ---- misfits, adjoint initial state
+  for (int k = 0; k < S; ++k) {
+    gjw_survey* s = b->sv[k];
+    const long long per = (long long)s->gr.m * s->gr.n, FD = (long long)s->LD * s->gr.n;
+    s->cur = cur[k];
+    TRY(misfit_enqueue(s, s->misfit_kind));
+    TRY(launch_fill_d(s->ge.p, per, 0.0, st));
+    TRY(launch_fill_d(s->gsg.p, per, 0.0, st));
+    TRY(launch_fill(s->acc.p, 2LL * s->ns * FD, 0.f, st));
+    TRY(adj_init(s, s->ns, 0, s->adj.p));
+  }
+  // ---- adjoint sweep, segment by segment, recomputing the forward
+  int acur[kMaxSurveys];
+  for (int k = 0; k < S; ++k) acur[k] = 0;
+  const int nseg = (int)cdiv(T, K);
+  for (int c = nseg - 1; c >= 0; --c) {
+    const int k0 = c * K;
+    int k1[kMaxSurveys], kmax = k0;
+    std::vector<int> seg;
+    for (int k = 0; k < S; ++k) {
+      k1[k] = std::min(k0 + K, nt[k]);
+      if (k0 < nt[k]) {
+        seg.push_back(k);
+        kmax = std::max(kmax, k1[k]);
+        gjw_survey* s = b->sv[k];
+        FwdBufs fb = fwd_bufs(s, s->ns);
+        cur[k] = xc[k] = 0;
+        TRY(ck_restore(s, fb, 0, 0, c));
+        const long long per = (long long)s->gr.m * s->gr.n;
+        DevGrid g = dev_grid(s);
+        hipLaunchKernelGGL(k_domain_frame, dim3((unsigned)cdiv(per * s->ns, 256)), dim3(256), 0, st,
+                           g, s->ns, fb.E[0], s->frames.p, hs[k].fstride);
+        HIPCHK(hipGetLastError());
+      }
+    }
+    for (int tau = k0; tau < kmax;) {
+      std::vector<int> act;
+      bool two = fuse == 2;
+      for (int k : seg)
+        if (tau < k1[k]) {
+          act.push_back(k);
+          if (k1[k] - tau < 2) two = false;
+          itv[k] = tau;
+          fsl[k] = tau + 1 - k0;
+        }
+      TRY(batch_launch(b, act, two ? 2 : 1, false, true, itv, cur, xc, fsl, 0));
+      for (int k : act) {
+        cur[k] ^= 1;
+        if (two) xc[k] ^= 1;
+      }
+      tau += two ? 2 : 1;
+    }
+    for (int tau = kmax - 1; tau >= k0;) {
+      std::vector<int> act;
+      bool two = fuse == 2 && tau - 1 >= k0;
+      for (int k : seg)
+        if (tau <= k1[k] - 1) {
+          act.push_back(k);
+          itv[k] = tau;
+          fsl[k] = tau - k0;
+        } else if (tau - 1 == k1[k] - 1) {
+          two = false;  // survey k starts at tau-1
+        }
+      TRY(batch_launch(b, act, two ? 2 : 1, true, false, itv, acur, nullptr, fsl, 0));
+      for (int k : act) acur[k] ^= 1;
+      tau -= two ? 2 : 1;
+    }
+    for (int k : seg) {
+      gjw_survey* s = b->sv[k];
+      const long long per = (long long)s->gr.m * s->gr.n, FD = (long long)s->LD * s->gr.n;
+      hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
+                         s->gr.n, s->LD, s->di0, s->ns, s->acc.p, FD, s->fe.p, s->fs.p, s->ge.p,
+                         s->gsg.p, 1.0);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  HIPCHK(hipEventRecord(b->ctx->ev1, st));
+  double cells = 0;
+  for (int k = 0; k < S; ++k) {
+    gjw_survey* s = b->sv[k];
+    const long long per = (long long)s->gr.m * s->gr.n;
+    s->kept = 0;
+    s->have_adj = true;
+    if (g_eps && g_eps[k]) TRY(host_or_dev_copy(g_eps[k], s->ge.p, per * 8, st));
+    if (g_sig && g_sig[k]) TRY(host_or_dev_copy(g_sig[k], s->gsg.p, per * 8, st));
+    if (J) HIPCHK(hipMemcpyAsync(J + k, s->Jd.p, 8, hipMemcpyDefault, st));
+    cells += 3.0 * (s->NX + 1) * (s->NY + 1) * (double)s->gr.nt * s->ns;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
+  b->last_ms = ms;
+  b->last_cells = cells;
+  return GJW_OK;
+}
+
+extern "C" int gjw_batch_last_timing(gjw_batch* b, double* ms, double* cells) {
+  if (!b) return fail(GJW_ERR_ARG, "gjw_batch_last_timing: NULL batch");
+  if (ms) *ms = b->last_ms;
+  if (cells) *cells = b->last_cells;
   return GJW_OK;
 }
 

@@ -50,6 +50,42 @@ struct DevShots {
   float* jz;         // source recorder Ez(source) per step, [shot][nt], or null
 };
 
+// ---- several surveys in one launch (gjw_batch_*) ----
+constexpr int kMaxSurveys = 8;
+
+// Static description of one survey of a multi-survey launch (device array).
+// Buffer layouts are those of a single survey: forward state fld = E, Hx, Hy
+// of parity 0, then of parity 1, then X, Y of parity 0 and 1 (fsz floats
+// each); adjoint state adjf = (M, Nx, Ny, X, Y) of parity 0, then 1.
+struct DevSurvey {
+  DevGrid g;
+  DevShots s;
+  int ns;              // shots
+  long long fsz;       // ns * FS
+  float* fld;
+  float* adjf;
+  float* tr;           // traces, [trace][nt]
+  const float* adj;    // adjoint source, traces layout
+  float* frames;       // domain frames of the current segment: slot q at q * FD
+  long long FD;        // floats per frame of one shot (LD * n)
+  long long fstride;   // floats per shot's frame set
+  float* acc;          // correlation partials: float2 per frame cell, FD per shot
+};
+
+// Per-launch part of a multi-survey launch (kernel argument): survey sid[k]
+// owns blocks [blk0[k], blk0[k+1]).
+struct MultiArgs {
+  int n;
+  int blk0[kMaxSurveys + 1];
+  int sid[kMaxSurveys];
+  int it[kMaxSurveys];     // step (forward) / later transposed step (adjoint)
+  int L[kMaxSurveys];      // columns per wavefront
+  int cur[kMaxSurveys];    // field parity (forward E/H, adjoint state)
+  int xc[kMaxSurveys];     // split-field parity (forward)
+  int fslot[kMaxSurveys];  // frame slot: of Ez(S_{it+1}) (recompute), of Ez(S_it) (adjoint); -1 none
+  int rec;                 // forward: record traces
+};
+
 enum { KIND_RING = 0, KIND_INNER = 1, KIND_PML = 2 };
 
 // floats per cell slot of the coefficient buffer (cc 2, split 4, H strips 4, table 9)

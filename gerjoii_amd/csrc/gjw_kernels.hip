@@ -108,13 +108,8 @@ struct TileId {
   int rowtile, strip, shot;
 };
 
-__device__ __forceinline__ TileId tile_of_block(int NX, int NY, int L, int rows = 64) {
-  const unsigned B = gridDim.x, b = blockIdx.x;
-  const unsigned per = B / 8u;
-  const unsigned t = (b < per * 8u) ? (b % 8u) * per + b / 8u : b;
-  const unsigned gx = (unsigned)((NX + 1 + rows - 1) / rows);
-  const unsigned gy = (unsigned)(((NY + 1 + L - 1) / L + 3) / 4);
-  const unsigned ns = B / (gx * gy);
+// Logical tile t of a grid of gx row tiles, gy strip groups and ns shots.
+__device__ __forceinline__ TileId tile_decode(unsigned t, unsigned gx, unsigned gy, unsigned ns) {
   const unsigned span = (unsigned)kShotGroup * gx * gy;  // tiles per full shot group
   const unsigned grp = t / span, rem = t - grp * span;
   const unsigned gc = min((unsigned)kShotGroup, ns - grp * (unsigned)kShotGroup);
@@ -124,6 +119,40 @@ __device__ __forceinline__ TileId tile_of_block(int NX, int NY, int L, int rows 
   id.rowtile = (int)(rest % gx);
   id.strip = (int)(rest / gx);
   return id;
+}
+
+__device__ __forceinline__ unsigned tiles_x(int NX, int rows) {
+  return (unsigned)((NX + 1 + rows - 1) / rows);
+}
+__device__ __forceinline__ unsigned tiles_y(int NY, int L) {
+  return (unsigned)(((NY + 1 + L - 1) / L + 3) / 4);
+}
+
+__device__ __forceinline__ TileId tile_of_block(int NX, int NY, int L, int rows = 64) {
+  const unsigned B = gridDim.x, b = blockIdx.x;
+  const unsigned per = B / 8u;
+  const unsigned t = (b < per * 8u) ? (b % 8u) * per + b / 8u : b;
+  const unsigned gx = tiles_x(NX, rows), gy = tiles_y(NY, L);
+  return tile_decode(t, gx, gy, B / (gx * gy));
+}
+
+// Several surveys in one launch: surveys[a.sid[k]] owns blocks
+// [a.blk0[k], a.blk0[k+1]) (a multiple of 8 each, so the XCD-aware order holds
+// per survey); its own step, tile width, buffer parities and frame slot come
+// with it.  Blocks past a survey's tiles exit.
+__device__ __forceinline__ bool multi_tile(const MultiArgs& a, const DevSurvey* __restrict__ sv,
+                                           int rows, int& k, TileId& tid) {
+  const int b = (int)blockIdx.x;
+  k = 0;
+  while (k + 1 < a.n && b >= a.blk0[k + 1]) ++k;
+  const DevSurvey& S = sv[a.sid[k]];
+  const unsigned B = (unsigned)(a.blk0[k + 1] - a.blk0[k]), bl = (unsigned)(b - a.blk0[k]);
+  const unsigned per = B / 8u;
+  const unsigned t = (bl % 8u) * per + bl / 8u;
+  const unsigned gx = tiles_x(S.g.NX, rows), gy = tiles_y(S.g.NY, a.L[k]);
+  if (t >= gx * gy * (unsigned)S.ns) return false;
+  tid = tile_decode(t, gx, gy, (unsigned)S.ns);
+  return true;
 }
 
 // ==========================================================================
@@ -173,15 +202,13 @@ __device__ __forceinline__ void fwd_column(const DevGrid& g, int lane, int i, in
   e_c = e_n;
 }
 
-__global__ void __launch_bounds__(256)
-k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
+__device__ __forceinline__ void fwd1_body(const DevGrid& g, const DevShots& s, int L, TileId tid, const float* __restrict__ E0,
            const float* __restrict__ Hx0, const float* __restrict__ Hy0, float* __restrict__ E1,
            float* __restrict__ Hx1, float* __restrict__ Hy1, float* __restrict__ X,
            float* __restrict__ Y, int it, float* __restrict__ tr, float* __restrict__ frame,
            long long fstride) {
   const int lane = threadIdx.x;
   const int NX = g.NX, NY = g.NY, P = g.P;
-  const TileId tid = tile_of_block(NX, NY, L);
   const int shot = tid.shot;
   const int i0 = tid.rowtile * 64;
   // threadIdx.y is constant across a 64-lane wavefront: make the strip start
@@ -280,6 +307,31 @@ k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
   }
 }
 
+__global__ void __launch_bounds__(256)
+k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
+           const float* __restrict__ Hx0, const float* __restrict__ Hy0, float* __restrict__ E1,
+           float* __restrict__ Hx1, float* __restrict__ Hy1, float* __restrict__ X,
+           float* __restrict__ Y, int it, float* __restrict__ tr, float* __restrict__ frame,
+           long long fstride) {
+  fwd1_body(g, s, L, tile_of_block(g.NX, g.NY, L), E0, Hx0, Hy0, E1, Hx1, Hy1, X, Y, it, tr, frame, fstride);
+}
+
+__global__ void __launch_bounds__(256)
+k_fwd_step_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
+  int k;
+  TileId tid;
+  if (!multi_tile(a, sv, 64, k, tid)) return;
+  const DevSurvey& S = sv[a.sid[k]];
+  const int c = a.cur[k], x = a.xc[k];
+  float* f = S.fld;
+  const long long z = S.fsz;
+  float* fr = a.fslot[k] >= 0 ? S.frames + (long long)a.fslot[k] * S.FD : nullptr;
+  fwd1_body(S.g, S.s, a.L[k], tid, f + 3 * c * z, f + (3 * c + 1) * z, f + (3 * c + 2) * z,
+            f + 3 * (c ^ 1) * z, f + (3 * (c ^ 1) + 1) * z, f + (3 * (c ^ 1) + 2) * z,
+            f + (6 + 2 * x) * z, f + (7 + 2 * x) * z, a.it[k], a.rec ? S.tr : nullptr, fr,
+            S.fstride);
+}
+
 // ==========================================================================
 // forward: two fused steps per pass (temporal blocking)
 // ==========================================================================
@@ -336,8 +388,7 @@ __device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned off_
 // The left/right PML columns, the grid's edge columns and the chunks that
 // straddle them use a per-cell coefficient table (univ_chunk).
 template <bool FRAMES>
-__global__ void __launch_bounds__(256)
-k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
+__device__ __forceinline__ void fwd2_body(const DevGrid& g, const DevShots& s, int L, TileId tid, const float* __restrict__ E0,
             const float* __restrict__ Hx0, const float* __restrict__ Hy0,
             const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ E2,
             float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
@@ -346,7 +397,6 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
   const int lane = threadIdx.x;
   const int NX = g.NX, NY = g.NY, P = g.P;
   const long long LD = g.LD;
-  const TileId tid = tile_of_block(NX, NY, L, ROWS2);
   const int shot = tid.shot;
   const int r0 = tid.rowtile * ROWS2 - 2;
   // threadIdx.y is constant across a 64-lane wavefront: make the strip start
@@ -706,6 +756,36 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
   for (; j <= je; j += U) univ_chunk(j);
 }
 
+template <bool FRAMES>
+__global__ void __launch_bounds__(256)
+k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
+            const float* __restrict__ Hx0, const float* __restrict__ Hy0,
+            const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ E2,
+            float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
+            float* __restrict__ Y2, int it, float* __restrict__ tr, float* __restrict__ frame1,
+            float* __restrict__ frame2, long long fstride, float* __restrict__ trash) {
+  fwd2_body<FRAMES>(g, s, L, tile_of_block(g.NX, g.NY, L, ROWS2), E0, Hx0, Hy0, X0, Y0, E2, Hx2, Hy2, X2, Y2, it, tr, frame1, frame2, fstride, trash);
+}
+
+template <bool FRAMES>
+__global__ void __launch_bounds__(256)
+k_fwd_step2_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
+  int k;
+  TileId tid;
+  if (!multi_tile(a, sv, ROWS2, k, tid)) return;
+  const DevSurvey& S = sv[a.sid[k]];
+  const int c = a.cur[k], x = a.xc[k];
+  float* f = S.fld;
+  const long long z = S.fsz;
+  float* fr1 = FRAMES ? S.frames + (long long)a.fslot[k] * S.FD : nullptr;
+  float* fr2 = FRAMES ? fr1 + S.FD : nullptr;
+  fwd2_body<FRAMES>(S.g, S.s, a.L[k], tid, f + 3 * c * z, f + (3 * c + 1) * z,
+                    f + (3 * c + 2) * z, f + (6 + 2 * x) * z, f + (7 + 2 * x) * z,
+                    f + 3 * (c ^ 1) * z, f + (3 * (c ^ 1) + 1) * z, f + (3 * (c ^ 1) + 2) * z,
+                    f + (6 + 2 * (x ^ 1)) * z, f + (7 + 2 * (x ^ 1)) * z, a.it[k],
+                    a.rec ? S.tr : nullptr, fr1, fr2, S.fstride, nullptr);
+}
+
 template __global__ void k_fwd_step2<false>(DevGrid, DevShots, int, const float*, const float*,
                                             const float*, const float*, const float*, float*,
                                             float*, float*, float*, float*, int, float*, float*,
@@ -714,6 +794,8 @@ template __global__ void k_fwd_step2<true>(DevGrid, DevShots, int, const float*,
                                            const float*, const float*, const float*, float*,
                                            float*, float*, float*, float*, int, float*, float*,
                                            float*, long long, float*);
+template __global__ void k_fwd_step2_multi<false>(const DevSurvey*, MultiArgs);
+template __global__ void k_fwd_step2_multi<true>(const DevSurvey*, MultiArgs);
 
 // ==========================================================================
 // adjoint: one step of the transpose of k_fwd_step
@@ -743,8 +825,7 @@ __device__ __forceinline__ void w_of(const DevGrid& g, int i, int j, long long k
 // with the data residual of step it already added; M1 receives dJ/dEz(S_it)
 // with the residual of step it-1 added.  F1/F0 are the stored forward frames
 // Ez(S_{it+1}), Ez(S_it) on the domain (F0 may be null for S_0 = 0).
-__global__ void __launch_bounds__(256)
-k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
+__device__ __forceinline__ void adj1_body(const DevGrid& g, const DevShots& s, int L, TileId tid, const float* __restrict__ M0,
            const float* __restrict__ Nx0, const float* __restrict__ Ny0,
            const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ M1,
            float* __restrict__ Nx1, float* __restrict__ Ny1, float* __restrict__ X1,
@@ -753,7 +834,6 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
            float* __restrict__ acc, long long astride, float* __restrict__ gw) {
   const int lane = threadIdx.x;
   const int NX = g.NX, NY = g.NY, P = g.P;
-  const TileId tid = tile_of_block(NX, NY, L);
   const int shot = tid.shot;
   const int i0 = tid.rowtile * 64;
   // threadIdx.y is constant across a 64-lane wavefront: make the strip start
@@ -950,6 +1030,33 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
   }
 }
 
+__global__ void __launch_bounds__(256)
+k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
+           const float* __restrict__ Nx0, const float* __restrict__ Ny0,
+           const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ M1,
+           float* __restrict__ Nx1, float* __restrict__ Ny1, float* __restrict__ X1,
+           float* __restrict__ Y1, int it, const float* __restrict__ adj,
+           const float* __restrict__ F1, const float* __restrict__ F0, long long fstride,
+           float* __restrict__ acc, long long astride, float* __restrict__ gw) {
+  adj1_body(g, s, L, tile_of_block(g.NX, g.NY, L), M0, Nx0, Ny0, X0, Y0, M1, Nx1, Ny1, X1, Y1, it, adj, F1, F0, fstride, acc, astride, gw);
+}
+
+__global__ void __launch_bounds__(256)
+k_adj_step_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
+  int k;
+  TileId tid;
+  if (!multi_tile(a, sv, 64, k, tid)) return;
+  const DevSurvey& S = sv[a.sid[k]];
+  const int c = a.cur[k], d = c ^ 1;
+  float* m = S.adjf;
+  const long long z = S.fsz;
+  const float* F0 = S.frames + (long long)a.fslot[k] * S.FD;
+  adj1_body(S.g, S.s, a.L[k], tid, m + 5 * c * z, m + (5 * c + 1) * z, m + (5 * c + 2) * z,
+            m + (5 * c + 3) * z, m + (5 * c + 4) * z, m + 5 * d * z, m + (5 * d + 1) * z,
+            m + (5 * d + 2) * z, m + (5 * d + 3) * z, m + (5 * d + 4) * z, a.it[k], S.adj,
+            F0 + S.FD, F0, S.fstride, S.acc, S.FD, nullptr);
+}
+
 // ==========================================================================
 // adjoint: two fused steps per pass (temporal blocking)
 // ==========================================================================
@@ -967,8 +1074,7 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
 // dJ/dEz(S_{it+1}) with the residual of step it already added; M2 receives
 // dJ/dEz(S_{it-1}) with the residual of step it-2 added (when it >= 2).
 // Fa, Fb, Fc: forward frames Ez(S_{it+1}), Ez(S_it), Ez(S_{it-1}) on the domain.
-__global__ void __launch_bounds__(256)
-k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
+__device__ __forceinline__ void adj2_body(const DevGrid& g, const DevShots& s, int L, TileId tid, const float* __restrict__ M0,
             const float* __restrict__ Nx0, const float* __restrict__ Ny0,
             const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ M2,
             float* __restrict__ Nx2, float* __restrict__ Ny2, float* __restrict__ X2,
@@ -979,7 +1085,6 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
   const int lane = threadIdx.x;
   const int NX = g.NX, NY = g.NY, P = g.P;
   const long long LD = g.LD;
-  const TileId tid = tile_of_block(NX, NY, L, ROWS2);
   const int shot = tid.shot;
   const int r0 = tid.rowtile * ROWS2 - 2;
   const int jb = __builtin_amdgcn_readfirstlane((tid.strip * blockDim.y + threadIdx.y) * L);
@@ -1403,6 +1508,34 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
     else
       class_chunk(j);
   }
+}
+
+__global__ void __launch_bounds__(256)
+k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
+            const float* __restrict__ Nx0, const float* __restrict__ Ny0,
+            const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ M2,
+            float* __restrict__ Nx2, float* __restrict__ Ny2, float* __restrict__ X2,
+            float* __restrict__ Y2, int it, const float* __restrict__ adj,
+            const float* __restrict__ Fa, const float* __restrict__ Fb,
+            const float* __restrict__ Fc, long long fstride, float* __restrict__ acc,
+            long long astride, float* __restrict__ gw) {
+  adj2_body(g, s, L, tile_of_block(g.NX, g.NY, L, ROWS2), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2, Y2, it, adj, Fa, Fb, Fc, fstride, acc, astride, gw);
+}
+
+__global__ void __launch_bounds__(256)
+k_adj_step2_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
+  int k;
+  TileId tid;
+  if (!multi_tile(a, sv, ROWS2, k, tid)) return;
+  const DevSurvey& S = sv[a.sid[k]];
+  const int c = a.cur[k], d = c ^ 1;
+  float* m = S.adjf;
+  const long long z = S.fsz;
+  const float* Fb = S.frames + (long long)a.fslot[k] * S.FD;
+  adj2_body(S.g, S.s, a.L[k], tid, m + 5 * c * z, m + (5 * c + 1) * z, m + (5 * c + 2) * z,
+            m + (5 * c + 3) * z, m + (5 * c + 4) * z, m + 5 * d * z, m + (5 * d + 1) * z,
+            m + (5 * d + 2) * z, m + (5 * d + 3) * z, m + (5 * d + 4) * z, a.it[k], S.adj,
+            Fb + S.FD, Fb, Fb - S.FD, S.fstride, S.acc, S.FD, nullptr);
 }
 
 // ==========================================================================

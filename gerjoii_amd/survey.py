@@ -268,7 +268,8 @@ class Survey:
         return (g_eps, g_sig, gw) if want_gw else (g_eps, g_sig)
 
     def fwi_gradient(self, d_obs, g_eps=None, g_sig=None):
-        """Forward + L2 misfit + exact gradient, all on the device."""
+        """Forward + misfit (L2 unless set_misfit chose another) + exact
+        gradient, all on the device."""
         m, n = self.grid.m, self.grid.n
         if g_eps is None:
             g_eps = np.zeros((m, n), dtype=np.float64, order="F")
@@ -299,4 +300,56 @@ class Survey:
     def last_timing(self):
         ms, cells = C.c_double(0), C.c_double(0)
         N.check(N.load().gjw_last_timing(self.h, C.byref(ms), C.byref(cells)), "gjw_last_timing")
+        return ms.value, cells.value
+
+
+class Batch:
+    """Several surveys (own grid, model, shots, nt, misfit each) whose FWI
+    gradients run in shared launches (gjw_batch_*)."""
+
+    def __init__(self, surveys):
+        self.surveys = list(surveys)
+        ctx = self.surveys[0].ctx
+        arr = (C.c_void_p * len(self.surveys))(*[sv.h for sv in self.surveys])
+        h = C.c_void_p()
+        N.check(N.load().gjw_batch_create(ctx.h, len(self.surveys), arr, C.byref(h)),
+                "gjw_batch_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            N.load().gjw_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fwi_gradient(self, d_obs, g_eps=None, g_sig=None):
+        """d_obs: one traces array per survey.  Returns (g_eps list, g_sig
+        list, J array)."""
+        S = len(self.surveys)
+        if g_eps is None:
+            g_eps = [np.zeros((sv.grid.m, sv.grid.n), dtype=np.float64, order="F")
+                     for sv in self.surveys]
+        if g_sig is None:
+            g_sig = [np.zeros((sv.grid.m, sv.grid.n), dtype=np.float64, order="F")
+                     for sv in self.surveys]
+        d_obs = [d if hasattr(d, "data_ptr") else np.ascontiguousarray(d, dtype=np.float32)
+                 for d in d_obs]
+        _sync_torch(*d_obs, *g_eps, *g_sig)
+        J = np.zeros(S, dtype=np.float64)
+        P = C.c_void_p * S
+        N.check(N.load().gjw_batch_fwi_gradient(self.h, P(*[N.ptr(d).value for d in d_obs]),
+                                                P(*[N.ptr(g).value for g in g_eps]),
+                                                P(*[N.ptr(g).value for g in g_sig]), N.ptr(J)),
+                "gjw_batch_fwi_gradient")
+        return g_eps, g_sig, J
+
+    def last_timing(self):
+        ms, cells = C.c_double(0), C.c_double(0)
+        N.check(N.load().gjw_batch_last_timing(self.h, C.byref(ms), C.byref(cells)),
+                "gjw_batch_last_timing")
         return ms.value, cells.value

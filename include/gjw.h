@@ -46,6 +46,7 @@ extern "C" {
 
 typedef struct gjw_ctx gjw_ctx;
 typedef struct gjw_survey gjw_survey;
+typedef struct gjw_batch gjw_batch;
 
 /* Geometry and physics of one padded grid.  Field names follow the MATLAB
  * structs (src/w/w_geom.m:78-172, src/joint/wdc_geom.m:43-125). */
@@ -157,6 +158,21 @@ int gjw_fwi_gradient(gjw_survey *s, const float *d_obs, double *g_eps, double *g
 /* Time derivative of an m x n x nt cube in place, second order
  * (src/shared/w-dc/differentiate_cube.m:23-51). */
 int gjw_differentiate_cube(gjw_ctx *ctx, float *v, int32_t nz, int32_t nx, int32_t nt, double dt);
+
+/* Several surveys (own grid, model, shots, nt and misfit each; at most 8,
+ * one context) whose FWI gradients run in shared launches: every time-step
+ * launch covers the tiles of all surveys still active.  Replaces running
+ * w_update_*.m once per survey; no counterpart in the reference. */
+int gjw_batch_create(gjw_ctx *ctx, int32_t n, gjw_survey *const *surveys, gjw_batch **out);
+int gjw_batch_destroy(gjw_batch *b);
+/* gjw_fwi_gradient of every survey of the batch: d_obs[k], g_eps[k], g_sig[k]
+ * as for survey k (g_* entries may be NULL), J[k] its misfit.  The surveys
+ * share one checkpoint interval (the largest GJW_OPT_CKPT_EVERY, else
+ * ceil(sqrt(max nt))). */
+int gjw_batch_fwi_gradient(gjw_batch *b, const float *const *d_obs, double *const *g_eps,
+                           double *const *g_sig, double *J);
+/* Device time and cell-updates of the last gjw_batch_fwi_gradient. */
+int gjw_batch_last_timing(gjw_batch *b, double *ms, double *cells);
 
 /* Device-time of the last gjw_forward / gjw_gradient / gjw_fwi_gradient
  * call and the number of cell-updates it executed. */
