@@ -147,6 +147,20 @@ struct DBuf {  // device buffer, grows on demand
   }
 };
 
+// a captured launch sequence and the addresses / parameters it bakes in
+struct GraphCache {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  std::vector<uintptr_t> key;
+  void reset() {
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+    x = nullptr;
+    g = nullptr;
+    key.clear();
+  }
+};
+
 struct gjw_survey {
   gjw_ctx* ctx = nullptr;
   gjw_grid gr{};
@@ -199,9 +213,7 @@ struct gjw_survey {
   DBuf<float> fin, fov, fP;
   // GJW_OPT_GRAPH: the captured batched FWI gradient and the buffers it uses
   int graph_on = 0, graph_calls = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  std::vector<uintptr_t> graph_key;
+  GraphCache g_all, g_fwd, g_adj;  // whole gradient (L2); forward / adjoint sweeps
   std::vector<hipEvent_t> kev_pool;
   std::vector<std::pair<int, int>> kev_used;  // (kernel id, index of start event)
   double kt_ms[4] = {0, 0, 0, 0};  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2
@@ -282,8 +294,9 @@ extern "C" int gjw_survey_destroy(gjw_survey* s) {
   if (s->fft_r2c) (void)hipfftDestroy(s->fft_r2c);
   if (s->fft_c2c) (void)hipfftDestroy(s->fft_c2c);
   s->fhalf.release(); s->fana.release(); s->fin.release(); s->fov.release(); s->fP.release();
-  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->g_all.reset();
+  s->g_fwd.reset();
+  s->g_adj.reset();
   s->coef.release(); s->coef_ref.release(); s->zero_sc.release(); s->trash.release();
   s->fe.release(); s->fs.release(); s->d_pml.release();
   s->d_si.release(); s->d_sj.release(); s->d_cptr.release(); s->d_crow.release();
@@ -1559,10 +1572,8 @@ static int fwi_batched_prepare(gjw_survey* s) {
   return GJW_OK;
 }
 
-static int fwi_batched_enqueue(gjw_survey* s) {
+static int fwi_fwd_enqueue(gjw_survey* s) {
   const int ns = s->ns, nt = s->gr.nt;
-  const long long per = (long long)s->gr.m * s->gr.n;
-  hipStream_t st = s->ctx->stream;
   DevGrid g = dev_grid(s);
   DevShots sh = dev_shots(s, 0);
   FwdBufs b = fwd_bufs(s, ns);
@@ -1576,10 +1587,49 @@ static int fwi_batched_enqueue(gjw_survey* s) {
     it += taken;
   }
   s->cur = cur;
-  TRY(misfit_enqueue(s, s->misfit_kind));
+  return GJW_OK;
+}
+
+static int fwi_adj_enqueue(gjw_survey* s) {
+  const long long per = (long long)s->gr.m * s->gr.n;
+  hipStream_t st = s->ctx->stream;
   TRY(launch_fill_d(s->ge.p, per, 0.0, st));
   TRY(launch_fill_d(s->gsg.p, per, 0.0, st));
   return grad_batched(s, s->adj.p, nullptr);
+}
+
+static int fwi_batched_enqueue(gjw_survey* s) {
+  TRY(fwi_fwd_enqueue(s));
+  TRY(misfit_enqueue(s, s->misfit_kind));
+  return fwi_adj_enqueue(s);
+}
+
+// Replay the sequence `enqueue` puts on stream st, capturing it first (or
+// again, when the key of addresses and parameters changed).  No timing events
+// are recorded while capturing.
+template <class F>
+static int graph_run(GraphCache& gc, hipStream_t st, const std::vector<uintptr_t>& key,
+                     int* ksample, F&& enqueue) {
+  if (!gc.x || key != gc.key) {
+    gc.reset();
+    const int ks = *ksample;
+    *ksample = 0;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    const int rc = enqueue();
+    hipGraph_t gr = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &gr);
+    *ksample = ks;
+    if (rc != GJW_OK) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc;
+    }
+    HIPCHK(ec);
+    gc.g = gr;
+    HIPCHK(hipGraphInstantiate(&gc.x, gc.g, nullptr, nullptr, 0));
+    gc.key = key;
+  }
+  HIPCHK(hipGraphLaunch(gc.x, st));
+  return GJW_OK;
 }
 
 // every device address and launch parameter the enqueued sequence bakes in
@@ -1602,32 +1652,16 @@ static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
           fbits(s->bx0), fbits(s->by0)};
 }
 
+// L2: the whole gradient is one graph.  Hilbert-based misfits run their FFTs
+// eagerly between a forward-sweep graph and an adjoint-sweep graph.
 static int fwi_graph_run(gjw_survey* s) {
   hipStream_t st = s->ctx->stream;
-  const std::vector<uintptr_t> key = fwi_graph_key(s);
-  if (!s->gexec || key != s->graph_key) {
-    if (s->gexec) HIPCHK(hipGraphExecDestroy(s->gexec));
-    if (s->graph) HIPCHK(hipGraphDestroy(s->graph));
-    s->gexec = nullptr;
-    s->graph = nullptr;
-    const int ks = s->ksample;
-    s->ksample = 0;  // no timing events inside the graph
-    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-    const int rc = fwi_batched_enqueue(s);
-    hipGraph_t gr = nullptr;
-    const hipError_t ec = hipStreamEndCapture(st, &gr);
-    s->ksample = ks;
-    if (rc != GJW_OK) {
-      if (gr) (void)hipGraphDestroy(gr);
-      return rc;
-    }
-    HIPCHK(ec);
-    s->graph = gr;
-    HIPCHK(hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0));
-    s->graph_key = key;
-  }
-  HIPCHK(hipGraphLaunch(s->gexec, st));
-  return GJW_OK;
+  std::vector<uintptr_t> key = fwi_graph_key(s);
+  if (s->misfit_kind == GJW_MISFIT_L2)
+    return graph_run(s->g_all, st, key, &s->ksample, [&] { return fwi_batched_enqueue(s); });
+  TRY(graph_run(s->g_fwd, st, key, &s->ksample, [&] { return fwi_fwd_enqueue(s); }));
+  TRY(misfit_enqueue(s, s->misfit_kind));
+  return graph_run(s->g_adj, st, key, &s->ksample, [&] { return fwi_adj_enqueue(s); });
 }
 
 extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps, double* g_sig,
@@ -1655,9 +1689,7 @@ extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps
     if (rc == GJW_OK) rc = host_or_dev_copy(s->obs.p, d_obs, (long long)s->ntr * s->gr.nt * 4, st);
     // the first call runs eagerly (and settles the autotuner); with
     // GJW_OPT_GRAPH later calls replay the captured sequence
-    // (the Hilbert-based misfits run their FFTs eagerly: no graph for them)
-    const bool use_graph =
-        s->graph_on && s->misfit_kind == GJW_MISFIT_L2 && s->graph_calls++ > 0;
+    const bool use_graph = s->graph_on && s->graph_calls++ > 0;
     if (rc == GJW_OK) rc = use_graph ? fwi_graph_run(s) : fwi_batched_enqueue(s);
     if (rc == GJW_OK && hipEventRecord(s->ctx->ev1, st) != hipSuccess)
       rc = fail(GJW_ERR_HIP, "event record");
@@ -1696,6 +1728,8 @@ struct gjw_batch {
   gjw_ctx* ctx = nullptr;
   std::vector<gjw_survey*> sv;
   DBuf<DevSurvey> dsv;
+  int K = 0, T = 0, fuse = 2, calls = 0;
+  GraphCache g_all, g_fwd, g_adj;
   double last_ms = 0, last_cells = 0;
 };
 
@@ -1722,6 +1756,9 @@ extern "C" int gjw_batch_destroy(gjw_batch* b) {
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
   b->dsv.release();
+  b->g_all.reset();
+  b->g_fwd.reset();
+  b->g_adj.reset();
   delete b;
   return GJW_OK;
 }
@@ -1783,51 +1820,19 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
   return GJW_OK;
 }
 
-extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
-                                      double* const* g_eps, double* const* g_sig, double* J) {
-  if (!b || !d_obs) return fail(GJW_ERR_ARG, "gjw_batch_fwi_gradient: NULL argument");
-  const int S = (int)b->sv.size();
-  HIPCHK(hipSetDevice(b->ctx->device));
-  hipStream_t st = b->ctx->stream;
-  int T = 0, fuse = 2;
+// forward sweep of every survey with checkpoints (shared launches)
+static int batch_fwd_enqueue(gjw_batch* b) {
+  const int S = (int)b->sv.size(), T = b->T, K = b->K;
+  int nt[kMaxSurveys], cur[kMaxSurveys], xc[kMaxSurveys], itv[kMaxSurveys];
   for (int k = 0; k < S; ++k) {
     gjw_survey* s = b->sv[k];
-    TRY(check_ready(s, "gjw_batch_fwi_gradient"));
-    if (!d_obs[k]) return fail(GJW_ERR_ARG, "gjw_batch_fwi_gradient: d_obs[%d] is NULL", k);
-    if (s->path != GJW_PATH_BATCHED)
-      return fail(GJW_ERR_ARG, "gjw_batch_fwi_gradient: survey %d is on the plain path", k);
-    T = std::max(T, s->gr.nt);
-    fuse = std::min(fuse, s->fuse);
-  }
-  int K = 0;
-  for (int k = 0; k < S; ++k) K = std::max(K, b->sv[k]->ck_every);
-  if (K <= 0) K = std::max(1, (int)std::ceil(std::sqrt((double)T)));
-  int nt[kMaxSurveys], cur[kMaxSurveys], xc[kMaxSurveys], itv[kMaxSurveys], fsl[kMaxSurveys];
-  for (int k = 0; k < S; ++k) {
-    gjw_survey* s = b->sv[k];
-    s->ck_batch = K;
-    const int rc = fwi_batched_prepare(s);
-    s->ck_batch = 0;
-    TRY(rc);
-    s->ck_K = K;
-    s->ck_n = (int)cdiv(s->gr.nt, K);
-    TRY(host_or_dev_copy(s->obs.p, d_obs[k], (long long)s->ntr * s->gr.nt * 4, st));
     nt[k] = s->gr.nt;
-  }
-  std::vector<DevSurvey> hs(S);
-  for (int k = 0; k < S; ++k) hs[k] = dev_survey(b->sv[k]);
-  TRY(b->dsv.ensure(S));
-  HIPCHK(hipStreamSynchronize(st));
-  HIPCHK(hipMemcpy(b->dsv.p, hs.data(), S * sizeof(DevSurvey), hipMemcpyHostToDevice));
-  HIPCHK(hipEventRecord(b->ctx->ev0, st));
-  // ---- forward sweep with checkpoints
-  for (int k = 0; k < S; ++k) {
-    TRY(zero_fwd_state(b->sv[k], b->sv[k]->ns));
+    TRY(zero_fwd_state(s, s->ns));
     cur[k] = xc[k] = 0;
   }
   for (int tau = 0; tau < T;) {
     std::vector<int> act;
-    bool two = fuse == 2 && K - tau % K >= 2;
+    bool two = b->fuse == 2 && K - tau % K >= 2;
     for (int k = 0; k < S; ++k)
       if (tau < nt[k]) {
         act.push_back(k);
@@ -1846,20 +1851,32 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
     }
     tau += two ? 2 : 1;
   }
-  // ---- misfits, adjoint initial state
+  for (int k = 0; k < S; ++k) b->sv[k]->cur = cur[k];
+  return GJW_OK;
+}
+
+static int batch_misfit_enqueue(gjw_batch* b, bool l2) {
+  for (gjw_survey* s : b->sv)
+    if ((s->misfit_kind == GJW_MISFIT_L2) == l2) TRY(misfit_enqueue(s, s->misfit_kind));
+  return GJW_OK;
+}
+
+// adjoint sweep of every survey, segment by segment, recomputing the forward
+static int batch_adj_enqueue(gjw_batch* b) {
+  const int S = (int)b->sv.size(), T = b->T, K = b->K;
+  hipStream_t st = b->ctx->stream;
+  int nt[kMaxSurveys], cur[kMaxSurveys], xc[kMaxSurveys], itv[kMaxSurveys], fsl[kMaxSurveys],
+      acur[kMaxSurveys];
   for (int k = 0; k < S; ++k) {
     gjw_survey* s = b->sv[k];
     const long long per = (long long)s->gr.m * s->gr.n, FD = (long long)s->LD * s->gr.n;
-    s->cur = cur[k];
-    TRY(misfit_enqueue(s, s->misfit_kind));
+    nt[k] = s->gr.nt;
+    acur[k] = 0;
     TRY(launch_fill_d(s->ge.p, per, 0.0, st));
     TRY(launch_fill_d(s->gsg.p, per, 0.0, st));
     TRY(launch_fill(s->acc.p, 2LL * s->ns * FD, 0.f, st));
     TRY(adj_init(s, s->ns, 0, s->adj.p));
   }
-  // ---- adjoint sweep, segment by segment, recomputing the forward
-  int acur[kMaxSurveys];
-  for (int k = 0; k < S; ++k) acur[k] = 0;
   const int nseg = (int)cdiv(T, K);
   for (int c = nseg - 1; c >= 0; --c) {
     const int k0 = c * K;
@@ -1867,23 +1884,22 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
     std::vector<int> seg;
     for (int k = 0; k < S; ++k) {
       k1[k] = std::min(k0 + K, nt[k]);
-      if (k0 < nt[k]) {
-        seg.push_back(k);
-        kmax = std::max(kmax, k1[k]);
-        gjw_survey* s = b->sv[k];
-        FwdBufs fb = fwd_bufs(s, s->ns);
-        cur[k] = xc[k] = 0;
-        TRY(ck_restore(s, fb, 0, 0, c));
-        const long long per = (long long)s->gr.m * s->gr.n;
-        DevGrid g = dev_grid(s);
-        hipLaunchKernelGGL(k_domain_frame, dim3((unsigned)cdiv(per * s->ns, 256)), dim3(256), 0, st,
-                           g, s->ns, fb.E[0], s->frames.p, hs[k].fstride);
-        HIPCHK(hipGetLastError());
-      }
+      if (k0 >= nt[k]) continue;
+      seg.push_back(k);
+      kmax = std::max(kmax, k1[k]);
+      gjw_survey* s = b->sv[k];
+      FwdBufs fb = fwd_bufs(s, s->ns);
+      cur[k] = xc[k] = 0;
+      TRY(ck_restore(s, fb, 0, 0, c));
+      const long long per = (long long)s->gr.m * s->gr.n;
+      hipLaunchKernelGGL(k_domain_frame, dim3((unsigned)cdiv(per * s->ns, 256)), dim3(256), 0, st,
+                         dev_grid(s), s->ns, fb.E[0], s->frames.p,
+                         (long long)(K + 1) * s->LD * s->gr.n);
+      HIPCHK(hipGetLastError());
     }
     for (int tau = k0; tau < kmax;) {
       std::vector<int> act;
-      bool two = fuse == 2;
+      bool two = b->fuse == 2;
       for (int k : seg)
         if (tau < k1[k]) {
           act.push_back(k);
@@ -1900,7 +1916,7 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
     }
     for (int tau = kmax - 1; tau >= k0;) {
       std::vector<int> act;
-      bool two = fuse == 2 && tau - 1 >= k0;
+      bool two = b->fuse == 2 && tau - 1 >= k0;
       for (int k : seg)
         if (tau <= k1[k] - 1) {
           act.push_back(k);
@@ -1920,6 +1936,80 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
                          s->gr.n, s->LD, s->di0, s->ns, s->acc.p, FD, s->fe.p, s->fs.p, s->ge.p,
                          s->gsg.p, 1.0);
       HIPCHK(hipGetLastError());
+    }
+  }
+  return GJW_OK;
+}
+
+extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
+                                      double* const* g_eps, double* const* g_sig, double* J) {
+  if (!b || !d_obs) return fail(GJW_ERR_ARG, "gjw_batch_fwi_gradient: NULL argument");
+  const int S = (int)b->sv.size();
+  HIPCHK(hipSetDevice(b->ctx->device));
+  hipStream_t st = b->ctx->stream;
+  int T = 0, fuse = 2, graph = 1, all_l2 = 1, K = 0;
+  for (int k = 0; k < S; ++k) {
+    gjw_survey* s = b->sv[k];
+    TRY(check_ready(s, "gjw_batch_fwi_gradient"));
+    if (!d_obs[k]) return fail(GJW_ERR_ARG, "gjw_batch_fwi_gradient: d_obs[%d] is NULL", k);
+    if (s->path != GJW_PATH_BATCHED)
+      return fail(GJW_ERR_ARG, "gjw_batch_fwi_gradient: survey %d is on the plain path", k);
+    T = std::max(T, s->gr.nt);
+    fuse = std::min(fuse, s->fuse);
+    graph &= s->graph_on;
+    all_l2 &= s->misfit_kind == GJW_MISFIT_L2;
+    K = std::max(K, s->ck_every);
+  }
+  if (K <= 0) K = std::max(1, (int)std::ceil(std::sqrt((double)T)));
+  b->T = T;
+  b->K = K;
+  b->fuse = fuse;
+  for (int k = 0; k < S; ++k) {
+    gjw_survey* s = b->sv[k];
+    s->ck_batch = K;
+    const int rc = fwi_batched_prepare(s);
+    s->ck_batch = 0;
+    TRY(rc);
+    s->ck_K = K;
+    s->ck_n = (int)cdiv(s->gr.nt, K);
+    TRY(host_or_dev_copy(s->obs.p, d_obs[k], (long long)s->ntr * s->gr.nt * 4, st));
+  }
+  std::vector<DevSurvey> hs(S);
+  for (int k = 0; k < S; ++k) hs[k] = dev_survey(b->sv[k]);
+  TRY(b->dsv.ensure(S));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpy(b->dsv.p, hs.data(), S * sizeof(DevSurvey), hipMemcpyHostToDevice));
+  HIPCHK(hipEventRecord(b->ctx->ev0, st));
+  // the first call runs eagerly; with GJW_OPT_GRAPH on every survey, later
+  // calls replay captured sweeps (one graph when every misfit is L2)
+  const bool use_graph = graph && b->calls++ > 0;
+  if (!use_graph) {
+    TRY(batch_fwd_enqueue(b));
+    TRY(batch_misfit_enqueue(b, true));
+    TRY(batch_misfit_enqueue(b, false));
+    TRY(batch_adj_enqueue(b));
+  } else {
+    std::vector<uintptr_t> key = {reinterpret_cast<uintptr_t>(b->dsv.p), (uintptr_t)K,
+                                  (uintptr_t)T, (uintptr_t)fuse};
+    for (gjw_survey* s : b->sv) {
+      const std::vector<uintptr_t> ks = fwi_graph_key(s);
+      key.insert(key.end(), ks.begin(), ks.end());
+      key.push_back((uintptr_t)s->misfit_kind);
+    }
+    int ks0 = 0;
+    if (all_l2) {
+      TRY(graph_run(b->g_all, st, key, &ks0, [&] {
+        TRY(batch_fwd_enqueue(b));
+        TRY(batch_misfit_enqueue(b, true));
+        return batch_adj_enqueue(b);
+      }));
+    } else {
+      TRY(graph_run(b->g_fwd, st, key, &ks0, [&] {
+        TRY(batch_fwd_enqueue(b));
+        return batch_misfit_enqueue(b, true);
+      }));
+      TRY(batch_misfit_enqueue(b, false));
+      TRY(graph_run(b->g_adj, st, key, &ks0, [&] { return batch_adj_enqueue(b); }));
     }
   }
   HIPCHK(hipEventRecord(b->ctx->ev1, st));

@@ -354,8 +354,8 @@ def test_source_recorder(oracle_mod, gpu_ctx):
             assert rel_l2(jz, ref) <= 1e-4
 
 
-@pytest.mark.parametrize("snap", [0, 1])
-def test_fwi_gradient_graph_replays(gpu_ctx, snap):
+@pytest.mark.parametrize("snap,misfit", [(0, "L2"), (1, "L2"), (0, "env_L2")])
+def test_fwi_gradient_graph_replays(gpu_ctx, snap, misfit):
     """GJW_OPT_GRAPH: every replay of the captured batched FWI gradient equals
     the eager result bit for bit, with eager work and synchronisations between
     the replays, and replays follow a change of the model (values, not
@@ -370,6 +370,7 @@ def test_fwi_gradient_graph_replays(gpu_ctx, snap):
     graph = make_survey(case, gpu_ctx, ck=7)
     for sv in (eager, graph):
         sv.set_option(N.GJW_OPT_SNAPSHOT, snap)
+        sv.set_misfit(misfit)
     graph.set_option(N.GJW_OPT_GRAPH, 1)
     models = [(case.eps0, case.sig0), (case.eps0 * 1.05, case.sig0 * 0.9)]
     for eps, sig in models:
@@ -483,3 +484,31 @@ def test_batch_uneven_surveys(gpu_ctx, misfit):
     ge2, gs2, J2 = bt.fwi_gradient(dobs)  # a second call reuses the buffers
     for k in range(len(cases)):
         np.testing.assert_array_equal(ge2[k], ge[k])
+
+
+def test_batch_graph_replays(gpu_ctx):
+    """A batch of uneven surveys with mixed misfits, replayed from captured
+    forward / adjoint sweep graphs (the Hilbert misfits run between them),
+    equals the eager batch bit for bit, call after call."""
+    import torch
+    from gerjoii_amd import _native as N
+    cases = _uneven_cases()
+    dobs, eager, graph = [], [], []
+    for case, misfit in zip(cases, ("env_L2", "L2", "phase_L2")):
+        svt = make_survey(case, gpu_ctx)
+        svt.set_model(case.eps_true, case.sig_true)
+        dobs.append(svt.forward())
+        for lst, on in ((eager, 0), (graph, 1)):
+            sv = make_survey(case, gpu_ctx)
+            sv.set_misfit(misfit)
+            sv.set_option(N.GJW_OPT_GRAPH, on)
+            lst.append(sv)
+    ge0, gs0, J0 = S.Batch(eager).fwi_gradient(dobs)
+    bg = S.Batch(graph)
+    for rep in range(3):
+        ge, gs, J = bg.fwi_gradient(dobs)
+        torch.cuda.synchronize()
+        for k in range(len(cases)):
+            np.testing.assert_array_equal(ge[k], ge0[k])
+            np.testing.assert_array_equal(gs[k], gs0[k])
+        np.testing.assert_array_equal(J, J0)
