@@ -145,10 +145,16 @@ def main():
         v["share_of_step"] = v["avg_ms"] * v["launches_per_step"] / ms_step if ms_step > 0 else None
     dom = max(kern, key=lambda k: kern[k]["avg_ms"] * kern[k]["launches_per_step"])
     traffic = _traffic_from_profiles(dom, cells, ns)
+    for k, v in kern.items():  # measured HBM bytes (committed PMC summary) / live duration
+        t = _traffic_from_profiles(k, cells, ns)
+        v["traffic_bytes"] = t
+        v["hbm_GBs"] = (t / (v["avg_ms"] * 1e-3) / 1e9) if (t and v["avg_ms"] > 0) else None
     roof = {"kernel": dom, "bound": "hbm", "achieved": kern[dom]["achieved_GBs"],
             "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": (kern[dom]["achieved_GBs"] or 0) / PEAK_HBM_GBS,
             "traffic": traffic,
+            "hbm_GBs_measured_traffic": kern[dom]["hbm_GBs"],
+            "hbm_frac_measured_traffic": (kern[dom]["hbm_GBs"] or 0) / PEAK_HBM_GBS,
             "bytes_per_launch_algorithmic": bytes_launch * kern[dom]["steps_per_launch"],
             "kernels": kern}
 
