@@ -41,7 +41,7 @@ def main():
     ap.add_argument("--nt", type=int, default=8000)
     ap.add_argument("--plain-shots", type=int, default=2)
     ap.add_argument("--cpu-shots", type=int, default=16)
-    ap.add_argument("--cpu-nt", type=int, default=160)
+    ap.add_argument("--cpu-nt", type=int, default=480)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
