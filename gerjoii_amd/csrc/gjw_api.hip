@@ -37,25 +37,24 @@ __global__ void k_adj_step2(DevGrid, DevShots, int, const float*, const float*, 
                             const float*, const float*, float*, float*, float*, float*, float*,
                             int, const float*, const float*, const float*, const float*,
                             long long, float*, long long, float*);
+#define GJW_FWDN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \
+    const float*, const float*, float*, float*, float*, float*, float*, int, float*, float*, long long
 template <bool FRAMES>
-__global__ void k_fwd_step2(DevGrid, DevShots, int, const float*, const float*, const float*,
-                            const float*, const float*, float*, float*, float*, float*, float*, int,
-                            float*, float*, float*, long long, float*);
-extern template __global__ void k_fwd_step2<false>(DevGrid, DevShots, int, const float*,
-                                                   const float*, const float*, const float*,
-                                                   const float*, float*, float*, float*, float*,
-                                                   float*, int, float*, float*, float*, long long,
-                                                   float*);
-extern template __global__ void k_fwd_step2<true>(DevGrid, DevShots, int, const float*,
-                                                  const float*, const float*, const float*,
-                                                  const float*, float*, float*, float*, float*,
-                                                  float*, int, float*, float*, float*, long long,
-                                                  float*);
+__global__ void k_fwd_step2(GJW_FWDN_ARGS);
+template <bool FRAMES>
+__global__ void k_fwd_step4(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step2<false>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step2<true>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step4<false>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step4<true>(GJW_FWDN_ARGS);
+#undef GJW_FWDN_ARGS
 __global__ void k_fwd_step_multi(const DevSurvey*, MultiArgs);
-template <bool FRAMES>
-__global__ void k_fwd_step2_multi(const DevSurvey*, MultiArgs);
-extern template __global__ void k_fwd_step2_multi<false>(const DevSurvey*, MultiArgs);
-extern template __global__ void k_fwd_step2_multi<true>(const DevSurvey*, MultiArgs);
+template <int NS, bool FRAMES>
+__global__ void k_fwd_stepN_multi(const DevSurvey*, MultiArgs);
+extern template __global__ void k_fwd_stepN_multi<2, false>(const DevSurvey*, MultiArgs);
+extern template __global__ void k_fwd_stepN_multi<2, true>(const DevSurvey*, MultiArgs);
+extern template __global__ void k_fwd_stepN_multi<4, false>(const DevSurvey*, MultiArgs);
+extern template __global__ void k_fwd_stepN_multi<4, true>(const DevSurvey*, MultiArgs);
 __global__ void k_adj_step_multi(const DevSurvey*, MultiArgs);
 __global__ void k_adj_step2_multi(const DevSurvey*, MultiArgs);
 __global__ void k_fill(float*, long long, float);
@@ -168,16 +167,16 @@ struct gjw_survey {
   long long FS = 0;
   bool have_model = false, have_shots = false;
   // options
-  int path = GJW_PATH_BATCHED, ck_every = 0, tile_L = 0, snap = GJW_SNAP_F32, fuse = 2;
+  int path = GJW_PATH_BATCHED, ck_every = 0, tile_L = 0, snap = GJW_SNAP_F32, fuse = 0;
   int ck_batch = 0;  // checkpoint interval imposed by gjw_batch_fwi_gradient
-  int tuned_L = 0, tuned_L1 = 0;  // two-step and one-step kernels
+  int tuned_L = 0, tuned_L1 = 0, tuned_L4 = 0;  // two-, one- and four-step kernels
+  int fuse_eff = 0;  // forward steps per launch in use (fuse, or the autotuner's choice)
   // model, padded (NX x NY, column-major, FP64)
   std::vector<double> pe, ps;
   float bx0 = 0.f, by0 = 0.f;
   DBuf<float> coef;  // cc (2), cxe, cxh, cye, cyh, hax, hbx, hay, hby, uc0 (4), uc1 (4), uc2
   DBuf<float> coef_ref;  // ce, ch with sigma negated (w_grad.m:26-27)
   DBuf<float> zero_sc;   // per-shot zero source coefficients
-  DBuf<float> trash;     // write target of lanes that own no cell (see k_fwd_step2)
   DBuf<double> fe, fs;
   std::vector<int> pml_idx;
   DBuf<int> d_pml;
@@ -216,9 +215,10 @@ struct gjw_survey {
   GraphCache g_all, g_fwd, g_adj;  // whole gradient (L2); forward / adjoint sweeps
   std::vector<hipEvent_t> kev_pool;
   std::vector<std::pair<int, int>> kev_used;  // (kernel id, index of start event)
-  double kt_ms[4] = {0, 0, 0, 0};  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2
-  long long kt_n[4] = {0, 0, 0, 0};
-  long long kl_n[4] = {0, 0, 0, 0};  // all launches while timing is on
+  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2, k_fwd_step4
+  double kt_ms[5] = {0, 0, 0, 0, 0};
+  long long kt_n[5] = {0, 0, 0, 0, 0};
+  long long kl_n[5] = {0, 0, 0, 0, 0};  // all launches while timing is on
 };
 
 static inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
@@ -297,7 +297,7 @@ extern "C" int gjw_survey_destroy(gjw_survey* s) {
   s->g_all.reset();
   s->g_fwd.reset();
   s->g_adj.reset();
-  s->coef.release(); s->coef_ref.release(); s->zero_sc.release(); s->trash.release();
+  s->coef.release(); s->coef_ref.release(); s->zero_sc.release();
   s->fe.release(); s->fs.release(); s->d_pml.release();
   s->d_si.release(); s->d_sj.release(); s->d_cptr.release(); s->d_crow.release();
   s->d_cid.release(); s->d_trs.release(); s->d_trr.release(); s->d_trc.release();
@@ -336,9 +336,11 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       s->graph_calls = 0;
       return GJW_OK;
     case GJW_OPT_FUSE:
-      if (v != 1 && v != 2) return fail(GJW_ERR_ARG, "GJW_OPT_FUSE must be 1 or 2");
+      if (v != 0 && v != 1 && v != 2 && v != 4)
+        return fail(GJW_ERR_ARG, "GJW_OPT_FUSE must be 0 (autotuned), 1, 2 or 4");
       s->fuse = (int)v;
       s->tuned_L = 0;
+      s->fuse_eff = 0;
       break;
     case GJW_OPT_KTIMING:
       if (v < 0) return fail(GJW_ERR_ARG, "GJW_OPT_KTIMING must be >= 0");
@@ -757,7 +759,7 @@ static int kev_begin(gjw_survey* s, int which, int it, int* slot) {
   s->kl_n[which] += 1;
   // two-step kernels advance `it` by 2: accept either residue so that one
   // launch in every ksample steps is timed
-  const bool two = which == 2 || which == 3;
+  const bool two = which >= 2;  // (four-step launches hit residue 0 or 1 once per 2 windows)
   if (it % s->ksample > 1 || (it % s->ksample == 1 && !two)) return GJW_OK;
   const size_t need = 2 * (s->kev_used.size() + 1);
   while (s->kev_pool.size() < need) {
@@ -802,48 +804,62 @@ static int fwd_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int n
   return kev_end(s, slot);
 }
 
-// two steps per launch; grid of 60-row overlapped tiles
+// NS steps per launch; grid of (64 - 2 NS)-row overlapped tiles
+static inline dim3 stepN_grid(const gjw_survey* s, int NS, int L, int nshots) {
+  return dim3((unsigned)(cdiv(s->NX + 1, 64 - 2 * NS) * cdiv(cdiv(s->NY + 1, L), 4) * nshots));
+}
 static inline dim3 step2_grid(const gjw_survey* s, int L, int nshots) {
-  return dim3((unsigned)(cdiv(s->NX + 1, 60) * cdiv(cdiv(s->NY + 1, L), 4) * nshots));
+  return stepN_grid(s, 2, L, nshots);
 }
 
-static int fwd2_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, int L,
-                       FwdBufs& b, int cur, int xc, int it, float* tr, float* fr1, float* fr2,
+// NS = 2 or 4 fused forward steps; fr1 (optional) receives the NS domain
+// frames Ez(S_{it+1}) .. Ez(S_{it+NS}) at fr1 + l * LD * n
+static int fwdN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& sh, int nsh, int L,
+                       FwdBufs& b, int cur, int xc, int it, float* tr, float* fr1,
                        long long fstride) {
   int slot;
-  TRY(s->trash.ensure(256));
-  TRY(kev_begin(s, 2, it, &slot));
-  if (fr1)
-    hipLaunchKernelGGL(k_fwd_step2<true>, step2_grid(s, L, nsh), dim3(64, 4), 0, s->ctx->stream,
-                       g, sh, L, b.E[cur], b.Hx[cur], b.Hy[cur], b.X[xc], b.Y[xc], b.E[cur ^ 1],
-                       b.Hx[cur ^ 1], b.Hy[cur ^ 1], b.X[xc ^ 1], b.Y[xc ^ 1], it, tr, fr1, fr2,
-                       fstride, s->trash.p);
-  else
-    hipLaunchKernelGGL(k_fwd_step2<false>, step2_grid(s, L, nsh), dim3(64, 4), 0, s->ctx->stream,
-                       g, sh, L, b.E[cur], b.Hx[cur], b.Hy[cur], b.X[xc], b.Y[xc], b.E[cur ^ 1],
-                       b.Hx[cur ^ 1], b.Hy[cur ^ 1], b.X[xc ^ 1], b.Y[xc ^ 1], it, tr, fr1, fr2,
-                       fstride, s->trash.p);
+  TRY(kev_begin(s, NS == 4 ? 4 : 2, it, &slot));
+  const dim3 grid = stepN_grid(s, NS, L, nsh), block(64, 4);
+  hipStream_t st = s->ctx->stream;
+#define GJW_FWDN_CALL(K)                                                                     \
+  hipLaunchKernelGGL(K, grid, block, 0, st, g, sh, L, b.E[cur], b.Hx[cur], b.Hy[cur], b.X[xc], \
+                     b.Y[xc], b.E[cur ^ 1], b.Hx[cur ^ 1], b.Hy[cur ^ 1], b.X[xc ^ 1],        \
+                     b.Y[xc ^ 1], it, tr, fr1, fstride)
+  if (NS == 4) {
+    if (fr1)
+      GJW_FWDN_CALL(k_fwd_step4<true>);
+    else
+      GJW_FWDN_CALL(k_fwd_step4<false>);
+  } else {
+    if (fr1)
+      GJW_FWDN_CALL(k_fwd_step2<true>);
+    else
+      GJW_FWDN_CALL(k_fwd_step2<false>);
+  }
+#undef GJW_FWDN_CALL
   HIPCHK(hipGetLastError());
   return kev_end(s, slot);
 }
 
-// Advance the batched forward state from step it by up to `limit` steps (one
-// or two per launch); frame (optional) receives Ez(S_{it+1}) and, for a
-// two-step launch, Ez(S_{it+2}) at frame + fstep.  Returns the steps taken.
+// Advance the batched forward state from step it by up to `limit` steps (4,
+// 2 or 1 per launch, at most the depth in use); frame (optional) receives
+// Ez(S_{it+1}), Ez(S_{it+2}), .. at frame, frame + fstep, ..  Returns the
+// steps taken.
 static int fwd_advance(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, FwdBufs& b,
                        int& cur, int& xc, int it, int limit, float* tr, float* frame,
                        long long fstep, long long fstride, int* taken) {
-  if (s->fuse == 2 && limit >= 2) {
-    TRY(fwd2_launch(s, g, sh, nsh, s->tuned_L, b, cur, xc, it, tr, frame,
-                    frame ? frame + fstep : nullptr, fstride));
-    cur ^= 1;
+  (void)fstep;  // frame slots are LD * n apart, the kernels know it
+  const int fz = s->fuse_eff > 0 ? s->fuse_eff : 2;
+  const int NS = (fz >= 4 && limit >= 4) ? 4 : (fz >= 2 && limit >= 2) ? 2 : 1;
+  if (NS > 1) {
+    TRY(fwdN_launch(s, NS, g, sh, nsh, NS == 4 ? s->tuned_L4 : s->tuned_L, b, cur, xc, it, tr,
+                    frame, fstride));
     xc ^= 1;
-    *taken = 2;
   } else {
     TRY(fwd_launch(s, g, sh, nsh, s->tuned_L1, b, cur, it, tr, frame, fstride, xc));
-    cur ^= 1;
-    *taken = 1;
   }
+  cur ^= 1;
+  *taken = NS;
   return GJW_OK;
 }
 
@@ -884,15 +900,17 @@ static int check_ready(gjw_survey* s, const char* who) {
   return GJW_OK;
 }
 
-// Tile width L (columns per wavefront): time a few launches of each candidate
-// on zeroed batch state and keep the fastest, for the two-step kernel and for
-// the one-step kernel (odd tails, plain path).
+// Tile width L (columns per wavefront) of the one-, two- and four-step
+// forward kernels, and the forward fusion depth: time a few launches of each
+// candidate on zeroed batch state and keep the fastest; with GJW_OPT_FUSE = 0
+// the depth is 4 if one four-step launch beats two two-step launches.
 static int autotune_L(gjw_survey* s, int nsh) {
   if (s->tile_L > 0) {
-    s->tuned_L = s->tuned_L1 = s->tile_L;
+    s->tuned_L = s->tuned_L1 = s->tuned_L4 = s->tile_L;
+    s->fuse_eff = s->fuse > 0 ? s->fuse : 2;
     return GJW_OK;
   }
-  if (s->tuned_L > 0 && s->tuned_L1 > 0) return GJW_OK;
+  if (s->tuned_L > 0 && s->tuned_L1 > 0 && s->tuned_L4 > 0 && s->fuse_eff > 0) return GJW_OK;
   hipStream_t st = s->ctx->stream;
   DevGrid g = dev_grid(s);
   DevShots sh = dev_shots(s, 0);
@@ -900,22 +918,23 @@ static int autotune_L(gjw_survey* s, int nsh) {
   const int ks = s->ksample;
   s->ksample = 0;
   const int reps = 4;
-  for (int which = 1; which <= 2; ++which) {
-    const int c1[] = {8, 16, 32, 64}, c2[] = {16, 32, 64, 128};
-    const int* cand = which == 1 ? c1 : c2;
+  float best_of[5] = {0, 0, 0, 0, 0};
+  for (int NS : {1, 2, 4}) {
+    const int c1[] = {8, 16, 32, 64}, c2[] = {12, 16, 24, 32}, c4[] = {16, 24, 32, 48};
+    const int* cand = NS == 1 ? c1 : NS == 2 ? c2 : c4;
     float best = 1e30f;
-    int bestL = which == 1 ? 16 : 32;
+    int bestL = 16;
     for (int ci = 0; ci < 4; ++ci) {
       const int L = cand[ci];
       int cur = 0, xc = 0;
       for (int r = -1; r < reps; ++r) {
         if (r == 0) HIPCHK(hipEventRecord(s->ctx->ev0, st));
-        if (which == 1)
+        if (NS == 1)
           TRY(fwd_launch(s, g, sh, nsh, L, b, cur, 0, nullptr, nullptr, 0, xc));
         else
-          TRY(fwd2_launch(s, g, sh, nsh, L, b, cur, xc, 0, nullptr, nullptr, nullptr, 0));
+          TRY(fwdN_launch(s, NS, g, sh, nsh, L, b, cur, xc, 0, nullptr, nullptr, 0));
         cur ^= 1;
-        if (which == 2) xc ^= 1;
+        if (NS > 1) xc ^= 1;
       }
       HIPCHK(hipEventRecord(s->ctx->ev1, st));
       HIPCHK(hipEventSynchronize(s->ctx->ev1));
@@ -926,8 +945,10 @@ static int autotune_L(gjw_survey* s, int nsh) {
         bestL = L;
       }
     }
-    (which == 1 ? s->tuned_L1 : s->tuned_L) = bestL;
+    (NS == 1 ? s->tuned_L1 : NS == 2 ? s->tuned_L : s->tuned_L4) = bestL;
+    best_of[NS] = best;
   }
+  s->fuse_eff = s->fuse > 0 ? s->fuse : (best_of[4] < 2.f * best_of[2] ? 4 : 2);
   s->ksample = ks;
   return GJW_OK;
 }
@@ -1317,7 +1338,7 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
     }
     for (int it = k1 - 1; it >= k0;) {
       const float* F = s->frames.p + (long long)(it - k0) * FD;  // Ez(S_it)
-      if (s->fuse == 2 && it - 1 >= k0) {
+      if (s->fuse_eff >= 2 && it - 1 >= k0) {
         TRY(adj2_launch(s, g, sh, ns, s->tuned_L, ab, acur, it, adj, F + FD, F, F - FD, fstride,
                         s->acc.p, FD, gw));
         it -= 2;
@@ -1645,8 +1666,9 @@ static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
           u(s->acc.p),    u(s->adjf.p),  u(s->coef.p),   u(s->fe.p),    u(s->fs.p),
           u(s->d_pml.p),  u(s->d_si.p),  u(s->d_sj.p),   u(s->d_sc.p),  u(s->d_wav.p),
           u(s->d_cptr.p), u(s->d_crow.p), u(s->d_cid.p), u(s->d_trs.p), u(s->d_trr.p),
-          u(s->d_trc.p),  u(s->trash.p), (uintptr_t)s->ns, (uintptr_t)s->ntr,
+          u(s->d_trc.p),  (uintptr_t)s->ns, (uintptr_t)s->ntr,
           (uintptr_t)s->gr.nt, (uintptr_t)s->tuned_L, (uintptr_t)s->tuned_L1,
+          (uintptr_t)s->tuned_L4, (uintptr_t)s->fuse_eff,
           (uintptr_t)s->ck_K, (uintptr_t)s->fuse, (uintptr_t)s->snap,
           (uintptr_t)s->rrow_min, (uintptr_t)s->rrow_max, (uintptr_t)s->pml_idx.size(),
           fbits(s->bx0), fbits(s->by0)};
@@ -1788,10 +1810,10 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
   a.n = (int)ids.size();
   a.rec = rec;
   int blk = 0;
-  const int rows = steps == 2 ? 60 : 64;
+  const int rows = steps == 1 ? 64 : 64 - 2 * steps;
   for (int k = 0; k < a.n; ++k) {
     gjw_survey* s = b->sv[ids[k]];
-    const int L = steps == 2 ? s->tuned_L : s->tuned_L1;
+    const int L = steps == 4 ? s->tuned_L4 : steps == 2 ? s->tuned_L : s->tuned_L1;
     a.blk0[k] = blk;
     a.sid[k] = ids[k];
     a.it[k] = it[ids[k]];
@@ -1808,10 +1830,14 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
   const dim3 grid((unsigned)blk), block(64, 4);
   if (!adj && steps == 1)
     hipLaunchKernelGGL(k_fwd_step_multi, grid, block, 0, st, b->dsv.p, a);
+  else if (!adj && steps == 4 && frames)
+    hipLaunchKernelGGL((k_fwd_stepN_multi<4, true>), grid, block, 0, st, b->dsv.p, a);
+  else if (!adj && steps == 4)
+    hipLaunchKernelGGL((k_fwd_stepN_multi<4, false>), grid, block, 0, st, b->dsv.p, a);
   else if (!adj && frames)
-    hipLaunchKernelGGL(k_fwd_step2_multi<true>, grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_fwd_stepN_multi<2, true>), grid, block, 0, st, b->dsv.p, a);
   else if (!adj)
-    hipLaunchKernelGGL(k_fwd_step2_multi<false>, grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_fwd_stepN_multi<2, false>), grid, block, 0, st, b->dsv.p, a);
   else if (steps == 1)
     hipLaunchKernelGGL(k_adj_step_multi, grid, block, 0, st, b->dsv.p, a);
   else
@@ -1832,7 +1858,7 @@ static int batch_fwd_enqueue(gjw_batch* b) {
   }
   for (int tau = 0; tau < T;) {
     std::vector<int> act;
-    bool two = b->fuse == 2 && K - tau % K >= 2;
+    int room = std::min(b->fuse, K - tau % K);  // steps this launch may take
     for (int k = 0; k < S; ++k)
       if (tau < nt[k]) {
         act.push_back(k);
@@ -1841,15 +1867,16 @@ static int batch_fwd_enqueue(gjw_batch* b) {
           FwdBufs fb = fwd_bufs(s, s->ns);
           TRY(ck_save(s, fb, cur[k], xc[k], tau / K));
         }
-        if (nt[k] - tau < 2) two = false;
+        room = std::min(room, nt[k] - tau);
         itv[k] = tau;
       }
-    TRY(batch_launch(b, act, two ? 2 : 1, false, false, itv, cur, xc, nullptr, 1));
+    const int steps = room >= 4 ? 4 : room >= 2 ? 2 : 1;
+    TRY(batch_launch(b, act, steps, false, false, itv, cur, xc, nullptr, 1));
     for (int k : act) {
       cur[k] ^= 1;
-      if (two) xc[k] ^= 1;
+      if (steps > 1) xc[k] ^= 1;
     }
-    tau += two ? 2 : 1;
+    tau += steps;
   }
   for (int k = 0; k < S; ++k) b->sv[k]->cur = cur[k];
   return GJW_OK;
@@ -1899,24 +1926,25 @@ static int batch_adj_enqueue(gjw_batch* b) {
     }
     for (int tau = k0; tau < kmax;) {
       std::vector<int> act;
-      bool two = b->fuse == 2;
+      int room = b->fuse;
       for (int k : seg)
         if (tau < k1[k]) {
           act.push_back(k);
-          if (k1[k] - tau < 2) two = false;
+          room = std::min(room, k1[k] - tau);
           itv[k] = tau;
           fsl[k] = tau + 1 - k0;
         }
-      TRY(batch_launch(b, act, two ? 2 : 1, false, true, itv, cur, xc, fsl, 0));
+      const int steps = room >= 4 ? 4 : room >= 2 ? 2 : 1;
+      TRY(batch_launch(b, act, steps, false, true, itv, cur, xc, fsl, 0));
       for (int k : act) {
         cur[k] ^= 1;
-        if (two) xc[k] ^= 1;
+        if (steps > 1) xc[k] ^= 1;
       }
-      tau += two ? 2 : 1;
+      tau += steps;
     }
     for (int tau = kmax - 1; tau >= k0;) {
       std::vector<int> act;
-      bool two = b->fuse == 2 && tau - 1 >= k0;
+      bool two = b->fuse >= 2 && tau - 1 >= k0;
       for (int k : seg)
         if (tau <= k1[k] - 1) {
           act.push_back(k);
@@ -1955,7 +1983,6 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
     if (s->path != GJW_PATH_BATCHED)
       return fail(GJW_ERR_ARG, "gjw_batch_fwi_gradient: survey %d is on the plain path", k);
     T = std::max(T, s->gr.nt);
-    fuse = std::min(fuse, s->fuse);
     graph &= s->graph_on;
     all_l2 &= s->misfit_kind == GJW_MISFIT_L2;
     K = std::max(K, s->ck_every);
@@ -1963,18 +1990,20 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
   if (K <= 0) K = std::max(1, (int)std::ceil(std::sqrt((double)T)));
   b->T = T;
   b->K = K;
-  b->fuse = fuse;
+  fuse = 4;
   for (int k = 0; k < S; ++k) {
     gjw_survey* s = b->sv[k];
     s->ck_batch = K;
     const int rc = fwi_batched_prepare(s);
     s->ck_batch = 0;
     TRY(rc);
+    fuse = std::min(fuse, s->fuse_eff);  // (after the autotuner)
     s->ck_K = K;
     s->ck_n = (int)cdiv(s->gr.nt, K);
     TRY(host_or_dev_copy(s->obs.p, d_obs[k], (long long)s->ntr * s->gr.nt * 4, st));
   }
   std::vector<DevSurvey> hs(S);
+  b->fuse = fuse;
   for (int k = 0; k < S; ++k) hs[k] = dev_survey(b->sv[k]);
   TRY(b->dsv.ensure(S));
   HIPCHK(hipStreamSynchronize(st));
@@ -2041,7 +2070,7 @@ extern "C" int gjw_batch_last_timing(gjw_batch* b, double* ms, double* cells) {
 
 extern "C" int gjw_kernel_stats(gjw_survey* s, int32_t which, double* avg_ms, int64_t* count,
                                 int64_t* launches, int32_t reset) {
-  if (!s || which < 0 || which > 3) return fail(GJW_ERR_ARG, "gjw_kernel_stats: bad argument");
+  if (!s || which < 0 || which > 4) return fail(GJW_ERR_ARG, "gjw_kernel_stats: bad argument");
   TRY(kev_resolve(s));
   if (avg_ms) *avg_ms = s->kt_n[which] ? s->kt_ms[which] / s->kt_n[which] : 0.0;
   if (count) *count = s->kt_n[which];

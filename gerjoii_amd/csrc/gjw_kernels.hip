@@ -387,18 +387,32 @@ __device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned off_
 //     (coefficients depend on eps(i, j) and are loaded per cell).
 // The left/right PML columns, the grid's edge columns and the chunks that
 // straddle them use a per-cell coefficient table (univ_chunk).
-template <bool FRAMES>
-__device__ __forceinline__ void fwd2_body(const DevGrid& g, const DevShots& s, int L, TileId tid, const float* __restrict__ E0,
-            const float* __restrict__ Hx0, const float* __restrict__ Hy0,
-            const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ E2,
-            float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
-            float* __restrict__ Y2, int it, float* __restrict__ tr, float* __restrict__ frame1,
-            float* __restrict__ frame2, long long fstride, float* __restrict__ trash) {
+// NS fused forward steps per pass (NS = 2 or 4).  A wavefront's 64 lanes
+// hold rows r0 .. r0+63 with r0 = (64 - 2 NS) t - NS; lanes NS .. 63-NS own
+// their rows, the outer lanes recompute the neighbours' rows the later steps
+// need.  Level l (step it+l) runs l columns behind level 0 along the march,
+// so every level reads its inputs from registers: the field values of the
+// previous level's current column and of its own previous column.  Only the
+// last level stores the fields; every level records receivers, the source
+// recorder and (FRAMES, recompute sweep) its domain frame, slot l at
+// frame1 + l * LD * n.
+template <int NS, bool FRAMES>
+__device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, int L, TileId tid,
+                                          const float* __restrict__ E0,
+                                          const float* __restrict__ Hx0,
+                                          const float* __restrict__ Hy0,
+                                          const float* __restrict__ X0,
+                                          const float* __restrict__ Y0, float* __restrict__ E2,
+                                          float* __restrict__ Hx2, float* __restrict__ Hy2,
+                                          float* __restrict__ X2, float* __restrict__ Y2, int it,
+                                          float* __restrict__ tr, float* __restrict__ frame1,
+                                          long long fstride) {
+  constexpr int R = 64 - 2 * NS;
   const int lane = threadIdx.x;
   const int NX = g.NX, NY = g.NY, P = g.P;
   const long long LD = g.LD;
   const int shot = tid.shot;
-  const int r0 = tid.rowtile * ROWS2 - 2;
+  const int r0 = tid.rowtile * R - NS;
   // threadIdx.y is constant across a 64-lane wavefront: make the strip start
   // (and everything derived from it) scalar so loop control stays uniform
   const int jb = __builtin_amdgcn_readfirstlane((tid.strip * blockDim.y + threadIdx.y) * L);
@@ -406,17 +420,16 @@ __device__ __forceinline__ void fwd2_body(const DevGrid& g, const DevShots& s, i
   const int je = min(jb + L, NY + 1);
   const int i = r0 + lane;
   const bool okrow = i >= 0 && i <= NX;
-  const bool own = okrow && lane >= 2 && lane <= 61;
+  const bool own = okrow && lane >= NS && lane <= 63 - NS;
   const long long so = (long long)shot * g.FS;
   E0 += so; Hx0 += so; Hy0 += so; X0 += so; Y0 += so;
   E2 += so; Hx2 += so; Hy2 += so; X2 += so; Y2 += so;
   const int si = s.si[shot], sj = s.sj[shot];
-  const float ws1 = s.sc[shot] * s.wav[(long long)shot * s.nt + it];
-  const float ws2 = s.sc[shot] * s.wav[(long long)shot * s.nt + it + 1];
+  float ws[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) ws[l] = s.sc[shot] * s.wav[(long long)shot * s.nt + it + l];
   const int* cp = s.cptr + (long long)shot * (NY + 2);
-  float* fr1 = FRAMES ? frame1 + (long long)shot * fstride : nullptr;
-  float* fr2 = FRAMES ? frame2 + (long long)shot * fstride : nullptr;
-  const bool has_rec = tr && r0 + 2 <= s.rmax && r0 + 61 >= s.rmin;
+  const bool has_rec = tr && r0 + NS <= s.rmax && r0 + 63 - NS >= s.rmin;
   const bool inner_rows = (r0 >= P + 1) && (r0 + 63 <= NX - P - 1);
   const bool dom_row = i - g.di0 >= 0 && i - g.di0 < g.m;
   auto dom_col = [&](int c) { return c - g.dj0 >= 0 && c - g.dj0 < g.n; };
@@ -426,6 +439,9 @@ __device__ __forceinline__ void fwd2_body(const DevGrid& g, const DevShots& s, i
   // source recorder (w_solve.m:104): Ez at the source cell after each step
   const __amdgpu_buffer_rsrc_t rJZ =
       rsrc_of(s.jz ? s.jz + (long long)shot * s.nt : nullptr, s.jz ? s.nt : 0);
+  // domain frames of the NS steps (slot l at l * FD)
+  const __amdgpu_buffer_rsrc_t rFR =
+      rsrc_of(FRAMES ? frame1 + (long long)shot * fstride : nullptr, FRAMES ? NS * FD : 0);
 
   const __amdgpu_buffer_rsrc_t rE0 = rsrc_of(const_cast<float*>(E0), g.FS),
                                rHx0 = rsrc_of(const_cast<float*>(Hx0), g.FS),
@@ -477,27 +493,58 @@ __device__ __forceinline__ void fwd2_body(const DevGrid& g, const DevShots& s, i
     for (int q = qa; q < qb; ++q)
       if (s.crow[q] == i) tr[(long long)s.cid[q] * s.nt + step] = e;
   };
+  // per-level outputs that leave the kernel: receivers, source recorder, frame
+  auto level_out = [&](int l, int cl, float e) {
+    const bool ol = own && cl >= jb && cl < je;
+    if (has_rec && ol) record(cl, e, it + l);
+    bstore(e, rJZ, (unsigned)(it + l), ol && i == si && cl == sj);
+    if (FRAMES)
+      bstore(e, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + ic), ol && dom_row && dom_col(cl));
+  };
 
-  // ---- prologue: step 1 starts at column jb-1; it needs Hx^{n+1}(jb-2).
-  // Branch-free loads, so they overlap with the first chunk's.
-  const int j1 = jb - 1;
-  float e0_c, hx1_prev;  // E^n(i, j), Hx^{n+1}(i, j-1)
+  // ---- state.  Level 0 steps column c from the loaded fields; level l >= 1
+  // steps column c-l from level l-1's outputs (its previous column: *_in[l];
+  // its current column: this iteration's output) with the coefficients of
+  // column c-l carried along (*_l[l]).  hxp[l]: Hx of level l at its previous
+  // column.  Level 0 starts at column jb-NS+1, which needs Hx^{n+1}(jb-NS).
+  const int j1 = jb - (NS - 1);
+  float e0_c, e_in[NS], hx_in[NS], hy_in[NS], x_in[NS], y_in[NS], hxp[NS];
+  float2 cc_l[NS];
+  float cxe_l[NS], cxh_l[NS], cyh_l[NS], hay_l[NS], hby_l[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    e_in[l] = hx_in[l] = hy_in[l] = x_in[l] = y_in[l] = hxp[l] = 0.f;
+    cc_l[l] = make_float2(0.f, 0.f);
+    cxe_l[l] = 1.f;
+    cxh_l[l] = cyh_l[l] = 0.f;
+    hay_l[l] = 1.f;
+    hby_l[l] = 0.f;
+  }
   {
     const unsigned ka = (unsigned)(j1 * LD + ic), kb = ka - (unsigned)LD;
     const bool ina = okrow && j1 >= 0 && j1 <= NY, inb = okrow && j1 - 1 >= 0 && j1 - 1 <= NY;
     e0_c = bload(rE0, ka, ina);
     const float eb = bload(rE0, kb, inb), hxb = bload(rHx0, kb, inb && j1 - 1 < NY);
     const float4 kbb = bload4(rU1, 4 * kb, inb);  // (cyh, AX, BX, AY) of column j1-1
-    hx1_prev = kbb.y * hxb + kbb.z * (e0_c - eb);
+    hxp[0] = kbb.y * hxb + kbb.z * (e0_c - eb);
   }
-  // step-2 carries (column j-1 of step 1): E, Hx, Hy, X, Y at n+1; Hx^{n+2}(j-2);
-  // and the column's coefficients
-  float e1_prev = 0.f, hx1_c2 = 0.f, hy1_c2 = 0.f, x1_c2 = 0.f, y1_c2 = 0.f, hx2_prev = 0.f;
-  float2 cc_c2 = make_float2(0.f, 0.f);
-  float cxe_c2 = 1.f, cxh_c2 = 0.f, cyh_c2 = 0.f, hay_c2 = 1.f, hby_c2 = 0.f;
+  // hand level l's outputs to level l+1 and shift the coefficient carries
+  auto shift = [&](const float* eo, const float* hxo, const float* hyo, const float* xo,
+                   const float* yo) {
+#pragma unroll
+    for (int l = NS - 1; l >= 1; --l) {
+      e_in[l] = eo[l - 1];
+      hx_in[l] = hxo[l - 1];
+      hy_in[l] = hyo[l - 1];
+      x_in[l] = xo[l - 1];
+      y_in[l] = yo[l - 1];
+    }
+#pragma unroll
+    for (int l = 0; l < NS; ++l) hxp[l] = hxo[l];
+  };
 
-  // both steps' columns of a chunk at jj are interior
-  auto chunkable = [&](int jj) { return jj - 1 >= P + 1 && jj + U - 1 <= NY - P - 1; };
+  // every level's columns of a chunk at jj are interior
+  auto chunkable = [&](int jj) { return jj - (NS - 1) >= P + 1 && jj + U - 1 <= NY - P - 1; };
 
   // ---- chunk of any columns (left/right strips, grid edges, strip ends):
   // every coefficient per cell from the table, Ez' = ce Ez + X' + Y'
@@ -519,78 +566,77 @@ __device__ __forceinline__ void fwd2_body(const DevGrid& g, const DevShots& s, i
       kb_[u] = bload4(rU1, 4 * kk, cin);
       kc_[u] = bload(rU2, kk, cin);
     }
-    float4 ka2, kb2;
-    float kc2;
-    {
-      const int c = jj - 1;
+    // table coefficients of the later levels' first columns
+    float4 ka_l[NS], kb_l[NS];
+    float kc_l[NS];
+#pragma unroll
+    for (int l = 1; l < NS; ++l) {
+      const int c = jj - l;
       const bool cin = okrow && c >= 0 && c <= NY;
       const unsigned kk = (unsigned)(c * LD + ic);
-      ka2 = bload4(rU0, 4 * kk, cin);
-      kb2 = bload4(rU1, 4 * kk, cin);
-      kc2 = bload(rU2, kk, cin);
+      ka_l[l] = bload4(rU0, 4 * kk, cin);
+      kb_l[l] = bload4(rU1, 4 * kk, cin);
+      kc_l[l] = bload(rU2, kk, cin);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int c1 = jj + u, c2 = c1 - 1;
-      // step 1 at c1: (ce, cxe, cxh, cye) = ka, (cyh, AX, BX, AY) = kb, BY = kc
-      const float e_dn = shfl_dn1(e0_c);
-      const float hy1 = kb_[u].w * hy_[u] + kc_[u] * (e_dn - e0_c);
-      const float hx1 = kb_[u].y * hx_[u] + kb_[u].z * (en_[u] - e0_c);
-      const float dhy1 = hy1 - shfl_up1(hy1), dhx1 = hx1 - hx1_prev;
-      const float x1 = ka_[u].y * x_[u] + ka_[u].z * dhy1;
-      const float y1 = ka_[u].w * y_[u] + kb_[u].x * dhx1;
-      float e1 = ka_[u].x * e0_c + x1 + y1;
-      e1 += (c1 == sj && i == si) ? ws1 : 0.f;
-      // step 2 at c2
-      const float e_dn2 = shfl_dn1(e1_prev);
-      const float hy2 = kb2.w * hy1_c2 + kc2 * (e_dn2 - e1_prev);
-      const float hx2 = kb2.y * hx1_c2 + kb2.z * (e1 - e1_prev);
-      const float dhy2 = hy2 - shfl_up1(hy2), dhx2 = hx2 - hx2_prev;
-      const float x2 = ka2.y * x1_c2 + ka2.z * dhy2;
-      const float y2 = ka2.w * y1_c2 + kb2.x * dhx2;
-      float e2 = ka2.x * e1_prev + x2 + y2;
-      e2 += (c2 == sj && i == si) ? ws2 : 0.f;
-      bstore(e1, rJZ, (unsigned)it, own && i == si && c1 == sj && c1 >= jb && c1 < je);
-      bstore(e2, rJZ, (unsigned)(it + 1), own && i == si && c2 == sj && c2 >= jb && c2 < je);
-      const bool o2 = own && c2 >= jb && c2 < je;
-      const bool pm2 = pml_cell(c2);
-      const unsigned k2 = (unsigned)(c2 * LD + ic);
-      bstore(e2, rE2, k2, o2);
-      bstore(hx2, rHx2, k2, o2 && c2 < NY);
-      bstore(hy2, rHy2, k2, o2 && h_any);
-      bstore(x2, rX2, k2, o2 && pm2);
-      bstore(y2, rY2, k2, o2 && pm2);
-      if (has_rec) {
-        if (own && c1 >= jb && c1 < je) record(c1, e1, it);
-        if (o2) record(c2, e2, it + 1);
-      }
-      if (FRAMES) {
-        bstore(e1, rsrc_of(fr1, FD), (unsigned)((c1 - g.dj0) * LD + ic),
-               own && dom_row && dom_col(c1) && c1 >= jb && c1 < je);
-        bstore(e2, rsrc_of(fr2, FD), (unsigned)((c2 - g.dj0) * LD + ic), o2 && dom_row && dom_col(c2));
+      const int c = jj + u;
+      float eo[NS], hxo[NS], hyo[NS], xo[NS], yo[NS];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const int cl = c - l;
+        const float ec = l == 0 ? e0_c : e_in[l], en = l == 0 ? en_[u] : eo[l - 1];
+        const float hx = l == 0 ? hx_[u] : hx_in[l], hy = l == 0 ? hy_[u] : hy_in[l];
+        const float x = l == 0 ? x_[u] : x_in[l], y = l == 0 ? y_[u] : y_in[l];
+        const float4 ka = l == 0 ? ka_[u] : ka_l[l], kb = l == 0 ? kb_[u] : kb_l[l];
+        const float kc = l == 0 ? kc_[u] : kc_l[l];
+        // (ce, cxe, cxh, cye) = ka, (cyh, AX, BX, AY) = kb, BY = kc
+        const float e_dn = shfl_dn1(ec);
+        const float hy1 = kb.w * hy + kc * (e_dn - ec);
+        const float hx1 = kb.y * hx + kb.z * (en - ec);
+        const float dhy = hy1 - shfl_up1(hy1), dhx = hx1 - hxp[l];
+        const float x1 = ka.y * x + ka.z * dhy;
+        const float y1 = ka.w * y + kb.x * dhx;
+        float e1 = ka.x * ec + x1 + y1;
+        e1 += (cl == sj && i == si) ? ws[l] : 0.f;
+        eo[l] = e1;
+        hxo[l] = hx1;
+        hyo[l] = hy1;
+        xo[l] = x1;
+        yo[l] = y1;
+        level_out(l, cl, e1);
+        if (l == NS - 1) {
+          const bool ol = own && cl >= jb && cl < je;
+          const bool pm = pml_cell(cl);
+          const unsigned k2 = (unsigned)(cl * LD + ic);
+          bstore(e1, rE2, k2, ol);
+          bstore(hx1, rHx2, k2, ol && cl < NY);
+          bstore(hy1, rHy2, k2, ol && h_any);
+          bstore(x1, rX2, k2, ol && pm);
+          bstore(y1, rY2, k2, ol && pm);
+        }
       }
       e0_c = en_[u];
-      hx1_prev = hx1;
-      e1_prev = e1;
-      hx1_c2 = hx1;
-      hy1_c2 = hy1;
-      x1_c2 = x1;
-      y1_c2 = y1;
-      hx2_prev = hx2;
-      ka2 = ka_[u];
-      kb2 = kb_[u];
-      kc2 = kc_[u];
+      shift(eo, hxo, hyo, xo, yo);
+#pragma unroll
+      for (int l = NS - 1; l >= 1; --l) {
+        ka_l[l] = l == 1 ? ka_[u] : ka_l[l - 1];
+        kb_l[l] = l == 1 ? kb_[u] : kb_l[l - 1];
+        kc_l[l] = l == 1 ? kc_[u] : kc_l[l - 1];
+      }
     }
-    // coefficients of the next step-2 column, in the interior chunks' form
-    cc_c2 = make_float2(ka2.x, ka2.z);
-    cxe_c2 = ka2.y;
-    cxh_c2 = ka2.z;
-    cyh_c2 = kb2.x;
-    hay_c2 = kb2.w;
-    hby_c2 = kc2;
+    // coefficients of the later levels' next columns, in the interior chunks' form
+#pragma unroll
+    for (int l = 1; l < NS; ++l) {
+      cc_l[l] = make_float2(ka_l[l].x, ka_l[l].z);
+      cxe_l[l] = ka_l[l].y;
+      cxh_l[l] = ka_l[l].z;
+      cyh_l[l] = kb_l[l].x;
+      hay_l[l] = kb_l[l].w;
+      hby_l[l] = kc_l[l];
+    }
   };
 
-  // ---- interior chunk, all rows inner: scalar H coefficients, (Ceze, Cezhy) loads
   // Loads of one interior chunk (all rows inner).  The interior run is
   // software-pipelined: the next chunk's loads are issued before the current
   // chunk is computed, so a wave always has a chunk of requests in flight.
@@ -615,42 +661,40 @@ __device__ __forceinline__ void fwd2_body(const DevGrid& g, const DevShots& s, i
   auto inner_compute = [&](int jj, const InnerBuf& b) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int c1 = jj + u, c2 = c1 - 1;
-      // step 1 at c1
-      const float e_dn = shfl_dn1(e0_c);
-      const float hy1 = b.hy[u] + by0 * (e_dn - e0_c);
-      const float hx1 = b.hx[u] + bx0 * (b.en[u] - e0_c);
-      const float hy1u = shfl_up1(hy1);
-      float e1 = b.c[u].x * e0_c + b.c[u].y * ((hy1 - hy1u) - (hx1 - hx1_prev));
-      e1 += (c1 == sj && i == si) ? ws1 : 0.f;
-      // step 2 at c2
-      const float e_dn2 = shfl_dn1(e1_prev);
-      const float hy2 = hy1_c2 + by0 * (e_dn2 - e1_prev);
-      const float hx2 = hx1_c2 + bx0 * (e1 - e1_prev);
-      const float hy2u = shfl_up1(hy2);
-      float e2 = cc_c2.x * e1_prev + cc_c2.y * ((hy2 - hy2u) - (hx2 - hx2_prev));
-      e2 += (c2 == sj && i == si) ? ws2 : 0.f;
-      bstore(e1, rJZ, (unsigned)it, own && i == si && c1 == sj && c1 >= jb && c1 < je);
-      bstore(e2, rJZ, (unsigned)(it + 1), own && i == si && c2 == sj && c2 >= jb && c2 < je);
-      const bool o2 = own && c2 >= jb && c2 < je;
-      const unsigned k2 = (unsigned)(c2 * LD + i);
-      bstore(e2, rE2, k2, o2);
-      bstore(hx2, rHx2, k2, o2);
-      bstore(hy2, rHy2, k2, o2);
-      if (FRAMES) {
-        bstore(e1, rsrc_of(fr1, FD), (unsigned)((c1 - g.dj0) * LD + i),
-               own && dom_row && dom_col(c1) && c1 >= jb && c1 < je);
-        bstore(e2, rsrc_of(fr2, FD), (unsigned)((c2 - g.dj0) * LD + i), o2 && dom_row && dom_col(c2));
+      const int c = jj + u;
+      float eo[NS], hxo[NS], hyo[NS], zo[NS];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const int cl = c - l;
+        const float ec = l == 0 ? e0_c : e_in[l], en = l == 0 ? b.en[u] : eo[l - 1];
+        const float hx = l == 0 ? b.hx[u] : hx_in[l], hy = l == 0 ? b.hy[u] : hy_in[l];
+        const float2 cf = l == 0 ? b.c[u] : cc_l[l];
+        const float e_dn = shfl_dn1(ec);
+        const float hy1 = hy + by0 * (e_dn - ec);
+        const float hx1 = hx + bx0 * (en - ec);
+        const float hyu = shfl_up1(hy1);
+        float e1 = cf.x * ec + cf.y * ((hy1 - hyu) - (hx1 - hxp[l]));
+        e1 += (cl == sj && i == si) ? ws[l] : 0.f;
+        eo[l] = e1;
+        hxo[l] = hx1;
+        hyo[l] = hy1;
+        zo[l] = 0.f;  // no split-field cells in these rows
+        const bool ol = own && cl >= jb && cl < je;
+        bstore(e1, rJZ, (unsigned)(it + l), ol && i == si && cl == sj);
+        if (FRAMES)
+          bstore(e1, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + i), ol && dom_row && dom_col(cl));
+        if (l == NS - 1) {
+          const unsigned k2 = (unsigned)(cl * LD + i);
+          bstore(e1, rE2, k2, ol);
+          bstore(hx1, rHx2, k2, ol);
+          bstore(hy1, rHy2, k2, ol);
+        }
       }
       e0_c = b.en[u];
-      hx1_prev = hx1;
-      e1_prev = e1;
-      hx1_c2 = hx1;
-      hy1_c2 = hy1;
-      hx2_prev = hx2;
-      cc_c2 = b.c[u];
+      shift(eo, hxo, hyo, zo, zo);
+#pragma unroll
+      for (int l = NS - 1; l >= 1; --l) cc_l[l] = l == 1 ? b.c[u] : cc_l[l - 1];
     }
-    x1_c2 = y1_c2 = 0.f;  // no split-field cells in these rows
   };
 
   // ---- interior chunk, rows of every class (top / bottom PML rows present)
@@ -663,7 +707,7 @@ __device__ __forceinline__ void fwd2_body(const DevGrid& g, const DevShots& s, i
       en_[u] = bload(rE0, kk + (unsigned)LD, okrow);
       hx_[u] = bload(rHx0, kk, okrow);
       hy_[u] = bload(rHy0, kk, h_any);
-      c_[u] = make_float2(bload(rCC, 2 * kk, e_inner), bload(rCC, 2 * kk + 1, e_inner));
+      c_[u] = bload2(rCC, 2 * kk, e_inner);
       x_[u] = bload(rX0, kk, e_pml);
       y_[u] = bload(rY0, kk, e_pml);
       cxe_[u] = e_xp ? bload(rCXE, kk, e_xp) : cxe_r;
@@ -674,70 +718,67 @@ __device__ __forceinline__ void fwd2_body(const DevGrid& g, const DevShots& s, i
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int c1 = jj + u, c2 = c1 - 1;
-      // step 1 at c1
-      const float e_dn = shfl_dn1(e0_c);
-      const float hy1 = h_any ? hay_[u] * hy_[u] + hby_[u] * (e_dn - e0_c) : 0.f;
-      const float hx1 = okrow ? hx_[u] + bx0 * (en_[u] - e0_c) : 0.f;
-      const float hy1u = shfl_up1(hy1);
-      const float dhy1 = hy1 - hy1u, dhx1 = hx1 - hx1_prev;
-      const float x1 = cxe_[u] * x_[u] + cxh_[u] * dhy1;
-      const float y1 = y_[u] + cyh_[u] * dhx1;
-      // ring rows (0, NX) are never updated: they keep their value
-      float e1 = e_inner ? c_[u].x * e0_c + c_[u].y * (dhy1 - dhx1) : (e_pml ? x1 + y1 : e0_c);
-      e1 += (c1 == sj && i == si) ? ws1 : 0.f;
-      // step 2 at c2
-      const float e_dn2 = shfl_dn1(e1_prev);
-      const float hy2 = h_any ? hay_c2 * hy1_c2 + hby_c2 * (e_dn2 - e1_prev) : 0.f;
-      const float hx2 = okrow ? hx1_c2 + bx0 * (e1 - e1_prev) : 0.f;
-      const float hy2u = shfl_up1(hy2);
-      const float dhy2 = hy2 - hy2u, dhx2 = hx2 - hx2_prev;
-      const float x2 = cxe_c2 * x1_c2 + cxh_c2 * dhy2;
-      const float y2 = y1_c2 + cyh_c2 * dhx2;
-      float e2 = e_inner ? cc_c2.x * e1_prev + cc_c2.y * (dhy2 - dhx2) : (e_pml ? x2 + y2 : e1_prev);
-      e2 += (c2 == sj && i == si) ? ws2 : 0.f;
-      bstore(e1, rJZ, (unsigned)it, own && i == si && c1 == sj && c1 >= jb && c1 < je);
-      bstore(e2, rJZ, (unsigned)(it + 1), own && i == si && c2 == sj && c2 >= jb && c2 < je);
-      const bool o2 = own && c2 >= jb && c2 < je;
-      const unsigned k2 = (unsigned)(c2 * LD + ic);
-      bstore(e2, rE2, k2, o2);
-      bstore(hx2, rHx2, k2, o2);
-      bstore(hy2, rHy2, k2, o2 && h_any);
-      bstore(x2, rX2, k2, o2 && e_pml);
-      bstore(y2, rY2, k2, o2 && e_pml);
-      if (has_rec) {
-        if (own && c1 >= jb && c1 < je) record(c1, e1, it);
-        if (o2) record(c2, e2, it + 1);
-      }
-      if (FRAMES) {
-        bstore(e1, rsrc_of(fr1, FD), (unsigned)((c1 - g.dj0) * LD + ic),
-               own && dom_row && dom_col(c1) && c1 >= jb && c1 < je);
-        bstore(e2, rsrc_of(fr2, FD), (unsigned)((c2 - g.dj0) * LD + ic), o2 && dom_row && dom_col(c2));
+      const int c = jj + u;
+      float eo[NS], hxo[NS], hyo[NS], xo[NS], yo[NS];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const int cl = c - l;
+        const float ec = l == 0 ? e0_c : e_in[l], en = l == 0 ? en_[u] : eo[l - 1];
+        const float hx = l == 0 ? hx_[u] : hx_in[l], hy = l == 0 ? hy_[u] : hy_in[l];
+        const float x = l == 0 ? x_[u] : x_in[l], y = l == 0 ? y_[u] : y_in[l];
+        const float2 cf = l == 0 ? c_[u] : cc_l[l];
+        const float cxe = l == 0 ? cxe_[u] : cxe_l[l], cxh = l == 0 ? cxh_[u] : cxh_l[l];
+        const float cyh = l == 0 ? cyh_[u] : cyh_l[l];
+        const float hay = l == 0 ? hay_[u] : hay_l[l], hby = l == 0 ? hby_[u] : hby_l[l];
+        const float e_dn = shfl_dn1(ec);
+        const float hy1 = h_any ? hay * hy + hby * (e_dn - ec) : 0.f;
+        const float hx1 = okrow ? hx + bx0 * (en - ec) : 0.f;
+        const float hyu = shfl_up1(hy1);
+        const float dhy = hy1 - hyu, dhx = hx1 - hxp[l];
+        const float x1 = cxe * x + cxh * dhy;
+        const float y1 = y + cyh * dhx;
+        // ring rows (0, NX) are never updated: they keep their value
+        float e1 = e_inner ? cf.x * ec + cf.y * (dhy - dhx) : (e_pml ? x1 + y1 : ec);
+        e1 += (cl == sj && i == si) ? ws[l] : 0.f;
+        eo[l] = e1;
+        hxo[l] = hx1;
+        hyo[l] = hy1;
+        xo[l] = x1;
+        yo[l] = y1;
+        level_out(l, cl, e1);
+        if (l == NS - 1) {
+          const bool ol = own && cl >= jb && cl < je;
+          const unsigned k2 = (unsigned)(cl * LD + ic);
+          bstore(e1, rE2, k2, ol);
+          bstore(hx1, rHx2, k2, ol);
+          bstore(hy1, rHy2, k2, ol && h_any);
+          bstore(x1, rX2, k2, ol && e_pml);
+          bstore(y1, rY2, k2, ol && e_pml);
+        }
       }
       e0_c = en_[u];
-      hx1_prev = hx1;
-      e1_prev = e1;
-      hx1_c2 = hx1;
-      hy1_c2 = hy1;
-      x1_c2 = x1;
-      y1_c2 = y1;
-      hx2_prev = hx2;
-      cc_c2 = c_[u];
-      cxe_c2 = cxe_[u];
-      cxh_c2 = cxh_[u];
-      cyh_c2 = cyh_[u];
-      hay_c2 = hay_[u];
-      hby_c2 = hby_[u];
+      shift(eo, hxo, hyo, xo, yo);
+#pragma unroll
+      for (int l = NS - 1; l >= 1; --l) {
+        cc_l[l] = l == 1 ? c_[u] : cc_l[l - 1];
+        cxe_l[l] = l == 1 ? cxe_[u] : cxe_l[l - 1];
+        cxh_l[l] = l == 1 ? cxh_[u] : cxh_l[l - 1];
+        cyh_l[l] = l == 1 ? cyh_[u] : cyh_l[l - 1];
+        hay_l[l] = l == 1 ? hay_[u] : hay_l[l - 1];
+        hby_l[l] = l == 1 ? hby_[u] : hby_l[l - 1];
+      }
     }
   };
 
-  // Step 1 at columns j .. j+U-1, step 2 one column behind.  Chunk starts
-  // j1 + kU split into a strip/edge prefix, an interior run and a suffix.
+  // Level 0 at columns j .. j+U-1 per chunk, from jb-NS+1 to je+NS-2 (the
+  // last level's last column is je-1).  Chunk starts split into a
+  // strip/edge prefix, an interior run and a suffix.
   const bool use_inner = inner_rows && !has_rec;
+  const int jlast = je + NS - 2;
   int j = j1;
-  for (; j <= je && !chunkable(j); j += U) univ_chunk(j);
+  for (; j <= jlast && !chunkable(j); j += U) univ_chunk(j);
   int jz = j;  // end of the interior run
-  while (jz <= je && chunkable(jz)) jz += U;
+  while (jz <= jlast && chunkable(jz)) jz += U;
   if (use_inner) {
     InnerBuf A, B;
     inner_load(j, A, j < jz);
@@ -753,7 +794,7 @@ __device__ __forceinline__ void fwd2_body(const DevGrid& g, const DevShots& s, i
   } else {
     for (; j < jz; j += U) class_chunk(j);
   }
-  for (; j <= je; j += U) univ_chunk(j);
+  for (; j <= jlast; j += U) univ_chunk(j);
 }
 
 template <bool FRAMES>
@@ -763,39 +804,52 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
             const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ E2,
             float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
             float* __restrict__ Y2, int it, float* __restrict__ tr, float* __restrict__ frame1,
-            float* __restrict__ frame2, long long fstride, float* __restrict__ trash) {
-  fwd2_body<FRAMES>(g, s, L, tile_of_block(g.NX, g.NY, L, ROWS2), E0, Hx0, Hy0, X0, Y0, E2, Hx2, Hy2, X2, Y2, it, tr, frame1, frame2, fstride, trash);
+            long long fstride) {
+  fwdN_body<2, FRAMES>(g, s, L, tile_of_block(g.NX, g.NY, L, 60), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
+                       Hy2, X2, Y2, it, tr, frame1, fstride);
 }
 
 template <bool FRAMES>
 __global__ void __launch_bounds__(256)
-k_fwd_step2_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
+k_fwd_step4(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
+            const float* __restrict__ Hx0, const float* __restrict__ Hy0,
+            const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ E2,
+            float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
+            float* __restrict__ Y2, int it, float* __restrict__ tr, float* __restrict__ frame1,
+            long long fstride) {
+  fwdN_body<4, FRAMES>(g, s, L, tile_of_block(g.NX, g.NY, L, 56), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
+                       Hy2, X2, Y2, it, tr, frame1, fstride);
+}
+
+template <int NS, bool FRAMES>
+__global__ void __launch_bounds__(256)
+k_fwd_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
   int k;
   TileId tid;
-  if (!multi_tile(a, sv, ROWS2, k, tid)) return;
+  if (!multi_tile(a, sv, 64 - 2 * NS, k, tid)) return;
   const DevSurvey& S = sv[a.sid[k]];
   const int c = a.cur[k], x = a.xc[k];
   float* f = S.fld;
   const long long z = S.fsz;
   float* fr1 = FRAMES ? S.frames + (long long)a.fslot[k] * S.FD : nullptr;
-  float* fr2 = FRAMES ? fr1 + S.FD : nullptr;
-  fwd2_body<FRAMES>(S.g, S.s, a.L[k], tid, f + 3 * c * z, f + (3 * c + 1) * z,
-                    f + (3 * c + 2) * z, f + (6 + 2 * x) * z, f + (7 + 2 * x) * z,
-                    f + 3 * (c ^ 1) * z, f + (3 * (c ^ 1) + 1) * z, f + (3 * (c ^ 1) + 2) * z,
-                    f + (6 + 2 * (x ^ 1)) * z, f + (7 + 2 * (x ^ 1)) * z, a.it[k],
-                    a.rec ? S.tr : nullptr, fr1, fr2, S.fstride, nullptr);
+  fwdN_body<NS, FRAMES>(S.g, S.s, a.L[k], tid, f + 3 * c * z, f + (3 * c + 1) * z,
+                        f + (3 * c + 2) * z, f + (6 + 2 * x) * z, f + (7 + 2 * x) * z,
+                        f + 3 * (c ^ 1) * z, f + (3 * (c ^ 1) + 1) * z, f + (3 * (c ^ 1) + 2) * z,
+                        f + (6 + 2 * (x ^ 1)) * z, f + (7 + 2 * (x ^ 1)) * z, a.it[k],
+                        a.rec ? S.tr : nullptr, fr1, S.fstride);
 }
 
-template __global__ void k_fwd_step2<false>(DevGrid, DevShots, int, const float*, const float*,
-                                            const float*, const float*, const float*, float*,
-                                            float*, float*, float*, float*, int, float*, float*,
-                                            float*, long long, float*);
-template __global__ void k_fwd_step2<true>(DevGrid, DevShots, int, const float*, const float*,
-                                           const float*, const float*, const float*, float*,
-                                           float*, float*, float*, float*, int, float*, float*,
-                                           float*, long long, float*);
-template __global__ void k_fwd_step2_multi<false>(const DevSurvey*, MultiArgs);
-template __global__ void k_fwd_step2_multi<true>(const DevSurvey*, MultiArgs);
+#define GJW_FWDN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \
+    const float*, const float*, float*, float*, float*, float*, float*, int, float*, float*, long long
+template __global__ void k_fwd_step2<false>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step2<true>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step4<false>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step4<true>(GJW_FWDN_ARGS);
+#undef GJW_FWDN_ARGS
+template __global__ void k_fwd_stepN_multi<2, false>(const DevSurvey*, MultiArgs);
+template __global__ void k_fwd_stepN_multi<2, true>(const DevSurvey*, MultiArgs);
+template __global__ void k_fwd_stepN_multi<4, false>(const DevSurvey*, MultiArgs);
+template __global__ void k_fwd_stepN_multi<4, true>(const DevSurvey*, MultiArgs);
 
 // ==========================================================================
 // adjoint: one step of the transpose of k_fwd_step

@@ -252,10 +252,11 @@ def test_autograd_function(oracle_mod, gpu_ctx):
 
 
 @pytest.mark.parametrize("ck", [7, 10])
-def test_two_step_kernel_matches_one_step(gpu_ctx, ck):
-    """The temporally blocked forward (two steps per launch, overlapped row
-    tiles) equals the one-step kernel: traces and gradient, odd nt, odd and
-    even checkpoint intervals, several tile widths."""
+@pytest.mark.parametrize("fuse", [2, 4])
+def test_two_step_kernel_matches_one_step(gpu_ctx, ck, fuse):
+    """The temporally blocked forward (two or four steps per launch,
+    overlapped row tiles) equals the one-step kernel: traces and gradient, odd
+    nt, odd and even checkpoint intervals, several tile widths."""
     from gerjoii_amd import _native as N
     case = configs.small(m=130, n=150, P=12, nt=241, ns=2)
     rng = np.random.default_rng(4)
@@ -266,7 +267,7 @@ def test_two_step_kernel_matches_one_step(gpu_ctx, ck):
     g1 = ref.gradient(adj)
     for L in (5, 16, 64):
         sv = make_survey(case, gpu_ctx, ck=ck, L=L)
-        sv.set_option(N.GJW_OPT_FUSE, 2)
+        sv.set_option(N.GJW_OPT_FUSE, fuse)
         d2 = sv.forward(keep=True)
         g2 = sv.gradient(adj)
         print("L", L, rel_l2(d2, d1), rel_l2(g2[0], g1[0]), rel_l2(g2[1], g1[1]))
@@ -296,7 +297,7 @@ def test_borehole_forward_and_gradient(oracle_mod, gpu_ctx):
     adj = rng.standard_normal(ref.shape).astype(np.float32)
     rp = np.concatenate([[0], np.cumsum([len(r) for r in case.receivers])])
     fe, fs, _ = oracle_grad(oracle_mod, case, case.eps0, case.sig0, adj, rp)
-    for fuse in (1, 2):
+    for fuse in (1, 2, 4):
         for L in (8, 32):
             sv = make_survey(case, gpu_ctx, ck=13, L=L)
             sv.set_option(N.GJW_OPT_FUSE, fuse)
@@ -345,7 +346,7 @@ def test_source_recorder(oracle_mod, gpu_ctx):
         ep, sp = oracle_mod.exp2pml(g, case.eps0, case.sig0)
         ref = np.stack([oracle_mod.forward(g, ep, sp, case.src[q], case.wavelets[q],
                                            case.receivers[q])[1] for q in range(len(case.src))])
-        for plain, fuse in ((False, 2), (False, 1), (True, 1)):
+        for plain, fuse in ((False, 4), (False, 2), (False, 1), (True, 1)):
             sv = make_survey(case, gpu_ctx, plain=plain)
             sv.set_option(N.GJW_OPT_FUSE, fuse)
             d, jz = sv.forward(jz=True)
