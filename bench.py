@@ -100,7 +100,7 @@ def main():
         sv.fwi_gradient(d_obs, g_eps, g_sig)
         log(f"warmup {w + 1}/{args.warmup} done")
     sv.set_option(N.GJW_OPT_KTIMING, 16)
-    for kk in range(5):
+    for kk in range(6):
         sv.kernel_stats(kk, reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -116,6 +116,7 @@ def main():
     fwd2_ms, fwd2_n, fwd2_l = sv.kernel_stats(2)
     adj2_ms, adj2_n, adj2_l = sv.kernel_stats(3)
     fwd4_ms, fwd4_n, fwd4_l = sv.kernel_stats(4)
+    adj4_ms, adj4_n, adj4_l = sv.kernel_stats(5)
     ms_step = dt_s * 1e3 / args.steps
     shot_grads = ns * args.steps / dt_s
     cell_updates = 3.0 * cells * g.nt * ns * args.steps  # forward, recompute, adjoint
@@ -128,9 +129,9 @@ def main():
 
     # per-kernel roofline: algorithmic bytes per launch / live average duration
     bytes_launch = BYTES_PER_CELL_UPDATE * cells * ns
-    # launches per gradient: the forward sweeps run 4 (k_fwd_step4) or 2
-    # (k_fwd_step2) steps per launch as the autotuner chose, the adjoint 2
-    # (k_adj_step2); the other kernels only take segment tails
+    # launches per gradient: every sweep runs 4 (k_*_step4) or 2 (k_*_step2)
+    # steps per launch as the autotuner chose; the other kernels only take
+    # segment tails
     kern = {
         "k_fwd_step2": {"avg_ms": fwd2_ms, "samples": fwd2_n, "steps_per_launch": 2,
                         "launches_per_step": fwd2_l / args.steps},
@@ -142,6 +143,8 @@ def main():
                         "launches_per_step": adj2_l / args.steps},
         "k_fwd_step4": {"avg_ms": fwd4_ms, "samples": fwd4_n, "steps_per_launch": 4,
                         "launches_per_step": fwd4_l / args.steps},
+        "k_adj_step4": {"avg_ms": adj4_ms, "samples": adj4_n, "steps_per_launch": 4,
+                        "launches_per_step": adj4_l / args.steps},
     }
     for k, v in kern.items():
         spl = v["steps_per_launch"]

@@ -33,10 +33,12 @@ __global__ void k_adj_step(DevGrid, DevShots, int, const float*, const float*, c
                            const float*, const float*, float*, float*, float*, float*, float*,
                            int, const float*, const float*, const float*, long long, float*,
                            long long, float*);
-__global__ void k_adj_step2(DevGrid, DevShots, int, const float*, const float*, const float*,
-                            const float*, const float*, float*, float*, float*, float*, float*,
-                            int, const float*, const float*, const float*, const float*,
-                            long long, float*, long long, float*);
+#define GJW_ADJN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \
+    const float*, const float*, float*, float*, float*, float*, float*, int, const float*, \
+    const float*, long long, float*, long long, float*
+__global__ void k_adj_step2(GJW_ADJN_ARGS);
+__global__ void k_adj_step4(GJW_ADJN_ARGS);
+#undef GJW_ADJN_ARGS
 #define GJW_FWDN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \
     const float*, const float*, float*, float*, float*, float*, float*, int, float*, float*, long long
 template <bool FRAMES>
@@ -56,7 +58,10 @@ extern template __global__ void k_fwd_stepN_multi<2, true>(const DevSurvey*, Mul
 extern template __global__ void k_fwd_stepN_multi<4, false>(const DevSurvey*, MultiArgs);
 extern template __global__ void k_fwd_stepN_multi<4, true>(const DevSurvey*, MultiArgs);
 __global__ void k_adj_step_multi(const DevSurvey*, MultiArgs);
-__global__ void k_adj_step2_multi(const DevSurvey*, MultiArgs);
+template <int NS>
+__global__ void k_adj_stepN_multi(const DevSurvey*, MultiArgs);
+extern template __global__ void k_adj_stepN_multi<2>(const DevSurvey*, MultiArgs);
+extern template __global__ void k_adj_stepN_multi<4>(const DevSurvey*, MultiArgs);
 __global__ void k_fill(float*, long long, float);
 __global__ void k_fill_d(double*, long long, double);
 __global__ void k_inject_last(DevGrid, int, int, int, const int*, const int*, const int*,
@@ -215,10 +220,10 @@ struct gjw_survey {
   GraphCache g_all, g_fwd, g_adj;  // whole gradient (L2); forward / adjoint sweeps
   std::vector<hipEvent_t> kev_pool;
   std::vector<std::pair<int, int>> kev_used;  // (kernel id, index of start event)
-  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2, k_fwd_step4
-  double kt_ms[5] = {0, 0, 0, 0, 0};
-  long long kt_n[5] = {0, 0, 0, 0, 0};
-  long long kl_n[5] = {0, 0, 0, 0, 0};  // all launches while timing is on
+  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2, k_fwd_step4, k_adj_step4
+  double kt_ms[6] = {0, 0, 0, 0, 0, 0};
+  long long kt_n[6] = {0, 0, 0, 0, 0, 0};
+  long long kl_n[6] = {0, 0, 0, 0, 0, 0};  // all launches while timing is on
 };
 
 static inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
@@ -808,9 +813,6 @@ static int fwd_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int n
 static inline dim3 stepN_grid(const gjw_survey* s, int NS, int L, int nshots) {
   return dim3((unsigned)(cdiv(s->NX + 1, 64 - 2 * NS) * cdiv(cdiv(s->NY + 1, L), 4) * nshots));
 }
-static inline dim3 step2_grid(const gjw_survey* s, int L, int nshots) {
-  return stepN_grid(s, 2, L, nshots);
-}
 
 // NS = 2 or 4 fused forward steps; fr1 (optional) receives the NS domain
 // frames Ez(S_{it+1}) .. Ez(S_{it+NS}) at fr1 + l * LD * n
@@ -877,17 +879,24 @@ static int adj_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int n
   return kev_end(s, slot);
 }
 
-// transposes forward steps it and it-1 (it >= 1) in one launch
-static int adj2_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, int L,
-                       AdjBufs& b, int cur, int it, const float* adj, const float* Fa,
-                       const float* Fb, const float* Fc, long long fstride, float* acc,
-                       long long astride, float* gw) {
+// transposes forward steps it, .., it-NS+1 (NS = 2 or 4, it-NS+1 >= 0) in
+// one launch; Fa is the frame Ez(S_{it+1}), the earlier ones precede it
+static int adjN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& sh, int nsh,
+                       int L, AdjBufs& b, int cur, int it, const float* adj, const float* Fa,
+                       long long fstride, float* acc, long long astride, float* gw) {
   int slot;
-  TRY(kev_begin(s, 3, it, &slot));
-  hipLaunchKernelGGL(k_adj_step2, step2_grid(s, L, nsh), dim3(64, 4), 0, s->ctx->stream, g, sh,
-                     L, b.M[cur], b.Nx[cur], b.Ny[cur], b.X[cur], b.Y[cur], b.M[cur ^ 1],
-                     b.Nx[cur ^ 1], b.Ny[cur ^ 1], b.X[cur ^ 1], b.Y[cur ^ 1], it, adj, Fa, Fb,
-                     Fc, fstride, acc, astride, gw);
+  TRY(kev_begin(s, NS == 4 ? 5 : 3, it, &slot));
+  const dim3 grid = stepN_grid(s, NS, L, nsh), block(64, 4);
+#define GJW_ADJN_CALL(K)                                                                   \
+  hipLaunchKernelGGL(K, grid, block, 0, s->ctx->stream, g, sh, L, b.M[cur], b.Nx[cur],    \
+                     b.Ny[cur], b.X[cur], b.Y[cur], b.M[cur ^ 1], b.Nx[cur ^ 1],          \
+                     b.Ny[cur ^ 1], b.X[cur ^ 1], b.Y[cur ^ 1], it, adj, Fa, fstride, acc, \
+                     astride, gw)
+  if (NS == 4)
+    GJW_ADJN_CALL(k_adj_step4);
+  else
+    GJW_ADJN_CALL(k_adj_step2);
+#undef GJW_ADJN_CALL
   HIPCHK(hipGetLastError());
   return kev_end(s, slot);
 }
@@ -1338,10 +1347,11 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
     }
     for (int it = k1 - 1; it >= k0;) {
       const float* F = s->frames.p + (long long)(it - k0) * FD;  // Ez(S_it)
-      if (s->fuse_eff >= 2 && it - 1 >= k0) {
-        TRY(adj2_launch(s, g, sh, ns, s->tuned_L, ab, acur, it, adj, F + FD, F, F - FD, fstride,
-                        s->acc.p, FD, gw));
-        it -= 2;
+      const int NS = (s->fuse_eff >= 4 && it - 3 >= k0) ? 4 : (s->fuse_eff >= 2 && it - 1 >= k0) ? 2 : 1;
+      if (NS > 1) {
+        TRY(adjN_launch(s, NS, g, sh, ns, NS == 4 ? s->tuned_L4 : s->tuned_L, ab, acur, it, adj,
+                        F + FD, fstride, s->acc.p, FD, gw));
+        it -= NS;
       } else {
         TRY(adj_launch(s, g, sh, ns, s->tuned_L1, ab, acur, it, adj, F + FD, F, fstride,
                        s->acc.p, FD, gw));
@@ -1840,8 +1850,10 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
     hipLaunchKernelGGL((k_fwd_stepN_multi<2, false>), grid, block, 0, st, b->dsv.p, a);
   else if (steps == 1)
     hipLaunchKernelGGL(k_adj_step_multi, grid, block, 0, st, b->dsv.p, a);
+  else if (steps == 4)
+    hipLaunchKernelGGL((k_adj_stepN_multi<4>), grid, block, 0, st, b->dsv.p, a);
   else
-    hipLaunchKernelGGL(k_adj_step2_multi, grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_adj_stepN_multi<2>), grid, block, 0, st, b->dsv.p, a);
   HIPCHK(hipGetLastError());
   return GJW_OK;
 }
@@ -1944,18 +1956,21 @@ static int batch_adj_enqueue(gjw_batch* b) {
     }
     for (int tau = kmax - 1; tau >= k0;) {
       std::vector<int> act;
-      bool two = b->fuse >= 2 && tau - 1 >= k0;
+      // steps this launch may take: none below k0, and no survey may start
+      // (take its first adjoint step) inside the launch
+      int room = std::min(b->fuse, tau - k0 + 1);
       for (int k : seg)
         if (tau <= k1[k] - 1) {
           act.push_back(k);
           itv[k] = tau;
           fsl[k] = tau - k0;
-        } else if (tau - 1 == k1[k] - 1) {
-          two = false;  // survey k starts at tau-1
+        } else {
+          room = std::min(room, tau - (k1[k] - 1));
         }
-      TRY(batch_launch(b, act, two ? 2 : 1, true, false, itv, acur, nullptr, fsl, 0));
+      const int steps = room >= 4 ? 4 : room >= 2 ? 2 : 1;
+      TRY(batch_launch(b, act, steps, true, false, itv, acur, nullptr, fsl, 0));
       for (int k : act) acur[k] ^= 1;
-      tau -= two ? 2 : 1;
+      tau -= steps;
     }
     for (int k : seg) {
       gjw_survey* s = b->sv[k];
@@ -2070,7 +2085,7 @@ extern "C" int gjw_batch_last_timing(gjw_batch* b, double* ms, double* cells) {
 
 extern "C" int gjw_kernel_stats(gjw_survey* s, int32_t which, double* avg_ms, int64_t* count,
                                 int64_t* launches, int32_t reset) {
-  if (!s || which < 0 || which > 4) return fail(GJW_ERR_ARG, "gjw_kernel_stats: bad argument");
+  if (!s || which < 0 || which > 5) return fail(GJW_ERR_ARG, "gjw_kernel_stats: bad argument");
   TRY(kev_resolve(s));
   if (avg_ms) *avg_ms = s->kt_n[which] ? s->kt_ms[which] / s->kt_n[which] : 0.0;
   if (count) *count = s->kt_n[which];

@@ -343,7 +343,6 @@ k_fwd_step_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
 // the second step needs in that direction.  Each field value is read and
 // written once per two steps.  Split-field aux X/Y ping-pong (X0 -> X2) so
 // that halo lanes can read the start-of-pass values of cells other waves own.
-constexpr int ROWS2 = 60;
 
 // Masked raw buffer load: a lane with on == false reads past the end of the
 // buffer and gets 0 without touching memory (no branch, see bstore).
@@ -1124,29 +1123,43 @@ k_adj_step_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
 // residual injection).  Requires it >= 1.
 
 
-// it: the later of the two forward steps transposed (it >= 1).  M0 holds
-// dJ/dEz(S_{it+1}) with the residual of step it already added; M2 receives
-// dJ/dEz(S_{it-1}) with the residual of step it-2 added (when it >= 2).
-// Fa, Fb, Fc: forward frames Ez(S_{it+1}), Ez(S_it), Ez(S_{it-1}) on the domain.
-__device__ __forceinline__ void adj2_body(const DevGrid& g, const DevShots& s, int L, TileId tid, const float* __restrict__ M0,
-            const float* __restrict__ Nx0, const float* __restrict__ Ny0,
-            const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ M2,
-            float* __restrict__ Nx2, float* __restrict__ Ny2, float* __restrict__ X2,
-            float* __restrict__ Y2, int it, const float* __restrict__ adj,
-            const float* __restrict__ Fa, const float* __restrict__ Fb,
-            const float* __restrict__ Fc, long long fstride, float* __restrict__ acc,
-            long long astride, float* __restrict__ gw) {
+// NS fused adjoint steps per pass (NS = 2 or 4), the transpose of NS
+// forward steps it, it-1, .., it-NS+1 (it >= NS-1), with the tiling of the
+// forward: rows r0 = (64 - 2 NS) t - NS .. r0+63, lanes NS .. 63-NS own their
+// rows, level l (forward step it-l) runs l columns behind level 0.  M0 holds
+// dJ/dEz(S_{it+1}) with the residual of step it added; M2 receives
+// dJ/dEz(S_{it-NS+1}) with the residual of step it-NS added (when >= 0).
+// Fa: the frame Ez(S_{it+1}); the frames of S_{it}, .., S_{it-NS+1} lie at
+// Fa - FD, .., Fa - NS FD (FD = LD * n).  The correlation of a cell is
+// accumulated once per pass, at the last level's column, from each level's
+// input M (kept for the levels' column lag) and the NS+1 frames:
+//   acc.x += sum_l M_l (F_l - F_{l+1}),  acc.y += sum_l M_l (F_l + F_{l+1}).
+template <int NS>
+__device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, int L, TileId tid,
+                                          const float* __restrict__ M0,
+                                          const float* __restrict__ Nx0,
+                                          const float* __restrict__ Ny0,
+                                          const float* __restrict__ X0,
+                                          const float* __restrict__ Y0, float* __restrict__ M2,
+                                          float* __restrict__ Nx2, float* __restrict__ Ny2,
+                                          float* __restrict__ X2, float* __restrict__ Y2, int it,
+                                          const float* __restrict__ adj,
+                                          const float* __restrict__ Fa, long long fstride,
+                                          float* __restrict__ acc, long long astride,
+                                          float* __restrict__ gw) {
+  constexpr int R = 64 - 2 * NS;
+  constexpr int UA = NS == 4 ? 2 : 4;  // columns per chunk
   const int lane = threadIdx.x;
   const int NX = g.NX, NY = g.NY, P = g.P;
   const long long LD = g.LD;
   const int shot = tid.shot;
-  const int r0 = tid.rowtile * ROWS2 - 2;
+  const int r0 = tid.rowtile * R - NS;
   const int jb = __builtin_amdgcn_readfirstlane((tid.strip * blockDim.y + threadIdx.y) * L);
   if (jb > NY) return;
   const int je = min(jb + L, NY + 1);
   const int i = r0 + lane;
   const bool okrow = i >= 0 && i <= NX;
-  const bool own = okrow && lane >= 2 && lane <= 61;
+  const bool own = okrow && lane >= NS && lane <= 63 - NS;
   const long long so = (long long)shot * g.FS;
   M0 += so; Nx0 += so; Ny0 += so; X0 += so; Y0 += so;
   M2 += so; Nx2 += so; Ny2 += so; X2 += so; Y2 += so;
@@ -1154,9 +1167,8 @@ __device__ __forceinline__ void adj2_body(const DevGrid& g, const DevShots& s, i
   const float sc = s.sc[shot];
   const int* cp = s.cptr + (long long)shot * (NY + 2);
   const long long FD = (long long)LD * g.n;
-  const long long fo = (long long)shot * fstride;
   float* ac = acc ? acc + 2 * (long long)shot * astride : nullptr;
-  // residual injection reaches halo lanes too: step 2 reads their step-1 values
+  // residual injection reaches halo lanes too: later levels read their values
   const bool has_rec = adj && r0 <= s.rmax && r0 + 63 >= s.rmin;
   const bool inner_rows = (r0 >= P + 1) && (r0 + 63 <= NX - P - 1);
   const bool dom_row = i - g.di0 >= 0 && i - g.di0 < g.m;
@@ -1178,9 +1190,9 @@ __device__ __forceinline__ void adj2_body(const DevGrid& g, const DevShots& s, i
                                rCYH = rsrc_of(const_cast<float*>(g.cyh), g.FS),
                                rHAY = rsrc_of(const_cast<float*>(g.hay), g.FS),
                                rHBY = rsrc_of(const_cast<float*>(g.hby), g.FS),
-                               rFa = rsrc_of(const_cast<float*>(Fa + fo), FD),
-                               rFb = rsrc_of(const_cast<float*>(Fb + fo), FD),
-                               rFc = rsrc_of(const_cast<float*>(Fc + fo), FD),
+                               // frames of S_{it-NS+1} .. S_{it+1}: slot k at k * FD
+                               rFR = rsrc_of(const_cast<float*>(Fa - NS * FD + (long long)shot * fstride),
+                                             (NS + 1) * FD),
                                rAC = rsrc_of(ac, ac ? 2 * FD : 0),
                                rGW = rsrc_of(gw ? gw + (long long)shot * s.nt : nullptr,
                                              gw ? s.nt : 0),
@@ -1221,230 +1233,121 @@ __device__ __forceinline__ void adj2_body(const DevGrid& g, const DevShots& s, i
     }
   };
 
-
-  // ---- state at the start of step-1 column j (step 2 at j-1):
-  // step 1: M0, W0, X0, Y0 at j; px1 at j-1; coefficients of column j
-  // step 2: M after step 1 at j-1 and its W; Ny, Nx, X, Y after step 1 at
-  //         j-1; px2 at j-2; M0 at j-1 (correlation); coefficients of j-1
-  // The prologue loads are branch-free so that they overlap with the first
-  // chunk's.
-  const int j1 = jb - 1;
-  float me1_c, x0_c, y0_c, wy1_c, wx1_c, px1_prev;
-  float ce1_c, cy1_c, cxe_1c, cxh_1c, cyh_1c;
-  {
+  // ---- state.  Level l works on column c-l (c: level 0's column): its
+  // input M, W = (wy, wx), split aux (x, y) and nu = (ny, nx) of that column
+  // (level 0: M, W, x, y from the previous column's look-ahead, nu loaded;
+  // level l >= 1: level l-1's outputs of the previous column), px of its
+  // previous column, and the coefficients of its column:
+  //   ce, cy (inner), cxe, cxh, cyh (row classes), AY/BY (hay, hby), or the
+  //   table entries ka, kb, kc (strip and edge columns).
+  // mh[l][k]: level l's input M k+1 columns ago, for the correlation at the
+  // last level's column.
+  const int j1 = jb - (NS - 1);
+  float me_in[NS], wy_in[NS], wx_in[NS], x_in[NS], y_in[NS], ny_in[NS], nx_in[NS], pxp[NS];
+  float ce_l[NS], cy_l[NS], cxe_l[NS], cxh_l[NS], cyh_l[NS], hay_l[NS], hby_l[NS];
+  float mh[NS][NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    me_in[l] = wy_in[l] = wx_in[l] = x_in[l] = y_in[l] = ny_in[l] = nx_in[l] = pxp[l] = 0.f;
+    ce_l[l] = cy_l[l] = 0.f;
+    cxe_l[l] = 1.f;
+    cxh_l[l] = cyh_l[l] = 0.f;
+    hay_l[l] = 1.f;
+    hby_l[l] = 0.f;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) mh[l][k] = 0.f;
+  }
+  {  // level 0 at column j1, px of level 0 at j1-1 (branch-free loads)
     const unsigned ka = (unsigned)(j1 * LD + ic), kb = ka - (unsigned)LD;
     const bool ina = okrow && j1 >= 0 && j1 <= NY, inb = okrow && j1 - 1 >= 0 && j1 - 1 <= NY;
     const bool pa = pml_cell(j1), pb = pml_cell(j1 - 1);
-    me1_c = bload(rM0, ka, ina);
-    x0_c = bload(rX0, ka, pa);
-    y0_c = bload(rY0, ka, pa);
+    me_in[0] = bload(rM0, ka, ina);
+    x_in[0] = bload(rX0, ka, pa);
+    y_in[0] = bload(rY0, ka, pa);
     const float4 a0 = bload4(rU0, 4 * ka, ina), a1 = bload4(rU1, 4 * ka, ina);
     const float mb = bload(rM0, kb, inb), yb = bload(rY0, kb, pb);
     const float nxb = bload(rNx0, kb, inb && j1 - 1 < NY);
     const float4 b1 = bload4(rU1, 4 * kb, inb);  // (cyh, AX, BX, AY) of column j1-1
-    wy1_c = a0.z * (x0_c + me1_c);
-    wx1_c = a1.x * (y0_c + me1_c);
-    px1_prev = b1.z * (b1.y * nxb + b1.x * (yb + mb) - wx1_c);
-    ce1_c = a0.x;
-    cy1_c = a0.z;
-    cxe_1c = a0.y;
-    cxh_1c = a0.z;
-    cyh_1c = a1.x;
+    wy_in[0] = a0.z * (x_in[0] + me_in[0]);
+    wx_in[0] = a1.x * (y_in[0] + me_in[0]);
+    pxp[0] = b1.z * (b1.y * nxb + b1.x * (yb + mb) - wx_in[0]);
+    ce_l[0] = a0.x;
+    cy_l[0] = a0.z;
+    cxe_l[0] = a0.y;
+    cxh_l[0] = a0.z;
+    cyh_l[0] = a1.x;
   }
-  float me2_c = 0.f, wy2_c = 0.f, wx2_c = 0.f, nyv1_c2 = 0.f, nxv1_c2 = 0.f, x1_c2 = 0.f,
-        y1_c2 = 0.f, px2_prev = 0.f, me1_c2 = 0.f;
-  float ce2_c = 0.f, cxe_2c = 1.f, hay_2c = 1.f, hby_2c = 0.f;
 
-  auto chunkable = [&](int jj) { return jj - 1 >= P + 1 && jj + U <= NY - P - 1; };
+  // every level's column and level 0's look-ahead column are interior
+  auto chunkable = [&](int jj) { return jj - (NS - 1) >= P + 1 && jj + UA <= NY - P - 1; };
+  // the last level's column cz (owned, in the domain) correlates
+  auto corr_on = [&](int cz) { return own && dom_row && cz >= jb && cz < je && dom_col(cz); };
 
-  auto correlate = [&](float2& t, float fa, float fb, float fc) {
-    t.x = (t.x + me1_c2 * (fa - fb)) + me2_c * (fb - fc);
-    t.y = (t.y + me1_c2 * (fa + fb)) + me2_c * (fb + fc);
+  // Hand level l's outputs to level l+1, keep the M history, store the last
+  // level's fields, correlate.  mo, wyo, wxo, xo, yo, nyo, nxo, pxo: outputs
+  // of each level this column; f[k]: frames S_{it+1-k} at column cz.
+  auto finish_column = [&](int c, const float* mo, const float* wyo, const float* wxo,
+                           const float* xo, const float* yo, const float* nyo, const float* nxo,
+                           const float* pxo, const float* f, float2 t, bool split_ok) {
+    const int cz = c - (NS - 1);
+    const bool oz = own && cz >= jb && cz < je;
+    const unsigned kz = (unsigned)(cz * LD + ic);
+    bstore(mo[NS - 1], rM2, kz, oz);
+    bstore(nxo[NS - 1], rNx2, kz, oz && cz < NY);
+    bstore(nyo[NS - 1], rNy2, kz, oz && h_any);
+    const bool pz = split_ok && pml_cell(cz);
+    bstore(xo[NS - 1], rX2, kz, oz && pz);
+    bstore(yo[NS - 1], rY2, kz, oz && pz);
+    // each level's input M at column cz
+    float mz[NS];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) mz[l] = l == NS - 1 ? me_in[l] : mh[l][NS - 2 - l];
+    const bool inner_z = e_inner && cz >= P + 1 && cz <= NY - P - 1;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      t.x = t.x + mz[l] * (f[l] - f[l + 1]);
+      t.y = t.y + mz[l] * (f[l] + f[l + 1]);
+    }
+    bstore2(t, rAC, 2 * (unsigned)((cz - g.dj0) * LD + ic), ac && inner_z && corr_on(cz));
+    const bool at_src = oz && i == si && cz == sj;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) bstore(sc * mz[l], rGW, (unsigned)(it - l), at_src);
+    // histories, then level l+1 inputs from level l outputs
+#pragma unroll
+    for (int l = 0; l < NS - 1; ++l) {
+#pragma unroll
+      for (int k = NS - 2 - l; k >= 1; --k) mh[l][k] = mh[l][k - 1];
+      mh[l][0] = me_in[l];
+    }
+#pragma unroll
+    for (int l = NS - 1; l >= 1; --l) {
+      me_in[l] = mo[l - 1];
+      wy_in[l] = wyo[l - 1];
+      wx_in[l] = wxo[l - 1];
+      x_in[l] = xo[l - 1];
+      y_in[l] = yo[l - 1];
+      ny_in[l] = nyo[l - 1];
+      nx_in[l] = nxo[l - 1];
+    }
+#pragma unroll
+    for (int l = 0; l < NS; ++l) pxp[l] = pxo[l];
+    (void)c;
+  };
+  // correlation operands of column cz: frames S_{it+1} .. S_{it-NS+1}, partials
+  auto load_corr = [&](int cz, float* f, float2& t, bool on) {
+    const unsigned fk = (unsigned)((cz - g.dj0) * LD + ic);
+#pragma unroll
+    for (int k = 0; k <= NS; ++k) f[k] = bload(rFR, (unsigned)((NS - k) * FD) + fk, on);
+    t = bload2(rAC, 2 * fk, on);
   };
 
-  // correlation / gradient operands of step-2 column c2 (owned domain cells)
-  auto corr_on = [&](int c2) { return own && dom_row && c2 >= jb && c2 < je && dom_col(c2); };
-
-  // ---- interior chunk, all rows inner, no receivers
-  auto inner_chunk = [&](int jj) {
-    float mL[U], nx_[U], ny_[U], fa_[U], fb_[U], fc_[U];
-    float2 cL[U], t_[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const unsigned kk = (unsigned)((jj + u) * LD + i);
-      mL[u] = bload(rM0, kk + (unsigned)LD, true);
-      cL[u] = bload2(rCC, 2 * (kk + (unsigned)LD), true);
-      nx_[u] = bload(rNx0, kk, true);
-      ny_[u] = bload(rNy0, kk, true);
-      const int c2 = jj + u - 1;
-      const bool on = ac && corr_on(c2);
-      const unsigned fk = (unsigned)((c2 - g.dj0) * LD + i);
-      fa_[u] = bload(rFa, fk, on);
-      fb_[u] = bload(rFb, fk, on);
-      fc_[u] = bload(rFc, fk, on);
-      t_[u] = bload2(rAC, 2 * fk, on);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int c1 = jj + u, c2 = c1 - 1;
-      // step 1 at c1
-      const float wy1_n = cL[u].y * mL[u];
-      const float wx1_n = -cL[u].y * mL[u];
-      const float nyv1 = ny_[u] + wy1_c - shfl_dn1(wy1_c);
-      const float nxv1 = nx_[u] + wx1_c - wx1_n;
-      const float py1 = by0 * nyv1, px1 = bx0 * nxv1;
-      const float mn1 = ce1_c * me1_c + shfl_up1(py1) - py1 + px1_prev - px1;
-      const float wy2_n = cy1_c * mn1;
-      const float wx2_n = -cy1_c * mn1;
-      // step 2 at c2
-      const float nyv2 = nyv1_c2 + wy2_c - shfl_dn1(wy2_c);
-      const float nxv2 = nxv1_c2 + wx2_c - wx2_n;
-      const float py2 = by0 * nyv2, px2 = bx0 * nxv2;
-      const float mn2 = ce2_c * me2_c + shfl_up1(py2) - py2 + px2_prev - px2;
-      const bool o2 = own && c2 >= jb && c2 < je;
-      const unsigned k2 = (unsigned)(c2 * LD + i);
-      bstore(mn2, rM2, k2, o2);
-      bstore(nxv2, rNx2, k2, o2);
-      bstore(nyv2, rNy2, k2, o2);
-      {
-        float2 t = t_[u];
-        correlate(t, fa_[u], fb_[u], fc_[u]);
-        bstore2(t, rAC, 2 * (unsigned)((c2 - g.dj0) * LD + i), ac && corr_on(c2));
-      }
-      const bool at_src = o2 && i == si && c2 == sj;
-      bstore(sc * me1_c2, rGW, (unsigned)it, at_src);
-      bstore(sc * me2_c, rGW, (unsigned)(it - 1), at_src);
-      me1_c2 = me1_c;
-      me1_c = mL[u];
-      wy1_c = wy1_n;
-      wx1_c = wx1_n;
-      px1_prev = px1;
-      me2_c = mn1;
-      wy2_c = wy2_n;
-      wx2_c = wx2_n;
-      nyv1_c2 = nyv1;
-      nxv1_c2 = nxv1;
-      px2_prev = px2;
-      ce2_c = ce1_c;
-      ce1_c = cL[u].x;
-      cy1_c = cL[u].y;
-    }
-    x0_c = y0_c = x1_c2 = y1_c2 = 0.f;
-    cxe_2c = 1.f;
-    hay_2c = 1.f;
-    hby_2c = by0;
-    cxe_1c = 1.f;
-    cxh_1c = cyh_1c = 0.f;
-  };
-
-  // ---- interior chunk, rows of every class (top / bottom PML rows, receivers)
-  auto class_chunk = [&](int jj) {
-    float mL[U], xL[U], yL[U], cxeL[U], cxhL[U], cyhL[U], nx_[U], ny_[U], hayA[U], hbyA[U],
-        fa_[U], fb_[U], fc_[U];
-    float2 cL[U], t_[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const unsigned kk = (unsigned)((jj + u) * LD + ic), kn = kk + (unsigned)LD;
-      mL[u] = bload(rM0, kn, okrow);
-      cL[u] = bload2(rCC, 2 * kn, e_inner);
-      xL[u] = bload(rX0, kn, e_pml);
-      yL[u] = bload(rY0, kn, e_pml);
-      cxeL[u] = e_xp ? bload(rCXE, kn, e_xp) : cxe_r;
-      cxhL[u] = e_xp ? bload(rCXH, kn, e_xp) : cxh_r;
-      cyhL[u] = e_xp ? bload(rCYH, kn, e_xp) : cyh_r;
-      nx_[u] = bload(rNx0, kk, okrow);
-      ny_[u] = bload(rNy0, kk, h_any);
-      hayA[u] = h_xp ? bload(rHAY, kk, h_xp) : hay_row;
-      hbyA[u] = h_xp ? bload(rHBY, kk, h_xp) : hby_row;
-      const int c2 = jj + u - 1;
-      const bool on = ac && e_inner && corr_on(c2);
-      const unsigned fk = (unsigned)((c2 - g.dj0) * LD + ic);
-      fa_[u] = bload(rFa, fk, on);
-      fb_[u] = bload(rFb, fk, on);
-      fc_[u] = bload(rFc, fk, on);
-      t_[u] = bload2(rAC, 2 * fk, on);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int c1 = jj + u, c2 = c1 - 1;
-      // step 1 at c1
-      const float wy1_n = e_inner ? cL[u].y * mL[u] : (e_pml ? cxhL[u] * (xL[u] + mL[u]) : 0.f);
-      const float wx1_n = e_inner ? -cL[u].y * mL[u] : (e_pml ? cyhL[u] * (yL[u] + mL[u]) : 0.f);
-      const float wdn1 = shfl_dn1(wy1_c);
-      const float nyv1 = h_any ? hayA[u] * ny_[u] + wy1_c - wdn1 : 0.f;
-      const float nxv1 = okrow ? nx_[u] + wx1_c - wx1_n : 0.f;
-      const float py1 = hbyA[u] * nyv1, px1 = bx0 * nxv1;
-      const float pup1 = shfl_up1(py1);
-      const float base1 = e_inner ? ce1_c * me1_c : (e_pml ? 0.f : me1_c);
-      float mn1 = base1 + pup1 - py1 + px1_prev - px1;
-      const float x1 = e_pml ? cxe_1c * (x0_c + me1_c) : 0.f;
-      const float y1 = e_pml ? y0_c + me1_c : 0.f;  // cye = 1 off the left/right strips
-      inject(c1, mn1, it - 1);
-      const float wy2_n = e_inner ? cy1_c * mn1 : (e_pml ? cxh_1c * (x1 + mn1) : 0.f);
-      const float wx2_n = e_inner ? -cy1_c * mn1 : (e_pml ? cyh_1c * (y1 + mn1) : 0.f);
-      // step 2 at c2
-      const float wdn2 = shfl_dn1(wy2_c);
-      const float nyv2 = h_any ? hay_2c * nyv1_c2 + wy2_c - wdn2 : 0.f;
-      const float nxv2 = okrow ? nxv1_c2 + wx2_c - wx2_n : 0.f;
-      const float py2 = hby_2c * nyv2, px2 = bx0 * nxv2;
-      const float pup2 = shfl_up1(py2);
-      const float base2 = e_inner ? ce2_c * me2_c : (e_pml ? 0.f : me2_c);
-      float mn2 = base2 + pup2 - py2 + px2_prev - px2;
-      const float x2 = e_pml ? cxe_2c * (x1_c2 + me2_c) : 0.f;
-      const float y2 = e_pml ? y1_c2 + me2_c : 0.f;
-      inject(c2, mn2, it - 2);
-      const bool o2 = own && c2 >= jb && c2 < je;
-      const unsigned k2 = (unsigned)(c2 * LD + ic);
-      bstore(mn2, rM2, k2, o2);
-      bstore(nxv2, rNx2, k2, o2);
-      bstore(nyv2, rNy2, k2, o2 && h_any);
-      bstore(x2, rX2, k2, o2 && e_pml);
-      bstore(y2, rY2, k2, o2 && e_pml);
-      {
-        float2 t = t_[u];
-        correlate(t, fa_[u], fb_[u], fc_[u]);
-        bstore2(t, rAC, 2 * (unsigned)((c2 - g.dj0) * LD + ic), ac && e_inner && corr_on(c2));
-      }
-      const bool at_src = o2 && i == si && c2 == sj;
-      bstore(sc * me1_c2, rGW, (unsigned)it, at_src);
-      bstore(sc * me2_c, rGW, (unsigned)(it - 1), at_src);
-      me1_c2 = me1_c;
-      me1_c = mL[u];
-      wy1_c = wy1_n;
-      wx1_c = wx1_n;
-      x0_c = xL[u];
-      y0_c = yL[u];
-      px1_prev = px1;
-      me2_c = mn1;
-      wy2_c = wy2_n;
-      wx2_c = wx2_n;
-      nyv1_c2 = nyv1;
-      nxv1_c2 = nxv1;
-      x1_c2 = x1;
-      y1_c2 = y1;
-      px2_prev = px2;
-      ce2_c = ce1_c;
-      cxe_2c = cxe_1c;
-      hay_2c = hayA[u];
-      hby_2c = hbyA[u];
-      ce1_c = cL[u].x;
-      cy1_c = cL[u].y;
-      cxe_1c = cxeL[u];
-      cxh_1c = cxhL[u];
-      cyh_1c = cyhL[u];
-    }
-  };
-
-  // ---- chunk of any columns (left/right strips, grid edges, strip ends):
-  // every coefficient per cell from the table:
-  //   W = (cxh (X + M), cyh (Y + M)),  M' = ce M + d(BY Ny')/di + d(BX Nx')/dj,
-  //   X' = cxe (X + M), Y' = cye (Y + M)
+  // ---- chunk of any columns (strips, grid edges, strip ends): the table
   auto univ_chunk = [&](int jj) {
-    float mL[U], xL[U], yL[U], kcL[U], nx_[U], ny_[U], fa_[U], fb_[U], fc_[U];
-    float4 kaL[U], kbL[U];
-    float2 t_[U];
+    float mL[UA], xL[UA], yL[UA], kcL[UA], nx_[UA], ny_[UA], f_[UA][NS + 1];
+    float4 kaL[UA], kbL[UA];
+    float2 t_[UA];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < UA; ++u) {
       const int c = jj + u, cn = c + 1;
       const unsigned kk = (unsigned)(c * LD + ic), kn = kk + (unsigned)LD;
       const bool nin = okrow && cn >= 0 && cn <= NY;
@@ -1457,111 +1360,227 @@ __device__ __forceinline__ void adj2_body(const DevGrid& g, const DevShots& s, i
       kcL[u] = bload(rU2, kn, nin);
       nx_[u] = bload(rNx0, kk, okrow && c >= 0 && c < NY);
       ny_[u] = bload(rNy0, kk, okrow && c >= 0 && c <= NY && i < NX);
-      const int c2 = c - 1;
-      const bool on = ac && e_inner && c2 >= P + 1 && c2 <= NY - P - 1 && corr_on(c2);
-      const unsigned fk = (unsigned)((c2 - g.dj0) * LD + ic);
-      fa_[u] = bload(rFa, fk, on);
-      fb_[u] = bload(rFb, fk, on);
-      fc_[u] = bload(rFc, fk, on);
-      t_[u] = bload2(rAC, 2 * fk, on);
+      const int cz = c - (NS - 1);
+      load_corr(cz, f_[u], t_[u], ac && e_inner && cz >= P + 1 && cz <= NY - P - 1 && corr_on(cz));
     }
-    float4 ka1, kb1, ka2, kb2;
-    float kc1, kc2;
-    {
-      const bool in1 = okrow && jj >= 0 && jj <= NY, in2 = okrow && jj - 1 >= 0 && jj - 1 <= NY;
-      const unsigned k1 = (unsigned)(jj * LD + ic), k2 = (unsigned)((jj - 1) * LD + ic);
-      ka1 = bload4(rU0, 4 * k1, in1);
-      kb1 = bload4(rU1, 4 * k1, in1);
-      kc1 = bload(rU2, k1, in1);
-      ka2 = bload4(rU0, 4 * k2, in2);
-      kb2 = bload4(rU1, 4 * k2, in2);
-      kc2 = bload(rU2, k2, in2);
+    // table entries of every level's first column (level 0: column jj)
+    float4 ka_l[NS], kb_l[NS];
+    float kc_l[NS];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const int c = jj - l;
+      const bool cin = okrow && c >= 0 && c <= NY;
+      const unsigned kk = (unsigned)(c * LD + ic);
+      ka_l[l] = bload4(rU0, 4 * kk, cin);
+      kb_l[l] = bload4(rU1, 4 * kk, cin);
+      kc_l[l] = bload(rU2, kk, cin);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int c1 = jj + u, c2 = c1 - 1;
-      // W at c1+1 before step 1: (ce, cxe, cxh, cye) = ka, (cyh, AX, BX, AY) = kb, BY = kc
-      const float wy1_n = kaL[u].z * (xL[u] + mL[u]);
-      const float wx1_n = kbL[u].x * (yL[u] + mL[u]);
-      // step 1 at c1
-      const float nyv1 = kb1.w * ny_[u] + wy1_c - shfl_dn1(wy1_c);
-      const float nxv1 = kb1.y * nx_[u] + wx1_c - wx1_n;
-      const float py1 = kc1 * nyv1, px1 = kb1.z * nxv1;
-      float mn1 = ka1.x * me1_c + shfl_up1(py1) - py1 + px1_prev - px1;
-      const float x1 = ka1.y * (x0_c + me1_c);
-      const float y1 = ka1.w * (y0_c + me1_c);
-      inject(c1, mn1, it - 1);
-      const float wy2_n = ka1.z * (x1 + mn1);
-      const float wx2_n = kb1.x * (y1 + mn1);
-      // step 2 at c2
-      const float nyv2 = kb2.w * nyv1_c2 + wy2_c - shfl_dn1(wy2_c);
-      const float nxv2 = kb2.y * nxv1_c2 + wx2_c - wx2_n;
-      const float py2 = kc2 * nyv2, px2 = kb2.z * nxv2;
-      float mn2 = ka2.x * me2_c + shfl_up1(py2) - py2 + px2_prev - px2;
-      const float x2 = ka2.y * (x1_c2 + me2_c);
-      const float y2 = ka2.w * (y1_c2 + me2_c);
-      inject(c2, mn2, it - 2);
-      const bool o2 = own && c2 >= jb && c2 < je;
-      const bool pm2 = pml_cell(c2);
-      const unsigned k2 = (unsigned)(c2 * LD + ic);
-      bstore(mn2, rM2, k2, o2);
-      bstore(nxv2, rNx2, k2, o2 && c2 < NY);
-      bstore(nyv2, rNy2, k2, o2 && h_any);
-      bstore(x2, rX2, k2, o2 && pm2);
-      bstore(y2, rY2, k2, o2 && pm2);
-      {
-        float2 t = t_[u];
-        correlate(t, fa_[u], fb_[u], fc_[u]);
-        bstore2(t, rAC, 2 * (unsigned)((c2 - g.dj0) * LD + ic),
-                ac && e_inner && c2 >= P + 1 && c2 <= NY - P - 1 && corr_on(c2));
+    for (int u = 0; u < UA; ++u) {
+      const int c = jj + u;
+      // W of level 0's look-ahead column c+1: (ce, cxe, cxh, cye) = ka, (cyh, AX, BX, AY) = kb
+      const float wy0n = kaL[u].z * (xL[u] + mL[u]);
+      const float wx0n = kbL[u].x * (yL[u] + mL[u]);
+      float mo[NS], wyo[NS], wxo[NS], xo[NS], yo[NS], nyo[NS], nxo[NS], pxo[NS];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const int cl = c - l;
+        const float4 ka = ka_l[l], kb = kb_l[l];
+        const float kc = kc_l[l];
+        const float me = me_in[l], ny = l == 0 ? ny_[u] : ny_in[l], nx = l == 0 ? nx_[u] : nx_in[l];
+        const float wxn = l == 0 ? wx0n : wxo[l - 1];
+        const float nyv = kb.w * ny + wy_in[l] - shfl_dn1(wy_in[l]);
+        const float nxv = kb.y * nx + wx_in[l] - wxn;
+        const float py = kc * nyv, px = kb.z * nxv;
+        float mn = ka.x * me + shfl_up1(py) - py + pxp[l] - px;
+        const float x1 = ka.y * (x_in[l] + me);
+        const float y1 = ka.w * (y_in[l] + me);
+        inject(cl, mn, it - l - 1);
+        mo[l] = mn;
+        wyo[l] = ka.z * (x1 + mn);
+        wxo[l] = kb.x * (y1 + mn);
+        xo[l] = x1;
+        yo[l] = y1;
+        nyo[l] = nyv;
+        nxo[l] = nxv;
+        pxo[l] = px;
       }
-      const bool at_src = o2 && i == si && c2 == sj;
-      bstore(sc * me1_c2, rGW, (unsigned)it, at_src);
-      bstore(sc * me2_c, rGW, (unsigned)(it - 1), at_src);
-      me1_c2 = me1_c;
-      me1_c = mL[u];
-      wy1_c = wy1_n;
-      wx1_c = wx1_n;
-      x0_c = xL[u];
-      y0_c = yL[u];
-      px1_prev = px1;
-      me2_c = mn1;
-      wy2_c = wy2_n;
-      wx2_c = wx2_n;
-      nyv1_c2 = nyv1;
-      nxv1_c2 = nxv1;
-      x1_c2 = x1;
-      y1_c2 = y1;
-      px2_prev = px2;
-      ka2 = ka1;
-      kb2 = kb1;
-      kc2 = kc1;
-      ka1 = kaL[u];
-      kb1 = kbL[u];
-      kc1 = kcL[u];
+      finish_column(c, mo, wyo, wxo, xo, yo, nyo, nxo, pxo, f_[u], t_[u], true);
+      me_in[0] = mL[u];
+      wy_in[0] = wy0n;
+      wx_in[0] = wx0n;
+      x_in[0] = xL[u];
+      y_in[0] = yL[u];
+#pragma unroll
+      for (int l = NS - 1; l >= 1; --l) {
+        ka_l[l] = ka_l[l - 1];
+        kb_l[l] = kb_l[l - 1];
+        kc_l[l] = kc_l[l - 1];
+      }
+      ka_l[0] = kaL[u];
+      kb_l[0] = kbL[u];
+      kc_l[0] = kcL[u];
     }
-    // coefficients of the next columns, in the interior chunks' form
-    ce1_c = ka1.x;
-    cy1_c = ka1.z;
-    cxe_1c = ka1.y;
-    cxh_1c = ka1.z;
-    cyh_1c = kb1.x;
-    ce2_c = ka2.x;
-    cxe_2c = ka2.y;
-    hay_2c = kb2.w;
-    hby_2c = kc2;
+    // coefficients of every level's next column, in the interior chunks' form
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      ce_l[l] = ka_l[l].x;
+      cy_l[l] = ka_l[l].z;
+      cxe_l[l] = ka_l[l].y;
+      cxh_l[l] = ka_l[l].z;
+      cyh_l[l] = kb_l[l].x;
+      hay_l[l] = kb_l[l].w;
+      hby_l[l] = kc_l[l];
+    }
   };
 
-  // step 1 at columns j .. j+U-1, step 2 one column behind
+  // ---- interior chunk, all rows inner, no receivers
+  auto inner_chunk = [&](int jj) {
+    float mL[UA], nx_[UA], ny_[UA], f_[UA][NS + 1];
+    float2 cL[UA], t_[UA];
+#pragma unroll
+    for (int u = 0; u < UA; ++u) {
+      const unsigned kk = (unsigned)((jj + u) * LD + i);
+      mL[u] = bload(rM0, kk + (unsigned)LD, true);
+      cL[u] = bload2(rCC, 2 * (kk + (unsigned)LD), true);
+      nx_[u] = bload(rNx0, kk, true);
+      ny_[u] = bload(rNy0, kk, true);
+      const int cz = jj + u - (NS - 1);
+      load_corr(cz, f_[u], t_[u], ac && corr_on(cz));
+    }
+#pragma unroll
+    for (int u = 0; u < UA; ++u) {
+      const int c = jj + u;
+      const float wy0n = cL[u].y * mL[u];
+      const float wx0n = -cL[u].y * mL[u];
+      float mo[NS], wyo[NS], wxo[NS], zo[NS], nyo[NS], nxo[NS], pxo[NS];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const float me = me_in[l], ny = l == 0 ? ny_[u] : ny_in[l], nx = l == 0 ? nx_[u] : nx_in[l];
+        const float wxn = l == 0 ? wx0n : wxo[l - 1];
+        const float nyv = ny + wy_in[l] - shfl_dn1(wy_in[l]);
+        const float nxv = nx + wx_in[l] - wxn;
+        const float py = by0 * nyv, px = bx0 * nxv;
+        const float mn = ce_l[l] * me + shfl_up1(py) - py + pxp[l] - px;
+        mo[l] = mn;
+        wyo[l] = cy_l[l] * mn;
+        wxo[l] = -cy_l[l] * mn;
+        zo[l] = 0.f;  // no split-field cells in these rows
+        nyo[l] = nyv;
+        nxo[l] = nxv;
+        pxo[l] = px;
+      }
+      finish_column(c, mo, wyo, wxo, zo, zo, nyo, nxo, pxo, f_[u], t_[u], false);
+      me_in[0] = mL[u];
+      wy_in[0] = wy0n;
+      wx_in[0] = wx0n;
+      x_in[0] = y_in[0] = 0.f;
+#pragma unroll
+      for (int l = NS - 1; l >= 1; --l) {
+        ce_l[l] = ce_l[l - 1];
+        cy_l[l] = cy_l[l - 1];
+      }
+      ce_l[0] = cL[u].x;
+      cy_l[0] = cL[u].y;
+    }
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cxe_l[l] = 1.f;
+      cxh_l[l] = cyh_l[l] = 0.f;
+      hay_l[l] = 1.f;
+      hby_l[l] = by0;
+    }
+  };
+
+  // ---- interior chunk, rows of every class (top / bottom PML rows, receivers)
+  auto class_chunk = [&](int jj) {
+    float mL[UA], xL[UA], yL[UA], cxeL[UA], cxhL[UA], cyhL[UA], nx_[UA], ny_[UA], hayA[UA],
+        hbyA[UA], f_[UA][NS + 1];
+    float2 cL[UA], t_[UA];
+#pragma unroll
+    for (int u = 0; u < UA; ++u) {
+      const unsigned kk = (unsigned)((jj + u) * LD + ic), kn = kk + (unsigned)LD;
+      mL[u] = bload(rM0, kn, okrow);
+      cL[u] = bload2(rCC, 2 * kn, e_inner);
+      xL[u] = bload(rX0, kn, e_pml);
+      yL[u] = bload(rY0, kn, e_pml);
+      cxeL[u] = e_xp ? bload(rCXE, kn, e_xp) : cxe_r;
+      cxhL[u] = e_xp ? bload(rCXH, kn, e_xp) : cxh_r;
+      cyhL[u] = e_xp ? bload(rCYH, kn, e_xp) : cyh_r;
+      nx_[u] = bload(rNx0, kk, okrow);
+      ny_[u] = bload(rNy0, kk, h_any);
+      hayA[u] = h_xp ? bload(rHAY, kk, h_xp) : hay_row;
+      hbyA[u] = h_xp ? bload(rHBY, kk, h_xp) : hby_row;
+      const int cz = jj + u - (NS - 1);
+      load_corr(cz, f_[u], t_[u], ac && e_inner && corr_on(cz));
+    }
+#pragma unroll
+    for (int u = 0; u < UA; ++u) {
+      const int c = jj + u;
+      const float wy0n = e_inner ? cL[u].y * mL[u] : (e_pml ? cxhL[u] * (xL[u] + mL[u]) : 0.f);
+      const float wx0n = e_inner ? -cL[u].y * mL[u] : (e_pml ? cyhL[u] * (yL[u] + mL[u]) : 0.f);
+      float mo[NS], wyo[NS], wxo[NS], xo[NS], yo[NS], nyo[NS], nxo[NS], pxo[NS];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const int cl = c - l;
+        const float me = me_in[l], ny = l == 0 ? ny_[u] : ny_in[l], nx = l == 0 ? nx_[u] : nx_in[l];
+        const float hay = l == 0 ? hayA[u] : hay_l[l], hby = l == 0 ? hbyA[u] : hby_l[l];
+        const float wxn = l == 0 ? wx0n : wxo[l - 1];
+        const float wdn = shfl_dn1(wy_in[l]);
+        const float nyv = h_any ? hay * ny + wy_in[l] - wdn : 0.f;
+        const float nxv = okrow ? nx + wx_in[l] - wxn : 0.f;
+        const float py = hby * nyv, px = bx0 * nxv;
+        const float pup = shfl_up1(py);
+        const float base = e_inner ? ce_l[l] * me : (e_pml ? 0.f : me);
+        float mn = base + pup - py + pxp[l] - px;
+        const float x1 = e_pml ? cxe_l[l] * (x_in[l] + me) : 0.f;
+        const float y1 = e_pml ? y_in[l] + me : 0.f;  // cye = 1 off the left/right strips
+        inject(cl, mn, it - l - 1);
+        mo[l] = mn;
+        wyo[l] = e_inner ? cy_l[l] * mn : (e_pml ? cxh_l[l] * (x1 + mn) : 0.f);
+        wxo[l] = e_inner ? -cy_l[l] * mn : (e_pml ? cyh_l[l] * (y1 + mn) : 0.f);
+        xo[l] = x1;
+        yo[l] = y1;
+        nyo[l] = nyv;
+        nxo[l] = nxv;
+        pxo[l] = px;
+      }
+      finish_column(c, mo, wyo, wxo, xo, yo, nyo, nxo, pxo, f_[u], t_[u], true);
+      me_in[0] = mL[u];
+      wy_in[0] = wy0n;
+      wx_in[0] = wx0n;
+      x_in[0] = xL[u];
+      y_in[0] = yL[u];
+#pragma unroll
+      for (int l = NS - 1; l >= 1; --l) {
+        ce_l[l] = ce_l[l - 1];
+        cy_l[l] = cy_l[l - 1];
+        cxe_l[l] = cxe_l[l - 1];
+        cxh_l[l] = cxh_l[l - 1];
+        cyh_l[l] = cyh_l[l - 1];
+        hay_l[l] = l == 1 ? hayA[u] : hay_l[l - 1];
+        hby_l[l] = l == 1 ? hbyA[u] : hby_l[l - 1];
+      }
+      ce_l[0] = cL[u].x;
+      cy_l[0] = cL[u].y;
+      cxe_l[0] = cxeL[u];
+      cxh_l[0] = cxhL[u];
+      cyh_l[0] = cyhL[u];
+    }
+  };
+
+  // level 0 at columns jb-NS+1 .. je+NS-2 (the last level's last column is je-1)
   const bool use_inner = inner_rows && !has_rec;
-  for (int j = j1; j <= je; j += U) {
-    if (!chunkable(j))
-      univ_chunk(j);
-    else if (use_inner)
+  const int jlast = je + NS - 2;
+  int j = j1;
+  for (; j <= jlast && !chunkable(j); j += UA) univ_chunk(j);
+  for (; j <= jlast && chunkable(j); j += UA) {
+    if (use_inner)
       inner_chunk(j);
     else
       class_chunk(j);
   }
+  for (; j <= jlast; j += UA) univ_chunk(j);
 }
 
 __global__ void __launch_bounds__(256)
@@ -1570,27 +1589,42 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ M2,
             float* __restrict__ Nx2, float* __restrict__ Ny2, float* __restrict__ X2,
             float* __restrict__ Y2, int it, const float* __restrict__ adj,
-            const float* __restrict__ Fa, const float* __restrict__ Fb,
-            const float* __restrict__ Fc, long long fstride, float* __restrict__ acc,
+            const float* __restrict__ Fa, long long fstride, float* __restrict__ acc,
             long long astride, float* __restrict__ gw) {
-  adj2_body(g, s, L, tile_of_block(g.NX, g.NY, L, ROWS2), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2, Y2, it, adj, Fa, Fb, Fc, fstride, acc, astride, gw);
+  adjN_body<2>(g, s, L, tile_of_block(g.NX, g.NY, L, 60), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
+               Y2, it, adj, Fa, fstride, acc, astride, gw);
 }
 
 __global__ void __launch_bounds__(256)
-k_adj_step2_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
+k_adj_step4(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
+            const float* __restrict__ Nx0, const float* __restrict__ Ny0,
+            const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ M2,
+            float* __restrict__ Nx2, float* __restrict__ Ny2, float* __restrict__ X2,
+            float* __restrict__ Y2, int it, const float* __restrict__ adj,
+            const float* __restrict__ Fa, long long fstride, float* __restrict__ acc,
+            long long astride, float* __restrict__ gw) {
+  adjN_body<4>(g, s, L, tile_of_block(g.NX, g.NY, L, 56), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
+               Y2, it, adj, Fa, fstride, acc, astride, gw);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(256)
+k_adj_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
   int k;
   TileId tid;
-  if (!multi_tile(a, sv, ROWS2, k, tid)) return;
+  if (!multi_tile(a, sv, 64 - 2 * NS, k, tid)) return;
   const DevSurvey& S = sv[a.sid[k]];
   const int c = a.cur[k], d = c ^ 1;
   float* m = S.adjf;
   const long long z = S.fsz;
-  const float* Fb = S.frames + (long long)a.fslot[k] * S.FD;
-  adj2_body(S.g, S.s, a.L[k], tid, m + 5 * c * z, m + (5 * c + 1) * z, m + (5 * c + 2) * z,
-            m + (5 * c + 3) * z, m + (5 * c + 4) * z, m + 5 * d * z, m + (5 * d + 1) * z,
-            m + (5 * d + 2) * z, m + (5 * d + 3) * z, m + (5 * d + 4) * z, a.it[k], S.adj,
-            Fb + S.FD, Fb, Fb - S.FD, S.fstride, S.acc, S.FD, nullptr);
+  const float* Fa = S.frames + (long long)(a.fslot[k] + 1) * S.FD;
+  adjN_body<NS>(S.g, S.s, a.L[k], tid, m + 5 * c * z, m + (5 * c + 1) * z, m + (5 * c + 2) * z,
+                m + (5 * c + 3) * z, m + (5 * c + 4) * z, m + 5 * d * z, m + (5 * d + 1) * z,
+                m + (5 * d + 2) * z, m + (5 * d + 3) * z, m + (5 * d + 4) * z, a.it[k], S.adj, Fa,
+                S.fstride, S.acc, S.FD, nullptr);
 }
+template __global__ void k_adj_stepN_multi<2>(const DevSurvey*, MultiArgs);
+template __global__ void k_adj_stepN_multi<4>(const DevSurvey*, MultiArgs);
 
 // ==========================================================================
 // small kernels

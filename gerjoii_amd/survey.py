@@ -285,8 +285,8 @@ class Survey:
 
     def kernel_stats(self, which, reset=False):
         """(average ms, samples, launches) of the one-step forward (0), one-step
-        adjoint (1), two-step forward (2), two-step adjoint (3) and four-step
-        forward (4) kernels while GJW_OPT_KTIMING is on."""
+        adjoint (1), two-step forward (2), two-step adjoint (3), four-step
+        forward (4) and four-step adjoint (5) kernels while GJW_OPT_KTIMING is on."""
         ms, n, nl = C.c_double(0), C.c_int64(0), C.c_int64(0)
         N.check(N.load().gjw_kernel_stats(self.h, int(which), C.byref(ms), C.byref(n), C.byref(nl),
                                           int(reset)), "gjw_kernel_stats")
