@@ -810,8 +810,9 @@ static int fwd_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int n
 }
 
 // NS steps per launch; grid of (64 - 2 NS)-row overlapped tiles
+// (a block stacks 4 row tiles of one strip)
 static inline dim3 stepN_grid(const gjw_survey* s, int NS, int L, int nshots) {
-  return dim3((unsigned)(cdiv(s->NX + 1, 64 - 2 * NS) * cdiv(cdiv(s->NY + 1, L), 4) * nshots));
+  return dim3((unsigned)(cdiv(cdiv(s->NX + 1, 64 - 2 * NS), 4) * cdiv(s->NY + 1, L) * nshots));
 }
 
 // NS = 2 or 4 fused forward steps; fr1 (optional) receives the NS domain
@@ -1831,7 +1832,8 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
     a.cur[k] = cur[ids[k]];
     a.xc[k] = xc ? xc[ids[k]] : 0;
     a.fslot[k] = fslot ? fslot[ids[k]] : -1;
-    const long long nb = cdiv(s->NX + 1, rows) * cdiv(cdiv(s->NY + 1, L), 4) * s->ns;
+    const long long nb = steps == 1 ? cdiv(s->NX + 1, rows) * cdiv(cdiv(s->NY + 1, L), 4) * s->ns
+                                    : cdiv(cdiv(s->NX + 1, rows), 4) * cdiv(s->NY + 1, L) * s->ns;
     blk += (int)(cdiv(nb, 8) * 8);
   }
   a.blk0[a.n] = blk;

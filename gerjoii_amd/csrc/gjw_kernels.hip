@@ -136,12 +136,29 @@ __device__ __forceinline__ TileId tile_of_block(int NX, int NY, int L, int rows 
   return tile_decode(t, gx, gy, B / (gx * gy));
 }
 
+// Multi-step kernels stack a block's 4 waves vertically: block (rowtile,
+// strip) covers row tiles 4*rowtile .. 4*rowtile+3 of one column strip, so
+// the block reads about 4 x 256 contiguous bytes of each column and field.
+__device__ __forceinline__ unsigned vtiles_x(int NX, int rows) {
+  return (unsigned)(((NX + 1 + rows - 1) / rows + 3) / 4);
+}
+__device__ __forceinline__ unsigned vtiles_y(int NY, int L) {
+  return (unsigned)((NY + 1 + L - 1) / L);
+}
+__device__ __forceinline__ TileId vtile_of_block(int NX, int NY, int L, int rows) {
+  const unsigned B = gridDim.x, b = blockIdx.x;
+  const unsigned per = B / 8u;
+  const unsigned t = (b < per * 8u) ? (b % 8u) * per + b / 8u : b;
+  const unsigned gx = vtiles_x(NX, rows), gy = vtiles_y(NY, L);
+  return tile_decode(t, gx, gy, B / (gx * gy));
+}
+
 // Several surveys in one launch: surveys[a.sid[k]] owns blocks
 // [a.blk0[k], a.blk0[k+1]) (a multiple of 8 each, so the XCD-aware order holds
 // per survey); its own step, tile width, buffer parities and frame slot come
 // with it.  Blocks past a survey's tiles exit.
 __device__ __forceinline__ bool multi_tile(const MultiArgs& a, const DevSurvey* __restrict__ sv,
-                                           int rows, int& k, TileId& tid) {
+                                           int rows, int& k, TileId& tid, bool vert = false) {
   const int b = (int)blockIdx.x;
   k = 0;
   while (k + 1 < a.n && b >= a.blk0[k + 1]) ++k;
@@ -149,7 +166,8 @@ __device__ __forceinline__ bool multi_tile(const MultiArgs& a, const DevSurvey* 
   const unsigned B = (unsigned)(a.blk0[k + 1] - a.blk0[k]), bl = (unsigned)(b - a.blk0[k]);
   const unsigned per = B / 8u;
   const unsigned t = (bl % 8u) * per + bl / 8u;
-  const unsigned gx = tiles_x(S.g.NX, rows), gy = tiles_y(S.g.NY, a.L[k]);
+  const unsigned gx = vert ? vtiles_x(S.g.NX, rows) : tiles_x(S.g.NX, rows);
+  const unsigned gy = vert ? vtiles_y(S.g.NY, a.L[k]) : tiles_y(S.g.NY, a.L[k]);
   if (t >= gx * gy * (unsigned)S.ns) return false;
   tid = tile_decode(t, gx, gy, (unsigned)S.ns);
   return true;
@@ -407,15 +425,20 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
                                           float* __restrict__ tr, float* __restrict__ frame1,
                                           long long fstride) {
   constexpr int R = 64 - 2 * NS;
+#ifndef GJW_FWD_UC4
+  constexpr int UC = NS == 4 ? 2 : U;  // columns per strip / row-class chunk
+#else
+  constexpr int UC = U;
+#endif
   const int lane = threadIdx.x;
   const int NX = g.NX, NY = g.NY, P = g.P;
   const long long LD = g.LD;
   const int shot = tid.shot;
-  const int r0 = tid.rowtile * R - NS;
-  // threadIdx.y is constant across a 64-lane wavefront: make the strip start
+  // threadIdx.y is constant across a 64-lane wavefront: make the tile origin
   // (and everything derived from it) scalar so loop control stays uniform
-  const int jb = __builtin_amdgcn_readfirstlane((tid.strip * blockDim.y + threadIdx.y) * L);
-  if (jb > NY) return;
+  const int r0 = __builtin_amdgcn_readfirstlane((tid.rowtile * blockDim.y + threadIdx.y) * R - NS);
+  const int jb = __builtin_amdgcn_readfirstlane(tid.strip * L);
+  if (r0 + NS > NX) return;  // no owned row
   const int je = min(jb + L, NY + 1);
   const int i = r0 + lane;
   const bool okrow = i >= 0 && i <= NX;
@@ -542,16 +565,16 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     for (int l = 0; l < NS; ++l) hxp[l] = hxo[l];
   };
 
-  // every level's columns of a chunk at jj are interior
-  auto chunkable = [&](int jj) { return jj - (NS - 1) >= P + 1 && jj + U - 1 <= NY - P - 1; };
+  // every level's columns of a chunk of w columns at jj are interior
+  auto chunkable = [&](int jj, int w) { return jj - (NS - 1) >= P + 1 && jj + w - 1 <= NY - P - 1; };
 
   // ---- chunk of any columns (left/right strips, grid edges, strip ends):
   // every coefficient per cell from the table, Ez' = ce Ez + X' + Y'
   auto univ_chunk = [&](int jj) {
-    float en_[U], hx_[U], hy_[U], x_[U], y_[U], kc_[U];
-    float4 ka_[U], kb_[U];
+    float en_[UC], hx_[UC], hy_[UC], x_[UC], y_[UC], kc_[UC];
+    float4 ka_[UC], kb_[UC];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < UC; ++u) {
       const int c = jj + u;
       const bool cin = okrow && c >= 0 && c <= NY;
       const unsigned kk = (unsigned)(c * LD + ic);
@@ -578,7 +601,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       kc_l[l] = bload(rU2, kk, cin);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < UC; ++u) {
       const int c = jj + u;
       float eo[NS], hxo[NS], hyo[NS], xo[NS], yo[NS];
 #pragma unroll
@@ -698,10 +721,10 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
 
   // ---- interior chunk, rows of every class (top / bottom PML rows present)
   auto class_chunk = [&](int jj) {
-    float en_[U], hx_[U], hy_[U], x_[U], y_[U], cxe_[U], cxh_[U], cyh_[U], hay_[U], hby_[U];
-    float2 c_[U];
+    float en_[UC], hx_[UC], hy_[UC], x_[UC], y_[UC], cxe_[UC], cxh_[UC], cyh_[UC], hay_[UC], hby_[UC];
+    float2 c_[UC];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < UC; ++u) {
       const unsigned kk = (unsigned)((jj + u) * LD + ic);
       en_[u] = bload(rE0, kk + (unsigned)LD, okrow);
       hx_[u] = bload(rHx0, kk, okrow);
@@ -716,7 +739,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       hby_[u] = h_xp ? bload(rHBY, kk, h_xp) : hby_row;
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < UC; ++u) {
       const int c = jj + u;
       float eo[NS], hxo[NS], hyo[NS], xo[NS], yo[NS];
 #pragma unroll
@@ -772,13 +795,14 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   // Level 0 at columns j .. j+U-1 per chunk, from jb-NS+1 to je+NS-2 (the
   // last level's last column is je-1).  Chunk starts split into a
   // strip/edge prefix, an interior run and a suffix.
-  const bool use_inner = inner_rows && !has_rec;
+  // The two wave kinds get separate copies of the loop, so that each keeps
+  // only its own coefficient carries live.
   const int jlast = je + NS - 2;
-  int j = j1;
-  for (; j <= jlast && !chunkable(j); j += U) univ_chunk(j);
-  int jz = j;  // end of the interior run
-  while (jz <= jlast && chunkable(jz)) jz += U;
-  if (use_inner) {
+  if (inner_rows && !has_rec) {
+    int j = j1;
+    for (; j <= jlast && !chunkable(j, U); j += UC) univ_chunk(j);
+    int jz = j;  // end of the interior run
+    while (jz <= jlast && chunkable(jz, U)) jz += U;
     InnerBuf A, B;
     inner_load(j, A, j < jz);
     while (j < jz) {
@@ -790,10 +814,13 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       inner_compute(j, B);
       j += U;
     }
+    for (; j <= jlast; j += UC) univ_chunk(j);
   } else {
-    for (; j < jz; j += U) class_chunk(j);
+    int j = j1;
+    for (; j <= jlast && !chunkable(j, UC); j += UC) univ_chunk(j);
+    for (; j <= jlast && chunkable(j, UC); j += UC) class_chunk(j);
+    for (; j <= jlast; j += UC) univ_chunk(j);
   }
-  for (; j <= jlast; j += U) univ_chunk(j);
 }
 
 template <bool FRAMES>
@@ -804,7 +831,7 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
             float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
             float* __restrict__ Y2, int it, float* __restrict__ tr, float* __restrict__ frame1,
             long long fstride) {
-  fwdN_body<2, FRAMES>(g, s, L, tile_of_block(g.NX, g.NY, L, 60), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
+  fwdN_body<2, FRAMES>(g, s, L, vtile_of_block(g.NX, g.NY, L, 60), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
                        Hy2, X2, Y2, it, tr, frame1, fstride);
 }
 
@@ -816,7 +843,7 @@ k_fwd_step4(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
             float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
             float* __restrict__ Y2, int it, float* __restrict__ tr, float* __restrict__ frame1,
             long long fstride) {
-  fwdN_body<4, FRAMES>(g, s, L, tile_of_block(g.NX, g.NY, L, 56), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
+  fwdN_body<4, FRAMES>(g, s, L, vtile_of_block(g.NX, g.NY, L, 56), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
                        Hy2, X2, Y2, it, tr, frame1, fstride);
 }
 
@@ -825,7 +852,7 @@ __global__ void __launch_bounds__(256)
 k_fwd_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
   int k;
   TileId tid;
-  if (!multi_tile(a, sv, 64 - 2 * NS, k, tid)) return;
+  if (!multi_tile(a, sv, 64 - 2 * NS, k, tid, true)) return;
   const DevSurvey& S = sv[a.sid[k]];
   const int c = a.cur[k], x = a.xc[k];
   float* f = S.fld;
@@ -1153,9 +1180,9 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   const int NX = g.NX, NY = g.NY, P = g.P;
   const long long LD = g.LD;
   const int shot = tid.shot;
-  const int r0 = tid.rowtile * R - NS;
-  const int jb = __builtin_amdgcn_readfirstlane((tid.strip * blockDim.y + threadIdx.y) * L);
-  if (jb > NY) return;
+  const int r0 = __builtin_amdgcn_readfirstlane((tid.rowtile * blockDim.y + threadIdx.y) * R - NS);
+  const int jb = __builtin_amdgcn_readfirstlane(tid.strip * L);
+  if (r0 + NS > NX) return;  // no owned row
   const int je = min(jb + L, NY + 1);
   const int i = r0 + lane;
   const bool okrow = i >= 0 && i <= NX;
@@ -1591,7 +1618,7 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             float* __restrict__ Y2, int it, const float* __restrict__ adj,
             const float* __restrict__ Fa, long long fstride, float* __restrict__ acc,
             long long astride, float* __restrict__ gw) {
-  adjN_body<2>(g, s, L, tile_of_block(g.NX, g.NY, L, 60), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
+  adjN_body<2>(g, s, L, vtile_of_block(g.NX, g.NY, L, 60), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
                Y2, it, adj, Fa, fstride, acc, astride, gw);
 }
 
@@ -1603,7 +1630,7 @@ k_adj_step4(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             float* __restrict__ Y2, int it, const float* __restrict__ adj,
             const float* __restrict__ Fa, long long fstride, float* __restrict__ acc,
             long long astride, float* __restrict__ gw) {
-  adjN_body<4>(g, s, L, tile_of_block(g.NX, g.NY, L, 56), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
+  adjN_body<4>(g, s, L, vtile_of_block(g.NX, g.NY, L, 56), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
                Y2, it, adj, Fa, fstride, acc, astride, gw);
 }
 
@@ -1612,7 +1639,7 @@ __global__ void __launch_bounds__(256)
 k_adj_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
   int k;
   TileId tid;
-  if (!multi_tile(a, sv, 64 - 2 * NS, k, tid)) return;
+  if (!multi_tile(a, sv, 64 - 2 * NS, k, tid, true)) return;
   const DevSurvey& S = sv[a.sid[k]];
   const int c = a.cur[k], d = c ^ 1;
   float* m = S.adjf;
