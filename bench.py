@@ -45,6 +45,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--fuse", type=int, default=0, help="steps per launch: 0 autotuned, 1, 2 or 4")
     ap.add_argument("--ckpt", type=int, default=0)
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run three 1-GiB device copies (calibrates PMC byte counters)")
@@ -84,6 +85,8 @@ def main():
     sv.set_shots(case.src, case.wavelets, case.receivers)
     if args.tile:
         sv.set_option(N.GJW_OPT_TILE_L, args.tile)
+    if args.fuse:
+        sv.set_option(N.GJW_OPT_FUSE, args.fuse)
     if args.ckpt:
         sv.set_option(N.GJW_OPT_CKPT_EVERY, args.ckpt)
     # observed data: true model, kept in HBM
@@ -91,7 +94,7 @@ def main():
     d_obs = torch.zeros(sv.traces_shape(), dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
     sv.forward(out=d_obs)
-    log("observed traces ready; tile L =", sv.tile())
+    log("observed traces ready; tile L =", sv.tile(4))
     sv.set_model(case.eps0, case.sig0)
     g_eps = torch.zeros((g.n, g.m), dtype=torch.float64, device=dev)  # (m, n) column-major
     g_sig = torch.zeros_like(g_eps)
@@ -122,7 +125,7 @@ def main():
     cell_updates = 3.0 * cells * g.nt * ns * args.steps  # forward, recompute, adjoint
     gcells = cell_updates / dt_s / 1e9
     ge_b = g_eps.cpu().numpy().T.copy()
-    tile_L = sv.tile()
+    tile_L = {k: sv.tile(w) for k, w in (("fwd2", 2), ("fwd4", 4), ("adj2", 3), ("adj4", 5))}
     sv.close()
     del sv
     torch.cuda.empty_cache()

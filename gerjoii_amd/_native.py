@@ -79,7 +79,7 @@ def load():
     lib.gjw_differentiate_cube.argtypes = [vp, vp, i32, i32, i32, C.c_double]
     lib.gjw_last_timing.argtypes = [vp, vp, vp]
     lib.gjw_kernel_stats.argtypes = [vp, i32, vp, vp, vp, i32]
-    lib.gjw_tile.argtypes = [vp, vp]
+    lib.gjw_tile.argtypes = [vp, C.c_int32, vp]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.gjw_last_error.restype = C.c_char_p

@@ -175,6 +175,7 @@ struct gjw_survey {
   int path = GJW_PATH_BATCHED, ck_every = 0, tile_L = 0, snap = GJW_SNAP_F32, fuse = 0;
   int ck_batch = 0;  // checkpoint interval imposed by gjw_batch_fwi_gradient
   int tuned_L = 0, tuned_L1 = 0, tuned_L4 = 0;  // two-, one- and four-step kernels
+  int tuned_La2 = 0, tuned_La4 = 0;             // two- and four-step adjoint kernels
   int fuse_eff = 0;  // forward steps per launch in use (fuse, or the autotuner's choice)
   // model, padded (NX x NY, column-major, FP64)
   std::vector<double> pe, ps;
@@ -329,6 +330,9 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
     case GJW_OPT_TILE_L:
       if (v < 0 || v > 4096) return fail(GJW_ERR_ARG, "GJW_OPT_TILE_L out of range");
       s->tile_L = (int)v;
+      s->tuned_L = s->tuned_L1 = s->tuned_L4 = 0;
+      s->tuned_La2 = s->tuned_La4 = 0;
+      s->fuse_eff = 0;
       break;
     case GJW_OPT_MISFIT:
       if (v < GJW_MISFIT_L2 || v > GJW_MISFIT_ENV_LOG)
@@ -345,6 +349,7 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
         return fail(GJW_ERR_ARG, "GJW_OPT_FUSE must be 0 (autotuned), 1, 2 or 4");
       s->fuse = (int)v;
       s->tuned_L = 0;
+      s->tuned_La2 = s->tuned_La4 = 0;
       s->fuse_eff = 0;
       break;
     case GJW_OPT_KTIMING:
@@ -761,11 +766,11 @@ static AdjBufs adj_bufs(gjw_survey* s, int nsh) {
 static int kev_begin(gjw_survey* s, int which, int it, int* slot) {
   *slot = -1;
   if (s->ksample <= 0) return GJW_OK;
-  s->kl_n[which] += 1;
-  // two-step kernels advance `it` by 2: accept either residue so that one
-  // launch in every ksample steps is timed
-  const bool two = which >= 2;  // (four-step launches hit residue 0 or 1 once per 2 windows)
-  if (it % s->ksample > 1 || (it % s->ksample == 1 && !two)) return GJW_OK;
+  (void)it;
+  // time one launch of this kernel in every ksample steps it covers
+  const int steps = which >= 4 ? 4 : which >= 2 ? 2 : 1;
+  const long long every = std::max(1, s->ksample / steps);
+  if (s->kl_n[which]++ % every != 0) return GJW_OK;
   const size_t need = 2 * (s->kev_used.size() + 1);
   while (s->kev_pool.size() < need) {
     hipEvent_t e;
@@ -963,6 +968,66 @@ static int autotune_L(gjw_survey* s, int nsh) {
   return GJW_OK;
 }
 
+// Tile width of the two- and four-step adjoint kernels: time a few launches
+// of each candidate on zeroed adjoint state, with the residual injection at
+// the receivers (adj: nrec x nt residuals; only the timing matters).
+// Clobbers the adjoint, frame and partial buffers, so it runs before a
+// gradient sweep sets them up.
+static int autotune_adj(gjw_survey* s, int nsh, const float* adj) {
+  if (s->tile_L > 0) {
+    s->tuned_La2 = s->tuned_La4 = s->tile_L;
+    return GJW_OK;
+  }
+  if (s->tuned_La2 > 0 && s->tuned_La4 > 0) return GJW_OK;
+  if (s->gr.nt < 4) {  // no multi-step adjoint launch fits: keep the forward's widths
+    s->tuned_La2 = s->tuned_L;
+    s->tuned_La4 = s->tuned_L4;
+    return GJW_OK;
+  }
+  hipStream_t st = s->ctx->stream;
+  const long long FD = (long long)s->LD * s->gr.n;
+  TRY(s->adjf.ensure((size_t)10 * nsh * s->FS));
+  // frames laid out as in a gradient sweep: K+1 per shot (at least the 5 a
+  // four-step launch reads), so that every shot reads its own
+  const long long fper = std::max(5, s->ck_K + 1);
+  TRY(s->frames.ensure((size_t)(nsh * fper * FD)));
+  TRY(s->acc.ensure((size_t)2 * nsh * FD));
+  TRY(launch_fill(s->adjf.p, (long long)10 * nsh * s->FS, 0.f, st));
+  TRY(launch_fill(s->frames.p, nsh * fper * FD, 0.f, st));
+  DevGrid g = dev_grid(s);
+  DevShots sh = dev_shots(s, 0);
+  AdjBufs ab = adj_bufs(s, nsh);
+  const int ks = s->ksample;
+  s->ksample = 0;
+  const int reps = 5;
+  for (int NS : {2, 4}) {
+    const int cand[] = {12, 16, 24, 32, 48};
+    float best = 1e30f;
+    int bestL = 16;
+    for (int L : cand) {
+      int cur = 0;
+      for (int r = -1; r < reps; ++r) {
+        if (r == 0) HIPCHK(hipEventRecord(s->ctx->ev0, st));
+        // frames of S_0 .. S_NS (stride 0 between shots: every shot reads shot 0's)
+        TRY(adjN_launch(s, NS, g, sh, nsh, L, ab, cur, NS - 1, adj, s->frames.p + NS * FD,
+                        fper * FD, s->acc.p, FD, nullptr));
+        cur ^= 1;
+      }
+      HIPCHK(hipEventRecord(s->ctx->ev1, st));
+      HIPCHK(hipEventSynchronize(s->ctx->ev1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
+      if (ms < best) {
+        best = ms;
+        bestL = L;
+      }
+    }
+    (NS == 2 ? s->tuned_La2 : s->tuned_La4) = bestL;
+  }
+  s->ksample = ks;
+  return GJW_OK;
+}
+
 static int ensure_fwd_state(gjw_survey* s, int nsh) {
   TRY(s->fld.ensure((size_t)10 * nsh * s->FS));
   return GJW_OK;
@@ -1077,7 +1142,9 @@ static int ck_restore(gjw_survey* s, FwdBufs& b, int cur, int xc, int c) {
 static int ck_interval(const gjw_survey* s) {
   if (s->ck_batch > 0) return s->ck_batch;  // common interval of a survey batch
   if (s->ck_every > 0) return s->ck_every;
-  return std::max(1, (int)std::ceil(std::sqrt((double)s->gr.nt)));
+  // about sqrt(nt) steps, rounded up to whole four-step launches
+  const int k = std::max(1, (int)std::ceil(std::sqrt((double)s->gr.nt)));
+  return std::min(s->gr.nt, (k + 3) / 4 * 4);
 }
 
 // --------------------------------------------------------------------------
@@ -1323,6 +1390,7 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
   const int ns = s->ns, nt = s->gr.nt, K = s->ck_K;
   const long long per = (long long)s->gr.m * s->gr.n, FD = (long long)s->LD * s->gr.n;
   hipStream_t st = s->ctx->stream;
+  TRY(autotune_adj(s, ns, adj));
   TRY(s->frames.ensure((size_t)ns * (K + 1) * FD));
   TRY(s->acc.ensure((size_t)2 * ns * FD));
   TRY(launch_fill(s->acc.p, 2LL * ns * FD, 0.f, st));
@@ -1350,8 +1418,8 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
       const float* F = s->frames.p + (long long)(it - k0) * FD;  // Ez(S_it)
       const int NS = (s->fuse_eff >= 4 && it - 3 >= k0) ? 4 : (s->fuse_eff >= 2 && it - 1 >= k0) ? 2 : 1;
       if (NS > 1) {
-        TRY(adjN_launch(s, NS, g, sh, ns, NS == 4 ? s->tuned_L4 : s->tuned_L, ab, acur, it, adj,
-                        F + FD, fstride, s->acc.p, FD, gw));
+        TRY(adjN_launch(s, NS, g, sh, ns, NS == 4 ? s->tuned_La4 : s->tuned_La2, ab, acur, it,
+                        adj, F + FD, fstride, s->acc.p, FD, gw));
         it -= NS;
       } else {
         TRY(adj_launch(s, g, sh, ns, s->tuned_L1, ab, acur, it, adj, F + FD, F, fstride,
@@ -1601,6 +1669,7 @@ static int fwi_batched_prepare(gjw_survey* s) {
   TRY(s->frames.ensure((size_t)ns * (s->ck_K + 1) * FD));
   TRY(s->acc.ensure((size_t)2 * ns * FD));
   TRY(s->adjf.ensure((size_t)10 * ns * s->FS));
+  TRY(autotune_adj(s, ns, s->adj.p));
   return GJW_OK;
 }
 
@@ -1679,7 +1748,8 @@ static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
           u(s->d_cptr.p), u(s->d_crow.p), u(s->d_cid.p), u(s->d_trs.p), u(s->d_trr.p),
           u(s->d_trc.p),  (uintptr_t)s->ns, (uintptr_t)s->ntr,
           (uintptr_t)s->gr.nt, (uintptr_t)s->tuned_L, (uintptr_t)s->tuned_L1,
-          (uintptr_t)s->tuned_L4, (uintptr_t)s->fuse_eff,
+          (uintptr_t)s->tuned_L4, (uintptr_t)s->fuse_eff, (uintptr_t)s->tuned_La2,
+          (uintptr_t)s->tuned_La4,
           (uintptr_t)s->ck_K, (uintptr_t)s->fuse, (uintptr_t)s->snap,
           (uintptr_t)s->rrow_min, (uintptr_t)s->rrow_max, (uintptr_t)s->pml_idx.size(),
           fbits(s->bx0), fbits(s->by0)};
@@ -1824,7 +1894,9 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
   const int rows = steps == 1 ? 64 : 64 - 2 * steps;
   for (int k = 0; k < a.n; ++k) {
     gjw_survey* s = b->sv[ids[k]];
-    const int L = steps == 4 ? s->tuned_L4 : steps == 2 ? s->tuned_L : s->tuned_L1;
+    const int L = steps == 4   ? (adj ? s->tuned_La4 : s->tuned_L4)
+                  : steps == 2 ? (adj ? s->tuned_La2 : s->tuned_L)
+                               : s->tuned_L1;
     a.blk0[k] = blk;
     a.sid[k] = ids[k];
     a.it[k] = it[ids[k]];
@@ -2100,9 +2172,10 @@ extern "C" int gjw_kernel_stats(gjw_survey* s, int32_t which, double* avg_ms, in
   return GJW_OK;
 }
 
-extern "C" int gjw_tile(gjw_survey* s, int32_t* L) {
-  if (!s || !L) return fail(GJW_ERR_ARG, "gjw_tile: NULL argument");
-  *L = s->tuned_L;
+extern "C" int gjw_tile(gjw_survey* s, int32_t which, int32_t* L) {
+  if (!s || !L || which < 0 || which > 5) return fail(GJW_ERR_ARG, "gjw_tile: bad argument");
+  const int t[6] = {s->tuned_L1, s->tuned_L1, s->tuned_L, s->tuned_La2, s->tuned_L4, s->tuned_La4};
+  *L = t[which];
   return GJW_OK;
 }
 

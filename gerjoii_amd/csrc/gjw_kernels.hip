@@ -22,14 +22,16 @@ namespace gjw {
 // Neighbour-lane reads through DPP wave shifts (a VALU modifier, no LDS
 // round trip): shfl_dn1 gives lane l the value of lane l+1 (wave_shl:1),
 // shfl_up1 the value of lane l-1 (wave_shr:1).  The lane without a source
-// (63, resp. 0) keeps its own value; callers overwrite it.
+// (63, resp. 0) reads 0 (bound_ctrl); callers overwrite it or treat it as
+// halo.  With a zero old value the compiler folds the shift into the
+// instruction that consumes it instead of issuing a separate move.
 __device__ __forceinline__ float shfl_dn1(float v) {
   const int x = __float_as_int(v);
-  return __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x130, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x130, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float shfl_up1(float v) {
   const int x = __float_as_int(v);
-  return __int_as_float(__builtin_amdgcn_update_dpp(x, x, 0x138, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, true));
 }
 
 // Raw buffer stores: a lane that has nothing to write passes an offset past
@@ -389,6 +391,15 @@ __device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned off_
                      __uint_as_float(v.w));
 }
 
+// Waves per SIMD the multi-step kernels are compiled for (the compiler keeps
+// their registers within that occupancy; the kernels are bandwidth-bound).
+#ifndef GJW_FWD_WAVES
+#define GJW_FWD_WAVES 4
+#endif
+#ifndef GJW_ADJ_WAVES
+#define GJW_ADJ_WAVES 4
+#endif
+
 // FRAMES: the recompute sweep stores Ez on the domain after both steps.
 //
 // Column handling.  Step-1 columns jb-1 .. je are computed (step 2 runs one
@@ -679,8 +690,11 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       b.c[u] = bload2(rCC, 2 * kk, on);
     }
   };
-  // ---- interior chunk, all rows inner: scalar H coefficients, (Ceze, Cezhy) loads
-  auto inner_compute = [&](int jj, const InnerBuf& b) {
+  // ---- interior chunk, all rows inner: scalar H coefficients, (Ceze, Cezhy) loads.
+  // SRC: the wave's rows hold the source row (injection and source recorder);
+  // the other waves carry neither.
+  auto inner_compute = [&](int jj, const InnerBuf& b, auto SRC) {
+    constexpr bool src = decltype(SRC)::value;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int c = jj + u;
@@ -696,13 +710,13 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
         const float hx1 = hx + bx0 * (en - ec);
         const float hyu = shfl_up1(hy1);
         float e1 = cf.x * ec + cf.y * ((hy1 - hyu) - (hx1 - hxp[l]));
-        e1 += (cl == sj && i == si) ? ws[l] : 0.f;
+        if (src) e1 += (cl == sj && i == si) ? ws[l] : 0.f;
         eo[l] = e1;
         hxo[l] = hx1;
         hyo[l] = hy1;
         zo[l] = 0.f;  // no split-field cells in these rows
         const bool ol = own && cl >= jb && cl < je;
-        bstore(e1, rJZ, (unsigned)(it + l), ol && i == si && cl == sj);
+        if (src) bstore(e1, rJZ, (unsigned)(it + l), ol && i == si && cl == sj);
         if (FRAMES)
           bstore(e1, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + i), ol && dom_row && dom_col(cl));
         if (l == NS - 1) {
@@ -803,17 +817,23 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     for (; j <= jlast && !chunkable(j, U); j += UC) univ_chunk(j);
     int jz = j;  // end of the interior run
     while (jz <= jlast && chunkable(jz, U)) jz += U;
-    InnerBuf A, B;
-    inner_load(j, A, j < jz);
-    while (j < jz) {
-      inner_load(j + U, B, j + U < jz);
-      inner_compute(j, A);
-      j += U;
-      if (j >= jz) break;
-      inner_load(j + U, A, j + U < jz);
-      inner_compute(j, B);
-      j += U;
-    }
+    auto run = [&](auto SRC) {
+      InnerBuf A, B;
+      inner_load(j, A, j < jz);
+      while (j < jz) {
+        inner_load(j + U, B, j + U < jz);
+        inner_compute(j, A, SRC);
+        j += U;
+        if (j >= jz) break;
+        inner_load(j + U, A, j + U < jz);
+        inner_compute(j, B, SRC);
+        j += U;
+      }
+    };
+    if (si >= r0 && si <= r0 + 63)
+      run(std::true_type{});
+    else
+      run(std::false_type{});
     for (; j <= jlast; j += UC) univ_chunk(j);
   } else {
     int j = j1;
@@ -824,7 +844,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
 }
 
 template <bool FRAMES>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, GJW_FWD_WAVES)
 k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
             const float* __restrict__ Hx0, const float* __restrict__ Hy0,
             const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ E2,
@@ -836,7 +856,7 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
 }
 
 template <bool FRAMES>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, GJW_FWD_WAVES)
 k_fwd_step4(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
             const float* __restrict__ Hx0, const float* __restrict__ Hy0,
             const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ E2,
@@ -848,7 +868,7 @@ k_fwd_step4(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
 }
 
 template <int NS, bool FRAMES>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, GJW_FWD_WAVES)
 k_fwd_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
   int k;
   TileId tid;
@@ -1313,18 +1333,22 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   // Hand level l's outputs to level l+1, keep the M history, store the last
   // level's fields, correlate.  mo, wyo, wxo, xo, yo, nyo, nxo, pxo: outputs
   // of each level this column; f[k]: frames S_{it+1-k} at column cz.
+  // SPLIT: the chunk may hold split-field cells (else X, Y are not stored);
+  // SRC: the wave's rows may hold the source (recorder of the source gradient).
   auto finish_column = [&](int c, const float* mo, const float* wyo, const float* wxo,
                            const float* xo, const float* yo, const float* nyo, const float* nxo,
-                           const float* pxo, const float* f, float2 t, bool split_ok) {
+                           const float* pxo, const float* f, float2 t, auto SPLIT, auto SRC) {
     const int cz = c - (NS - 1);
     const bool oz = own && cz >= jb && cz < je;
     const unsigned kz = (unsigned)(cz * LD + ic);
     bstore(mo[NS - 1], rM2, kz, oz);
     bstore(nxo[NS - 1], rNx2, kz, oz && cz < NY);
     bstore(nyo[NS - 1], rNy2, kz, oz && h_any);
-    const bool pz = split_ok && pml_cell(cz);
-    bstore(xo[NS - 1], rX2, kz, oz && pz);
-    bstore(yo[NS - 1], rY2, kz, oz && pz);
+    if (decltype(SPLIT)::value) {
+      const bool pz = pml_cell(cz);
+      bstore(xo[NS - 1], rX2, kz, oz && pz);
+      bstore(yo[NS - 1], rY2, kz, oz && pz);
+    }
     // each level's input M at column cz
     float mz[NS];
 #pragma unroll
@@ -1336,9 +1360,11 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
       t.y = t.y + mz[l] * (f[l] + f[l + 1]);
     }
     bstore2(t, rAC, 2 * (unsigned)((cz - g.dj0) * LD + ic), ac && inner_z && corr_on(cz));
-    const bool at_src = oz && i == si && cz == sj;
+    if (decltype(SRC)::value) {
+      const bool at_src = oz && i == si && cz == sj;
 #pragma unroll
-    for (int l = 0; l < NS; ++l) bstore(sc * mz[l], rGW, (unsigned)(it - l), at_src);
+      for (int l = 0; l < NS; ++l) bstore(sc * mz[l], rGW, (unsigned)(it - l), at_src);
+    }
     // histories, then level l+1 inputs from level l outputs
 #pragma unroll
     for (int l = 0; l < NS - 1; ++l) {
@@ -1432,7 +1458,8 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
         nxo[l] = nxv;
         pxo[l] = px;
       }
-      finish_column(c, mo, wyo, wxo, xo, yo, nyo, nxo, pxo, f_[u], t_[u], true);
+      finish_column(c, mo, wyo, wxo, xo, yo, nyo, nxo, pxo, f_[u], t_[u], std::true_type{},
+                    std::true_type{});
       me_in[0] = mL[u];
       wy_in[0] = wy0n;
       wx_in[0] = wx0n;
@@ -1462,7 +1489,7 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   };
 
   // ---- interior chunk, all rows inner, no receivers
-  auto inner_chunk = [&](int jj) {
+  auto inner_chunk = [&](int jj, auto SRC) {
     float mL[UA], nx_[UA], ny_[UA], f_[UA][NS + 1];
     float2 cL[UA], t_[UA];
 #pragma unroll
@@ -1497,7 +1524,7 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
         nxo[l] = nxv;
         pxo[l] = px;
       }
-      finish_column(c, mo, wyo, wxo, zo, zo, nyo, nxo, pxo, f_[u], t_[u], false);
+      finish_column(c, mo, wyo, wxo, zo, zo, nyo, nxo, pxo, f_[u], t_[u], std::false_type{}, SRC);
       me_in[0] = mL[u];
       wy_in[0] = wy0n;
       wx_in[0] = wx0n;
@@ -1520,7 +1547,7 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   };
 
   // ---- interior chunk, rows of every class (top / bottom PML rows, receivers)
-  auto class_chunk = [&](int jj) {
+  auto class_chunk = [&](int jj, auto SRC) {
     float mL[UA], xL[UA], yL[UA], cxeL[UA], cxhL[UA], cyhL[UA], nx_[UA], ny_[UA], hayA[UA],
         hbyA[UA], f_[UA][NS + 1];
     float2 cL[UA], t_[UA];
@@ -1572,7 +1599,7 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
         nxo[l] = nxv;
         pxo[l] = px;
       }
-      finish_column(c, mo, wyo, wxo, xo, yo, nyo, nxo, pxo, f_[u], t_[u], true);
+      finish_column(c, mo, wyo, wxo, xo, yo, nyo, nxo, pxo, f_[u], t_[u], std::true_type{}, SRC);
       me_in[0] = mL[u];
       wy_in[0] = wy0n;
       wx_in[0] = wx0n;
@@ -1603,14 +1630,14 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   for (; j <= jlast && !chunkable(j); j += UA) univ_chunk(j);
   for (; j <= jlast && chunkable(j); j += UA) {
     if (use_inner)
-      inner_chunk(j);
+      inner_chunk(j, std::true_type{});
     else
-      class_chunk(j);
+      class_chunk(j, std::true_type{});
   }
   for (; j <= jlast; j += UA) univ_chunk(j);
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, GJW_ADJ_WAVES)
 k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             const float* __restrict__ Nx0, const float* __restrict__ Ny0,
             const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ M2,
@@ -1622,7 +1649,7 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
                Y2, it, adj, Fa, fstride, acc, astride, gw);
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, GJW_ADJ_WAVES)
 k_adj_step4(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             const float* __restrict__ Nx0, const float* __restrict__ Ny0,
             const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ M2,
@@ -1635,7 +1662,7 @@ k_adj_step4(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
 }
 
 template <int NS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, GJW_ADJ_WAVES)
 k_adj_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
   int k;
   TileId tid;

@@ -292,9 +292,12 @@ class Survey:
                                           int(reset)), "gjw_kernel_stats")
         return ms.value, n.value, nl.value
 
-    def tile(self):
+    def tile(self, which=2):
+        """Autotuned columns per wavefront of step kernel `which` (numbered
+        as in kernel_stats: 0/1 one-step forward/adjoint, 2/3 two-step,
+        4/5 four-step)."""
         L = C.c_int32(0)
-        N.check(N.load().gjw_tile(self.h, C.byref(L)), "gjw_tile")
+        N.check(N.load().gjw_tile(self.h, int(which), C.byref(L)), "gjw_tile")
         return L.value
 
     def last_timing(self):

@@ -91,7 +91,8 @@ int gjw_set_shots(gjw_survey *s, int32_t ns, const int32_t *src, const float *wa
 #define GJW_OPT_CKPT_EVERY 2 /* checkpoint interval in steps; 0 = ceil(sqrt(nt))   */
 #define GJW_OPT_TILE_L 3     /* columns marched per wavefront; 0 = autotuned        */
 #define GJW_OPT_SNAPSHOT 4   /* GJW_SNAP_F32 (default) | GJW_SNAP_BF16 | GJW_SNAP_F16 */
-#define GJW_OPT_KTIMING 5    /* n > 0: time every n-th step kernel with HIP events  */
+#define GJW_OPT_KTIMING 5    /* n > 0: time one launch of each step kernel per n
+                                time steps it covers, with HIP events */
 #define GJW_OPT_FUSE 6       /* time steps fused per forward launch on the batched path:
                                 0 (default: 2 or 4, picked by the autotuner), 1, 2 or 4; the
                                 adjoint sweep fuses as many */
@@ -188,8 +189,11 @@ int gjw_last_timing(gjw_survey *s, double *ms, double *cell_updates);
 int gjw_kernel_stats(gjw_survey *s, int32_t which, double *avg_ms, int64_t *count,
                      int64_t *launches, int32_t reset);
 
-/* Columns per wavefront chosen by the autotuner (0 before the first forward). */
-int gjw_tile(gjw_survey *s, int32_t *L);
+/* Columns per wavefront chosen by the autotuner for the step kernel `which`
+ * (numbered as in gjw_kernel_stats; 0 before the first forward, and for the
+ * adjoint kernels before the first gradient).  Has no counterpart in the
+ * reference (measurement only). */
+int gjw_tile(gjw_survey *s, int32_t which, int32_t *L);
 
 #ifdef __cplusplus
 }
