@@ -26,6 +26,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
 BYTES_PER_CELL_UPDATE = 32  # Ez, Hx, Hy read+write + 2 coefficient reads (FP32)
+# step kernels in the numbering of gjw_kernel_stats / gjw_tile: (name, steps per launch)
+KERNELS = [("k_fwd_step", 1), ("k_adj_step", 1), ("k_fwd_step2", 2), ("k_adj_step2", 2),
+           ("k_fwd_step4", 4), ("k_adj_step4", 4), ("k_fwd_step8", 8)]
 
 
 def log(*a):
@@ -103,7 +106,7 @@ def main():
         sv.fwi_gradient(d_obs, g_eps, g_sig)
         log(f"warmup {w + 1}/{args.warmup} done")
     sv.set_option(N.GJW_OPT_KTIMING, 16)
-    for kk in range(6):
+    for kk in range(len(KERNELS)):
         sv.kernel_stats(kk, reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -114,41 +117,25 @@ def main():
     torch.cuda.synchronize()
     dt_s = time.perf_counter() - t0
     sv.set_option(N.GJW_OPT_KTIMING, 0)
-    fwd1_ms, fwd1_n, fwd1_l = sv.kernel_stats(0)
-    adj_ms, adj_n, adj_l = sv.kernel_stats(1)
-    fwd2_ms, fwd2_n, fwd2_l = sv.kernel_stats(2)
-    adj2_ms, adj2_n, adj2_l = sv.kernel_stats(3)
-    fwd4_ms, fwd4_n, fwd4_l = sv.kernel_stats(4)
-    adj4_ms, adj4_n, adj4_l = sv.kernel_stats(5)
+    kstats = [sv.kernel_stats(kk) for kk in range(len(KERNELS))]
     ms_step = dt_s * 1e3 / args.steps
     shot_grads = ns * args.steps / dt_s
     cell_updates = 3.0 * cells * g.nt * ns * args.steps  # forward, recompute, adjoint
     gcells = cell_updates / dt_s / 1e9
     ge_b = g_eps.cpu().numpy().T.copy()
-    tile_L = {k: sv.tile(w) for k, w in (("fwd2", 2), ("fwd4", 4), ("adj2", 3), ("adj4", 5))}
+    tile_L = {name: sv.tile(w) for w, (name, _) in enumerate(KERNELS) if w >= 2}
     sv.close()
     del sv
     torch.cuda.empty_cache()
 
     # per-kernel roofline: algorithmic bytes per launch / live average duration
     bytes_launch = BYTES_PER_CELL_UPDATE * cells * ns
-    # launches per gradient: every sweep runs 4 (k_*_step4) or 2 (k_*_step2)
-    # steps per launch as the autotuner chose; the other kernels only take
-    # segment tails
-    kern = {
-        "k_fwd_step2": {"avg_ms": fwd2_ms, "samples": fwd2_n, "steps_per_launch": 2,
-                        "launches_per_step": fwd2_l / args.steps},
-        "k_fwd_step": {"avg_ms": fwd1_ms, "samples": fwd1_n, "steps_per_launch": 1,
-                       "launches_per_step": fwd1_l / args.steps},
-        "k_adj_step": {"avg_ms": adj_ms, "samples": adj_n, "steps_per_launch": 1,
-                       "launches_per_step": adj_l / args.steps},
-        "k_adj_step2": {"avg_ms": adj2_ms, "samples": adj2_n, "steps_per_launch": 2,
-                        "launches_per_step": adj2_l / args.steps},
-        "k_fwd_step4": {"avg_ms": fwd4_ms, "samples": fwd4_n, "steps_per_launch": 4,
-                        "launches_per_step": fwd4_l / args.steps},
-        "k_adj_step4": {"avg_ms": adj4_ms, "samples": adj4_n, "steps_per_launch": 4,
-                        "launches_per_step": adj4_l / args.steps},
-    }
+    # launches per gradient: the sweeps run the deepest fused kernels the
+    # autotuner chose; the shallower ones only take segment tails
+    kern = {}
+    for (name, spl), (avg_ms, samples, launches) in zip(KERNELS, kstats):
+        kern[name] = {"avg_ms": avg_ms, "samples": samples, "steps_per_launch": spl,
+                      "launches_per_step": launches / args.steps}
     for k, v in kern.items():
         spl = v["steps_per_launch"]
         v["achieved_GBs"] = (spl * bytes_launch / (v["avg_ms"] * 1e-3) / 1e9) if v["avg_ms"] > 0 else None

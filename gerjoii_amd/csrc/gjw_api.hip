@@ -45,10 +45,14 @@ template <bool FRAMES>
 __global__ void k_fwd_step2(GJW_FWDN_ARGS);
 template <bool FRAMES>
 __global__ void k_fwd_step4(GJW_FWDN_ARGS);
+template <bool FRAMES>
+__global__ void k_fwd_step8(GJW_FWDN_ARGS);
 extern template __global__ void k_fwd_step2<false>(GJW_FWDN_ARGS);
 extern template __global__ void k_fwd_step2<true>(GJW_FWDN_ARGS);
 extern template __global__ void k_fwd_step4<false>(GJW_FWDN_ARGS);
 extern template __global__ void k_fwd_step4<true>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step8<false>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step8<true>(GJW_FWDN_ARGS);
 #undef GJW_FWDN_ARGS
 __global__ void k_fwd_step_multi(const DevSurvey*, MultiArgs);
 template <int NS, bool FRAMES>
@@ -165,6 +169,9 @@ struct GraphCache {
   }
 };
 
+// step kernels timed by GJW_OPT_KTIMING (numbered as in gjw_kernel_stats)
+constexpr int kKernKinds = 7;
+
 struct gjw_survey {
   gjw_ctx* ctx = nullptr;
   gjw_grid gr{};
@@ -174,8 +181,9 @@ struct gjw_survey {
   // options
   int path = GJW_PATH_BATCHED, ck_every = 0, tile_L = 0, snap = GJW_SNAP_F32, fuse = 0;
   int ck_batch = 0;  // checkpoint interval imposed by gjw_batch_fwi_gradient
-  int tuned_L = 0, tuned_L1 = 0, tuned_L4 = 0;  // two-, one- and four-step kernels
+  int tuned_L = 0, tuned_L1 = 0, tuned_L4 = 0, tuned_L8 = 0;  // 2-, 1-, 4-, 8-step forward
   int tuned_La2 = 0, tuned_La4 = 0;             // two- and four-step adjoint kernels
+  std::vector<hipEvent_t> ex_ev;                // adjoint tile-width trials of the first sweep
   int fuse_eff = 0;  // forward steps per launch in use (fuse, or the autotuner's choice)
   // model, padded (NX x NY, column-major, FP64)
   std::vector<double> pe, ps;
@@ -221,10 +229,10 @@ struct gjw_survey {
   GraphCache g_all, g_fwd, g_adj;  // whole gradient (L2); forward / adjoint sweeps
   std::vector<hipEvent_t> kev_pool;
   std::vector<std::pair<int, int>> kev_used;  // (kernel id, index of start event)
-  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2, k_fwd_step4, k_adj_step4
-  double kt_ms[6] = {0, 0, 0, 0, 0, 0};
-  long long kt_n[6] = {0, 0, 0, 0, 0, 0};
-  long long kl_n[6] = {0, 0, 0, 0, 0, 0};  // all launches while timing is on
+  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2, k_fwd_step4, k_adj_step4, k_fwd_step8
+  double kt_ms[kKernKinds] = {};
+  long long kt_n[kKernKinds] = {};
+  long long kl_n[kKernKinds] = {};  // all launches while timing is on
 };
 
 static inline long long cdiv(long long a, long long b) { return (a + b - 1) / b; }
@@ -297,6 +305,7 @@ extern "C" int gjw_survey_destroy(gjw_survey* s) {
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
   for (auto e : s->kev_pool) (void)hipEventDestroy(e);
+  for (auto e : s->ex_ev) (void)hipEventDestroy(e);
   if (s->fft_r2c) (void)hipfftDestroy(s->fft_r2c);
   if (s->fft_c2c) (void)hipfftDestroy(s->fft_c2c);
   s->fhalf.release(); s->fana.release(); s->fin.release(); s->fov.release(); s->fP.release();
@@ -330,7 +339,7 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
     case GJW_OPT_TILE_L:
       if (v < 0 || v > 4096) return fail(GJW_ERR_ARG, "GJW_OPT_TILE_L out of range");
       s->tile_L = (int)v;
-      s->tuned_L = s->tuned_L1 = s->tuned_L4 = 0;
+      s->tuned_L = s->tuned_L1 = s->tuned_L4 = s->tuned_L8 = 0;
       s->tuned_La2 = s->tuned_La4 = 0;
       s->fuse_eff = 0;
       break;
@@ -345,8 +354,8 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       s->graph_calls = 0;
       return GJW_OK;
     case GJW_OPT_FUSE:
-      if (v != 0 && v != 1 && v != 2 && v != 4)
-        return fail(GJW_ERR_ARG, "GJW_OPT_FUSE must be 0 (autotuned), 1, 2 or 4");
+      if (v != 0 && v != 1 && v != 2 && v != 4 && v != 8)
+        return fail(GJW_ERR_ARG, "GJW_OPT_FUSE must be 0 (autotuned), 1, 2, 4 or 8");
       s->fuse = (int)v;
       s->tuned_L = 0;
       s->tuned_La2 = s->tuned_La4 = 0;
@@ -768,7 +777,7 @@ static int kev_begin(gjw_survey* s, int which, int it, int* slot) {
   if (s->ksample <= 0) return GJW_OK;
   (void)it;
   // time one launch of this kernel in every ksample steps it covers
-  const int steps = which >= 4 ? 4 : which >= 2 ? 2 : 1;
+  const int steps = which == 6 ? 8 : which >= 4 ? 4 : which >= 2 ? 2 : 1;
   const long long every = std::max(1, s->ksample / steps);
   if (s->kl_n[which]++ % every != 0) return GJW_OK;
   const size_t need = 2 * (s->kev_used.size() + 1);
@@ -826,14 +835,19 @@ static int fwdN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& 
                        FwdBufs& b, int cur, int xc, int it, float* tr, float* fr1,
                        long long fstride) {
   int slot;
-  TRY(kev_begin(s, NS == 4 ? 4 : 2, it, &slot));
+  TRY(kev_begin(s, NS == 8 ? 6 : NS == 4 ? 4 : 2, it, &slot));
   const dim3 grid = stepN_grid(s, NS, L, nsh), block(64, 4);
   hipStream_t st = s->ctx->stream;
 #define GJW_FWDN_CALL(K)                                                                     \
   hipLaunchKernelGGL(K, grid, block, 0, st, g, sh, L, b.E[cur], b.Hx[cur], b.Hy[cur], b.X[xc], \
                      b.Y[xc], b.E[cur ^ 1], b.Hx[cur ^ 1], b.Hy[cur ^ 1], b.X[xc ^ 1],        \
                      b.Y[xc ^ 1], it, tr, fr1, fstride)
-  if (NS == 4) {
+  if (NS == 8) {
+    if (fr1)
+      GJW_FWDN_CALL(k_fwd_step8<true>);
+    else
+      GJW_FWDN_CALL(k_fwd_step8<false>);
+  } else if (NS == 4) {
     if (fr1)
       GJW_FWDN_CALL(k_fwd_step4<true>);
     else
@@ -849,8 +863,8 @@ static int fwdN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& 
   return kev_end(s, slot);
 }
 
-// Advance the batched forward state from step it by up to `limit` steps (4,
-// 2 or 1 per launch, at most the depth in use); frame (optional) receives
+// Advance the batched forward state from step it by up to `limit` steps (8,
+// 4, 2 or 1 per launch, at most the depth in use); frame (optional) receives
 // Ez(S_{it+1}), Ez(S_{it+2}), .. at frame, frame + fstep, ..  Returns the
 // steps taken.
 static int fwd_advance(gjw_survey* s, const DevGrid& g, const DevShots& sh, int nsh, FwdBufs& b,
@@ -858,10 +872,13 @@ static int fwd_advance(gjw_survey* s, const DevGrid& g, const DevShots& sh, int 
                        long long fstep, long long fstride, int* taken) {
   (void)fstep;  // frame slots are LD * n apart, the kernels know it
   const int fz = s->fuse_eff > 0 ? s->fuse_eff : 2;
-  const int NS = (fz >= 4 && limit >= 4) ? 4 : (fz >= 2 && limit >= 2) ? 2 : 1;
+  const int NS = (fz >= 8 && limit >= 8)   ? 8
+                 : (fz >= 4 && limit >= 4) ? 4
+                 : (fz >= 2 && limit >= 2) ? 2
+                                           : 1;
   if (NS > 1) {
-    TRY(fwdN_launch(s, NS, g, sh, nsh, NS == 4 ? s->tuned_L4 : s->tuned_L, b, cur, xc, it, tr,
-                    frame, fstride));
+    const int L = NS == 8 ? s->tuned_L8 : NS == 4 ? s->tuned_L4 : s->tuned_L;
+    TRY(fwdN_launch(s, NS, g, sh, nsh, L, b, cur, xc, it, tr, frame, fstride));
     xc ^= 1;
   } else {
     TRY(fwd_launch(s, g, sh, nsh, s->tuned_L1, b, cur, it, tr, frame, fstride, xc));
@@ -915,17 +932,19 @@ static int check_ready(gjw_survey* s, const char* who) {
   return GJW_OK;
 }
 
-// Tile width L (columns per wavefront) of the one-, two- and four-step
-// forward kernels, and the forward fusion depth: time a few launches of each
-// candidate on zeroed batch state and keep the fastest; with GJW_OPT_FUSE = 0
-// the depth is 4 if one four-step launch beats two two-step launches.
+// Tile width L (columns per wavefront) of the one-, two-, four- and
+// eight-step forward kernels, and the forward fusion depth: time a few
+// launches of each candidate on zeroed batch state and keep the fastest;
+// with GJW_OPT_FUSE = 0 the depth is the deepest whose launch beats two
+// launches of half its depth.
 static int autotune_L(gjw_survey* s, int nsh) {
   if (s->tile_L > 0) {
-    s->tuned_L = s->tuned_L1 = s->tuned_L4 = s->tile_L;
+    s->tuned_L = s->tuned_L1 = s->tuned_L4 = s->tuned_L8 = s->tile_L;
     s->fuse_eff = s->fuse > 0 ? s->fuse : 2;
     return GJW_OK;
   }
-  if (s->tuned_L > 0 && s->tuned_L1 > 0 && s->tuned_L4 > 0 && s->fuse_eff > 0) return GJW_OK;
+  if (s->tuned_L > 0 && s->tuned_L1 > 0 && s->tuned_L4 > 0 && s->tuned_L8 > 0 && s->fuse_eff > 0)
+    return GJW_OK;
   hipStream_t st = s->ctx->stream;
   DevGrid g = dev_grid(s);
   DevShots sh = dev_shots(s, 0);
@@ -933,10 +952,11 @@ static int autotune_L(gjw_survey* s, int nsh) {
   const int ks = s->ksample;
   s->ksample = 0;
   const int reps = 4;
-  float best_of[5] = {0, 0, 0, 0, 0};
-  for (int NS : {1, 2, 4}) {
-    const int c1[] = {8, 16, 32, 64}, c2[] = {12, 16, 24, 32}, c4[] = {16, 24, 32, 48};
-    const int* cand = NS == 1 ? c1 : NS == 2 ? c2 : c4;
+  float best_of[9] = {};
+  for (int NS : {1, 2, 4, 8}) {
+    const int c1[] = {8, 16, 32, 64}, c2[] = {12, 16, 24, 32}, c4[] = {16, 24, 32, 48},
+              c8[] = {16, 24, 32, 48};
+    const int* cand = NS == 1 ? c1 : NS == 2 ? c2 : NS == 4 ? c4 : c8;
     float best = 1e30f;
     int bestL = 16;
     for (int ci = 0; ci < 4; ++ci) {
@@ -960,70 +980,12 @@ static int autotune_L(gjw_survey* s, int nsh) {
         bestL = L;
       }
     }
-    (NS == 1 ? s->tuned_L1 : NS == 2 ? s->tuned_L : s->tuned_L4) = bestL;
+    (NS == 1 ? s->tuned_L1 : NS == 2 ? s->tuned_L : NS == 4 ? s->tuned_L4 : s->tuned_L8) = bestL;
     best_of[NS] = best;
   }
-  s->fuse_eff = s->fuse > 0 ? s->fuse : (best_of[4] < 2.f * best_of[2] ? 4 : 2);
-  s->ksample = ks;
-  return GJW_OK;
-}
-
-// Tile width of the two- and four-step adjoint kernels: time a few launches
-// of each candidate on zeroed adjoint state, with the residual injection at
-// the receivers (adj: nrec x nt residuals; only the timing matters).
-// Clobbers the adjoint, frame and partial buffers, so it runs before a
-// gradient sweep sets them up.
-static int autotune_adj(gjw_survey* s, int nsh, const float* adj) {
-  if (s->tile_L > 0) {
-    s->tuned_La2 = s->tuned_La4 = s->tile_L;
-    return GJW_OK;
-  }
-  if (s->tuned_La2 > 0 && s->tuned_La4 > 0) return GJW_OK;
-  if (s->gr.nt < 4) {  // no multi-step adjoint launch fits: keep the forward's widths
-    s->tuned_La2 = s->tuned_L;
-    s->tuned_La4 = s->tuned_L4;
-    return GJW_OK;
-  }
-  hipStream_t st = s->ctx->stream;
-  const long long FD = (long long)s->LD * s->gr.n;
-  TRY(s->adjf.ensure((size_t)10 * nsh * s->FS));
-  // frames laid out as in a gradient sweep: K+1 per shot (at least the 5 a
-  // four-step launch reads), so that every shot reads its own
-  const long long fper = std::max(5, s->ck_K + 1);
-  TRY(s->frames.ensure((size_t)(nsh * fper * FD)));
-  TRY(s->acc.ensure((size_t)2 * nsh * FD));
-  TRY(launch_fill(s->adjf.p, (long long)10 * nsh * s->FS, 0.f, st));
-  TRY(launch_fill(s->frames.p, nsh * fper * FD, 0.f, st));
-  DevGrid g = dev_grid(s);
-  DevShots sh = dev_shots(s, 0);
-  AdjBufs ab = adj_bufs(s, nsh);
-  const int ks = s->ksample;
-  s->ksample = 0;
-  const int reps = 5;
-  for (int NS : {2, 4}) {
-    const int cand[] = {12, 16, 24, 32, 48};
-    float best = 1e30f;
-    int bestL = 16;
-    for (int L : cand) {
-      int cur = 0;
-      for (int r = -1; r < reps; ++r) {
-        if (r == 0) HIPCHK(hipEventRecord(s->ctx->ev0, st));
-        // frames of S_0 .. S_NS (stride 0 between shots: every shot reads shot 0's)
-        TRY(adjN_launch(s, NS, g, sh, nsh, L, ab, cur, NS - 1, adj, s->frames.p + NS * FD,
-                        fper * FD, s->acc.p, FD, nullptr));
-        cur ^= 1;
-      }
-      HIPCHK(hipEventRecord(s->ctx->ev1, st));
-      HIPCHK(hipEventSynchronize(s->ctx->ev1));
-      float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
-      if (ms < best) {
-        best = ms;
-        bestL = L;
-      }
-    }
-    (NS == 2 ? s->tuned_La2 : s->tuned_La4) = bestL;
-  }
+  int fz = 2;
+  if (best_of[4] < 2.f * best_of[2]) fz = best_of[8] < 2.f * best_of[4] ? 8 : 4;
+  s->fuse_eff = s->fuse > 0 ? s->fuse : fz;
   s->ksample = ks;
   return GJW_OK;
 }
@@ -1142,9 +1104,10 @@ static int ck_restore(gjw_survey* s, FwdBufs& b, int cur, int xc, int c) {
 static int ck_interval(const gjw_survey* s) {
   if (s->ck_batch > 0) return s->ck_batch;  // common interval of a survey batch
   if (s->ck_every > 0) return s->ck_every;
-  // about sqrt(nt) steps, rounded up to whole four-step launches
+  // about sqrt(nt) steps, rounded up to whole launches of the forward depth
+  const int q = s->fuse_eff >= 8 ? 8 : 4;
   const int k = std::max(1, (int)std::ceil(std::sqrt((double)s->gr.nt)));
-  return std::min(s->gr.nt, (k + 3) / 4 * 4);
+  return std::min(s->gr.nt, (k + q - 1) / q * q);
 }
 
 // --------------------------------------------------------------------------
@@ -1386,11 +1349,15 @@ static int adj_init(gjw_survey* s, int nsh, int q0, const float* adj) {
   return GJW_OK;
 }
 
+// candidate tile widths of the four-step adjoint, tried in turn by the first
+// gradient sweep (kAdjRounds launches each, the first round a warm-up)
+constexpr int kAdjCand[] = {12, 16, 24, 32, 48};
+constexpr int kAdjNCand = 5, kAdjRounds = 3;
+
 static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
   const int ns = s->ns, nt = s->gr.nt, K = s->ck_K;
   const long long per = (long long)s->gr.m * s->gr.n, FD = (long long)s->LD * s->gr.n;
   hipStream_t st = s->ctx->stream;
-  TRY(autotune_adj(s, ns, adj));
   TRY(s->frames.ensure((size_t)ns * (K + 1) * FD));
   TRY(s->acc.ensure((size_t)2 * ns * FD));
   TRY(launch_fill(s->acc.p, 2LL * ns * FD, 0.f, st));
@@ -1401,6 +1368,22 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
   AdjBufs ab = adj_bufs(s, ns);
   const long long fstride = (long long)(K + 1) * FD;  // frames per shot
   int acur = 0;
+  // Adjoint tile width, tuned in context: the first (uncaptured) sweep gives
+  // its first four-step launches the candidate widths in turn, timed with
+  // events; later sweeps use the fastest.
+  if (s->tile_L > 0) s->tuned_La2 = s->tuned_La4 = s->tile_L;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  HIPCHK(hipStreamIsCapturing(st, &cap));
+  const bool explore = s->tuned_La4 <= 0 && cap == hipStreamCaptureStatusNone;
+  const int nx = explore ? kAdjRounds * kAdjNCand : 0;
+  int ex = 0;  // trial launches issued
+  if (explore)
+    while ((int)s->ex_ev.size() < 2 * nx) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      s->ex_ev.push_back(e);
+    }
+  const int La_fallback = s->tuned_L4 > 0 ? s->tuned_L4 : 16;
   for (int c = s->ck_n - 1; c >= 0; --c) {
     const int k0 = c * K, k1 = std::min(k0 + K, nt);
     int cur = 0, xc = 0;
@@ -1418,8 +1401,14 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
       const float* F = s->frames.p + (long long)(it - k0) * FD;  // Ez(S_it)
       const int NS = (s->fuse_eff >= 4 && it - 3 >= k0) ? 4 : (s->fuse_eff >= 2 && it - 1 >= k0) ? 2 : 1;
       if (NS > 1) {
-        TRY(adjN_launch(s, NS, g, sh, ns, NS == 4 ? s->tuned_La4 : s->tuned_La2, ab, acur, it,
-                        adj, F + FD, fstride, s->acc.p, FD, gw));
+        int L = NS == 4 ? s->tuned_La4 : s->tuned_La2;
+        const bool trial = NS == 4 && ex < nx;
+        if (trial) L = kAdjCand[ex % kAdjNCand];
+        if (L <= 0) L = La_fallback;
+        if (trial) HIPCHK(hipEventRecord(s->ex_ev[2 * ex], st));
+        TRY(adjN_launch(s, NS, g, sh, ns, L, ab, acur, it, adj, F + FD, fstride, s->acc.p, FD,
+                        gw));
+        if (trial) HIPCHK(hipEventRecord(s->ex_ev[2 * ex++ + 1], st));
         it -= NS;
       } else {
         TRY(adj_launch(s, g, sh, ns, s->tuned_L1, ab, acur, it, adj, F + FD, F, fstride,
@@ -1431,6 +1420,29 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
     hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
                        s->gr.n, s->LD, s->di0, ns, s->acc.p, FD, s->fe.p, s->fs.p, s->ge.p, s->gsg.p, 1.0);
     HIPCHK(hipGetLastError());
+  }
+  if (explore) {
+    // fastest candidate over the trial rounds after the first (warm-up) one
+    float best = 1e30f;
+    int bestL = La_fallback;
+    if (ex >= 2 * kAdjNCand) {
+      HIPCHK(hipEventSynchronize(s->ex_ev[2 * ex - 1]));
+      for (int c = 0; c < kAdjNCand; ++c) {
+        float tot = 0;
+        int n = 0;
+        for (int k = kAdjNCand + c; k < ex; k += kAdjNCand) {
+          float ms = 0;
+          HIPCHK(hipEventElapsedTime(&ms, s->ex_ev[2 * k], s->ex_ev[2 * k + 1]));
+          tot += ms;
+          ++n;
+        }
+        if (n > 0 && tot / n < best) {
+          best = tot / n;
+          bestL = kAdjCand[c];
+        }
+      }
+    }
+    s->tuned_La2 = s->tuned_La4 = bestL;
   }
   return GJW_OK;
 }
@@ -1669,7 +1681,6 @@ static int fwi_batched_prepare(gjw_survey* s) {
   TRY(s->frames.ensure((size_t)ns * (s->ck_K + 1) * FD));
   TRY(s->acc.ensure((size_t)2 * ns * FD));
   TRY(s->adjf.ensure((size_t)10 * ns * s->FS));
-  TRY(autotune_adj(s, ns, s->adj.p));
   return GJW_OK;
 }
 
@@ -1748,7 +1759,8 @@ static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
           u(s->d_cptr.p), u(s->d_crow.p), u(s->d_cid.p), u(s->d_trs.p), u(s->d_trr.p),
           u(s->d_trc.p),  (uintptr_t)s->ns, (uintptr_t)s->ntr,
           (uintptr_t)s->gr.nt, (uintptr_t)s->tuned_L, (uintptr_t)s->tuned_L1,
-          (uintptr_t)s->tuned_L4, (uintptr_t)s->fuse_eff, (uintptr_t)s->tuned_La2,
+          (uintptr_t)s->tuned_L4, (uintptr_t)s->tuned_L8, (uintptr_t)s->fuse_eff,
+          (uintptr_t)s->tuned_La2,
           (uintptr_t)s->tuned_La4,
           (uintptr_t)s->ck_K, (uintptr_t)s->fuse, (uintptr_t)s->snap,
           (uintptr_t)s->rrow_min, (uintptr_t)s->rrow_max, (uintptr_t)s->pml_idx.size(),
@@ -1894,8 +1906,11 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
   const int rows = steps == 1 ? 64 : 64 - 2 * steps;
   for (int k = 0; k < a.n; ++k) {
     gjw_survey* s = b->sv[ids[k]];
-    const int L = steps == 4   ? (adj ? s->tuned_La4 : s->tuned_L4)
-                  : steps == 2 ? (adj ? s->tuned_La2 : s->tuned_L)
+    // (the adjoint widths are tuned by single-survey gradient sweeps)
+    const int La4 = s->tuned_La4 > 0 ? s->tuned_La4 : s->tuned_L4;
+    const int La2 = s->tuned_La2 > 0 ? s->tuned_La2 : s->tuned_L;
+    const int L = steps == 4   ? (adj ? La4 : s->tuned_L4)
+                  : steps == 2 ? (adj ? La2 : s->tuned_L)
                                : s->tuned_L1;
     a.blk0[k] = blk;
     a.sid[k] = ids[k];
@@ -2159,7 +2174,8 @@ extern "C" int gjw_batch_last_timing(gjw_batch* b, double* ms, double* cells) {
 
 extern "C" int gjw_kernel_stats(gjw_survey* s, int32_t which, double* avg_ms, int64_t* count,
                                 int64_t* launches, int32_t reset) {
-  if (!s || which < 0 || which > 5) return fail(GJW_ERR_ARG, "gjw_kernel_stats: bad argument");
+  if (!s || which < 0 || which >= kKernKinds)
+    return fail(GJW_ERR_ARG, "gjw_kernel_stats: bad argument");
   TRY(kev_resolve(s));
   if (avg_ms) *avg_ms = s->kt_n[which] ? s->kt_ms[which] / s->kt_n[which] : 0.0;
   if (count) *count = s->kt_n[which];
@@ -2173,8 +2189,9 @@ extern "C" int gjw_kernel_stats(gjw_survey* s, int32_t which, double* avg_ms, in
 }
 
 extern "C" int gjw_tile(gjw_survey* s, int32_t which, int32_t* L) {
-  if (!s || !L || which < 0 || which > 5) return fail(GJW_ERR_ARG, "gjw_tile: bad argument");
-  const int t[6] = {s->tuned_L1, s->tuned_L1, s->tuned_L, s->tuned_La2, s->tuned_L4, s->tuned_La4};
+  if (!s || !L || which < 0 || which >= kKernKinds) return fail(GJW_ERR_ARG, "gjw_tile: bad argument");
+  const int t[kKernKinds] = {s->tuned_L1, s->tuned_L1, s->tuned_L,  s->tuned_La2,
+                             s->tuned_L4, s->tuned_La4, s->tuned_L8};
   *L = t[which];
   return GJW_OK;
 }

@@ -399,6 +399,9 @@ __device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned off_
 #ifndef GJW_ADJ_WAVES
 #define GJW_ADJ_WAVES 4
 #endif
+#ifndef GJW_FWD8_WAVES
+#define GJW_FWD8_WAVES 3
+#endif
 
 // FRAMES: the recompute sweep stores Ez on the domain after both steps.
 //
@@ -436,11 +439,8 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
                                           float* __restrict__ tr, float* __restrict__ frame1,
                                           long long fstride) {
   constexpr int R = 64 - 2 * NS;
-#ifndef GJW_FWD_UC4
-  constexpr int UC = NS == 4 ? 2 : U;  // columns per strip / row-class chunk
-#else
-  constexpr int UC = U;
-#endif
+  constexpr int UC = NS >= 8 ? 1 : NS == 4 ? 2 : U;  // columns per strip / row-class chunk
+  constexpr int UI = NS >= 8 ? 2 : U;                // columns per interior chunk
   const int lane = threadIdx.x;
   const int NX = g.NX, NY = g.NY, P = g.P;
   const long long LD = g.LD;
@@ -677,12 +677,12 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   // out of bounds, which returns 0 without a memory access) so the compiler
   // can count outstanding requests exactly.
   struct InnerBuf {
-    float en[U], hx[U], hy[U];
-    float2 c[U];
+    float en[UI], hx[UI], hy[UI];
+    float2 c[UI];
   };
   auto inner_load = [&](int jj, InnerBuf& b, bool on) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < UI; ++u) {
       const unsigned kk = (unsigned)((jj + u) * LD + i);
       b.en[u] = bload(rE0, kk + (unsigned)LD, on);
       b.hx[u] = bload(rHx0, kk, on);
@@ -696,7 +696,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   auto inner_compute = [&](int jj, const InnerBuf& b, auto SRC) {
     constexpr bool src = decltype(SRC)::value;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
+    for (int u = 0; u < UI; ++u) {
       const int c = jj + u;
       float eo[NS], hxo[NS], hyo[NS], zo[NS];
 #pragma unroll
@@ -814,20 +814,20 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   const int jlast = je + NS - 2;
   if (inner_rows && !has_rec) {
     int j = j1;
-    for (; j <= jlast && !chunkable(j, U); j += UC) univ_chunk(j);
+    for (; j <= jlast && !chunkable(j, UI); j += UC) univ_chunk(j);
     int jz = j;  // end of the interior run
-    while (jz <= jlast && chunkable(jz, U)) jz += U;
+    while (jz <= jlast && chunkable(jz, UI)) jz += UI;
     auto run = [&](auto SRC) {
       InnerBuf A, B;
       inner_load(j, A, j < jz);
       while (j < jz) {
-        inner_load(j + U, B, j + U < jz);
+        inner_load(j + UI, B, j + UI < jz);
         inner_compute(j, A, SRC);
-        j += U;
+        j += UI;
         if (j >= jz) break;
-        inner_load(j + U, A, j + U < jz);
+        inner_load(j + UI, A, j + UI < jz);
         inner_compute(j, B, SRC);
-        j += U;
+        j += UI;
       }
     };
     if (si >= r0 && si <= r0 + 63)
@@ -867,6 +867,18 @@ k_fwd_step4(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
                        Hy2, X2, Y2, it, tr, frame1, fstride);
 }
 
+template <bool FRAMES>
+__global__ void __launch_bounds__(256, GJW_FWD8_WAVES)
+k_fwd_step8(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
+            const float* __restrict__ Hx0, const float* __restrict__ Hy0,
+            const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ E2,
+            float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
+            float* __restrict__ Y2, int it, float* __restrict__ tr, float* __restrict__ frame1,
+            long long fstride) {
+  fwdN_body<8, FRAMES>(g, s, L, vtile_of_block(g.NX, g.NY, L, 48), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
+                       Hy2, X2, Y2, it, tr, frame1, fstride);
+}
+
 template <int NS, bool FRAMES>
 __global__ void __launch_bounds__(256, GJW_FWD_WAVES)
 k_fwd_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
@@ -891,6 +903,8 @@ template __global__ void k_fwd_step2<false>(GJW_FWDN_ARGS);
 template __global__ void k_fwd_step2<true>(GJW_FWDN_ARGS);
 template __global__ void k_fwd_step4<false>(GJW_FWDN_ARGS);
 template __global__ void k_fwd_step4<true>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step8<false>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step8<true>(GJW_FWDN_ARGS);
 #undef GJW_FWDN_ARGS
 template __global__ void k_fwd_stepN_multi<2, false>(const DevSurvey*, MultiArgs);
 template __global__ void k_fwd_stepN_multi<2, true>(const DevSurvey*, MultiArgs);

@@ -286,7 +286,8 @@ class Survey:
     def kernel_stats(self, which, reset=False):
         """(average ms, samples, launches) of the one-step forward (0), one-step
         adjoint (1), two-step forward (2), two-step adjoint (3), four-step
-        forward (4) and four-step adjoint (5) kernels while GJW_OPT_KTIMING is on."""
+        forward (4), four-step adjoint (5) and eight-step forward (6) kernels
+        while GJW_OPT_KTIMING is on."""
         ms, n, nl = C.c_double(0), C.c_int64(0), C.c_int64(0)
         N.check(N.load().gjw_kernel_stats(self.h, int(which), C.byref(ms), C.byref(n), C.byref(nl),
                                           int(reset)), "gjw_kernel_stats")
@@ -295,7 +296,7 @@ class Survey:
     def tile(self, which=2):
         """Autotuned columns per wavefront of step kernel `which` (numbered
         as in kernel_stats: 0/1 one-step forward/adjoint, 2/3 two-step,
-        4/5 four-step)."""
+        4/5 four-step, 6 eight-step forward)."""
         L = C.c_int32(0)
         N.check(N.load().gjw_tile(self.h, int(which), C.byref(L)), "gjw_tile")
         return L.value

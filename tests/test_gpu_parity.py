@@ -251,10 +251,9 @@ def test_autograd_function(oracle_mod, gpu_ctx):
         assert abs(fd - ge[a, b]) <= 0.05 * abs(ge[a, b])
 
 
-@pytest.mark.parametrize("ck", [7, 10])
-@pytest.mark.parametrize("fuse", [2, 4])
+@pytest.mark.parametrize("fuse,ck", [(2, 7), (2, 10), (4, 7), (4, 10), (8, 10), (8, 21)])
 def test_two_step_kernel_matches_one_step(gpu_ctx, ck, fuse):
-    """The temporally blocked forward (two or four steps per launch,
+    """The temporally blocked forward (two, four or eight steps per launch,
     overlapped row tiles) equals the one-step kernel: traces and gradient, odd
     nt, odd and even checkpoint intervals, several tile widths."""
     from gerjoii_amd import _native as N
@@ -289,7 +288,7 @@ def _borehole_case(nt=420):
 
 
 def test_borehole_forward_and_gradient(oracle_mod, gpu_ctx):
-    """Interior (fast-path) tiles against the FP64 oracle, one- and two-step."""
+    """Interior (fast-path) tiles against the FP64 oracle, one to eight steps per launch."""
     from gerjoii_amd import _native as N
     case = _borehole_case()
     ref = oracle_forward(oracle_mod, case, case.eps0, case.sig0)
@@ -297,7 +296,7 @@ def test_borehole_forward_and_gradient(oracle_mod, gpu_ctx):
     adj = rng.standard_normal(ref.shape).astype(np.float32)
     rp = np.concatenate([[0], np.cumsum([len(r) for r in case.receivers])])
     fe, fs, _ = oracle_grad(oracle_mod, case, case.eps0, case.sig0, adj, rp)
-    for fuse in (1, 2, 4):
+    for fuse in (1, 2, 4, 8):
         for L in (8, 32):
             sv = make_survey(case, gpu_ctx, ck=13, L=L)
             sv.set_option(N.GJW_OPT_FUSE, fuse)
@@ -339,14 +338,14 @@ def test_snapshot_checkpoints(oracle_mod, gpu_ctx, mode):
 def test_source_recorder(oracle_mod, gpu_ctx):
     """gjw_forward's jz output equals the oracle's Jz (w_solve.m:104): the
     field at the source cell after each step, for sources in PML-row tiles and
-    in interior tiles, one- and two-step kernels and the plain path."""
+    in interior tiles, one- to eight-step kernels and the plain path."""
     from gerjoii_amd import _native as N
     for case in (configs.small(air=3), _borehole_case(nt=200)):
         g = oracle_grid(oracle_mod, case.grid)
         ep, sp = oracle_mod.exp2pml(g, case.eps0, case.sig0)
         ref = np.stack([oracle_mod.forward(g, ep, sp, case.src[q], case.wavelets[q],
                                            case.receivers[q])[1] for q in range(len(case.src))])
-        for plain, fuse in ((False, 4), (False, 2), (False, 1), (True, 1)):
+        for plain, fuse in ((False, 8), (False, 4), (False, 2), (False, 1), (True, 1)):
             sv = make_survey(case, gpu_ctx, plain=plain)
             sv.set_option(N.GJW_OPT_FUSE, fuse)
             d, jz = sv.forward(jz=True)
@@ -367,8 +366,10 @@ def test_fwi_gradient_graph_replays(gpu_ctx, snap, misfit):
     svt = make_survey(case, gpu_ctx)
     svt.set_model(case.eps_true, case.sig_true)
     dobs = torch.from_numpy(svt.forward()).cuda()
-    eager = make_survey(case, gpu_ctx, ck=7)
-    graph = make_survey(case, gpu_ctx, ck=7)
+    # a fixed tile width: with autotuning, each survey's first sweep tries
+    # its own adjoint widths, and results differ with the tiling in the last bits
+    eager = make_survey(case, gpu_ctx, ck=7, L=16)
+    graph = make_survey(case, gpu_ctx, ck=7, L=16)
     for sv in (eager, graph):
         sv.set_option(N.GJW_OPT_SNAPSHOT, snap)
         sv.set_misfit(misfit)
@@ -387,6 +388,27 @@ def test_fwi_gradient_graph_replays(gpu_ctx, snap, misfit):
             svt.forward()  # eager work on another survey between replays
             torch.cuda.synchronize()
         print("graph replays ok, J", J0)
+
+
+def test_adjoint_width_tuned_in_context(gpu_ctx):
+    """The first gradient sweep of an autotuned survey gives its four-step
+    adjoint launches the candidate tile widths in turn and keeps the fastest;
+    the gradients of that sweep and of later ones agree with a fixed-width
+    survey to rounding."""
+    import torch
+    case = configs.small(m=150, n=260, nt=400, ns=2)
+    svt = make_survey(case, gpu_ctx)
+    svt.set_model(case.eps_true, case.sig_true)
+    dobs = torch.from_numpy(svt.forward()).cuda()
+    ref = make_survey(case, gpu_ctx, L=16).fwi_gradient(dobs)
+    sv = make_survey(case, gpu_ctx)
+    assert sv.tile(5) == 0
+    for call in range(2):
+        ge, gs, J = sv.fwi_gradient(dobs)
+        print("call", call, "adjoint L", sv.tile(5), rel_l2(ge, ref[0]), rel_l2(gs, ref[1]))
+        assert sv.tile(5) in (12, 16, 24, 32, 48)
+        assert rel_l2(ge, ref[0]) <= 1e-5 and rel_l2(gs, ref[1]) <= 1e-5
+        assert abs(J - ref[2]) <= 1e-6 * abs(ref[2])
 
 
 @pytest.mark.parametrize("nt", [240, 241])
