@@ -155,6 +155,30 @@ __device__ __forceinline__ TileId vtile_of_block(int NX, int NY, int L, int rows
   return tile_decode(t, gx, gy, B / (gx * gy));
 }
 
+// Row tile t of the multi-step kernels (R = 64 - 2 NS owned rows per wave):
+// tile t owns rows t*R .. t*R+R-1, except that the last tile is aligned to
+// the bottom row (it owns NX+1-R .. NX) and the tile before it stops where
+// the last one begins.  The wave's 64 lanes hold rows r0 = (first owned row)
+// - NS .. r0+63.  With the bottom alignment a wave whose rows reach into the
+// bottom PML strip owns only rows near it: for a 40-cell PML and NS = 8 the
+// top and the bottom strip take one wave each.
+struct RowTile {
+  int r0, own_hi;
+};
+__device__ __forceinline__ RowTile row_tile(int t, int NX, int R, int NS) {
+  const int T = (NX + 1 + R - 1) / R;
+  RowTile rt;
+  if (t >= T) {  // no owned row
+    rt.r0 = t * R - NS;
+    rt.own_hi = -1;
+    return rt;
+  }
+  const int lo = t == T - 1 ? max(NX + 1 - R, 0) : t * R;
+  rt.r0 = lo - NS;
+  rt.own_hi = t == T - 1 ? NX : t + 1 == T - 1 ? NX - R : (t + 1) * R - 1;
+  return rt;
+}
+
 // Several surveys in one launch: surveys[a.sid[k]] owns blocks
 // [a.blk0[k], a.blk0[k+1]) (a multiple of 8 each, so the XCD-aware order holds
 // per survey); its own step, tile width, buffer parities and frame slot come
@@ -447,13 +471,15 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   const int shot = tid.shot;
   // threadIdx.y is constant across a 64-lane wavefront: make the tile origin
   // (and everything derived from it) scalar so loop control stays uniform
-  const int r0 = __builtin_amdgcn_readfirstlane((tid.rowtile * blockDim.y + threadIdx.y) * R - NS);
+  const RowTile rt = row_tile((int)(tid.rowtile * blockDim.y + threadIdx.y), NX, R, NS);
+  const int r0 = __builtin_amdgcn_readfirstlane(rt.r0);
+  const int own_hi = __builtin_amdgcn_readfirstlane(rt.own_hi);
   const int jb = __builtin_amdgcn_readfirstlane(tid.strip * L);
   if (r0 + NS > NX) return;  // no owned row
   const int je = min(jb + L, NY + 1);
   const int i = r0 + lane;
   const bool okrow = i >= 0 && i <= NX;
-  const bool own = okrow && lane >= NS && lane <= 63 - NS;
+  const bool own = okrow && lane >= NS && lane <= 63 - NS && i <= own_hi;
   const long long so = (long long)shot * g.FS;
   E0 += so; Hx0 += so; Hy0 += so; X0 += so; Y0 += so;
   E2 += so; Hx2 += so; Hy2 += so; X2 += so; Y2 += so;
@@ -462,8 +488,12 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
 #pragma unroll
   for (int l = 0; l < NS; ++l) ws[l] = s.sc[shot] * s.wav[(long long)shot * s.nt + it + l];
   const int* cp = s.cptr + (long long)shot * (NY + 2);
-  const bool has_rec = tr && r0 + NS <= s.rmax && r0 + 63 - NS >= s.rmin;
-  const bool inner_rows = (r0 >= P + 1) && (r0 + 63 <= NX - P - 1);
+  const bool has_rec = tr && r0 + NS <= s.rmax && own_hi >= s.rmin;
+  // Every row the owned rows depend on is an interior row: level l's values
+  // are needed on rows within NS-1-l of the owned ones, so the rows of lanes 0
+  // and 63 (and, in a tile that stops early, the rows past own_hi+NS-1) are
+  // never needed; the interior update may run there whatever class they are.
+  const bool inner_rows = (r0 + 1 >= P + 1) && (own_hi + NS - 1 <= NX - P - 1);
   const bool dom_row = i - g.di0 >= 0 && i - g.di0 < g.m;
   auto dom_col = [&](int c) { return c - g.dj0 >= 0 && c - g.dj0 < g.n; };
   const float bx0 = g.bx0, by0 = g.by0;
@@ -1214,13 +1244,15 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   const int NX = g.NX, NY = g.NY, P = g.P;
   const long long LD = g.LD;
   const int shot = tid.shot;
-  const int r0 = __builtin_amdgcn_readfirstlane((tid.rowtile * blockDim.y + threadIdx.y) * R - NS);
+  const RowTile rt = row_tile((int)(tid.rowtile * blockDim.y + threadIdx.y), NX, R, NS);
+  const int r0 = __builtin_amdgcn_readfirstlane(rt.r0);
+  const int own_hi = __builtin_amdgcn_readfirstlane(rt.own_hi);
   const int jb = __builtin_amdgcn_readfirstlane(tid.strip * L);
   if (r0 + NS > NX) return;  // no owned row
   const int je = min(jb + L, NY + 1);
   const int i = r0 + lane;
   const bool okrow = i >= 0 && i <= NX;
-  const bool own = okrow && lane >= NS && lane <= 63 - NS;
+  const bool own = okrow && lane >= NS && lane <= 63 - NS && i <= own_hi;
   const long long so = (long long)shot * g.FS;
   M0 += so; Nx0 += so; Ny0 += so; X0 += so; Y0 += so;
   M2 += so; Nx2 += so; Ny2 += so; X2 += so; Y2 += so;
@@ -1231,7 +1263,8 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   float* ac = acc ? acc + 2 * (long long)shot * astride : nullptr;
   // residual injection reaches halo lanes too: later levels read their values
   const bool has_rec = adj && r0 <= s.rmax && r0 + 63 >= s.rmin;
-  const bool inner_rows = (r0 >= P + 1) && (r0 + 63 <= NX - P - 1);
+  // the rows the owned rows depend on are interior rows (see fwdN_body)
+  const bool inner_rows = (r0 + 1 >= P + 1) && (own_hi + NS - 1 <= NX - P - 1);
   const bool dom_row = i - g.di0 >= 0 && i - g.di0 < g.m;
   auto dom_col = [&](int c) { return c - g.dj0 >= 0 && c - g.dj0 < g.n; };
   const float bx0 = g.bx0, by0 = g.by0;
