@@ -199,7 +199,8 @@ struct gjw_survey {
   std::vector<int> si, sj, tr_shot, tr_row, tr_col;
   std::vector<long long> rec_ptr;
   std::vector<float> sc;
-  DBuf<int> d_si, d_sj, d_cptr, d_crow, d_cid, d_trs, d_trr, d_trc;
+  DBuf<int> d_si, d_sj, d_cptr, d_crow, d_cid, d_trs, d_trr, d_trc, d_rp;
+  int rec_multi = 0;  // some cell holds several receivers of one shot
   DBuf<float> d_sc, d_wav;
   // state
   DBuf<float> fld;   // forward: E0,Hx0,Hy0,E1,Hx1,Hy1,X,Y  (8 * ns * FS)
@@ -316,6 +317,7 @@ extern "C" int gjw_survey_destroy(gjw_survey* s) {
   s->fe.release(); s->fs.release(); s->d_pml.release();
   s->d_si.release(); s->d_sj.release(); s->d_cptr.release(); s->d_crow.release();
   s->d_cid.release(); s->d_trs.release(); s->d_trr.release(); s->d_trc.release();
+  s->d_rp.release();
   s->d_sc.release(); s->d_wav.release(); s->fld.release(); s->adjf.release();
   s->tr.release(); s->adj.release(); s->obs.release(); s->ck.release(); s->jzb.release();
   s->frames.release(); s->acc.release(); s->ge.release(); s->gsg.release();
@@ -653,6 +655,20 @@ extern "C" int gjw_set_shots(gjw_survey* s, int32_t ns, const int32_t* src, cons
   TRY(s->d_trs.ensure(ntr));
   TRY(s->d_trr.ensure(ntr));
   TRY(s->d_trc.ensure(ntr));
+  TRY(s->d_rp.ensure(ns + 1));
+  {
+    std::vector<int> rp(ns + 1);
+    for (int q = 0; q <= ns; ++q) rp[q] = (int)s->rec_ptr[q];
+    HIPCHK(hipMemcpy(s->d_rp.p, rp.data(), (ns + 1) * 4, hipMemcpyHostToDevice));
+    // a cell with several receivers of one shot: the kernels' per-lane
+    // receiver lookup keeps one trace per cell, so such surveys use the
+    // per-receiver loop
+    std::vector<long long> cells(ntr);
+    for (long long t = 0; t < ntr; ++t)
+      cells[t] = ((long long)s->tr_shot[t] * (s->NY + 1) + s->tr_col[t]) * (s->NX + 1) + s->tr_row[t];
+    std::sort(cells.begin(), cells.end());
+    s->rec_multi = std::adjacent_find(cells.begin(), cells.end()) != cells.end();
+  }
   HIPCHK(hipMemcpy(s->d_si.p, s->si.data(), ns * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(s->d_sj.p, s->sj.data(), ns * 4, hipMemcpyHostToDevice));
   TRY(host_or_dev_copy(s->d_wav.p, wavelets, (size_t)ns * nt * 4, st));
@@ -716,6 +732,8 @@ static DevShots dev_shots(const gjw_survey* s, int q0, bool jz = false) {
   d.rmin = s->rrow_min;
   d.rmax = s->rrow_max;
   d.jz = jz ? s->jzb.p + (long long)q0 * s->gr.nt : nullptr;
+  d.rp = s->d_rp.p + q0;
+  d.rec_multi = s->rec_multi;
   return d;
 }
 

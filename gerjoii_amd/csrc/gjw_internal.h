@@ -48,6 +48,8 @@ struct DevShots {
   const int* cid;    // global trace index; trace t occupies tr[t*nt .. t*nt+nt)
   int rmin, rmax;    // receiver rows over all shots (waves outside skip the lookup)
   float* jz;         // source recorder Ez(source) per step, [shot][nt], or null
+  const int* rp;     // first trace of each shot, [shot], and one past the last
+  int rec_multi;     // some cell holds several receivers of one shot
 };
 
 // ---- several surveys in one launch (gjw_batch_*) ----

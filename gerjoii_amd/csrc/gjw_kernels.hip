@@ -556,6 +556,24 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     for (int q = qa; q < qb; ++q)
       if (s.crow[q] == i) tr[(long long)s.cid[q] * s.nt + step] = e;
   };
+  // the interior path's receivers: the trace of the lane's cell in column j
+  // (-1 if none; one receiver per cell), written through the shot's traces
+  const int t_lo = has_rec ? s.rp[shot] : 0;
+  const __amdgpu_buffer_rsrc_t rTR =
+      rsrc_of(has_rec ? tr + (long long)t_lo * s.nt : nullptr,
+              has_rec ? (long long)(s.rp[shot + 1] - t_lo) * s.nt : 0);
+  auto rec_lookup = [&](int j) {
+    int r = -1;
+    if (has_rec && j >= 0 && j <= NY) {
+      const int qa = cp[j], qb = cp[j + 1];
+      for (int q = qa; q < qb; ++q)
+        if (s.crow[q] == i) r = s.cid[q];
+    }
+    return r;
+  };
+  int rid_l[NS];  // trace of the lane's cell in the column of level l
+#pragma unroll
+  for (int l = 0; l < NS; ++l) rid_l[l] = -1;
   // per-level outputs that leave the kernel: receivers, source recorder, frame
   auto level_out = [&](int l, int cl, float e) {
     const bool ol = own && cl >= jb && cl < je;
@@ -729,6 +747,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     for (int u = 0; u < UI; ++u) {
       const int c = jj + u;
       float eo[NS], hxo[NS], hyo[NS], zo[NS];
+      if (src) rid_l[0] = rec_lookup(c);
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
         const int cl = c - l;
@@ -746,7 +765,10 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
         hyo[l] = hy1;
         zo[l] = 0.f;  // no split-field cells in these rows
         const bool ol = own && cl >= jb && cl < je;
-        if (src) bstore(e1, rJZ, (unsigned)(it + l), ol && i == si && cl == sj);
+        if (src) {
+          bstore(e1, rJZ, (unsigned)(it + l), ol && i == si && cl == sj);
+          bstore(e1, rTR, (unsigned)((rid_l[l] - t_lo) * s.nt + it + l), ol && rid_l[l] >= 0);
+        }
         if (FRAMES)
           bstore(e1, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + i), ol && dom_row && dom_col(cl));
         if (l == NS - 1) {
@@ -760,6 +782,10 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       shift(eo, hxo, hyo, zo, zo);
 #pragma unroll
       for (int l = NS - 1; l >= 1; --l) cc_l[l] = l == 1 ? b.c[u] : cc_l[l - 1];
+      if (src) {
+#pragma unroll
+        for (int l = NS - 1; l >= 1; --l) rid_l[l] = rid_l[l - 1];
+      }
     }
   };
 
@@ -842,11 +868,19 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   // The two wave kinds get separate copies of the loop, so that each keeps
   // only its own coefficient carries live.
   const int jlast = je + NS - 2;
-  if (inner_rows && !has_rec) {
+  if (inner_rows && (!has_rec || !s.rec_multi)) {
     int j = j1;
     for (; j <= jlast && !chunkable(j, UI); j += UC) univ_chunk(j);
     int jz = j;  // end of the interior run
     while (jz <= jlast && chunkable(jz, UI)) jz += UI;
+    // the receivers of the interior columns are looked up per lane once per
+    // column (level 0) and carried along with the column to the later levels
+    if (has_rec && j < jz) {
+      // levels >= 1 start on columns j-1 .. j-NS+1, which level 0 passed in
+      // the strip chunks before the run
+#pragma unroll
+      for (int l = 1; l < NS; ++l) rid_l[l] = rec_lookup(j - l);
+    }
     auto run = [&](auto SRC) {
       InnerBuf A, B;
       inner_load(j, A, j < jz);
@@ -860,7 +894,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
         j += UI;
       }
     };
-    if (si >= r0 && si <= r0 + 63)
+    if (has_rec || (si >= r0 && si <= r0 + 63))
       run(std::true_type{});
     else
       run(std::false_type{});
