@@ -423,6 +423,9 @@ __device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned off_
 #ifndef GJW_ADJ_WAVES
 #define GJW_ADJ_WAVES 4
 #endif
+#ifndef GJW_ADJM_WAVES  // multi-survey adjoint (carries the survey table too)
+#define GJW_ADJM_WAVES 3
+#endif
 #ifndef GJW_FWD8_WAVES
 #define GJW_FWD8_WAVES 3
 #endif
@@ -1360,6 +1363,24 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
         if (s.crow[q] == i) mn += adj[(long long)s.cid[q] * s.nt + step];
     }
   };
+  // the interior path's receivers: the trace of the lane's cell in column c
+  // (-1 if none; one receiver per cell), read through the shot's residuals
+  const int t_lo = has_rec ? s.rp[shot] : 0;
+  const __amdgpu_buffer_rsrc_t rAD =
+      rsrc_of(has_rec ? const_cast<float*>(adj) + (long long)t_lo * s.nt : nullptr,
+              has_rec ? (long long)(s.rp[shot + 1] - t_lo) * s.nt : 0);
+  auto rec_lookup = [&](int c) {
+    int r = -1;
+    if (has_rec && c >= 0 && c <= NY) {
+      const int qa = cp[c], qb = cp[c + 1];
+      for (int q = qa; q < qb; ++q)
+        if (s.crow[q] == i) r = s.cid[q];
+    }
+    return r;
+  };
+  int rid_h[NS];  // rid_h[k]: trace of the lane's cell k columns before the chunk (k >= 1)
+#pragma unroll
+  for (int k = 0; k < NS; ++k) rid_h[k] = -1;
 
   // ---- state.  Level l works on column c-l (c: level 0's column): its
   // input M, W = (wy, wx), split aux (x, y) and nu = (ny, nx) of that column
@@ -1570,7 +1591,9 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   };
 
   // ---- interior chunk, all rows inner, no receivers
-  auto inner_chunk = [&](int jj, auto SRC) {
+  // REC: the wave's rows hold receivers (their residuals are injected)
+  auto inner_chunk = [&](int jj, auto SRC, auto REC) {
+    constexpr bool rec = decltype(REC)::value;
     float mL[UA], nx_[UA], ny_[UA], f_[UA][NS + 1];
     float2 cL[UA], t_[UA];
 #pragma unroll
@@ -1582,6 +1605,21 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
       ny_[u] = bload(rNy0, kk, true);
       const int cz = jj + u - (NS - 1);
       load_corr(cz, f_[u], t_[u], ac && corr_on(cz));
+    }
+    // residuals injected by level l at column jj+u-l, loaded with the chunk
+    float ra[UA][NS];
+    int rc[UA];
+    if (rec) {
+#pragma unroll
+      for (int u = 0; u < UA; ++u) rc[u] = rec_lookup(jj + u);
+#pragma unroll
+      for (int u = 0; u < UA; ++u)
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          const int r = u - l >= 0 ? rc[u - l] : rid_h[l - u];
+          const int step = it - l - 1;
+          ra[u][l] = bload(rAD, (unsigned)((r - t_lo) * s.nt + step), r >= 0 && step >= 0);
+        }
     }
 #pragma unroll
     for (int u = 0; u < UA; ++u) {
@@ -1596,7 +1634,8 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
         const float nyv = ny + wy_in[l] - shfl_dn1(wy_in[l]);
         const float nxv = nx + wx_in[l] - wxn;
         const float py = by0 * nyv, px = bx0 * nxv;
-        const float mn = ce_l[l] * me + shfl_up1(py) - py + pxp[l] - px;
+        float mn = ce_l[l] * me + shfl_up1(py) - py + pxp[l] - px;
+        if (rec) mn += ra[u][l];
         mo[l] = mn;
         wyo[l] = cy_l[l] * mn;
         wxo[l] = -cy_l[l] * mn;
@@ -1617,6 +1656,10 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
       }
       ce_l[0] = cL[u].x;
       cy_l[0] = cL[u].y;
+    }
+    if (rec) {  // receiver history for the next chunk
+#pragma unroll
+      for (int k = NS - 1; k >= 1; --k) rid_h[k] = k <= UA ? rc[UA - k] : rid_h[k - UA];
     }
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
@@ -1705,13 +1748,21 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   };
 
   // level 0 at columns jb-NS+1 .. je+NS-2 (the last level's last column is je-1)
-  const bool use_inner = inner_rows && !has_rec;
+  const bool use_inner = inner_rows && (!has_rec || !s.rec_multi);
   const int jlast = je + NS - 2;
   int j = j1;
   for (; j <= jlast && !chunkable(j); j += UA) univ_chunk(j);
+  if (use_inner && has_rec) {
+    // levels >= 1 start on columns j-1 .. j-NS+1, passed by the strip chunks
+    if (j <= jlast && chunkable(j)) {
+#pragma unroll
+      for (int k = 1; k < NS; ++k) rid_h[k] = rec_lookup(j - k);
+    }
+    for (; j <= jlast && chunkable(j); j += UA) inner_chunk(j, std::true_type{}, std::true_type{});
+  }
   for (; j <= jlast && chunkable(j); j += UA) {
     if (use_inner)
-      inner_chunk(j, std::true_type{});
+      inner_chunk(j, std::true_type{}, std::false_type{});
     else
       class_chunk(j, std::true_type{});
   }
@@ -1743,7 +1794,7 @@ k_adj_step4(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
 }
 
 template <int NS>
-__global__ void __launch_bounds__(256, GJW_ADJ_WAVES)
+__global__ void __launch_bounds__(256, GJW_ADJM_WAVES)
 k_adj_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
   int k;
   TileId tid;
