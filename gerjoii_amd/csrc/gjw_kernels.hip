@@ -559,8 +559,11 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     for (int q = qa; q < qb; ++q)
       if (s.crow[q] == i) tr[(long long)s.cid[q] * s.nt + step] = e;
   };
-  // the interior path's receivers: the trace of the lane's cell in column j
-  // (-1 if none; one receiver per cell), written through the shot's traces
+  // Receivers: level 0 looks up the trace of the lane's cell in its column
+  // (-1 if none; one receiver per cell) and the lookup travels with the
+  // column to the later levels (rid_l[l]: level l's column); the samples are
+  // written through the shot's traces.  Surveys with several receivers in one
+  // cell use the per-receiver loop instead.
   const int t_lo = has_rec ? s.rp[shot] : 0;
   const __amdgpu_buffer_rsrc_t rTR =
       rsrc_of(has_rec ? tr + (long long)t_lo * s.nt : nullptr,
@@ -580,7 +583,13 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   // per-level outputs that leave the kernel: receivers, source recorder, frame
   auto level_out = [&](int l, int cl, float e) {
     const bool ol = own && cl >= jb && cl < je;
-    if (has_rec && ol) record(cl, e, it + l);
+    if (has_rec) {
+      if (s.rec_multi) {
+        if (ol) record(cl, e, it + l);
+      } else {
+        bstore(e, rTR, (unsigned)((rid_l[l] - t_lo) * s.nt + it + l), ol && rid_l[l] >= 0);
+      }
+    }
     bstore(e, rJZ, (unsigned)(it + l), ol && i == si && cl == sj);
     if (FRAMES)
       bstore(e, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + ic), ol && dom_row && dom_col(cl));
@@ -666,6 +675,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     for (int u = 0; u < UC; ++u) {
       const int c = jj + u;
       float eo[NS], hxo[NS], hyo[NS], xo[NS], yo[NS];
+      if (has_rec) rid_l[0] = rec_lookup(c);
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
         const int cl = c - l;
@@ -702,6 +712,10 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       }
       e0_c = en_[u];
       shift(eo, hxo, hyo, xo, yo);
+      if (has_rec) {
+#pragma unroll
+        for (int l = NS - 1; l >= 1; --l) rid_l[l] = rid_l[l - 1];
+      }
 #pragma unroll
       for (int l = NS - 1; l >= 1; --l) {
         ka_l[l] = l == 1 ? ka_[u] : ka_l[l - 1];
@@ -815,6 +829,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     for (int u = 0; u < UC; ++u) {
       const int c = jj + u;
       float eo[NS], hxo[NS], hyo[NS], xo[NS], yo[NS];
+      if (has_rec) rid_l[0] = rec_lookup(c);
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
         const int cl = c - l;
@@ -853,6 +868,10 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       }
       e0_c = en_[u];
       shift(eo, hxo, hyo, xo, yo);
+      if (has_rec) {
+#pragma unroll
+        for (int l = NS - 1; l >= 1; --l) rid_l[l] = rid_l[l - 1];
+      }
 #pragma unroll
       for (int l = NS - 1; l >= 1; --l) {
         cc_l[l] = l == 1 ? c_[u] : cc_l[l - 1];
@@ -876,14 +895,6 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     for (; j <= jlast && !chunkable(j, UI); j += UC) univ_chunk(j);
     int jz = j;  // end of the interior run
     while (jz <= jlast && chunkable(jz, UI)) jz += UI;
-    // the receivers of the interior columns are looked up per lane once per
-    // column (level 0) and carried along with the column to the later levels
-    if (has_rec && j < jz) {
-      // levels >= 1 start on columns j-1 .. j-NS+1, which level 0 passed in
-      // the strip chunks before the run
-#pragma unroll
-      for (int l = 1; l < NS; ++l) rid_l[l] = rec_lookup(j - l);
-    }
     auto run = [&](auto SRC) {
       InnerBuf A, B;
       inner_load(j, A, j < jz);
