@@ -28,7 +28,7 @@ PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
 BYTES_PER_CELL_UPDATE = 32  # Ez, Hx, Hy read+write + 2 coefficient reads (FP32)
 # step kernels in the numbering of gjw_kernel_stats / gjw_tile: (name, steps per launch)
 KERNELS = [("k_fwd_step", 1), ("k_adj_step", 1), ("k_fwd_step2", 2), ("k_adj_step2", 2),
-           ("k_fwd_step4", 4), ("k_adj_step4", 4), ("k_fwd_step8", 8)]
+           ("k_fwd_step4", 4), ("k_adj_step4", 4), ("k_fwd_step8", 8), ("k_adj_step8", 8)]
 
 
 def log(*a):
@@ -48,7 +48,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
-    ap.add_argument("--fuse", type=int, default=0, help="steps per launch: 0 autotuned, 1, 2 or 4")
+    ap.add_argument("--fuse", type=int, default=0, help="forward steps per launch: 0 autotuned, 1, 2, 4, 8")
+    ap.add_argument("--adj-fuse", type=int, default=0, help="adjoint steps per launch: 0 (four), 4, 8")
     ap.add_argument("--ckpt", type=int, default=0)
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run three 1-GiB device copies (calibrates PMC byte counters)")
@@ -90,6 +91,8 @@ def main():
         sv.set_option(N.GJW_OPT_TILE_L, args.tile)
     if args.fuse:
         sv.set_option(N.GJW_OPT_FUSE, args.fuse)
+    if args.adj_fuse:
+        sv.set_option(N.GJW_OPT_ADJ_FUSE, args.adj_fuse)
     if args.ckpt:
         sv.set_option(N.GJW_OPT_CKPT_EVERY, args.ckpt)
     # observed data: true model, kept in HBM

@@ -18,6 +18,7 @@ GJW_OPT_PATH, GJW_OPT_CKPT_EVERY, GJW_OPT_TILE_L, GJW_OPT_SNAPSHOT, GJW_OPT_KTIM
 GJW_OPT_FUSE = 6
 GJW_OPT_GRAPH = 7
 GJW_OPT_MISFIT = 8
+GJW_OPT_ADJ_FUSE = 9
 GJW_SNAP_F32, GJW_SNAP_BF16, GJW_SNAP_F16 = 0, 1, 2
 GJW_MISFIT_L2 = 0
 GJW_MISFIT_ENV_L2, GJW_MISFIT_PHASE_L2, GJW_MISFIT_ENV_L22, GJW_MISFIT_ENV_LOG = 1, 2, 3, 4

@@ -38,6 +38,7 @@ __global__ void k_adj_step(DevGrid, DevShots, int, const float*, const float*, c
     const float*, long long, float*, long long, float*
 __global__ void k_adj_step2(GJW_ADJN_ARGS);
 __global__ void k_adj_step4(GJW_ADJN_ARGS);
+__global__ void k_adj_step8(GJW_ADJN_ARGS);
 #undef GJW_ADJN_ARGS
 #define GJW_FWDN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \
     const float*, const float*, float*, float*, float*, float*, float*, int, float*, float*, long long
@@ -170,7 +171,7 @@ struct GraphCache {
 };
 
 // step kernels timed by GJW_OPT_KTIMING (numbered as in gjw_kernel_stats)
-constexpr int kKernKinds = 7;
+constexpr int kKernKinds = 8;
 
 struct gjw_survey {
   gjw_ctx* ctx = nullptr;
@@ -182,7 +183,8 @@ struct gjw_survey {
   int path = GJW_PATH_BATCHED, ck_every = 0, tile_L = 0, snap = GJW_SNAP_F32, fuse = 0;
   int ck_batch = 0;  // checkpoint interval imposed by gjw_batch_fwi_gradient
   int tuned_L = 0, tuned_L1 = 0, tuned_L4 = 0, tuned_L8 = 0;  // 2-, 1-, 4-, 8-step forward
-  int tuned_La2 = 0, tuned_La4 = 0;             // two- and four-step adjoint kernels
+  int tuned_La2 = 0, tuned_La4 = 0, tuned_La8 = 0;  // two-, four-, eight-step adjoint kernels
+  int adj_fuse = 0;  // adjoint steps per launch (GJW_OPT_ADJ_FUSE; 0: four)
   std::vector<hipEvent_t> ex_ev;                // adjoint tile-width trials of the first sweep
   int fuse_eff = 0;  // forward steps per launch in use (fuse, or the autotuner's choice)
   // model, padded (NX x NY, column-major, FP64)
@@ -230,7 +232,8 @@ struct gjw_survey {
   GraphCache g_all, g_fwd, g_adj;  // whole gradient (L2); forward / adjoint sweeps
   std::vector<hipEvent_t> kev_pool;
   std::vector<std::pair<int, int>> kev_used;  // (kernel id, index of start event)
-  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2, k_fwd_step4, k_adj_step4, k_fwd_step8
+  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2, k_fwd_step4, k_adj_step4, k_fwd_step8,
+  // k_adj_step8
   double kt_ms[kKernKinds] = {};
   long long kt_n[kKernKinds] = {};
   long long kl_n[kKernKinds] = {};  // all launches while timing is on
@@ -342,13 +345,18 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       if (v < 0 || v > 4096) return fail(GJW_ERR_ARG, "GJW_OPT_TILE_L out of range");
       s->tile_L = (int)v;
       s->tuned_L = s->tuned_L1 = s->tuned_L4 = s->tuned_L8 = 0;
-      s->tuned_La2 = s->tuned_La4 = 0;
+      s->tuned_La2 = s->tuned_La4 = s->tuned_La8 = 0;
       s->fuse_eff = 0;
       break;
     case GJW_OPT_MISFIT:
       if (v < GJW_MISFIT_L2 || v > GJW_MISFIT_ENV_LOG)
         return fail(GJW_ERR_ARG, "GJW_OPT_MISFIT: unknown misfit %lld", (long long)v);
       s->misfit_kind = (int)v;
+      return GJW_OK;
+    case GJW_OPT_ADJ_FUSE:
+      if (v != 0 && v != 4 && v != 8)
+        return fail(GJW_ERR_ARG, "GJW_OPT_ADJ_FUSE must be 0 (four), 4 or 8");
+      s->adj_fuse = (int)v;
       return GJW_OK;
     case GJW_OPT_GRAPH:
       if (v != 0 && v != 1) return fail(GJW_ERR_ARG, "GJW_OPT_GRAPH must be 0 or 1");
@@ -360,7 +368,7 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
         return fail(GJW_ERR_ARG, "GJW_OPT_FUSE must be 0 (autotuned), 1, 2, 4 or 8");
       s->fuse = (int)v;
       s->tuned_L = 0;
-      s->tuned_La2 = s->tuned_La4 = 0;
+      s->tuned_La2 = s->tuned_La4 = s->tuned_La8 = 0;
       s->fuse_eff = 0;
       break;
     case GJW_OPT_KTIMING:
@@ -795,7 +803,7 @@ static int kev_begin(gjw_survey* s, int which, int it, int* slot) {
   if (s->ksample <= 0) return GJW_OK;
   (void)it;
   // time one launch of this kernel in every ksample steps it covers
-  const int steps = which == 6 ? 8 : which >= 4 ? 4 : which >= 2 ? 2 : 1;
+  const int steps = which >= 6 ? 8 : which >= 4 ? 4 : which >= 2 ? 2 : 1;
   const long long every = std::max(1, s->ksample / steps);
   if (s->kl_n[which]++ % every != 0) return GJW_OK;
   const size_t need = 2 * (s->kev_used.size() + 1);
@@ -926,14 +934,16 @@ static int adjN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& 
                        int L, AdjBufs& b, int cur, int it, const float* adj, const float* Fa,
                        long long fstride, float* acc, long long astride, float* gw) {
   int slot;
-  TRY(kev_begin(s, NS == 4 ? 5 : 3, it, &slot));
+  TRY(kev_begin(s, NS == 8 ? 7 : NS == 4 ? 5 : 3, it, &slot));
   const dim3 grid = stepN_grid(s, NS, L, nsh), block(64, 4);
 #define GJW_ADJN_CALL(K)                                                                   \
   hipLaunchKernelGGL(K, grid, block, 0, s->ctx->stream, g, sh, L, b.M[cur], b.Nx[cur],    \
                      b.Ny[cur], b.X[cur], b.Y[cur], b.M[cur ^ 1], b.Nx[cur ^ 1],          \
                      b.Ny[cur ^ 1], b.X[cur ^ 1], b.Y[cur ^ 1], it, adj, Fa, fstride, acc, \
                      astride, gw)
-  if (NS == 4)
+  if (NS == 8)
+    GJW_ADJN_CALL(k_adj_step8);
+  else if (NS == 4)
     GJW_ADJN_CALL(k_adj_step4);
   else
     GJW_ADJN_CALL(k_adj_step2);
@@ -1386,13 +1396,17 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
   AdjBufs ab = adj_bufs(s, ns);
   const long long fstride = (long long)(K + 1) * FD;  // frames per shot
   int acur = 0;
+  // Adjoint depth: four steps per launch, or eight with GJW_OPT_ADJ_FUSE = 8
+  // (when the forward fuses at least as many).
+  const int dmax = s->adj_fuse == 8 && s->fuse_eff >= 8 ? 8 : 4;
+  int& tuned_deep = dmax == 8 ? s->tuned_La8 : s->tuned_La4;
   // Adjoint tile width, tuned in context: the first (uncaptured) sweep gives
-  // its first four-step launches the candidate widths in turn, timed with
-  // events; later sweeps use the fastest.
-  if (s->tile_L > 0) s->tuned_La2 = s->tuned_La4 = s->tile_L;
+  // its first launches of the deepest kernel the candidate widths in turn,
+  // timed with events; later sweeps use the fastest.
+  if (s->tile_L > 0) s->tuned_La2 = s->tuned_La4 = s->tuned_La8 = s->tile_L;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   HIPCHK(hipStreamIsCapturing(st, &cap));
-  const bool explore = s->tuned_La4 <= 0 && cap == hipStreamCaptureStatusNone;
+  const bool explore = tuned_deep <= 0 && cap == hipStreamCaptureStatusNone;
   const int nx = explore ? kAdjRounds * kAdjNCand : 0;
   int ex = 0;  // trial launches issued
   if (explore)
@@ -1417,10 +1431,13 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
     }
     for (int it = k1 - 1; it >= k0;) {
       const float* F = s->frames.p + (long long)(it - k0) * FD;  // Ez(S_it)
-      const int NS = (s->fuse_eff >= 4 && it - 3 >= k0) ? 4 : (s->fuse_eff >= 2 && it - 1 >= k0) ? 2 : 1;
+      const int NS = (dmax == 8 && it - 7 >= k0)                ? 8
+                     : (s->fuse_eff >= 4 && it - 3 >= k0) ? 4
+                     : (s->fuse_eff >= 2 && it - 1 >= k0) ? 2
+                                                          : 1;
       if (NS > 1) {
-        int L = NS == 4 ? s->tuned_La4 : s->tuned_La2;
-        const bool trial = NS == 4 && ex < nx;
+        int L = NS == 8 ? s->tuned_La8 : NS == 4 ? s->tuned_La4 : s->tuned_La2;
+        const bool trial = NS == dmax && ex < nx;
         if (trial) L = kAdjCand[ex % kAdjNCand];
         if (L <= 0) L = La_fallback;
         if (trial) HIPCHK(hipEventRecord(s->ex_ev[2 * ex], st));
@@ -1460,7 +1477,10 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
         }
       }
     }
-    s->tuned_La2 = s->tuned_La4 = bestL;
+    if (dmax == 8)
+      s->tuned_La8 = bestL;
+    else
+      s->tuned_La2 = s->tuned_La4 = bestL;
   }
   return GJW_OK;
 }
@@ -1779,7 +1799,7 @@ static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
           (uintptr_t)s->gr.nt, (uintptr_t)s->tuned_L, (uintptr_t)s->tuned_L1,
           (uintptr_t)s->tuned_L4, (uintptr_t)s->tuned_L8, (uintptr_t)s->fuse_eff,
           (uintptr_t)s->tuned_La2,
-          (uintptr_t)s->tuned_La4,
+          (uintptr_t)s->tuned_La4, (uintptr_t)s->tuned_La8, (uintptr_t)s->adj_fuse,
           (uintptr_t)s->ck_K, (uintptr_t)s->fuse, (uintptr_t)s->snap,
           (uintptr_t)s->rrow_min, (uintptr_t)s->rrow_max, (uintptr_t)s->pml_idx.size(),
           fbits(s->bx0), fbits(s->by0)};
@@ -2209,7 +2229,7 @@ extern "C" int gjw_kernel_stats(gjw_survey* s, int32_t which, double* avg_ms, in
 extern "C" int gjw_tile(gjw_survey* s, int32_t which, int32_t* L) {
   if (!s || !L || which < 0 || which >= kKernKinds) return fail(GJW_ERR_ARG, "gjw_tile: bad argument");
   const int t[kKernKinds] = {s->tuned_L1, s->tuned_L1, s->tuned_L,  s->tuned_La2,
-                             s->tuned_L4, s->tuned_La4, s->tuned_L8};
+                             s->tuned_L4, s->tuned_La4, s->tuned_L8, s->tuned_La8};
   *L = t[which];
   return GJW_OK;
 }

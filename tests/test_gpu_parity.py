@@ -267,6 +267,8 @@ def test_two_step_kernel_matches_one_step(gpu_ctx, ck, fuse):
     for L in (5, 16, 64):
         sv = make_survey(case, gpu_ctx, ck=ck, L=L)
         sv.set_option(N.GJW_OPT_FUSE, fuse)
+        if fuse == 8:
+            sv.set_option(N.GJW_OPT_ADJ_FUSE, 8)
         d2 = sv.forward(keep=True)
         g2 = sv.gradient(adj)
         print("L", L, rel_l2(d2, d1), rel_l2(g2[0], g1[0]), rel_l2(g2[1], g1[1]))
@@ -300,6 +302,8 @@ def test_borehole_forward_and_gradient(oracle_mod, gpu_ctx):
         for L in (8, 32):
             sv = make_survey(case, gpu_ctx, ck=13, L=L)
             sv.set_option(N.GJW_OPT_FUSE, fuse)
+            if fuse == 8:
+                sv.set_option(N.GJW_OPT_ADJ_FUSE, 8)
             d = sv.forward(keep=True)
             ge, gs = sv.gradient(adj)
             print("fuse", fuse, "L", L, "traces", rel_l2(d, ref), "eps", rel_l2(ge, fe), "sig", rel_l2(gs, fs))
