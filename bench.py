@@ -28,7 +28,8 @@ PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
 BYTES_PER_CELL_UPDATE = 32  # Ez, Hx, Hy read+write + 2 coefficient reads (FP32)
 # step kernels in the numbering of gjw_kernel_stats / gjw_tile: (name, steps per launch)
 KERNELS = [("k_fwd_step", 1), ("k_adj_step", 1), ("k_fwd_step2", 2), ("k_adj_step2", 2),
-           ("k_fwd_step4", 4), ("k_adj_step4", 4), ("k_fwd_step8", 8), ("k_adj_step8", 8)]
+           ("k_fwd_step4", 4), ("k_adj_step4", 4), ("k_fwd_step8", 8), ("k_adj_step8", 8),
+           ("k_adj_step6", 6)]
 
 
 def log(*a):
@@ -49,7 +50,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--fuse", type=int, default=0, help="forward steps per launch: 0 autotuned, 1, 2, 4, 8")
-    ap.add_argument("--adj-fuse", type=int, default=0, help="adjoint steps per launch: 0 (four), 4, 8")
+    ap.add_argument("--adj-fuse", type=int, default=0, help="adjoint steps per launch: 0 (six), 4, 6, 8")
     ap.add_argument("--ckpt", type=int, default=0)
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run three 1-GiB device copies (calibrates PMC byte counters)")

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -39,6 +40,7 @@ __global__ void k_adj_step(DevGrid, DevShots, int, const float*, const float*, c
 __global__ void k_adj_step2(GJW_ADJN_ARGS);
 __global__ void k_adj_step4(GJW_ADJN_ARGS);
 __global__ void k_adj_step8(GJW_ADJN_ARGS);
+__global__ void k_adj_step6(GJW_ADJN_ARGS);
 #undef GJW_ADJN_ARGS
 #define GJW_FWDN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \
     const float*, const float*, float*, float*, float*, float*, float*, int, float*, float*, long long
@@ -171,7 +173,7 @@ struct GraphCache {
 };
 
 // step kernels timed by GJW_OPT_KTIMING (numbered as in gjw_kernel_stats)
-constexpr int kKernKinds = 8;
+constexpr int kKernKinds = 9;
 
 struct gjw_survey {
   gjw_ctx* ctx = nullptr;
@@ -183,8 +185,8 @@ struct gjw_survey {
   int path = GJW_PATH_BATCHED, ck_every = 0, tile_L = 0, snap = GJW_SNAP_F32, fuse = 0;
   int ck_batch = 0;  // checkpoint interval imposed by gjw_batch_fwi_gradient
   int tuned_L = 0, tuned_L1 = 0, tuned_L4 = 0, tuned_L8 = 0;  // 2-, 1-, 4-, 8-step forward
-  int tuned_La2 = 0, tuned_La4 = 0, tuned_La8 = 0;  // two-, four-, eight-step adjoint kernels
-  int adj_fuse = 0;  // adjoint steps per launch (GJW_OPT_ADJ_FUSE; 0: four)
+  int tuned_La2 = 0, tuned_La4 = 0, tuned_La6 = 0, tuned_La8 = 0;  // 2-, 4-, 6-, 8-step adjoint
+  int adj_fuse = 0;  // adjoint steps per launch (GJW_OPT_ADJ_FUSE; 0: six)
   std::vector<hipEvent_t> ex_ev;                // adjoint tile-width trials of the first sweep
   int fuse_eff = 0;  // forward steps per launch in use (fuse, or the autotuner's choice)
   // model, padded (NX x NY, column-major, FP64)
@@ -233,7 +235,7 @@ struct gjw_survey {
   std::vector<hipEvent_t> kev_pool;
   std::vector<std::pair<int, int>> kev_used;  // (kernel id, index of start event)
   // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2, k_fwd_step4, k_adj_step4, k_fwd_step8,
-  // k_adj_step8
+  // k_adj_step8, k_adj_step6
   double kt_ms[kKernKinds] = {};
   long long kt_n[kKernKinds] = {};
   long long kl_n[kKernKinds] = {};  // all launches while timing is on
@@ -345,7 +347,7 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       if (v < 0 || v > 4096) return fail(GJW_ERR_ARG, "GJW_OPT_TILE_L out of range");
       s->tile_L = (int)v;
       s->tuned_L = s->tuned_L1 = s->tuned_L4 = s->tuned_L8 = 0;
-      s->tuned_La2 = s->tuned_La4 = s->tuned_La8 = 0;
+      s->tuned_La2 = s->tuned_La4 = s->tuned_La6 = s->tuned_La8 = 0;
       s->fuse_eff = 0;
       break;
     case GJW_OPT_MISFIT:
@@ -354,8 +356,8 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       s->misfit_kind = (int)v;
       return GJW_OK;
     case GJW_OPT_ADJ_FUSE:
-      if (v != 0 && v != 4 && v != 8)
-        return fail(GJW_ERR_ARG, "GJW_OPT_ADJ_FUSE must be 0 (four), 4 or 8");
+      if (v != 0 && v != 4 && v != 6 && v != 8)
+        return fail(GJW_ERR_ARG, "GJW_OPT_ADJ_FUSE must be 0 (six), 4, 6 or 8");
       s->adj_fuse = (int)v;
       return GJW_OK;
     case GJW_OPT_GRAPH:
@@ -368,7 +370,7 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
         return fail(GJW_ERR_ARG, "GJW_OPT_FUSE must be 0 (autotuned), 1, 2, 4 or 8");
       s->fuse = (int)v;
       s->tuned_L = 0;
-      s->tuned_La2 = s->tuned_La4 = s->tuned_La8 = 0;
+      s->tuned_La2 = s->tuned_La4 = s->tuned_La6 = s->tuned_La8 = 0;
       s->fuse_eff = 0;
       break;
     case GJW_OPT_KTIMING:
@@ -803,7 +805,7 @@ static int kev_begin(gjw_survey* s, int which, int it, int* slot) {
   if (s->ksample <= 0) return GJW_OK;
   (void)it;
   // time one launch of this kernel in every ksample steps it covers
-  const int steps = which >= 6 ? 8 : which >= 4 ? 4 : which >= 2 ? 2 : 1;
+  const int steps = which == 8 ? 6 : which >= 6 ? 8 : which >= 4 ? 4 : which >= 2 ? 2 : 1;
   const long long every = std::max(1, s->ksample / steps);
   if (s->kl_n[which]++ % every != 0) return GJW_OK;
   const size_t need = 2 * (s->kev_used.size() + 1);
@@ -934,7 +936,7 @@ static int adjN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& 
                        int L, AdjBufs& b, int cur, int it, const float* adj, const float* Fa,
                        long long fstride, float* acc, long long astride, float* gw) {
   int slot;
-  TRY(kev_begin(s, NS == 8 ? 7 : NS == 4 ? 5 : 3, it, &slot));
+  TRY(kev_begin(s, NS == 8 ? 7 : NS == 6 ? 8 : NS == 4 ? 5 : 3, it, &slot));
   const dim3 grid = stepN_grid(s, NS, L, nsh), block(64, 4);
 #define GJW_ADJN_CALL(K)                                                                   \
   hipLaunchKernelGGL(K, grid, block, 0, s->ctx->stream, g, sh, L, b.M[cur], b.Nx[cur],    \
@@ -943,6 +945,8 @@ static int adjN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& 
                      astride, gw)
   if (NS == 8)
     GJW_ADJN_CALL(k_adj_step8);
+  else if (NS == 6)
+    GJW_ADJN_CALL(k_adj_step6);
   else if (NS == 4)
     GJW_ADJN_CALL(k_adj_step4);
   else
@@ -1129,11 +1133,22 @@ static int ck_restore(gjw_survey* s, FwdBufs& b, int cur, int xc, int c) {
   return GJW_OK;
 }
 
+// adjoint steps per launch: six by default (GJW_OPT_ADJ_FUSE: 4, 6, or 8 when
+// the forward fuses at least as many), at most the forward's depth when it
+// is shallower than four
+static int adj_depth(const gjw_survey* s) {
+  if (s->fuse_eff > 0 && s->fuse_eff < 4) return s->fuse_eff;
+  if (s->adj_fuse == 8) return s->fuse_eff >= 8 ? 8 : 6;
+  return s->adj_fuse == 4 ? 4 : 6;
+}
+
 static int ck_interval(const gjw_survey* s) {
   if (s->ck_batch > 0) return s->ck_batch;  // common interval of a survey batch
   if (s->ck_every > 0) return s->ck_every;
-  // about sqrt(nt) steps, rounded up to whole launches of the forward depth
-  const int q = s->fuse_eff >= 8 ? 8 : 4;
+  // about sqrt(nt) steps, rounded up to whole launches of the forward and
+  // the adjoint depths
+  const int qf = s->fuse_eff >= 8 ? 8 : 4, qa = adj_depth(s);
+  const int q = qf * qa / std::gcd(qf, qa);
   const int k = std::max(1, (int)std::ceil(std::sqrt((double)s->gr.nt)));
   return std::min(s->gr.nt, (k + q - 1) / q * q);
 }
@@ -1377,9 +1392,9 @@ static int adj_init(gjw_survey* s, int nsh, int q0, const float* adj) {
   return GJW_OK;
 }
 
-// candidate tile widths of the four-step adjoint, tried in turn by the first
-// gradient sweep (kAdjRounds launches each, the first round a warm-up)
-constexpr int kAdjCand[] = {12, 16, 24, 32, 48};
+// candidate tile widths of the deepest adjoint kernel, tried in turn by the
+// first gradient sweep (kAdjRounds launches each, the first round a warm-up)
+constexpr int kAdjCand4[] = {12, 16, 24, 32, 48}, kAdjCand6[] = {16, 24, 32, 48, 64};
 constexpr int kAdjNCand = 5, kAdjRounds = 3;
 
 static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
@@ -1396,14 +1411,12 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
   AdjBufs ab = adj_bufs(s, ns);
   const long long fstride = (long long)(K + 1) * FD;  // frames per shot
   int acur = 0;
-  // Adjoint depth: four steps per launch, or eight with GJW_OPT_ADJ_FUSE = 8
-  // (when the forward fuses at least as many).
-  const int dmax = s->adj_fuse == 8 && s->fuse_eff >= 8 ? 8 : 4;
-  int& tuned_deep = dmax == 8 ? s->tuned_La8 : s->tuned_La4;
+  const int dmax = adj_depth(s);
+  int& tuned_deep = dmax == 8 ? s->tuned_La8 : dmax == 6 ? s->tuned_La6 : s->tuned_La4;
   // Adjoint tile width, tuned in context: the first (uncaptured) sweep gives
   // its first launches of the deepest kernel the candidate widths in turn,
   // timed with events; later sweeps use the fastest.
-  if (s->tile_L > 0) s->tuned_La2 = s->tuned_La4 = s->tuned_La8 = s->tile_L;
+  if (s->tile_L > 0) s->tuned_La2 = s->tuned_La4 = s->tuned_La6 = s->tuned_La8 = s->tile_L;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   HIPCHK(hipStreamIsCapturing(st, &cap));
   const bool explore = tuned_deep <= 0 && cap == hipStreamCaptureStatusNone;
@@ -1431,14 +1444,17 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
     }
     for (int it = k1 - 1; it >= k0;) {
       const float* F = s->frames.p + (long long)(it - k0) * FD;  // Ez(S_it)
-      const int NS = (dmax == 8 && it - 7 >= k0)                ? 8
+      const int NS = (dmax >= 6 && it - (dmax - 1) >= k0)       ? dmax
                      : (s->fuse_eff >= 4 && it - 3 >= k0) ? 4
                      : (s->fuse_eff >= 2 && it - 1 >= k0) ? 2
                                                           : 1;
       if (NS > 1) {
-        int L = NS == 8 ? s->tuned_La8 : NS == 4 ? s->tuned_La4 : s->tuned_La2;
+        int L = NS == 8   ? s->tuned_La8
+                : NS == 6 ? s->tuned_La6
+                : NS == 4 ? s->tuned_La4
+                          : s->tuned_La2;
         const bool trial = NS == dmax && ex < nx;
-        if (trial) L = kAdjCand[ex % kAdjNCand];
+        if (trial) L = (dmax >= 6 ? kAdjCand6 : kAdjCand4)[ex % kAdjNCand];
         if (L <= 0) L = La_fallback;
         if (trial) HIPCHK(hipEventRecord(s->ex_ev[2 * ex], st));
         TRY(adjN_launch(s, NS, g, sh, ns, L, ab, acur, it, adj, F + FD, fstride, s->acc.p, FD,
@@ -1473,12 +1489,12 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
         }
         if (n > 0 && tot / n < best) {
           best = tot / n;
-          bestL = kAdjCand[c];
+          bestL = (dmax >= 6 ? kAdjCand6 : kAdjCand4)[c];
         }
       }
     }
-    if (dmax == 8)
-      s->tuned_La8 = bestL;
+    if (dmax >= 6)
+      tuned_deep = bestL;
     else
       s->tuned_La2 = s->tuned_La4 = bestL;
   }
@@ -1799,7 +1815,8 @@ static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
           (uintptr_t)s->gr.nt, (uintptr_t)s->tuned_L, (uintptr_t)s->tuned_L1,
           (uintptr_t)s->tuned_L4, (uintptr_t)s->tuned_L8, (uintptr_t)s->fuse_eff,
           (uintptr_t)s->tuned_La2,
-          (uintptr_t)s->tuned_La4, (uintptr_t)s->tuned_La8, (uintptr_t)s->adj_fuse,
+          (uintptr_t)s->tuned_La4, (uintptr_t)s->tuned_La6, (uintptr_t)s->tuned_La8,
+          (uintptr_t)s->adj_fuse,
           (uintptr_t)s->ck_K, (uintptr_t)s->fuse, (uintptr_t)s->snap,
           (uintptr_t)s->rrow_min, (uintptr_t)s->rrow_max, (uintptr_t)s->pml_idx.size(),
           fbits(s->bx0), fbits(s->by0)};
@@ -2229,7 +2246,8 @@ extern "C" int gjw_kernel_stats(gjw_survey* s, int32_t which, double* avg_ms, in
 extern "C" int gjw_tile(gjw_survey* s, int32_t which, int32_t* L) {
   if (!s || !L || which < 0 || which >= kKernKinds) return fail(GJW_ERR_ARG, "gjw_tile: bad argument");
   const int t[kKernKinds] = {s->tuned_L1, s->tuned_L1, s->tuned_L,  s->tuned_La2,
-                             s->tuned_L4, s->tuned_La4, s->tuned_L8, s->tuned_La8};
+                             s->tuned_L4, s->tuned_La4, s->tuned_L8, s->tuned_La8,
+                             s->tuned_La6};
   *L = t[which];
   return GJW_OK;
 }

@@ -426,6 +426,9 @@ __device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned off_
 #ifndef GJW_ADJM_WAVES  // multi-survey adjoint (carries the survey table too)
 #define GJW_ADJM_WAVES 3
 #endif
+#ifndef GJW_ADJ6_WAVES
+#define GJW_ADJ6_WAVES 3
+#endif
 #ifndef GJW_ADJ8_WAVES
 #define GJW_ADJ8_WAVES 2
 #endif
@@ -1290,7 +1293,7 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
                                           float* __restrict__ acc, long long astride,
                                           float* __restrict__ gw) {
   constexpr int R = 64 - 2 * NS;
-  constexpr int UA = NS >= 8 ? 1 : NS == 4 ? 2 : 4;  // columns per chunk
+  constexpr int UA = NS >= 6 ? 1 : NS == 4 ? 2 : 4;  // columns per chunk
   const int lane = threadIdx.x;
   const int NX = g.NX, NY = g.NY, P = g.P;
   const long long LD = g.LD;
@@ -1804,6 +1807,18 @@ k_adj_step4(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             const float* __restrict__ Fa, long long fstride, float* __restrict__ acc,
             long long astride, float* __restrict__ gw) {
   adjN_body<4>(g, s, L, vtile_of_block(g.NX, g.NY, L, 56), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
+               Y2, it, adj, Fa, fstride, acc, astride, gw);
+}
+
+__global__ void __launch_bounds__(256, GJW_ADJ6_WAVES)
+k_adj_step6(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
+            const float* __restrict__ Nx0, const float* __restrict__ Ny0,
+            const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ M2,
+            float* __restrict__ Nx2, float* __restrict__ Ny2, float* __restrict__ X2,
+            float* __restrict__ Y2, int it, const float* __restrict__ adj,
+            const float* __restrict__ Fa, long long fstride, float* __restrict__ acc,
+            long long astride, float* __restrict__ gw) {
+  adjN_body<6>(g, s, L, vtile_of_block(g.NX, g.NY, L, 52), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
                Y2, it, adj, Fa, fstride, acc, astride, gw);
 }
 

@@ -98,7 +98,7 @@ int gjw_set_shots(gjw_survey *s, int32_t ns, const int32_t *src, const float *wa
                                 8; the adjoint sweep fuses as many, at most 4 */
 #define GJW_OPT_MISFIT 8     /* misfit of gjw_fwi_gradient: GJW_MISFIT_* (batched path) */
 #define GJW_OPT_ADJ_FUSE 9   /* time steps fused per adjoint launch on the batched path:
-                                0 (default: four), 4, or 8 (with an eight-step forward) */
+                                0 (default: six), 4, 6, or 8 (with an eight-step forward) */
 #define GJW_OPT_GRAPH 7      /* 1: gjw_fwi_gradient (batched path) replays its whole launch
                                 sequence as one captured hipGraph (captured on the second call
                                 and again whenever a buffer or option it uses changes) */
@@ -186,7 +186,8 @@ int gjw_last_timing(gjw_survey *s, double *ms, double *cell_updates);
 /* Average device duration (ms) and count of the sampled step kernels, and the
  * number of launches of that kernel, while GJW_OPT_KTIMING is on.  which: 0 = one-step forward, 1 = one-step adjoint,
  * 2 = two-step forward, 3 = two-step adjoint, 4 = four-step forward,
- * 5 = four-step adjoint, 6 = eight-step forward, 7 = eight-step adjoint.  Has no
+ * 5 = four-step adjoint, 6 = eight-step forward, 7 = eight-step adjoint,
+ * 8 = six-step adjoint.  Has no
  * counterpart in the reference (measurement only). */
 int gjw_kernel_stats(gjw_survey *s, int32_t which, double *avg_ms, int64_t *count,
                      int64_t *launches, int32_t reset);

@@ -251,11 +251,13 @@ def test_autograd_function(oracle_mod, gpu_ctx):
         assert abs(fd - ge[a, b]) <= 0.05 * abs(ge[a, b])
 
 
-@pytest.mark.parametrize("fuse,ck", [(2, 7), (2, 10), (4, 7), (4, 10), (8, 10), (8, 21)])
-def test_two_step_kernel_matches_one_step(gpu_ctx, ck, fuse):
+@pytest.mark.parametrize("fuse,ck,adj_fuse", [(2, 7, 0), (2, 10, 0), (4, 7, 4), (4, 10, 4), (4, 10, 0),
+                                              (8, 10, 8), (8, 21, 8), (8, 21, 6), (8, 10, 6)])
+def test_two_step_kernel_matches_one_step(gpu_ctx, ck, fuse, adj_fuse):
     """The temporally blocked forward (two, four or eight steps per launch,
-    overlapped row tiles) equals the one-step kernel: traces and gradient, odd
-    nt, odd and even checkpoint intervals, several tile widths."""
+    overlapped row tiles) and adjoint (four, six or eight) equal the one-step
+    kernels: traces and gradient, odd nt, odd and even checkpoint intervals,
+    several tile widths."""
     from gerjoii_amd import _native as N
     case = configs.small(m=130, n=150, P=12, nt=241, ns=2)
     rng = np.random.default_rng(4)
@@ -267,8 +269,7 @@ def test_two_step_kernel_matches_one_step(gpu_ctx, ck, fuse):
     for L in (5, 16, 64):
         sv = make_survey(case, gpu_ctx, ck=ck, L=L)
         sv.set_option(N.GJW_OPT_FUSE, fuse)
-        if fuse == 8:
-            sv.set_option(N.GJW_OPT_ADJ_FUSE, 8)
+        sv.set_option(N.GJW_OPT_ADJ_FUSE, adj_fuse)
         d2 = sv.forward(keep=True)
         g2 = sv.gradient(adj)
         print("L", L, rel_l2(d2, d1), rel_l2(g2[0], g1[0]), rel_l2(g2[1], g1[1]))
@@ -298,15 +299,15 @@ def test_borehole_forward_and_gradient(oracle_mod, gpu_ctx):
     adj = rng.standard_normal(ref.shape).astype(np.float32)
     rp = np.concatenate([[0], np.cumsum([len(r) for r in case.receivers])])
     fe, fs, _ = oracle_grad(oracle_mod, case, case.eps0, case.sig0, adj, rp)
-    for fuse in (1, 2, 4, 8):
+    for fuse, adj_fuse in ((1, 0), (2, 0), (4, 4), (4, 6), (8, 6), (8, 8)):
         for L in (8, 32):
             sv = make_survey(case, gpu_ctx, ck=13, L=L)
             sv.set_option(N.GJW_OPT_FUSE, fuse)
-            if fuse == 8:
-                sv.set_option(N.GJW_OPT_ADJ_FUSE, 8)
+            sv.set_option(N.GJW_OPT_ADJ_FUSE, adj_fuse)
             d = sv.forward(keep=True)
             ge, gs = sv.gradient(adj)
-            print("fuse", fuse, "L", L, "traces", rel_l2(d, ref), "eps", rel_l2(ge, fe), "sig", rel_l2(gs, fs))
+            print("fuse", fuse, adj_fuse, "L", L, "traces", rel_l2(d, ref), "eps", rel_l2(ge, fe),
+                  "sig", rel_l2(gs, fs))
             assert rel_l2(d, ref) <= 1e-4
             assert rel_l2(ge, fe) <= 1e-3 and cosine(ge, fe) >= 0.9999
             assert rel_l2(gs, fs) <= 1e-3
@@ -406,11 +407,11 @@ def test_adjoint_width_tuned_in_context(gpu_ctx):
     dobs = torch.from_numpy(svt.forward()).cuda()
     ref = make_survey(case, gpu_ctx, L=16).fwi_gradient(dobs)
     sv = make_survey(case, gpu_ctx)
-    assert sv.tile(5) == 0
+    assert sv.tile(8) == 0  # six-step adjoint, the default depth
     for call in range(2):
         ge, gs, J = sv.fwi_gradient(dobs)
-        print("call", call, "adjoint L", sv.tile(5), rel_l2(ge, ref[0]), rel_l2(gs, ref[1]))
-        assert sv.tile(5) in (12, 16, 24, 32, 48)
+        print("call", call, "adjoint L", sv.tile(8), rel_l2(ge, ref[0]), rel_l2(gs, ref[1]))
+        assert sv.tile(8) in (16, 24, 32, 48, 64)
         assert rel_l2(ge, ref[0]) <= 1e-5 and rel_l2(gs, ref[1]) <= 1e-5
         assert abs(J - ref[2]) <= 1e-6 * abs(ref[2])
 
