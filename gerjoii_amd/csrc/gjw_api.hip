@@ -987,7 +987,7 @@ static int autotune_L(gjw_survey* s, int nsh) {
   float best_of[9] = {};
   for (int NS : {1, 2, 4, 8}) {
     const int c1[] = {8, 16, 32, 64}, c2[] = {12, 16, 24, 32}, c4[] = {16, 24, 32, 48},
-              c8[] = {16, 24, 32, 48};
+              c8[] = {32, 48, 64, 96};
     const int* cand = NS == 1 ? c1 : NS == 2 ? c2 : NS == 4 ? c4 : c8;
     float best = 1e30f;
     int bestL = 16;
@@ -1394,7 +1394,7 @@ static int adj_init(gjw_survey* s, int nsh, int q0, const float* adj) {
 
 // candidate tile widths of the deepest adjoint kernel, tried in turn by the
 // first gradient sweep (kAdjRounds launches each, the first round a warm-up)
-constexpr int kAdjCand4[] = {12, 16, 24, 32, 48}, kAdjCand6[] = {16, 24, 32, 48, 64};
+constexpr int kAdjCand4[] = {12, 16, 24, 32, 48}, kAdjCand6[] = {32, 48, 64, 96, 128};
 constexpr int kAdjNCand = 5, kAdjRounds = 3;
 
 static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
