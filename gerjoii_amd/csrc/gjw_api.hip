@@ -987,7 +987,7 @@ static int autotune_L(gjw_survey* s, int nsh) {
   float best_of[9] = {};
   for (int NS : {1, 2, 4, 8}) {
     const int c1[] = {8, 16, 32, 64}, c2[] = {12, 16, 24, 32}, c4[] = {16, 24, 32, 48},
-              c8[] = {32, 48, 64, 96};
+              c8[] = {48, 64, 96, 128};
     const int* cand = NS == 1 ? c1 : NS == 2 ? c2 : NS == 4 ? c4 : c8;
     float best = 1e30f;
     int bestL = 16;
