@@ -395,6 +395,36 @@ def test_fwi_gradient_graph_replays(gpu_ctx, snap, misfit):
         print("graph replays ok, J", J0)
 
 
+@pytest.mark.parametrize("kind", ["duplicate_receivers", "tiny_grid"])
+def test_receiver_fallback_and_tiny_grids(oracle_mod, gpu_ctx, kind):
+    """Two receivers of one shot in one cell (the kernels' per-receiver
+    loop instead of the per-lane lookup), and a grid smaller than one row
+    tile of the deepest kernels: traces and gradient against the FP64 oracle
+    with the eight-step forward and the six- and eight-step adjoints."""
+    from gerjoii_amd import _native as N
+    if kind == "duplicate_receivers":
+        case = configs.small(nt=240, ns=2)
+        r = case.receivers[0]
+        case.receivers = [np.concatenate([r, r[3:5]]), case.receivers[1]]
+    else:
+        case = configs.small(m=12, n=30, P=6, nt=200, ns=2, nr=5)
+    ref = oracle_forward(oracle_mod, case, case.eps0, case.sig0)
+    rng = np.random.default_rng(9)
+    adj = rng.standard_normal(ref.shape).astype(np.float32)
+    rp = np.concatenate([[0], np.cumsum([len(r) for r in case.receivers])])
+    fe, fs, _ = oracle_grad(oracle_mod, case, case.eps0, case.sig0, adj, rp)
+    for adj_fuse in (6, 8):
+        sv = make_survey(case, gpu_ctx, ck=16, L=8)
+        sv.set_option(N.GJW_OPT_FUSE, 8)
+        sv.set_option(N.GJW_OPT_ADJ_FUSE, adj_fuse)
+        d = sv.forward(keep=True)
+        ge, gs = sv.gradient(adj)
+        print(kind, adj_fuse, "traces", rel_l2(d, ref), "eps", rel_l2(ge, fe), "sig", rel_l2(gs, fs))
+        assert rel_l2(d, ref) <= 1e-4
+        assert rel_l2(ge, fe) <= 1e-3 and cosine(ge, fe) >= 0.9999
+        assert rel_l2(gs, fs) <= 1e-3
+
+
 def test_adjoint_width_tuned_in_context(gpu_ctx):
     """The first gradient sweep of an autotuned survey gives its four-step
     adjoint launches the candidate tile widths in turn and keeps the fastest;
