@@ -31,7 +31,7 @@ EXPORTS = ["gjw_create", "gjw_destroy", "gjw_last_error", "gjw_version", "gjw_su
            "gjw_forward", "gjw_misfit", "gjw_bozdag", "gjw_gradient", "gjw_fwi_gradient",
            "gjw_differentiate_cube", "gjw_last_timing", "gjw_kernel_stats", "gjw_tile",
            "gjw_batch_create", "gjw_batch_destroy", "gjw_batch_fwi_gradient",
-           "gjw_batch_last_timing"]
+           "gjw_batch_last_timing", "gjw_set_adj_filter", "gjw_filt_gauss", "gjw_lmute"]
 
 
 class GjwGrid(C.Structure):
@@ -79,6 +79,9 @@ def load():
     lib.gjw_fwi_gradient.argtypes = [vp, vp, vp, vp, vp]
     lib.gjw_differentiate_cube.argtypes = [vp, vp, i32, i32, i32, C.c_double]
     lib.gjw_last_timing.argtypes = [vp, vp, vp]
+    lib.gjw_set_adj_filter.argtypes = [vp, C.c_double, C.c_double, i32]
+    lib.gjw_filt_gauss.argtypes = [vp, vp, i32, i32, C.c_double, C.c_double, C.c_double, i32]
+    lib.gjw_lmute.argtypes = [vp, vp, vp, vp, i32, i32, vp, vp, C.c_double, C.c_double]
     lib.gjw_kernel_stats.argtypes = [vp, i32, vp, vp, vp, i32]
     lib.gjw_tile.argtypes = [vp, C.c_int32, vp]
     for name in EXPORTS:

@@ -93,6 +93,11 @@ __global__ void k_misfit_stage1(const float*, const float2*, const float*, long 
 __global__ void k_misfit_stage2(const float*, const float2*, long long, int, int, float*);
 __global__ void k_reduce_d(const double*, int, double, double*);
 __global__ void k_dcube(float*, long long, int, double);
+__global__ void k_tukey_pad(const float*, int, int, long long, double, float*);
+__global__ void k_gauss_band(float2*, int, long long, double, double, double, int);
+__global__ void k_trim(const float*, int, int, long long, float*);
+__global__ void k_lmute(float*, float*, float*, int, long long, const double*, const double*, double,
+                        double);
 }  // namespace gjw
 
 using namespace gjw;
@@ -129,10 +134,14 @@ static int fail(int code, const char* fmt, ...) {
 // --------------------------------------------------------------------------
 // objects
 // --------------------------------------------------------------------------
+struct GaussFilter;  // band filter of traces (filt_gauss.m), defined with the FFT code
+static void filter_free(GaussFilter* f);
+
 struct gjw_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  GaussFilter* flt = nullptr;  // gjw_filt_gauss's plans and buffers
 };
 
 template <class T>
@@ -187,6 +196,11 @@ struct gjw_survey {
   int tuned_L = 0, tuned_L1 = 0, tuned_L4 = 0, tuned_L8 = 0;  // 2-, 1-, 4-, 8-step forward
   int tuned_La2 = 0, tuned_La4 = 0, tuned_La6 = 0, tuned_La8 = 0;  // 2-, 4-, 6-, 8-step adjoint
   int adj_fuse = 0;  // adjoint steps per launch (GJW_OPT_ADJ_FUSE; 0: six)
+  // adjoint-source band filter of gjw_fwi_gradient (w_update_e.m:69-70); off
+  // while flt_hi <= flt_lo
+  double flt_lo = 0, flt_hi = 0;
+  int flt_steep = 8;
+  GaussFilter* flt = nullptr;
   std::vector<hipEvent_t> ex_ev;                // adjoint tile-width trials of the first sweep
   int fuse_eff = 0;  // forward steps per launch in use (fuse, or the autotuner's choice)
   // model, padded (NX x NY, column-major, FP64)
@@ -277,6 +291,7 @@ extern "C" int gjw_destroy(gjw_ctx* c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  filter_free(c->flt);
   delete c;
   return GJW_OK;
 }
@@ -312,6 +327,7 @@ extern "C" int gjw_survey_destroy(gjw_survey* s) {
   (void)hipStreamSynchronize(s->ctx->stream);
   for (auto e : s->kev_pool) (void)hipEventDestroy(e);
   for (auto e : s->ex_ev) (void)hipEventDestroy(e);
+  filter_free(s->flt);
   if (s->fft_r2c) (void)hipfftDestroy(s->fft_r2c);
   if (s->fft_c2c) (void)hipfftDestroy(s->fft_c2c);
   s->fhalf.release(); s->fana.release(); s->fin.release(); s->fov.release(); s->fP.release();
@@ -1263,6 +1279,81 @@ extern "C" int gjw_forward(gjw_survey* s, float* traces, float* jz, int32_t keep
     if (r_ != HIPFFT_SUCCESS) return fail(GJW_ERR_HIP, "hipfft %d at %s:%d", (int)r_, __FILE__, __LINE__); \
   } while (0)
 
+// --------------------------------------------------------------------------
+// band filter of traces (filt_gauss.m:16-47): Tukey taper (r = 0.1), zero
+// padding to the next power of two N, one-sided super-Gaussian band in the
+// frequency domain, real part of the inverse transform, trim to nt.  Traces
+// are filtered in place, trace-major.
+// --------------------------------------------------------------------------
+struct GaussFilter {
+  hipfftHandle r2c = 0, c2r = 0;
+  int N = 0, ntr = 0;
+  DBuf<float> pad;
+  DBuf<float2> spec;
+  void release() {
+    if (r2c) (void)hipfftDestroy(r2c);
+    if (c2r) (void)hipfftDestroy(c2r);
+    r2c = c2r = 0;
+    N = ntr = 0;
+    pad.release();
+    spec.release();
+  }
+  int run(hipStream_t st, float* d, int ntr_, int nt, double dt, double f_low, double f_high,
+          int steep) {
+    if (ntr_ <= 0) return GJW_OK;
+    int n2 = 1;
+    while (n2 < nt) n2 *= 2;  // nextpow2
+    if (!r2c || N != n2 || ntr != ntr_) {
+      if (r2c) FFTCHK(hipfftDestroy(r2c));
+      if (c2r) FFTCHK(hipfftDestroy(c2r));
+      r2c = c2r = 0;
+      int n1 = n2;
+      FFTCHK(hipfftPlanMany(&r2c, 1, &n1, nullptr, 1, n2, nullptr, 1, n2 / 2 + 1, HIPFFT_R2C,
+                            ntr_));
+      FFTCHK(hipfftPlanMany(&c2r, 1, &n1, nullptr, 1, n2 / 2 + 1, nullptr, 1, n2, HIPFFT_C2R,
+                            ntr_));
+      N = n2;
+      ntr = ntr_;
+    }
+    TRY(pad.ensure((size_t)ntr_ * N));
+    TRY(spec.ensure((size_t)ntr_ * (N / 2 + 1)));
+    FFTCHK(hipfftSetStream(r2c, st));
+    FFTCHK(hipfftSetStream(c2r, st));
+    const long long np = (long long)ntr_ * N;
+    const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(np, 256), 1), 8192);
+    hipLaunchKernelGGL(k_tukey_pad, dim3(blocks), dim3(256), 0, st, d, nt, N, (long long)ntr_, 0.1,
+                       pad.p);
+    HIPCHK(hipGetLastError());
+    FFTCHK(hipfftExecR2C(r2c, reinterpret_cast<hipfftReal*>(pad.p),
+                         reinterpret_cast<hipfftComplex*>(spec.p)));
+    const double df = 1.0 / dt / N, width = f_high - f_low, mid = f_low + 0.5 * width;
+    hipLaunchKernelGGL(k_gauss_band, dim3(blocks), dim3(256), 0, st, spec.p, N, (long long)ntr_,
+                       df, mid, width, steep);
+    HIPCHK(hipGetLastError());
+    FFTCHK(hipfftExecC2R(c2r, reinterpret_cast<hipfftComplex*>(spec.p),
+                         reinterpret_cast<hipfftReal*>(pad.p)));
+    hipLaunchKernelGGL(k_trim, dim3(blocks), dim3(256), 0, st, pad.p, N, nt, (long long)ntr_, d);
+    HIPCHK(hipGetLastError());
+    return GJW_OK;
+  }
+};
+
+static void filter_free(GaussFilter* f) {
+  if (!f) return;
+  f->release();
+  delete f;
+}
+
+static bool filter_on(const gjw_survey* s) { return s->flt_hi > s->flt_lo; }
+
+// band filter of the adjoint sources of traces [t0, t0+cnt) (w_update_e.m:69-70)
+static int adj_filter(gjw_survey* s, long long t0, long long cnt) {
+  if (!filter_on(s) || cnt <= 0) return GJW_OK;
+  if (!s->flt) s->flt = new GaussFilter();
+  return s->flt->run(s->ctx->stream, s->adj.p + t0 * s->gr.nt, (int)cnt, s->gr.nt, s->gr.dt,
+                     s->flt_lo, s->flt_hi, s->flt_steep);
+}
+
 // analytic signal (unnormalised: x nt) of the ntr traces in s->fin, into s->fana
 static int hilbert_dev(gjw_survey* s) {
   const int nt = s->gr.nt, ntr = (int)s->ntr;
@@ -1567,6 +1658,7 @@ static int fwi_plain(gjw_survey* s, const float* d_obs) {
                          s->adj.p + t0, cnt, s->part.p + (long long)q * B);
       HIPCHK(hipGetLastError());
     }
+    TRY(adj_filter(s, s->rec_ptr[q], s->rec_ptr[q + 1] - s->rec_ptr[q]));
     TRY(grad_plain_shot(s, q, s->adj.p, s->frames.p, nullptr));
   }
   hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, ns * B, 0.5, s->Jd.p);
@@ -1767,6 +1859,7 @@ static int fwi_adj_enqueue(gjw_survey* s) {
 static int fwi_batched_enqueue(gjw_survey* s) {
   TRY(fwi_fwd_enqueue(s));
   TRY(misfit_enqueue(s, s->misfit_kind));
+  TRY(adj_filter(s, 0, s->ntr));
   return fwi_adj_enqueue(s);
 }
 
@@ -1806,6 +1899,11 @@ static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
     std::memcpy(&b, &f, 4);
     return (uintptr_t)b;
   };
+  auto dbits = [](double f) {
+    uint64_t b;
+    std::memcpy(&b, &f, 8);
+    return (uintptr_t)b;
+  };
   return {u(s->fld.p),    u(s->tr.p),    u(s->ck.p),     u(s->adj.p),   u(s->obs.p),
           u(s->part.p),   u(s->Jd.p),    u(s->ge.p),     u(s->gsg.p),   u(s->frames.p),
           u(s->acc.p),    u(s->adjf.p),  u(s->coef.p),   u(s->fe.p),    u(s->fs.p),
@@ -1819,6 +1917,7 @@ static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
           (uintptr_t)s->adj_fuse,
           (uintptr_t)s->ck_K, (uintptr_t)s->fuse, (uintptr_t)s->snap,
           (uintptr_t)s->rrow_min, (uintptr_t)s->rrow_max, (uintptr_t)s->pml_idx.size(),
+          dbits(s->flt_lo), dbits(s->flt_hi), (uintptr_t)s->flt_steep,
           fbits(s->bx0), fbits(s->by0)};
 }
 
@@ -1827,10 +1926,12 @@ static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
 static int fwi_graph_run(gjw_survey* s) {
   hipStream_t st = s->ctx->stream;
   std::vector<uintptr_t> key = fwi_graph_key(s);
-  if (s->misfit_kind == GJW_MISFIT_L2)
+  if (s->misfit_kind == GJW_MISFIT_L2 && !filter_on(s))
     return graph_run(s->g_all, st, key, &s->ksample, [&] { return fwi_batched_enqueue(s); });
+  // FFT misfits and the adjoint-source filter run eagerly between the sweeps
   TRY(graph_run(s->g_fwd, st, key, &s->ksample, [&] { return fwi_fwd_enqueue(s); }));
   TRY(misfit_enqueue(s, s->misfit_kind));
+  TRY(adj_filter(s, 0, s->ntr));
   return graph_run(s->g_adj, st, key, &s->ksample, [&] { return fwi_adj_enqueue(s); });
 }
 
@@ -2038,9 +2139,14 @@ static int batch_fwd_enqueue(gjw_batch* b) {
   return GJW_OK;
 }
 
+// l2: the surveys whose misfit and adjoint source go into a captured graph
+// (L2 without the FFT filter), else the others
 static int batch_misfit_enqueue(gjw_batch* b, bool l2) {
   for (gjw_survey* s : b->sv)
-    if ((s->misfit_kind == GJW_MISFIT_L2) == l2) TRY(misfit_enqueue(s, s->misfit_kind));
+    if ((s->misfit_kind == GJW_MISFIT_L2 && !filter_on(s)) == l2) {
+      TRY(misfit_enqueue(s, s->misfit_kind));
+      TRY(adj_filter(s, 0, s->ntr));
+    }
   return GJW_OK;
 }
 
@@ -2143,7 +2249,7 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
       return fail(GJW_ERR_ARG, "gjw_batch_fwi_gradient: survey %d is on the plain path", k);
     T = std::max(T, s->gr.nt);
     graph &= s->graph_on;
-    all_l2 &= s->misfit_kind == GJW_MISFIT_L2;
+    all_l2 &= s->misfit_kind == GJW_MISFIT_L2 && !filter_on(s);
     K = std::max(K, s->ck_every);
   }
   if (K <= 0) K = std::max(1, (int)std::ceil(std::sqrt((double)T)));
@@ -2256,6 +2362,84 @@ extern "C" int gjw_last_timing(gjw_survey* s, double* ms, double* cells) {
   if (!s) return fail(GJW_ERR_ARG, "gjw_last_timing: NULL survey");
   if (ms) *ms = s->last_ms;
   if (cells) *cells = s->last_cells;
+  return GJW_OK;
+}
+
+extern "C" int gjw_set_adj_filter(gjw_survey* s, double f_low, double f_high, int32_t steep) {
+  if (!s) return fail(GJW_ERR_ARG, "gjw_set_adj_filter: NULL survey");
+  if (f_high > f_low && (steep < 1 || steep > 64))
+    return fail(GJW_ERR_ARG, "gjw_set_adj_filter: steep must be in 1..64");
+  s->flt_lo = f_low;
+  s->flt_hi = f_high;
+  s->flt_steep = steep;
+  return GJW_OK;
+}
+
+// device copy of a host or device array for an in-place operation: *dp is v
+// when v is device memory, else a fresh device copy (freed by stage_end)
+static int stage_begin(gjw_ctx* c, void* v, size_t bytes, void** dp, bool* own) {
+  hipPointerAttribute_t at;
+  const hipError_t pe = hipPointerGetAttributes(&at, v);
+  const bool on_dev = pe == hipSuccess && at.type == hipMemoryTypeDevice;
+  (void)hipGetLastError();
+  *own = !on_dev;
+  *dp = v;
+  if (on_dev) return GJW_OK;
+  HIPCHK(hipMalloc(dp, std::max<size_t>(bytes, 1)));
+  HIPCHK(hipMemcpyAsync(*dp, v, bytes, hipMemcpyDefault, c->stream));
+  return GJW_OK;
+}
+static int stage_end(gjw_ctx* c, void* v, void* dp, size_t bytes, bool own, bool back) {
+  if (own && back) HIPCHK(hipMemcpyAsync(v, dp, bytes, hipMemcpyDefault, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (own) HIPCHK(hipFree(dp));
+  return GJW_OK;
+}
+
+extern "C" int gjw_filt_gauss(gjw_ctx* c, float* d, int32_t ntr, int32_t nt, double dt,
+                              double f_low, double f_high, int32_t steep) {
+  if (!c || (!d && ntr > 0)) return fail(GJW_ERR_ARG, "gjw_filt_gauss: NULL argument");
+  if (ntr < 0 || nt < 2 || !(dt > 0) || !(f_high > f_low) || steep < 1 || steep > 64)
+    return fail(GJW_ERR_ARG, "gjw_filt_gauss: bad arguments");
+  if (ntr == 0) return GJW_OK;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t bytes = (size_t)ntr * nt * 4;
+  void* dp;
+  bool own;
+  TRY(stage_begin(c, d, bytes, &dp, &own));
+  if (!c->flt) c->flt = new GaussFilter();
+  const int rc = c->flt->run(c->stream, static_cast<float*>(dp), ntr, nt, dt, f_low, f_high, steep);
+  TRY(stage_end(c, d, dp, bytes, own, rc == GJW_OK));
+  return rc;
+}
+
+extern "C" int gjw_lmute(gjw_ctx* c, float* d, float* d_o, float* e, int32_t ntr, int32_t nt,
+                         const double* t, const double* rx, double v_mute, double t_mute) {
+  if (!c || !d || !d_o || !t || !rx) return fail(GJW_ERR_ARG, "gjw_lmute: NULL argument");
+  if (ntr < 1 || nt < 1 || !(v_mute != 0)) return fail(GJW_ERR_ARG, "gjw_lmute: bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t bytes = (size_t)ntr * nt * 4;
+  void *dd, *dob, *de = nullptr, *dt_, *drx;
+  bool od, oo, oe = false, ot, orx;
+  TRY(stage_begin(c, d, bytes, &dd, &od));
+  TRY(stage_begin(c, d_o, bytes, &dob, &oo));
+  if (e) TRY(stage_begin(c, e, bytes, &de, &oe));
+  TRY(stage_begin(c, const_cast<double*>(t), (size_t)nt * 8, &dt_, &ot));
+  TRY(stage_begin(c, const_cast<double*>(rx), (size_t)ntr * 8, &drx, &orx));
+  const long long n = (long long)ntr * nt;
+  const int blocks = (int)std::min<long long>(cdiv(n, 256), 8192);
+  // m/ns -> m/s, ns -> s (w_lmute.m:22-23)
+  hipLaunchKernelGGL(k_lmute, dim3(blocks), dim3(256), 0, c->stream, static_cast<float*>(dd),
+                     static_cast<float*>(dob), static_cast<float*>(de), nt, (long long)ntr,
+                     static_cast<const double*>(dt_), static_cast<const double*>(drx),
+                     v_mute * 1e9, t_mute * 1e-9);
+  const bool ok = hipGetLastError() == hipSuccess;
+  TRY(stage_end(c, d, dd, bytes, od, ok));
+  TRY(stage_end(c, d_o, dob, bytes, oo, ok));
+  if (e) TRY(stage_end(c, e, de, bytes, oe, ok));
+  TRY(stage_end(c, const_cast<double*>(t), dt_, (size_t)nt * 8, ot, false));
+  TRY(stage_end(c, const_cast<double*>(rx), drx, (size_t)ntr * 8, orx, false));
+  if (!ok) return fail(GJW_ERR_HIP, "gjw_lmute: launch failed");
   return GJW_OK;
 }
 

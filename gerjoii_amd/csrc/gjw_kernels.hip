@@ -2212,4 +2212,99 @@ __global__ void k_dcube(float* __restrict__ v, long long fr, int nt, double dt) 
   v[(long long)(nt - 1) * fr + p] = (float)(-c * u_km2 - d * u_km1 - a * u_k);
 }
 
+// ==========================================================================
+// adjoint-source conditioning: band filter (filt_gauss.m) and linear mute
+// (w_lmute.m).  Traces are trace-major, trace r at d + r*nt (the layout of a
+// MATLAB nt x nr array).
+// ==========================================================================
+// Tukey window of n samples with taper fraction r at sample k (tukeywin.m:
+// raised-cosine flanks over floor(r/2*(n-1))+1 samples at each end)
+__device__ __forceinline__ double tukey_w(int k, int n, double r) {
+  if (n <= 1 || r <= 0.0) return 1.0;
+  const double per = 0.5 * r, x = (double)k / (double)(n - 1);
+  const int tl = (int)floor(per * (double)(n - 1));  // last 0-based sample of the left flank
+  if (k <= tl) return 0.5 * (1.0 + cos(M_PI / per * (x - per)));
+  if (k >= n - 1 - tl) return 0.5 * (1.0 + cos(M_PI / per * (x - 1.0 + per)));
+  return 1.0;
+}
+
+// pad[r][k] = d[r][k] * tukey(k) for k < nt, 0 up to N (filt_gauss.m:19-24)
+__global__ void k_tukey_pad(const float* __restrict__ d, int nt, int N, long long ntr, double r,
+                            float* __restrict__ pad) {
+  const long long n = ntr * N;
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < n;
+       p += (long long)gridDim.x * blockDim.x) {
+    const long long tr = p / N;
+    const int k = (int)(p - tr * N);
+    pad[p] = k < nt ? (float)((double)d[tr * nt + k] * tukey_w(k, nt, r)) : 0.f;
+  }
+}
+
+// integer power of a double (MATLAB's x.^steep for an integer steep)
+__device__ __forceinline__ double ipow(double x, int e) {
+  double y = 1.0;
+  for (int i = 0; i < e; ++i) y *= x;
+  return y;
+}
+
+// Band weight g(f) = exp(-((f - mid)^steep) / ((df/2) * width^(steep-1)))
+// (filt_gauss.m:32-35) on the frequencies of the FFT order (bin k: k*df for
+// k < N/2, (k-N)*df above).  The filtered trace is the real part of the
+// inverse transform, which is the inverse of the Hermitian part of the
+// product: bin k of the one-sided spectrum takes (g(f_k) + g(-f_k)) / 2
+// (bins 0 and N/2 their own weight), and the 1/N of the inverse.
+__global__ void k_gauss_band(float2* __restrict__ spec, int N, long long ntr, double df,
+                             double mid, double width, int steep) {
+  const int nb = N / 2 + 1;
+  const long long n = ntr * nb;
+  const double den = (0.5 * df) * ipow(width, steep - 1);
+  auto g = [&](double f) { return exp(-ipow(f - mid, steep) / den); };
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < n;
+       p += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(p % nb);
+    double w;
+    if (k == 0)
+      w = g(0.0);
+    else if (2 * k == N)
+      w = g(-0.5 * N * df);
+    else
+      w = 0.5 * (g(k * df) + g(-k * df));
+    w /= (double)N;
+    const float2 z = spec[p];
+    spec[p] = make_float2((float)(z.x * w), (float)(z.y * w));
+  }
+}
+
+// d[r][k] = pad[r][k], k < nt (filt_gauss.m:44)
+__global__ void k_trim(const float* __restrict__ pad, int N, int nt, long long ntr,
+                       float* __restrict__ d) {
+  const long long n = ntr * nt;
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < n;
+       p += (long long)gridDim.x * blockDim.x) {
+    const long long tr = p / nt;
+    d[p] = pad[tr * N + (p - tr * nt)];
+  }
+}
+
+// Sigmoid mute of the linear arrival t0 + x_r/v (w_lmute.m:27-41):
+// m(t, r) = 1 / (1 + exp(5 (t0 + (x_r - x_0)/v - t))); d and d_o are muted in
+// place and e = d - d_o.  Times in seconds, v in m/s.
+__global__ void k_lmute(float* __restrict__ d, float* __restrict__ d_o, float* __restrict__ e,
+                        int nt, long long ntr, const double* __restrict__ t,
+                        const double* __restrict__ rx, double v, double t0) {
+  const long long n = ntr * nt;
+  const double x0 = rx[0];
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < n;
+       p += (long long)gridDim.x * blockDim.x) {
+    const long long tr = p / nt;
+    const int k = (int)(p - tr * nt);
+    const double to = t0 + (rx[tr] - x0) / v;
+    const double m = 1.0 / (1.0 + exp(5.0 * (to - t[k])));
+    const float a = (float)(d[p] * m), b = (float)(d_o[p] * m);
+    d[p] = a;
+    d_o[p] = b;
+    if (e) e[p] = a - b;
+  }
+}
+
 }  // namespace gjw

@@ -122,6 +122,33 @@ class Context:
                 "gjw_differentiate_cube")
         return v
 
+    def filt_gauss(self, d, dt, f_low, f_high, steep=8):
+        """filt_gauss.m on the GPU, in place: d is (ntraces, nt) float32
+        (C-order numpy or torch cuda; trace-major like MATLAB's nt x nr)."""
+        assert str(d.dtype) in ("float32", "torch.float32")
+        ntr, nt = d.shape
+        if isinstance(d, np.ndarray):
+            assert d.flags.c_contiguous
+        _sync_torch(d)
+        N.check(N.load().gjw_filt_gauss(self.h, N.ptr(d), ntr, nt, float(dt), float(f_low),
+                                        float(f_high), int(steep)), "gjw_filt_gauss")
+        return d
+
+    def lmute(self, d, d_o, t, rx, v_mute, t_mute):
+        """w_lmute.m on the GPU: mutes d and d_o in place ((ntraces, nt)
+        float32, C-order) and returns e = d - d_o.  t [s], rx [m], v_mute
+        [m/ns], t_mute [ns]."""
+        ntr, nt = d.shape
+        for a in (d, d_o):
+            assert isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        rx = np.ascontiguousarray(rx, dtype=np.float64)
+        assert t.size == nt and rx.size == ntr
+        e = np.empty_like(d)
+        N.check(N.load().gjw_lmute(self.h, N.ptr(d), N.ptr(d_o), N.ptr(e), ntr, nt, N.ptr(t),
+                                   N.ptr(rx), float(v_mute), float(t_mute)), "gjw_lmute")
+        return e
+
 
 _default_ctx = {}
 
@@ -250,6 +277,12 @@ class Survey:
     def set_misfit(self, kind):
         """Misfit of fwi_gradient (batched path): an obj_FNC name as in misfit()."""
         self.set_option(N.GJW_OPT_MISFIT, N.MISFIT_KINDS[kind])
+
+    def set_adj_filter(self, f_low, f_high, steep=8):
+        """Band filter (filt_gauss.m) of the adjoint source in fwi_gradient,
+        as w_update_e.m:69-70 applies it; f_high <= f_low turns it off."""
+        N.check(N.load().gjw_set_adj_filter(self.h, float(f_low), float(f_high), int(steep)),
+                "gjw_set_adj_filter")
 
     def gradient(self, adj=None, mode=N.GJW_GRAD_EXACT, want_gw=False, g_eps=None, g_sig=None):
         """FP64 gradients (m, n) w.r.t. eps_r and sigma (and optionally the

@@ -160,6 +160,27 @@ int gjw_gradient(gjw_survey *s, const float *adj, int32_t mode, double *g_eps, d
  * (what w_update_e.m / w_update_s.m need per iteration, for all shots). */
 int gjw_fwi_gradient(gjw_survey *s, const float *d_obs, double *g_eps, double *g_sig, double *J);
 
+/* Band filter of the adjoint source that gjw_fwi_gradient applies before the
+ * adjoint sweep (src/w/w_update_e.m:69-70: filt_gauss(e, dt, f_, f__, 8)).
+ * f_high <= f_low turns it off (the default). */
+int gjw_set_adj_filter(gjw_survey *s, double f_low, double f_high, int32_t steep);
+
+/* Band filter of ntr traces of nt samples in place, trace-major (MATLAB's
+ * nt x nr array): Tukey taper (r = 0.1), zero padding to the next power of
+ * two, super-Gaussian band exp(-((f-mid)^steep)/((df/2) width^(steep-1))) on
+ * the signed frequencies, real part of the inverse FFT
+ * (src/shared/w-dc/filt_gauss.m:1-47; dt in s, frequencies in Hz). */
+int gjw_filt_gauss(gjw_ctx *ctx, float *d, int32_t ntr, int32_t nt, double dt, double f_low,
+                   double f_high, int32_t steep);
+
+/* Smooth mute of a linear arrival (src/w/w_lmute.m:13-41): d and d_o (ntr
+ * traces of nt samples, trace-major) are multiplied in place by
+ * 1/(1+exp(5 (t_mute + (rx - rx[0])/v_mute - t))) and e = d - d_o (e may be
+ * NULL).  t: nt times [s]; rx: ntr receiver x [m]; v_mute [m/ns], t_mute [ns]
+ * as in gerjoii_.w.v_mute / t_mute. */
+int gjw_lmute(gjw_ctx *ctx, float *d, float *d_o, float *e, int32_t ntr, int32_t nt,
+              const double *t, const double *rx, double v_mute, double t_mute);
+
 /* Time derivative of an m x n x nt cube in place, second order
  * (src/shared/w-dc/differentiate_cube.m:23-51). */
 int gjw_differentiate_cube(gjw_ctx *ctx, float *v, int32_t nz, int32_t nx, int32_t nt, double dt);

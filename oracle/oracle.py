@@ -12,7 +12,9 @@ Two layers:
   (w_grad.m), differentiate_cube.m and the exact discrete adjoint.
 * numpy restatements of the small host-side pieces: geometry (w_geom.m),
   the source wavelet (w_wavelet.m), index maps (w_load.m, w_M.m), the
-  misfits and adjoint sources (w_bozdag.m, instant_.m, w_E.m).
+  misfits and adjoint sources (w_bozdag.m, instant_.m, w_E.m), and the
+  adjoint-source conditioning (filt_gauss.m with MATLAB's tukeywin,
+  w_lmute.m).
 
 Arrays follow the MATLAB layout: column-major (Fortran order), depth first.
 """
@@ -417,3 +419,69 @@ def w_E(d_o, d, Nw=None, obj_fnc="L2"):
     if Nw is None:
         return 0.5 * float(e @ e)
     return 0.5 * float(e @ (Nw @ e))
+
+
+# --------------------------------------------------------------------------
+# adjoint-source conditioning: filt_gauss.m, tukeywin, w_lmute.m
+# --------------------------------------------------------------------------
+
+def tukeywin(n, r=0.5):
+    """MATLAB ``tukeywin(n, r)``: ones, with raised-cosine flanks over the
+    first and last floor(r/2*(n-1))+1 samples of x = linspace(0, 1, n);
+    r <= 0 gives ones, r >= 1 a Hann window."""
+    if n == 1:
+        return np.ones(1)
+    if r <= 0:
+        return np.ones(n)
+    r = min(r, 1.0)
+    x = np.linspace(0.0, 1.0, n)
+    per = r / 2.0
+    tl = int(np.floor(per * (n - 1))) + 1  # samples in each flank
+    w = np.ones(n)
+    w[:tl] = 0.5 * (1 + np.cos(np.pi / per * (x[:tl] - per)))
+    w[n - tl:] = 0.5 * (1 + np.cos(np.pi / per * (x[n - tl:] - 1 + per)))
+    return w
+
+
+def nextpow2(n):
+    """MATLAB ``2^nextpow2(n)``: the smallest power of two >= n."""
+    p = 1
+    while p < n:
+        p *= 2
+    return p
+
+
+def filt_gauss(d, dt, f_low, f_high, steep):
+    """filt_gauss.m:16-47 on d (nt x nr, MATLAB layout): Tukey taper
+    (r = 0.1), zero padding to 2^nextpow2(nt), FFT along time, the band
+    exp(-((f-mid)^steep) / ((df/2) width^(steep-1))) on f = (-N/2 .. N/2-1) df
+    brought to FFT order by fftshift, real part of the inverse FFT, trim."""
+    d = np.asarray(d, dtype=np.float64)
+    nt, nr = d.shape
+    d = d * tukeywin(nt, 0.1)[:, None]
+    N = nextpow2(nt)
+    dpad = np.zeros((N, nr))
+    dpad[:nt] = d
+    F = np.fft.fft(dpad, axis=0)
+    df = 1.0 / dt / N
+    f = np.arange(-N // 2, N // 2) * df
+    width = f_high - f_low
+    middle = f_low + width * 0.5
+    g = np.exp(-((f - middle) ** steep) / ((df * 0.5) * width ** (steep - 1)))
+    g = np.fft.fftshift(g)
+    out = np.real(np.fft.ifft(F * g[:, None], axis=0))
+    return out[:nt]
+
+
+def lmute(d, d_o, t, rx, v_mute, t_mute):
+    """w_lmute.m:13-41 (d, d_o: nt x nr; t [s]; rx [m]; v_mute [m/ns];
+    t_mute [ns]): sigmoid mute 1/(1+exp(5 (t0 + (rx-rx[0])/v - t))); returns
+    (muted d, muted d_o, e = d - d_o)."""
+    v = v_mute * 1e9
+    t0 = t_mute * 1e-9
+    rx = np.asarray(rx, dtype=np.float64)
+    to_ = t0 + (rx - rx[0]) / v
+    mute = 1.0 / (1.0 + np.exp(5.0 * (to_[None, :] - np.asarray(t, dtype=np.float64)[:, None])))
+    dm = np.asarray(d, dtype=np.float64) * mute
+    dom = np.asarray(d_o, dtype=np.float64) * mute
+    return dm, dom, dm - dom

@@ -155,6 +155,75 @@ def test_fwi_gradient_l2(oracle_mod, gpu_ctx):
     assert rel_l2(gs, fs) <= 1e-3
 
 
+@pytest.mark.parametrize("nt", [240, 241, 256])
+def test_filt_gauss_gpu(oracle_mod, gpu_ctx, nt):
+    """gjw_filt_gauss (filt_gauss.m) on host and device traces against the
+    FP64 oracle; odd, even and power-of-two lengths."""
+    import torch
+    rng = np.random.default_rng(nt)
+    dt, f_low, f_high = 4.2e-11, 1e8, 5e8
+    d = rng.standard_normal((5, nt)).astype(np.float32)
+    ref = oracle_mod.filt_gauss(d.T.astype(np.float64), dt, f_low, f_high, 8).T
+    h = d.copy()
+    gpu_ctx.filt_gauss(h, dt, f_low, f_high)
+    dd = torch.from_numpy(d).cuda()
+    gpu_ctx.filt_gauss(dd, dt, f_low, f_high)
+    print("filt_gauss", nt, rel_l2(h, ref), rel_l2(dd.cpu().numpy(), ref))
+    assert rel_l2(h, ref) <= 1e-5 and rel_l2(dd.cpu().numpy(), ref) <= 1e-5
+
+
+def test_lmute_gpu(oracle_mod, gpu_ctx):
+    """gjw_lmute (w_lmute.m) against the oracle."""
+    rng = np.random.default_rng(3)
+    nt, nr = 300, 7
+    t = np.arange(nt) * 4e-11
+    rx = np.sort(rng.uniform(0, 5, nr))
+    d = rng.standard_normal((nr, nt)).astype(np.float32)
+    d_o = rng.standard_normal((nr, nt)).astype(np.float32)
+    dm, dom, e = oracle_mod.lmute(d.T, d_o.T, t, rx, 0.12, 3.0)
+    a, b = d.copy(), d_o.copy()
+    eg = gpu_ctx.lmute(a, b, t, rx, 0.12, 3.0)
+    for x, y in ((a, dm.T), (b, dom.T), (eg, e.T)):
+        assert rel_l2(x, y) <= 1e-6
+
+
+def test_fwi_gradient_adjoint_filter(oracle_mod, gpu_ctx):
+    """gjw_fwi_gradient with the adjoint-source band filter (w_update_e.m:
+    69-70): the oracle's gradient for the oracle-filtered L2 residual; graph
+    replays equal the eager result bit for bit (the filter runs eagerly
+    between the captured sweeps)."""
+    import torch
+    from gerjoii_amd import _native as N
+    case = configs.small(nt=240, ns=2)
+    svt = make_survey(case, gpu_ctx)
+    svt.set_model(case.eps_true, case.sig_true)
+    dobs = svt.forward()
+    f_low, f_high = 1e8, 4e8
+    sv = make_survey(case, gpu_ctx, ck=9, L=16)
+    sv.set_adj_filter(f_low, f_high)
+    ge, gs, J = sv.fwi_gradient(dobs)
+    d = sv.forward(keep=True)
+    e = (d.astype(np.float64) - dobs.astype(np.float64))
+    ef = oracle_mod.filt_gauss(e.T, case.grid.dt, f_low, f_high, 8).T
+    fe, fs, _ = oracle_grad(oracle_mod, case, case.eps0, case.sig0, ef, sv.rec_ptr)
+    print("filtered", rel_l2(ge, fe), cosine(ge, fe), rel_l2(gs, fs))
+    assert rel_l2(ge, fe) <= 1e-3 and cosine(ge, fe) >= 0.9999
+    assert rel_l2(gs, fs) <= 1e-3
+    # the filter changes the gradient
+    g0 = make_survey(case, gpu_ctx, ck=9, L=16).fwi_gradient(dobs)[0]
+    assert rel_l2(g0, ge) > 1e-2
+    gr = make_survey(case, gpu_ctx, ck=9, L=16)
+    gr.set_adj_filter(f_low, f_high)
+    gr.set_option(N.GJW_OPT_GRAPH, 1)
+    dt_ = torch.from_numpy(dobs).cuda()
+    for rep in range(3):
+        a, b, Jr = gr.fwi_gradient(dt_)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(a, ge)
+        np.testing.assert_array_equal(b, gs)
+        assert Jr == J
+
+
 def test_differentiate_cube_gpu(oracle_mod, gpu_ctx):
     rng = np.random.default_rng(2)
     for nt in (7, 8):

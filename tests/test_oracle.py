@@ -297,3 +297,87 @@ def test_oracle_snapshot_gradient(O):
             err = np.linalg.norm(a - b) / np.linalg.norm(b)
             print("snap", mode, "rel err", err)
             assert 0 < err <= tol, (mode, err)
+
+
+# --------------------------------------------------------------------------
+# adjoint-source conditioning (filt_gauss.m, tukeywin, w_lmute.m)
+# --------------------------------------------------------------------------
+
+def test_tukeywin_matches_scipy(O):
+    """MATLAB's tukeywin and scipy's tukey window are the same definition
+    (flank length floor(r/2 (n-1)) + 1, raised cosine): equal for odd and
+    even n and several taper fractions, including the Hann limit r = 1."""
+    from scipy.signal import windows
+    for n in (1, 2, 3, 5, 8, 11, 240, 241, 1000):
+        for r in (0.1, 0.3, 0.5, 1.0):
+            np.testing.assert_allclose(O.tukeywin(n, r), windows.tukey(n, r), rtol=0, atol=1e-13)
+
+
+def _filt_gauss_dft(d, dt, f_low, f_high, steep):
+    """The same filter by explicit sums over the signed frequencies
+    f_j = (j - N/2) df, without FFT ordering or fftshift."""
+    nt, nr = d.shape
+    N = 1
+    while N < nt:
+        N *= 2
+    x = np.zeros((N, nr))
+    from scipy.signal import windows
+    x[:nt] = d * windows.tukey(nt, 0.1)[:, None]
+    df = 1.0 / dt / N
+    f = (np.arange(N) - N // 2) * df
+    width = f_high - f_low
+    mid = f_low + width / 2
+    g = np.exp(-((f - mid) ** steep) / ((df / 2) * width ** (steep - 1)))
+    n = np.arange(N)
+    E = np.exp(-2j * np.pi * np.outer(f * dt, n))  # (freq, time)
+    X = E @ x
+    y = (np.conj(E).T @ (g[:, None] * X)) / N
+    return np.real(y[:nt])
+
+
+def test_filt_gauss_closed_forms(O):
+    """filt_gauss.m keeps only the positive-frequency band, so a tone at the
+    band centre comes back at half amplitude (away from the taper), a tone
+    far outside the band vanishes, the filter is linear, and it equals the
+    filter written as explicit sums over the signed frequencies (odd, even
+    and power-of-two lengths)."""
+    dt, f_low, f_high = 1e-10, 2e8, 6e8
+    nt = 2000
+    t = np.arange(nt) * dt
+    tone = np.cos(2 * np.pi * 4e8 * t)[:, None]
+    y = O.filt_gauss(tone, dt, f_low, f_high, 8)
+    mid = slice(300, 1700)
+    assert np.abs(y[mid, 0] - 0.5 * tone[mid, 0]).max() < 1e-5
+    far = np.cos(2 * np.pi * 2e9 * t)[:, None]
+    assert np.abs(O.filt_gauss(far, dt, f_low, f_high, 8)).max() < 1e-4
+    rng = np.random.default_rng(11)
+    a, b = rng.standard_normal((2, 300, 3))
+    np.testing.assert_allclose(O.filt_gauss(2 * a - 3 * b, dt, f_low, f_high, 8),
+                               2 * O.filt_gauss(a, dt, f_low, f_high, 8)
+                               - 3 * O.filt_gauss(b, dt, f_low, f_high, 8), atol=1e-12)
+    for nt in (37, 64, 100):
+        d = rng.standard_normal((nt, 2))
+        np.testing.assert_allclose(O.filt_gauss(d, dt, f_low, f_high, 8),
+                                   _filt_gauss_dft(d, dt, f_low, f_high, 8), rtol=0, atol=1e-10)
+
+
+def test_lmute_closed_forms(O):
+    """w_lmute.m: the mute is 1/2 on the arrival t0 + (x - x0)/v, close to 1
+    well after it, and e = d - d_o after muting both."""
+    nt, nr = 50, 4
+    t = np.arange(nt) * 1e-9  # [s]
+    rx = np.array([2.0, 2.5, 3.0, 4.0])  # [m]
+    v_mute, t_mute = 0.1, 5.0  # [m/ns], [ns]
+    d = np.ones((nt, nr))
+    d_o = np.full((nt, nr), 0.25)
+    dm, dom, e = O.lmute(d, d_o, t, rx, v_mute, t_mute)
+    to_ = t_mute * 1e-9 + (rx - rx[0]) / (v_mute * 1e9)
+    for r in range(nr):
+        m = 1.0 / (1.0 + np.exp(5.0 * (to_[r] - t)))
+        np.testing.assert_allclose(dm[:, r], m, rtol=1e-15)
+    # the arrival itself, evaluated on a time axis that holds it
+    dm2, _, _ = O.lmute(np.ones((1, 1)), np.zeros((1, 1)), np.array([t_mute * 1e-9]), rx[:1],
+                        v_mute, t_mute)
+    assert dm2[0, 0] == 0.5
+    np.testing.assert_allclose(e, dm - dom, rtol=0, atol=0)
+    np.testing.assert_allclose(dom, 0.25 * dm, rtol=1e-15)
