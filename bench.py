@@ -186,6 +186,7 @@ def main():
         log(f"plain: {plain['shot_gradients_per_s']:.3f} shot-gradients/s")
 
     cpu = None if args.no_cpu else _cpu_baseline(case, args)
+    cond = _conditioning(ctx, d_obs, g.dt, args.no_cpu)
 
     out = {
         "metric": "FWI shot-gradients/s, batched vs plain, at <=1e-3 grad err; Gcell-updates/s",
@@ -210,8 +211,43 @@ def main():
         "batched_vs_plain": (shot_grads / plain["shot_gradients_per_s"]) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "adjoint_source_filter": cond,
     }
     print(json.dumps(out))
+
+
+def _conditioning(ctx, d_obs, dt, no_cpu):
+    """The adjoint-source band filter (filt_gauss.m, gjw_filt_gauss) on all
+    traces of the workload: device time per call (HIP events around 5 calls
+    after one warm-up) and the FP64 numpy oracle on a 64-trace sample."""
+    import torch
+    ntr, nt = d_obs.shape
+    f_low, f_high = 5e7, 4e8
+    d = d_obs.clone()
+    ctx.filt_gauss(d, dt, f_low, f_high)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        ctx.filt_gauss(d, dt, f_low, f_high)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    out = {"traces": ntr, "nt": nt, "ms_per_call": ms,
+           "traces_per_s": ntr / (ms * 1e-3),
+           "io_GBs": 8.0 * ntr * nt / (ms * 1e-3) / 1e9,
+           "note": "in place on device traces; io_GBs counts one read and one write of the traces"}
+    if not no_cpu:
+        from oracle import oracle as O
+        q = 64
+        x = d_obs[:q].cpu().numpy().T.astype(np.float64)
+        c0 = time.perf_counter()
+        O.filt_gauss(x, dt, f_low, f_high, 8)
+        cs = time.perf_counter() - c0
+        out["cpu_traces_per_s"] = q / cs
+        out["cpu_sample"] = "%d traces, numpy FFT (FP64), one thread of the host" % q
+    del d
+    torch.cuda.empty_cache()
+    return out
 
 
 def _traffic_from_profiles(kernel, cells, ns):
