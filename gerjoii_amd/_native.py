@@ -31,7 +31,8 @@ EXPORTS = ["gjw_create", "gjw_destroy", "gjw_last_error", "gjw_version", "gjw_su
            "gjw_forward", "gjw_misfit", "gjw_bozdag", "gjw_gradient", "gjw_fwi_gradient",
            "gjw_differentiate_cube", "gjw_last_timing", "gjw_kernel_stats", "gjw_tile",
            "gjw_batch_create", "gjw_batch_destroy", "gjw_batch_fwi_gradient",
-           "gjw_batch_last_timing", "gjw_set_adj_filter", "gjw_filt_gauss", "gjw_lmute"]
+           "gjw_batch_last_timing", "gjw_set_adj_filter", "gjw_filt_gauss", "gjw_lmute",
+           "gjw_batch_objective"]
 
 
 class GjwGrid(C.Structure):
@@ -75,6 +76,7 @@ def load():
     lib.gjw_batch_destroy.argtypes = [vp]
     lib.gjw_batch_fwi_gradient.argtypes = [vp, vp, vp, vp, vp]
     lib.gjw_batch_last_timing.argtypes = [vp, vp, vp]
+    lib.gjw_batch_objective.argtypes = [vp, vp, vp]
     lib.gjw_gradient.argtypes = [vp, vp, i32, vp, vp, vp, vp]
     lib.gjw_fwi_gradient.argtypes = [vp, vp, vp, vp, vp]
     lib.gjw_differentiate_cube.argtypes = [vp, vp, i32, i32, i32, C.c_double]

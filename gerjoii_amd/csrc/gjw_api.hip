@@ -94,6 +94,7 @@ __global__ void k_misfit_stage2(const float*, const float2*, long long, int, int
 __global__ void k_reduce_d(const double*, int, double, double*);
 __global__ void k_dcube(float*, long long, int, double);
 __global__ void k_tukey_pad(const float*, int, int, long long, double, float*);
+__global__ void k_sumsq(const float*, long long, double*);
 __global__ void k_gauss_band(float2*, int, long long, double, double, double, int);
 __global__ void k_trim(const float*, int, int, long long, float*);
 __global__ void k_lmute(float*, float*, float*, int, long long, const double*, const double*, double,
@@ -1346,12 +1347,17 @@ static void filter_free(GaussFilter* f) {
 
 static bool filter_on(const gjw_survey* s) { return s->flt_hi > s->flt_lo; }
 
-// band filter of the adjoint sources of traces [t0, t0+cnt) (w_update_e.m:69-70)
-static int adj_filter(gjw_survey* s, long long t0, long long cnt) {
+// the survey's band filter on cnt traces at p (nt samples each)
+static int trace_filter(gjw_survey* s, float* p, long long cnt) {
   if (!filter_on(s) || cnt <= 0) return GJW_OK;
   if (!s->flt) s->flt = new GaussFilter();
-  return s->flt->run(s->ctx->stream, s->adj.p + t0 * s->gr.nt, (int)cnt, s->gr.nt, s->gr.dt,
-                     s->flt_lo, s->flt_hi, s->flt_steep);
+  return s->flt->run(s->ctx->stream, p, (int)cnt, s->gr.nt, s->gr.dt, s->flt_lo, s->flt_hi,
+                     s->flt_steep);
+}
+
+// band filter of the adjoint sources of traces [t0, t0+cnt) (w_update_e.m:69-70)
+static int adj_filter(gjw_survey* s, long long t0, long long cnt) {
+  return trace_filter(s, s->adj.p + t0 * s->gr.nt, cnt);
 }
 
 // analytic signal (unnormalised: x nt) of the ntr traces in s->fin, into s->fana
@@ -2000,6 +2006,7 @@ struct gjw_batch {
   std::vector<gjw_survey*> sv;
   DBuf<DevSurvey> dsv;
   int K = 0, T = 0, fuse = 2, calls = 0;
+  bool keep = true;  // forward sweep saves checkpoints (gradients); not for objectives
   GraphCache g_all, g_fwd, g_adj;
   double last_ms = 0, last_cells = 0;
 };
@@ -2120,7 +2127,7 @@ static int batch_fwd_enqueue(gjw_batch* b) {
       if (tau < nt[k]) {
         act.push_back(k);
         gjw_survey* s = b->sv[k];
-        if (tau % K == 0) {
+        if (b->keep && tau % K == 0) {
           FwdBufs fb = fwd_bufs(s, s->ns);
           TRY(ck_save(s, fb, cur[k], xc[k], tau / K));
         }
@@ -2319,6 +2326,84 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
     cells += 3.0 * (s->NX + 1) * (s->NY + 1) * (double)s->gr.nt * s->ns;
   }
   HIPCHK(hipStreamSynchronize(st));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
+  b->last_ms = ms;
+  b->last_cells = cells;
+  return GJW_OK;
+}
+
+// Objective of each survey's current model (w_objs_e.m:22-63 with one L2
+// objective): forward without checkpoints for all surveys in shared launches,
+// the survey's band filter on the synthetic and the observed traces, then
+// J = 0.5 |d - d_o|^2 / |d_o|^2 in FP64.
+extern "C" int gjw_batch_objective(gjw_batch* b, const float* const* d_obs, double* J) {
+  if (!b || !d_obs || !J) return fail(GJW_ERR_ARG, "gjw_batch_objective: NULL argument");
+  const int S = (int)b->sv.size();
+  HIPCHK(hipSetDevice(b->ctx->device));
+  hipStream_t st = b->ctx->stream;
+  int T = 0, fuse = 4;
+  for (int k = 0; k < S; ++k) {
+    gjw_survey* s = b->sv[k];
+    TRY(check_ready(s, "gjw_batch_objective"));
+    if (!d_obs[k]) return fail(GJW_ERR_ARG, "gjw_batch_objective: d_obs[%d] is NULL", k);
+    if (s->path != GJW_PATH_BATCHED)
+      return fail(GJW_ERR_ARG, "gjw_batch_objective: survey %d is on the plain path", k);
+    if (s->misfit_kind != GJW_MISFIT_L2)
+      return fail(GJW_ERR_ARG, "gjw_batch_objective: survey %d: the L2 misfit only", k);
+    T = std::max(T, s->gr.nt);
+  }
+  for (int k = 0; k < S; ++k) {
+    gjw_survey* s = b->sv[k];
+    const long long n = (long long)s->ntr * s->gr.nt;
+    TRY(ensure_fwd_state(s, s->ns));
+    TRY(s->tr.ensure(std::max<long long>(n, 1)));
+    TRY(s->obs.ensure(std::max<long long>(n, 1)));
+    TRY(s->adj.ensure(std::max<long long>(n, 1)));
+    TRY(s->part.ensure(1024));
+    TRY(s->Jd.ensure(2));
+    TRY(zero_fwd_state(s, s->ns));
+    TRY(autotune_L(s, s->ns));
+    fuse = std::min(fuse, s->fuse_eff);
+    TRY(host_or_dev_copy(s->obs.p, d_obs[k], n * 4, st));
+  }
+  std::vector<DevSurvey> hs(S);
+  for (int k = 0; k < S; ++k) hs[k] = dev_survey(b->sv[k]);
+  TRY(b->dsv.ensure(S));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpy(b->dsv.p, hs.data(), S * sizeof(DevSurvey), hipMemcpyHostToDevice));
+  b->T = T;
+  b->K = T;  // one segment: no checkpoints
+  b->fuse = fuse;
+  b->keep = false;
+  HIPCHK(hipEventRecord(b->ctx->ev0, st));
+  const int rc = batch_fwd_enqueue(b);
+  b->keep = true;
+  TRY(rc);
+  double cells = 0;
+  for (int k = 0; k < S; ++k) {
+    gjw_survey* s = b->sv[k];
+    const long long n = (long long)s->ntr * s->gr.nt;
+    TRY(trace_filter(s, s->tr.p, s->ntr));
+    TRY(trace_filter(s, s->obs.p, s->ntr));
+    const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(n, 256), 1), 1024);
+    hipLaunchKernelGGL(k_resid_l2, dim3(blocks), dim3(256), 0, st, s->tr.p, s->obs.p, s->adj.p,
+                       n, s->part.p);
+    hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, blocks, 0.5, s->Jd.p);
+    hipLaunchKernelGGL(k_sumsq, dim3(blocks), dim3(256), 0, st, s->obs.p, n, s->part.p);
+    hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, blocks, 1.0,
+                       s->Jd.p + 1);
+    HIPCHK(hipGetLastError());
+    cells += (double)(s->NX + 1) * (s->NY + 1) * (double)s->gr.nt * s->ns;
+    s->kept = 0;
+  }
+  HIPCHK(hipEventRecord(b->ctx->ev1, st));
+  for (int k = 0; k < S; ++k) {
+    double jd[2];
+    HIPCHK(hipMemcpyAsync(jd, b->sv[k]->Jd.p, 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    J[k] = jd[1] > 0 ? jd[0] / jd[1] : 0.0;
+  }
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
   b->last_ms = ms;

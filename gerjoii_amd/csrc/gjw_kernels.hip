@@ -2118,6 +2118,22 @@ __global__ void k_flush(int m, int n, int LD, int di0, int ns, float* __restrict
 
 // L2 residual e = d - d_o (w_bozdag.m:13-15) and per-block partial sums of
 // e^2 in FP64 (w_E.m:15), finished by k_reduce_d.
+// per-block partial sums of x^2 in FP64 (the energy of the observed traces)
+__global__ void k_sumsq(const float* __restrict__ x, long long n, double* __restrict__ part) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x)
+    acc += (double)x[t] * (double)x[t];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 __global__ void k_resid_l2(const float* __restrict__ d, const float* __restrict__ dob,
                            float* __restrict__ e, long long n, double* __restrict__ part) {
   __shared__ double sh[256];

@@ -364,6 +364,20 @@ class Batch:
         except Exception:
             pass
 
+    def objective(self, d_obs):
+        """Normalised L2 objective of each survey's current model
+        (w_objs_e.m): forward only, in shared launches, traces band-filtered
+        when the survey has a filter.  d_obs: one traces array per survey."""
+        S = len(self.surveys)
+        d_obs = [d if hasattr(d, "data_ptr") else np.ascontiguousarray(d, dtype=np.float32)
+                 for d in d_obs]
+        _sync_torch(*d_obs)
+        J = np.zeros(S, dtype=np.float64)
+        P = C.c_void_p * S
+        N.check(N.load().gjw_batch_objective(self.h, P(*[N.ptr(d).value for d in d_obs]),
+                                             N.ptr(J)), "gjw_batch_objective")
+        return J
+
     def fwi_gradient(self, d_obs, g_eps=None, g_sig=None):
         """d_obs: one traces array per survey.  Returns (g_eps list, g_sig
         list, J array)."""

@@ -197,7 +197,16 @@ int gjw_batch_destroy(gjw_batch *b);
  * ceil(sqrt(max nt))). */
 int gjw_batch_fwi_gradient(gjw_batch *b, const float *const *d_obs, double *const *g_eps,
                            double *const *g_sig, double *J);
-/* Device time and cell-updates of the last gjw_batch_fwi_gradient. */
+/* Objective of every survey's current model, as w_objs_e.m:22-63 evaluates
+ * it for one line-search step (src/w/w_stepparabola_e.m:20-31 calls it per
+ * step): forward without wavefield storage for all surveys in shared
+ * launches, the survey's band filter (gjw_set_adj_filter) on the synthetic
+ * and the observed traces, J[k] = 0.5 |d - d_o|^2 / |d_o|^2 (L2 misfit).
+ * Line searches batch their perturbed models as surveys of one batch. */
+int gjw_batch_objective(gjw_batch *b, const float *const *d_obs, double *J);
+
+/* Device time and cell-updates of the last gjw_batch_fwi_gradient or
+ * gjw_batch_objective. */
 int gjw_batch_last_timing(gjw_batch *b, double *ms, double *cells);
 
 /* Device-time of the last gjw_forward / gjw_gradient / gjw_fwi_gradient

@@ -1,0 +1,32 @@
+"""The C ABI (include/gjw.h): the in-tree library loads on a host without a
+GPU and exports every function the header declares, and the ctypes binding
+lists exactly those (no compute calls here)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "gjw.h")).read()
+    src = re.sub(r"/\*.*?\*/", " ", src, flags=re.S)  # drop comments
+    names = re.findall(r"^\s*(?:int\s+|const char\s*\*\s*)(gjw_\w+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_the_entry_points():
+    names = _declared()
+    for must in ("gjw_create", "gjw_forward", "gjw_gradient", "gjw_fwi_gradient",
+                 "gjw_batch_fwi_gradient", "gjw_batch_objective", "gjw_filt_gauss", "gjw_lmute"):
+        assert must in names
+    assert len(names) >= 25
+
+
+def test_library_exports_every_declared_symbol():
+    from gerjoii_amd import _native as N
+    assert os.path.exists(N.LIB_PATH), "build the library first (gerjoii_amd/csrc/Makefile)"
+    lib = ctypes.CDLL(N.LIB_PATH)
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(N.EXPORTS) == _declared()

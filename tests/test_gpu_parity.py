@@ -224,6 +224,57 @@ def test_fwi_gradient_adjoint_filter(oracle_mod, gpu_ctx):
         assert Jr == J
 
 
+def _oracle_objective(O, case, eps, dobs, band):
+    """w_objs_e.m:22-63 in FP64: forward, band filter of both trace sets,
+    0.5 |d - d_o|^2 / |d_o|^2."""
+    d = oracle_forward(O, case, eps, case.sig0)
+    dt = case.grid.dt
+    df = O.filt_gauss(d.T, dt, band[0], band[1], 8)
+    do = O.filt_gauss(dobs.T.astype(np.float64), dt, band[0], band[1], 8)
+    return 0.5 * float(((df - do) ** 2).sum()) / float((do ** 2).sum())
+
+
+def test_batch_objective_and_line_search(oracle_mod, gpu_ctx):
+    """Batched line-search objectives (gjw_batch_objective, w_objs_e.m) of
+    perturbed models against the FP64 oracle, and the parabolic step of
+    w_stepparabola_e.m from the device objectives against the step from the
+    oracle's objectives at the same trial steps."""
+    from gerjoii_amd import linesearch as LS
+    case = configs.small(nt=240, ns=2)
+    svt = make_survey(case, gpu_ctx)
+    svt.set_model(case.eps_true, case.sig_true)
+    dobs = svt.forward()
+    band = (1e8, 4e8)
+    ge = make_survey(case, gpu_ctx).fwi_gradient(dobs)[0]
+    g = ge / np.abs(ge).max()
+    ks = [-0.01, 0.0, 0.02]
+    svs = []
+    for k in ks:
+        sv = S.Survey(case.grid, gpu_ctx)
+        sv.set_model(LS.perturb_eps(case.eps0, g, k), case.sig0)
+        sv.set_shots(case.src, case.wavelets, case.receivers)
+        sv.set_adj_filter(*band)
+        svs.append(sv)
+    J = S.Batch(svs).objective([dobs] * len(ks))
+    ref = [_oracle_objective(oracle_mod, case, LS.perturb_eps(case.eps0, g, k), dobs, band)
+           for k in ks]
+    print("objectives", J, ref)
+    for a, b in zip(J, ref):
+        assert abs(a - b) <= 1e-4 * abs(b)
+    Ee0 = J[1]
+    step, ke, Ee = LS.stepparabola_e(gpu_ctx, case.grid, case.src, case.wavelets, case.receivers,
+                                     case.eps0, case.sig0, g, dobs, Ee0, k_e=[1e-3, 0.05],
+                                     keprct_=0.8, keprct__=0.8, eps_max=30.0, eps_min=0.5,
+                                     nparabo=3, band=band)
+    Eref = [_oracle_objective(oracle_mod, case, LS.perturb_eps(case.eps0, g, k), dobs, band)
+            for k in ke]
+    step_ref = LS.parabola_step(ke, Eref)
+    print("line search", step, step_ref, ke, Ee, Eref)
+    np.testing.assert_allclose(Ee, Eref, rtol=1e-4)
+    assert ke.max() - ke.min() > 1e-3  # trial steps wide enough for a curved fit
+    assert abs(step - step_ref) <= 1e-2 * max(abs(step_ref), 1e-12)
+
+
 def test_differentiate_cube_gpu(oracle_mod, gpu_ctx):
     rng = np.random.default_rng(2)
     for nt in (7, 8):
