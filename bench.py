@@ -48,6 +48,7 @@ def main():
     ap.add_argument("--cpu-nt", type=int, default=480)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-linesearch", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--fuse", type=int, default=0, help="forward steps per launch: 0 autotuned, 1, 2, 4, 8")
     ap.add_argument("--adj-fuse", type=int, default=0, help="adjoint steps per launch: 0 (six), 4, 6, 8")
@@ -187,6 +188,7 @@ def main():
 
     cpu = None if args.no_cpu else _cpu_baseline(case, args)
     cond = _conditioning(ctx, d_obs, g.dt, args.no_cpu)
+    lsearch = None if args.no_linesearch else _line_search(ctx, case, d_obs)
 
     out = {
         "metric": "FWI shot-gradients/s, batched vs plain, at <=1e-3 grad err; Gcell-updates/s",
@@ -212,8 +214,51 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "adjoint_source_filter": cond,
+        "line_search": lsearch,
     }
     print(json.dumps(out))
+
+
+def _line_search(ctx, case, d_obs, nparabo=3):
+    """Objectives of a parabolic line search (w_stepparabola_e.m /
+    w_objs_e.m) at the workload: nparabo perturbed models of all shots,
+    batched (one Batch of nparabo surveys, shared launches) and one model
+    at a time, after a warm-up call each."""
+    import torch
+    from gerjoii_amd import survey as S
+    from gerjoii_amd import linesearch as LS
+    rng = np.random.default_rng(5)
+    gdir = rng.standard_normal(case.eps0.shape)
+    gdir /= np.abs(gdir).max()
+    svs = []
+    for k in np.linspace(-0.002, 0.002, nparabo):
+        sv = S.Survey(case.grid, ctx)
+        sv.set_model(LS.perturb_eps(case.eps0, gdir, k), case.sig0)
+        sv.set_shots(case.src, case.wavelets, case.receivers)
+        svs.append(sv)
+    batch = S.Batch(svs)
+    batch.objective([d_obs] * nparabo)  # warm-up (allocations, autotuner)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = batch.objective([d_obs] * nparabo)
+    tb = time.perf_counter() - t0
+    batch.close()
+    one = S.Batch(svs[:1])
+    one.objective([d_obs])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sv in svs:
+        S.Batch([sv]).objective([d_obs])
+    ts = time.perf_counter() - t0
+    one.close()
+    ns = len(case.src)
+    for sv in svs:
+        sv.close()
+    torch.cuda.empty_cache()
+    return {"models": nparabo, "shots": ns, "s_batched": tb, "s_one_model_at_a_time": ts,
+            "shot_objectives_per_s": nparabo * ns / tb,
+            "objectives": [float(x) for x in J],
+            "note": "forward only (no wavefield), L2 normalised as w_objs_e.m"}
 
 
 def _conditioning(ctx, d_obs, dt, no_cpu):
