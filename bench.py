@@ -189,6 +189,7 @@ def main():
     cpu = None if args.no_cpu else _cpu_baseline(case, args)
     cond = _conditioning(ctx, d_obs, g.dt, args.no_cpu)
     lsearch = None if args.no_linesearch else _line_search(ctx, case, d_obs)
+    post = _postprocess(ctx, case, args.no_cpu)
 
     out = {
         "metric": "FWI shot-gradients/s, batched vs plain, at <=1e-3 grad err; Gcell-updates/s",
@@ -215,6 +216,7 @@ def main():
         "cpu_baseline": cpu,
         "adjoint_source_filter": cond,
         "line_search": lsearch,
+        "gradient_postprocess": post,
     }
     print(json.dumps(out))
 
@@ -259,6 +261,62 @@ def _line_search(ctx, case, d_obs, nparabo=3):
             "shot_objectives_per_s": nparabo * ns / tb,
             "objectives": [float(x) for x in J],
             "note": "forward only (no wavefield), L2 normalised as w_objs_e.m"}
+
+
+def _postprocess(ctx, case, no_cpu):
+    """Per-shot gradient post-processing and the update (w_update_e.m:108-145,
+    w_update_e_.m:38-41, w_image_e.m:50-51) on one FP64 gradient image per
+    shot of the workload, resident on the device: time per stage (each entry
+    point synchronises on return) over 5 passes after one warm-up, and the
+    FP64 numpy oracle chain on a 2-image sample."""
+    import torch
+    from gerjoii_amd import postproc as PP
+    g = case.grid
+    m, n, ns = g.m, g.n, len(case.src)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    G = torch.randn((ns, n, m), dtype=torch.float64, device="cuda", generator=gen)
+    shape = torch.from_numpy(np.ascontiguousarray(case.shape.T, dtype=np.float64)).cuda()
+    model = torch.from_numpy(np.ascontiguousarray(case.eps0.T, dtype=np.float64)).cuda()
+    X = np.arange(n) * g.dx
+    Y = np.arange(m) * g.dx
+    s = PP.taper_index(g, case.src)
+    lo = PP.source_wavelength(case.eps0, s, case.f0)
+    ax = az = g.dx / (1.5 * float(np.median(lo)))  # 1/(1.5 lo) [1/m] times dx, as the project scripts
+    steps = np.full(ns, 1e-3)
+    stages = {"gaussi": lambda: ctx.gaussi(G, X, Y, s, 2 * lo),
+              "image_gaussian": lambda: ctx.image_gaussian(G, ax, az, "LOW_PASS", out=G),
+              "normali_guard_mask": lambda: ctx.normali(G, nan_guard=True, shape=shape),
+              "stack_update": lambda: ctx.stack_update(G, steps, model=model)}
+    for f in stages.values():
+        f()
+    torch.cuda.synchronize()
+    reps = 5
+    ms = {}
+    for k, f in stages.items():
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        ms[k] = (time.perf_counter() - t0) * 1e3 / reps
+    tot = sum(ms.values())
+    out = {"images": ns, "m": m, "n": n, "ms_per_stage": ms, "ms_total": tot,
+           "images_per_s": ns / (tot * 1e-3),
+           "io_GBs_images": 8.0 * m * n * ns * 7 / (tot * 1e-3) / 1e9,
+           "note": "FP64 images on the device; io_GBs_images counts 7 passes over the images "
+                   "(gaussi R+W, filter R+W, normali R + R+W) against the whole chain's time"}
+    if not no_cpu:
+        from oracle import oracle as O
+        q = 2
+        h = G[:q].cpu().numpy()
+        c0 = time.perf_counter()
+        imgs = [O.w_update_post(h[k].T, X, Y, s[k], lo[k], ax, az, case.shape) for k in range(q)]
+        O.stack_update(np.stack(imgs, axis=2), steps[:q], model=case.eps0)
+        cs = time.perf_counter() - c0
+        out["cpu_images_per_s"] = q / cs
+        out["cpu_sample"] = "%d images, numpy (FP64), one thread of the host" % q
+    del G, shape, model
+    torch.cuda.empty_cache()
+    return out
 
 
 def _conditioning(ctx, d_obs, dt, no_cpu):

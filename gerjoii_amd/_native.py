@@ -19,6 +19,8 @@ GJW_OPT_FUSE = 6
 GJW_OPT_GRAPH = 7
 GJW_OPT_MISFIT = 8
 GJW_OPT_ADJ_FUSE = 9
+GJW_OPT_SHOT_GRADS = 10
+GJW_LOW_PASS, GJW_HI_PASS = 0, 1
 GJW_SNAP_F32, GJW_SNAP_BF16, GJW_SNAP_F16 = 0, 1, 2
 GJW_MISFIT_L2 = 0
 GJW_MISFIT_ENV_L2, GJW_MISFIT_PHASE_L2, GJW_MISFIT_ENV_L22, GJW_MISFIT_ENV_LOG = 1, 2, 3, 4
@@ -32,7 +34,8 @@ EXPORTS = ["gjw_create", "gjw_destroy", "gjw_last_error", "gjw_version", "gjw_su
            "gjw_differentiate_cube", "gjw_last_timing", "gjw_kernel_stats", "gjw_tile",
            "gjw_batch_create", "gjw_batch_destroy", "gjw_batch_fwi_gradient",
            "gjw_batch_last_timing", "gjw_set_adj_filter", "gjw_filt_gauss", "gjw_lmute",
-           "gjw_batch_objective"]
+           "gjw_batch_objective", "gjw_gaussi", "gjw_image_gaussian", "gjw_normali",
+           "gjw_stack_update"]
 
 
 class GjwGrid(C.Structure):
@@ -84,6 +87,10 @@ def load():
     lib.gjw_set_adj_filter.argtypes = [vp, C.c_double, C.c_double, i32]
     lib.gjw_filt_gauss.argtypes = [vp, vp, i32, i32, C.c_double, C.c_double, C.c_double, i32]
     lib.gjw_lmute.argtypes = [vp, vp, vp, vp, i32, i32, vp, vp, C.c_double, C.c_double]
+    lib.gjw_gaussi.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp]
+    lib.gjw_image_gaussian.argtypes = [vp, vp, vp, i32, i32, i32, C.c_double, C.c_double, i32]
+    lib.gjw_normali.argtypes = [vp, vp, i32, i32, i32, i32, vp]
+    lib.gjw_stack_update.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp]
     lib.gjw_kernel_stats.argtypes = [vp, i32, vp, vp, vp, i32]
     lib.gjw_tile.argtypes = [vp, C.c_int32, vp]
     for name in EXPORTS:

@@ -81,7 +81,7 @@ __global__ void k_unpack16(const unsigned short*, long long, int, const int*, in
 __global__ void k_scatter(const float*, long long, const int*, int, int, float*);
 __global__ void k_domain_frame(DevGrid, int, const float*, float*, long long);
 __global__ void k_flush(int, int, int, int, int, float*, long long, const double*, const double*,
-                        double*, double*, double);
+                        double*, double*, double, long long);
 __global__ void k_ref_overwrite(DevGrid, int, int, int, const int*, const int*, const float*, int,
                                 float*);
 __global__ void k_ref_corr(DevGrid, const float*, const float*, int, int, double, float*, float*);
@@ -99,6 +99,18 @@ __global__ void k_gauss_band(float2*, int, long long, double, double, double, in
 __global__ void k_trim(const float*, int, int, long long, float*);
 __global__ void k_lmute(float*, float*, float*, int, long long, const double*, const double*, double,
                         double);
+__global__ void k_gaussi_norm(const double*, int, const double*, int, const int*, const double*,
+                              double*);
+__global__ void k_gaussi_apply(double*, int, int, long long, const double*, const double*,
+                               const int*, const double*, const double*);
+__global__ void k_img_pad(const double*, int, int, int, int, long long, double*);
+__global__ void k_img_kfilter(double2*, int, int, long long, double, double, int);
+__global__ void k_img_trim(const double*, int, int, int, int, long long, double*);
+__global__ void k_absmax_part(const double*, long long, double*, int*);
+__global__ void k_normali(double*, long long, int, const double*, const int*, int,
+                          const double*);
+__global__ void k_stack_update(const double*, long long, int, const double*, int, double*,
+                               double*);
 }  // namespace gjw
 
 using namespace gjw;
@@ -137,12 +149,15 @@ static int fail(int code, const char* fmt, ...) {
 // --------------------------------------------------------------------------
 struct GaussFilter;  // band filter of traces (filt_gauss.m), defined with the FFT code
 static void filter_free(GaussFilter* f);
+struct PostWork;  // gradient post-processing: 2-D FFT plans and workspace
+static void post_free(PostWork* w);
 
 struct gjw_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   GaussFilter* flt = nullptr;  // gjw_filt_gauss's plans and buffers
+  PostWork* post = nullptr;    // plans and buffers of the gradient post-processing
 };
 
 template <class T>
@@ -240,6 +255,7 @@ struct gjw_survey {
   int ksample = 0;
   // Hilbert-based misfits: FFT plans over all traces and their buffers
   int misfit_kind = GJW_MISFIT_L2;  // misfit of gjw_fwi_gradient (GJW_OPT_MISFIT)
+  int shot_grads = 0;               // 1: one gradient per shot (GJW_OPT_SHOT_GRADS)
   hipfftHandle fft_r2c = 0, fft_c2c = 0;
   int fft_nt = 0, fft_ntr = 0;
   DBuf<float2> fhalf, fana;
@@ -293,6 +309,7 @@ extern "C" int gjw_destroy(gjw_ctx* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   filter_free(c->flt);
+  post_free(c->post);
   delete c;
   return GJW_OK;
 }
@@ -390,6 +407,10 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       s->tuned_La2 = s->tuned_La4 = s->tuned_La6 = s->tuned_La8 = 0;
       s->fuse_eff = 0;
       break;
+    case GJW_OPT_SHOT_GRADS:
+      if (v != 0 && v != 1) return fail(GJW_ERR_ARG, "GJW_OPT_SHOT_GRADS must be 0 or 1");
+      s->shot_grads = (int)v;
+      return GJW_OK;
     case GJW_OPT_KTIMING:
       if (v < 0) return fail(GJW_ERR_ARG, "GJW_OPT_KTIMING must be >= 0");
       s->ksample = (int)v;
@@ -728,6 +749,15 @@ extern "C" int gjw_set_shots(gjw_survey* s, int32_t ns, const int32_t* src, cons
 // --------------------------------------------------------------------------
 // launch helpers
 // --------------------------------------------------------------------------
+// FP64 gradient values per parameter: m*n, or m*n per shot (shot-major) with
+// GJW_OPT_SHOT_GRADS; the stride is the flush's distance between shots.
+static long long grad_len(const gjw_survey* s) {
+  return (long long)s->gr.m * s->gr.n * (s->shot_grads ? s->ns : 1);
+}
+static long long grad_stride(const gjw_survey* s) {
+  return s->shot_grads ? (long long)s->gr.m * s->gr.n : 0;
+}
+
 static DevGrid dev_grid(const gjw_survey* s) {
   DevGrid g;
   g.NX = s->NX; g.NY = s->NY; g.P = s->P; g.LD = s->LD; g.FS = s->FS;
@@ -1345,6 +1375,76 @@ static void filter_free(GaussFilter* f) {
   delete f;
 }
 
+// --------------------------------------------------------------------------
+// k-space filter of images (image_gaussian.m:21-61): zero padding to powers
+// of two M x N, batched 2-D D2Z, the Gaussian cake of lines 37-50 on the
+// one-sided spectrum, Z2D, trim.  FP64 throughout, as MATLAB's fft2.
+// --------------------------------------------------------------------------
+struct PostWork {
+  hipfftHandle d2z = 0, z2d = 0;
+  int M = 0, N = 0, count = 0;
+  DBuf<double> pad, dsmall;
+  DBuf<double2> spec;
+  DBuf<int> ismall;
+  void release() {
+    if (d2z) (void)hipfftDestroy(d2z);
+    if (z2d) (void)hipfftDestroy(z2d);
+    d2z = z2d = 0;
+    M = N = count = 0;
+    pad.release();
+    spec.release();
+    dsmall.release();
+    ismall.release();
+  }
+  int kfilter(hipStream_t st, const double* in, double* out, int m, int n, int cnt, double ax,
+              double az, int type) {
+    int M2 = 1, N2 = 1;
+    while (M2 < m) M2 *= 2;  // 2^nextpow2
+    while (N2 < n) N2 *= 2;
+    if (!d2z || M != M2 || N != N2 || count != cnt) {
+      if (d2z) FFTCHK(hipfftDestroy(d2z));
+      if (z2d) FFTCHK(hipfftDestroy(z2d));
+      d2z = z2d = 0;
+      int dims[2] = {N2, M2};  // slowest first: columns, then depth
+      const int nr = M2 * N2, nc = N2 * (M2 / 2 + 1);
+      FFTCHK(hipfftPlanMany(&d2z, 2, dims, nullptr, 1, nr, nullptr, 1, nc, HIPFFT_D2Z, cnt));
+      FFTCHK(hipfftPlanMany(&z2d, 2, dims, nullptr, 1, nc, nullptr, 1, nr, HIPFFT_Z2D, cnt));
+      M = M2;
+      N = N2;
+      count = cnt;
+    }
+    TRY(pad.ensure((size_t)cnt * M * N));
+    TRY(spec.ensure((size_t)cnt * N * (M / 2 + 1)));
+    FFTCHK(hipfftSetStream(d2z, st));
+    FFTCHK(hipfftSetStream(z2d, st));
+    auto blocks = [](long long tot) {
+      return dim3((unsigned)std::min<long long>(std::max<long long>(cdiv(tot, 256), 1), 8192));
+    };
+    hipLaunchKernelGGL(k_img_pad, blocks((long long)cnt * M * N), dim3(256), 0, st, in, m, n, M, N,
+                       (long long)cnt, pad.p);
+    HIPCHK(hipGetLastError());
+    FFTCHK(hipfftExecD2Z(d2z, reinterpret_cast<hipfftDoubleReal*>(pad.p),
+                         reinterpret_cast<hipfftDoubleComplex*>(spec.p)));
+    // image_gaussian.m:37: a = n_ (ax/2) sqrt(2), b = m_ (az/2) sqrt(2)
+    const double a = N * (ax / 2) * std::sqrt(2.0), b = M * (az / 2) * std::sqrt(2.0);
+    hipLaunchKernelGGL(k_img_kfilter, blocks((long long)cnt * N * (M / 2 + 1)), dim3(256), 0, st,
+                       spec.p, M, N, (long long)cnt, a, b, type);
+    HIPCHK(hipGetLastError());
+    FFTCHK(hipfftExecZ2D(z2d, reinterpret_cast<hipfftDoubleComplex*>(spec.p),
+                         reinterpret_cast<hipfftDoubleReal*>(pad.p)));
+    hipLaunchKernelGGL(k_img_trim, blocks((long long)cnt * m * n), dim3(256), 0, st, pad.p, M, N,
+                       m, n, (long long)cnt, out);
+    HIPCHK(hipGetLastError());
+    return GJW_OK;
+  }
+};
+
+static void post_free(PostWork* w) {
+  if (!w) return;
+  w->release();
+  delete w;
+}
+
 static bool filter_on(const gjw_survey* s) { return s->flt_hi > s->flt_lo; }
 
 // the survey's band filter on cnt traces at p (nt samples each)
@@ -1566,7 +1666,8 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
       acur ^= 1;
     }
     hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
-                       s->gr.n, s->LD, s->di0, ns, s->acc.p, FD, s->fe.p, s->fs.p, s->ge.p, s->gsg.p, 1.0);
+                       s->gr.n, s->LD, s->di0, ns, s->acc.p, FD, s->fe.p, s->fs.p, s->ge.p, s->gsg.p, 1.0,
+                       grad_stride(s));
     HIPCHK(hipGetLastError());
   }
   if (explore) {
@@ -1615,7 +1716,8 @@ static int grad_plain_shot(gjw_survey* s, int q, const float* adj, const float* 
     acur ^= 1;
     if (it % K == 0) {
       hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
-                         s->gr.n, s->LD, s->di0, 1, s->acc.p, FD, s->fe.p, s->fs.p, s->ge.p, s->gsg.p, 1.0);
+                         s->gr.n, s->LD, s->di0, 1, s->acc.p, FD, s->fe.p, s->fs.p,
+                         s->ge.p + q * grad_stride(s), s->gsg.p + q * grad_stride(s), 1.0, 0LL);
       HIPCHK(hipGetLastError());
     }
   }
@@ -1637,7 +1739,7 @@ static int fwi_plain(gjw_survey* s, const float* d_obs) {
   if (s->misfit_kind != GJW_MISFIT_L2)
     return fail(GJW_ERR_ARG, "gjw_fwi_gradient: the plain path computes the L2 misfit only");
   const int ns = s->ns, nt = s->gr.nt;
-  const long long per = (long long)s->gr.m * s->gr.n, n = (long long)s->ntr * nt;
+  const long long n = (long long)s->ntr * nt;
   const long long FD = (long long)s->LD * s->gr.n;
   hipStream_t st = s->ctx->stream;
   const int B = 64;  // residual blocks per shot
@@ -1649,12 +1751,12 @@ static int fwi_plain(gjw_survey* s, const float* d_obs) {
   TRY(s->Jd.ensure(1));
   TRY(s->frames.ensure((size_t)(nt + 1) * FD));
   TRY(s->acc.ensure((size_t)2 * FD));
-  TRY(s->ge.ensure(per));
-  TRY(s->gsg.ensure(per));
+  TRY(s->ge.ensure(grad_len(s)));
+  TRY(s->gsg.ensure(grad_len(s)));
   TRY(host_or_dev_copy(s->obs.p, d_obs, n * 4, st));
   TRY(launch_fill(s->acc.p, 2LL * FD, 0.f, st));
-  TRY(launch_fill_d(s->ge.p, per, 0.0, st));
-  TRY(launch_fill_d(s->gsg.p, per, 0.0, st));
+  TRY(launch_fill_d(s->ge.p, grad_len(s), 0.0, st));
+  TRY(launch_fill_d(s->gsg.p, grad_len(s), 0.0, st));
   TRY(launch_fill_d(s->part.p, (long long)ns * B, 0.0, st));
   for (int q = 0; q < ns; ++q) {
     TRY(forward_plain_shot(s, q, s->frames.p));
@@ -1727,8 +1829,9 @@ static int grad_ref(gjw_survey* s, const float* adj, float* amax) {
       HIPCHK(hipGetLastError());
       if (it % K == K - 1 || it == nt - 1) {
         hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
-                           s->gr.n, s->LD, s->di0, 1, s->acc.p, FD, nullptr, nullptr, s->ge.p, s->gsg.p,
-                           -s->gr.dt);
+                           s->gr.n, s->LD, s->di0, 1, s->acc.p, FD, nullptr, nullptr,
+                           s->ge.p + q * grad_stride(s), s->gsg.p + q * grad_stride(s), -s->gr.dt,
+                           0LL);
         HIPCHK(hipGetLastError());
       }
     }
@@ -1757,10 +1860,10 @@ static int grad_run(gjw_survey* s, const float* adj_in, int32_t mode, double* g_
     if (!s->have_adj) return fail(GJW_ERR_STATE, "gjw_gradient: no adjoint source (call gjw_misfit)");
     adj = s->adj.p;
   }
-  TRY(s->ge.ensure(per));
-  TRY(s->gsg.ensure(per));
-  TRY(launch_fill_d(s->ge.p, per, 0.0, st));
-  TRY(launch_fill_d(s->gsg.p, per, 0.0, st));
+  TRY(s->ge.ensure(grad_len(s)));
+  TRY(s->gsg.ensure(grad_len(s)));
+  TRY(launch_fill_d(s->ge.p, grad_len(s), 0.0, st));
+  TRY(launch_fill_d(s->gsg.p, grad_len(s), 0.0, st));
   float* gwf = nullptr;
   DBuf<float> gwtmp;
   if (g_w) {
@@ -1781,8 +1884,8 @@ static int grad_run(gjw_survey* s, const float* adj_in, int32_t mode, double* g_
   HIPCHK(hipEventRecord(s->ctx->ev1, st));
   TRY(finish_timing(s));
   s->last_cells = (double)(s->NX + 1) * (s->NY + 1) * nt * ns * (s->kept == 1 ? 2 : 1);
-  if (g_eps) HIPCHK(hipMemcpyAsync(g_eps, s->ge.p, per * 8, hipMemcpyDefault, st));
-  if (g_sig) HIPCHK(hipMemcpyAsync(g_sig, s->gsg.p, per * 8, hipMemcpyDefault, st));
+  if (g_eps) HIPCHK(hipMemcpyAsync(g_eps, s->ge.p, grad_len(s) * 8, hipMemcpyDefault, st));
+  if (g_sig) HIPCHK(hipMemcpyAsync(g_sig, s->gsg.p, grad_len(s) * 8, hipMemcpyDefault, st));
   if (g_w) {
     std::vector<float> h((size_t)ns * nt);
     HIPCHK(hipMemcpyAsync(h.data(), gwf, h.size() * 4, hipMemcpyDeviceToHost, st));
@@ -1815,7 +1918,7 @@ extern "C" int gjw_gradient(gjw_survey* s, const float* adj, int32_t mode, doubl
 // are expected in s->obs.
 static int fwi_batched_prepare(gjw_survey* s) {
   const int ns = s->ns, nt = s->gr.nt;
-  const long long n = (long long)s->ntr * nt, per = (long long)s->gr.m * s->gr.n;
+  const long long n = (long long)s->ntr * nt;
   const long long FD = (long long)s->LD * s->gr.n;
   TRY(ensure_fwd_state(s, ns));
   TRY(s->tr.ensure(std::max<long long>(n, 1)));
@@ -1828,8 +1931,8 @@ static int fwi_batched_prepare(gjw_survey* s) {
   TRY(s->obs.ensure(std::max<long long>(n, 1)));
   TRY(s->part.ensure(1024));
   TRY(s->Jd.ensure(1));
-  TRY(s->ge.ensure(per));
-  TRY(s->gsg.ensure(per));
+  TRY(s->ge.ensure(grad_len(s)));
+  TRY(s->gsg.ensure(grad_len(s)));
   TRY(s->frames.ensure((size_t)ns * (s->ck_K + 1) * FD));
   TRY(s->acc.ensure((size_t)2 * ns * FD));
   TRY(s->adjf.ensure((size_t)10 * ns * s->FS));
@@ -1855,10 +1958,9 @@ static int fwi_fwd_enqueue(gjw_survey* s) {
 }
 
 static int fwi_adj_enqueue(gjw_survey* s) {
-  const long long per = (long long)s->gr.m * s->gr.n;
   hipStream_t st = s->ctx->stream;
-  TRY(launch_fill_d(s->ge.p, per, 0.0, st));
-  TRY(launch_fill_d(s->gsg.p, per, 0.0, st));
+  TRY(launch_fill_d(s->ge.p, grad_len(s), 0.0, st));
+  TRY(launch_fill_d(s->gsg.p, grad_len(s), 0.0, st));
   return grad_batched(s, s->adj.p, nullptr);
 }
 
@@ -1915,7 +2017,7 @@ static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
           u(s->acc.p),    u(s->adjf.p),  u(s->coef.p),   u(s->fe.p),    u(s->fs.p),
           u(s->d_pml.p),  u(s->d_si.p),  u(s->d_sj.p),   u(s->d_sc.p),  u(s->d_wav.p),
           u(s->d_cptr.p), u(s->d_crow.p), u(s->d_cid.p), u(s->d_trs.p), u(s->d_trr.p),
-          u(s->d_trc.p),  (uintptr_t)s->ns, (uintptr_t)s->ntr,
+          u(s->d_trc.p),  (uintptr_t)s->ns, (uintptr_t)s->ntr, (uintptr_t)s->shot_grads,
           (uintptr_t)s->gr.nt, (uintptr_t)s->tuned_L, (uintptr_t)s->tuned_L1,
           (uintptr_t)s->tuned_L4, (uintptr_t)s->tuned_L8, (uintptr_t)s->fuse_eff,
           (uintptr_t)s->tuned_La2,
@@ -1946,7 +2048,6 @@ extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps
   TRY(check_ready(s, "gjw_fwi_gradient"));
   if (!d_obs) return fail(GJW_ERR_ARG, "gjw_fwi_gradient: d_obs is NULL");
   hipStream_t st = s->ctx->stream;
-  const long long per = (long long)s->gr.m * s->gr.n;
   hipEvent_t e0;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventRecord(e0, st));
@@ -1957,9 +2058,9 @@ extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps
       if (hipEventRecord(s->ctx->ev1, st) != hipSuccess) rc = fail(GJW_ERR_HIP, "event record");
       if (rc == GJW_OK) rc = kev_resolve(s);
       if (rc == GJW_OK && g_eps)
-        rc = host_or_dev_copy(g_eps, s->ge.p, per * 8, st);
+        rc = host_or_dev_copy(g_eps, s->ge.p, grad_len(s) * 8, st);
       if (rc == GJW_OK && g_sig)
-        rc = host_or_dev_copy(g_sig, s->gsg.p, per * 8, st);
+        rc = host_or_dev_copy(g_sig, s->gsg.p, grad_len(s) * 8, st);
     }
   } else {
     rc = fwi_batched_prepare(s);
@@ -1974,8 +2075,8 @@ extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps
     if (rc == GJW_OK) {
       s->kept = 1;
       s->have_adj = true;
-      if (g_eps) rc = host_or_dev_copy(g_eps, s->ge.p, per * 8, st);
-      if (rc == GJW_OK && g_sig) rc = host_or_dev_copy(g_sig, s->gsg.p, per * 8, st);
+      if (g_eps) rc = host_or_dev_copy(g_eps, s->ge.p, grad_len(s) * 8, st);
+      if (rc == GJW_OK && g_sig) rc = host_or_dev_copy(g_sig, s->gsg.p, grad_len(s) * 8, st);
     }
   }
   if (rc != GJW_OK) {
@@ -2165,11 +2266,11 @@ static int batch_adj_enqueue(gjw_batch* b) {
       acur[kMaxSurveys];
   for (int k = 0; k < S; ++k) {
     gjw_survey* s = b->sv[k];
-    const long long per = (long long)s->gr.m * s->gr.n, FD = (long long)s->LD * s->gr.n;
+    const long long FD = (long long)s->LD * s->gr.n;
     nt[k] = s->gr.nt;
     acur[k] = 0;
-    TRY(launch_fill_d(s->ge.p, per, 0.0, st));
-    TRY(launch_fill_d(s->gsg.p, per, 0.0, st));
+    TRY(launch_fill_d(s->ge.p, grad_len(s), 0.0, st));
+    TRY(launch_fill_d(s->gsg.p, grad_len(s), 0.0, st));
     TRY(launch_fill(s->acc.p, 2LL * s->ns * FD, 0.f, st));
     TRY(adj_init(s, s->ns, 0, s->adj.p));
   }
@@ -2234,7 +2335,7 @@ static int batch_adj_enqueue(gjw_batch* b) {
       const long long per = (long long)s->gr.m * s->gr.n, FD = (long long)s->LD * s->gr.n;
       hipLaunchKernelGGL(k_flush, dim3((unsigned)cdiv(per, 256)), dim3(256), 0, st, s->gr.m,
                          s->gr.n, s->LD, s->di0, s->ns, s->acc.p, FD, s->fe.p, s->fs.p, s->ge.p,
-                         s->gsg.p, 1.0);
+                         s->gsg.p, 1.0, grad_stride(s));
       HIPCHK(hipGetLastError());
     }
   }
@@ -2317,11 +2418,10 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
   double cells = 0;
   for (int k = 0; k < S; ++k) {
     gjw_survey* s = b->sv[k];
-    const long long per = (long long)s->gr.m * s->gr.n;
     s->kept = 0;
     s->have_adj = true;
-    if (g_eps && g_eps[k]) TRY(host_or_dev_copy(g_eps[k], s->ge.p, per * 8, st));
-    if (g_sig && g_sig[k]) TRY(host_or_dev_copy(g_sig[k], s->gsg.p, per * 8, st));
+    if (g_eps && g_eps[k]) TRY(host_or_dev_copy(g_eps[k], s->ge.p, grad_len(s) * 8, st));
+    if (g_sig && g_sig[k]) TRY(host_or_dev_copy(g_sig[k], s->gsg.p, grad_len(s) * 8, st));
     if (J) HIPCHK(hipMemcpyAsync(J + k, s->Jd.p, 8, hipMemcpyDefault, st));
     cells += 3.0 * (s->NX + 1) * (s->NY + 1) * (double)s->gr.nt * s->ns;
   }
@@ -2550,5 +2650,157 @@ extern "C" int gjw_differentiate_cube(gjw_ctx* c, float* v, int32_t nz, int32_t 
     HIPCHK(hipFree(d));
   }
   HIPCHK(hipStreamSynchronize(c->stream));
+  return GJW_OK;
+}
+
+// --------------------------------------------------------------------------
+// gradient post-processing (w_update_e.m:108-145, w_update_e_.m:28-42,
+// w_image_e.m:50-51) on batches of FP64 m x n images
+// --------------------------------------------------------------------------
+static PostWork* post_work(gjw_ctx* c) {
+  if (!c->post) c->post = new PostWork();
+  return c->post;
+}
+
+static dim3 grid_for(long long tot) {
+  return dim3((unsigned)std::min<long long>(std::max<long long>(cdiv(tot, 256), 1), 8192));
+}
+
+extern "C" int gjw_gaussi(gjw_ctx* c, double* g, int32_t m, int32_t n, int32_t count,
+                          const double* X, const double* Y, const int32_t* s, const double* lo) {
+  if (!c || (count > 0 && (!g || !X || !Y || !s || !lo)))
+    return fail(GJW_ERR_ARG, "gjw_gaussi: NULL argument");
+  if (m < 1 || n < 1 || count < 0) return fail(GJW_ERR_ARG, "gjw_gaussi: bad sizes");
+  if (count == 0) return GJW_OK;
+  for (int k = 0; k < count; ++k) {
+    if (s[2 * k] < 1 || s[2 * k] > m || s[2 * k + 1] < 1 || s[2 * k + 1] > n)
+      return fail(GJW_ERR_ARG, "gjw_gaussi: source %d at (%d, %d) outside the %d x %d model", k,
+                  s[2 * k], s[2 * k + 1], m, n);
+    if (!(lo[k] > 0) || !std::isfinite(lo[k]))
+      return fail(GJW_ERR_ARG, "gjw_gaussi: lo[%d] must be positive and finite", k);
+  }
+  HIPCHK(hipSetDevice(c->device));
+  PostWork* w = post_work(c);
+  // small host arrays -> device: X (n), Y (m), lo (count), gmax (count); s
+  std::vector<double> hv((size_t)n + m + count);
+  std::copy(X, X + n, hv.begin());
+  std::copy(Y, Y + m, hv.begin() + n);
+  std::copy(lo, lo + count, hv.begin() + n + m);
+  TRY(w->dsmall.ensure(hv.size() + count));
+  TRY(w->ismall.ensure((size_t)2 * count));
+  HIPCHK(hipMemcpy(w->dsmall.p, hv.data(), hv.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(w->ismall.p, s, (size_t)2 * count * 4, hipMemcpyHostToDevice));
+  const double *dX = w->dsmall.p, *dY = dX + n, *dlo = dY + m;
+  double* gmax = w->dsmall.p + hv.size();
+  const size_t bytes = (size_t)count * m * n * 8;
+  void* dp;
+  bool own;
+  TRY(stage_begin(c, g, bytes, &dp, &own));
+  hipLaunchKernelGGL(k_gaussi_norm, dim3(count), dim3(256), 0, c->stream, dX, n, dY, m,
+                     w->ismall.p, dlo, gmax);
+  bool ok = hipGetLastError() == hipSuccess;
+  if (ok) {
+    hipLaunchKernelGGL(k_gaussi_apply, grid_for((long long)count * m * n), dim3(256), 0, c->stream,
+                       static_cast<double*>(dp), m, n, (long long)count, dX, dY, w->ismall.p, dlo,
+                       gmax);
+    ok = hipGetLastError() == hipSuccess;
+  }
+  TRY(stage_end(c, g, dp, bytes, own, ok));
+  if (!ok) return fail(GJW_ERR_HIP, "gjw_gaussi: launch failed");
+  return GJW_OK;
+}
+
+extern "C" int gjw_image_gaussian(gjw_ctx* c, const double* in, double* out, int32_t m,
+                                  int32_t n, int32_t count, double ax, double az, int32_t type) {
+  if (!c || (count > 0 && (!in || !out)))
+    return fail(GJW_ERR_ARG, "gjw_image_gaussian: NULL argument");
+  if (m < 2 || n < 2 || count < 0)
+    return fail(GJW_ERR_ARG, "gjw_image_gaussian: images must be at least 2 x 2");
+  if (type != GJW_LOW_PASS && type != GJW_HI_PASS)
+    return fail(GJW_ERR_ARG, "gjw_image_gaussian: type must be GJW_LOW_PASS or GJW_HI_PASS");
+  if (!(ax > 0) || !(az > 0) || !std::isfinite(ax) || !std::isfinite(az))
+    return fail(GJW_ERR_ARG, "gjw_image_gaussian: ax and az must be positive and finite");
+  if (count == 0) return GJW_OK;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t bytes = (size_t)count * m * n * 8;
+  void *di, *dout;
+  bool oi, oo;
+  TRY(stage_begin(c, const_cast<double*>(in), bytes, &di, &oi));
+  if (out == in) {
+    dout = di;
+    oo = false;
+  } else {
+    TRY(stage_begin(c, out, bytes, &dout, &oo));
+  }
+  const int rc = post_work(c)->kfilter(c->stream, static_cast<const double*>(di),
+                                       static_cast<double*>(dout), m, n, count, ax, az, type);
+  if (out == in) {
+    TRY(stage_end(c, out, di, bytes, oi, rc == GJW_OK));
+  } else {
+    TRY(stage_end(c, const_cast<double*>(in), di, bytes, oi, false));
+    TRY(stage_end(c, out, dout, bytes, oo, rc == GJW_OK));
+  }
+  return rc;
+}
+
+extern "C" int gjw_normali(gjw_ctx* c, double* a, int32_t m, int32_t n, int32_t count,
+                           int32_t nan_guard, const double* shape) {
+  if (!c || (count > 0 && !a)) return fail(GJW_ERR_ARG, "gjw_normali: NULL argument");
+  if (m < 1 || n < 1 || count < 0) return fail(GJW_ERR_ARG, "gjw_normali: bad sizes");
+  if (count == 0) return GJW_OK;
+  if (count > 65535) return fail(GJW_ERR_ARG, "gjw_normali: at most 65535 images per call");
+  HIPCHK(hipSetDevice(c->device));
+  PostWork* w = post_work(c);
+  const long long per = (long long)m * n;
+  const int P = (int)std::min<long long>(256, std::max<long long>(1, cdiv(per, 256 * 16)));
+  TRY(w->dsmall.ensure((size_t)count * P));
+  TRY(w->ismall.ensure((size_t)count * P));
+  const size_t bytes = (size_t)per * count * 8;
+  void *dp, *dsh = nullptr;
+  bool own, osh = false;
+  TRY(stage_begin(c, a, bytes, &dp, &own));
+  if (shape) TRY(stage_begin(c, const_cast<double*>(shape), (size_t)per * 8, &dsh, &osh));
+  hipLaunchKernelGGL(k_absmax_part, dim3(P, count), dim3(256), 0, c->stream,
+                     static_cast<const double*>(dp), per, w->dsmall.p, w->ismall.p);
+  bool ok = hipGetLastError() == hipSuccess;
+  if (ok) {
+    const unsigned bx = (unsigned)std::min<long long>(cdiv(per, 256), 1024);
+    hipLaunchKernelGGL(k_normali, dim3(bx, count), dim3(256), 0, c->stream,
+                       static_cast<double*>(dp), per, P, w->dsmall.p, w->ismall.p, nan_guard ? 1 : 0,
+                       static_cast<const double*>(dsh));
+    ok = hipGetLastError() == hipSuccess;
+  }
+  if (shape) TRY(stage_end(c, const_cast<double*>(shape), dsh, (size_t)per * 8, osh, false));
+  TRY(stage_end(c, a, dp, bytes, own, ok));
+  if (!ok) return fail(GJW_ERR_HIP, "gjw_normali: launch failed");
+  return GJW_OK;
+}
+
+extern "C" int gjw_stack_update(gjw_ctx* c, const double* g, int32_t m, int32_t n, int32_t count,
+                                const double* steps, double* dparam, double* model) {
+  if (!c || !g || !steps || !dparam) return fail(GJW_ERR_ARG, "gjw_stack_update: NULL argument");
+  if (m < 1 || n < 1 || count < 1) return fail(GJW_ERR_ARG, "gjw_stack_update: bad sizes");
+  HIPCHK(hipSetDevice(c->device));
+  PostWork* w = post_work(c);
+  // w_update_e_.m:40-41: the update is dropped when the steps sum below zero
+  double ssum = 0.0;
+  for (int k = 0; k < count; ++k) ssum += steps[k];
+  const int zero = ssum < 0 ? 1 : 0;
+  TRY(w->dsmall.ensure((size_t)count));
+  HIPCHK(hipMemcpy(w->dsmall.p, steps, (size_t)count * 8, hipMemcpyHostToDevice));
+  const long long per = (long long)m * n;
+  void *dg, *dd, *dm = nullptr;
+  bool og, od, om = false;
+  TRY(stage_begin(c, const_cast<double*>(g), (size_t)per * count * 8, &dg, &og));
+  TRY(stage_begin(c, dparam, (size_t)per * 8, &dd, &od));
+  if (model) TRY(stage_begin(c, model, (size_t)per * 8, &dm, &om));
+  hipLaunchKernelGGL(k_stack_update, grid_for(per), dim3(256), 0, c->stream,
+                     static_cast<const double*>(dg), per, count, w->dsmall.p, zero,
+                     static_cast<double*>(dd), static_cast<double*>(dm));
+  const bool ok = hipGetLastError() == hipSuccess;
+  TRY(stage_end(c, const_cast<double*>(g), dg, (size_t)per * count * 8, og, false));
+  TRY(stage_end(c, dparam, dd, (size_t)per * 8, od, ok));
+  if (model) TRY(stage_end(c, model, dm, (size_t)per * 8, om, ok));
+  if (!ok) return fail(GJW_ERR_HIP, "gjw_stack_update: launch failed");
   return GJW_OK;
 }

@@ -2095,25 +2095,35 @@ __global__ void k_domain_frame(DevGrid g, int ns, const float* __restrict__ E,
 }
 
 // acc (float2 per domain cell per shot) -> FP64 gradient, summed over shots
-// in shot order; acc is cleared for the next segment.
+// in shot order, or with gstride != 0 kept per shot (shot sh at
+// ge + sh*gstride, as w_update_e.m keeps g_e per source before its
+// post-processing); acc is cleared for the next segment.
 __global__ void k_flush(int m, int n, int LD, int di0, int ns, float* __restrict__ acc,
                         long long astride, const double* __restrict__ fe,
                         const double* __restrict__ fs, double* __restrict__ ge,
-                        double* __restrict__ gs, double scale) {
+                        double* __restrict__ gs, double scale, long long gstride) {
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const long long per = (long long)m * n;
   if (t >= per) return;
   const long long ka = (t / m) * LD + (t % m) + di0;  // partials use the field pitch
   float2* a2 = reinterpret_cast<float2*>(acc);
+  const double ce = fe ? fe[t] : scale, cs = fs ? fs[t] : scale;
   double se = 0.0, ss = 0.0;
   for (int sh = 0; sh < ns; ++sh) {
     float2 v = a2[(long long)sh * astride + ka];
-    se += (double)v.x;
-    ss += (double)v.y;
+    if (gstride) {  // one gradient per shot
+      ge[sh * gstride + t] += ce * (double)v.x;
+      gs[sh * gstride + t] += cs * (double)v.y;
+    } else {
+      se += (double)v.x;
+      ss += (double)v.y;
+    }
     a2[(long long)sh * astride + ka] = make_float2(0.f, 0.f);
   }
-  ge[t] += (fe ? fe[t] : scale) * se;
-  gs[t] += (fs ? fs[t] : scale) * ss;
+  if (!gstride) {
+    ge[t] += ce * se;
+    gs[t] += cs * ss;
+  }
 }
 
 // L2 residual e = d - d_o (w_bozdag.m:13-15) and per-block partial sums of
@@ -2320,6 +2330,209 @@ __global__ void k_lmute(float* __restrict__ d, float* __restrict__ d_o, float* _
     d[p] = a;
     d_o[p] = b;
     if (e) e[p] = a - b;
+  }
+}
+
+
+// --------------------------------------------------------------------------
+// gradient post-processing (w_update_e.m:108-145, w_update_e_.m:38-41,
+// w_image_e.m:50-51).  Images are FP64, m x n column-major (depth fastest),
+// image c at offset c*m*n.
+// --------------------------------------------------------------------------
+
+// Normaliser of the source taper of image c: the maximum over the grid of
+// 1 - exp(-(((x_j - x_s)/L)^2 + ((z_i - z_s)/L)^2)), L = 2*lo (gaussi.m:12-16).
+// The exponent is a sum of a term in j and a term in i and the map is
+// increasing, so the maximum sits at the largest term of each; 0 becomes 1
+// (normali.m:3-5).  One block per image.
+__global__ void k_gaussi_norm(const double* __restrict__ X, int n, const double* __restrict__ Y,
+                              int m, const int* __restrict__ s, const double* __restrict__ lo,
+                              double* __restrict__ gmax) {
+  __shared__ double sx[256], sz[256];
+  const int c = blockIdx.x;
+  const double L = 2.0 * lo[c];
+  const double xs = X[s[2 * c + 1] - 1], zs = Y[s[2 * c] - 1];
+  double ax = 0.0, az = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const double a = (X[j] - xs) / L;
+    ax = fmax(ax, a * a);
+  }
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const double b = (Y[i] - zs) / L;
+    az = fmax(az, b * b);
+  }
+  sx[threadIdx.x] = ax;
+  sz[threadIdx.x] = az;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + w]);
+      sz[threadIdx.x] = fmax(sz[threadIdx.x], sz[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double g = 1.0 - exp(-(sx[0] + sz[0]));
+    gmax[c] = g == 0.0 ? 1.0 : g;
+  }
+}
+
+// g_c(i,j) *= (1 - exp(-(((x_j - x_s)/L)^2 + ((z_i - z_s)/L)^2))) / gmax_c
+// (gaussi.m:10-16 times the gradient, w_update_e.m:118)
+__global__ void k_gaussi_apply(double* __restrict__ g, int m, int n, long long count,
+                               const double* __restrict__ X, const double* __restrict__ Y,
+                               const int* __restrict__ s, const double* __restrict__ lo,
+                               const double* __restrict__ gmax) {
+  const long long per = (long long)m * n, tot = per * count;
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < tot;
+       p += (long long)gridDim.x * blockDim.x) {
+    const long long c = p / per, q = p - c * per;
+    const int j = (int)(q / m), i = (int)(q - (long long)j * m);
+    const double L = 2.0 * lo[c];
+    const double a = (X[j] - X[s[2 * c + 1] - 1]) / L, b = (Y[i] - Y[s[2 * c] - 1]) / L;
+    const double t = (1.0 - exp(-(a * a + b * b))) / gmax[c];
+    g[p] = t * g[p];
+  }
+}
+
+// zero-padded copy of m x n images into M x N (powers of two) real planes
+// (image_gaussian.m:22-26)
+__global__ void k_img_pad(const double* __restrict__ in, int m, int n, int M, int N,
+                          long long count, double* __restrict__ pad) {
+  const long long per = (long long)M * N, tot = per * count;
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < tot;
+       p += (long long)gridDim.x * blockDim.x) {
+    const long long c = p / per, q = p - c * per;
+    const int j = (int)(q / M), i = (int)(q - (long long)j * M);
+    pad[p] = (i < m && j < n) ? in[c * (long long)m * n + (long long)j * m + i] : 0.0;
+  }
+}
+
+// k-space filter of image_gaussian.m:37-53 on the one-sided spectrum of the
+// padded plane (bins kx = 0..N-1 along x, ky = 0..M/2 along depth).  After
+// the fftshift of line 50, bin k carries the filter value of the centred
+// grid at (k + N/2) mod N, i.e. the signed offset ((k + N/2) mod N) - N/2.
+// The filter is even in both offsets, so the product stays Hermitian and the
+// real part of ifft2 is the C2R inverse; 1/(M N) is that inverse's scale.
+// type 0: LOW_PASS exp(-r2) (maximum 1); type 1: HI_PASS 1 - exp(-r2),
+// whose maximum is at the grid corner (offsets -N/2, -M/2).
+__global__ void k_img_kfilter(double2* __restrict__ spec, int M, int N, long long count, double a,
+                              double b, int type) {
+  const int mb = M / 2 + 1;
+  const long long per = (long long)N * mb, tot = per * count;
+  const double x0 = 0.5 * N, y0 = 0.5 * M;
+  double vmax = 1.0;
+  if (type == 1) {
+    const double u = (0.0 - x0) / a, v = (0.0 - y0) / b;
+    vmax = 1.0 - exp(-(u * u + v * v));
+  }
+  const double inv = 1.0 / ((double)M * (double)N);
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < tot;
+       p += (long long)gridDim.x * blockDim.x) {
+    const long long q = p % per;
+    const int kx = (int)(q / mb), ky = (int)(q - (long long)kx * mb);
+    const double u = ((double)((kx + N / 2) % N) - x0) / a;
+    const double v = ((double)((ky + M / 2) % M) - y0) / b;
+    const double e = exp(-(u * u + v * v));
+    const double h = (type == 1 ? 1.0 - e : e) / vmax * inv;
+    const double2 z = spec[p];
+    spec[p] = make_double2(h * z.x, h * z.y);
+  }
+}
+
+// out_c(i,j) = pad_c(i,j), i < m, j < n (image_gaussian.m:59)
+__global__ void k_img_trim(const double* __restrict__ pad, int M, int N, int m, int n,
+                           long long count, double* __restrict__ out) {
+  const long long per = (long long)m * n, tot = per * count;
+  for (long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x; p < tot;
+       p += (long long)gridDim.x * blockDim.x) {
+    const long long c = p / per, q = p - c * per;
+    const int j = (int)(q / m), i = (int)(q - (long long)j * m);
+    out[p] = pad[c * (long long)M * N + (long long)j * M + i];
+  }
+}
+
+// per-block max |a| (NaN ignored, as MATLAB's max) and NaN count over image
+// blockIdx.y; gridDim.x blocks per image
+__global__ void k_absmax_part(const double* __restrict__ a, long long per,
+                              double* __restrict__ pmax, int* __restrict__ pnan) {
+  __shared__ double sm[256];
+  __shared__ int sn[256];
+  const long long c = blockIdx.y;
+  const double* x = a + c * per;
+  double mx = 0.0;
+  int nn = 0;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < per;
+       q += (long long)gridDim.x * blockDim.x) {
+    const double v = x[q];
+    mx = fmax(mx, fabs(v));
+    nn += isnan(v) ? 1 : 0;
+  }
+  sm[threadIdx.x] = mx;
+  sn[threadIdx.x] = nn;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+      sn[threadIdx.x] += sn[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pmax[c * gridDim.x + blockIdx.x] = sm[0];
+    pnan[c * gridDim.x + blockIdx.x] = sn[0];
+  }
+}
+
+// a /= max|a| (0 -> 1) per image (normali.m:2-6); with guard, an image that
+// holds a NaN becomes zeros (w_update_e.m:133-137); then a .*= shape when a
+// mask is given (w_update_e.m:143).  Each block first combines its image's
+// P partials (P <= 256).
+__global__ void k_normali(double* __restrict__ a, long long per, int P,
+                          const double* __restrict__ pmax, const int* __restrict__ pnan,
+                          int guard, const double* __restrict__ shape) {
+  __shared__ double sm[256];
+  __shared__ int sn[256];
+  const long long c = blockIdx.y;
+  sm[threadIdx.x] = (int)threadIdx.x < P ? pmax[c * P + threadIdx.x] : 0.0;
+  sn[threadIdx.x] = (int)threadIdx.x < P ? pnan[c * P + threadIdx.x] : 0;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+      sn[threadIdx.x] += sn[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  const double mx = sm[0] == 0.0 ? 1.0 : sm[0];
+  const bool zero = guard && sn[0] > 0;
+  double* x = a + c * per;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < per;
+       q += (long long)gridDim.x * blockDim.x) {
+    const double v = zero ? 0.0 : x[q] / mx;
+    x[q] = shape ? v * shape[q] : v;
+  }
+}
+
+// d(t) = sum_c (-step_c * g_c(t)) / count, in image order (w_update_e.m:177
+// and w_update_e_.m:38); zero when the steps sum below zero
+// (w_update_e_.m:40-41, flag computed by the caller).  With a model,
+// model(t) *= exp(model(t) * d(t)) (w_image_e.m:50-51).
+__global__ void k_stack_update(const double* __restrict__ g, long long per, int count,
+                               const double* __restrict__ step, int zero, double* __restrict__ d,
+                               double* __restrict__ model) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < per;
+       t += (long long)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    if (!zero) {
+      for (int c = 0; c < count; ++c) acc += -step[c] * g[c * per + t];
+      acc /= (double)count;
+    }
+    d[t] = acc;
+    if (model) {
+      const double e = model[t];
+      model[t] = e * exp(e * acc);
+    }
   }
 }
 

@@ -150,6 +150,74 @@ class Context:
         return e
 
 
+    # -- gradient post-processing (w_update_e.m:108-145) ------------------
+    @staticmethod
+    def _images(a):
+        """(m, n, count) view of an FP64 image batch: numpy F-order (m, n) or
+        (m, n, count), or a torch tensor of shape (count, n, m) / (n, m)
+        (contiguous, i.e. column-major images)."""
+        if hasattr(a, "data_ptr"):
+            assert str(a.dtype) == "torch.float64" and a.is_contiguous()
+            sh = tuple(a.shape)
+            if len(sh) == 2:
+                sh = (1,) + sh
+            return sh[2], sh[1], sh[0]
+        assert isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.f_contiguous
+        if a.ndim == 2:
+            return a.shape[0], a.shape[1], 1
+        return a.shape
+
+    def gaussi(self, g, X, Y, s, lo):
+        """gaussi.m taper of each image in place: s is (count, 2) 1-based
+        (iz, ix) model indices, lo (count,) gaussi's lo argument."""
+        m, n, cnt = self._images(g)
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        s = np.ascontiguousarray(np.reshape(s, (cnt, 2)), dtype=np.int32)
+        lo = np.ascontiguousarray(np.reshape(lo, (cnt,)), dtype=np.float64)
+        assert X.size == n and Y.size == m
+        _sync_torch(g)
+        N.check(N.load().gjw_gaussi(self.h, N.ptr(g), m, n, cnt, N.ptr(X), N.ptr(Y), N.ptr(s),
+                                    N.ptr(lo)), "gjw_gaussi")
+        return g
+
+    def image_gaussian(self, img, ax, az, kind="LOW_PASS", out=None):
+        """image_gaussian.m on each image (returns the filtered images)."""
+        m, n, cnt = self._images(img)
+        if out is None:
+            out = img.clone() if hasattr(img, "data_ptr") else np.empty_like(img, order="F")
+        typ = {"LOW_PASS": N.GJW_LOW_PASS, "HI_PASS": N.GJW_HI_PASS}[kind]
+        _sync_torch(img, out)
+        N.check(N.load().gjw_image_gaussian(self.h, N.ptr(img), N.ptr(out), m, n, cnt, float(ax),
+                                            float(az), typ), "gjw_image_gaussian")
+        return out
+
+    def normali(self, a, nan_guard=False, shape=None):
+        """normali.m on each image in place; nan_guard zeroes images with a
+        NaN (w_update_e.m:133-137); shape (m, n) then masks every image
+        (w_update_e.m:141-143)."""
+        m, n, cnt = self._images(a)
+        if shape is not None and not hasattr(shape, "data_ptr"):
+            shape = np.asfortranarray(shape, dtype=np.float64)
+        _sync_torch(a, shape)
+        N.check(N.load().gjw_normali(self.h, N.ptr(a), m, n, cnt, int(bool(nan_guard)),
+                                     N.ptr(shape)), "gjw_normali")
+        return a
+
+    def stack_update(self, g, steps, model=None, out=None):
+        """w_update_e_.m:38-41 (and w_image_e.m:50-51 when a model is given,
+        updated in place): returns dparam = sum_c(-steps[c] g_c)/count."""
+        m, n, cnt = self._images(g)
+        steps = np.ascontiguousarray(np.reshape(steps, (cnt,)), dtype=np.float64)
+        if out is None:
+            out = (g.new_empty((n, m)) if hasattr(g, "data_ptr")
+                   else np.empty((m, n), dtype=np.float64, order="F"))
+        _sync_torch(g, model, out)
+        N.check(N.load().gjw_stack_update(self.h, N.ptr(g), m, n, cnt, N.ptr(steps), N.ptr(out),
+                                          N.ptr(model)), "gjw_stack_update")
+        return out
+
+
 _default_ctx = {}
 
 
@@ -284,14 +352,24 @@ class Survey:
         N.check(N.load().gjw_set_adj_filter(self.h, float(f_low), float(f_high), int(steep)),
                 "gjw_set_adj_filter")
 
+    def set_shot_gradients(self, on=True):
+        """GJW_OPT_SHOT_GRADS: gradients come back per shot, (m, n, ns)
+        F-order, as w_update_e.m keeps g_e per source."""
+        self.set_option(N.GJW_OPT_SHOT_GRADS, int(bool(on)))
+        self._shot_grads = bool(on)
+
+    def _grad_shape(self):
+        m, n = self.grid.m, self.grid.n
+        return (m, n, self.ns) if getattr(self, "_shot_grads", False) else (m, n)
+
     def gradient(self, adj=None, mode=N.GJW_GRAD_EXACT, want_gw=False, g_eps=None, g_sig=None):
         """FP64 gradients (m, n) w.r.t. eps_r and sigma (and optionally the
-        wavelet gradient (ns, nt)) for adjoint source ``adj`` (traces layout)."""
-        m, n = self.grid.m, self.grid.n
+        wavelet gradient (ns, nt)) for adjoint source ``adj`` (traces layout);
+        (m, n, ns) per shot after set_shot_gradients()."""
         if g_eps is None:
-            g_eps = np.zeros((m, n), dtype=np.float64, order="F")
+            g_eps = np.zeros(self._grad_shape(), dtype=np.float64, order="F")
         if g_sig is None:
-            g_sig = np.zeros((m, n), dtype=np.float64, order="F")
+            g_sig = np.zeros(self._grad_shape(), dtype=np.float64, order="F")
         gw = np.zeros((self.ns, self.grid.nt), dtype=np.float64) if want_gw else None
         if adj is not None and not hasattr(adj, "data_ptr"):
             adj = np.ascontiguousarray(adj, dtype=np.float32)
@@ -303,11 +381,10 @@ class Survey:
     def fwi_gradient(self, d_obs, g_eps=None, g_sig=None):
         """Forward + misfit (L2 unless set_misfit chose another) + exact
         gradient, all on the device."""
-        m, n = self.grid.m, self.grid.n
         if g_eps is None:
-            g_eps = np.zeros((m, n), dtype=np.float64, order="F")
+            g_eps = np.zeros(self._grad_shape(), dtype=np.float64, order="F")
         if g_sig is None:
-            g_sig = np.zeros((m, n), dtype=np.float64, order="F")
+            g_sig = np.zeros(self._grad_shape(), dtype=np.float64, order="F")
         if not hasattr(d_obs, "data_ptr"):
             d_obs = np.ascontiguousarray(d_obs, dtype=np.float32)
         _sync_torch(d_obs, g_eps, g_sig)
@@ -383,10 +460,10 @@ class Batch:
         list, J array)."""
         S = len(self.surveys)
         if g_eps is None:
-            g_eps = [np.zeros((sv.grid.m, sv.grid.n), dtype=np.float64, order="F")
+            g_eps = [np.zeros(sv._grad_shape(), dtype=np.float64, order="F")
                      for sv in self.surveys]
         if g_sig is None:
-            g_sig = [np.zeros((sv.grid.m, sv.grid.n), dtype=np.float64, order="F")
+            g_sig = [np.zeros(sv._grad_shape(), dtype=np.float64, order="F")
                      for sv in self.surveys]
         d_obs = [d if hasattr(d, "data_ptr") else np.ascontiguousarray(d, dtype=np.float32)
                  for d in d_obs]

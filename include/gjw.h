@@ -102,6 +102,10 @@ int gjw_set_shots(gjw_survey *s, int32_t ns, const int32_t *src, const float *wa
 #define GJW_OPT_GRAPH 7      /* 1: gjw_fwi_gradient (batched path) replays its whole launch
                                 sequence as one captured hipGraph (captured on the second call
                                 and again whenever a buffer or option it uses changes) */
+#define GJW_OPT_SHOT_GRADS 10 /* 1: gjw_gradient / gjw_fwi_gradient / gjw_batch_fwi_gradient
+                                return one gradient per shot, ns*m*n values each (shot-major),
+                                as w_update_e.m:98 keeps g_e per source before its
+                                post-processing; 0 (default): the sum over shots */
 #define GJW_PATH_BATCHED 0   /* all shots per launch, checkpoint + recompute        */
 #define GJW_PATH_PLAIN 1     /* one shot at a time, full wavefield stored (as w_solve.m) */
 #define GJW_SNAP_F32 0
@@ -212,6 +216,46 @@ int gjw_batch_last_timing(gjw_batch *b, double *ms, double *cells);
 /* Device-time of the last gjw_forward / gjw_gradient / gjw_fwi_gradient
  * call and the number of cell-updates it executed. */
 int gjw_last_timing(gjw_survey *s, double *ms, double *cell_updates);
+
+/* Gradient post-processing, on `count` FP64 images of m x n (column-major,
+ * image c at offset c*m*n; host or device memory).  The reference runs it per
+ * source inside its parfor (src/w/w_update_e.m:108-145, w_update_es.m:105-180);
+ * here every source's image goes through one launch sequence. */
+
+/* Source taper (src/shared/w-dc/gaussi.m:10-16, applied at
+ * src/w/w_update_e.m:112-118): image c is multiplied in place by
+ * normali(1 - exp(-(((x - X[s_x-1])/L)^2 + ((z - Y[s_z-1])/L)^2))), L = 2*lo[c].
+ * X: n column coordinates (geome_.X), Y: m depth coordinates (geome_.Y), s:
+ * (iz, ix) per image, 1-based model indices, lo: gaussi's lo argument (w_update_e
+ * passes twice the wavelength at the source).  X, Y, s and lo are host arrays. */
+int gjw_gaussi(gjw_ctx *ctx, double *g, int32_t m, int32_t n, int32_t count, const double *X,
+               const double *Y, const int32_t *s, const double *lo);
+
+/* k-space Gaussian filter (src/shared/w-dc/image_gaussian.m:21-61): zero
+ * padding to 2^nextpow2 in both directions, fft2, the cake exp(-r^2)
+ * (GJW_LOW_PASS) or 1 - exp(-r^2) (GJW_HI_PASS) normalised to its maximum,
+ * r^2 = ((kx - n_/2)/a)^2 + ((kz - m_/2)/b)^2 with a = n_ ax/2 sqrt(2),
+ * b = m_ az/2 sqrt(2), fftshifted; real part of ifft2, trimmed.  in may equal
+ * out; m, n >= 2. */
+#define GJW_LOW_PASS 0
+#define GJW_HI_PASS 1
+int gjw_image_gaussian(gjw_ctx *ctx, const double *in, double *out, int32_t m, int32_t n,
+                       int32_t count, double ax, double az, int32_t type);
+
+/* normali (src/shared/w-dc/normali.m:1-7) of each image in place: a / max|a|
+ * (NaN ignored by the maximum, as MATLAB's max; an all-zero image is left
+ * alone).  nan_guard != 0 then zeroes every image holding a NaN
+ * (src/w/w_update_e.m:133-137); a shape mask (m x n, or NULL) then multiplies
+ * every image (src/w/w_update_e.m:141-143). */
+int gjw_normali(gjw_ctx *ctx, double *a, int32_t m, int32_t n, int32_t count, int32_t nan_guard,
+                const double *shape);
+
+/* Shot reduction and update (src/w/w_update_e.m:177, src/w/w_update_e_.m:38-41,
+ * src/w/w_image_e.m:50-51): dparam = sum_c(-steps[c] * g_c) / count, all zero
+ * when sum(steps) < 0; with a model (may be NULL), model .*= exp(model .*
+ * dparam).  steps: count host doubles. */
+int gjw_stack_update(gjw_ctx *ctx, const double *g, int32_t m, int32_t n, int32_t count,
+                     const double *steps, double *dparam, double *model);
 
 /* Average device duration (ms) and count of the sampled step kernels, and the
  * number of launches of that kernel, while GJW_OPT_KTIMING is on.  which: 0 = one-step forward, 1 = one-step adjoint,

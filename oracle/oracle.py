@@ -14,7 +14,9 @@ Two layers:
   the source wavelet (w_wavelet.m), index maps (w_load.m, w_M.m), the
   misfits and adjoint sources (w_bozdag.m, instant_.m, w_E.m), and the
   adjoint-source conditioning (filt_gauss.m with MATLAB's tukeywin,
-  w_lmute.m).
+  w_lmute.m), and the gradient post-processing (gaussi.m, image_gaussian.m,
+  normali.m, the shot reduction of w_update_e_.m and the multiplicative
+  update of w_image_e.m).
 
 Arrays follow the MATLAB layout: column-major (Fortran order), depth first.
 """
@@ -485,3 +487,95 @@ def lmute(d, d_o, t, rx, v_mute, t_mute):
     dm = np.asarray(d, dtype=np.float64) * mute
     dom = np.asarray(d_o, dtype=np.float64) * mute
     return dm, dom, dm - dom
+
+
+# ---------------------------------------------------------------------------
+# gradient post-processing (w_update_e.m:108-145, w_update_e_.m:28-42,
+# w_image_e.m:50-51)
+# ---------------------------------------------------------------------------
+def normali(a):
+    """normali.m:1-7: a / max|a|, with a zero maximum taken as 1.  MATLAB's
+    max skips NaN, so the maximum is numpy's nanmax (an all-NaN image stays
+    NaN either way)."""
+    a = np.asarray(a, dtype=np.float64)
+    with np.errstate(invalid="ignore"):
+        mx = np.nanmax(np.abs(a)) if not np.all(np.isnan(a)) else np.nan
+    if mx == 0:
+        mx = 1.0
+    return a / mx
+
+
+def gaussi(x, z, s, lo, pow_=2):
+    """gaussi.m:10-16: 2-D taper 1 - exp(-(((X - x_s)/L)^2 + ((Z - z_s)/L)^2)),
+    L = 2 lo, on the grid X = x (columns), Z = z (rows), centred at the
+    1-based (iz, ix) = s, then normali.  pow_ is accepted and unused, as in
+    the reference."""
+    x = np.asarray(x, dtype=np.float64).ravel()
+    z = np.asarray(z, dtype=np.float64).ravel()
+    nx, nz = x.size, z.size
+    X = np.tile(x[None, :], (nz, 1))
+    Z = np.tile(z[:, None], (1, nx))
+    L = 2 * lo
+    g = ((X - x[s[1] - 1]) / L) ** 2 + ((Z - z[s[0] - 1]) / L) ** 2
+    g = 1 - np.exp(-g)
+    return normali(g)
+
+
+def image_gaussian(img, a, b, filt_type="LOW_PASS"):
+    """image_gaussian.m:21-61, read line by line: pad to 2^nextpow2 in both
+    directions, fft2, build the centred cake over X = 0..n_-1, Y = 0..m_-1
+    (meshgrid), exp(-r^2) or 1 - exp(-r^2), divide by its maximum, fftshift,
+    multiply, real(ifft2), trim."""
+    img = np.asarray(img, dtype=np.float64)
+    m, n = img.shape
+    m_, n_ = nextpow2(m), nextpow2(n)
+    pad = np.zeros((m_, n_))
+    pad[:m, :n] = img
+    K = np.fft.fft2(pad)
+    a = n_ * (a / 2) * np.sqrt(2)
+    b = m_ * (b / 2) * np.sqrt(2)
+    X = np.arange(n_)
+    x = n_ / 2
+    Y = np.arange(m_)
+    y = m_ / 2
+    XY, YX = np.meshgrid(X, Y)
+    kk = ((XY - x) / a) ** 2 + ((YX - y) / b) ** 2
+    kk = -kk
+    if filt_type == "LOW_PASS":
+        kk = np.exp(kk)
+    elif filt_type == "HI_PASS":
+        kk = 1 - np.exp(kk)
+    kk = kk / np.max(np.abs(kk))
+    out = np.real(np.fft.ifft2(np.fft.fftshift(kk) * K))
+    return out[:m, :n]
+
+
+def w_update_post(g, x, z, s, lo, ax, az, shape=None):
+    """One source's gradient through w_update_e.m:118-143: gaussi taper
+    (called with 2 lo), image_gaussian LOW_PASS, normali, NaN guard, shape
+    mask."""
+    g = gaussi(x, z, s, 2 * lo, 2) * g
+    g = image_gaussian(g, ax, az, "LOW_PASS")
+    g = normali(g)
+    if np.isnan(g).any():
+        g = np.zeros_like(g)
+    if shape is not None:
+        g = g * shape
+    return g
+
+
+def stack_update(gs, steps, model=None):
+    """w_update_e.m:177 per source, w_update_e_.m:38-41 over sources
+    (sum / ns; zero when the steps sum below zero) and w_image_e.m:50-51
+    (model .* exp(model .* d)).  gs: (m, n, ns)."""
+    gs = np.asarray(gs, dtype=np.float64)
+    ns = gs.shape[2]
+    d = np.zeros(gs.shape[:2])
+    for k in range(ns):
+        d = d + (-steps[k] * gs[:, :, k])
+    d = d / ns
+    if np.sum(steps) < 0:
+        d = np.zeros_like(d)
+    if model is None:
+        return d
+    return d, model * np.exp(model * d)
