@@ -190,6 +190,7 @@ def main():
     cond = _conditioning(ctx, d_obs, g.dt, args.no_cpu)
     lsearch = None if args.no_linesearch else _line_search(ctx, case, d_obs)
     post = _postprocess(ctx, case, args.no_cpu)
+    mat = _matio(case, d_obs)
 
     out = {
         "metric": "FWI shot-gradients/s, batched vs plain, at <=1e-3 grad err; Gcell-updates/s",
@@ -217,6 +218,7 @@ def main():
         "adjoint_source_filter": cond,
         "line_search": lsearch,
         "gradient_postprocess": post,
+        "mat_io": mat,
     }
     print(json.dumps(out))
 
@@ -261,6 +263,41 @@ def _line_search(ctx, case, d_obs, nparabo=3):
             "shot_objectives_per_s": nparabo * ns / tb,
             "objectives": [float(x) for x in J],
             "note": "forward only (no wavefield), L2 normalised as w_objs_e.m"}
+
+
+def _matio(case, d_obs, lines=8):
+    """Radargram .mat I/O (w_natur_u.m writer, w_load.m reader) on a sample
+    of the workload's lines: write `lines` line<is>.mat files plus the
+    position cells to a temporary folder, read them back as survey inputs,
+    and copy the traces to the device."""
+    import shutil
+    import tempfile
+    import torch
+    from gerjoii_amd import matio as MIO
+    g = case.grid
+    recs = case.receivers[:lines]
+    ntr = sum(len(r) for r in recs)
+    tr = d_obs[:ntr].cpu().numpy()
+    tmp = tempfile.mkdtemp(prefix="gjw_mat_")
+    try:
+        t0 = time.perf_counter()
+        MIO.write_survey(tmp, g, case.src[:lines], recs, tr, case.wavelets[:lines], case.f0)
+        tw = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        src, wav, rr, d = MIO.load_survey(g, tmp, range(1, lines + 1))
+        dd = torch.from_numpy(d).cuda()
+        torch.cuda.synchronize()
+        tl = time.perf_counter() - t0
+        ok = bool(np.array_equal(d, tr))
+        mb = 4.0 * ntr * g.nt / 1e6
+        nbytes = sum(os.path.getsize(os.path.join(tmp, f)) for f in os.listdir(tmp))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    del dd
+    return {"lines": lines, "traces": ntr, "nt": g.nt, "trace_MB": mb, "file_MB": nbytes / 1e6,
+            "write_MBs": mb / tw, "load_to_device_MBs": mb / tl, "round_trip_exact": ok,
+            "note": "MATLAB v5 files with zlib compression (MATLAB's default -v7), scipy.io, "
+                    "one host thread; load includes the copy of the traces to the device"}
 
 
 def _postprocess(ctx, case, no_cpu):
