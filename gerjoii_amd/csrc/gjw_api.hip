@@ -64,6 +64,7 @@ extern template __global__ void k_fwd_stepN_multi<2, false>(const DevSurvey*, Mu
 extern template __global__ void k_fwd_stepN_multi<2, true>(const DevSurvey*, MultiArgs);
 extern template __global__ void k_fwd_stepN_multi<4, false>(const DevSurvey*, MultiArgs);
 extern template __global__ void k_fwd_stepN_multi<4, true>(const DevSurvey*, MultiArgs);
+__global__ void k_fwd_step8_multi(const DevSurvey*, MultiArgs);
 __global__ void k_adj_step_multi(const DevSurvey*, MultiArgs);
 template <int NS>
 __global__ void k_adj_stepN_multi(const DevSurvey*, MultiArgs);
@@ -1036,9 +1037,10 @@ static int autotune_L(gjw_survey* s, int nsh) {
     const int c1[] = {8, 16, 32, 64}, c2[] = {12, 16, 24, 32}, c4[] = {16, 24, 32, 48},
               c8[] = {48, 64, 96, 128};
     const int* cand = NS == 1 ? c1 : NS == 2 ? c2 : NS == 4 ? c4 : c8;
+    const int ncand = NS == 8 ? (int)(sizeof(c8) / sizeof(int)) : 4;
     float best = 1e30f;
     int bestL = 16;
-    for (int ci = 0; ci < 4; ++ci) {
+    for (int ci = 0; ci < ncand; ++ci) {
       const int L = cand[ci];
       int cur = 0, xc = 0;
       for (int r = -1; r < reps; ++r) {
@@ -2173,7 +2175,8 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
     // (the adjoint widths are tuned by single-survey gradient sweeps)
     const int La4 = s->tuned_La4 > 0 ? s->tuned_La4 : s->tuned_L4;
     const int La2 = s->tuned_La2 > 0 ? s->tuned_La2 : s->tuned_L;
-    const int L = steps == 4   ? (adj ? La4 : s->tuned_L4)
+    const int L = steps == 8   ? s->tuned_L8
+                  : steps == 4 ? (adj ? La4 : s->tuned_L4)
                   : steps == 2 ? (adj ? La2 : s->tuned_L)
                                : s->tuned_L1;
     a.blk0[k] = blk;
@@ -2191,8 +2194,11 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
   if (blk == 0) return GJW_OK;
   hipStream_t st = b->ctx->stream;
   const dim3 grid((unsigned)blk), block(64, 4);
+  if ((adj || frames) && steps == 8) return fail(GJW_ERR_STATE, "batch_launch: 8 steps forward only");
   if (!adj && steps == 1)
     hipLaunchKernelGGL(k_fwd_step_multi, grid, block, 0, st, b->dsv.p, a);
+  else if (!adj && steps == 8)
+    hipLaunchKernelGGL(k_fwd_step8_multi, grid, block, 0, st, b->dsv.p, a);
   else if (!adj && steps == 4 && frames)
     hipLaunchKernelGGL((k_fwd_stepN_multi<4, true>), grid, block, 0, st, b->dsv.p, a);
   else if (!adj && steps == 4)
@@ -2235,7 +2241,7 @@ static int batch_fwd_enqueue(gjw_batch* b) {
         room = std::min(room, nt[k] - tau);
         itv[k] = tau;
       }
-    const int steps = room >= 4 ? 4 : room >= 2 ? 2 : 1;
+    const int steps = room >= 8 ? 8 : room >= 4 ? 4 : room >= 2 ? 2 : 1;
     TRY(batch_launch(b, act, steps, false, false, itv, cur, xc, nullptr, 1));
     for (int k : act) {
       cur[k] ^= 1;
@@ -2442,7 +2448,7 @@ extern "C" int gjw_batch_objective(gjw_batch* b, const float* const* d_obs, doub
   const int S = (int)b->sv.size();
   HIPCHK(hipSetDevice(b->ctx->device));
   hipStream_t st = b->ctx->stream;
-  int T = 0, fuse = 4;
+  int T = 0, fuse = 8;  // forward only: the 8-step kernel when every survey's autotuner chose it
   for (int k = 0; k < S; ++k) {
     gjw_survey* s = b->sv[k];
     TRY(check_ready(s, "gjw_batch_objective"));

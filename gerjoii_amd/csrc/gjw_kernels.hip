@@ -997,8 +997,7 @@ k_fwd_step8(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
 }
 
 template <int NS, bool FRAMES>
-__global__ void __launch_bounds__(256, GJW_FWD_WAVES)
-k_fwd_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
+__device__ __forceinline__ void fwd_multi_body(const DevSurvey* __restrict__ sv, const MultiArgs& a) {
   int k;
   TileId tid;
   if (!multi_tile(a, sv, 64 - 2 * NS, k, tid, true, true)) return;
@@ -1012,6 +1011,19 @@ k_fwd_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
                         f + 3 * (c ^ 1) * z, f + (3 * (c ^ 1) + 1) * z, f + (3 * (c ^ 1) + 2) * z,
                         f + (6 + 2 * (x ^ 1)) * z, f + (7 + 2 * (x ^ 1)) * z, a.it[k],
                         a.rec ? S.tr : nullptr, fr1, S.fstride);
+}
+
+template <int NS, bool FRAMES>
+__global__ void __launch_bounds__(256, GJW_FWD_WAVES)
+k_fwd_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
+  fwd_multi_body<NS, FRAMES>(sv, a);
+}
+
+// eight steps over several surveys (forward without frames: the line-search
+// objectives), with the 8-step forward's register budget
+__global__ void __launch_bounds__(256, GJW_FWD8_WAVES)
+k_fwd_step8_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
+  fwd_multi_body<8, false>(sv, a);
 }
 
 #define GJW_FWDN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \

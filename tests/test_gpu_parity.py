@@ -275,6 +275,31 @@ def test_batch_objective_and_line_search(oracle_mod, gpu_ctx):
     assert abs(step - step_ref) <= 1e-2 * max(abs(step_ref), 1e-12)
 
 
+@pytest.mark.parametrize("fuse", [8, 4])
+def test_batch_objective_fused_depths(oracle_mod, gpu_ctx, fuse):
+    """gjw_batch_objective with the 8-step multi-survey forward (and the
+    4-step one) on two surveys of uneven trace length, so that the shared
+    launches also take the 4-, 2- and 1-step tails: each objective against the
+    FP64 oracle."""
+    cases = [configs.small(nt=240, ns=2), configs.small(nt=203, ns=2, seed=9)]
+    band = (1e8, 4e8)
+    svs, dobs, ref = [], [], []
+    for case in cases:
+        svt = make_survey(case, gpu_ctx)
+        svt.set_model(case.eps_true, case.sig_true)
+        d = svt.forward()
+        sv = make_survey(case, gpu_ctx)
+        sv.set_option(6, fuse)
+        sv.set_adj_filter(*band)
+        svs.append(sv)
+        dobs.append(d)
+        ref.append(_oracle_objective(oracle_mod, case, case.eps0, d, band))
+    J = S.Batch(svs).objective(dobs)
+    print("fused", fuse, J, ref, flush=True)
+    for a, b in zip(J, ref):
+        assert abs(a - b) <= 1e-4 * abs(b)
+
+
 def test_differentiate_cube_gpu(oracle_mod, gpu_ctx):
     rng = np.random.default_rng(2)
     for nt in (7, 8):
