@@ -504,8 +504,9 @@ def test_source_recorder(oracle_mod, gpu_ctx):
             assert rel_l2(jz, ref) <= 1e-4
 
 
-@pytest.mark.parametrize("snap,misfit", [(0, "L2"), (1, "L2"), (0, "env_L2")])
-def test_fwi_gradient_graph_replays(gpu_ctx, snap, misfit):
+@pytest.mark.parametrize("snap,misfit,per_shot", [(0, "L2", False), (1, "L2", False),
+                                                  (0, "env_L2", False), (0, "L2", True)])
+def test_fwi_gradient_graph_replays(gpu_ctx, snap, misfit, per_shot):
     """GJW_OPT_GRAPH: every replay of the captured batched FWI gradient equals
     the eager result bit for bit, with eager work and synchronisations between
     the replays, and replays follow a change of the model (values, not
@@ -523,6 +524,7 @@ def test_fwi_gradient_graph_replays(gpu_ctx, snap, misfit):
     for sv in (eager, graph):
         sv.set_option(N.GJW_OPT_SNAPSHOT, snap)
         sv.set_misfit(misfit)
+        sv.set_shot_gradients(per_shot)
     graph.set_option(N.GJW_OPT_GRAPH, 1)
     models = [(case.eps0, case.sig0), (case.eps0 * 1.05, case.sig0 * 0.9)]
     for eps, sig in models:

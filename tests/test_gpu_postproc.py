@@ -144,6 +144,31 @@ def test_shot_gradients(oracle_mod, gpu_ctx, plain):
         assert rel_l2(ps[:, :, q], fs) <= 1e-3
 
 
+def test_shot_gradients_in_a_batch(gpu_ctx):
+    """gjw_batch_fwi_gradient with GJW_OPT_SHOT_GRADS on one of two surveys:
+    its per-shot gradients sum to its shot-summed gradient in the same batch,
+    and the other survey is unaffected."""
+    from gerjoii_amd import survey as S
+    from test_gpu_parity import make_survey
+    cases = [configs.small(nt=200, ns=3), configs.small(nt=170, ns=2, seed=11)]
+    dobs = []
+    for case in cases:
+        svt = make_survey(case, gpu_ctx)
+        svt.set_model(case.eps_true, case.sig_true)
+        dobs.append(svt.forward())
+    ref = S.Batch([make_survey(c, gpu_ctx, L=16) for c in cases]).fwi_gradient(dobs)
+    svs = [make_survey(c, gpu_ctx, L=16) for c in cases]
+    svs[0].set_shot_gradients(True)
+    got = S.Batch(svs).fwi_gradient(dobs)
+    ge_ref, gs_ref = ref[0][0], ref[1][0]
+    ge, gs = got[0][0], got[1][0]
+    assert ge.shape == (cases[0].grid.m, cases[0].grid.n, 3)
+    print("batch shots", rel_l2(ge.sum(axis=2), ge_ref), rel_l2(gs.sum(axis=2), gs_ref), flush=True)
+    assert rel_l2(ge.sum(axis=2), ge_ref) <= 1e-12 and rel_l2(gs.sum(axis=2), gs_ref) <= 1e-12
+    np.testing.assert_array_equal(got[0][1], ref[0][1])
+    np.testing.assert_array_equal(got[1][1], ref[1][1])
+
+
 def test_update_pipeline(oracle_mod, gpu_ctx):
     """w_update_e.m:108-145 + w_update_e_.m:38-41 + w_image_e.m:50-51 on the
     device (postproc.w_update_post / w_update_stack) from device per-shot FWI
