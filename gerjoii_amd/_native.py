@@ -90,7 +90,7 @@ def load():
     lib.gjw_gaussi.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp]
     lib.gjw_image_gaussian.argtypes = [vp, vp, vp, i32, i32, i32, C.c_double, C.c_double, i32]
     lib.gjw_normali.argtypes = [vp, vp, i32, i32, i32, i32, vp]
-    lib.gjw_stack_update.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp]
+    lib.gjw_stack_update.argtypes = [vp, vp, i32, i32, i32, vp, i32, vp, vp]
     lib.gjw_kernel_stats.argtypes = [vp, i32, vp, vp, vp, i32]
     lib.gjw_tile.argtypes = [vp, C.c_int32, vp]
     for name in EXPORTS:

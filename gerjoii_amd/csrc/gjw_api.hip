@@ -2783,15 +2783,17 @@ extern "C" int gjw_normali(gjw_ctx* c, double* a, int32_t m, int32_t n, int32_t 
 }
 
 extern "C" int gjw_stack_update(gjw_ctx* c, const double* g, int32_t m, int32_t n, int32_t count,
-                                const double* steps, double* dparam, double* model) {
+                                const double* steps, int32_t neg_rule, double* dparam,
+                                double* model) {
   if (!c || !g || !steps || !dparam) return fail(GJW_ERR_ARG, "gjw_stack_update: NULL argument");
   if (m < 1 || n < 1 || count < 1) return fail(GJW_ERR_ARG, "gjw_stack_update: bad sizes");
   HIPCHK(hipSetDevice(c->device));
   PostWork* w = post_work(c);
   // w_update_e_.m:40-41: the update is dropped when the steps sum below zero
+  // (the conductivity and joint reductions have no such rule)
   double ssum = 0.0;
   for (int k = 0; k < count; ++k) ssum += steps[k];
-  const int zero = ssum < 0 ? 1 : 0;
+  const int zero = (neg_rule && ssum < 0) ? 1 : 0;
   TRY(w->dsmall.ensure((size_t)count));
   HIPCHK(hipMemcpy(w->dsmall.p, steps, (size_t)count * 8, hipMemcpyHostToDevice));
   const long long per = (long long)m * n;

@@ -204,17 +204,20 @@ class Context:
                                      N.ptr(shape)), "gjw_normali")
         return a
 
-    def stack_update(self, g, steps, model=None, out=None):
+    def stack_update(self, g, steps, model=None, out=None, neg_rule=True):
         """w_update_e_.m:38-41 (and w_image_e.m:50-51 when a model is given,
-        updated in place): returns dparam = sum_c(-steps[c] g_c)/count."""
+        updated in place): returns dparam = sum_c(-steps[c] g_c)/count, zero
+        when the steps sum below zero unless neg_rule is False (the
+        conductivity reduction, w_update_s_.m:38)."""
         m, n, cnt = self._images(g)
         steps = np.ascontiguousarray(np.reshape(steps, (cnt,)), dtype=np.float64)
         if out is None:
             out = (g.new_empty((n, m)) if hasattr(g, "data_ptr")
                    else np.empty((m, n), dtype=np.float64, order="F"))
         _sync_torch(g, model, out)
-        N.check(N.load().gjw_stack_update(self.h, N.ptr(g), m, n, cnt, N.ptr(steps), N.ptr(out),
-                                          N.ptr(model)), "gjw_stack_update")
+        N.check(N.load().gjw_stack_update(self.h, N.ptr(g), m, n, cnt, N.ptr(steps),
+                                          int(bool(neg_rule)), N.ptr(out), N.ptr(model)),
+                "gjw_stack_update")
         return out
 
 

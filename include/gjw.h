@@ -251,11 +251,12 @@ int gjw_normali(gjw_ctx *ctx, double *a, int32_t m, int32_t n, int32_t count, in
                 const double *shape);
 
 /* Shot reduction and update (src/w/w_update_e.m:177, src/w/w_update_e_.m:38-41,
- * src/w/w_image_e.m:50-51): dparam = sum_c(-steps[c] * g_c) / count, all zero
- * when sum(steps) < 0; with a model (may be NULL), model .*= exp(model .*
- * dparam).  steps: count host doubles. */
+ * src/w/w_update_s_.m:38, src/w/w_image_e.m:50-51, src/w/w_image_s.m:38-39):
+ * dparam = sum_c(-steps[c] * g_c) / count; with neg_rule != 0 all zero when
+ * sum(steps) < 0 (the permittivity reduction's rule); with a model (may be
+ * NULL), model .*= exp(model .* dparam).  steps: count host doubles. */
 int gjw_stack_update(gjw_ctx *ctx, const double *g, int32_t m, int32_t n, int32_t count,
-                     const double *steps, double *dparam, double *model);
+                     const double *steps, int32_t neg_rule, double *dparam, double *model);
 
 /* Average device duration (ms) and count of the sampled step kernels, and the
  * number of launches of that kernel, while GJW_OPT_KTIMING is on.  which: 0 = one-step forward, 1 = one-step adjoint,

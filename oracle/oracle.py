@@ -564,17 +564,30 @@ def w_update_post(g, x, z, s, lo, ax, az, shape=None):
     return g
 
 
-def stack_update(gs, steps, model=None):
+def w_update_es_post(g_e, g_s, x, z, s, lo, ax, az):
+    """One source's gradients through w_update_es.m:105-173: the gaussi taper
+    (2 lo) on both, image_gaussian LOW_PASS, division by max|g| (MATLAB's
+    max skips NaN); no mask and no NaN guard on this path."""
+    t = gaussi(x, z, s, 2 * lo, 2)
+    out = []
+    for g in (g_e, g_s):
+        h = image_gaussian(t * g, ax, az, "LOW_PASS")
+        out.append(h / np.nanmax(np.abs(h)))
+    return out[0], out[1]
+
+
+def stack_update(gs, steps, model=None, neg_rule=True):
     """w_update_e.m:177 per source, w_update_e_.m:38-41 over sources
-    (sum / ns; zero when the steps sum below zero) and w_image_e.m:50-51
-    (model .* exp(model .* d)).  gs: (m, n, ns)."""
+    (sum / ns; zero when the steps sum below zero -- neg_rule; the
+    conductivity reduction w_update_s_.m:38 has no such rule) and
+    w_image_e.m:50-51 (model .* exp(model .* d)).  gs: (m, n, ns)."""
     gs = np.asarray(gs, dtype=np.float64)
     ns = gs.shape[2]
     d = np.zeros(gs.shape[:2])
     for k in range(ns):
         d = d + (-steps[k] * gs[:, :, k])
     d = d / ns
-    if np.sum(steps) < 0:
+    if neg_rule and np.sum(steps) < 0:
         d = np.zeros_like(d)
     if model is None:
         return d

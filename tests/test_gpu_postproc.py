@@ -113,6 +113,9 @@ def test_stack_update_gpu(oracle_mod, gpu_ctx):
     neg = steps.copy()
     neg[0] = -1.0
     np.testing.assert_array_equal(gpu_ctx.stack_update(g, neg), np.zeros((m, n)))
+    # the conductivity reduction (w_update_s_.m:38) keeps the mean
+    d2 = gpu_ctx.stack_update(g, neg, neg_rule=False)
+    assert rel_max(d2, O.stack_update(g, neg, neg_rule=False)) <= 1e-14
 
 
 @pytest.mark.parametrize("plain", [False, True])
@@ -167,6 +170,33 @@ def test_shot_gradients_in_a_batch(gpu_ctx):
     assert rel_l2(ge.sum(axis=2), ge_ref) <= 1e-12 and rel_l2(gs.sum(axis=2), gs_ref) <= 1e-12
     np.testing.assert_array_equal(got[0][1], ref[0][1])
     np.testing.assert_array_equal(got[1][1], ref[1][1])
+
+
+def test_joint_update_post(oracle_mod, gpu_ctx):
+    """w_update_es.m:105-173 on the device (postproc.w_update_es_post) for
+    both gradients of every source, against the oracle, from device per-shot
+    gradients."""
+    from test_gpu_parity import make_survey
+    O = oracle_mod
+    case = configs.small(nt=200)
+    g = case.grid
+    sv = make_survey(case, gpu_ctx)
+    sv.set_shot_gradients(True)
+    svt = make_survey(case, gpu_ctx)
+    svt.set_model(case.eps_true, case.sig_true)
+    pe, ps, _ = sv.fwi_gradient(svt.forward())
+    X = np.arange(g.n) * g.dx
+    Y = np.arange(g.m) * g.dx
+    s = PP.taper_index(g, case.src)
+    lo = PP.source_wavelength(case.eps0, s, case.f0)
+    refs = [O.w_update_es_post(pe[:, :, k], ps[:, :, k], X, Y, s[k], lo[k], 0.2, 0.3)
+            for k in range(sv.ns)]
+    ge, gs = PP.w_update_es_post(gpu_ctx, pe.copy(order="F"), ps.copy(order="F"), g, case.src,
+                                 case.eps0, X, Y, case.f0, 0.2, 0.3)
+    re = np.stack([r[0] for r in refs], axis=2)
+    rs = np.stack([r[1] for r in refs], axis=2)
+    print("joint post", rel_max(ge, re), rel_max(gs, rs), flush=True)
+    assert rel_max(ge, re) <= 1e-12 and rel_max(gs, rs) <= 1e-12
 
 
 def test_update_pipeline(oracle_mod, gpu_ctx):

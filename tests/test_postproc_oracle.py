@@ -112,9 +112,27 @@ def test_w_update_post_and_stack(O):
     d = O.stack_update(gs, [2.0, 1.0])
     np.testing.assert_allclose(d, np.full((2, 3), -(2.0 * 1 + 1.0 * 3) / 2))
     np.testing.assert_array_equal(O.stack_update(gs, [2.0, -3.0]), np.zeros((2, 3)))
+    # the conductivity reduction (w_update_s_.m:38) has no negative-sum rule
+    np.testing.assert_allclose(O.stack_update(gs, [2.0, -3.0], neg_rule=False),
+                               np.full((2, 3), -(2.0 * 1 - 3.0 * 3) / 2))
     eps = np.full((2, 3), 4.0)
     d, e2 = O.stack_update(gs, [0.01, 0.02], model=eps)
     np.testing.assert_allclose(e2, 4.0 * np.exp(4.0 * (-(0.01 + 0.06) / 2)), rtol=1e-15)
+
+
+def test_w_update_es_post_scales_to_unit_max(O):
+    """w_update_es.m's chain: both outputs have max |g| = 1, and a positive
+    scalar factor on either input leaves its output unchanged (the division
+    by the maximum)."""
+    rng = np.random.default_rng(4)
+    x = np.linspace(0, 1, 16)
+    z = np.linspace(0, 0.5, 8)
+    ge, gs = rng.standard_normal((2, 8, 16))
+    a, b = O.w_update_es_post(ge, gs, x, z, (2, 9), 0.1, 0.3, 0.3)
+    assert np.isclose(np.abs(a).max(), 1.0) and np.isclose(np.abs(b).max(), 1.0)
+    a2, b2 = O.w_update_es_post(3.0 * ge, 0.5 * gs, x, z, (2, 9), 0.1, 0.3, 0.3)
+    np.testing.assert_allclose(a2, a, atol=1e-14)
+    np.testing.assert_allclose(b2, b, atol=1e-14)
 
 
 def test_taper_index_follows_w_update_e():
