@@ -64,12 +64,16 @@ extern template __global__ void k_fwd_stepN_multi<2, false>(const DevSurvey*, Mu
 extern template __global__ void k_fwd_stepN_multi<2, true>(const DevSurvey*, MultiArgs);
 extern template __global__ void k_fwd_stepN_multi<4, false>(const DevSurvey*, MultiArgs);
 extern template __global__ void k_fwd_stepN_multi<4, true>(const DevSurvey*, MultiArgs);
+template <bool FRAMES>
 __global__ void k_fwd_step8_multi(const DevSurvey*, MultiArgs);
+extern template __global__ void k_fwd_step8_multi<false>(const DevSurvey*, MultiArgs);
+extern template __global__ void k_fwd_step8_multi<true>(const DevSurvey*, MultiArgs);
 __global__ void k_adj_step_multi(const DevSurvey*, MultiArgs);
 template <int NS>
 __global__ void k_adj_stepN_multi(const DevSurvey*, MultiArgs);
 extern template __global__ void k_adj_stepN_multi<2>(const DevSurvey*, MultiArgs);
 extern template __global__ void k_adj_stepN_multi<4>(const DevSurvey*, MultiArgs);
+extern template __global__ void k_adj_stepN_multi<6>(const DevSurvey*, MultiArgs);
 __global__ void k_fill(float*, long long, float);
 __global__ void k_fill_d(double*, long long, double);
 __global__ void k_inject_last(DevGrid, int, int, int, const int*, const int*, const int*,
@@ -2108,7 +2112,7 @@ struct gjw_batch {
   gjw_ctx* ctx = nullptr;
   std::vector<gjw_survey*> sv;
   DBuf<DevSurvey> dsv;
-  int K = 0, T = 0, fuse = 2, calls = 0;
+  int K = 0, T = 0, fuse = 2, afuse = 2, calls = 0;  // afuse: adjoint steps per launch
   bool keep = true;  // forward sweep saves checkpoints (gradients); not for objectives
   GraphCache g_all, g_fwd, g_adj;
   double last_ms = 0, last_cells = 0;
@@ -2173,9 +2177,11 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
   for (int k = 0; k < a.n; ++k) {
     gjw_survey* s = b->sv[ids[k]];
     // (the adjoint widths are tuned by single-survey gradient sweeps)
+    const int La6 = s->tuned_La6 > 0 ? s->tuned_La6 : s->tuned_L8 > 0 ? s->tuned_L8 : 64;
     const int La4 = s->tuned_La4 > 0 ? s->tuned_La4 : s->tuned_L4;
     const int La2 = s->tuned_La2 > 0 ? s->tuned_La2 : s->tuned_L;
     const int L = steps == 8   ? s->tuned_L8
+                  : steps == 6 ? La6
                   : steps == 4 ? (adj ? La4 : s->tuned_L4)
                   : steps == 2 ? (adj ? La2 : s->tuned_L)
                                : s->tuned_L1;
@@ -2194,11 +2200,15 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
   if (blk == 0) return GJW_OK;
   hipStream_t st = b->ctx->stream;
   const dim3 grid((unsigned)blk), block(64, 4);
-  if ((adj || frames) && steps == 8) return fail(GJW_ERR_STATE, "batch_launch: 8 steps forward only");
+  if ((adj && steps == 8) || (!adj && steps == 6))
+    return fail(GJW_ERR_STATE, "batch_launch: no %d-step %s kernel", steps,
+                adj ? "adjoint" : "forward");
   if (!adj && steps == 1)
     hipLaunchKernelGGL(k_fwd_step_multi, grid, block, 0, st, b->dsv.p, a);
+  else if (!adj && steps == 8 && frames)
+    hipLaunchKernelGGL((k_fwd_step8_multi<true>), grid, block, 0, st, b->dsv.p, a);
   else if (!adj && steps == 8)
-    hipLaunchKernelGGL(k_fwd_step8_multi, grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_fwd_step8_multi<false>), grid, block, 0, st, b->dsv.p, a);
   else if (!adj && steps == 4 && frames)
     hipLaunchKernelGGL((k_fwd_stepN_multi<4, true>), grid, block, 0, st, b->dsv.p, a);
   else if (!adj && steps == 4)
@@ -2209,6 +2219,8 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
     hipLaunchKernelGGL((k_fwd_stepN_multi<2, false>), grid, block, 0, st, b->dsv.p, a);
   else if (steps == 1)
     hipLaunchKernelGGL(k_adj_step_multi, grid, block, 0, st, b->dsv.p, a);
+  else if (steps == 6)
+    hipLaunchKernelGGL((k_adj_stepN_multi<6>), grid, block, 0, st, b->dsv.p, a);
   else if (steps == 4)
     hipLaunchKernelGGL((k_adj_stepN_multi<4>), grid, block, 0, st, b->dsv.p, a);
   else
@@ -2310,7 +2322,7 @@ static int batch_adj_enqueue(gjw_batch* b) {
           itv[k] = tau;
           fsl[k] = tau + 1 - k0;
         }
-      const int steps = room >= 4 ? 4 : room >= 2 ? 2 : 1;
+      const int steps = room >= 8 ? 8 : room >= 4 ? 4 : room >= 2 ? 2 : 1;
       TRY(batch_launch(b, act, steps, false, true, itv, cur, xc, fsl, 0));
       for (int k : act) {
         cur[k] ^= 1;
@@ -2322,7 +2334,7 @@ static int batch_adj_enqueue(gjw_batch* b) {
       std::vector<int> act;
       // steps this launch may take: none below k0, and no survey may start
       // (take its first adjoint step) inside the launch
-      int room = std::min(b->fuse, tau - k0 + 1);
+      int room = std::min(b->afuse, tau - k0 + 1);
       for (int k : seg)
         if (tau <= k1[k] - 1) {
           act.push_back(k);
@@ -2331,7 +2343,7 @@ static int batch_adj_enqueue(gjw_batch* b) {
         } else {
           room = std::min(room, tau - (k1[k] - 1));
         }
-      const int steps = room >= 4 ? 4 : room >= 2 ? 2 : 1;
+      const int steps = room >= 6 ? 6 : room >= 4 ? 4 : room >= 2 ? 2 : 1;
       TRY(batch_launch(b, act, steps, true, false, itv, acur, nullptr, fsl, 0));
       for (int k : act) acur[k] ^= 1;
       tau -= steps;
@@ -2366,23 +2378,44 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
     all_l2 &= s->misfit_kind == GJW_MISFIT_L2 && !filter_on(s);
     K = std::max(K, s->ck_every);
   }
-  if (K <= 0) K = std::max(1, (int)std::ceil(std::sqrt((double)T)));
+  const bool user_K = K > 0;
+  if (!user_K) K = std::max(1, (int)std::ceil(std::sqrt((double)T)));
   b->T = T;
+  // forward and adjoint depths: the shallowest any survey's autotuner chose
+  // (the shared adjoint kernels go up to six steps)
+  int afuse = 6;
+  fuse = 8;
+  auto prepare_all = [&](int Kc) {
+    for (int k = 0; k < S; ++k) {
+      gjw_survey* s = b->sv[k];
+      s->ck_batch = Kc;
+      const int rc = fwi_batched_prepare(s);
+      s->ck_batch = 0;
+      TRY(rc);
+      fuse = std::min(fuse, s->fuse_eff);  // (after the autotuner)
+      afuse = std::min(afuse, std::min(adj_depth(s), 6));
+    }
+    return (int)GJW_OK;
+  };
+  TRY(prepare_all(K));
+  if (!user_K) {  // about sqrt(T), in whole launches of both depths (as ck_interval)
+    const int q = fuse * afuse / std::gcd(fuse, afuse);
+    const int K2 = std::min(T, (K + q - 1) / q * q);
+    if (K2 != K) {
+      K = K2;
+      TRY(prepare_all(K));  // the autotuner's results are kept; buffers for the new K
+    }
+  }
   b->K = K;
-  fuse = 4;
   for (int k = 0; k < S; ++k) {
     gjw_survey* s = b->sv[k];
-    s->ck_batch = K;
-    const int rc = fwi_batched_prepare(s);
-    s->ck_batch = 0;
-    TRY(rc);
-    fuse = std::min(fuse, s->fuse_eff);  // (after the autotuner)
     s->ck_K = K;
     s->ck_n = (int)cdiv(s->gr.nt, K);
     TRY(host_or_dev_copy(s->obs.p, d_obs[k], (long long)s->ntr * s->gr.nt * 4, st));
   }
   std::vector<DevSurvey> hs(S);
   b->fuse = fuse;
+  b->afuse = afuse;
   for (int k = 0; k < S; ++k) hs[k] = dev_survey(b->sv[k]);
   TRY(b->dsv.ensure(S));
   HIPCHK(hipStreamSynchronize(st));
@@ -2398,7 +2431,7 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
     TRY(batch_adj_enqueue(b));
   } else {
     std::vector<uintptr_t> key = {reinterpret_cast<uintptr_t>(b->dsv.p), (uintptr_t)K,
-                                  (uintptr_t)T, (uintptr_t)fuse};
+                                  (uintptr_t)T, (uintptr_t)fuse, (uintptr_t)afuse};
     for (gjw_survey* s : b->sv) {
       const std::vector<uintptr_t> ks = fwi_graph_key(s);
       key.insert(key.end(), ks.begin(), ks.end());

@@ -1019,12 +1019,14 @@ k_fwd_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
   fwd_multi_body<NS, FRAMES>(sv, a);
 }
 
-// eight steps over several surveys (forward without frames: the line-search
-// objectives), with the 8-step forward's register budget
+// eight steps over several surveys, with the 8-step forward's register budget
+template <bool FRAMES>
 __global__ void __launch_bounds__(256, GJW_FWD8_WAVES)
 k_fwd_step8_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
-  fwd_multi_body<8, false>(sv, a);
+  fwd_multi_body<8, FRAMES>(sv, a);
 }
+template __global__ void k_fwd_step8_multi<false>(const DevSurvey*, MultiArgs);
+template __global__ void k_fwd_step8_multi<true>(const DevSurvey*, MultiArgs);
 
 #define GJW_FWDN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \
     const float*, const float*, float*, float*, float*, float*, float*, int, float*, float*, long long
@@ -1897,6 +1899,7 @@ k_adj_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
 }
 template __global__ void k_adj_stepN_multi<2>(const DevSurvey*, MultiArgs);
 template __global__ void k_adj_stepN_multi<4>(const DevSurvey*, MultiArgs);
+template __global__ void k_adj_stepN_multi<6>(const DevSurvey*, MultiArgs);
 
 // ==========================================================================
 // small kernels

@@ -661,13 +661,16 @@ def _uneven_cases():
     return [a, b, c]
 
 
-@pytest.mark.parametrize("misfit", ["L2", "env_L2"])
-def test_batch_uneven_surveys(gpu_ctx, misfit):
+@pytest.mark.parametrize("misfit,fuse,K", [("L2", 0, 12), ("env_L2", 0, 12), ("L2", 8, 12),
+                                           ("L2", 8, 0), ("L2", 4, 0)])
+def test_batch_uneven_surveys(gpu_ctx, misfit, fuse, K):
     """Surveys of uneven grid size and trace length in shared launches give
-    each survey's own FWI gradient and misfit (same checkpoint interval)."""
+    each survey's own FWI gradient and misfit (same checkpoint interval):
+    with the autotuned depths, and with eight-step forwards and six-step
+    adjoints (fuse 8; K 12 splits segments into 8 + 4 and 6 + 6 steps, K 0
+    takes the default interval in whole launches of both depths)."""
     from gerjoii_amd import _native as N
     cases = _uneven_cases()
-    K = 12
     dobs, single, batch_sv = [], [], []
     for case in cases:
         svt = make_survey(case, gpu_ctx)
@@ -676,6 +679,8 @@ def test_batch_uneven_surveys(gpu_ctx, misfit):
         for lst in (single, batch_sv):
             sv = make_survey(case, gpu_ctx, ck=K)
             sv.set_misfit(misfit)
+            if fuse:
+                sv.set_option(N.GJW_OPT_FUSE, fuse)
             lst.append(sv)
     ref = [sv.fwi_gradient(d) for sv, d in zip(single, dobs)]
     bt = S.Batch(batch_sv)
