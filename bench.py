@@ -49,6 +49,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-linesearch", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--fuse", type=int, default=0, help="forward steps per launch: 0 autotuned, 1, 2, 4, 8")
     ap.add_argument("--adj-fuse", type=int, default=0, help="adjoint steps per launch: 0 (six), 4, 6, 8")
@@ -191,6 +192,7 @@ def main():
     lsearch = None if args.no_linesearch else _line_search(ctx, case, d_obs)
     post = _postprocess(ctx, case, args.no_cpu)
     mat = _matio(case, d_obs)
+    uneven = None if args.no_batch else _uneven_batch(ctx)
 
     out = {
         "metric": "FWI shot-gradients/s, batched vs plain, at <=1e-3 grad err; Gcell-updates/s",
@@ -219,6 +221,7 @@ def main():
         "line_search": lsearch,
         "gradient_postprocess": post,
         "mat_io": mat,
+        "uneven_batch": uneven,
     }
     print(json.dumps(out))
 
@@ -263,6 +266,59 @@ def _line_search(ctx, case, d_obs, nparabo=3):
             "shot_objectives_per_s": nparabo * ns / tb,
             "objectives": [float(x) for x in J],
             "note": "forward only (no wavefield), L2 normalised as w_objs_e.m"}
+
+
+def _uneven_batch(ctx, specs=((8, 2000), (8, 1800), (8, 1600), (8, 1400), (6, 1200), (6, 1000))):
+    """Surveys of uneven shot count and trace length on the workload's grid
+    (each (shots, nt) of `specs`; by default six lines of 6-8 shots, each too
+    small to fill the GPU alone), their FWI gradients in shared launches
+    (gjw_batch_fwi_gradient) against the same surveys one after another; one
+    warm-up call each, then one timed call."""
+    import torch
+    from gerjoii_amd import configs
+    from gerjoii_amd import survey as S
+    cases = [configs.c3(ns=a, nt=b) for a, b in specs]
+    svs, dobs = [], []
+    for c in cases:
+        sv = S.Survey(c.grid, ctx)
+        sv.set_shots(c.src, c.wavelets, c.receivers)
+        sv.set_model(c.eps_true, c.sig_true)
+        d = torch.zeros(sv.traces_shape(), dtype=torch.float32, device="cuda")
+        sv.forward(out=d)
+        sv.set_model(c.eps0, c.sig0)
+        svs.append(sv)
+        dobs.append(d)
+    cells = sum(3.0 * c.grid.cells() * c.grid.nt * len(c.src) for c in cases)
+    # one survey at a time first: each survey's own first sweep tunes its
+    # adjoint width, which the shared launches then use
+    for sv, d in zip(svs, dobs):
+        sv.fwi_gradient(d)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sv, d in zip(svs, dobs):
+        sv.fwi_gradient(d)
+    torch.cuda.synchronize()
+    tq = time.perf_counter() - t0
+    bt = S.Batch(svs)
+    bt.fwi_gradient(dobs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bt.fwi_gradient(dobs)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    bt.close()
+    for sv in svs:
+        sv.close()
+    del svs, dobs
+    torch.cuda.empty_cache()
+    shots = sum(a for a, _ in specs)
+    return {"surveys": [{"shots": a, "nt": b} for a, b in specs], "grid_cells": cases[0].grid.cells(),
+            "s_batched": tb, "s_one_survey_at_a_time": tq,
+            "shot_gradients_per_s_batched": shots / tb,
+            "gcell_updates_per_s_batched": cells / tb / 1e9,
+            "gcell_updates_per_s_one_at_a_time": cells / tq / 1e9,
+            "note": "shared launches: eight-step forwards and six-step adjoints across surveys, "
+                    "one checkpoint interval; each survey alone uses its own interval"}
 
 
 def _matio(case, d_obs, lines=8):
