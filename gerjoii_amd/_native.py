@@ -60,6 +60,13 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise GjwError(f"{LIB_PATH} is missing: build it with gerjoii_amd.build() "
                        "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    # torch ships its own HIP runtime under the same soname.  Loaded first, it
+    # is the one runtime of the process and libgjw binds to it; loaded after
+    # libgjw, a second runtime comes in and torch finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     i32, i64 = C.c_int32, C.c_int64

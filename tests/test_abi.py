@@ -30,3 +30,19 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in _declared() if not hasattr(lib, n)]
     assert not missing, missing
     assert sorted(N.EXPORTS) == _declared()
+
+
+def test_one_hip_runtime_per_process():
+    """Loading libgjw before torch must not bring a second HIP runtime into
+    the process (torch ships its own under the same soname; two runtimes
+    leave torch without a GPU).  Run in a fresh interpreter."""
+    import subprocess
+    import sys
+    code = ("import gerjoii_amd._native as N; N.load(); import torch; "
+            "m = open('/proc/self/maps').read().splitlines(); "
+            "print(len({l.split()[-1] for l in m if 'libamdhip64' in l}))")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    assert out.stdout.strip().splitlines()[-1] == "1"
