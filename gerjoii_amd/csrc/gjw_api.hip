@@ -57,23 +57,23 @@ extern template __global__ void k_fwd_step4<true>(GJW_FWDN_ARGS);
 extern template __global__ void k_fwd_step8<false>(GJW_FWDN_ARGS);
 extern template __global__ void k_fwd_step8<true>(GJW_FWDN_ARGS);
 #undef GJW_FWDN_ARGS
-__global__ void k_fwd_step_multi(const DevSurvey*, MultiArgs);
+__global__ void k_fwd_step_multi(const MultiTable, MultiArgs);
 template <int NS, bool FRAMES>
-__global__ void k_fwd_stepN_multi(const DevSurvey*, MultiArgs);
-extern template __global__ void k_fwd_stepN_multi<2, false>(const DevSurvey*, MultiArgs);
-extern template __global__ void k_fwd_stepN_multi<2, true>(const DevSurvey*, MultiArgs);
-extern template __global__ void k_fwd_stepN_multi<4, false>(const DevSurvey*, MultiArgs);
-extern template __global__ void k_fwd_stepN_multi<4, true>(const DevSurvey*, MultiArgs);
+__global__ void k_fwd_stepN_multi(const MultiTable, MultiArgs);
+extern template __global__ void k_fwd_stepN_multi<2, false>(const MultiTable, MultiArgs);
+extern template __global__ void k_fwd_stepN_multi<2, true>(const MultiTable, MultiArgs);
+extern template __global__ void k_fwd_stepN_multi<4, false>(const MultiTable, MultiArgs);
+extern template __global__ void k_fwd_stepN_multi<4, true>(const MultiTable, MultiArgs);
 template <bool FRAMES>
-__global__ void k_fwd_step8_multi(const DevSurvey*, MultiArgs);
-extern template __global__ void k_fwd_step8_multi<false>(const DevSurvey*, MultiArgs);
-extern template __global__ void k_fwd_step8_multi<true>(const DevSurvey*, MultiArgs);
-__global__ void k_adj_step_multi(const DevSurvey*, MultiArgs);
+__global__ void k_fwd_step8_multi(const MultiTable, MultiArgs);
+extern template __global__ void k_fwd_step8_multi<false>(const MultiTable, MultiArgs);
+extern template __global__ void k_fwd_step8_multi<true>(const MultiTable, MultiArgs);
+__global__ void k_adj_step_multi(const MultiTable, MultiArgs);
 template <int NS>
-__global__ void k_adj_stepN_multi(const DevSurvey*, MultiArgs);
-extern template __global__ void k_adj_stepN_multi<2>(const DevSurvey*, MultiArgs);
-extern template __global__ void k_adj_stepN_multi<4>(const DevSurvey*, MultiArgs);
-extern template __global__ void k_adj_stepN_multi<6>(const DevSurvey*, MultiArgs);
+__global__ void k_adj_stepN_multi(const MultiTable, MultiArgs);
+extern template __global__ void k_adj_stepN_multi<2>(const MultiTable, MultiArgs);
+extern template __global__ void k_adj_stepN_multi<4>(const MultiTable, MultiArgs);
+extern template __global__ void k_adj_stepN_multi<6>(const MultiTable, MultiArgs);
 __global__ void k_fill(float*, long long, float);
 __global__ void k_fill_d(double*, long long, double);
 __global__ void k_inject_last(DevGrid, int, int, int, const int*, const int*, const int*,
@@ -1195,6 +1195,16 @@ static int adj_depth(const gjw_survey* s) {
   return s->adj_fuse == 4 ? 4 : 6;
 }
 
+// Depths of the shared launches of a survey batch.  A survey's own autotuner
+// judges a depth by the survey alone, which a small survey cannot fill; the
+// batch fills the GPU together, so it takes the deepest kernels (8-step
+// forward, 6-step adjoint) unless an option asks a survey for less.
+static int batch_fwd_depth(const gjw_survey* s) { return s->fuse > 0 ? s->fuse : 8; }
+static int batch_adj_depth(const gjw_survey* s) {
+  if (s->fuse > 0 && s->fuse < 4) return s->fuse;
+  return s->adj_fuse == 4 ? 4 : 6;
+}
+
 static int ck_interval(const gjw_survey* s) {
   if (s->ck_batch > 0) return s->ck_batch;  // common interval of a survey batch
   if (s->ck_every > 0) return s->ck_every;
@@ -2111,8 +2121,10 @@ extern "C" int gjw_fwi_gradient(gjw_survey* s, const float* d_obs, double* g_eps
 struct gjw_batch {
   gjw_ctx* ctx = nullptr;
   std::vector<gjw_survey*> sv;
-  DBuf<DevSurvey> dsv;
+  DBuf<DevSurvey> dsv;  // the table on the device (its address keys the captured graphs)
+  MultiTable tab{};     // the table the kernels take by value
   int K = 0, T = 0, fuse = 2, afuse = 2, calls = 0;  // afuse: adjoint steps per launch
+  int L8 = 0, La6 = 0;  // strip widths of the shared 8-step forward / 6-step adjoint launches
   bool keep = true;  // forward sweep saves checkpoints (gradients); not for objectives
   GraphCache g_all, g_fwd, g_adj;
   double last_ms = 0, last_cells = 0;
@@ -2180,8 +2192,8 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
     const int La6 = s->tuned_La6 > 0 ? s->tuned_La6 : s->tuned_L8 > 0 ? s->tuned_L8 : 64;
     const int La4 = s->tuned_La4 > 0 ? s->tuned_La4 : s->tuned_L4;
     const int La2 = s->tuned_La2 > 0 ? s->tuned_La2 : s->tuned_L;
-    const int L = steps == 8   ? s->tuned_L8
-                  : steps == 6 ? La6
+    const int L = steps == 8   ? (b->L8 > 0 ? b->L8 : s->tuned_L8)
+                  : steps == 6 ? (b->La6 > 0 ? b->La6 : La6)
                   : steps == 4 ? (adj ? La4 : s->tuned_L4)
                   : steps == 2 ? (adj ? La2 : s->tuned_L)
                                : s->tuned_L1;
@@ -2204,28 +2216,67 @@ static int batch_launch(gjw_batch* b, const std::vector<int>& ids, int steps, bo
     return fail(GJW_ERR_STATE, "batch_launch: no %d-step %s kernel", steps,
                 adj ? "adjoint" : "forward");
   if (!adj && steps == 1)
-    hipLaunchKernelGGL(k_fwd_step_multi, grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL(k_fwd_step_multi, grid, block, 0, st, b->tab, a);
   else if (!adj && steps == 8 && frames)
-    hipLaunchKernelGGL((k_fwd_step8_multi<true>), grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_fwd_step8_multi<true>), grid, block, 0, st, b->tab, a);
   else if (!adj && steps == 8)
-    hipLaunchKernelGGL((k_fwd_step8_multi<false>), grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_fwd_step8_multi<false>), grid, block, 0, st, b->tab, a);
   else if (!adj && steps == 4 && frames)
-    hipLaunchKernelGGL((k_fwd_stepN_multi<4, true>), grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_fwd_stepN_multi<4, true>), grid, block, 0, st, b->tab, a);
   else if (!adj && steps == 4)
-    hipLaunchKernelGGL((k_fwd_stepN_multi<4, false>), grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_fwd_stepN_multi<4, false>), grid, block, 0, st, b->tab, a);
   else if (!adj && frames)
-    hipLaunchKernelGGL((k_fwd_stepN_multi<2, true>), grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_fwd_stepN_multi<2, true>), grid, block, 0, st, b->tab, a);
   else if (!adj)
-    hipLaunchKernelGGL((k_fwd_stepN_multi<2, false>), grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_fwd_stepN_multi<2, false>), grid, block, 0, st, b->tab, a);
   else if (steps == 1)
-    hipLaunchKernelGGL(k_adj_step_multi, grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL(k_adj_step_multi, grid, block, 0, st, b->tab, a);
   else if (steps == 6)
-    hipLaunchKernelGGL((k_adj_stepN_multi<6>), grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_adj_stepN_multi<6>), grid, block, 0, st, b->tab, a);
   else if (steps == 4)
-    hipLaunchKernelGGL((k_adj_stepN_multi<4>), grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_adj_stepN_multi<4>), grid, block, 0, st, b->tab, a);
   else
-    hipLaunchKernelGGL((k_adj_stepN_multi<2>), grid, block, 0, st, b->dsv.p, a);
+    hipLaunchKernelGGL((k_adj_stepN_multi<2>), grid, block, 0, st, b->tab, a);
   HIPCHK(hipGetLastError());
+  return GJW_OK;
+}
+
+// Strip width of the shared deep launches, tuned for the batch as a whole:
+// a survey's own tuning sees the survey alone, which a small survey cannot
+// fill, and picks narrow strips (many blocks) that cost the filled batch
+// halo work.  Times the 8-step forward of every survey on its current state
+// at each candidate (the state is zeroed before any sweep); the 6-step
+// adjoint takes 3/4 of that width (the ratio measured on single surveys).
+static int batch_tune(gjw_batch* b) {
+  if (b->L8 > 0 || b->fuse < 8) return GJW_OK;
+  const int S = (int)b->sv.size();
+  bool fixed = true;  // every survey's width set by GJW_OPT_TILE_L: keep those
+  for (gjw_survey* s : b->sv) fixed &= s->tile_L > 0;
+  if (fixed) return GJW_OK;
+  hipStream_t st = b->ctx->stream;
+  std::vector<int> all(S);
+  int it[kMaxSurveys] = {}, cur[kMaxSurveys] = {}, xc[kMaxSurveys] = {};
+  for (int k = 0; k < S; ++k) all[k] = k;
+  float best = 1e30f;
+  int bestL = 0;
+  for (int L : {48, 64, 96, 128}) {
+    b->L8 = L;
+    for (int r = -1; r < 3; ++r) {
+      if (r == 0) HIPCHK(hipEventRecord(b->ctx->ev0, st));
+      TRY(batch_launch(b, all, 8, false, false, it, cur, xc, nullptr, 0));
+      for (int k = 0; k < S; ++k) cur[k] ^= 1, xc[k] ^= 1;
+    }
+    HIPCHK(hipEventRecord(b->ctx->ev1, st));
+    HIPCHK(hipEventSynchronize(b->ctx->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
+    if (ms < best) {
+      best = ms;
+      bestL = L;
+    }
+  }
+  b->L8 = bestL;
+  b->La6 = std::max(32, bestL * 3 / 4 / 8 * 8);
   return GJW_OK;
 }
 
@@ -2381,8 +2432,8 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
   const bool user_K = K > 0;
   if (!user_K) K = std::max(1, (int)std::ceil(std::sqrt((double)T)));
   b->T = T;
-  // forward and adjoint depths: the shallowest any survey's autotuner chose
-  // (the shared adjoint kernels go up to six steps)
+  // forward and adjoint depths: the shallowest any survey asks for (the
+  // shared adjoint kernels go up to six steps)
   int afuse = 6;
   fuse = 8;
   auto prepare_all = [&](int Kc) {
@@ -2392,8 +2443,8 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
       const int rc = fwi_batched_prepare(s);
       s->ck_batch = 0;
       TRY(rc);
-      fuse = std::min(fuse, s->fuse_eff);  // (after the autotuner)
-      afuse = std::min(afuse, std::min(adj_depth(s), 6));
+      fuse = std::min(fuse, batch_fwd_depth(s));
+      afuse = std::min(afuse, batch_adj_depth(s));
     }
     return (int)GJW_OK;
   };
@@ -2420,6 +2471,8 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
   TRY(b->dsv.ensure(S));
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipMemcpy(b->dsv.p, hs.data(), S * sizeof(DevSurvey), hipMemcpyHostToDevice));
+  for (int k = 0; k < S; ++k) b->tab.sv[k] = hs[k];
+  TRY(batch_tune(b));
   HIPCHK(hipEventRecord(b->ctx->ev0, st));
   // the first call runs eagerly; with GJW_OPT_GRAPH on every survey, later
   // calls replay captured sweeps (one graph when every misfit is L2)
@@ -2431,7 +2484,8 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
     TRY(batch_adj_enqueue(b));
   } else {
     std::vector<uintptr_t> key = {reinterpret_cast<uintptr_t>(b->dsv.p), (uintptr_t)K,
-                                  (uintptr_t)T, (uintptr_t)fuse, (uintptr_t)afuse};
+                                  (uintptr_t)T, (uintptr_t)fuse, (uintptr_t)afuse,
+                                  (uintptr_t)b->L8, (uintptr_t)b->La6};
     for (gjw_survey* s : b->sv) {
       const std::vector<uintptr_t> ks = fwi_graph_key(s);
       key.insert(key.end(), ks.begin(), ks.end());
@@ -2481,7 +2535,7 @@ extern "C" int gjw_batch_objective(gjw_batch* b, const float* const* d_obs, doub
   const int S = (int)b->sv.size();
   HIPCHK(hipSetDevice(b->ctx->device));
   hipStream_t st = b->ctx->stream;
-  int T = 0, fuse = 8;  // forward only: the 8-step kernel when every survey's autotuner chose it
+  int T = 0, fuse = 8;  // forward only
   for (int k = 0; k < S; ++k) {
     gjw_survey* s = b->sv[k];
     TRY(check_ready(s, "gjw_batch_objective"));
@@ -2503,7 +2557,7 @@ extern "C" int gjw_batch_objective(gjw_batch* b, const float* const* d_obs, doub
     TRY(s->Jd.ensure(2));
     TRY(zero_fwd_state(s, s->ns));
     TRY(autotune_L(s, s->ns));
-    fuse = std::min(fuse, s->fuse_eff);
+    fuse = std::min(fuse, batch_fwd_depth(s));
     TRY(host_or_dev_copy(s->obs.p, d_obs[k], n * 4, st));
   }
   std::vector<DevSurvey> hs(S);
@@ -2511,10 +2565,12 @@ extern "C" int gjw_batch_objective(gjw_batch* b, const float* const* d_obs, doub
   TRY(b->dsv.ensure(S));
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipMemcpy(b->dsv.p, hs.data(), S * sizeof(DevSurvey), hipMemcpyHostToDevice));
+  for (int k = 0; k < S; ++k) b->tab.sv[k] = hs[k];
   b->T = T;
   b->K = T;  // one segment: no checkpoints
   b->fuse = fuse;
   b->keep = false;
+  TRY(batch_tune(b));
   HIPCHK(hipEventRecord(b->ctx->ev0, st));
   const int rc = batch_fwd_enqueue(b);
   b->keep = true;

@@ -76,6 +76,15 @@ struct DevSurvey {
 
 // Per-launch part of a multi-survey launch (kernel argument): survey sid[k]
 // owns blocks [blk0[k], blk0[k+1]).
+// The surveys of a multi-survey launch, passed by value in the kernel
+// arguments: pointers read from the kernel arguments are known to be global
+// memory, so the bodies' loads and stores compile to global/buffer
+// instructions (pointers read from a device array would take the slower
+// flat path).
+struct MultiTable {
+  DevSurvey sv[kMaxSurveys];
+};
+
 struct MultiArgs {
   int n;
   int blk0[kMaxSurveys + 1];

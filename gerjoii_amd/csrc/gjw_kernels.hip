@@ -157,7 +157,11 @@ __device__ __forceinline__ unsigned vtiles_y(int NY, int L) {
 // tile_decode.
 __device__ __forceinline__ TileId vtile_decode(unsigned t, unsigned gx, unsigned gy, unsigned ns,
                                               bool slow_first) {
-  if (!slow_first || gx < 3) return tile_decode(t, gx, gy, ns);
+  // Each XCD takes a contiguous eighth of the logical tiles; the reordering
+  // keeps the XCDs balanced only when that eighth is whole shot groups (else
+  // some XCDs would get only the slow block-rows).
+  const unsigned groups = (ns + (unsigned)kShotGroup - 1) / (unsigned)kShotGroup;
+  if (!slow_first || gx < 3 || groups % 8u != 0) return tile_decode(t, gx, gy, ns);
   const unsigned span = (unsigned)kShotGroup * gx * gy;
   const unsigned grp = t / span, rem = t - grp * span;
   const unsigned gc = min((unsigned)kShotGroup, ns - grp * (unsigned)kShotGroup);
@@ -214,13 +218,13 @@ __device__ __forceinline__ RowTile row_tile(int t, int NX, int R, int NS) {
 // [a.blk0[k], a.blk0[k+1]) (a multiple of 8 each, so the XCD-aware order holds
 // per survey); its own step, tile width, buffer parities and frame slot come
 // with it.  Blocks past a survey's tiles exit.
-__device__ __forceinline__ bool multi_tile(const MultiArgs& a, const DevSurvey* __restrict__ sv,
+__device__ __forceinline__ bool multi_tile(const MultiArgs& a, const MultiTable& tab,
                                            int rows, int& k, TileId& tid, bool vert = false,
                                            bool slow_first = false) {
   const int b = (int)blockIdx.x;
   k = 0;
   while (k + 1 < a.n && b >= a.blk0[k + 1]) ++k;
-  const DevSurvey& S = sv[a.sid[k]];
+  const DevSurvey& S = tab.sv[a.sid[k]];
   const unsigned B = (unsigned)(a.blk0[k + 1] - a.blk0[k]), bl = (unsigned)(b - a.blk0[k]);
   const unsigned per = B / 8u;
   const unsigned t = (bl % 8u) * per + bl / 8u;
@@ -394,11 +398,11 @@ k_fwd_step(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
 }
 
 __global__ void __launch_bounds__(256)
-k_fwd_step_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
+k_fwd_step_multi(const MultiTable tab, MultiArgs a) {
   int k;
   TileId tid;
-  if (!multi_tile(a, sv, 64, k, tid)) return;
-  const DevSurvey& S = sv[a.sid[k]];
+  if (!multi_tile(a, tab, 64, k, tid)) return;
+  const DevSurvey& S = tab.sv[a.sid[k]];
   const int c = a.cur[k], x = a.xc[k];
   float* f = S.fld;
   const long long z = S.fsz;
@@ -997,11 +1001,11 @@ k_fwd_step8(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
 }
 
 template <int NS, bool FRAMES>
-__device__ __forceinline__ void fwd_multi_body(const DevSurvey* __restrict__ sv, const MultiArgs& a) {
+__device__ __forceinline__ void fwd_multi_body(const MultiTable& tab, const MultiArgs& a) {
   int k;
   TileId tid;
-  if (!multi_tile(a, sv, 64 - 2 * NS, k, tid, true, true)) return;
-  const DevSurvey& S = sv[a.sid[k]];
+  if (!multi_tile(a, tab, 64 - 2 * NS, k, tid, true, true)) return;
+  const DevSurvey& S = tab.sv[a.sid[k]];
   const int c = a.cur[k], x = a.xc[k];
   float* f = S.fld;
   const long long z = S.fsz;
@@ -1015,18 +1019,18 @@ __device__ __forceinline__ void fwd_multi_body(const DevSurvey* __restrict__ sv,
 
 template <int NS, bool FRAMES>
 __global__ void __launch_bounds__(256, GJW_FWD_WAVES)
-k_fwd_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
-  fwd_multi_body<NS, FRAMES>(sv, a);
+k_fwd_stepN_multi(const MultiTable tab, MultiArgs a) {
+  fwd_multi_body<NS, FRAMES>(tab, a);
 }
 
 // eight steps over several surveys, with the 8-step forward's register budget
 template <bool FRAMES>
 __global__ void __launch_bounds__(256, GJW_FWD8_WAVES)
-k_fwd_step8_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
-  fwd_multi_body<8, FRAMES>(sv, a);
+k_fwd_step8_multi(const MultiTable tab, MultiArgs a) {
+  fwd_multi_body<8, FRAMES>(tab, a);
 }
-template __global__ void k_fwd_step8_multi<false>(const DevSurvey*, MultiArgs);
-template __global__ void k_fwd_step8_multi<true>(const DevSurvey*, MultiArgs);
+template __global__ void k_fwd_step8_multi<false>(const MultiTable, MultiArgs);
+template __global__ void k_fwd_step8_multi<true>(const MultiTable, MultiArgs);
 
 #define GJW_FWDN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \
     const float*, const float*, float*, float*, float*, float*, float*, int, float*, float*, long long
@@ -1037,10 +1041,10 @@ template __global__ void k_fwd_step4<true>(GJW_FWDN_ARGS);
 template __global__ void k_fwd_step8<false>(GJW_FWDN_ARGS);
 template __global__ void k_fwd_step8<true>(GJW_FWDN_ARGS);
 #undef GJW_FWDN_ARGS
-template __global__ void k_fwd_stepN_multi<2, false>(const DevSurvey*, MultiArgs);
-template __global__ void k_fwd_stepN_multi<2, true>(const DevSurvey*, MultiArgs);
-template __global__ void k_fwd_stepN_multi<4, false>(const DevSurvey*, MultiArgs);
-template __global__ void k_fwd_stepN_multi<4, true>(const DevSurvey*, MultiArgs);
+template __global__ void k_fwd_stepN_multi<2, false>(const MultiTable, MultiArgs);
+template __global__ void k_fwd_stepN_multi<2, true>(const MultiTable, MultiArgs);
+template __global__ void k_fwd_stepN_multi<4, false>(const MultiTable, MultiArgs);
+template __global__ void k_fwd_stepN_multi<4, true>(const MultiTable, MultiArgs);
 
 // ==========================================================================
 // adjoint: one step of the transpose of k_fwd_step
@@ -1287,11 +1291,11 @@ k_adj_step(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
 }
 
 __global__ void __launch_bounds__(256)
-k_adj_step_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
+k_adj_step_multi(const MultiTable tab, MultiArgs a) {
   int k;
   TileId tid;
-  if (!multi_tile(a, sv, 64, k, tid)) return;
-  const DevSurvey& S = sv[a.sid[k]];
+  if (!multi_tile(a, tab, 64, k, tid)) return;
+  const DevSurvey& S = tab.sv[a.sid[k]];
   const int c = a.cur[k], d = c ^ 1;
   float* m = S.adjf;
   const long long z = S.fsz;
@@ -1883,11 +1887,11 @@ k_adj_step8(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
 
 template <int NS>
 __global__ void __launch_bounds__(256, GJW_ADJM_WAVES)
-k_adj_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
+k_adj_stepN_multi(const MultiTable tab, MultiArgs a) {
   int k;
   TileId tid;
-  if (!multi_tile(a, sv, 64 - 2 * NS, k, tid, true)) return;
-  const DevSurvey& S = sv[a.sid[k]];
+  if (!multi_tile(a, tab, 64 - 2 * NS, k, tid, true)) return;
+  const DevSurvey& S = tab.sv[a.sid[k]];
   const int c = a.cur[k], d = c ^ 1;
   float* m = S.adjf;
   const long long z = S.fsz;
@@ -1897,9 +1901,9 @@ k_adj_stepN_multi(const DevSurvey* __restrict__ sv, MultiArgs a) {
                 m + (5 * d + 2) * z, m + (5 * d + 3) * z, m + (5 * d + 4) * z, a.it[k], S.adj, Fa,
                 S.fstride, S.acc, S.FD, nullptr);
 }
-template __global__ void k_adj_stepN_multi<2>(const DevSurvey*, MultiArgs);
-template __global__ void k_adj_stepN_multi<4>(const DevSurvey*, MultiArgs);
-template __global__ void k_adj_stepN_multi<6>(const DevSurvey*, MultiArgs);
+template __global__ void k_adj_stepN_multi<2>(const MultiTable, MultiArgs);
+template __global__ void k_adj_stepN_multi<4>(const MultiTable, MultiArgs);
+template __global__ void k_adj_stepN_multi<6>(const MultiTable, MultiArgs);
 
 // ==========================================================================
 // small kernels
