@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-linesearch", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--study", choices=["c4", "c5"], default=None,
+                    help="instead of the headline: the precision study at 4000 x 1500 (c4) or the "
+                         "uneven three-survey batch (c5), one JSON line")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--fuse", type=int, default=0, help="forward steps per launch: 0 autotuned, 1, 2, 4, 8")
     ap.add_argument("--adj-fuse", type=int, default=0, help="adjoint steps per launch: 0 (six), 4, 6, 8")
@@ -80,6 +83,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
+    if args.study:
+        print(json.dumps(_study_c4() if args.study == "c4" else _study_c5()))
+        return
     case = configs.c3(ns=args.shots, nt=args.nt)
     g = case.grid
     cells = g.cells()
@@ -266,6 +272,123 @@ def _line_search(ctx, case, d_obs, nparabo=3):
             "shot_objectives_per_s": nparabo * ns / tb,
             "objectives": [float(x) for x in J],
             "note": "forward only (no wavefield), L2 normalised as w_objs_e.m"}
+
+
+def _observed(sv, case):
+    """Observed traces of the true model, on the device."""
+    import torch
+    sv.set_model(case.eps_true, case.sig_true)
+    d = torch.zeros(sv.traces_shape(), dtype=torch.float32, device="cuda")
+    sv.forward(out=d)
+    sv.set_model(case.eps0, case.sig0)
+    return d
+
+
+def _study_c4(ns_cmp=16):
+    """Precision of 16-bit checkpoints at 4000 x 1500, nt 12000: the FWI
+    gradient of the first `ns_cmp` shots with FP32, BF16 and FP16 snapshots
+    (FP32 as the reference: at this size FP32 snapshots of all 32 shots do
+    not fit one GPU next to the frames), then all 32 shots with BF16
+    snapshots timed."""
+    import torch
+    from gerjoii_amd import configs
+    from gerjoii_amd import survey as S
+    from gerjoii_amd import _native as N
+    ctx = S.context(0)
+    case = configs.c4()
+    g = case.grid
+    sub = dict(src=case.src[:ns_cmp], wavelets=case.wavelets[:ns_cmp], receivers=case.receivers[:ns_cmp])
+    grads = {}
+    for name, snap in (("fp32", N.GJW_SNAP_F32), ("bf16", N.GJW_SNAP_BF16), ("fp16", N.GJW_SNAP_F16)):
+        sv = S.Survey(g, ctx)
+        sv.set_shots(sub["src"], sub["wavelets"], sub["receivers"])
+        d = _observed(sv, case)
+        sv.set_option(N.GJW_OPT_SNAPSHOT, snap)
+        t0 = time.perf_counter()
+        ge, gs, J = sv.fwi_gradient(d)
+        grads[name] = (ge, gs, J, time.perf_counter() - t0)
+        log(f"c4 {name}: J={J:.6e} {grads[name][3]:.1f} s")
+        sv.close()
+        del sv, d
+        torch.cuda.empty_cache()
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    cos = lambda a, b: float(np.ravel(a) @ np.ravel(b) / (np.linalg.norm(a) * np.linalg.norm(b)))
+    ref = grads["fp32"]
+    out = {"grid": [g.n, g.m], "nt": g.nt, "shots_compared": ns_cmp}
+    for name in ("bf16", "fp16"):
+        ge, gs, J, _ = grads[name]
+        out[name] = {"eps_rel_l2": rel(ge, ref[0]), "eps_cosine": cos(ge, ref[0]),
+                     "sig_rel_l2": rel(gs, ref[1]), "sig_cosine": cos(gs, ref[1]),
+                     "J_rel": abs(J - ref[2]) / abs(ref[2])}
+    sv = S.Survey(g, ctx)
+    sv.set_shots(case.src, case.wavelets, case.receivers)
+    d = _observed(sv, case)
+    sv.set_option(N.GJW_OPT_SNAPSHOT, N.GJW_SNAP_BF16)
+    sv.fwi_gradient(d)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sv.fwi_gradient(d)
+    torch.cuda.synchronize()
+    dt_s = time.perf_counter() - t0
+    cells = 3.0 * g.cells() * g.nt * len(case.src)
+    out["bf16_all_shots"] = {"shots": len(case.src), "s": dt_s,
+                             "shot_gradients_per_s": len(case.src) / dt_s,
+                             "gcell_updates_per_s": cells / dt_s / 1e9}
+    sv.close()
+    return {"study": "c4 precision", **out,
+            "note": "reference: the FP32-snapshot gradient of the same shots (the library's FP32 path "
+                    "meets relative L2 <= 1e-3 against the FP64 oracle in the parity tests)"}
+
+
+def _study_c5():
+    """The uneven three-survey batch at full size: envelope, phase and envelope
+    misfits, BF16 checkpoints, captured graphs, per-shot gradients reduced in
+    FP64 per survey; shared launches against one survey at a time."""
+    import torch
+    from gerjoii_amd import configs
+    from gerjoii_amd import survey as S
+    from gerjoii_amd import _native as N
+    ctx = S.context(0)
+    cases = configs.c5()
+    svs, dobs = [], []
+    for case, misfit in zip(cases, ("env_L2", "phase_L2", "env_L2")):
+        sv = S.Survey(case.grid, ctx)
+        sv.set_shots(case.src, case.wavelets, case.receivers)
+        dobs.append(_observed(sv, case))
+        sv.set_misfit(misfit)
+        sv.set_option(N.GJW_OPT_SNAPSHOT, N.GJW_SNAP_BF16)
+        sv.set_option(N.GJW_OPT_GRAPH, 1)
+        svs.append(sv)
+    cells = sum(3.0 * c.grid.cells() * c.grid.nt * len(c.src) for c in cases)
+    for sv, d in zip(svs, dobs):
+        sv.fwi_gradient(d)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sv, d in zip(svs, dobs):
+        sv.fwi_gradient(d)
+    torch.cuda.synchronize()
+    tq = time.perf_counter() - t0
+    bt = S.Batch(svs)
+    bt.fwi_gradient(dobs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ge, gs, J = bt.fwi_gradient(dobs)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    finite = all(bool(np.isfinite(a).all()) for a in ge + gs)
+    shots = sum(len(c.src) for c in cases)
+    bt.close()
+    for sv in svs:
+        sv.close()
+    return {"study": "c5 uneven batch",
+            "surveys": [{"grid": [c.grid.n, c.grid.m], "nt": c.grid.nt, "shots": len(c.src)}
+                        for c in cases],
+            "misfits": ["env_L2", "phase_L2", "env_L2"], "snapshots": "bf16", "graph": True,
+            "s_batched": tb, "s_one_survey_at_a_time": tq,
+            "shot_gradients_per_s_batched": shots / tb,
+            "gcell_updates_per_s_batched": cells / tb / 1e9,
+            "gcell_updates_per_s_one_at_a_time": cells / tq / 1e9,
+            "J": [float(x) for x in J], "gradients_finite": finite}
 
 
 def _uneven_batch(ctx, specs=((8, 2000), (8, 1800), (8, 1600), (8, 1400), (6, 1200), (6, 1000))):

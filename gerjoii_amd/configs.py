@@ -157,6 +157,58 @@ def c3(seed=1003, ns=64, nt=8000):
     return _case("C3", g, 200e6, eps, sig, shape, src, recs, 8)
 
 
+def c4(seed=1004, ns=32, nt=12000):
+    """Precision-study scale: 4000 x 1500, dx 0.02 m, 40-cell PML, nt 12000,
+    250 MHz, 32 shots; layered eps in [3, 20] with a water-table layer
+    (eps 81) and void inclusions (eps 1, sigma 0); lognormal sigma; 400
+    receivers every 10 cells."""
+    rng = np.random.default_rng(seed)
+    m, n, P = 1500, 4000, 40
+    dx = 0.02
+    g = Grid(m=m, n=n, pml=P, air=0, nt=nt, dx=dx, dt=dt_cfl(dx))
+    eps = _layered(rng, m, n, 6, 3.0, 20.0)
+    sig = _lognormal_sigma(rng, m, n)
+    z = np.arange(m)[:, None]
+    top = int(0.45 * m) + (np.sin(np.arange(n) / 300.0 + rng.uniform(0, 6)) * 20)[None, :]
+    water = (z >= top) & (z < top + m // 20)
+    eps[water] = 81.0
+    sig[water] = 1e-2
+    for _ in range(12):
+        cz, cx = rng.uniform(0.15 * m, 0.9 * m), rng.uniform(0, n)
+        rz, rx = rng.uniform(8, 25), rng.uniform(10, 40)
+        void = ((z - cz) / rz) ** 2 + ((np.arange(n)[None, :] - cx) / rx) ** 2 <= 1
+        eps[void] = 1.0
+        sig[void] = 0.0
+    h = _topography(rng, n)
+    eps, sig, shape = _apply_topo(eps, sig, h)
+    first = (n - 120 * (ns - 1)) // 2
+    src, recs = _surface_geometry(g, h, [first + 120 * k for k in range(ns)],
+                                  np.arange(0, n, 10)[:400])
+    return _case("C4", g, 250e6, eps, sig, shape, src, recs, 8)
+
+
+def c5(seed=1005):
+    """Uneven batch: survey A 600 x 200 (nt 3000, 24 shots), B 1500 x 500
+    (nt 7000, 48 shots), C 3000 x 800 (nt 10000, 16 shots); dx 0.025 m,
+    40-cell PML, 200 MHz, layered models with topography as C3, receivers
+    every 5 cells (at most 400)."""
+    out = []
+    for k, (n, m, nt, ns) in enumerate(((600, 200, 3000, 24), (1500, 500, 7000, 48),
+                                        (3000, 800, 10000, 16))):
+        rng = np.random.default_rng(seed + k)
+        dx = 0.025
+        g = Grid(m=m, n=n, pml=40, air=0, nt=nt, dx=dx, dt=dt_cfl(dx))
+        eps = _layered(rng, m, n, 6, 3.0, 20.0)
+        sig = _lognormal_sigma(rng, m, n)
+        h = _topography(rng, n)
+        eps, sig, shape = _apply_topo(eps, sig, h)
+        step = max(1, (n - 20) // ns)
+        src, recs = _surface_geometry(g, h, [10 + step * q for q in range(ns)],
+                                      np.arange(0, n, 5)[:400])
+        out.append(_case("C5" + "ABC"[k], g, 200e6, eps, sig, shape, src, recs, 8))
+    return out
+
+
 def small(seed=7, m=40, n=60, P=10, air=0, nt=300, ns=3, nr=12, dx=0.02, f0=300e6):
     """Small layered survey with topography for parity tests."""
     rng = np.random.default_rng(seed)
@@ -171,4 +223,4 @@ def small(seed=7, m=40, n=60, P=10, air=0, nt=300, ns=3, nr=12, dx=0.02, f0=300e
     return _case("small", g, f0, eps, sig, shape, src, recs, 3)
 
 
-CASES = {"C1": c1, "C2": c2, "C3": c3}
+CASES = {"C1": c1, "C2": c2, "C3": c3, "C4": c4}

@@ -1205,6 +1205,8 @@ static int batch_adj_depth(const gjw_survey* s) {
   return s->adj_fuse == 4 ? 4 : 6;
 }
 
+static long long ck_stride(const gjw_survey* s);
+
 static int ck_interval(const gjw_survey* s) {
   if (s->ck_batch > 0) return s->ck_batch;  // common interval of a survey batch
   if (s->ck_every > 0) return s->ck_every;
@@ -1213,10 +1215,27 @@ static int ck_interval(const gjw_survey* s) {
   // state, so frames + checkpoints take least memory near 1.8 sqrt(nt); the
   // longer segments also halve the per-segment work (restore, first frame,
   // flush): 1.2% faster than sqrt(nt) at nt = 8000.
+  const int nt = s->gr.nt;
   const int qf = s->fuse_eff >= 8 ? 8 : 4, qa = adj_depth(s);
   const int q = qf * qa / std::gcd(qf, qa);
-  const int k = std::max(1, (int)std::ceil(2.0 * std::sqrt((double)s->gr.nt)));
-  return std::min(s->gr.nt, (k + q - 1) / q * q);
+  const int k = std::max(1, (int)std::ceil(2.0 * std::sqrt((double)nt)));
+  int K = std::min(nt, (k + q - 1) / q * q);
+  // A survey whose checkpoints and frames would pass half of the device's
+  // memory at that interval takes the interval that needs the least memory:
+  // ceil(nt/K) checkpoints of c bytes plus K+1 frames of f bytes.
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return K;
+  }
+  const double c = 4.0 * (double)ck_stride(s), f = 4.0 * s->ns * (double)s->LD * s->gr.n;
+  auto bytes = [&](int kk) { return std::ceil((double)nt / kk) * c + (kk + 1) * f; };
+  if (bytes(K) <= 0.5 * (double)tot) return K;
+  const int k0 = std::max(1, (int)std::lround(std::sqrt(nt * c / f)));
+  int best = K;
+  for (int kk = std::max(q, k0 / q * q); kk <= std::min(nt, (k0 / q + 1) * q); kk += q)
+    if (bytes(kk) < bytes(best)) best = kk;
+  return best;
 }
 
 // --------------------------------------------------------------------------
