@@ -394,54 +394,60 @@ def _study_c5():
 def _uneven_batch(ctx, specs=((8, 2000), (8, 1800), (8, 1600), (8, 1400), (6, 1200), (6, 1000))):
     """Surveys of uneven shot count and trace length on the workload's grid
     (each (shots, nt) of `specs`; by default six lines of 6-8 shots, each too
-    small to fill the GPU alone), their FWI gradients in shared launches
-    (gjw_batch_fwi_gradient) against the same surveys one after another; one
-    warm-up call each, then one timed call."""
+    small to fill the GPU alone): their FWI gradients one survey at a time,
+    concurrently (each survey on its own context and stream, one host thread
+    each) and in the shared launches of gjw_batch_fwi_gradient; one warm-up
+    call each, then one timed call."""
     import torch
     from gerjoii_amd import configs
     from gerjoii_amd import survey as S
     cases = [configs.c3(ns=a, nt=b) for a, b in specs]
-    svs, dobs = [], []
-    for c in cases:
-        sv = S.Survey(c.grid, ctx)
+
+    def build(c, cx):
+        sv = S.Survey(c.grid, cx)
         sv.set_shots(c.src, c.wavelets, c.receivers)
-        sv.set_model(c.eps_true, c.sig_true)
-        d = torch.zeros(sv.traces_shape(), dtype=torch.float32, device="cuda")
-        sv.forward(out=d)
-        sv.set_model(c.eps0, c.sig0)
-        svs.append(sv)
-        dobs.append(d)
+        return sv, _observed(sv, c)
+
+    def timed(f):
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
     cells = sum(3.0 * c.grid.cells() * c.grid.nt * len(c.src) for c in cases)
-    # one survey at a time first: each survey's own first sweep tunes its
-    # adjoint width, which the shared launches then use
-    for sv, d in zip(svs, dobs):
-        sv.fwi_gradient(d)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for sv, d in zip(svs, dobs):
-        sv.fwi_gradient(d)
-    torch.cuda.synchronize()
-    tq = time.perf_counter() - t0
-    bt = S.Batch(svs)
-    bt.fwi_gradient(dobs)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    bt.fwi_gradient(dobs)
-    torch.cuda.synchronize()
-    tb = time.perf_counter() - t0
-    bt.close()
+    ctxs = [S.Context(0) for _ in cases]
+    own = [build(c, cx) for c, cx in zip(cases, ctxs)]
+    svs, dobs = [a for a, _ in own], [b for _, b in own]
+    tq = timed(lambda: [sv.fwi_gradient(d) for sv, d in zip(svs, dobs)])
+    tc = timed(lambda: S.concurrent_fwi_gradients(svs, dobs))
     for sv in svs:
         sv.close()
-    del svs, dobs
+    for cx in ctxs:
+        cx.close()
+    shared = [build(c, ctx) for c in cases]
+    # each survey's own first sweep tunes its adjoint width, which the shared
+    # launches then use
+    for sv, d in shared:
+        sv.fwi_gradient(d)
+    bt = S.Batch([sv for sv, _ in shared])
+    tb = timed(lambda: bt.fwi_gradient([d for _, d in shared]))
+    bt.close()
+    for sv, _ in shared:
+        sv.close()
+    del own, svs, dobs, shared
     torch.cuda.empty_cache()
     shots = sum(a for a, _ in specs)
     return {"surveys": [{"shots": a, "nt": b} for a, b in specs], "grid_cells": cases[0].grid.cells(),
-            "s_batched": tb, "s_one_survey_at_a_time": tq,
-            "shot_gradients_per_s_batched": shots / tb,
-            "gcell_updates_per_s_batched": cells / tb / 1e9,
-            "gcell_updates_per_s_one_at_a_time": cells / tq / 1e9,
-            "note": "shared launches: eight-step forwards and six-step adjoints across surveys, "
-                    "one checkpoint interval; each survey alone uses its own interval"}
+            "s_one_survey_at_a_time": tq, "s_concurrent": tc, "s_shared_launches": tb,
+            "shot_gradients_per_s_concurrent": shots / tc,
+            "gcell_updates_per_s": {"one_at_a_time": cells / tq / 1e9,
+                                    "concurrent": cells / tc / 1e9,
+                                    "shared_launches": cells / tb / 1e9},
+            "note": "concurrent: each survey on its own context (HIP stream), single-survey kernels, "
+                    "one host thread per survey; shared launches: eight-step forwards and six-step "
+                    "adjoints across surveys, one checkpoint interval"}
 
 
 def _matio(case, d_obs, lines=8):

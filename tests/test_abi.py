@@ -46,3 +46,38 @@ def test_one_hip_runtime_per_process():
                          timeout=300)
     assert out.returncode == 0, out.stderr[-2000:]
     assert out.stdout.strip().splitlines()[-1] == "1"
+
+
+def test_context_close_releases_its_surveys_first():
+    """Context.close() destroys the context's batches and surveys before the
+    context itself (a survey destroyed after its context would use freed
+    memory); the closed objects' later close() is a no-op."""
+    from gerjoii_amd import survey as S
+    calls = []
+
+    class FakeLib:
+        def __getattr__(self, name):
+            def f(*a):
+                calls.append(name)
+                return 0
+            return f
+
+    from gerjoii_amd import _native as N
+    saved = N._lib
+    N._lib = FakeLib()
+    try:
+        import ctypes
+        cx = S.Context(0)
+        cx.h = ctypes.c_void_p(1)  # the stand-in library hands out no handles
+        sv = S.Survey(S.Grid(m=4, n=5, pml=2, air=0, nt=3, dx=0.1, dt=1e-10), cx)
+        sv.h = ctypes.c_void_p(2)
+        bt = S.Batch([sv])
+        bt.h = ctypes.c_void_p(3)
+        cx.close()
+        assert calls[-3:] == ["gjw_batch_destroy", "gjw_survey_destroy", "gjw_destroy"]
+        n = len(calls)
+        bt.close()
+        sv.close()
+        assert len(calls) == n
+    finally:
+        N._lib = saved

@@ -199,6 +199,35 @@ def test_joint_update_post(oracle_mod, gpu_ctx):
     assert rel_max(ge, re) <= 1e-12 and rel_max(gs, rs) <= 1e-12
 
 
+def test_concurrent_fwi_gradients(gpu_ctx):
+    """Surveys on their own contexts, their FWI gradients from concurrent host
+    threads: equal to one survey at a time bit for bit; a shared context is
+    refused."""
+    from gerjoii_amd import survey as S
+    from test_gpu_parity import make_survey
+    cases = [configs.small(nt=180, ns=2, seed=s) for s in (3, 4, 5)]
+    ctxs = [S.Context(0) for _ in cases]
+    try:
+        svs, dobs = [], []
+        for case, cx in zip(cases, ctxs):
+            svt = make_survey(case, cx)
+            svt.set_model(case.eps_true, case.sig_true)
+            dobs.append(svt.forward())
+            svs.append(make_survey(case, cx, L=16))
+        ref = [sv.fwi_gradient(d) for sv, d in zip(svs, dobs)]
+        got = S.concurrent_fwi_gradients(svs, dobs)
+        for (a, b, ja), (c, d, jc) in zip(got, ref):
+            np.testing.assert_array_equal(a, c)
+            np.testing.assert_array_equal(b, d)
+            assert ja == jc
+        with pytest.raises(ValueError):
+            S.concurrent_fwi_gradients([make_survey(cases[0], gpu_ctx), make_survey(cases[1], gpu_ctx)],
+                                       dobs[:2])
+    finally:
+        for cx in ctxs:
+            cx.close()
+
+
 def test_update_pipeline(oracle_mod, gpu_ctx):
     """w_update_e.m:108-145 + w_update_e_.m:38-41 + w_image_e.m:50-51 on the
     device (postproc.w_update_post / w_update_stack) from device per-shot FWI
