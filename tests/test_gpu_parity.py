@@ -105,6 +105,35 @@ def test_forward_c1(oracle_mod, gpu_ctx):
     assert err <= 1e-4
 
 
+def test_forward_c2(oracle_mod, gpu_ctx):
+    """The single-shot 800 x 300, nt = 4000 configuration at full size:
+    traces against the FP64 oracle."""
+    case = configs.c2()
+    print("forward C2", flush=True)
+    d = make_survey(case, gpu_ctx).forward()
+    ref = oracle_forward(oracle_mod, case, case.eps0, case.sig0)
+    err = rel_l2(d, ref)
+    print("C2 trace rel L2", err)
+    assert np.abs(ref).max() > 0 and err <= 1e-4
+
+
+def test_gradient_c1(oracle_mod, gpu_ctx):
+    """The 200 x 100, nt = 1500, 4-shot configuration at full size: the
+    batched exact gradient (default depths, tiles and checkpoint interval)
+    against the FP64 oracle's exact adjoint."""
+    case = configs.c1()
+    print("gradient C1", flush=True)
+    sv = make_survey(case, gpu_ctx)
+    sv.forward(keep=True)
+    rng = np.random.default_rng(11)
+    adj = rng.standard_normal(sv.traces_shape()).astype(np.float32)
+    ge, gs = sv.gradient(adj)
+    fe, fs, _ = oracle_grad(oracle_mod, case, case.eps0, case.sig0, adj, sv.rec_ptr)
+    print("C1 eps", rel_l2(ge, fe), cosine(ge, fe), "sig", rel_l2(gs, fs), cosine(gs, fs))
+    assert rel_l2(ge, fe) <= 1e-3 and cosine(ge, fe) >= 0.9999
+    assert rel_l2(gs, fs) <= 1e-3 and cosine(gs, fs) >= 0.9999
+
+
 @pytest.mark.parametrize("plain", [False, True])
 @pytest.mark.parametrize("air", [0, 3])
 def test_gradient_small(oracle_mod, gpu_ctx, plain, air):
