@@ -134,6 +134,41 @@ def test_gradient_c1(oracle_mod, gpu_ctx):
     assert rel_l2(gs, fs) <= 1e-3 and cosine(gs, fs) >= 0.9999
 
 
+def test_c3_full_size_properties(gpu_ctx):
+    """The 2000 x 600, nt = 8000 workload at full size (two of its shots; the
+    FP64 oracle would take hours): the batched gradient equals the plain
+    path's (one step per launch, whole wavefield stored: an independent
+    code path) to 1e-4 (measured 2.6e-5 for eps and 4.7e-6 for sigma: FP32
+    rounding in two operation orders over 8000 steps), and the gradient is
+    linear in the adjoint source:
+    doubling it doubles the gradient to 1e-5 (measured 1.8e-6 here; bit for
+    bit on the small cases, where every step keeps its values well inside
+    the FP32 range)."""
+    import dataclasses
+    case = configs.c3()
+    case = dataclasses.replace(case, src=case.src[30:32], wavelets=case.wavelets[30:32],
+                               receivers=case.receivers[30:32])
+    print("C3 two shots", flush=True)
+    # a fixed strip width: the first sweep of a tuned survey tries several
+    # adjoint widths, and results differ with the tiling in the last bits
+    sv = make_survey(case, gpu_ctx, L=96)
+    sv.forward(keep=True)
+    rng = np.random.default_rng(8)
+    adj = rng.standard_normal(sv.traces_shape()).astype(np.float32)
+    ge, gs = sv.gradient(adj)
+    sv.forward(keep=True)
+    ge2, gs2 = sv.gradient(2 * adj)
+    print("C3 linearity", rel_l2(ge2, 2 * ge), rel_l2(gs2, 2 * gs), flush=True)
+    assert rel_l2(ge2, 2 * ge) <= 1e-5 and rel_l2(gs2, 2 * gs) <= 1e-5
+    sp = make_survey(case, gpu_ctx, plain=True)
+    sp.forward(keep=True)
+    pe, ps = sp.gradient(adj)
+    print("C3 batched vs plain", rel_l2(ge, pe), rel_l2(gs, ps), flush=True)
+    assert rel_l2(ge, pe) <= 1e-4 and rel_l2(gs, ps) <= 1e-4
+    sp.close()
+    sv.close()
+
+
 @pytest.mark.parametrize("plain", [False, True])
 @pytest.mark.parametrize("air", [0, 3])
 def test_gradient_small(oracle_mod, gpu_ctx, plain, air):
