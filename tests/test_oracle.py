@@ -238,6 +238,158 @@ def test_misfits(O):
     assert abs(fd - e[k]) <= 1e-5 * max(abs(fd), 1e-6)
 
 
+def _envelope_scipy(x):
+    from scipy.signal import hilbert
+    return np.abs(hilbert(x, axis=0))
+
+
+def _misfit_traces(seed, nt=96, nr=3):
+    """Smooth wavelets on a DC floor, so that every envelope stays away from
+    zero and the log and quotient misfits are differentiable at each sample."""
+    rng = np.random.default_rng(seed)
+    t = np.linspace(0, 1, nt)[:, None]
+    d = 0.6 + np.sin(30 * t + rng.uniform(0, 1, (1, nr))) * np.exp(-((t - 0.5) / 0.2) ** 2)
+    do = 0.6 + 0.8 * np.sin(30 * t - 0.3 + rng.uniform(0, 1, (1, nr))) * np.exp(-((t - 0.45) / 0.25) ** 2)
+    return d, do
+
+
+def _central_fd(J, d, cells, h=1e-6):
+    out = []
+    for k in cells:
+        dp, dm = d.copy(), d.copy()
+        dp[k] += h
+        dm[k] -= h
+        out.append((J(dp) - J(dm)) / (2 * h))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("nt", [96, 97])
+def test_env_log_adjoint_is_gradient(O, nt):
+    """w_bozdag.m:47-54 (env_LOG): the adjoint source equals the gradient of
+    J = 1/2 ||log A(d) - log A(d_o)||^2, A = |analytic signal| (w_E.m's
+    env_LOG objective on the envelopes), with A from scipy.signal.hilbert.
+    A flipped sign, a missing 1/A^2, or d and H[d] swapped fail it."""
+    d, do = _misfit_traces(11, nt)
+    logAo = np.log(_envelope_scipy(do))
+
+    def J(x):
+        return 0.5 * ((np.log(_envelope_scipy(x)) - logAo) ** 2).sum()
+
+    e, de, doe = O.w_bozdag(do, d, "env_LOG")
+    np.testing.assert_allclose(de, _envelope_scipy(d), rtol=1e-12)
+    cells = [(0, 0), (5, 1), (nt // 2, 2), (nt // 3, 0), (nt - 1, 1)]
+    fd = _central_fd(J, d, cells)
+    np.testing.assert_allclose(np.array([e[k] for k in cells]), fd, rtol=2e-6,
+                               atol=2e-7 * np.abs(e).max())
+
+
+@pytest.mark.parametrize("nt", [96, 97])
+def test_env_l22_adjoint_is_half_gradient(O, nt):
+    """w_bozdag.m:38-46 (env_L22): the adjoint source is one half of the
+    gradient of J = 1/2 ||A(d)^2 - A(d_o)^2||^2 (d A^2 = 2 (d dd + H[d] H[dd]);
+    the reference keeps no factor 2).  A wrong power of the envelope, a wrong
+    sign of the Hilbert term, or the factor 2 kept fail it."""
+    d, do = _misfit_traces(12, nt)
+    Ao2 = _envelope_scipy(do) ** 2
+
+    def J(x):
+        return 0.5 * ((_envelope_scipy(x) ** 2 - Ao2) ** 2).sum()
+
+    e, _, _ = O.w_bozdag(do, d, "env_L22")
+    cells = [(0, 0), (7, 1), (nt // 2, 2), (nt // 4, 1), (nt - 1, 0)]
+    fd = _central_fd(J, d, cells)
+    np.testing.assert_allclose(2 * np.array([e[k] for k in cells]), fd, rtol=2e-6,
+                               atol=2e-7 * np.abs(e).max())
+    # env_LOG and env_L2 objectives through w_E.m's own transforms
+    A, Ao = _envelope_scipy(d), _envelope_scipy(do)
+    assert np.isclose(O.w_E(Ao, A, obj_fnc="env_L22"), 0.5 * ((A**2 - Ao**2) ** 2).sum(), rtol=1e-12)
+    assert np.isclose(O.w_E(Ao, A, obj_fnc="env_LOG"), 0.5 * ((np.log(A) - np.log(Ao)) ** 2).sum(),
+                      rtol=1e-12)
+
+
+def _hilbert_cot(x):
+    """Discrete Hilbert transform of an even-length periodic sequence as the
+    closed-form circular convolution y_n = sum_m h_(n-m) x_m with
+    h_k = (2/N) cot(pi k / N) for odd k and 0 for even k: the inverse DFT of
+    the multiplier -i sgn(f) (0 at f = 0 and f = N/2), summed in closed form.
+    No FFT and no bin ordering are involved."""
+    N = x.shape[0]
+    assert N % 2 == 0
+    k = np.arange(N)
+    h = np.zeros(N)
+    odd = k % 2 == 1
+    h[odd] = (2.0 / N) / np.tan(np.pi * k[odd] / N)
+    idx = (k[:, None] - k[None, :]) % N
+    return h[idx] @ x
+
+
+def _wrap(p):
+    """angle()'s branch (-pi, pi]."""
+    return np.pi - np.mod(np.pi - p, 2 * np.pi)
+
+
+@pytest.mark.parametrize("case", ["tone", "tone_on_dc"])
+def test_phase_l2_closed_form(O, case):
+    """w_bozdag.m:29-37 (phase_L2) on signals whose analytic signal is known
+    exactly, so that phases and envelopes are closed forms (no Hilbert
+    transform) and the one Hilbert transform left is the cot-kernel
+    convolution above:
+    * tone: d = a cos(w t + q), d_o = b cos(w t + q - delta), whole periods:
+      A = a, phi = wrap(w t + q), phi_o = wrap(w t + q - delta), so the
+      phase difference is delta or delta - 2 pi on the samples where only
+      phi wrapped (angle() is not unwrapped in the reference);
+    * tone on a DC floor: d = c + cos(w t), analytic signal c + e^(i w t),
+      phi = atan2(sin w t, c + cos w t), A^2 = c^2 + 1 + 2 c cos w t, and the
+      same for d_o with (c_o, shift delta): no wrap, a varying envelope.
+    Then e = -psi + H[psi d] with psi = (phi - phi_o) / A^2.  A sign slip in
+    the phase difference or in the combination, A in place of A^2, H[d] in
+    place of d inside the transform, or the opposite Hilbert convention all
+    fail it (checked below on the wrong variants)."""
+    N, k0 = 64, 5
+    n = np.arange(N)[:, None]
+    wt = 2 * np.pi * k0 * n / N
+    if case == "tone":
+        a, b, q, delta = 1.7, 0.6, 0.123, 0.9
+        d = a * np.cos(wt + q)
+        do = b * np.cos(wt + q - delta)
+        phi, phio, A2 = _wrap(wt + q), _wrap(wt + q - delta), np.full_like(wt, a * a)
+    else:
+        c, co, delta = 2.5, 1.8, 0.7
+        d = c + np.cos(wt)
+        do = co + np.cos(wt - delta)
+        phi = np.arctan2(np.sin(wt), c + np.cos(wt))
+        phio = np.arctan2(np.sin(wt - delta), co + np.cos(wt - delta))
+        A2 = c * c + 1 + 2 * c * np.cos(wt)
+    e, ph, pho = O.w_bozdag(do, d, "phase_L2")
+    np.testing.assert_allclose(ph, phi, atol=1e-12)
+    np.testing.assert_allclose(pho, phio, atol=1e-12)
+    psi = (phi - phio) / A2
+    want = -psi + _hilbert_cot(psi * d)
+    np.testing.assert_allclose(e, want, rtol=0, atol=1e-11 * np.abs(want).max())
+    if case == "tone":
+        # the phase difference takes both values delta and delta - 2 pi
+        dphi = phi - phio
+        assert np.any(np.isclose(dphi, delta)) and np.any(np.isclose(dphi, delta - 2 * np.pi))
+    psiA = (phi - phio) / np.sqrt(A2)
+    wrong = [psi - _hilbert_cot(psi * d),        # phase difference flipped
+             -psi - _hilbert_cot(psi * d),       # opposite Hilbert convention / combination sign
+             -psiA + _hilbert_cot(psiA * d),     # A in place of A^2
+             -psi + _hilbert_cot(psi * _hilbert_cot(d))]  # H[d] in place of d
+    for w_ in wrong:
+        assert np.abs(e - w_).max() > 1e-3 * np.abs(want).max()
+
+
+def test_hilbert_cot_kernel_is_the_hilbert_of_the_oracle(O):
+    """The cot-kernel convolution equals imag(hilbert(x)) of the oracle and
+    maps cos to sin (k0 != 0, N/2), so it pins the convention by itself."""
+    N = 64
+    x = np.random.default_rng(3).standard_normal((N, 2))
+    np.testing.assert_allclose(_hilbert_cot(x), np.imag(O.hilbert_analytic(x, axis=0)),
+                               atol=1e-12)
+    t = 2 * np.pi * 3 * np.arange(N) / N
+    np.testing.assert_allclose(_hilbert_cot(np.cos(t)), np.sin(t), atol=1e-12)
+
+
 def test_golden_fixtures(O):
     path = os.path.join(HERE, "golden", "tiny_forward_grad.npz")
     z = np.load(path, allow_pickle=False)
