@@ -25,11 +25,44 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
-BYTES_PER_CELL_UPDATE = 32  # Ez, Hx, Hy read+write + 2 coefficient reads (FP32)
-# step kernels in the numbering of gjw_kernel_stats / gjw_tile: (name, steps per launch)
-KERNELS = [("k_fwd_step", 1), ("k_adj_step", 1), ("k_fwd_step2", 2), ("k_adj_step2", 2),
-           ("k_fwd_step4", 4), ("k_adj_step4", 4), ("k_fwd_step8", 8), ("k_adj_step8", 8),
-           ("k_adj_step6", 6)]
+BYTES_PER_CELL_UPDATE = 32  # single-pass model: Ez, Hx, Hy read+write + 2 coefficient reads (FP32)
+# step kernels in the numbering of gjw_kernel_stats / gjw_tile:
+# (name, steps per launch, kind of launch for the compulsory-byte model)
+KERNELS = [("k_fwd_step", 1, "fwd"), ("k_adj_step", 1, "adj"), ("k_fwd_step2", 2, "fwd"),
+           ("k_adj_step2", 2, "adj"), ("k_fwd_step4", 4, "fwd"), ("k_adj_step4", 4, "adj"),
+           ("k_fwd_step8", 8, "fwd"), ("k_adj_step8", 8, "adj"), ("k_adj_step6", 6, "adj"),
+           ("k_fwd_step8_frames", 8, "fwd_frames")]
+
+
+def compulsory_bytes(kind, S, g, ns, ntr):
+    """HBM bytes one S-step launch over all shots must move at least: every
+    field read and written once (Ez, Hx, Hy on all padded cells, the split
+    fields on the PML cells), the survey's coefficients read once (shared by
+    the shots: (Ceze, Cezhy) per cell, the 36-B table per PML cell), plus
+    what the launch exists to move: the traces it records (forward sweep),
+    the S Ez frames of the domain it stores (recompute sweep), or the S+1
+    frames, the adjoint-source samples and the FP32 correlation partials
+    (read and written) it uses (adjoint)."""
+    NX, NY = g.nx, g.ny
+    cells = (NX + 1) * (NY + 1)
+    pml = (NX - 1) * (NY - 1) - (NX - 1 - 2 * g.pml) * (NY - 1 - 2 * g.pml)
+    dom = g.m * g.n
+    b = (24 * cells + 16 * pml) * ns + 8 * cells + 36 * pml
+    if kind == "fwd":
+        return b + 4 * S * ntr
+    if kind == "fwd_frames":
+        return b + 4 * S * dom * ns
+    return b + 4 * (S + 1) * dom * ns + 16 * dom * ns + 4 * S * ntr
+
+
+def kernel_source_sha16():
+    """Hash of the kernel sources, so a PMC summary can be matched to them."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("gjw_kernels.hip", "gjw_internal.h"):
+        with open(os.path.join(ROOT, "gerjoii_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
 
 
 def log(*a):
@@ -57,6 +90,9 @@ def main():
     ap.add_argument("--fuse", type=int, default=0, help="forward steps per launch: 0 autotuned, 1, 2, 4, 8")
     ap.add_argument("--adj-fuse", type=int, default=0, help="adjoint steps per launch: 0 (six), 4, 6, 8")
     ap.add_argument("--ckpt", type=int, default=0)
+    ap.add_argument("--graph", type=int, default=1, help="1: replay the gradient as captured HIP graphs")
+    ap.add_argument("--no-sqrt", action="store_true",
+                    help="skip the extra gradient timed at a checkpoint interval of about sqrt(nt)")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run three 1-GiB device copies (calibrates PMC byte counters)")
     args = ap.parse_args()
@@ -104,6 +140,8 @@ def main():
         sv.set_option(N.GJW_OPT_ADJ_FUSE, args.adj_fuse)
     if args.ckpt:
         sv.set_option(N.GJW_OPT_CKPT_EVERY, args.ckpt)
+    if args.graph:
+        sv.set_option(N.GJW_OPT_GRAPH, 1)
     # observed data: true model, kept in HBM
     sv.set_model(case.eps_true, case.sig_true)
     d_obs = torch.zeros(sv.traces_shape(), dtype=torch.float32, device=dev)
@@ -117,9 +155,6 @@ def main():
     for w in range(args.warmup):
         sv.fwi_gradient(d_obs, g_eps, g_sig)
         log(f"warmup {w + 1}/{args.warmup} done")
-    sv.set_option(N.GJW_OPT_KTIMING, 16)
-    for kk in range(len(KERNELS)):
-        sv.kernel_stats(kk, reset=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     J = 0.0
@@ -128,44 +163,95 @@ def main():
         log(f"step {k + 1}/{args.steps} J={J:.6e}")
     torch.cuda.synchronize()
     dt_s = time.perf_counter() - t0
+    # kernel durations: one more gradient, launched eagerly (graph replays
+    # record no events) with HIP event pairs around one launch in 16 steps of
+    # each step kernel, on the survey's stream
+    sv.set_option(N.GJW_OPT_GRAPH, 0)
+    sv.set_option(N.GJW_OPT_KTIMING, 16)
+    for kk in range(len(KERNELS)):
+        sv.kernel_stats(kk, reset=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sv.fwi_gradient(d_obs, g_eps, g_sig)
+    torch.cuda.synchronize()
+    ms_eager = (time.perf_counter() - t1) * 1e3
     sv.set_option(N.GJW_OPT_KTIMING, 0)
     kstats = [sv.kernel_stats(kk) for kk in range(len(KERNELS))]
     ms_step = dt_s * 1e3 / args.steps
     shot_grads = ns * args.steps / dt_s
     cell_updates = 3.0 * cells * g.nt * ns * args.steps  # forward, recompute, adjoint
     gcells = cell_updates / dt_s / 1e9
-    ge_b = g_eps.cpu().numpy().T.copy()
-    tile_L = {name: sv.tile(w) for w, (name, _) in enumerate(KERNELS) if w >= 2}
+    tile_L = {name: sv.tile(w) for w, (name, _, _) in enumerate(KERNELS) if w >= 2}
+    K_used = sv.query(N.GJW_Q_CKPT_INTERVAL)
+    depths = (sv.query(N.GJW_Q_FWD_DEPTH), sv.query(N.GJW_Q_ADJ_DEPTH))
+    ntr = sv.ntraces
+    # the same gradient at BASELINE's checkpoint interval, about sqrt(nt) steps
+    # (rounded up to whole launches of both depths)
+    sqrt_run = None
+    if not args.no_sqrt and not args.ckpt:
+        q = int(np.lcm(*depths))
+        Ks = int(np.ceil(np.sqrt(g.nt) / q) * q)
+        sv.set_option(N.GJW_OPT_CKPT_EVERY, Ks)
+        sv.set_option(N.GJW_OPT_GRAPH, 1 if args.graph else 0)
+        sv.fwi_gradient(d_obs, g_eps, g_sig)  # new buffers and (graph) capture
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sv.fwi_gradient(d_obs, g_eps, g_sig)
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t1
+        sqrt_run = {"checkpoint_interval": Ks, "s": ts, "shot_gradients_per_s": ns / ts,
+                    "note": "one gradient after one untimed call, same survey and options"}
+        sv.set_option(N.GJW_OPT_CKPT_EVERY, 0)
+        log(f"sqrt(nt) interval {Ks}: {ns / ts:.3f} shot-gradients/s")
     sv.close()
     del sv
     torch.cuda.empty_cache()
 
-    # per-kernel roofline: algorithmic bytes per launch / live average duration
-    bytes_launch = BYTES_PER_CELL_UPDATE * cells * ns
-    # launches per gradient: the sweeps run the deepest fused kernels the
-    # autotuner chose; the shallower ones only take segment tails
+    # Per-kernel roofline.  achieved = compulsory bytes of one launch (all
+    # shots; compulsory_bytes) / live average launch duration (HIP events on
+    # the survey's stream around one launch in 16 steps).  A fused launch
+    # moves each field once per S steps, so these bytes, not S x 32 B per
+    # cell, are the ones an S-step launch can be held to: frac <= 1.
+    # traffic = measured HBM bytes per launch from the committed rocprofv3
+    # PMC summary of the same kernel (profiles/), used only when that summary
+    # was taken on the current kernel sources.
+    src_sha = kernel_source_sha16()
     kern = {}
-    for (name, spl), (avg_ms, samples, launches) in zip(KERNELS, kstats):
-        kern[name] = {"avg_ms": avg_ms, "samples": samples, "steps_per_launch": spl,
-                      "launches_per_step": launches / args.steps}
-    for k, v in kern.items():
-        spl = v["steps_per_launch"]
-        v["achieved_GBs"] = (spl * bytes_launch / (v["avg_ms"] * 1e-3) / 1e9) if v["avg_ms"] > 0 else None
-        v["share_of_step"] = v["avg_ms"] * v["launches_per_step"] / ms_step if ms_step > 0 else None
-    dom = max(kern, key=lambda k: kern[k]["avg_ms"] * kern[k]["launches_per_step"])
-    traffic = _traffic_from_profiles(dom, cells, ns)
-    for k, v in kern.items():  # measured HBM bytes (committed PMC summary) / live duration
-        t = _traffic_from_profiles(k, cells, ns)
+    for (name, spl, kind), (avg_ms, samples, launches) in zip(KERNELS, kstats):
+        v = {"avg_ms": avg_ms, "samples": samples, "steps_per_launch": spl,
+             "launches_per_step": launches,
+             "bytes_per_launch_compulsory": compulsory_bytes(kind, spl, g, ns, ntr)}
+        v["achieved_GBs"] = (v["bytes_per_launch_compulsory"] / (avg_ms * 1e-3) / 1e9
+                             if avg_ms > 0 else None)
+        v["share_of_step"] = avg_ms * launches / ms_eager if ms_eager > 0 else None
+        t, note = _traffic_from_profiles(name, src_sha)
         v["traffic_bytes"] = t
-        v["hbm_GBs"] = (t / (v["avg_ms"] * 1e-3) / 1e9) if (t and v["avg_ms"] > 0) else None
-    roof = {"kernel": dom, "bound": "hbm", "achieved": kern[dom]["achieved_GBs"],
+        if note:
+            v["traffic_note"] = note
+        v["hbm_GBs"] = (t / (avg_ms * 1e-3) / 1e9) if (t and avg_ms > 0) else None
+        v["traffic_over_compulsory"] = t / v["bytes_per_launch_compulsory"] if t else None
+        kern[name] = v
+    dom = max(kern, key=lambda k: kern[k]["avg_ms"] * kern[k]["launches_per_step"])
+    kd = kern[dom]
+    roof = {"kernel": dom, "bound": "hbm", "achieved": kd["achieved_GBs"],
             "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": (kern[dom]["achieved_GBs"] or 0) / PEAK_HBM_GBS,
-            "traffic": traffic,
-            "hbm_GBs_measured_traffic": kern[dom]["hbm_GBs"],
-            "hbm_frac_measured_traffic": (kern[dom]["hbm_GBs"] or 0) / PEAK_HBM_GBS,
-            "bytes_per_launch_algorithmic": bytes_launch * kern[dom]["steps_per_launch"],
+            "frac": (kd["achieved_GBs"] or 0) / PEAK_HBM_GBS,
+            "traffic": kd["traffic_bytes"],
+            "hbm_GBs_measured_traffic": kd["hbm_GBs"],
+            "hbm_frac_measured_traffic": (kd["hbm_GBs"] or 0) / PEAK_HBM_GBS if kd["hbm_GBs"] else None,
+            "bytes_per_launch_compulsory": kd["bytes_per_launch_compulsory"],
+            "single_pass_model": {
+                "bytes_per_cell_update": BYTES_PER_CELL_UPDATE,
+                "GBs_equivalent": cell_updates * BYTES_PER_CELL_UPDATE / dt_s / 1e9,
+                "note": "32 B per cell-update as if every step read and wrote HBM; the fused "
+                        "kernels do not, so this figure may exceed the peak and is not a "
+                        "roofline fraction"},
+            "kernel_source_sha16": src_sha,
             "kernels": kern}
+    # the whole step's compulsory bytes against its wall time (must stay below the peak)
+    step_bytes = sum(v["bytes_per_launch_compulsory"] * v["launches_per_step"] for v in kern.values())
+    roof["step_compulsory_GBs"] = step_bytes / (ms_step * 1e-3) / 1e9
+    roof["instrumented_gradient_ms"] = ms_eager
 
     # plain single-shot path on a subset of the same shots
     plain = None
@@ -216,12 +302,23 @@ def main():
         "config": {"workload": "batched FWI gradient, 2000x600 model, 40-cell PML, nt=%d, %d shots, "
                                "400 receivers/shot, checkpoint + recompute, FP64 accumulation" % (g.nt, ns),
                    "grid_cells": cells, "nt": g.nt, "shots": ns, "tile_L": tile_L,
+                   "checkpoint_interval": K_used,
+                   "checkpoint_rule": "about 2 sqrt(nt) in whole launches (sqrt(nt) run beside: "
+                                      "checkpoint_sqrt_nt)" if not args.ckpt else "set by --ckpt",
+                   "steps_per_launch": {"forward": depths[0], "adjoint": depths[1]},
+                   "launches_per_gradient": {k: v["launches_per_step"] for k, v in kern.items()
+                                             if v["launches_per_step"] > 0},
+                   "hip_graph": bool(args.graph),
+                   "launch_scheme": "one launch per fused step group for all shots (not one "
+                                    "persistent kernel); the sequence replayed as captured "
+                                    "HIP graphs" if args.graph else "eager launches",
                    "parallelism": "single GPU, all shots in each launch"},
         "gcell_updates_per_s": gcells,
         "misfit_J": J,
         "plain": plain,
         "batched_vs_plain": (shot_grads / plain["shot_gradients_per_s"]) if plain else None,
         "roofline": roof,
+        "checkpoint_sqrt_nt": sqrt_run,
         "cpu_baseline": cpu,
         "adjoint_source_filter": cond,
         "line_search": lsearch,
@@ -575,20 +672,26 @@ def _conditioning(ctx, d_obs, dt, no_cpu):
     return out
 
 
-def _traffic_from_profiles(kernel, cells, ns):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if one
-    exists for this kernel and workload (profiles/<kernel>_C3_pmc.txt, line
-    'bytes_per_launch: <float>')."""
+def _traffic_from_profiles(kernel, src_sha):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary of this
+    kernel at this workload (profiles/<kernel>_C3_pmc.txt, lines
+    'bytes_per_launch: <float>' and 'kernel_source_sha16: <hex>').  Returns
+    (bytes or None, note): a summary taken on other kernel sources is not used."""
     p = os.path.join(ROOT, "profiles", f"{kernel}_C3_pmc.txt")
     if not os.path.exists(p):
-        return None
+        return None, None
+    val, sha = None, None
     for line in open(p):
         if line.startswith("bytes_per_launch:"):
             try:
-                return float(line.split(":")[1])
+                val = float(line.split(":")[1])
             except ValueError:
-                return None
-    return None
+                val = None
+        elif line.startswith("kernel_source_sha16:"):
+            sha = line.split(":")[1].strip()
+    if sha != src_sha:
+        return None, f"{os.path.basename(p)} was taken on other kernel sources ({sha})"
+    return val, None
 
 
 def _cpu_baseline(case, args):
@@ -611,8 +714,18 @@ def _cpu_baseline(case, args):
     per_shot_full = dt_s / q * (g.nt / nt_s)
     cells = g.cells()
     log(f"cpu baseline: {q} shots x {nt_s} steps in {dt_s:.1f} s on {thr} threads")
+    cpu_model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": 1.0 / per_shot_full,
             "unit": "shot-gradients/s", "cores": thr, "kind": "port",
+            "cpu_model": cpu_model, "nproc": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)),
             "gcell_updates_per_s": 3.0 * cells * nt_s * q / dt_s / 1e9,
             "sample": f"{q} shots of the workload, first {nt_s} of {g.nt} steps (forward, checkpointed "
                       f"recompute, exact adjoint), float oracle, extrapolated linearly in nt"}

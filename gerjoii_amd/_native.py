@@ -26,6 +26,7 @@ GJW_MISFIT_L2 = 0
 GJW_MISFIT_ENV_L2, GJW_MISFIT_PHASE_L2, GJW_MISFIT_ENV_L22, GJW_MISFIT_ENV_LOG = 1, 2, 3, 4
 MISFIT_KINDS = {"L2": 0, "FREQstp": 0, "env_L2": 1, "phase_L2": 2, "env_L22": 3, "env_LOG": 4}
 GJW_GRAD_EXACT, GJW_GRAD_REF = 0, 1
+GJW_Q_CKPT_INTERVAL, GJW_Q_FWD_DEPTH, GJW_Q_ADJ_DEPTH = 1, 2, 3
 
 # every symbol include/gjw.h declares
 EXPORTS = ["gjw_create", "gjw_destroy", "gjw_last_error", "gjw_version", "gjw_survey_create",
@@ -35,7 +36,7 @@ EXPORTS = ["gjw_create", "gjw_destroy", "gjw_last_error", "gjw_version", "gjw_su
            "gjw_batch_create", "gjw_batch_destroy", "gjw_batch_fwi_gradient",
            "gjw_batch_last_timing", "gjw_set_adj_filter", "gjw_filt_gauss", "gjw_lmute",
            "gjw_batch_objective", "gjw_gaussi", "gjw_image_gaussian", "gjw_normali",
-           "gjw_stack_update"]
+           "gjw_stack_update", "gjw_query"]
 
 
 class GjwGrid(C.Structure):
@@ -100,6 +101,7 @@ def load():
     lib.gjw_stack_update.argtypes = [vp, vp, i32, i32, i32, vp, i32, vp, vp]
     lib.gjw_kernel_stats.argtypes = [vp, i32, vp, vp, vp, i32]
     lib.gjw_tile.argtypes = [vp, C.c_int32, vp]
+    lib.gjw_query.argtypes = [vp, i32, vp]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.gjw_last_error.restype = C.c_char_p

@@ -203,7 +203,7 @@ struct GraphCache {
 };
 
 // step kernels timed by GJW_OPT_KTIMING (numbered as in gjw_kernel_stats)
-constexpr int kKernKinds = 9;
+constexpr int kKernKinds = 10;
 
 struct gjw_survey {
   gjw_ctx* ctx = nullptr;
@@ -270,8 +270,8 @@ struct gjw_survey {
   GraphCache g_all, g_fwd, g_adj;  // whole gradient (L2); forward / adjoint sweeps
   std::vector<hipEvent_t> kev_pool;
   std::vector<std::pair<int, int>> kev_used;  // (kernel id, index of start event)
-  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2, k_fwd_step4, k_adj_step4, k_fwd_step8,
-  // k_adj_step8, k_adj_step6
+  // k_fwd_step, k_adj_step, k_fwd_step2, k_adj_step2, k_fwd_step4, k_adj_step4, k_fwd_step8
+  // (forward sweep), k_adj_step8, k_adj_step6, k_fwd_step8 with frames (recompute sweep)
   double kt_ms[kKernKinds] = {};
   long long kt_n[kKernKinds] = {};
   long long kl_n[kKernKinds] = {};  // all launches while timing is on
@@ -857,7 +857,7 @@ static int kev_begin(gjw_survey* s, int which, int it, int* slot) {
   if (s->ksample <= 0) return GJW_OK;
   (void)it;
   // time one launch of this kernel in every ksample steps it covers
-  const int steps = which == 8 ? 6 : which >= 6 ? 8 : which >= 4 ? 4 : which >= 2 ? 2 : 1;
+  const int steps = which == 8 ? 6 : which >= 6 ? 8 : which >= 4 ? 4 : which >= 2 ? 2 : 1;  // 9: 8
   const long long every = std::max(1, s->ksample / steps);
   if (s->kl_n[which]++ % every != 0) return GJW_OK;
   const size_t need = 2 * (s->kev_used.size() + 1);
@@ -915,7 +915,7 @@ static int fwdN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& 
                        FwdBufs& b, int cur, int xc, int it, float* tr, float* fr1,
                        long long fstride) {
   int slot;
-  TRY(kev_begin(s, NS == 8 ? 6 : NS == 4 ? 4 : 2, it, &slot));
+  TRY(kev_begin(s, NS == 8 ? (fr1 ? 9 : 6) : NS == 4 ? 4 : 2, it, &slot));
   const dim3 grid = stepN_grid(s, NS, L, nsh), block(64, 4);
   hipStream_t st = s->ctx->stream;
 #define GJW_FWDN_CALL(K)                                                                     \
@@ -2655,9 +2655,28 @@ extern "C" int gjw_tile(gjw_survey* s, int32_t which, int32_t* L) {
   if (!s || !L || which < 0 || which >= kKernKinds) return fail(GJW_ERR_ARG, "gjw_tile: bad argument");
   const int t[kKernKinds] = {s->tuned_L1, s->tuned_L1, s->tuned_L,  s->tuned_La2,
                              s->tuned_L4, s->tuned_La4, s->tuned_L8, s->tuned_La8,
-                             s->tuned_La6};
+                             s->tuned_La6, s->tuned_L8};
   *L = t[which];
   return GJW_OK;
+}
+
+extern "C" int gjw_query(gjw_survey* s, int32_t what, int64_t* value) {
+  if (!s || !value) return fail(GJW_ERR_ARG, "gjw_query: NULL argument");
+  switch (what) {
+    case GJW_Q_CKPT_INTERVAL:
+      if (!s->have_model || !s->have_shots)
+        return fail(GJW_ERR_STATE, "gjw_query: set the model and the shots first");
+      *value = s->path == GJW_PATH_PLAIN ? 0 : ck_interval(s);
+      return GJW_OK;
+    case GJW_Q_FWD_DEPTH:
+      *value = s->path == GJW_PATH_PLAIN ? 1 : s->fuse_eff;
+      return GJW_OK;
+    case GJW_Q_ADJ_DEPTH:
+      *value = s->path == GJW_PATH_PLAIN ? 1 : s->fuse_eff > 0 ? adj_depth(s) : 0;
+      return GJW_OK;
+    default:
+      return fail(GJW_ERR_ARG, "gjw_query: unknown query %d", what);
+  }
 }
 
 extern "C" int gjw_last_timing(gjw_survey* s, double* ms, double* cells) {

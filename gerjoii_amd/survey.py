@@ -260,6 +260,9 @@ class Survey:
                 "gjw_survey_create")
         self.h = h
         self.rec_ptr = None
+        # bumped by every call that replaces the model, the shots or the kept
+        # forward state (checkpoints); autograd nodes compare it
+        self.generation = 0
         self.ctx._surveys.add(self)
 
     def close(self):
@@ -288,6 +291,7 @@ class Survey:
         e = np.asfortranarray(eps_r, dtype=np.float64)
         s = np.asfortranarray(sigma, dtype=np.float64)
         assert e.shape == (self.grid.m, self.grid.n) and s.shape == e.shape
+        self.generation += 1
         N.check(N.load().gjw_set_model(self.h, N.ptr(e), N.ptr(s)), "gjw_set_model")
 
     def set_shots(self, src, wavelets, receivers):
@@ -301,6 +305,7 @@ class Survey:
         rp = np.zeros(ns + 1, dtype=np.int64)
         rp[1:] = np.cumsum([len(r) for r in recs])
         rl = np.ascontiguousarray(np.concatenate(recs) if rp[-1] else np.zeros(1, np.int32))
+        self.generation += 1
         N.check(N.load().gjw_set_shots(self.h, ns, N.ptr(src), N.ptr(wv), N.ptr(rp), N.ptr(rl)),
                 "gjw_set_shots")
         self.rec_ptr = rp
@@ -326,6 +331,7 @@ class Survey:
             jz = np.zeros((len(self.rec_ptr) - 1 if self.rec_ptr is not None else 0, self.grid.nt),
                           dtype=np.float32)
         _sync_torch(out, jz)
+        self.generation += 1
         N.check(N.load().gjw_forward(self.h, N.ptr(out), N.ptr(jz) if jz is not None else None,
                                      int(bool(keep))), "gjw_forward")
         return out if jz is None else (out, jz)
@@ -403,19 +409,28 @@ class Survey:
             d_obs = np.ascontiguousarray(d_obs, dtype=np.float32)
         _sync_torch(d_obs, g_eps, g_sig)
         J = C.c_double(0)
+        self.generation += 1
         N.check(N.load().gjw_fwi_gradient(self.h, N.ptr(d_obs), N.ptr(g_eps), N.ptr(g_sig),
                                           C.byref(J)), "gjw_fwi_gradient")
         return g_eps, g_sig, J.value
 
     def kernel_stats(self, which, reset=False):
-        """(average ms, samples, launches) of the one-step forward (0), one-step
-        adjoint (1), two-step forward (2), two-step adjoint (3), four-step
-        forward (4), four-step adjoint (5) and eight-step forward (6) kernels
-        while GJW_OPT_KTIMING is on."""
+        """(average ms, samples, launches) of a step kernel while
+        GJW_OPT_KTIMING is on: 0/1 one-step forward/adjoint, 2/3 two-step,
+        4/5 four-step, 6 eight-step forward (forward sweep), 7 eight-step
+        adjoint, 8 six-step adjoint, 9 eight-step forward with frames
+        (recompute sweep)."""
         ms, n, nl = C.c_double(0), C.c_int64(0), C.c_int64(0)
         N.check(N.load().gjw_kernel_stats(self.h, int(which), C.byref(ms), C.byref(n), C.byref(nl),
                                           int(reset)), "gjw_kernel_stats")
         return ms.value, n.value, nl.value
+
+    def query(self, what):
+        """Effective setting (gjw_query): N.GJW_Q_CKPT_INTERVAL,
+        N.GJW_Q_FWD_DEPTH or N.GJW_Q_ADJ_DEPTH."""
+        v = C.c_int64(0)
+        N.check(N.load().gjw_query(self.h, int(what), C.byref(v)), "gjw_query")
+        return v.value
 
     def tile(self, which=2):
         """Autotuned columns per wavefront of step kernel `which` (numbered

@@ -262,8 +262,9 @@ int gjw_stack_update(gjw_ctx *ctx, const double *g, int32_t m, int32_t n, int32_
 /* Average device duration (ms) and count of the sampled step kernels, and the
  * number of launches of that kernel, while GJW_OPT_KTIMING is on.  which: 0 = one-step forward, 1 = one-step adjoint,
  * 2 = two-step forward, 3 = two-step adjoint, 4 = four-step forward,
- * 5 = four-step adjoint, 6 = eight-step forward, 7 = eight-step adjoint,
- * 8 = six-step adjoint.  Has no
+ * 5 = four-step adjoint, 6 = eight-step forward (forward sweep), 7 = eight-step
+ * adjoint, 8 = six-step adjoint, 9 = eight-step forward that also stores the
+ * Ez frames (recompute sweep).  Has no
  * counterpart in the reference (measurement only). */
 int gjw_kernel_stats(gjw_survey *s, int32_t which, double *avg_ms, int64_t *count,
                      int64_t *launches, int32_t reset);
@@ -273,6 +274,17 @@ int gjw_kernel_stats(gjw_survey *s, int32_t which, double *avg_ms, int64_t *coun
  * adjoint kernels before the first gradient).  Has no counterpart in the
  * reference (measurement only). */
 int gjw_tile(gjw_survey *s, int32_t which, int32_t *L);
+
+/* Effective settings of a survey: GJW_Q_CKPT_INTERVAL, the checkpoint interval
+ * the batched path uses (the option, else about 2 sqrt(nt) in whole launches,
+ * else the least-memory interval; 0 on the plain path); GJW_Q_FWD_DEPTH and
+ * GJW_Q_ADJ_DEPTH, the forward and adjoint steps per launch (0 before the
+ * first forward has tuned them).  Has no counterpart in the reference (the
+ * MATLAB code stores the whole wavefield and steps one at a time). */
+#define GJW_Q_CKPT_INTERVAL 1
+#define GJW_Q_FWD_DEPTH 2
+#define GJW_Q_ADJ_DEPTH 3
+int gjw_query(gjw_survey *s, int32_t what, int64_t *value);
 
 #ifdef __cplusplus
 }

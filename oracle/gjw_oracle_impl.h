@@ -16,6 +16,25 @@
  *   Ez (NX+1) x (NY+1), Hx (NX+1) x NY, Hy NX x (NY+1), eps/sig NX x NY.
  */
 
+/* Column loops of one step run on OpenMP threads when the grid is large: each
+ * iteration writes its own cells with the same arithmetic as the serial loop,
+ * so results are bit-identical for any thread count.  Inside an enclosing
+ * parallel region (one shot per thread) the loops stay serial. */
+#ifndef ORC_PAR_DEFINED
+#define ORC_PAR_DEFINED
+#define ORC_PAR_CELLS 65536
+#define OMP_COLS _Pragma("omp parallel for schedule(static) if (par)")
+static void orc_par_copy(void *dst, const void *src, size_t bytes, int par) {
+  const size_t chunk = (size_t)1 << 20;
+  const long nb = (long)((bytes + chunk - 1) / chunk);
+#pragma omp parallel for schedule(static) if (par)
+  for (long b = 0; b < nb; ++b) {
+    const size_t o = (size_t)b * chunk, n = o + chunk <= bytes ? chunk : bytes - o;
+    memcpy((char *)dst + o, (const char *)src + o, n);
+  }
+}
+#endif
+
 #define CAT_(a, b) a##b
 #define CAT(a, b) CAT_(a, b)
 #define FN(name) CAT(name, SFX)
@@ -279,37 +298,44 @@ static void FN(step_literal)(const orc_dims *g, const TY(orc_coeffs) * c,
   const size_t le = (size_t)NX + 1; /* Ez and Hx leading dimension */
   const size_t lh = (size_t)NX;     /* Hy leading dimension        */
   REAL *Ez = s->Ez, *Hx = s->Hx, *Hy = s->Hy;
+  const int par = (size_t)(NX + 1) * (NY + 1) >= ORC_PAR_CELLS && !omp_in_parallel();
 #define EZ(i, j) Ez[((i)-1) + (size_t)((j)-1) * le]
 #define HX(i, j) Hx[((i)-1) + (size_t)((j)-1) * le]
 #define HY(i, j) Hy[((i)-1) + (size_t)((j)-1) * lh]
 #define A2(A, a, b, ld) (A)[((a)-1) + (size_t)((b)-1) * (size_t)(ld)]
 #define CC(A, i, j) A2(c->A, i, j, NX)
   /* new_H, w_solve.m:136-148 */
+  OMP_COLS
   for (int j = pjs; j <= pje - 1; ++j)
     for (int i = 1; i <= NX + 1; ++i)
       HX(i, j) = c->Chxh * HX(i, j) + c->Chxez * (EZ(i, j + 1) - EZ(i, j));
+  OMP_COLS
   for (int j = 1; j <= NY + 1; ++j)
     for (int i = pis; i <= pie - 1; ++i)
       HY(i, j) = c->Chyh * HY(i, j) + c->Chyez * (EZ(i + 1, j) - EZ(i, j));
   /* new_H_pml, w_solve.m:154-187 */
+  OMP_COLS
   for (int b = 1; b <= NY - 1; ++b)
     for (int a = 1; a <= P; ++a) {
       int i = a, j = b + 1;
       HY(i, j) = A2(c->Chyh_xn, a, b, P) * HY(i, j) +
                  A2(c->Chyez_xn, a, b, P) * (EZ(i + 1, j) - EZ(i, j));
     }
+  OMP_COLS
   for (int b = 1; b <= NY - 1; ++b)
     for (int a = 1; a <= P; ++a) {
       int i = pie - 1 + a, j = b + 1;
       HY(i, j) = A2(c->Chyh_xp, a, b, P) * HY(i, j) +
                  A2(c->Chyez_xp, a, b, P) * (EZ(i + 1, j) - EZ(i, j));
     }
+  OMP_COLS
   for (int b = 1; b <= P; ++b)
     for (int a = 1; a <= NX - 1; ++a) {
       int i = a + 1, j = b;
       HX(i, j) = A2(c->Chxh_yn, a, b, NX - 1) * HX(i, j) +
                  A2(c->Chxez_yn, a, b, NX - 1) * (EZ(i, j + 1) - EZ(i, j));
     }
+  OMP_COLS
   for (int b = 1; b <= P; ++b)
     for (int a = 1; a <= NX - 1; ++a) {
       int i = a + 1, j = pje - 1 + b;
@@ -317,6 +343,7 @@ static void FN(step_literal)(const orc_dims *g, const TY(orc_coeffs) * c,
                  A2(c->Chxez_yp, a, b, NX - 1) * (EZ(i, j + 1) - EZ(i, j));
     }
   /* new_Ez, w_solve.m:193-207 */
+  OMP_COLS
   for (int j = pjs + 1; j <= pje - 1; ++j)
     for (int i = pis + 1; i <= pie - 1; ++i)
       EZ(i, j) = CC(Ceze, i, j) * EZ(i, j) +
@@ -326,6 +353,7 @@ static void FN(step_literal)(const orc_dims *g, const TY(orc_coeffs) * c,
   if (with_source) EZ(si, sj) = EZ(si, sj) + c->Cezj * wsrc;
   /* new_Ez_pml, w_solve.m:215-268 */
   const int wxy = NY - 1 - 2 * P, hyx = NX - 1 - 2 * P;
+  OMP_COLS
   for (int b = 1; b <= NY - 1; ++b)
     for (int a = 1; a <= P; ++a) {
       int i = a + 1, j = b + 1;
@@ -333,12 +361,14 @@ static void FN(step_literal)(const orc_dims *g, const TY(orc_coeffs) * c,
           A2(c->Cezxe_xn, a, b, P) * A2(s->Ezx_xn, a, b, P) +
           A2(c->Cezxhy_xn, a, b, P) * (HY(i, j) - HY(i - 1, j));
     }
+  OMP_COLS
   for (int b = 1; b <= wxy; ++b)
     for (int a = 1; a <= P; ++a) {
       int i = a + 1, j = pjs + b;
       A2(s->Ezy_xn, a, b, P) = c->Cezye_xn * A2(s->Ezy_xn, a, b, P) +
                                c->Cezyhx_xn * (HX(i, j) - HX(i, j - 1));
     }
+  OMP_COLS
   for (int b = 1; b <= NY - 1; ++b)
     for (int a = 1; a <= P; ++a) {
       int i = pie - 1 + a, j = b + 1;
@@ -346,6 +376,7 @@ static void FN(step_literal)(const orc_dims *g, const TY(orc_coeffs) * c,
           A2(c->Cezxe_xp, a, b, P) * A2(s->Ezx_xp, a, b, P) +
           A2(c->Cezxhy_xp, a, b, P) * (HY(i, j) - HY(i - 1, j));
     }
+  OMP_COLS
   for (int b = 1; b <= wxy; ++b)
     for (int a = 1; a <= P; ++a) {
       int i = pie - 1 + a, j = pjs + b;
@@ -353,6 +384,7 @@ static void FN(step_literal)(const orc_dims *g, const TY(orc_coeffs) * c,
           c->Cezye_xp * A2(s->Ezy_xp, a, b, P) +
           A2(c->Cezyhx_xp, a, b, P) * (HX(i, j) - HX(i, j - 1));
     }
+  OMP_COLS
   for (int b = 1; b <= P; ++b)
     for (int a = 1; a <= hyx; ++a) {
       int i = pis + a, j = b + 1;
@@ -360,6 +392,7 @@ static void FN(step_literal)(const orc_dims *g, const TY(orc_coeffs) * c,
           c->Cezxe_yn * A2(s->Ezx_yn, a, b, hyx) +
           A2(c->Cezxhy_yn, a, b, hyx) * (HY(i, j) - HY(i - 1, j));
     }
+  OMP_COLS
   for (int b = 1; b <= P; ++b)
     for (int a = 1; a <= NX - 1; ++a) {
       int i = a + 1, j = b + 1;
@@ -367,6 +400,7 @@ static void FN(step_literal)(const orc_dims *g, const TY(orc_coeffs) * c,
           A2(c->Cezye_yn, a, b, NX - 1) * A2(s->Ezy_yn, a, b, NX - 1) +
           A2(c->Cezyhx_yn, a, b, NX - 1) * (HX(i, j) - HX(i, j - 1));
     }
+  OMP_COLS
   for (int b = 1; b <= P; ++b)
     for (int a = 1; a <= hyx; ++a) {
       int i = pis + a, j = pje - 1 + b;
@@ -374,6 +408,7 @@ static void FN(step_literal)(const orc_dims *g, const TY(orc_coeffs) * c,
           c->Cezxe_yp * A2(s->Ezx_yp, a, b, hyx) +
           A2(c->Cezxhy_yp, a, b, hyx) * (HY(i, j) - HY(i - 1, j));
     }
+  OMP_COLS
   for (int b = 1; b <= P; ++b)
     for (int a = 1; a <= NX - 1; ++a) {
       int i = a + 1, j = pje - 1 + b;
@@ -382,33 +417,41 @@ static void FN(step_literal)(const orc_dims *g, const TY(orc_coeffs) * c,
           A2(c->Cezyhx_yp, a, b, NX - 1) * (HX(i, j) - HX(i, j - 1));
     }
   /* Ez = Ezx + Ezy, eight regions (w_solve.m:260-267), in that order */
+  OMP_COLS
   for (int j = 2; j <= pjs; ++j)
     for (int i = 2; i <= pis; ++i)
       EZ(i, j) = A2(s->Ezx_xn, i - 1, j - 1, P) +
                  A2(s->Ezy_yn, i - 1, j - 1, NX - 1);
+  OMP_COLS
   for (int j = pje; j <= NY; ++j)
     for (int i = 2; i <= pis; ++i)
       EZ(i, j) = A2(s->Ezx_xn, i - 1, j - 1, P) +
                  A2(s->Ezy_yp, i - 1, j - pje + 1, NX - 1);
+  OMP_COLS
   for (int j = pje; j <= NY; ++j)
     for (int i = pie; i <= NX; ++i)
       EZ(i, j) = A2(s->Ezx_xp, i - pie + 1, j - 1, P) +
                  A2(s->Ezy_yp, i - 1, j - pje + 1, NX - 1);
+  OMP_COLS
   for (int j = 2; j <= pjs; ++j)
     for (int i = pie; i <= NX; ++i)
       EZ(i, j) = A2(s->Ezx_xp, i - pie + 1, j - 1, P) +
                  A2(s->Ezy_yn, i - 1, j - 1, NX - 1);
+  OMP_COLS
   for (int j = 2; j <= pjs; ++j)
     for (int i = pis + 1; i <= pie - 1; ++i)
       EZ(i, j) = A2(s->Ezx_yn, i - pis, j - 1, hyx) +
                  A2(s->Ezy_yn, i - 1, j - 1, NX - 1);
+  OMP_COLS
   for (int j = pje; j <= NY; ++j)
     for (int i = pis + 1; i <= pie - 1; ++i)
       EZ(i, j) = A2(s->Ezx_yp, i - pis, j - pje + 1, hyx) +
                  A2(s->Ezy_yp, i - 1, j - pje + 1, NX - 1);
+  OMP_COLS
   for (int j = pjs + 1; j <= pje - 1; ++j)
     for (int i = 2; i <= pis; ++i)
       EZ(i, j) = A2(s->Ezx_xn, i - 1, j - 1, P) + A2(s->Ezy_xn, i - 1, j - pjs, P);
+  OMP_COLS
   for (int j = pjs + 1; j <= pje - 1; ++j)
     for (int i = pie; i <= NX; ++i)
       EZ(i, j) = A2(s->Ezx_xp, i - pie + 1, j - 1, P) +
@@ -870,6 +913,7 @@ int FN(orc_grad_exact)(const orc_dims *g, const orc_phys *ph, const REAL *eps,
   const size_t nE = le * (NY + 1), nHx = le * NY, nHy = lh * (NY + 1);
   if (si <= 1 || sj <= 1 || si > NX || sj > NY) return -3; /* ring source */
   if (K < 1) K = 1;
+  const int par = (size_t)(NX + 1) * (NY + 1) >= ORC_PAR_CELLS && !omp_in_parallel();
   TY(orc_coeffs) c;
   TY(orc_cellmap) q;
   TY(orc_state) s0;
@@ -896,7 +940,7 @@ int FN(orc_grad_exact)(const orc_dims *g, const orc_phys *ph, const REAL *eps,
   /* forward sweep, keeping checkpoints */
   for (int it = 0; it < nt; ++it) {
     if (it % K == 0) {
-      memcpy(ck + (size_t)(it / K) * SL, s0.buf, SL * sizeof(REAL));
+      orc_par_copy(ck + (size_t)(it / K) * SL, s0.buf, SL * sizeof(REAL), par);
       if (snap) {
         TY(orc_state) cv;
         FN(state_rebase)(&s0, ck + (size_t)(it / K) * SL, &cv);
@@ -911,10 +955,10 @@ int FN(orc_grad_exact)(const orc_dims *g, const orc_phys *ph, const REAL *eps,
   for (int cidx = nck - 1; cidx >= 0; --cidx) {
     const int k0 = cidx * K, k1 = (k0 + K < nt) ? k0 + K : nt;
     /* recompute states k0..k1 into seg */
-    memcpy(seg, ck + (size_t)cidx * SL, SL * sizeof(REAL));
+    orc_par_copy(seg, ck + (size_t)cidx * SL, SL * sizeof(REAL), par);
     for (int k = k0 + 1; k <= k1; ++k) {
-      memcpy(seg + (size_t)(k - k0) * SL, seg + (size_t)(k - 1 - k0) * SL,
-             SL * sizeof(REAL));
+      orc_par_copy(seg + (size_t)(k - k0) * SL, seg + (size_t)(k - 1 - k0) * SL,
+                   SL * sizeof(REAL), par);
       TY(orc_state) sv;
       FN(state_rebase)(&s0, seg + (size_t)(k - k0) * SL, &sv);
       FN(step_literal)(g, &c, &sv, si, sj, wvlet[k - 1], 1);
@@ -937,6 +981,7 @@ int FN(orc_grad_exact)(const orc_dims *g, const orc_phys *ph, const REAL *eps,
         if (g_sig_pad) GS(si, sj) += gcj * q.dCJ_s;
       }
       /* 3. E* pass with the E-coefficient gradient */
+      OMP_COLS
       for (int j = 1; j <= NY + 1; ++j)
         for (int i = 1; i <= NX + 1; ++i) {
           size_t k = IE(i, j);
@@ -974,6 +1019,7 @@ int FN(orc_grad_exact)(const orc_dims *g, const orc_phys *ph, const REAL *eps,
           }
         }
       /* 4. H* pass: lambda_H' = A * lambda_H'(next) + curl^T W */
+      OMP_COLS
       for (int j = 1; j <= NY; ++j)
         for (int i = 1; i <= NX + 1; ++i) {
           size_t k = IHX(i, j);
@@ -984,6 +1030,7 @@ int FN(orc_grad_exact)(const orc_dims *g, const orc_phys *ph, const REAL *eps,
             GE(i, j) += nHx_[k] * ((double)Hxp[k] * q.dAX[k] + dez * q.dBX[k]);
           }
         }
+      OMP_COLS
       for (int j = 1; j <= NY + 1; ++j)
         for (int i = 1; i <= NX; ++i) {
           size_t k = IHY(i, j);
@@ -995,6 +1042,7 @@ int FN(orc_grad_exact)(const orc_dims *g, const orc_phys *ph, const REAL *eps,
           }
         }
       /* 5. E-final: mE(n-1) = (CE mE | 0 | mE) + grad^T (B lambda_H') */
+      OMP_COLS
       for (int j = 1; j <= NY + 1; ++j)
         for (int i = 1; i <= NX + 1; ++i) {
           size_t k = IE(i, j);

@@ -460,6 +460,41 @@ def test_autograd_function(oracle_mod, gpu_ctx):
         assert abs(fd - ge[a, b]) <= 0.05 * abs(ge[a, b])
 
 
+def test_autograd_two_nodes_one_survey(oracle_mod, gpu_ctx):
+    """Two traces() calls on one survey (two models) before either backward:
+    each node's gradient is the gradient of its own model (the first node
+    finds the survey moved on and rebuilds its checkpoints), equal to the
+    FP64 oracle's to 1e-3, and to a fresh survey's gradient bit for bit."""
+    import torch
+    from gerjoii_amd.autograd import traces
+    case = configs.small(nt=240, ns=2)
+    sv = make_survey(case, gpu_ctx, ck=11, L=16)
+    rng = np.random.default_rng(21)
+    m2 = case.eps0 * (1 + 0.05 * rng.standard_normal(case.eps0.shape))
+    e1 = torch.tensor(case.eps0, dtype=torch.float64, requires_grad=True)
+    e2 = torch.tensor(m2, dtype=torch.float64, requires_grad=True)
+    s1 = torch.tensor(case.sig0, dtype=torch.float64, requires_grad=True)
+    s2 = torch.tensor(case.sig0, dtype=torch.float64, requires_grad=True)
+    d1 = traces(e1, s1, sv)
+    d2 = traces(e2, s2, sv)
+    w = torch.tensor(rng.standard_normal(tuple(d1.shape)), dtype=torch.float32, device="cuda")
+    (d1 * w).sum().backward()
+    (d2 * w).sum().backward()
+    adj = w.cpu().numpy()
+    for eps, E, S_ in ((e1, case.eps0, s1), (e2, m2, s2)):
+        fe, fs, _ = oracle_grad(oracle_mod, case, E, case.sig0, adj, sv.rec_ptr)
+        print("two nodes", rel_l2(eps.grad.numpy(), fe), rel_l2(S_.grad.numpy(), fs))
+        assert rel_l2(eps.grad.numpy(), fe) <= 1e-3 and rel_l2(S_.grad.numpy(), fs) <= 1e-3
+        fresh = make_survey(case, gpu_ctx, ck=11, L=16)
+        fresh.set_model(E, case.sig0)
+        fresh.forward(keep=True)
+        ge, gs = fresh.gradient(adj)
+        np.testing.assert_array_equal(eps.grad.numpy(), ge)
+        np.testing.assert_array_equal(S_.grad.numpy(), gs)
+        fresh.close()
+    sv.close()
+
+
 @pytest.mark.parametrize("fuse,ck,adj_fuse", [(2, 7, 0), (2, 10, 0), (4, 7, 4), (4, 10, 4), (4, 10, 0),
                                               (8, 10, 8), (8, 21, 8), (8, 21, 6), (8, 10, 6)])
 def test_two_step_kernel_matches_one_step(gpu_ctx, ck, fuse, adj_fuse):
