@@ -36,7 +36,7 @@ EXPORTS = ["gjw_create", "gjw_destroy", "gjw_last_error", "gjw_version", "gjw_su
            "gjw_batch_create", "gjw_batch_destroy", "gjw_batch_fwi_gradient",
            "gjw_batch_last_timing", "gjw_set_adj_filter", "gjw_filt_gauss", "gjw_lmute",
            "gjw_batch_objective", "gjw_gaussi", "gjw_image_gaussian", "gjw_normali",
-           "gjw_stack_update", "gjw_query"]
+           "gjw_stack_update", "gjw_query", "gjw_w_bozdag"]
 
 
 class GjwGrid(C.Structure):
@@ -102,6 +102,7 @@ def load():
     lib.gjw_kernel_stats.argtypes = [vp, i32, vp, vp, vp, i32]
     lib.gjw_tile.argtypes = [vp, C.c_int32, vp]
     lib.gjw_query.argtypes = [vp, i32, vp]
+    lib.gjw_w_bozdag.argtypes = [vp, vp, vp, i32, i32, i32, vp, vp, vp, vp]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.gjw_last_error.restype = C.c_char_p

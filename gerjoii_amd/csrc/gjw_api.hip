@@ -163,6 +163,7 @@ struct gjw_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   GaussFilter* flt = nullptr;  // gjw_filt_gauss's plans and buffers
   PostWork* post = nullptr;    // plans and buffers of the gradient post-processing
+  gjw_survey* mws = nullptr;   // gjw_w_bozdag's trace workspace (FFT plans, buffers)
 };
 
 template <class T>
@@ -315,6 +316,7 @@ extern "C" int gjw_destroy(gjw_ctx* c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   filter_free(c->flt);
   post_free(c->post);
+  if (c->mws) gjw_survey_destroy(c->mws);
   delete c;
   return GJW_OK;
 }
@@ -1603,6 +1605,56 @@ extern "C" int gjw_misfit(gjw_survey* s, const float* d_obs, int32_t kind, float
   if (e_out)
     HIPCHK(hipMemcpyAsync(e_out, s->adj.p, (size_t)s->ntr * s->gr.nt * 4, hipMemcpyDefault, st));
   if (J) HIPCHK(hipMemcpyAsync(J, s->Jd.p, 8, hipMemcpyDefault, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return GJW_OK;
+}
+
+// [e, d_e, d_oe] = w_bozdag(d_o, d, kind) and J = w_E for any ntr x nt traces,
+// with no survey: the context keeps a trace workspace (plans, buffers) that
+// the misfit kernels of the surveys use in the same way.
+extern "C" int gjw_w_bozdag(gjw_ctx* c, const float* d_o, const float* d, int32_t ntr,
+                            int32_t nt, int32_t kind, float* e, float* d_e, float* d_oe,
+                            double* J) {
+  if (!c || !d_o || !d) return fail(GJW_ERR_ARG, "gjw_w_bozdag: NULL argument");
+  if (ntr < 1 || nt < 3) return fail(GJW_ERR_ARG, "gjw_w_bozdag: bad sizes ntr=%d nt=%d", ntr, nt);
+  if (kind < GJW_MISFIT_L2 || kind > GJW_MISFIT_ENV_LOG)
+    return fail(GJW_ERR_ARG, "gjw_w_bozdag: unknown kind %d", kind);
+  HIPCHK(hipSetDevice(c->device));
+  if (!c->mws) {
+    c->mws = new gjw_survey;
+    c->mws->ctx = c;
+  }
+  gjw_survey* w = c->mws;
+  w->gr.nt = nt;
+  w->ntr = ntr;
+  hipStream_t st = c->stream;
+  const long long n = (long long)ntr * nt;
+  TRY(w->tr.ensure(n));
+  TRY(w->obs.ensure(n));
+  TRY(w->adj.ensure(n));
+  TRY(w->part.ensure(1024));
+  TRY(w->Jd.ensure(1));
+  TRY(host_or_dev_copy(w->obs.p, d_o, n * 4, st));
+  TRY(host_or_dev_copy(w->tr.p, d, n * 4, st));
+  TRY(misfit_enqueue(w, kind));
+  if (e) HIPCHK(hipMemcpyAsync(e, w->adj.p, n * 4, hipMemcpyDefault, st));
+  if (J) HIPCHK(hipMemcpyAsync(J, w->Jd.p, 8, hipMemcpyDefault, st));
+  if (kind == GJW_MISFIT_L2) {  // w_bozdag.m:14-15: d_e = d, d_oe = d_o
+    if (d_e) HIPCHK(hipMemcpyAsync(d_e, w->tr.p, n * 4, hipMemcpyDefault, st));
+    if (d_oe) HIPCHK(hipMemcpyAsync(d_oe, w->obs.p, n * 4, hipMemcpyDefault, st));
+  } else {
+    // the misfit left the envelope (phase for phase_L2) of d_o in fov
+    if (d_oe) HIPCHK(hipMemcpyAsync(d_oe, w->fov.p, n * 4, hipMemcpyDefault, st));
+    if (d_e) {
+      HIPCHK(hipMemcpyAsync(w->fin.p, w->tr.p, n * 4, hipMemcpyDeviceToDevice, st));
+      TRY(hilbert_dev(w));
+      const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(n, 256), 1), 1024);
+      hipLaunchKernelGGL(k_obs_instant, dim3(blocks), dim3(256), 0, st, w->fana.p, n, nt, kind,
+                         w->fov.p);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(d_e, w->fov.p, n * 4, hipMemcpyDefault, st));
+    }
+  }
   HIPCHK(hipStreamSynchronize(st));
   return GJW_OK;
 }

@@ -158,6 +158,27 @@ class Context:
         return e
 
 
+    def w_bozdag(self, d_o, d, kind="L2"):
+        """w_bozdag.m + w_E.m on the GPU for nt x nr traces (numpy, any
+        layout): returns (e, d_e, d_oe, J), each array nt x nr float32
+        (F-order): e the adjoint source, d_e / d_oe the synthetic and
+        observed traces (L2), envelopes (env_*) or instantaneous phases
+        (phase_L2)."""
+        d_o = np.asfortranarray(d_o, dtype=np.float32)
+        d = np.asfortranarray(d, dtype=np.float32)
+        if d.ndim == 1:
+            d, d_o = d[:, None], d_o[:, None]
+        assert d.shape == d_o.shape
+        nt, nr = d.shape
+        e = np.empty_like(d, order="F")
+        de = np.empty_like(d, order="F")
+        doe = np.empty_like(d, order="F")
+        J = C.c_double(0)
+        N.check(N.load().gjw_w_bozdag(self.h, N.ptr(d_o), N.ptr(d), nr, nt, N.MISFIT_KINDS[kind],
+                                      N.ptr(e), N.ptr(de), N.ptr(doe), C.byref(J)),
+                "gjw_w_bozdag")
+        return e, de, doe, J.value
+
     # -- gradient post-processing (w_update_e.m:108-145) ------------------
     @staticmethod
     def _images(a):

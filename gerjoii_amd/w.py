@@ -12,14 +12,19 @@ code does:
     v = differentiate_cube(v, dt)                    # src/shared/w-dc/differentiate_cube.m:1
 
 Fields read: geome_.{dx, w.dt, w.nt}; parame_.w.{epsilon, sigma, eps_0, mu_0,
-air}; finite_.w.{pml_w, pml_d, pml_order, R_0}; gerjoii_.w.{s, M, wvlet_,
-obj_FNC}; parame_.natu.w.d_2d.  Fields written: gerjoii_.w.{d_2d, e_2d, u_2d}.
-``u_2d`` is a handle to the wavefield held on the GPU (the plain path's
-stored domain wavefield); ``w_grad`` accepts that handle, or the string
-'dtu' variant, the way w_update_e.m passes differentiate_cube(u_2d).
-Errors raise ``gerjoii_amd._native.GjwError`` where MATLAB would error().
+air, t}; finite_.w.{pml_w, pml_d, pml_order, R_0}; gerjoii_.w.{s, M, wvlet_,
+obj_FNC, MUTE, v_mute, t_mute, r_real}; parame_.natu.w.d_2d.  Fields written:
+gerjoii_.w.{d_2d, e_2d, u_2d, Jy}.  As in MATLAB, w_fwd returns finite_ and
+gerjoii_ only: the muted observed data it computes stays local.  ``u_2d`` is
+a handle to the wavefield held on the GPU (the plain path's stored domain
+wavefield); ``w_grad`` accepts that handle, or the handle marked by
+differentiate_cube, the way w_update_e.m passes differentiate_cube(u_2d).
+The solve, the mute, the misfits and the gradients run in libgjw; errors
+raise ``gerjoii_amd._native.GjwError`` where MATLAB would error().
 """
 from __future__ import annotations
+
+import warnings
 
 import numpy as np
 
@@ -47,38 +52,86 @@ def _grid(geome_, parame_, finite_):
 
 
 def w_fwd(geome_, parame_, finite_, gerjoii_, keep_wavefield=True):
-    """src/w/w_fwd.m:17-35: pad, coefficients, solve, misfit (L2)."""
+    """src/w/w_fwd.m:17-35: pad, coefficients and solve (w_exp2pml, w_cpml,
+    w_coeff, w_solve), the mute of w_lmute.m when gerjoii_.w.MUTE is
+    'yes_MUTE', then [e_2d, d_2d, ~] = w_bozdag(d_o, d_2d, obj_FNC): d_2d
+    becomes w_bozdag's second output (the envelope or the phase for the
+    Hilbert-based objectives), as in the reference."""
     g = _grid(geome_, parame_, finite_)
     if "dy" in geome_ and float(geome_["dy"]) != g.dx:
         raise N.GjwError("dy must equal dx (w_geom.m:116)")
-    sv = Survey(g, context(0))
+    ctx = context(0)
+    sv = Survey(g, ctx)
     sv.set_model(parame_["w"]["epsilon"], parame_["w"]["sigma"])
     gw = gerjoii_["w"]
     sv.set_shots(np.asarray(gw["s"]).reshape(1, 2), np.asarray(gw["wvlet_"]).reshape(1, -1),
                  [np.asarray(gw["M"]).ravel()])
     sv.set_path(True)
-    d, jz = sv.forward(keep=keep_wavefield, jz=True)
+    d, jz = sv.forward(keep=keep_wavefield, jz=True)   # d: (nr, nt) traces
     gerjoii_ = dict(gerjoii_)
-    gerjoii_["w"] = dict(gw)
-    gerjoii_["w"]["d_2d"] = d.T.copy()                    # nt x nr
-    gerjoii_["w"]["Jy"] = jz[0].copy()                     # source recorder (w_solve.m:104,124)
-    gerjoii_["w"]["u_2d"] = WavefieldHandle(sv) if keep_wavefield else None
-    obj = gw.get("obj_FNC", "L2")
+    gerjoii_["w"] = gw = dict(gw)
+    gw["Jy"] = jz[0].copy()                             # source recorder (w_solve.m:104,124)
+    gw["u_2d"] = WavefieldHandle(sv) if keep_wavefield else None
     d_o = parame_.get("natu", {}).get("w", {}).get("d_2d")
-    if d_o is not None:
-        e, _, _ = w_bozdag(d_o, gerjoii_["w"]["d_2d"], obj)
-        gerjoii_["w"]["e_2d"] = e
+    if d_o is None:
+        gw["d_2d"] = d.T.copy(order="F")                # nt x nr
+        return finite_, gerjoii_
+    d_o = np.ascontiguousarray(np.asarray(d_o, dtype=np.float32).T)  # (nr, nt)
+    if d_o.shape != d.shape:
+        raise N.GjwError(f"parame_.natu.w.d_2d is {d_o.shape[::-1]}, the solve gives {d.shape[::-1]}")
+    if gw.get("MUTE") == "yes_MUTE":
+        # w_lmute.m:14-44 on the device: both gathers muted in place
+        rx = np.asarray(gw["r_real"], dtype=np.float64)
+        rx = rx[:, 0] if rx.ndim == 2 else rx
+        ctx.lmute(d, d_o, np.asarray(parame_["w"]["t"], dtype=np.float64).ravel(), rx,
+                  float(gw["v_mute"]), float(gw["t_mute"]))
+    obj = gw.get("obj_FNC", "L2")
+    gw["obj_FNC"] = obj                                 # w_fwd.m:33
+    e, d_e, _, _ = ctx.w_bozdag(d_o.T, d.T, obj)
+    gw["e_2d"] = e
+    gw["d_2d"] = d_e
     return finite_, gerjoii_
 
 
-def w_grad(u_w, geome_, finite_, gerjoii_, parame_, mode="REF"):
+def ref_growth_exponent(parame_, geome_):
+    """Largest exponent of the per-cell growth that w_grad.m's negated
+    conductivity gives its backward run: each step multiplies by
+    (2 eps eps0 + dt sigma) / (2 eps eps0 - dt sigma) ~ exp(dt sigma / (eps eps0)),
+    so nt steps by exp(sigma T / (eps eps0)), T = nt dt."""
+    eps = np.asarray(parame_["w"]["epsilon"], dtype=np.float64)
+    sig = np.asarray(parame_["w"]["sigma"], dtype=np.float64)
+    T = float(geome_["w"]["nt"]) * float(geome_["w"]["dt"])
+    return float(np.max(sig * T / (eps * float(parame_["w"]["eps_0"]))))
+
+
+# FP32 overflows at e^88.7; a field of order 1e3 amplified by e^60 stays below
+# it with margin
+REF_GROWTH_LIMIT = 60.0
+
+
+def w_grad(u_w, geome_, finite_, gerjoii_, parame_, mode="auto"):
     """src/w/w_grad.m: gradient for one shot.  ``u_w`` is the handle left by
     w_fwd (for the conductivity gradient, w_update_s.m:77) or
     differentiate_cube(handle) (for the permittivity gradient,
-    w_update_e.m:83-98).  mode='REF' reproduces w_grad.m; mode='EXACT' returns
-    the exact discrete gradient instead."""
+    w_update_e.m:83-98).
+
+    mode 'REF' reproduces w_grad.m (negated conductivity, residual written
+    into Ez at the receivers, correlation with u or d_t u) and also returns
+    a_max; mode 'EXACT' returns the exact discrete gradient (the transpose
+    of the forward solve; a_max is None).  mode 'auto' (the default) takes
+    REF while its backward growth exp(sigma T / (eps eps0)) stays within
+    e^60, and EXACT with a warning beyond it, where REF would overflow FP32
+    (the reference's own arrays are single precision)."""
     if not isinstance(u_w, WavefieldHandle):
         raise N.GjwError("w_grad needs the wavefield handle returned in gerjoii_.w.u_2d")
+    if mode == "auto":
+        k = ref_growth_exponent(parame_, geome_)
+        mode = "REF" if k <= REF_GROWTH_LIMIT else "EXACT"
+        if mode == "EXACT":
+            warnings.warn(f"w_grad: the REF backward run would grow as e^{k:.0f} (FP32 overflows "
+                          f"at e^88); using the EXACT gradient", RuntimeWarning, stacklevel=2)
+    if mode not in ("REF", "EXACT"):
+        raise N.GjwError(f"w_grad: unknown mode {mode!r}")
     sv = u_w.survey
     e = np.ascontiguousarray(np.asarray(gerjoii_["w"]["e_2d"], dtype=np.float32).T)
     m, n = sv.grid.m, sv.grid.n
@@ -104,55 +157,18 @@ def differentiate_cube(v, dt):
     return context(0).differentiate_cube(v, dt)
 
 
-def _analytic(x):
-    """MATLAB hilbert along the time axis (columns): FFT of length nt."""
-    x = np.asarray(x, dtype=np.float64)
-    Nn = x.shape[0]
-    X = np.fft.fft(x, axis=0)
-    h = np.zeros(Nn)
-    h[0] = 1
-    if Nn % 2 == 0:
-        h[Nn // 2] = 1
-        h[1:Nn // 2] = 2
-    else:
-        h[1:(Nn + 1) // 2] = 2
-    return np.fft.ifft(X * h.reshape((-1,) + (1,) * (x.ndim - 1)), axis=0)
-
-
 def w_bozdag(d_o, d, obj_fnc="L2"):
-    """src/w/w_bozdag.m:13-55 (L2 on the GPU path; the Hilbert-based misfits
-    evaluated on the host in FP64)."""
-    d_o = np.asarray(d_o, dtype=np.float64)
-    d = np.asarray(d, dtype=np.float64)
-    if obj_fnc in ("L2", "FREQstp"):
-        return (d - d_o).astype(np.float32), d, d_o
-    with np.errstate(divide="ignore", invalid="ignore"):
-        A = _analytic(d)
-        d_e, d_ph, d_H = np.abs(A), np.angle(A), np.imag(A)
-        Ao = _analytic(d_o)
-        if obj_fnc == "env_L2":
-            dedoe = (d_e - np.abs(Ao)) / d_e
-            e = d * dedoe - np.imag(_analytic(dedoe * d_H))
-            e[np.isnan(e)] = 0
-            return e.astype(np.float32), d_e, np.abs(Ao)
-        if obj_fnc == "phase_L2":
-            d_do_ = (d_ph - np.angle(Ao)) / d_e**2
-            e = -d_do_ + np.imag(_analytic(d_do_ * d))
-            return e.astype(np.float32), d_ph, np.angle(Ao)
-        if obj_fnc == "env_L22":
-            dedoe = d_e**2 - np.abs(Ao) ** 2
-            e = d * dedoe - np.imag(_analytic(dedoe * d_H))
-            return e.astype(np.float32), d_e, np.abs(Ao)
-        if obj_fnc == "env_LOG":
-            dedoe = np.real(np.log((np.abs(Ao) / d_e).astype(complex)))
-            e = -dedoe * (d / d_e**2) + np.imag(_analytic(dedoe * (d_H / d_e**2)))
-            e[np.isnan(e)] = 0
-            return e.astype(np.float32), d_e, np.abs(Ao)
-    raise N.GjwError(f"w_bozdag: unknown objective {obj_fnc!r}")
+    """src/w/w_bozdag.m:13-55 on the GPU (gjw_w_bozdag): returns (e_, d_e,
+    d_oe), nt x nr float32."""
+    if obj_fnc not in N.MISFIT_KINDS:
+        raise N.GjwError(f"w_bozdag: unknown objective {obj_fnc!r}")
+    e, de, doe, _ = context(0).w_bozdag(d_o, d, obj_fnc)
+    return e, de, doe
 
 
 def w_E(d_o, d, Nw=None, obj_fnc="L2"):
-    """src/w/w_E.m:7-16: 0.5 e' N e in FP64 (N = identity when None)."""
+    """src/w/w_E.m:7-16: 0.5 e' N e in FP64 (N = identity when None); for
+    env_L22 / env_LOG the inputs are squared / logged first, as w_E.m does."""
     d = np.asarray(d, dtype=np.float64).ravel(order="F")
     d_o = np.asarray(d_o, dtype=np.float64).ravel(order="F")
     if obj_fnc == "env_L22":

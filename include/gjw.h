@@ -145,6 +145,15 @@ int gjw_misfit(gjw_survey *s, const float *d_obs, int32_t kind, float *e_out, do
 int gjw_bozdag(gjw_survey *s, const float *d_obs, const float *d, int32_t kind, float *e_out,
                double *J);
 
+/* [e_, d_e, d_oe] = w_bozdag(d_o, d, obj_fnc) with all three outputs, and
+ * J = w_E(d_o, d, I, obj_fnc) on d_e / d_oe (src/w/w_bozdag.m:1, src/w/w_E.m:1,
+ * src/shared/w-dc/instant_.m:1), for ntr traces of nt samples (trace-major,
+ * MATLAB's nt x nr) with no survey.  d_e / d_oe are d and d_o for L2, the
+ * envelopes (env_*) or instantaneous phases (phase_L2) otherwise.  Any output
+ * may be NULL. */
+int gjw_w_bozdag(gjw_ctx *ctx, const float *d_o, const float *d, int32_t ntr, int32_t nt,
+                 int32_t kind, float *e, float *d_e, float *d_oe, double *J);
+
 /* Gradient of the misfit w.r.t. the model (m x n doubles, FP64 accumulation
  * over time and shots).
  * mode GJW_GRAD_EXACT: the exact discrete adjoint of the forward solve with
