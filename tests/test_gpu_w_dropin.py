@@ -7,7 +7,9 @@ modes.
 Tolerances: traces, envelopes and L2 / envelope adjoint sources relative L2
 <= 1e-4; instantaneous phases and the phase adjoint source on the samples
 whose envelope is above 1e-3 of the gather maximum (below it the phase of
-FP32 traces is rounding noise, in the reference as here) <= 1e-3; gradients
+FP32 traces is rounding noise, in the reference as here) <= 1e-3, and the
+log-envelope adjoint source where both envelopes are above 1e-2 of their
+maxima <= 1e-3; gradients
 relative L2 <= 1e-3 and cosine >= 0.9999."""
 import warnings
 
@@ -58,8 +60,8 @@ def oracle_shot(O, case, q, eps, sig):
     return og, ep, sp, d, jz, u
 
 
-def floor_mask(env):
-    return env > 1e-3 * np.abs(env).max()
+def floor_mask(env, frac=1e-3):
+    return env > frac * np.abs(env).max()
 
 
 def test_w_fwd_l2_traces_recorder_residual(oracle_mod, gpu_ctx):
@@ -98,8 +100,9 @@ def test_w_fwd_mute_and_objectives(oracle_mod, gpu_ctx, obj):
         err_d = rel_l2(gj["w"]["d_2d"][keep], de[keep])
         tol = 1e-3
     elif obj == "env_LOG":
-        keep = floor_mask(np.abs(O.hilbert_analytic(dm, axis=0))) & floor_mask(
-            np.abs(O.hilbert_analytic(dom, axis=0)))
+        # log of both envelopes: the floor is 1e-2 of the maximum
+        keep = floor_mask(np.abs(O.hilbert_analytic(dm, axis=0)), 1e-2) & floor_mask(
+            np.abs(O.hilbert_analytic(dom, axis=0)), 1e-2)
         err_e = rel_l2(gj["w"]["e_2d"][keep], e[keep])
         err_d = rel_l2(gj["w"]["d_2d"], de)
         tol = 1e-3
