@@ -25,6 +25,7 @@ GJW_SNAP_F32, GJW_SNAP_BF16, GJW_SNAP_F16 = 0, 1, 2
 GJW_MISFIT_L2 = 0
 GJW_MISFIT_ENV_L2, GJW_MISFIT_PHASE_L2, GJW_MISFIT_ENV_L22, GJW_MISFIT_ENV_LOG = 1, 2, 3, 4
 MISFIT_KINDS = {"L2": 0, "FREQstp": 0, "env_L2": 1, "phase_L2": 2, "env_L22": 3, "env_LOG": 4}
+GJW_MISFIT_FREQSTP = 5  # line-search objectives: L2 after an extra band filter
 GJW_GRAD_EXACT, GJW_GRAD_REF = 0, 1
 GJW_Q_CKPT_INTERVAL, GJW_Q_FWD_DEPTH, GJW_Q_ADJ_DEPTH = 1, 2, 3
 
@@ -36,7 +37,8 @@ EXPORTS = ["gjw_create", "gjw_destroy", "gjw_last_error", "gjw_version", "gjw_su
            "gjw_batch_create", "gjw_batch_destroy", "gjw_batch_fwi_gradient",
            "gjw_batch_last_timing", "gjw_set_adj_filter", "gjw_filt_gauss", "gjw_lmute",
            "gjw_batch_objective", "gjw_gaussi", "gjw_image_gaussian", "gjw_normali",
-           "gjw_stack_update", "gjw_query", "gjw_w_bozdag"]
+           "gjw_stack_update", "gjw_query", "gjw_w_bozdag", "gjw_batch_objective_multi",
+           "gjw_batch_forward"]
 
 
 class GjwGrid(C.Structure):
@@ -103,6 +105,9 @@ def load():
     lib.gjw_tile.argtypes = [vp, C.c_int32, vp]
     lib.gjw_query.argtypes = [vp, i32, vp]
     lib.gjw_w_bozdag.argtypes = [vp, vp, vp, i32, i32, i32, vp, vp, vp, vp]
+    lib.gjw_batch_objective_multi.argtypes = [vp, vp, i32, vp, vp, i32, C.c_double, C.c_double,
+                                              vp]
+    lib.gjw_batch_forward.argtypes = [vp, vp]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.gjw_last_error.restype = C.c_char_p

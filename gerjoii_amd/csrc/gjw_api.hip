@@ -100,6 +100,7 @@ __global__ void k_reduce_d(const double*, int, double, double*);
 __global__ void k_dcube(float*, long long, int, double);
 __global__ void k_tukey_pad(const float*, int, int, long long, double, float*);
 __global__ void k_sumsq(const float*, long long, double*);
+__global__ void k_obj_terms(const float*, const float*, long long, int, double*, double*);
 __global__ void k_gauss_band(float2*, int, long long, double, double, double, int);
 __global__ void k_trim(const float*, int, int, long long, float*);
 __global__ void k_lmute(float*, float*, float*, int, long long, const double*, const double*, double,
@@ -2604,20 +2605,22 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
 // objective): forward without checkpoints for all surveys in shared launches,
 // the survey's band filter on the synthetic and the observed traces, then
 // J = 0.5 |d - d_o|^2 / |d_o|^2 in FP64.
-extern "C" int gjw_batch_objective(gjw_batch* b, const float* const* d_obs, double* J) {
-  if (!b || !d_obs || !J) return fail(GJW_ERR_ARG, "gjw_batch_objective: NULL argument");
+// Forward sweep of every survey of the batch in shared launches, without
+// checkpoints or wavefield (the line-search forwards w_fwd_(..., 0) of
+// w_objs_e.m / w_pica_e_.m): traces in each survey's s->tr; d_obs[k] (may be
+// NULL) copied to s->obs.  Returns the cell-updates in *cells.
+static int batch_forward_only(gjw_batch* b, const float* const* d_obs, const char* who,
+                              double* cells) {
   const int S = (int)b->sv.size();
   HIPCHK(hipSetDevice(b->ctx->device));
   hipStream_t st = b->ctx->stream;
-  int T = 0, fuse = 8;  // forward only
+  int T = 0, fuse = 8;
   for (int k = 0; k < S; ++k) {
     gjw_survey* s = b->sv[k];
-    TRY(check_ready(s, "gjw_batch_objective"));
-    if (!d_obs[k]) return fail(GJW_ERR_ARG, "gjw_batch_objective: d_obs[%d] is NULL", k);
+    TRY(check_ready(s, who));
+    if (d_obs && !d_obs[k]) return fail(GJW_ERR_ARG, "%s: d_obs[%d] is NULL", who, k);
     if (s->path != GJW_PATH_BATCHED)
-      return fail(GJW_ERR_ARG, "gjw_batch_objective: survey %d is on the plain path", k);
-    if (s->misfit_kind != GJW_MISFIT_L2)
-      return fail(GJW_ERR_ARG, "gjw_batch_objective: survey %d: the L2 misfit only", k);
+      return fail(GJW_ERR_ARG, "%s: survey %d is on the plain path", who, k);
     T = std::max(T, s->gr.nt);
   }
   for (int k = 0; k < S; ++k) {
@@ -2627,12 +2630,12 @@ extern "C" int gjw_batch_objective(gjw_batch* b, const float* const* d_obs, doub
     TRY(s->tr.ensure(std::max<long long>(n, 1)));
     TRY(s->obs.ensure(std::max<long long>(n, 1)));
     TRY(s->adj.ensure(std::max<long long>(n, 1)));
-    TRY(s->part.ensure(1024));
+    TRY(s->part.ensure(2048));
     TRY(s->Jd.ensure(2));
     TRY(zero_fwd_state(s, s->ns));
     TRY(autotune_L(s, s->ns));
     fuse = std::min(fuse, batch_fwd_depth(s));
-    TRY(host_or_dev_copy(s->obs.p, d_obs[k], n * 4, st));
+    if (d_obs) TRY(host_or_dev_copy(s->obs.p, d_obs[k], n * 4, st));
   }
   std::vector<DevSurvey> hs(S);
   for (int k = 0; k < S; ++k) hs[k] = dev_survey(b->sv[k]);
@@ -2649,34 +2652,141 @@ extern "C" int gjw_batch_objective(gjw_batch* b, const float* const* d_obs, doub
   const int rc = batch_fwd_enqueue(b);
   b->keep = true;
   TRY(rc);
-  double cells = 0;
+  *cells = 0;
   for (int k = 0; k < S; ++k) {
     gjw_survey* s = b->sv[k];
-    const long long n = (long long)s->ntr * s->gr.nt;
-    TRY(trace_filter(s, s->tr.p, s->ntr));
-    TRY(trace_filter(s, s->obs.p, s->ntr));
-    const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(n, 256), 1), 1024);
-    hipLaunchKernelGGL(k_resid_l2, dim3(blocks), dim3(256), 0, st, s->tr.p, s->obs.p, s->adj.p,
-                       n, s->part.p);
-    hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, blocks, 0.5, s->Jd.p);
-    hipLaunchKernelGGL(k_sumsq, dim3(blocks), dim3(256), 0, st, s->obs.p, n, s->part.p);
-    hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, blocks, 1.0,
-                       s->Jd.p + 1);
-    HIPCHK(hipGetLastError());
-    cells += (double)(s->NX + 1) * (s->NY + 1) * (double)s->gr.nt * s->ns;
+    *cells += (double)(s->NX + 1) * (s->NY + 1) * (double)s->gr.nt * s->ns;
     s->kept = 0;
   }
+  return GJW_OK;
+}
+
+static int batch_finish_timing(gjw_batch* b, double cells) {
+  hipStream_t st = b->ctx->stream;
   HIPCHK(hipEventRecord(b->ctx->ev1, st));
-  for (int k = 0; k < S; ++k) {
-    double jd[2];
-    HIPCHK(hipMemcpyAsync(jd, b->sv[k]->Jd.p, 16, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    J[k] = jd[1] > 0 ? jd[0] / jd[1] : 0.0;
-  }
+  HIPCHK(hipEventSynchronize(b->ctx->ev1));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
   b->last_ms = ms;
   b->last_cells = cells;
+  return GJW_OK;
+}
+
+// 0.5 sum (T(d) - T(o))^2 and sum o^2 over n samples into Jd[0], Jd[1]
+// (T: w_E.m's transform of kind e_kind)
+static int obj_terms(gjw_survey* s, const float* d, const float* o, long long n, int e_kind,
+                     double out[2]) {
+  hipStream_t st = s->ctx->stream;
+  const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(n, 256), 1), 1024);
+  hipLaunchKernelGGL(k_obj_terms, dim3(blocks), dim3(256), 0, st, d, o, n, e_kind, s->part.p,
+                     s->part.p + 1024);
+  hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p, blocks, 1.0, s->Jd.p);
+  hipLaunchKernelGGL(k_reduce_d, dim3(1), dim3(256), 0, st, s->part.p + 1024, blocks, 1.0,
+                     s->Jd.p + 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->Jd.p, 16, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return GJW_OK;
+}
+
+// w_bozdag.m's second and third outputs of the survey's working traces:
+// envelopes (env_*) or instantaneous phases (phase_L2) of s->tr into s->fP
+// and of s->obs into s->fov
+static int instants(gjw_survey* s, int kind) {
+  hipStream_t st = s->ctx->stream;
+  const int nt = s->gr.nt;
+  const long long n = (long long)s->ntr * nt;
+  TRY(s->fin.ensure(n));
+  TRY(s->fov.ensure(n));
+  TRY(s->fP.ensure(n));
+  TRY(s->fana.ensure(n));
+  TRY(s->fhalf.ensure((size_t)s->ntr * (nt / 2 + 1)));
+  const int blocks = (int)std::min<long long>(std::max<long long>(cdiv(n, 256), 1), 1024);
+  float* dst[2] = {s->fov.p, s->fP.p};
+  const float* src[2] = {s->obs.p, s->tr.p};
+  for (int w = 0; w < 2; ++w) {
+    HIPCHK(hipMemcpyAsync(s->fin.p, src[w], n * 4, hipMemcpyDeviceToDevice, st));
+    TRY(hilbert_dev(s));
+    hipLaunchKernelGGL(k_obs_instant, dim3(blocks), dim3(256), 0, st, s->fana.p, n, nt, kind,
+                       dst[w]);
+    HIPCHK(hipGetLastError());
+  }
+  return GJW_OK;
+}
+
+extern "C" int gjw_batch_objective_multi(gjw_batch* b, const float* const* d_obs, int32_t nobj,
+                                         const int32_t* kinds, const double* weights,
+                                         int32_t e_kind, double fstp_lo, double fstp_hi,
+                                         double* J) {
+  const char* who = "gjw_batch_objective_multi";
+  if (!b || !d_obs || !J) return fail(GJW_ERR_ARG, "%s: NULL argument", who);
+  if (nobj < 0 || (nobj > 0 && (!kinds || !weights)))
+    return fail(GJW_ERR_ARG, "%s: nobj kinds and weights expected", who);
+  if (e_kind < GJW_MISFIT_L2 || e_kind > GJW_MISFIT_ENV_LOG)
+    return fail(GJW_ERR_ARG, "%s: unknown e_kind %d", who, e_kind);
+  for (int q = 0; q < nobj; ++q) {
+    if (kinds[q] < GJW_MISFIT_L2 || kinds[q] > GJW_MISFIT_FREQSTP)
+      return fail(GJW_ERR_ARG, "%s: unknown objective kind %d", who, kinds[q]);
+    if (kinds[q] == GJW_MISFIT_FREQSTP && !(fstp_hi > fstp_lo))
+      return fail(GJW_ERR_ARG, "%s: FREQstp needs fstp_hi > fstp_lo", who);
+  }
+  double cells = 0;
+  TRY(batch_forward_only(b, d_obs, who, &cells));
+  for (size_t k = 0; k < b->sv.size(); ++k) {
+    gjw_survey* s = b->sv[k];
+    const long long n = (long long)s->ntr * s->gr.nt;
+    // w_objs_e.m:34-37: the band filter of both trace sets
+    TRY(trace_filter(s, s->tr.p, s->ntr));
+    TRY(trace_filter(s, s->obs.p, s->ntr));
+    double t[2];
+    if (nobj == 0) {  // w_objs_e.m:61-62: one objective, w_E of the traces
+      TRY(obj_terms(s, s->tr.p, s->obs.p, n, e_kind, t));
+      J[k] = t[1] > 0 ? t[0] / t[1] : 0.0;
+      continue;
+    }
+    // w_objs_e.m:41-60: weighted objectives; FREQstp filters both trace sets
+    // again (and keeps them filtered for the objectives after it)
+    double acc = 0;
+    for (int q = 0; q < nobj; ++q) {
+      const int kind = kinds[q];
+      if (kind == GJW_MISFIT_FREQSTP) {
+        if (!s->flt) s->flt = new GaussFilter();
+        TRY(s->flt->run(s->ctx->stream, s->tr.p, (int)s->ntr, s->gr.nt, s->gr.dt, fstp_lo,
+                        fstp_hi, 8));
+        TRY(s->flt->run(s->ctx->stream, s->obs.p, (int)s->ntr, s->gr.nt, s->gr.dt, fstp_lo,
+                        fstp_hi, 8));
+      }
+      if (kind == GJW_MISFIT_L2 || kind == GJW_MISFIT_FREQSTP) {
+        TRY(obj_terms(s, s->tr.p, s->obs.p, n, e_kind, t));
+      } else {
+        TRY(instants(s, kind));
+        TRY(obj_terms(s, s->fP.p, s->fov.p, n, e_kind, t));
+      }
+      acc += weights[q] * (t[1] > 0 ? t[0] / t[1] : 0.0);
+    }
+    J[k] = acc;
+  }
+  return batch_finish_timing(b, cells);
+}
+
+extern "C" int gjw_batch_objective(gjw_batch* b, const float* const* d_obs, double* J) {
+  return gjw_batch_objective_multi(b, d_obs, 0, nullptr, nullptr, GJW_MISFIT_L2, 0, 0, J);
+}
+
+extern "C" int gjw_batch_forward(gjw_batch* b, float* const* traces) {
+  const char* who = "gjw_batch_forward";
+  if (!b || !traces) return fail(GJW_ERR_ARG, "%s: NULL argument", who);
+  for (size_t k = 0; k < b->sv.size(); ++k)
+    if (!traces[k]) return fail(GJW_ERR_ARG, "%s: traces[%d] is NULL", who, (int)k);
+  double cells = 0;
+  TRY(batch_forward_only(b, nullptr, who, &cells));
+  TRY(batch_finish_timing(b, cells));
+  hipStream_t st = b->ctx->stream;
+  for (size_t k = 0; k < b->sv.size(); ++k) {
+    const gjw_survey* s = b->sv[k];
+    HIPCHK(hipMemcpyAsync(traces[k], s->tr.p, (size_t)s->ntr * s->gr.nt * 4, hipMemcpyDefault, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
   return GJW_OK;
 }
 

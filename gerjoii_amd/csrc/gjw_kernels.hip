@@ -2196,6 +2196,44 @@ __global__ void k_sumsq(const float* __restrict__ x, long long n, double* __rest
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// Line-search objective terms (w_E.m:7-16 and the normalisation of
+// w_objs_e.m:56-57) of n samples d (synthetic) and o (observed), each taken
+// through w_E's transform T of kind e_kind (3, env_L22: squares; 4, env_LOG:
+// logarithms; else the values): per block, 0.5 sum (T(d) - T(o))^2 into
+// pa[block] and sum o^2 into pb[block], FP64.
+__global__ void k_obj_terms(const float* __restrict__ d, const float* __restrict__ o, long long n,
+                            int e_kind, double* __restrict__ pa, double* __restrict__ pb) {
+  double a = 0.0, b = 0.0;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    double x = d[t], y = o[t];
+    b += y * y;
+    if (e_kind == 3) {
+      x = x * x;
+      y = y * y;
+    } else if (e_kind == 4) {
+      x = log(x);
+      y = log(y);
+    }
+    const double r = x - y;
+    a += 0.5 * r * r;
+  }
+  for (int s = 32; s > 0; s >>= 1) {
+    a += __shfl_xor(a, s);
+    b += __shfl_xor(b, s);
+  }
+  __shared__ double ps[2][4];
+  if ((threadIdx.x & 63) == 0) {
+    ps[0][threadIdx.x >> 6] = a;
+    ps[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    pa[blockIdx.x] = (ps[0][0] + ps[0][1]) + (ps[0][2] + ps[0][3]);
+    pb[blockIdx.x] = (ps[1][0] + ps[1][1]) + (ps[1][2] + ps[1][3]);
+  }
+}
+
 __global__ void k_resid_l2(const float* __restrict__ d, const float* __restrict__ dob,
                            float* __restrict__ e, long long n, double* __restrict__ part) {
   __shared__ double sh[256];

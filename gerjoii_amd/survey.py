@@ -526,19 +526,41 @@ class Batch:
         except Exception:
             pass
 
-    def objective(self, d_obs):
-        """Normalised L2 objective of each survey's current model
-        (w_objs_e.m): forward only, in shared launches, traces band-filtered
-        when the survey has a filter.  d_obs: one traces array per survey."""
+    def objective(self, d_obs, objectives=None, obj_fnc="L2", fstp=None):
+        """Line-search objective of each survey's current model (w_objs_e.m):
+        forward only, in shared launches, traces band-filtered when the
+        survey has a filter.  d_obs: one traces array per survey.
+        objectives: None for w_E(d_o, d, obj_fnc) / |d_o|^2, or a list of
+        (obj_FNC name, weight) pairs, gerjoii_.w.obj_FNCs with g_e_weights
+        ('FREQstp' filters with fstp = (f_low, f_high) first); obj_fnc is
+        gerjoii_.w.obj_FNC (w_E's transform)."""
         S = len(self.surveys)
         d_obs = [d if hasattr(d, "data_ptr") else np.ascontiguousarray(d, dtype=np.float32)
                  for d in d_obs]
         _sync_torch(*d_obs)
         J = np.zeros(S, dtype=np.float64)
         P = C.c_void_p * S
-        N.check(N.load().gjw_batch_objective(self.h, P(*[N.ptr(d).value for d in d_obs]),
-                                             N.ptr(J)), "gjw_batch_objective")
+        objectives = list(objectives or [])
+        kinds = np.array([N.GJW_MISFIT_FREQSTP if k == "FREQstp" else N.MISFIT_KINDS[k]
+                          for k, _ in objectives] or [0], dtype=np.int32)
+        weights = np.array([float(w) for _, w in objectives] or [0.0], dtype=np.float64)
+        lo, hi = fstp if fstp is not None else (0.0, 0.0)
+        N.check(N.load().gjw_batch_objective_multi(
+            self.h, P(*[N.ptr(d).value for d in d_obs]), len(objectives), N.ptr(kinds),
+            N.ptr(weights), N.MISFIT_KINDS[obj_fnc], float(lo), float(hi), N.ptr(J)),
+            "gjw_batch_objective_multi")
         return J
+
+    def forward(self, out=None):
+        """Traces of each survey's current model (forward in shared
+        launches, no wavefield): a list of (ntraces, nt) float32 arrays."""
+        if out is None:
+            out = [np.zeros(sv.traces_shape(), dtype=np.float32) for sv in self.surveys]
+        _sync_torch(*out)
+        P = C.c_void_p * len(self.surveys)
+        N.check(N.load().gjw_batch_forward(self.h, P(*[N.ptr(o).value for o in out])),
+                "gjw_batch_forward")
+        return out
 
     def fwi_gradient(self, d_obs, g_eps=None, g_sig=None):
         """d_obs: one traces array per survey.  Returns (g_eps list, g_sig

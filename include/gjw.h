@@ -137,6 +137,7 @@ int gjw_forward(gjw_survey *s, float *traces, float *jz, int32_t keep);
 #define GJW_MISFIT_PHASE_L2 2
 #define GJW_MISFIT_ENV_L22 3
 #define GJW_MISFIT_ENV_LOG 4
+#define GJW_MISFIT_FREQSTP 5 /* line-search objectives only: L2 after an extra band filter */
 int gjw_misfit(gjw_survey *s, const float *d_obs, int32_t kind, float *e_out, double *J);
 
 /* [e, ~, ~] = w_bozdag(d_o, d, obj_fnc) and J = w_E(...) (src/w/w_bozdag.m:1,
@@ -218,6 +219,29 @@ int gjw_batch_fwi_gradient(gjw_batch *b, const float *const *d_obs, double *cons
  * and the observed traces, J[k] = 0.5 |d - d_o|^2 / |d_o|^2 (L2 misfit).
  * Line searches batch their perturbed models as surveys of one batch. */
 int gjw_batch_objective(gjw_batch *b, const float *const *d_obs, double *J);
+
+/* The line-search objective of w_objs_e.m:22-63 in full, for every survey's
+ * current model (forward in shared launches, no wavefield; the survey's band
+ * filter on both trace sets, as w_objs_e.m:34-37):
+ * - nobj = 0 (no gerjoii_.w.obj_FNCs): J[k] = w_E(d_o, d, I, e_kind) / |d_o|^2
+ *   (w_objs_e.m:61-62; w_objs_s.m:37-38 for a conductivity step);
+ * - nobj > 0 (gerjoii_.w.obj_FNCs, w_objs_e.m:41-60): for each objective q in
+ *   order, kind GJW_MISFIT_FREQSTP first band-filters both trace sets again
+ *   with (fstp_lo, fstp_hi) (they stay filtered for the objectives after it);
+ *   [~, d', d_o'] = w_bozdag(d_o, d, kinds[q]) (the traces for L2 / FREQstp,
+ *   envelopes or phases otherwise); J[k] += weights[q] * w_E(d_o', d', I,
+ *   e_kind) / |d_o'|^2 (gerjoii_.w.g_e_weights).
+ * e_kind is gerjoii_.w.obj_FNC, whose w_E transform (squares for env_L22,
+ * logarithms for env_LOG) applies to every term, as in the reference. */
+int gjw_batch_objective_multi(gjw_batch *b, const float *const *d_obs, int32_t nobj,
+                              const int32_t *kinds, const double *weights, int32_t e_kind,
+                              double fstp_lo, double fstp_hi, double *J);
+
+/* Forward of every survey's current model in shared launches, no wavefield
+ * or checkpoints: traces[k] (survey k's traces layout, host or device)
+ * receives its traces.  Replaces the forward runs w_fwd_(..., 0) that
+ * w_pica_e_.m:24 and w_pica_s_.m:24 make for one trial model each. */
+int gjw_batch_forward(gjw_batch *b, float *const *traces);
 
 /* Device time and cell-updates of the last gjw_batch_fwi_gradient or
  * gjw_batch_objective. */

@@ -592,3 +592,39 @@ def stack_update(gs, steps, model=None, neg_rule=True):
     if model is None:
         return d
     return d, model * np.exp(model * d)
+
+
+# ---------------------------------------------------------------------------
+# line-search objective and Pica step on given gathers (w_objs_e.m:34-63,
+# w_objs_s.m:34-38, w_pica_e_.m:25-27, w_pica_s_.m:25-27)
+# ---------------------------------------------------------------------------
+def w_objs_traces(d_o, d, dt, band, objectives=None, obj_fnc="L2", fstp=None):
+    """The objective w_objs_e.m forms after its forward, on nt x nr gathers:
+    the band filter (f_, f__) of both; then, without gerjoii_.w.obj_FNCs,
+    w_E(d_o, d, N, obj_fnc) / |d_o|^2; with them (``objectives``: (name,
+    weight) pairs in order), for each: FREQstp filters both gathers again
+    with ``fstp`` (kept for the later terms), [~, d', d_o'] = w_bozdag(d_o, d,
+    name), and weight * w_E(d_o', d', N, obj_fnc) / |d_o'|^2 is added."""
+    d_o = filt_gauss(d_o, dt, band[0], band[1], 8)
+    d = filt_gauss(d, dt, band[0], band[1], 8)
+    if not objectives:
+        return w_E(d_o, d, None, obj_fnc) / float(d_o.ravel() @ d_o.ravel())
+    E = 0.0
+    for name, wgt in objectives:
+        if name == "FREQstp":
+            d_o = filt_gauss(d_o, dt, fstp[0], fstp[1], 8)
+            d = filt_gauss(d, dt, fstp[0], fstp[1], 8)
+        _, dd, do = w_bozdag(d_o, d, name)
+        E += wgt * w_E(do, dd, None, obj_fnc) / float(do.ravel() @ do.ravel())
+    return E
+
+
+def pica_step(d_new, d_w, e_w, k, kind):
+    """The Pica step of one shot from its trial-model gather d_new, its
+    current gather d_w and residual e_w: kind 'e' (w_pica_e_.m, fwd_bwd = -1)
+    dd = -(d_new - d_w); kind 's' (w_pica_s_.m, fwd_bwd = +1) dd = d_new + d_w
+    as the reference writes it; step = k (dd . e_w) / (dd . dd)."""
+    d_new = np.asarray(d_new, dtype=np.float64).ravel()
+    d_w = np.asarray(d_w, dtype=np.float64).ravel()
+    dd = -(d_new - d_w) if kind == "e" else d_new + d_w
+    return k * float(dd @ np.asarray(e_w, dtype=np.float64).ravel()) / float(dd @ dd)

@@ -1,6 +1,7 @@
-"""Host arithmetic of the parabolic line search (w_stepparabola_e.m,
+"""Host arithmetic of the line searches (w_stepparabola_e.m,
 w_boundstep2.m, w_objs_e.m): closed forms and exhaustive checks on the
-search grids; no GPU."""
+search grids; and the pins of the oracle's objective composition
+(w_objs_e.m:34-63) and Pica step (w_pica_e_.m, w_pica_s_.m); no GPU."""
 import numpy as np
 
 from gerjoii_amd import linesearch as LS
@@ -49,3 +50,52 @@ def test_perturbation_matches_w_objs_e():
     g = np.array([[0.1, -0.2], [0.0, 1.0]])
     np.testing.assert_allclose(LS.perturb_eps(e, g, 0.05), e * np.exp(-0.05 * e * g))
     np.testing.assert_array_equal(LS.perturb_eps(e, g, 0.0), e)
+
+
+def _gathers(seed=3, nt=256, nr=4):
+    rng = np.random.default_rng(seed)
+    t = np.linspace(0, 1, nt)[:, None]
+    d = np.sin(60 * t + rng.uniform(0, 3, (1, nr))) * np.exp(-((t - 0.5) / 0.15) ** 2)
+    do = 0.8 * np.sin(60 * t - 0.5 + rng.uniform(0, 3, (1, nr))) * np.exp(-((t - 0.45) / 0.2) ** 2)
+    return d, do
+
+
+def test_oracle_objective_terms(oracle_mod):
+    """w_objs_e.m:34-63 as the oracle composes it: one L2 term of weight 1 is
+    the single objective; env_L22 under obj_FNC 'env_L22' is
+    1/2 |A^2 - A_o^2|^2 / |A_o|^2 with envelopes from scipy.signal.hilbert;
+    FREQstp is L2 after the second band filter, and that filter stays on for
+    the terms after it; weights combine linearly."""
+    from scipy.signal import hilbert
+    O = oracle_mod
+    d, do = _gathers()
+    dt, band, fstp = 1e-9 / 64, (5e7, 2e9), (2e8, 1.2e9)
+    single = O.w_objs_traces(do, d, dt, band)
+    assert np.isclose(O.w_objs_traces(do, d, dt, band, [("L2", 1.0)]), single, rtol=1e-14)
+    fd, fo = O.filt_gauss(d, dt, *band, 8), O.filt_gauss(do, dt, *band, 8)
+    assert np.isclose(single, 0.5 * ((fd - fo) ** 2).sum() / (fo ** 2).sum(), rtol=1e-12)
+    A, Ao = np.abs(hilbert(fd, axis=0)), np.abs(hilbert(fo, axis=0))
+    want = 0.5 * ((A**2 - Ao**2) ** 2).sum() / (Ao**2).sum()
+    got = O.w_objs_traces(do, d, dt, band, [("env_L22", 1.0)], obj_fnc="env_L22")
+    assert np.isclose(got, want, rtol=1e-10)
+    ffd, ffo = O.filt_gauss(fd, dt, *fstp, 8), O.filt_gauss(fo, dt, *fstp, 8)
+    l2f = 0.5 * ((ffd - ffo) ** 2).sum() / (ffo ** 2).sum()
+    assert np.isclose(O.w_objs_traces(do, d, dt, band, [("FREQstp", 1.0)], fstp=fstp), l2f,
+                      rtol=1e-12)
+    assert np.isclose(O.w_objs_traces(do, d, dt, band, [("FREQstp", 0.0), ("L2", 1.0)], fstp=fstp),
+                      l2f, rtol=1e-12)
+    a = O.w_objs_traces(do, d, dt, band, [("env_L2", 1.0)])
+    mix = O.w_objs_traces(do, d, dt, band, [("env_L2", 0.3), ("L2", 0.7)])
+    assert np.isclose(mix, 0.3 * a + 0.7 * single, rtol=1e-12)
+
+
+def test_oracle_pica_recovers_a_planted_scale(oracle_mod):
+    """When the residual is beta times the Pica direction, the step is
+    k beta exactly: a sign slip in fwd_bwd or a swapped ratio fails."""
+    O = oracle_mod
+    rng = np.random.default_rng(5)
+    d_new, d_w = rng.standard_normal((50, 3)), rng.standard_normal((50, 3))
+    k, beta = 0.37, -1.9
+    assert np.isclose(O.pica_step(d_new, d_w, beta * -(d_new - d_w), k, "e"), k * beta, rtol=1e-13)
+    assert np.isclose(O.pica_step(d_new, d_w, beta * (d_new + d_w), k, "s"), k * beta, rtol=1e-13)
+    assert not np.isclose(O.pica_step(d_new, d_w, beta * (d_new - d_w), k, "e"), k * beta)
