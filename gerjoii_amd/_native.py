@@ -38,7 +38,7 @@ EXPORTS = ["gjw_create", "gjw_destroy", "gjw_last_error", "gjw_version", "gjw_su
            "gjw_batch_last_timing", "gjw_set_adj_filter", "gjw_filt_gauss", "gjw_lmute",
            "gjw_batch_objective", "gjw_gaussi", "gjw_image_gaussian", "gjw_normali",
            "gjw_stack_update", "gjw_query", "gjw_w_bozdag", "gjw_batch_objective_multi",
-           "gjw_batch_forward"]
+           "gjw_batch_forward", "gjw_normali_noalloc"]
 
 
 class GjwGrid(C.Structure):
@@ -108,6 +108,7 @@ def load():
     lib.gjw_batch_objective_multi.argtypes = [vp, vp, i32, vp, vp, i32, C.c_double, C.c_double,
                                               vp]
     lib.gjw_batch_forward.argtypes = [vp, vp]
+    lib.gjw_normali_noalloc.argtypes = [vp, vp, i32, i32, i32, i32, vp]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.gjw_last_error.restype = C.c_char_p

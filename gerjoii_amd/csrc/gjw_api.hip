@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <cmath>
 #include <cstdarg>
@@ -293,6 +294,42 @@ static int host_or_dev_copy(void* dst, const void* src, size_t bytes, hipStream_
 extern "C" const char* gjw_last_error(void) { return g_err.c_str(); }
 extern "C" const char* gjw_version(void) { return "gjw 0.1 (gfx950)"; }
 
+// Device scratch of gjw_normali_noalloc: the per-block partial maxima and NaN
+// counts of k_absmax_part, kNoAllocParts of each (768 KiB), in a file-scope
+// device array so that the entry point allocates nothing.  Each GPU holds its
+// own copy until the process ends; host code uses the address that
+// hipGetSymbolAddress returns on that device (noalloc_scratch), never the
+// array's name.  Every call on a GPU shares that GPU's copy: calls that
+// overlap on one GPU (several host threads or streams) race.
+constexpr int kNoAllocParts = 65536;
+static __device__ __align__(256) unsigned char scratch[(size_t)kNoAllocParts * (8 + 4)];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::once_flag g_scratch_once[kMaxDevices];
+static int g_scratch_rc[kMaxDevices] = {};
+
+// Resolve this device's copy of the scratch array, once per device, with the
+// device current (gjw_create calls it).
+static int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(GJW_ERR_ARG, "device %d beyond the scratch table (%d devices)", device, kMaxDevices);
+  std::call_once(g_scratch_once[device], [device] {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) {
+      g_scratch_ptr[device] = p;
+    } else {
+      (void)hipGetLastError();
+      g_scratch_rc[device] = (int)e;
+    }
+  });
+  if (!g_scratch_ptr[device])
+    return fail(GJW_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                hipGetErrorString((hipError_t)g_scratch_rc[device]));
+  return GJW_OK;
+}
+
 extern "C" int gjw_create(int32_t device, gjw_ctx** out) {
   if (!out) return fail(GJW_ERR_ARG, "gjw_create: out is NULL");
   int n = 0;
@@ -300,6 +337,7 @@ extern "C" int gjw_create(int32_t device, gjw_ctx** out) {
   if (device < 0 || device >= n)
     return fail(GJW_ERR_ARG, "gjw_create: device %d out of range (%d devices)", device, n);
   HIPCHK(hipSetDevice(device));
+  TRY(noalloc_scratch_init(device));
   gjw_ctx* c = new gjw_ctx;
   c->device = device;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -3071,6 +3109,46 @@ extern "C" int gjw_normali(gjw_ctx* c, double* a, int32_t m, int32_t n, int32_t 
   if (shape) TRY(stage_end(c, const_cast<double*>(shape), dsh, (size_t)per * 8, osh, false));
   TRY(stage_end(c, a, dp, bytes, own, ok));
   if (!ok) return fail(GJW_ERR_HIP, "gjw_normali: launch failed");
+  return GJW_OK;
+}
+
+// gjw_normali for device-resident images, allocating nothing: the partials
+// go to this device's scratch array (every slot k_normali reads is written by
+// k_absmax_part of the same call, so nothing needs clearing first).
+extern "C" int gjw_normali_noalloc(gjw_ctx* c, double* a, int32_t m, int32_t n, int32_t count,
+                                   int32_t nan_guard, const double* shape) {
+  if (!c || (count > 0 && !a)) return fail(GJW_ERR_ARG, "gjw_normali_noalloc: NULL argument");
+  if (m < 1 || n < 1 || count < 0) return fail(GJW_ERR_ARG, "gjw_normali_noalloc: bad sizes");
+  if (count == 0) return GJW_OK;
+  const long long per = (long long)m * n;
+  const int P = (int)std::min<long long>(256, std::max<long long>(1, cdiv(per, 256 * 16)));
+  if ((long long)count * P > kNoAllocParts || count > 65535)
+    return fail(GJW_ERR_ARG,
+                "gjw_normali_noalloc: %d images of %lld cells need %lld partials; the scratch "
+                "holds %d (at most %d images of 1M cells or more)",
+                count, per, (long long)count * P, kNoAllocParts, kNoAllocParts / 256);
+  auto on_device = [](const void* p) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    return at.type == hipMemoryTypeDevice;
+  };
+  if (!on_device(a) || (shape && !on_device(shape)))
+    return fail(GJW_ERR_ARG, "gjw_normali_noalloc: the images and the mask must be device memory");
+  HIPCHK(hipSetDevice(c->device));
+  TRY(noalloc_scratch_init(c->device));
+  unsigned char* base = static_cast<unsigned char*>(g_scratch_ptr[c->device]);
+  double* pmax = reinterpret_cast<double*>(base);
+  int* pnan = reinterpret_cast<int*>(base + (size_t)kNoAllocParts * 8);
+  hipLaunchKernelGGL(k_absmax_part, dim3(P, count), dim3(256), 0, c->stream, a, per, pmax, pnan);
+  HIPCHK(hipGetLastError());
+  const unsigned bx = (unsigned)std::min<long long>(cdiv(per, 256), 1024);
+  hipLaunchKernelGGL(k_normali, dim3(bx, count), dim3(256), 0, c->stream, a, per, P, pmax, pnan,
+                     nan_guard ? 1 : 0, shape);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
   return GJW_OK;
 }
 

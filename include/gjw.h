@@ -284,6 +284,18 @@ int gjw_image_gaussian(gjw_ctx *ctx, const double *in, double *out, int32_t m, i
 int gjw_normali(gjw_ctx *ctx, double *a, int32_t m, int32_t n, int32_t count, int32_t nan_guard,
                 const double *shape);
 
+/* gjw_normali for images and mask already in device memory, allocating
+ * nothing: no counterpart in the reference; for callers that cannot allocate
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Its partial maxima live in a static device scratch array of
+ * 768 KiB per GPU (count * P <= 65536 partials, P = min(256,
+ * ceil(m n / 4096)): up to 256 images of a million cells or more per call;
+ * larger calls fail with GJW_ERR_ARG), resolved once per device by
+ * gjw_create.  Host pointers fail with GJW_ERR_ARG.  Calls that overlap on
+ * one GPU share the scratch and race. */
+int gjw_normali_noalloc(gjw_ctx *ctx, double *a, int32_t m, int32_t n, int32_t count,
+                        int32_t nan_guard, const double *shape);
+
 /* Shot reduction and update (src/w/w_update_e.m:177, src/w/w_update_e_.m:38-41,
  * src/w/w_update_s_.m:38, src/w/w_image_e.m:50-51, src/w/w_image_s.m:38-39):
  * dparam = sum_c(-steps[c] * g_c) / count; with neg_rule != 0 all zero when
