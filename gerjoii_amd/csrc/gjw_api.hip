@@ -2524,6 +2524,36 @@ static int batch_adj_enqueue(gjw_batch* b) {
   return GJW_OK;
 }
 
+// Checkpoint interval of a batch that needs the least memory, when the
+// interval K would put checkpoints and frames of all surveys over half of
+// the device: ceil(nt_s / K) checkpoints of c_s bytes plus K + 1 frames of f_s
+// bytes per survey, minimised over multiples of q near sqrt(sum nt c / sum f).
+static int batch_mem_interval(const gjw_batch* b, int K, int q, int T) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return K;
+  }
+  double ntc = 0, fsum = 0;
+  for (const gjw_survey* s : b->sv) {
+    ntc += (double)s->gr.nt * 4.0 * (double)ck_stride(s);
+    fsum += 4.0 * s->ns * (double)s->LD * s->gr.n;
+  }
+  auto bytes = [&](int kk) {
+    double t = 0;
+    for (const gjw_survey* s : b->sv)
+      t += std::ceil((double)s->gr.nt / kk) * 4.0 * (double)ck_stride(s) +
+           (kk + 1) * 4.0 * s->ns * (double)s->LD * s->gr.n;
+    return t;
+  };
+  if (bytes(K) <= 0.5 * (double)tot) return K;
+  const int k0 = std::max(1, (int)std::lround(std::sqrt(ntc / std::max(fsum, 1.0))));
+  int best = K;
+  for (int kk = std::max(q, k0 / q * q); kk <= std::min(T, (k0 / q + 1) * q); kk += q)
+    if (bytes(kk) < bytes(best)) best = kk;
+  return best;
+}
+
 extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
                                       double* const* g_eps, double* const* g_sig, double* J) {
   if (!b || !d_obs) return fail(GJW_ERR_ARG, "gjw_batch_fwi_gradient: NULL argument");
@@ -2543,7 +2573,20 @@ extern "C" int gjw_batch_fwi_gradient(gjw_batch* b, const float* const* d_obs,
     K = std::max(K, s->ck_every);
   }
   const bool user_K = K > 0;
-  if (!user_K) K = std::max(1, (int)std::ceil(2.0 * std::sqrt((double)T)));
+  if (!user_K) {
+    // about 2 sqrt(T) in whole launches of both depths, or, when the
+    // checkpoints and frames of all surveys would pass half of the device,
+    // the interval that needs the least memory (as ck_interval for one survey)
+    int f0 = 8, a0 = 6;
+    for (int k = 0; k < S; ++k) {
+      f0 = std::min(f0, batch_fwd_depth(b->sv[k]));
+      a0 = std::min(a0, batch_adj_depth(b->sv[k]));
+    }
+    const int q = f0 * a0 / std::gcd(f0, a0);
+    K = std::max(1, (int)std::ceil(2.0 * std::sqrt((double)T)));
+    K = std::min(T, (K + q - 1) / q * q);
+    K = batch_mem_interval(b, K, q, T);
+  }
   b->T = T;
   // forward and adjoint depths: the shallowest any survey asks for (the
   // shared adjoint kernels go up to six steps)
