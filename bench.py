@@ -557,6 +557,7 @@ def _matio(case, d_obs, lines=8):
     import torch
     from gerjoii_amd import matio as MIO
     g = case.grid
+    lines = min(lines, len(case.src))
     recs = case.receivers[:lines]
     ntr = sum(len(r) for r in recs)
     tr = d_obs[:ntr].cpu().numpy()
@@ -625,7 +626,7 @@ def _postprocess(ctx, case, no_cpu):
                    "(gaussi R+W, filter R+W, normali R + R+W) against the whole chain's time"}
     if not no_cpu:
         from oracle import oracle as O
-        q = 2
+        q = min(2, ns)
         h = G[:q].cpu().numpy()
         c0 = time.perf_counter()
         imgs = [O.w_update_post(h[k].T, X, Y, s[k], lo[k], ax, az, case.shape) for k in range(q)]

@@ -379,6 +379,15 @@ extern "C" int gjw_survey_create(gjw_ctx* c, const gjw_grid* g, gjw_survey** out
   s->NY = g->n + 2 * g->pml;           // w_geom.m:135
   s->LD = (int)cdiv(s->NX + 1, 64) * 64;
   s->FS = (long long)s->LD * (s->NY + 1);
+  // the multi-step kernels address the 19 FS-float coefficient block (and
+  // every field) with 32-bit byte offsets below 2^31
+  if (19LL * 4 * s->FS >= (1LL << 31)) {
+    const long long FS = s->FS;
+    delete s;
+    return fail(GJW_ERR_ARG,
+                "gjw_survey_create: padded grid of %lld cells per field exceeds the %lld-cell limit",
+                FS, ((1LL << 31) - 1) / 76);
+  }
   s->di0 = s->P + g->air;  // pis + air, 0-based (w_geom.m:168)
   s->dj0 = s->P;           // pjs, 0-based (w_geom.m:169)
   *out = s;
@@ -816,6 +825,7 @@ static DevGrid dev_grid(const gjw_survey* s) {
   g.uc0 = reinterpret_cast<const float4*>(c + 10 * FS);
   g.uc1 = reinterpret_cast<const float4*>(c + 14 * FS);
   g.uc2 = c + 18 * FS;
+  g.coef = c;
   g.di0 = s->di0; g.dj0 = s->dj0; g.m = s->gr.m; g.n = s->gr.n;
   return g;
 }

@@ -32,6 +32,11 @@ struct DevGrid {
   const float4* uc0;
   const float4* uc1;
   const float* uc2;
+  // the block all of the above live in (19 FS floats: cc at 0, cxe 2 FS,
+  // cxh 3 FS, cye 4 FS, cyh 5 FS, hax 6 FS, hbx 7 FS, hay 8 FS, hby 9 FS,
+  // uc0 10 FS, uc1 14 FS, uc2 18 FS); the multi-step kernels address every
+  // coefficient through one buffer descriptor over it
+  const float* coef;
   // stored domain window (w_geom.m:168-172): rows di0..di0+m-1, cols dj0..dj0+n-1
   int di0, dj0, m, n;
 };

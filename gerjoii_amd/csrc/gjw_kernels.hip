@@ -452,6 +452,38 @@ __device__ __forceinline__ float4 bload4(__amdgpu_buffer_rsrc_t r, unsigned off_
                      __uint_as_float(v.w));
 }
 
+// Loads from a survey's coefficient block through one descriptor (DevGrid::coef):
+// soff is the array's byte offset in the block.  It is wave-uniform and rides
+// in the instruction's scalar offset, so the per-lane offset and the
+// out-of-range mask stay those of the array itself; one descriptor instead of
+// one per array keeps the kernels' scalar registers from spilling.
+__device__ __forceinline__ float bloadk(__amdgpu_buffer_rsrc_t r, unsigned soff,
+                                        unsigned off_floats, bool on) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+      r, (int)(on ? off_floats * 4u : kOOB), (int)soff, 0));
+}
+__device__ __forceinline__ float2 bload2k(__amdgpu_buffer_rsrc_t r, unsigned soff,
+                                          unsigned off_floats, bool on) {
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(on ? off_floats * 4u : kOOB),
+                                                       (int)soff, 0);
+  return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+__device__ __forceinline__ float4 bload4k(__amdgpu_buffer_rsrc_t r, unsigned soff,
+                                          unsigned off_floats, bool on) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(on ? off_floats * 4u : kOOB),
+                                                        (int)soff, 0);
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
+                     __uint_as_float(v.w));
+}
+// byte offsets of the arrays in the coefficient block (units of FS floats: see DevGrid)
+struct CoefOff {
+  unsigned cc, cxe, cxh, cyh, hay, hby, u0, u1, u2;
+};
+__device__ __forceinline__ CoefOff coef_off(const DevGrid& g) {
+  const unsigned f = (unsigned)g.FS * 4u;
+  return CoefOff{0u, 2u * f, 3u * f, 5u * f, 8u * f, 9u * f, 10u * f, 14u * f, 18u * f};
+}
+
 // Waves per SIMD the multi-step kernels are compiled for (the compiler keeps
 // their registers within that occupancy; the kernels are bandwidth-bound).
 #ifndef GJW_FWD_WAVES
@@ -560,15 +592,8 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
                                rE2 = rsrc_of(E2, g.FS), rHx2 = rsrc_of(Hx2, g.FS),
                                rHy2 = rsrc_of(Hy2, g.FS), rX2 = rsrc_of(X2, g.FS),
                                rY2 = rsrc_of(Y2, g.FS),
-                               rCC = rsrc_of(const_cast<float*>(reinterpret_cast<const float*>(g.cc)), 2 * g.FS),
-                               rCXE = rsrc_of(const_cast<float*>(g.cxe), g.FS),
-                               rCXH = rsrc_of(const_cast<float*>(g.cxh), g.FS),
-                               rCYH = rsrc_of(const_cast<float*>(g.cyh), g.FS),
-                               rHAY = rsrc_of(const_cast<float*>(g.hay), g.FS),
-                               rHBY = rsrc_of(const_cast<float*>(g.hby), g.FS),
-                               rU0 = rsrc_of(const_cast<float*>(&g.uc0->x), 4 * g.FS),
-                               rU1 = rsrc_of(const_cast<float*>(&g.uc1->x), 4 * g.FS),
-                               rU2 = rsrc_of(const_cast<float*>(g.uc2), g.FS);
+                               rK = rsrc_of(const_cast<float*>(g.coef), 19 * g.FS);
+  const CoefOff ko = coef_off(g);
 
   // row classes (valid in interior columns)
   const bool e_inner = okrow && i >= P + 1 && i <= NX - P - 1;
@@ -588,8 +613,8 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   // (from the coefficient table at column P+1, loaded without a branch)
   const bool rows_on = okrow && !inner_rows;
   const unsigned kr = (unsigned)((P + 1) * LD + ic);
-  const float4 kr0 = bload4(rU0, 4 * kr, rows_on), kr1 = bload4(rU1, 4 * kr, rows_on);
-  const float kr2 = bload(rU2, kr, rows_on);
+  const float4 kr0 = bload4k(rK, ko.u0, 4 * kr, rows_on), kr1 = bload4k(rK, ko.u1, 4 * kr, rows_on);
+  const float kr2 = bloadk(rK, ko.u2, kr, rows_on);
   const float cxe_r = rows_on ? kr0.y : 1.f, cxh_r = rows_on ? kr0.z : 0.f;
   const float cyh_r = rows_on ? kr1.x : 0.f;
   // Hy coefficients of rows whose Hy update does not depend on the column
@@ -661,7 +686,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     const bool ina = okrow && j1 >= 0 && j1 <= NY, inb = okrow && j1 - 1 >= 0 && j1 - 1 <= NY;
     e0_c = bload(rE0, ka, ina);
     const float eb = bload(rE0, kb, inb), hxb = bload(rHx0, kb, inb && j1 - 1 < NY);
-    const float4 kbb = bload4(rU1, 4 * kb, inb);  // (cyh, AX, BX, AY) of column j1-1
+    const float4 kbb = bload4k(rK, ko.u1, 4 * kb, inb);  // (cyh, AX, BX, AY) of column j1-1
     hxp[0] = kbb.y * hxb + kbb.z * (e0_c - eb);
   }
   // hand level l's outputs to level l+1 and shift the coefficient carries
@@ -698,9 +723,9 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       const bool pm = pml_cell(c);
       x_[u] = bload(rX0, kk, pm);
       y_[u] = bload(rY0, kk, pm);
-      ka_[u] = bload4(rU0, 4 * kk, cin);
-      kb_[u] = bload4(rU1, 4 * kk, cin);
-      kc_[u] = bload(rU2, kk, cin);
+      ka_[u] = bload4k(rK, ko.u0, 4 * kk, cin);
+      kb_[u] = bload4k(rK, ko.u1, 4 * kk, cin);
+      kc_[u] = bloadk(rK, ko.u2, kk, cin);
     }
     // table coefficients of the later levels' first columns
     float4 ka_l[NS], kb_l[NS];
@@ -710,9 +735,9 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       const int c = jj - l;
       const bool cin = okrow && c >= 0 && c <= NY;
       const unsigned kk = (unsigned)(c * LD + ic);
-      ka_l[l] = bload4(rU0, 4 * kk, cin);
-      kb_l[l] = bload4(rU1, 4 * kk, cin);
-      kc_l[l] = bload(rU2, kk, cin);
+      ka_l[l] = bload4k(rK, ko.u0, 4 * kk, cin);
+      kb_l[l] = bload4k(rK, ko.u1, 4 * kk, cin);
+      kc_l[l] = bloadk(rK, ko.u2, kk, cin);
     }
 #pragma unroll
     for (int u = 0; u < UC; ++u) {
@@ -795,7 +820,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       b.en[u] = bload(rE0, kk + (unsigned)LD, on);
       b.hx[u] = bload(rHx0, kk, on);
       b.hy[u] = bload(rHy0, kk, on);
-      b.c[u] = bload2(rCC, 2 * kk, on);
+      b.c[u] = bload2k(rK, ko.cc, 2 * kk, on);
     }
   };
   // ---- interior chunk, all rows inner: scalar H coefficients, (Ceze, Cezhy) loads.
@@ -859,14 +884,14 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       en_[u] = bload(rE0, kk + (unsigned)LD, okrow);
       hx_[u] = bload(rHx0, kk, okrow);
       hy_[u] = bload(rHy0, kk, h_any);
-      c_[u] = bload2(rCC, 2 * kk, e_inner);
+      c_[u] = bload2k(rK, ko.cc, 2 * kk, e_inner);
       x_[u] = bload(rX0, kk, e_pml);
       y_[u] = bload(rY0, kk, e_pml);
-      cxe_[u] = e_xp ? bload(rCXE, kk, e_xp) : cxe_r;
-      cxh_[u] = e_xp ? bload(rCXH, kk, e_xp) : cxh_r;
-      cyh_[u] = e_xp ? bload(rCYH, kk, e_xp) : cyh_r;
-      hay_[u] = h_xp ? bload(rHAY, kk, h_xp) : hay_row;
-      hby_[u] = h_xp ? bload(rHBY, kk, h_xp) : hby_row;
+      cxe_[u] = e_xp ? bloadk(rK, ko.cxe, kk, e_xp) : cxe_r;
+      cxh_[u] = e_xp ? bloadk(rK, ko.cxh, kk, e_xp) : cxh_r;
+      cyh_[u] = e_xp ? bloadk(rK, ko.cyh, kk, e_xp) : cyh_r;
+      hay_[u] = h_xp ? bloadk(rK, ko.hay, kk, h_xp) : hay_row;
+      hby_[u] = h_xp ? bloadk(rK, ko.hby, kk, h_xp) : hby_row;
     }
 #pragma unroll
     for (int u = 0; u < UC; ++u) {
@@ -1368,6 +1393,9 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   float* ac = acc ? acc + 2 * (long long)shot * astride : nullptr;
   // residual injection reaches halo lanes too: later levels read their values
   const bool has_rec = adj && r0 <= s.rmax && r0 + 63 >= s.rmin;
+  // the wavelet gradient is recorded only when it is asked for, by the waves
+  // whose rows hold the source row; the other waves issue no stores for it
+  const bool src_rows = gw && si >= r0 && si <= r0 + 63;
   // the rows the owned rows depend on are interior rows (see fwdN_body)
   const bool inner_rows = (r0 + 1 >= P + 1) && (own_hi + NS - 1 <= NX - P - 1);
   const bool dom_row = i - g.di0 >= 0 && i - g.di0 < g.m;
@@ -1383,21 +1411,14 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
                                rM2 = rsrc_of(M2, g.FS), rNx2 = rsrc_of(Nx2, g.FS),
                                rNy2 = rsrc_of(Ny2, g.FS), rX2 = rsrc_of(X2, g.FS),
                                rY2 = rsrc_of(Y2, g.FS),
-                               rCC = rsrc_of(const_cast<float*>(reinterpret_cast<const float*>(g.cc)), 2 * g.FS),
-                               rCXE = rsrc_of(const_cast<float*>(g.cxe), g.FS),
-                               rCXH = rsrc_of(const_cast<float*>(g.cxh), g.FS),
-                               rCYH = rsrc_of(const_cast<float*>(g.cyh), g.FS),
-                               rHAY = rsrc_of(const_cast<float*>(g.hay), g.FS),
-                               rHBY = rsrc_of(const_cast<float*>(g.hby), g.FS),
+                               rK = rsrc_of(const_cast<float*>(g.coef), 19 * g.FS),
                                // frames of S_{it-NS+1} .. S_{it+1}: slot k at k * FD
                                rFR = rsrc_of(const_cast<float*>(Fa - NS * FD + (long long)shot * fstride),
                                              (NS + 1) * FD),
                                rAC = rsrc_of(ac, ac ? 2 * FD : 0),
                                rGW = rsrc_of(gw ? gw + (long long)shot * s.nt : nullptr,
-                                             gw ? s.nt : 0),
-                               rU0 = rsrc_of(const_cast<float*>(&g.uc0->x), 4 * g.FS),
-                               rU1 = rsrc_of(const_cast<float*>(&g.uc1->x), 4 * g.FS),
-                               rU2 = rsrc_of(const_cast<float*>(g.uc2), g.FS);
+                                             gw ? s.nt : 0);
+  const CoefOff ko = coef_off(g);
 
   // row classes (valid in interior columns)
   const bool e_inner = okrow && i >= P + 1 && i <= NX - P - 1;
@@ -1415,8 +1436,8 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   // (from the coefficient table at column P+1, loaded without a branch)
   const bool rows_on = okrow && !inner_rows;
   const unsigned kr = (unsigned)((P + 1) * LD + ic);
-  const float4 kr0 = bload4(rU0, 4 * kr, rows_on), kr1 = bload4(rU1, 4 * kr, rows_on);
-  const float kr2 = bload(rU2, kr, rows_on);
+  const float4 kr0 = bload4k(rK, ko.u0, 4 * kr, rows_on), kr1 = bload4k(rK, ko.u1, 4 * kr, rows_on);
+  const float kr2 = bloadk(rK, ko.u2, kr, rows_on);
   const float cxe_r = rows_on ? kr0.y : 1.f, cxh_r = rows_on ? kr0.z : 0.f;
   const float cyh_r = rows_on ? kr1.x : 0.f;
   // Hy coefficients of rows whose Hy update does not depend on the column
@@ -1481,10 +1502,10 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
     me_in[0] = bload(rM0, ka, ina);
     x_in[0] = bload(rX0, ka, pa);
     y_in[0] = bload(rY0, ka, pa);
-    const float4 a0 = bload4(rU0, 4 * ka, ina), a1 = bload4(rU1, 4 * ka, ina);
+    const float4 a0 = bload4k(rK, ko.u0, 4 * ka, ina), a1 = bload4k(rK, ko.u1, 4 * ka, ina);
     const float mb = bload(rM0, kb, inb), yb = bload(rY0, kb, pb);
     const float nxb = bload(rNx0, kb, inb && j1 - 1 < NY);
-    const float4 b1 = bload4(rU1, 4 * kb, inb);  // (cyh, AX, BX, AY) of column j1-1
+    const float4 b1 = bload4k(rK, ko.u1, 4 * kb, inb);  // (cyh, AX, BX, AY) of column j1-1
     wy_in[0] = a0.z * (x_in[0] + me_in[0]);
     wx_in[0] = a1.x * (y_in[0] + me_in[0]);
     pxp[0] = b1.z * (b1.y * nxb + b1.x * (yb + mb) - wx_in[0]);
@@ -1530,7 +1551,7 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
       t.y = t.y + mz[l] * (f[l] + f[l + 1]);
     }
     bstore2(t, rAC, 2 * (unsigned)((cz - g.dj0) * LD + ic), ac && inner_z && corr_on(cz));
-    if (decltype(SRC)::value) {
+    if (decltype(SRC)::value && src_rows) {  // wave-uniform: a scalar branch
       const bool at_src = oz && i == si && cz == sj;
 #pragma unroll
       for (int l = 0; l < NS; ++l) bstore(sc * mz[l], rGW, (unsigned)(it - l), at_src);
@@ -1578,9 +1599,9 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
       const bool pmn = pml_cell(cn);
       xL[u] = bload(rX0, kn, pmn);
       yL[u] = bload(rY0, kn, pmn);
-      kaL[u] = bload4(rU0, 4 * kn, nin);
-      kbL[u] = bload4(rU1, 4 * kn, nin);
-      kcL[u] = bload(rU2, kn, nin);
+      kaL[u] = bload4k(rK, ko.u0, 4 * kn, nin);
+      kbL[u] = bload4k(rK, ko.u1, 4 * kn, nin);
+      kcL[u] = bloadk(rK, ko.u2, kn, nin);
       nx_[u] = bload(rNx0, kk, okrow && c >= 0 && c < NY);
       ny_[u] = bload(rNy0, kk, okrow && c >= 0 && c <= NY && i < NX);
       const int cz = c - (NS - 1);
@@ -1594,9 +1615,9 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
       const int c = jj - l;
       const bool cin = okrow && c >= 0 && c <= NY;
       const unsigned kk = (unsigned)(c * LD + ic);
-      ka_l[l] = bload4(rU0, 4 * kk, cin);
-      kb_l[l] = bload4(rU1, 4 * kk, cin);
-      kc_l[l] = bload(rU2, kk, cin);
+      ka_l[l] = bload4k(rK, ko.u0, 4 * kk, cin);
+      kb_l[l] = bload4k(rK, ko.u1, 4 * kk, cin);
+      kc_l[l] = bloadk(rK, ko.u2, kk, cin);
     }
 #pragma unroll
     for (int u = 0; u < UA; ++u) {
@@ -1668,7 +1689,7 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
     for (int u = 0; u < UA; ++u) {
       const unsigned kk = (unsigned)((jj + u) * LD + i);
       mL[u] = bload(rM0, kk + (unsigned)LD, true);
-      cL[u] = bload2(rCC, 2 * (kk + (unsigned)LD), true);
+      cL[u] = bload2k(rK, ko.cc, 2 * (kk + (unsigned)LD), true);
       nx_[u] = bload(rNx0, kk, true);
       ny_[u] = bload(rNy0, kk, true);
       const int cz = jj + u - (NS - 1);
@@ -1747,16 +1768,16 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
     for (int u = 0; u < UA; ++u) {
       const unsigned kk = (unsigned)((jj + u) * LD + ic), kn = kk + (unsigned)LD;
       mL[u] = bload(rM0, kn, okrow);
-      cL[u] = bload2(rCC, 2 * kn, e_inner);
+      cL[u] = bload2k(rK, ko.cc, 2 * kn, e_inner);
       xL[u] = bload(rX0, kn, e_pml);
       yL[u] = bload(rY0, kn, e_pml);
-      cxeL[u] = e_xp ? bload(rCXE, kn, e_xp) : cxe_r;
-      cxhL[u] = e_xp ? bload(rCXH, kn, e_xp) : cxh_r;
-      cyhL[u] = e_xp ? bload(rCYH, kn, e_xp) : cyh_r;
+      cxeL[u] = e_xp ? bloadk(rK, ko.cxe, kn, e_xp) : cxe_r;
+      cxhL[u] = e_xp ? bloadk(rK, ko.cxh, kn, e_xp) : cxh_r;
+      cyhL[u] = e_xp ? bloadk(rK, ko.cyh, kn, e_xp) : cyh_r;
       nx_[u] = bload(rNx0, kk, okrow);
       ny_[u] = bload(rNy0, kk, h_any);
-      hayA[u] = h_xp ? bload(rHAY, kk, h_xp) : hay_row;
-      hbyA[u] = h_xp ? bload(rHBY, kk, h_xp) : hby_row;
+      hayA[u] = h_xp ? bloadk(rK, ko.hay, kk, h_xp) : hay_row;
+      hbyA[u] = h_xp ? bloadk(rK, ko.hby, kk, h_xp) : hby_row;
       const int cz = jj + u - (NS - 1);
       load_corr(cz, f_[u], t_[u], ac && e_inner && corr_on(cz));
     }
