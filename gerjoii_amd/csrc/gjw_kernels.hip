@@ -662,6 +662,30 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     if (FRAMES)
       bstore(e, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + ic), ol && dom_row && dom_col(cl));
   };
+  // Levels whose column c-l is one of the strip's own columns [jb, je), as
+  // bit l of a wave-uniform mask (one range per column instead of two
+  // compares per level).
+  auto level_mask = [&](int c) {
+    const int lo = max(0, c - je + 1), hi = min(NS - 1, c - jb);
+    return hi >= lo ? (2u << hi) - (1u << lo) : 0u;
+  };
+  // level_out for the interior chunks: their columns all lie inside the
+  // stored domain's columns (P+1 .. NY-P-1 within dj0 .. dj0+n-1), so only the
+  // strip's own columns and the lane's row decide a frame store
+  const bool own_dom = own && dom_row;
+  auto level_out_in = [&](int l, int cl, float e, unsigned lv) {
+    const bool on = (lv >> l) & 1u;
+    const bool ol = own && on;
+    if (has_rec) {
+      if (s.rec_multi) {
+        if (ol) record(cl, e, it + l);
+      } else {
+        bstore(e, rTR, (unsigned)((rid_l[l] - t_lo) * s.nt + it + l), ol && rid_l[l] >= 0);
+      }
+    }
+    bstore(e, rJZ, (unsigned)(it + l), ol && i == si && cl == sj);
+    if (FRAMES) bstore(e, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + ic), own_dom && on);
+  };
 
   // ---- state.  Level 0 steps column c from the loaded fields; level l >= 1
   // steps column c-l from level l-1's outputs (its previous column: *_in[l];
@@ -833,6 +857,8 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       const int c = jj + u;
       float eo[NS], hxo[NS], hyo[NS], zo[NS];
       if (src) rid_l[0] = rec_lookup(c);
+      // per-level stores (frames, receivers, source recorder) test the level mask
+      const unsigned lv = (FRAMES || src) ? level_mask(c) : 0u;
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
         const int cl = c - l;
@@ -849,14 +875,15 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
         hxo[l] = hx1;
         hyo[l] = hy1;
         zo[l] = 0.f;  // no split-field cells in these rows
-        const bool ol = own && cl >= jb && cl < je;
+        const bool on = (lv >> l) & 1u;
         if (src) {
-          bstore(e1, rJZ, (unsigned)(it + l), ol && i == si && cl == sj);
-          bstore(e1, rTR, (unsigned)((rid_l[l] - t_lo) * s.nt + it + l), ol && rid_l[l] >= 0);
+          bstore(e1, rJZ, (unsigned)(it + l), own && on && i == si && cl == sj);
+          bstore(e1, rTR, (unsigned)((rid_l[l] - t_lo) * s.nt + it + l), own && on && rid_l[l] >= 0);
         }
-        if (FRAMES)
-          bstore(e1, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + i), ol && dom_row && dom_col(cl));
+        // interior columns are columns of the stored domain (see level_out_in)
+        if (FRAMES) bstore(e1, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + i), own_dom && on);
         if (l == NS - 1) {
+          const bool ol = own && cl >= jb && cl < je;
           const unsigned k2 = (unsigned)(cl * LD + i);
           bstore(e1, rE2, k2, ol);
           bstore(hx1, rHx2, k2, ol);
@@ -898,6 +925,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       const int c = jj + u;
       float eo[NS], hxo[NS], hyo[NS], xo[NS], yo[NS];
       if (has_rec) rid_l[0] = rec_lookup(c);
+      const unsigned lv = level_mask(c);
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
         const int cl = c - l;
@@ -923,9 +951,9 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
         hyo[l] = hy1;
         xo[l] = x1;
         yo[l] = y1;
-        level_out(l, cl, e1);
+        level_out_in(l, cl, e1, lv);
         if (l == NS - 1) {
-          const bool ol = own && cl >= jb && cl < je;
+          const bool ol = own && ((lv >> l) & 1u);
           const unsigned k2 = (unsigned)(cl * LD + ic);
           bstore(e1, rE2, k2, ol);
           bstore(hx1, rHx2, k2, ol);
