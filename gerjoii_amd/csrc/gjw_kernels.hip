@@ -475,6 +475,11 @@ __device__ __forceinline__ float4 bload4k(__amdgpu_buffer_rsrc_t r, unsigned sof
   return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
                      __uint_as_float(v.w));
 }
+__device__ __forceinline__ void bstorek(float v, __amdgpu_buffer_rsrc_t r, unsigned soff,
+                                        unsigned off_floats, bool on) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(on ? off_floats * 4u : kOOB),
+                                        (int)soff, 0);
+}
 // byte offsets of the arrays in the coefficient block (units of FS floats: see DevGrid)
 struct CoefOff {
   unsigned cc, cxe, cxh, cyh, hay, hby, u0, u1, u2;
@@ -684,7 +689,9 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       }
     }
     bstore(e, rJZ, (unsigned)(it + l), ol && i == si && cl == sj);
-    if (FRAMES) bstore(e, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + ic), own_dom && on);
+    if (FRAMES)
+      bstorek(e, rFR, (unsigned)(l * (FD - LD)) * 4u, (unsigned)((cl + l - g.dj0) * LD + ic),
+              own_dom && on);
   };
 
   // ---- state.  Level 0 steps column c from the loaded fields; level l >= 1
@@ -881,7 +888,11 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
           bstore(e1, rTR, (unsigned)((rid_l[l] - t_lo) * s.nt + it + l), own && on && rid_l[l] >= 0);
         }
         // interior columns are columns of the stored domain (see level_out_in)
-        if (FRAMES) bstore(e1, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + i), own_dom && on);
+        // level l's frame: plane l, column c-l; the per-level part l (FD - LD)
+        // rides in the scalar offset, the lanes share one offset per column
+        if (FRAMES)
+          bstorek(e1, rFR, (unsigned)(l * (FD - LD)) * 4u, (unsigned)((c - g.dj0) * LD + i),
+                  own_dom && on);
         if (l == NS - 1) {
           const bool ol = own && cl >= jb && cl < je;
           const unsigned k2 = (unsigned)(cl * LD + i);
@@ -1606,10 +1617,12 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
     (void)c;
   };
   // correlation operands of column cz: frames S_{it+1} .. S_{it-NS+1}, partials
+  // (the frames differ by whole planes: that part rides in the scalar offset,
+  // so the NS+1 loads share one per-lane offset)
   auto load_corr = [&](int cz, float* f, float2& t, bool on) {
     const unsigned fk = (unsigned)((cz - g.dj0) * LD + ic);
 #pragma unroll
-    for (int k = 0; k <= NS; ++k) f[k] = bload(rFR, (unsigned)((NS - k) * FD) + fk, on);
+    for (int k = 0; k <= NS; ++k) f[k] = bloadk(rFR, (unsigned)((NS - k) * FD) * 4u, fk, on);
     t = bload2(rAC, 2 * fk, on);
   };
 
