@@ -1880,6 +1880,22 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   // level 0 at columns jb-NS+1 .. je+NS-2 (the last level's last column is je-1)
   const bool use_inner = inner_rows && (!has_rec || !s.rec_multi);
   const int jlast = je + NS - 2;
+  // Interior tiles (all rows inner, no receivers, no source, every column of
+  // the march interior) run a loop of their own and return: in it only the
+  // interior chunk's descriptors and coefficients are live, so the scalar
+  // registers of the strip and row-class paths do not crowd it.
+  if (use_inner && !has_rec && !src_rows && chunkable(j1) &&
+      chunkable(j1 + (jlast - j1) / UA * UA)) {
+    // two chunks per trip: the level carries alternate between two sets of
+    // registers instead of being moved at every column
+    int jj = j1;
+    for (; jj + UA <= jlast; jj += 2 * UA) {
+      inner_chunk(jj, std::false_type{}, std::false_type{});
+      inner_chunk(jj + UA, std::false_type{}, std::false_type{});
+    }
+    if (jj <= jlast) inner_chunk(jj, std::false_type{}, std::false_type{});
+    return;
+  }
   int j = j1;
   for (; j <= jlast && !chunkable(j); j += UA) univ_chunk(j);
   if (use_inner && has_rec) {
