@@ -997,6 +997,25 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   // The two wave kinds get separate copies of the loop, so that each keeps
   // only its own coefficient carries live.
   const int jlast = je + NS - 2;
+  // Interior tiles (all rows inner, no receivers, no source row, every column
+  // of the march in interior chunks) run a loop of their own and return: in
+  // it only the interior chunk's descriptors and coefficients are live.
+  if (inner_rows && !has_rec && !(si >= r0 && si <= r0 + 63) && chunkable(j1, UI) &&
+      (jlast - j1 + 1) % UI == 0 && chunkable(jlast - UI + 1, UI)) {
+    int j = j1;
+    InnerBuf A, B;
+    inner_load(j, A, true);
+    while (j <= jlast) {
+      inner_load(j + UI, B, j + UI <= jlast);
+      inner_compute(j, A, std::false_type{});
+      j += UI;
+      if (j > jlast) break;
+      inner_load(j + UI, A, j + UI <= jlast);
+      inner_compute(j, B, std::false_type{});
+      j += UI;
+    }
+    return;
+  }
   if (inner_rows && (!has_rec || !s.rec_multi)) {
     int j = j1;
     for (; j <= jlast && !chunkable(j, UI); j += UC) univ_chunk(j);
