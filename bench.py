@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--fuse", type=int, default=0, help="forward steps per launch: 0 autotuned, 1, 2, 4, 8")
     ap.add_argument("--adj-fuse", type=int, default=0, help="adjoint steps per launch: 0 (six), 4, 6, 8")
+    ap.add_argument("--adj-tile", type=int, default=0,
+                    help="adjoint tile width (0: tuned in the first gradient sweep)")
     ap.add_argument("--ckpt", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1, help="1: replay the gradient as captured HIP graphs")
     ap.add_argument("--no-sqrt", action="store_true",
@@ -138,6 +140,8 @@ def main():
         sv.set_option(N.GJW_OPT_FUSE, args.fuse)
     if args.adj_fuse:
         sv.set_option(N.GJW_OPT_ADJ_FUSE, args.adj_fuse)
+    if args.adj_tile:
+        sv.set_option(N.GJW_OPT_ADJ_TILE, args.adj_tile)
     if args.ckpt:
         sv.set_option(N.GJW_OPT_CKPT_EVERY, args.ckpt)
     if args.graph:

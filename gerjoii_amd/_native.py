@@ -20,6 +20,7 @@ GJW_OPT_GRAPH = 7
 GJW_OPT_MISFIT = 8
 GJW_OPT_ADJ_FUSE = 9
 GJW_OPT_SHOT_GRADS = 10
+GJW_OPT_ADJ_TILE = 11
 GJW_LOW_PASS, GJW_HI_PASS = 0, 1
 GJW_SNAP_F32, GJW_SNAP_BF16, GJW_SNAP_F16 = 0, 1, 2
 GJW_MISFIT_L2 = 0

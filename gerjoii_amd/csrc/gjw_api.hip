@@ -220,6 +220,7 @@ struct gjw_survey {
   int tuned_L = 0, tuned_L1 = 0, tuned_L4 = 0, tuned_L8 = 0;  // 2-, 1-, 4-, 8-step forward
   int tuned_La2 = 0, tuned_La4 = 0, tuned_La6 = 0, tuned_La8 = 0;  // 2-, 4-, 6-, 8-step adjoint
   int adj_fuse = 0;  // adjoint steps per launch (GJW_OPT_ADJ_FUSE; 0: six)
+  int adj_tile = 0;  // adjoint tile width (GJW_OPT_ADJ_TILE; 0: tuned in context)
   // adjoint-source band filter of gjw_fwi_gradient (w_update_e.m:69-70); off
   // while flt_hi <= flt_lo
   double flt_lo = 0, flt_hi = 0;
@@ -462,6 +463,12 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       s->tuned_La2 = s->tuned_La4 = s->tuned_La6 = s->tuned_La8 = 0;
       s->fuse_eff = 0;
       break;
+    case GJW_OPT_ADJ_TILE:
+      if (v < 0 || v > 4096) return fail(GJW_ERR_ARG, "GJW_OPT_ADJ_TILE out of range");
+      s->adj_tile = (int)v;
+      s->tuned_La2 = s->tuned_La4 = s->tuned_La6 = s->tuned_La8 = 0;
+      s->graph_calls = 0;
+      return GJW_OK;
     case GJW_OPT_SHOT_GRADS:
       if (v != 0 && v != 1) return fail(GJW_ERR_ARG, "GJW_OPT_SHOT_GRADS must be 0 or 1");
       s->shot_grads = (int)v;
@@ -1093,7 +1100,7 @@ static int autotune_L(gjw_survey* s, int nsh) {
               c8[] = {48, 64, 96, 128};
     const int* cand = NS == 1 ? c1 : NS == 2 ? c2 : NS == 4 ? c4 : c8;
     const int ncand = NS == 8 ? (int)(sizeof(c8) / sizeof(int)) : 4;
-    float best = 1e30f;
+    float best = 1e30f, tms[8] = {};
     int bestL = 16;
     for (int ci = 0; ci < ncand; ++ci) {
       const int L = cand[ci];
@@ -1111,11 +1118,13 @@ static int autotune_L(gjw_survey* s, int nsh) {
       HIPCHK(hipEventSynchronize(s->ctx->ev1));
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
-      if (ms < best) {
-        best = ms;
-        bestL = L;
-      }
+      tms[ci] = ms;
+      best = std::min(best, ms);
     }
+    // the widest candidate within 1.5% of the fastest (candidates ascend):
+    // near-equal widths would otherwise flip between runs on timing noise
+    for (int ci = 0; ci < ncand; ++ci)
+      if (tms[ci] <= 1.015f * best) bestL = cand[ci];
     (NS == 1 ? s->tuned_L1 : NS == 2 ? s->tuned_L : NS == 4 ? s->tuned_L4 : s->tuned_L8) = bestL;
     best_of[NS] = best;
   }
@@ -1753,6 +1762,7 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
   // its first launches of the deepest kernel the candidate widths in turn,
   // timed with events; later sweeps use the fastest.
   if (s->tile_L > 0) s->tuned_La2 = s->tuned_La4 = s->tuned_La6 = s->tuned_La8 = s->tile_L;
+  if (s->adj_tile > 0) s->tuned_La2 = s->tuned_La4 = s->tuned_La6 = s->tuned_La8 = s->adj_tile;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   HIPCHK(hipStreamIsCapturing(st, &cap));
   const bool explore = tuned_deep <= 0 && cap == hipStreamCaptureStatusNone;
@@ -1810,11 +1820,15 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
     HIPCHK(hipGetLastError());
   }
   if (explore) {
-    // fastest candidate over the trial rounds after the first (warm-up) one
-    float best = 1e30f;
+    // Mean time of each candidate over the trial rounds after the first
+    // (warm-up) one; the widest candidate within 1.5% of the fastest wins
+    // (near-equal widths would otherwise flip between runs on timing noise;
+    // the widest re-reads the fewest strip-edge columns).
     int bestL = La_fallback;
     if (ex >= 2 * kAdjNCand) {
       HIPCHK(hipEventSynchronize(s->ex_ev[2 * ex - 1]));
+      float mean[kAdjNCand];
+      float best = 1e30f;
       for (int c = 0; c < kAdjNCand; ++c) {
         float tot = 0;
         int n = 0;
@@ -1824,11 +1838,11 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
           tot += ms;
           ++n;
         }
-        if (n > 0 && tot / n < best) {
-          best = tot / n;
-          bestL = (dmax >= 6 ? kAdjCand6 : kAdjCand4)[c];
-        }
+        mean[c] = n > 0 ? tot / n : 1e30f;
+        best = std::min(best, mean[c]);
       }
+      for (int c = 0; c < kAdjNCand; ++c)  // candidates ascend: the last one within wins
+        if (mean[c] <= 1.015f * best) bestL = (dmax >= 6 ? kAdjCand6 : kAdjCand4)[c];
     }
     if (dmax >= 6)
       tuned_deep = bestL;

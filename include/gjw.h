@@ -107,6 +107,10 @@ int gjw_set_shots(gjw_survey *s, int32_t ns, const int32_t *src, const float *wa
                                 return one gradient per shot, ns*m*n values each (shot-major),
                                 as w_update_e.m:98 keeps g_e per source before its
                                 post-processing; 0 (default): the sum over shots */
+#define GJW_OPT_ADJ_TILE 11  /* columns per wavefront of the 4-, 6- and 8-step adjoint on the
+                                batched path; 0 (default): tuned in the first gradient sweep
+                                (the widest of the candidates within 1.5% of the fastest).
+                                A fixed width makes gradients repeat bit for bit across runs */
 #define GJW_PATH_BATCHED 0   /* all shots per launch, checkpoint + recompute        */
 #define GJW_PATH_PLAIN 1     /* one shot at a time, full wavefield stored (as w_solve.m) */
 #define GJW_SNAP_F32 0

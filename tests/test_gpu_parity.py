@@ -730,6 +730,29 @@ def test_adjoint_width_tuned_in_context(gpu_ctx):
         assert abs(J - ref[2]) <= 1e-6 * abs(ref[2])
 
 
+def test_adjoint_width_pinned(gpu_ctx):
+    """GJW_OPT_ADJ_TILE fixes the adjoint's tile width: the survey reports it,
+    its gradients repeat bit for bit from the first sweep on, and they agree
+    with an autotuned survey's to rounding; a width out of range is refused."""
+    import torch
+    from gerjoii_amd import _native as N
+    case = configs.small(m=150, n=260, nt=400, ns=2)
+    svt = make_survey(case, gpu_ctx)
+    svt.set_model(case.eps_true, case.sig_true)
+    dobs = torch.from_numpy(svt.forward()).cuda()
+    tuned = make_survey(case, gpu_ctx).fwi_gradient(dobs)
+    sv = make_survey(case, gpu_ctx)
+    sv.set_option(N.GJW_OPT_ADJ_TILE, 64)
+    first = sv.fwi_gradient(dobs)
+    assert sv.tile(8) == 64
+    again = sv.fwi_gradient(dobs)
+    np.testing.assert_array_equal(again[0], first[0])
+    np.testing.assert_array_equal(again[1], first[1])
+    assert rel_l2(first[0], tuned[0]) <= 1e-5 and rel_l2(first[1], tuned[1]) <= 1e-5
+    with pytest.raises(RuntimeError):
+        sv.set_option(N.GJW_OPT_ADJ_TILE, -1)
+
+
 @pytest.mark.parametrize("nt", [240, 241])
 def test_hilbert_misfits(oracle_mod, gpu_ctx, nt):
     """Envelope / phase misfits on the device (FP32 FFTs) against the oracle's
