@@ -423,6 +423,12 @@ def test_errors_are_reported(gpu_ctx):
     bad[0] = (1, 5)  # on the never-updated outer ring
     with pytest.raises(N.GjwError, match="source"):
         sv.set_shots(bad, case.wavelets, case.receivers)
+    # a grid past the step kernels' 32-bit offsets is refused when the survey
+    # is created (6000 x 6000 cells: 38 M padded cells per field)
+    import dataclasses
+    big = dataclasses.replace(case.grid, m=6000, n=6000)
+    with pytest.raises(N.GjwError, match="exceeds"):
+        S.Survey(big, gpu_ctx)
 
 
 def test_gradient_ref_mode(oracle_mod, gpu_ctx):
