@@ -157,15 +157,20 @@ def c3(seed=1003, ns=64, nt=8000):
     return _case("C3", g, 200e6, eps, sig, shape, src, recs, 8)
 
 
-def c4(seed=1004, ns=32, nt=12000):
+def c4(seed=1004, ns=32, nt=12000, coarse=1):
     """Precision-study scale: 4000 x 1500, dx 0.02 m, 40-cell PML, nt 12000,
     250 MHz, 32 shots; layered eps in [3, 20] with a water-table layer
     (eps 81) and void inclusions (eps 1, sigma 0); lognormal sigma; 400
-    receivers every 10 cells."""
+    receivers every 10 cells.
+
+    ``coarse`` = c > 1 gives the same survey downsampled: the model built at
+    full resolution and sampled every c-th cell in both directions, dx times
+    c, dt at the same Courant number (so nt // c steps cover the same
+    duration), sources and receivers at the nearest coarse column, starting
+    model smoothed over the same length, PML still 40 cells."""
     rng = np.random.default_rng(seed)
     m, n, P = 1500, 4000, 40
     dx = 0.02
-    g = Grid(m=m, n=n, pml=P, air=0, nt=nt, dx=dx, dt=dt_cfl(dx))
     eps = _layered(rng, m, n, 6, 3.0, 20.0)
     sig = _lognormal_sigma(rng, m, n)
     z = np.arange(m)[:, None]
@@ -182,9 +187,17 @@ def c4(seed=1004, ns=32, nt=12000):
     h = _topography(rng, n)
     eps, sig, shape = _apply_topo(eps, sig, h)
     first = (n - 120 * (ns - 1)) // 2
-    src, recs = _surface_geometry(g, h, [first + 120 * k for k in range(ns)],
-                                  np.arange(0, n, 10)[:400])
-    return _case("C4", g, 250e6, eps, sig, shape, src, recs, 8)
+    scols = np.array([first + 120 * k for k in range(ns)])
+    rcols = np.arange(0, n, 10)[:400]
+    c = int(coarse)
+    if c > 1:
+        eps, sig, shape = eps[::c, ::c].copy(), sig[::c, ::c].copy(), shape[::c, ::c].copy()
+        m, n = eps.shape
+        h = (shape == 0).sum(axis=0)  # air is a top-contiguous run of rows
+        scols, rcols = scols // c, rcols // c
+    g = Grid(m=m, n=n, pml=P, air=0, nt=nt // c, dx=dx * c, dt=dt_cfl(dx * c))
+    src, recs = _surface_geometry(g, h, scols, rcols)
+    return _case("C4" if c == 1 else f"C4/{c}", g, 250e6, eps, sig, shape, src, recs, 8 / c)
 
 
 def c5(seed=1005):

@@ -7,9 +7,10 @@
   default batched path (8-step forward, 6-step adjoint, default checkpoint
   interval) against the FP64 oracle, same bar;
 * 4000 x 1500, nt 12000: two shots with BF16 and FP16 checkpoint snapshots
-  against FP32 snapshots, and one shot with BF16 snapshots against the FP64
-  oracle over the first 1200 steps (the FP64 CPU run of all 12000 steps on
-  this grid would take the whole test budget);
+  against FP32 snapshots, one shot with BF16 snapshots against the FP64
+  oracle over the first 1200 steps, and the same survey downsampled 2x
+  (2000 x 750, nt 6000, the full duration), one shot with FP32, BF16 and
+  FP16 snapshots against the FP64 oracle over all steps;
 * the uneven three-survey batch (600 x 200 / 3000 / 24 shots, 1500 x 500 /
   7000 / 48, 3000 x 800 / 10000 / 16; envelope, phase, envelope misfits;
   BF16 snapshots; captured graphs): each survey's gradient from the shared
@@ -202,6 +203,31 @@ def test_c4_shot_fp64_first_steps(oracle_mod, gpu_ctx, snap):
           flush=True)
     assert rel_l2(ge, fe) <= 1e-3 and cosine(ge, fe) >= 0.9999
     assert rel_l2(gs, fs) <= 1e-3 and cosine(gs, fs) >= 0.9999
+
+
+def test_c4_downsampled_full_duration_fp64(oracle_mod, gpu_ctx):
+    """The same survey downsampled 2x (2000 x 750, dx 0.04 m, nt 6000: the
+    full 0.51 us), one shot, all steps: gradients with FP32, BF16 and FP16
+    snapshots on the default batched path against one FP64 oracle run."""
+    case = shots(configs.c4(coarse=2), [16])
+    g = case.grid
+    print(f"C4/2 one shot: {g.n} x {g.m}, nt {g.nt}, FP64 oracle", flush=True)
+    dobs = observed(case, gpu_ctx)
+    runs = {}
+    for name, snap in (("fp32", N.GJW_SNAP_F32), ("bf16", N.GJW_SNAP_BF16), ("fp16", N.GJW_SNAP_F16)):
+        runs[name] = gpu_l2_gradient(case, gpu_ctx, dobs, snap=snap)
+    # the forward sweep reads no snapshot: one residual serves all three
+    e, rp = runs["fp32"][2], runs["fp32"][4]
+    for name in ("bf16", "fp16"):
+        assert np.array_equal(runs[name][2], e)
+    t0 = time.perf_counter()
+    fe, fs = oracle_grad(oracle_mod, case, e, rp)
+    print(f"C4/2 FP64 oracle {time.perf_counter() - t0:.1f} s", flush=True)
+    for name, (ge, gs, _, J, _) in runs.items():
+        print(f"C4/2 {name} vs FP64: eps rel {rel_l2(ge, fe):.2e} cos {cosine(ge, fe):.8f}; "
+              f"sig rel {rel_l2(gs, fs):.2e} cos {cosine(gs, fs):.8f}", flush=True)
+        assert rel_l2(ge, fe) <= 1e-3 and cosine(ge, fe) >= 0.9999
+        assert rel_l2(gs, fs) <= 1e-3 and cosine(gs, fs) >= 0.9999
 
 
 def test_c5_uneven_batch_full_size(gpu_ctx):
