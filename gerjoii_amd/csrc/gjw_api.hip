@@ -963,8 +963,11 @@ static int fwd_launch(gjw_survey* s, const DevGrid& g, const DevShots& sh, int n
 
 // NS steps per launch; grid of (64 - 2 NS)-row overlapped tiles
 // (a block stacks 4 row tiles of one strip)
-static inline dim3 stepN_grid(const gjw_survey* s, int NS, int L, int nshots) {
-  return dim3((unsigned)(cdiv(cdiv(s->NX + 1, 64 - 2 * NS), 4) * cdiv(s->NY + 1, L) * nshots));
+// wpb: waves per block (4, or 2: each block-row's stack of four row tiles
+// as two blocks, see vtile_of_block)
+static inline dim3 stepN_grid(const gjw_survey* s, int NS, int L, int nshots, int wpb = 4) {
+  return dim3((unsigned)(cdiv(cdiv(s->NX + 1, 64 - 2 * NS), 4) * cdiv(s->NY + 1, L) * nshots *
+                         (4 / wpb)));
 }
 
 // NS = 2 or 4 fused forward steps; fr1 (optional) receives the NS domain
@@ -974,7 +977,9 @@ static int fwdN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& 
                        long long fstride) {
   int slot;
   TRY(kev_begin(s, NS == 8 ? (fr1 ? 9 : 6) : NS == 4 ? 4 : 2, it, &slot));
-  const dim3 grid = stepN_grid(s, NS, L, nsh), block(64, 4);
+  // the forward sweep's 8-step kernel runs its stacks as blocks of two waves
+  const int wpb = (NS == 8 && !fr1) ? 2 : 4;
+  const dim3 grid = stepN_grid(s, NS, L, nsh, wpb), block(64, wpb);
   hipStream_t st = s->ctx->stream;
 #define GJW_FWDN_CALL(K)                                                                     \
   hipLaunchKernelGGL(K, grid, block, 0, st, g, sh, L, b.E[cur], b.Hx[cur], b.Hy[cur], b.X[xc], \

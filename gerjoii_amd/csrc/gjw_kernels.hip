@@ -108,6 +108,7 @@ constexpr int kShotGroup = 8;  // measured best against 4 and 16
 
 struct TileId {
   int rowtile, strip, shot;
+  int wy;  // multi-step kernels: the wave's place in its block-row's stack of four row tiles
 };
 
 // Logical tile t of a grid of gx row tiles, gy strip groups and ns shots.
@@ -120,6 +121,7 @@ __device__ __forceinline__ TileId tile_decode(unsigned t, unsigned gx, unsigned 
   id.shot = (int)(grp * (unsigned)kShotGroup + rem % gc);
   id.rowtile = (int)(rest % gx);
   id.strip = (int)(rest / gx);
+  id.wy = (int)threadIdx.y;
   return id;
 }
 
@@ -167,6 +169,7 @@ __device__ __forceinline__ TileId vtile_decode(unsigned t, unsigned gx, unsigned
   const unsigned gc = min((unsigned)kShotGroup, ns - grp * (unsigned)kShotGroup);
   const unsigned nb = 2u * gy * gc;  // tiles of the boundary block-rows
   TileId id;
+  id.wy = (int)threadIdx.y;
   if (rem < nb) {
     const unsigned r = rem / gc;
     id.shot = (int)(grp * (unsigned)kShotGroup + rem % gc);
@@ -181,13 +184,30 @@ __device__ __forceinline__ TileId vtile_decode(unsigned t, unsigned gx, unsigned
   return id;
 }
 
+// A block-row's stack of four row tiles may also run as q = 4 / blockDim.y
+// blocks of blockDim.y waves (the forward sweep's 8-step kernel takes two
+// blocks of two: 1.017 against 1.071 ms per launch, where the other step
+// kernels ran 1.5-2% slower that way).  Physical block b takes sub-stack
+// k mod q of logical block k / q, with k its place in its XCD's run (b / 8):
+// the q blocks of one stack stay on one XCD and its L2.
 __device__ __forceinline__ TileId vtile_of_block(int NX, int NY, int L, int rows,
                                                 bool slow_first) {
-  const unsigned B = gridDim.x, b = blockIdx.x;
+  const unsigned q = 4u / blockDim.y;
+  const unsigned b = blockIdx.x, B = gridDim.x / q;  // logical blocks
   const unsigned per = B / 8u;
-  const unsigned t = (b < per * 8u) ? (b % 8u) * per + b / 8u : b;
+  unsigned t, sub;
+  if (b < per * 8u * q) {
+    const unsigned k = b / 8u;
+    t = (b % 8u) * per + k / q;
+    sub = k % q;
+  } else {
+    t = b / q;
+    sub = b % q;
+  }
   const unsigned gx = vtiles_x(NX, rows), gy = vtiles_y(NY, L);
-  return vtile_decode(t, gx, gy, B / (gx * gy), slow_first);
+  TileId id = vtile_decode(t, gx, gy, B / (gx * gy), slow_first);
+  id.wy = (int)(sub * blockDim.y + threadIdx.y);
+  return id;
 }
 
 // Row tile t of the multi-step kernels (R = 64 - 2 NS owned rows per wave):
@@ -554,7 +574,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   const int shot = tid.shot;
   // threadIdx.y is constant across a 64-lane wavefront: make the tile origin
   // (and everything derived from it) scalar so loop control stays uniform
-  const RowTile rt = row_tile((int)(tid.rowtile * blockDim.y + threadIdx.y), NX, R, NS);
+  const RowTile rt = row_tile((int)(tid.rowtile * 4 + tid.wy), NX, R, NS);
   const int r0 = __builtin_amdgcn_readfirstlane(rt.r0);
   const int own_hi = __builtin_amdgcn_readfirstlane(rt.own_hi);
   const int jb = __builtin_amdgcn_readfirstlane(tid.strip * L);
@@ -1432,7 +1452,7 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   const int NX = g.NX, NY = g.NY, P = g.P;
   const long long LD = g.LD;
   const int shot = tid.shot;
-  const RowTile rt = row_tile((int)(tid.rowtile * blockDim.y + threadIdx.y), NX, R, NS);
+  const RowTile rt = row_tile((int)(tid.rowtile * 4 + tid.wy), NX, R, NS);
   const int r0 = __builtin_amdgcn_readfirstlane(rt.r0);
   const int own_hi = __builtin_amdgcn_readfirstlane(rt.own_hi);
   const int jb = __builtin_amdgcn_readfirstlane(tid.strip * L);
