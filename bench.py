@@ -34,7 +34,7 @@ KERNELS = [("k_fwd_step", 1, "fwd"), ("k_adj_step", 1, "adj"), ("k_fwd_step2", 2
            ("k_fwd_step8_frames", 8, "fwd_frames")]
 
 
-def compulsory_bytes(kind, S, g, ns, ntr):
+def compulsory_bytes(kind, S, g, ns, ntr, fb=4):
     """HBM bytes one S-step launch over all shots must move at least: every
     field read and written once (Ez, Hx, Hy on all padded cells, the split
     fields on the PML cells), the survey's coefficients read once (shared by
@@ -42,7 +42,8 @@ def compulsory_bytes(kind, S, g, ns, ntr):
     what the launch exists to move: the traces it records (forward sweep),
     the S Ez frames of the domain it stores (recompute sweep), or the S+1
     frames, the adjoint-source samples and the FP32 correlation partials
-    (read and written) it uses (adjoint)."""
+    (read and written) it uses (adjoint).  fb: bytes per frame element (4
+    for FP32 frames, 2 for BF16)."""
     NX, NY = g.nx, g.ny
     cells = (NX + 1) * (NY + 1)
     pml = (NX - 1) * (NY - 1) - (NX - 1 - 2 * g.pml) * (NY - 1 - 2 * g.pml)
@@ -51,8 +52,8 @@ def compulsory_bytes(kind, S, g, ns, ntr):
     if kind == "fwd":
         return b + 4 * S * ntr
     if kind == "fwd_frames":
-        return b + 4 * S * dom * ns
-    return b + 4 * (S + 1) * dom * ns + 16 * dom * ns + 4 * S * ntr
+        return b + fb * S * dom * ns
+    return b + fb * (S + 1) * dom * ns + 16 * dom * ns + 4 * S * ntr
 
 
 def kernel_source_sha16():
@@ -92,6 +93,9 @@ def main():
     ap.add_argument("--adj-tile", type=int, default=0,
                     help="adjoint tile width (0: tuned in the first gradient sweep)")
     ap.add_argument("--ckpt", type=int, default=0)
+    ap.add_argument("--frames", choices=["fp32", "bf16"], default="bf16",
+                    help="element type of the recompute sweep's Ez frames (GJW_OPT_FRAMES); "
+                         "with bf16 a gradient with FP32 frames is timed beside (frames_fp32)")
     ap.add_argument("--graph", type=int, default=1, help="1: replay the gradient as captured HIP graphs")
     ap.add_argument("--no-sqrt", action="store_true",
                     help="skip the extra gradient timed at a checkpoint interval of about sqrt(nt)")
@@ -144,6 +148,8 @@ def main():
         sv.set_option(N.GJW_OPT_ADJ_TILE, args.adj_tile)
     if args.ckpt:
         sv.set_option(N.GJW_OPT_CKPT_EVERY, args.ckpt)
+    if args.frames == "bf16":
+        sv.set_option(N.GJW_OPT_FRAMES, N.GJW_FRAMES_BF16)
     if args.graph:
         sv.set_option(N.GJW_OPT_GRAPH, 1)
     # observed data: true model, kept in HBM
@@ -188,6 +194,7 @@ def main():
     tile_L = {name: sv.tile(w) for w, (name, _, _) in enumerate(KERNELS) if w >= 2}
     K_used = sv.query(N.GJW_Q_CKPT_INTERVAL)
     depths = (sv.query(N.GJW_Q_FWD_DEPTH), sv.query(N.GJW_Q_ADJ_DEPTH))
+    frames_used = "bf16" if sv.query(N.GJW_Q_FRAMES) == N.GJW_FRAMES_BF16 else "fp32"
     ntr = sv.ntraces
     # the same gradient at BASELINE's checkpoint interval, about sqrt(nt) steps
     # (rounded up to whole launches of both depths)
@@ -207,6 +214,21 @@ def main():
                     "note": "one gradient after one untimed call, same survey and options"}
         sv.set_option(N.GJW_OPT_CKPT_EVERY, 0)
         log(f"sqrt(nt) interval {Ks}: {ns / ts:.3f} shot-gradients/s")
+    # the same gradient with FP32 frames (the library's default)
+    f32_run = None
+    if frames_used == "bf16":
+        sv.set_option(N.GJW_OPT_FRAMES, N.GJW_FRAMES_F32)
+        sv.set_option(N.GJW_OPT_GRAPH, 1 if args.graph else 0)
+        sv.fwi_gradient(d_obs, g_eps, g_sig)  # new buffers and (graph) capture
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sv.fwi_gradient(d_obs, g_eps, g_sig)
+        torch.cuda.synchronize()
+        tf = time.perf_counter() - t1
+        f32_run = {"s": tf, "shot_gradients_per_s": ns / tf,
+                   "frames": "fp32" if sv.query(N.GJW_Q_FRAMES) == N.GJW_FRAMES_F32 else "bf16",
+                   "note": "one gradient after one untimed call, same survey, FP32 recompute frames"}
+        log(f"FP32 frames: {ns / tf:.3f} shot-gradients/s")
     sv.close()
     del sv
     torch.cuda.empty_cache()
@@ -224,11 +246,12 @@ def main():
     for (name, spl, kind), (avg_ms, samples, launches) in zip(KERNELS, kstats):
         v = {"avg_ms": avg_ms, "samples": samples, "steps_per_launch": spl,
              "launches_per_step": launches,
-             "bytes_per_launch_compulsory": compulsory_bytes(kind, spl, g, ns, ntr)}
+             "bytes_per_launch_compulsory": compulsory_bytes(kind, spl, g, ns, ntr,
+                                                             2 if frames_used == "bf16" else 4)}
         v["achieved_GBs"] = (v["bytes_per_launch_compulsory"] / (avg_ms * 1e-3) / 1e9
                              if avg_ms > 0 else None)
         v["share_of_step"] = avg_ms * launches / ms_eager if ms_eager > 0 else None
-        t, note = _traffic_from_profiles(name, src_sha)
+        t, note = _traffic_from_profiles(name, src_sha, frames_used)
         v["traffic_bytes"] = t
         if note:
             v["traffic_note"] = note
@@ -310,6 +333,12 @@ def main():
                    "checkpoint_rule": "about 2 sqrt(nt) in whole launches (sqrt(nt) run beside: "
                                       "checkpoint_sqrt_nt)" if not args.ckpt else "set by --ckpt",
                    "steps_per_launch": {"forward": depths[0], "adjoint": depths[1]},
+                   "frames": frames_used,
+                   "frames_note": "element type of the Ez frames the recompute sweep hands to the "
+                                  "adjoint (GJW_OPT_FRAMES); the BF16-frame gradient of one shot of "
+                                  "this workload against the FP64 oracle is checked to <= 1e-3 in "
+                                  "tests/test_gpu_frames16.py; FP32 frames timed beside as "
+                                  "frames_fp32",
                    "launches_per_gradient": {k: v["launches_per_step"] for k, v in kern.items()
                                              if v["launches_per_step"] > 0},
                    "hip_graph": bool(args.graph),
@@ -323,6 +352,7 @@ def main():
         "batched_vs_plain": (shot_grads / plain["shot_gradients_per_s"]) if plain else None,
         "roofline": roof,
         "checkpoint_sqrt_nt": sqrt_run,
+        "frames_fp32": f32_run,
         "cpu_baseline": cpu,
         "adjoint_source_filter": cond,
         "line_search": lsearch,
@@ -677,7 +707,7 @@ def _conditioning(ctx, d_obs, dt, no_cpu):
     return out
 
 
-def _traffic_from_profiles(kernel, src_sha):
+def _traffic_from_profiles(kernel, src_sha, frames=None):
     """HBM bytes per launch from the committed rocprofv3 PMC summary of this
     kernel at this workload (profiles/<kernel>_C3_pmc.txt, lines
     'bytes_per_launch: <float>' and 'kernel_source_sha16: <hex>').  Returns
@@ -685,7 +715,7 @@ def _traffic_from_profiles(kernel, src_sha):
     p = os.path.join(ROOT, "profiles", f"{kernel}_C3_pmc.txt")
     if not os.path.exists(p):
         return None, None
-    val, sha = None, None
+    val, sha, fr = None, None, "fp32"
     for line in open(p):
         if line.startswith("bytes_per_launch:"):
             try:
@@ -694,8 +724,12 @@ def _traffic_from_profiles(kernel, src_sha):
                 val = None
         elif line.startswith("kernel_source_sha16:"):
             sha = line.split(":")[1].strip()
+        elif line.startswith("frames:"):
+            fr = line.split(":")[1].strip()
     if sha != src_sha:
         return None, f"{os.path.basename(p)} was taken on other kernel sources ({sha})"
+    if frames is not None and kernel in ("k_fwd_step8_frames", "k_adj_step6") and fr != frames:
+        return None, f"{os.path.basename(p)} was taken with {fr} frames"
     return val, None
 
 

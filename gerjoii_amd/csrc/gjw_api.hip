@@ -38,25 +38,42 @@ __global__ void k_adj_step(DevGrid, DevShots, int, const float*, const float*, c
 #define GJW_ADJN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \
     const float*, const float*, float*, float*, float*, float*, float*, int, const float*, \
     const float*, long long, float*, long long, float*
+// FR: frames FP32 (1) or BF16 (2)
+template <int FR>
 __global__ void k_adj_step2(GJW_ADJN_ARGS);
+template <int FR>
 __global__ void k_adj_step4(GJW_ADJN_ARGS);
+template <int FR>
 __global__ void k_adj_step8(GJW_ADJN_ARGS);
+template <int FR>
 __global__ void k_adj_step6(GJW_ADJN_ARGS);
+extern template __global__ void k_adj_step2<1>(GJW_ADJN_ARGS);
+extern template __global__ void k_adj_step2<2>(GJW_ADJN_ARGS);
+extern template __global__ void k_adj_step4<1>(GJW_ADJN_ARGS);
+extern template __global__ void k_adj_step4<2>(GJW_ADJN_ARGS);
+extern template __global__ void k_adj_step6<1>(GJW_ADJN_ARGS);
+extern template __global__ void k_adj_step6<2>(GJW_ADJN_ARGS);
+extern template __global__ void k_adj_step8<1>(GJW_ADJN_ARGS);
+extern template __global__ void k_adj_step8<2>(GJW_ADJN_ARGS);
 #undef GJW_ADJN_ARGS
 #define GJW_FWDN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \
     const float*, const float*, float*, float*, float*, float*, float*, int, float*, float*, long long
-template <bool FRAMES>
+// FR: no frames (0), FP32 frames (1), BF16 frames (2)
+template <int FR>
 __global__ void k_fwd_step2(GJW_FWDN_ARGS);
-template <bool FRAMES>
+template <int FR>
 __global__ void k_fwd_step4(GJW_FWDN_ARGS);
-template <bool FRAMES>
+template <int FR>
 __global__ void k_fwd_step8(GJW_FWDN_ARGS);
-extern template __global__ void k_fwd_step2<false>(GJW_FWDN_ARGS);
-extern template __global__ void k_fwd_step2<true>(GJW_FWDN_ARGS);
-extern template __global__ void k_fwd_step4<false>(GJW_FWDN_ARGS);
-extern template __global__ void k_fwd_step4<true>(GJW_FWDN_ARGS);
-extern template __global__ void k_fwd_step8<false>(GJW_FWDN_ARGS);
-extern template __global__ void k_fwd_step8<true>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step2<0>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step2<1>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step2<2>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step4<0>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step4<1>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step4<2>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step8<0>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step8<1>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step8<2>(GJW_FWDN_ARGS);
 #undef GJW_FWDN_ARGS
 __global__ void k_fwd_step_multi(const MultiTable, MultiArgs);
 template <int NS, bool FRAMES>
@@ -85,7 +102,7 @@ __global__ void k_pack16(const float*, long long, const int*, long long, int, in
 __global__ void k_unpack16(const unsigned short*, long long, int, const int*, int, const int*,
                            long long, float*);
 __global__ void k_scatter(const float*, long long, const int*, int, int, float*);
-__global__ void k_domain_frame(DevGrid, int, const float*, float*, long long);
+__global__ void k_domain_frame(DevGrid, int, const float*, float*, long long, int);
 __global__ void k_flush(int, int, int, int, int, float*, long long, const double*, const double*,
                         double*, double*, double, long long);
 __global__ void k_ref_overwrite(DevGrid, int, int, int, const int*, const int*, const float*, int,
@@ -221,6 +238,9 @@ struct gjw_survey {
   int tuned_La2 = 0, tuned_La4 = 0, tuned_La6 = 0, tuned_La8 = 0;  // 2-, 4-, 6-, 8-step adjoint
   int adj_fuse = 0;  // adjoint steps per launch (GJW_OPT_ADJ_FUSE; 0: six)
   int adj_tile = 0;  // adjoint tile width (GJW_OPT_ADJ_TILE; 0: tuned in context)
+  int frames16 = 0;  // GJW_OPT_FRAMES: BF16 recompute frames asked for
+  int fr16 = 0;      // the gradient in progress keeps its frames in BF16 (frames_bf16)
+  int fr16_used = 0; // ... and the last one did (GJW_Q_FRAMES)
   // adjoint-source band filter of gjw_fwi_gradient (w_update_e.m:69-70); off
   // while flt_hi <= flt_lo
   double flt_lo = 0, flt_hi = 0;
@@ -469,6 +489,11 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       s->tuned_La2 = s->tuned_La4 = s->tuned_La6 = s->tuned_La8 = 0;
       s->graph_calls = 0;
       return GJW_OK;
+    case GJW_OPT_FRAMES:
+      if (v != GJW_FRAMES_F32 && v != GJW_FRAMES_BF16)
+        return fail(GJW_ERR_ARG, "GJW_OPT_FRAMES: GJW_FRAMES_F32 or GJW_FRAMES_BF16");
+      s->frames16 = (int)v;
+      break;
     case GJW_OPT_SHOT_GRADS:
       if (v != 0 && v != 1) return fail(GJW_ERR_ARG, "GJW_OPT_SHOT_GRADS must be 0 or 1");
       s->shot_grads = (int)v;
@@ -985,21 +1010,28 @@ static int fwdN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& 
   hipLaunchKernelGGL(K, grid, block, 0, st, g, sh, L, b.E[cur], b.Hx[cur], b.Hy[cur], b.X[xc], \
                      b.Y[xc], b.E[cur ^ 1], b.Hx[cur ^ 1], b.Hy[cur ^ 1], b.X[xc ^ 1],        \
                      b.Y[xc ^ 1], it, tr, fr1, fstride)
+  const int FR = fr1 ? (s->fr16 ? 2 : 1) : 0;
   if (NS == 8) {
-    if (fr1)
-      GJW_FWDN_CALL(k_fwd_step8<true>);
+    if (FR == 2)
+      GJW_FWDN_CALL(k_fwd_step8<2>);
+    else if (FR == 1)
+      GJW_FWDN_CALL(k_fwd_step8<1>);
     else
-      GJW_FWDN_CALL(k_fwd_step8<false>);
+      GJW_FWDN_CALL(k_fwd_step8<0>);
   } else if (NS == 4) {
-    if (fr1)
-      GJW_FWDN_CALL(k_fwd_step4<true>);
+    if (FR == 2)
+      GJW_FWDN_CALL(k_fwd_step4<2>);
+    else if (FR == 1)
+      GJW_FWDN_CALL(k_fwd_step4<1>);
     else
-      GJW_FWDN_CALL(k_fwd_step4<false>);
+      GJW_FWDN_CALL(k_fwd_step4<0>);
   } else {
-    if (fr1)
-      GJW_FWDN_CALL(k_fwd_step2<true>);
+    if (FR == 2)
+      GJW_FWDN_CALL(k_fwd_step2<2>);
+    else if (FR == 1)
+      GJW_FWDN_CALL(k_fwd_step2<1>);
     else
-      GJW_FWDN_CALL(k_fwd_step2<false>);
+      GJW_FWDN_CALL(k_fwd_step2<0>);
   }
 #undef GJW_FWDN_CALL
   HIPCHK(hipGetLastError());
@@ -1024,6 +1056,7 @@ static int fwd_advance(gjw_survey* s, const DevGrid& g, const DevShots& sh, int 
     TRY(fwdN_launch(s, NS, g, sh, nsh, L, b, cur, xc, it, tr, frame, fstride));
     xc ^= 1;
   } else {
+    if (frame && s->fr16) return fail(GJW_ERR_STATE, "BF16 frames reached a one-step forward launch");
     TRY(fwd_launch(s, g, sh, nsh, s->tuned_L1, b, cur, it, tr, frame, fstride, xc));
   }
   cur ^= 1;
@@ -1058,14 +1091,25 @@ static int adjN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& 
                      b.Ny[cur], b.X[cur], b.Y[cur], b.M[cur ^ 1], b.Nx[cur ^ 1],          \
                      b.Ny[cur ^ 1], b.X[cur ^ 1], b.Y[cur ^ 1], it, adj, Fa, fstride, acc, \
                      astride, gw)
-  if (NS == 8)
-    GJW_ADJN_CALL(k_adj_step8);
-  else if (NS == 6)
-    GJW_ADJN_CALL(k_adj_step6);
-  else if (NS == 4)
-    GJW_ADJN_CALL(k_adj_step4);
-  else
-    GJW_ADJN_CALL(k_adj_step2);
+  if (s->fr16) {
+    if (NS == 8)
+      GJW_ADJN_CALL(k_adj_step8<2>);
+    else if (NS == 6)
+      GJW_ADJN_CALL(k_adj_step6<2>);
+    else if (NS == 4)
+      GJW_ADJN_CALL(k_adj_step4<2>);
+    else
+      GJW_ADJN_CALL(k_adj_step2<2>);
+  } else {
+    if (NS == 8)
+      GJW_ADJN_CALL(k_adj_step8<1>);
+    else if (NS == 6)
+      GJW_ADJN_CALL(k_adj_step6<1>);
+    else if (NS == 4)
+      GJW_ADJN_CALL(k_adj_step4<1>);
+    else
+      GJW_ADJN_CALL(k_adj_step2<1>);
+  }
 #undef GJW_ADJN_CALL
   HIPCHK(hipGetLastError());
   return kev_end(s, slot);
@@ -1272,6 +1316,19 @@ static int batch_adj_depth(const gjw_survey* s) {
 
 static long long ck_stride(const gjw_survey* s);
 
+// bytes per frame element: BF16 frames when asked for with FP32 snapshots
+// (the checkpoint intervals the library picks are even; frames_bf16 checks
+// the rest)
+static int frame_esize(const gjw_survey* s) {
+  return s->frames16 && s->snap == GJW_SNAP_F32 ? 2 : 4;
+}
+// BF16 frames for a gradient at interval K: every segment then has an even
+// number of steps, which the fused kernels cover without one-step launches
+// (the one-step kernels keep FP32 frames)
+static bool frames_bf16(const gjw_survey* s, int K) {
+  return frame_esize(s) == 2 && K % 2 == 0 && s->gr.nt % 2 == 0 && s->fuse_eff >= 2;
+}
+
 static int ck_interval(const gjw_survey* s) {
   if (s->ck_batch > 0) return s->ck_batch;  // common interval of a survey batch
   if (s->ck_every > 0) return s->ck_every;
@@ -1293,7 +1350,8 @@ static int ck_interval(const gjw_survey* s) {
     (void)hipGetLastError();
     return K;
   }
-  const double c = 4.0 * (double)ck_stride(s), f = 4.0 * s->ns * (double)s->LD * s->gr.n;
+  const double c = 4.0 * (double)ck_stride(s),
+               f = (double)frame_esize(s) * s->ns * (double)s->LD * s->gr.n;
   auto bytes = [&](int kk) { return std::ceil((double)nt / kk) * c + (kk + 1) * f; };
   if (bytes(K) <= 0.5 * (double)tot) return K;
   const int k0 = std::max(1, (int)std::lround(std::sqrt(nt * c / f)));
@@ -1747,11 +1805,24 @@ static int adj_init(gjw_survey* s, int nsh, int q0, const float* adj) {
 constexpr int kAdjCand4[] = {12, 16, 24, 32, 48}, kAdjCand6[] = {32, 48, 64, 96, 128};
 constexpr int kAdjNCand = 5, kAdjRounds = 3;
 
+static int grad_batched_run(gjw_survey* s, const float* adj, float* gw);
 static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
+  s->fr16 = frames_bf16(s, s->ck_K) ? 1 : 0;
+  const int rc = grad_batched_run(s, adj, gw);
+  s->fr16_used = s->fr16;
+  s->fr16 = 0;
+  return rc;
+}
+static int grad_batched_run(gjw_survey* s, const float* adj, float* gw) {
   const int ns = s->ns, nt = s->gr.nt, K = s->ck_K;
   const long long per = (long long)s->gr.m * s->gr.n, FD = (long long)s->LD * s->gr.n;
   hipStream_t st = s->ctx->stream;
-  TRY(s->frames.ensure((size_t)ns * (K + 1) * FD));
+  // FP32 frames: slot q of the segment at q * FD; BF16 pairs: word plane
+  // (q + 1) / 2 holds slots q and q + 1 for odd q, plane 0 slot 0 (the
+  // kernels' layout).  The frames of one shot are fstride floats apart.
+  const bool f16 = s->fr16 != 0;
+  const long long planes = f16 ? K / 2 + 1 : K + 1;
+  TRY(s->frames.ensure((size_t)ns * planes * FD));
   TRY(s->acc.ensure((size_t)2 * ns * FD));
   TRY(launch_fill(s->acc.p, 2LL * ns * FD, 0.f, st));
   TRY(adj_init(s, ns, 0, adj));
@@ -1759,7 +1830,7 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
   DevShots sh = dev_shots(s, 0);
   FwdBufs fb = fwd_bufs(s, ns);
   AdjBufs ab = adj_bufs(s, ns);
-  const long long fstride = (long long)(K + 1) * FD;  // frames per shot
+  const long long fstride = planes * FD;  // frames per shot
   int acur = 0;
   const int dmax = adj_depth(s);
   int& tuned_deep = dmax == 8 ? s->tuned_La8 : dmax == 6 ? s->tuned_La6 : s->tuned_La4;
@@ -1785,16 +1856,22 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
     int cur = 0, xc = 0;
     TRY(ck_restore(s, fb, cur, xc, c));
     hipLaunchKernelGGL(k_domain_frame, dim3((unsigned)cdiv(per * ns, 256)), dim3(256), 0, st, g, ns,
-                       fb.E[cur], s->frames.p, fstride);
+                       fb.E[cur], s->frames.p, fstride, s->fr16);
     HIPCHK(hipGetLastError());
     for (int it = k0; it < k1;) {
       int taken = 0;
-      TRY(fwd_advance(s, g, sh, ns, fb, cur, xc, it, k1 - it, nullptr,
-                      s->frames.p + (long long)(it + 1 - k0) * FD, FD, fstride, &taken));
+      // frames of steps it+1 ..: slot it+1-k0, or (BF16, it - k0 even) the
+      // plane of the pair (it+1-k0, it+2-k0)
+      float* fr = s->frames.p + (f16 ? (long long)((it - k0) / 2 + 1) : (long long)(it + 1 - k0)) * FD;
+      TRY(fwd_advance(s, g, sh, ns, fb, cur, xc, it, k1 - it, nullptr, fr, FD, fstride, &taken));
       it += taken;
     }
     for (int it = k1 - 1; it >= k0;) {
-      const float* F = s->frames.p + (long long)(it - k0) * FD;  // Ez(S_it)
+      const float* F = s->frames.p + (long long)(it - k0) * FD;  // Ez(S_it) (FP32 frames)
+      if (f16 && (it - k0) % 2 == 0)
+        return fail(GJW_ERR_STATE, "BF16 frames: an adjoint pass starts at an even slot");
+      // the frame of S_{it+1}: slot it+1-k0, or (BF16) the plane holding it
+      const float* Fa = f16 ? s->frames.p + (long long)((it - k0 + 2) / 2) * FD : F + FD;
       const int NS = (dmax >= 6 && it - (dmax - 1) >= k0)       ? dmax
                      : (s->fuse_eff >= 4 && it - 3 >= k0) ? 4
                      : (s->fuse_eff >= 2 && it - 1 >= k0) ? 2
@@ -1808,11 +1885,11 @@ static int grad_batched(gjw_survey* s, const float* adj, float* gw) {
         if (trial) L = (dmax >= 6 ? kAdjCand6 : kAdjCand4)[ex % kAdjNCand];
         if (L <= 0) L = La_fallback;
         if (trial) HIPCHK(hipEventRecord(s->ex_ev[2 * ex], st));
-        TRY(adjN_launch(s, NS, g, sh, ns, L, ab, acur, it, adj, F + FD, fstride, s->acc.p, FD,
-                        gw));
+        TRY(adjN_launch(s, NS, g, sh, ns, L, ab, acur, it, adj, Fa, fstride, s->acc.p, FD, gw));
         if (trial) HIPCHK(hipEventRecord(s->ex_ev[2 * ex++ + 1], st));
         it -= NS;
       } else {
+        if (s->fr16) return fail(GJW_ERR_STATE, "BF16 frames reached a one-step adjoint launch");
         TRY(adj_launch(s, g, sh, ns, s->tuned_L1, ab, acur, it, adj, F + FD, F, fstride,
                        s->acc.p, FD, gw));
         it -= 1;
@@ -2091,7 +2168,8 @@ static int fwi_batched_prepare(gjw_survey* s) {
   TRY(s->Jd.ensure(1));
   TRY(s->ge.ensure(grad_len(s)));
   TRY(s->gsg.ensure(grad_len(s)));
-  TRY(s->frames.ensure((size_t)ns * (s->ck_K + 1) * FD));
+  TRY(s->frames.ensure((size_t)ns *
+                       (frames_bf16(s, s->ck_K) ? s->ck_K / 2 + 1 : s->ck_K + 1) * FD));
   TRY(s->acc.ensure((size_t)2 * ns * FD));
   TRY(s->adjf.ensure((size_t)10 * ns * s->FS));
   return GJW_OK;
@@ -2175,7 +2253,7 @@ static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
           u(s->acc.p),    u(s->adjf.p),  u(s->coef.p),   u(s->fe.p),    u(s->fs.p),
           u(s->d_pml.p),  u(s->d_si.p),  u(s->d_sj.p),   u(s->d_sc.p),  u(s->d_wav.p),
           u(s->d_cptr.p), u(s->d_crow.p), u(s->d_cid.p), u(s->d_trs.p), u(s->d_trr.p),
-          u(s->d_trc.p),  (uintptr_t)s->ns, (uintptr_t)s->ntr, (uintptr_t)s->shot_grads,
+          u(s->d_trc.p),  (uintptr_t)s->frames16, (uintptr_t)s->ns, (uintptr_t)s->ntr, (uintptr_t)s->shot_grads,
           (uintptr_t)s->gr.nt, (uintptr_t)s->tuned_L, (uintptr_t)s->tuned_L1,
           (uintptr_t)s->tuned_L4, (uintptr_t)s->tuned_L8, (uintptr_t)s->fuse_eff,
           (uintptr_t)s->tuned_La2,
@@ -2502,7 +2580,7 @@ static int batch_adj_enqueue(gjw_batch* b) {
       const long long per = (long long)s->gr.m * s->gr.n;
       hipLaunchKernelGGL(k_domain_frame, dim3((unsigned)cdiv(per * s->ns, 256)), dim3(256), 0, st,
                          dev_grid(s), s->ns, fb.E[0], s->frames.p,
-                         (long long)(K + 1) * s->LD * s->gr.n);
+                         (long long)(K + 1) * s->LD * s->gr.n, 0);
       HIPCHK(hipGetLastError());
     }
     for (int tau = k0; tau < kmax;) {
@@ -2945,6 +3023,9 @@ extern "C" int gjw_query(gjw_survey* s, int32_t what, int64_t* value) {
       return GJW_OK;
     case GJW_Q_ADJ_DEPTH:
       *value = s->path == GJW_PATH_PLAIN ? 1 : s->fuse_eff > 0 ? adj_depth(s) : 0;
+      return GJW_OK;
+    case GJW_Q_FRAMES:
+      *value = s->fr16_used ? GJW_FRAMES_BF16 : GJW_FRAMES_F32;
       return GJW_OK;
     default:
       return fail(GJW_ERR_ARG, "gjw_query: unknown query %d", what);

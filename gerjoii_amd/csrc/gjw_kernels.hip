@@ -50,6 +50,25 @@ __device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsign
                                         0, 0);
 }
 
+// Ez frames of the recompute sweep: FP32 (frame kind FR = 1), or BF16 pairs
+// (FR = 2): the BF16 values (upper half of the FP32 bits, rounded to nearest
+// even) of two consecutive steps in one 32-bit word, the earlier step in the
+// low half.  Word plane q of a segment holds the frames of its slots 2q-1
+// (low) and 2q (high); slot 0, the segment's first frame, is the high half
+// of plane 0.  One store instruction per two frames halves the recompute
+// sweep's frame stores, the instruction count that bounds it.
+// Both conversions are one v_cvt_pk_bf16_f32 (round to nearest even), so a
+// frame written alone (slot 0) and in a pair round alike.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned frame_pair_bits(float lo, float hi) {
+  const f32x2_t v = {lo, hi};
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2_t));
+}
+__device__ __forceinline__ unsigned short frame_bf16(float v) {
+  return __builtin_bit_cast(unsigned short, (__bf16)v);
+}
+
 // --------------------------------------------------------------------------
 // per-cell coefficient selection (w_solve.m:144-147 inner, :170-186 strips)
 // --------------------------------------------------------------------------
@@ -530,7 +549,7 @@ __device__ __forceinline__ CoefOff coef_off(const DevGrid& g) {
 #define GJW_FWD8_WAVES 3
 #endif
 
-// FRAMES: the recompute sweep stores Ez on the domain after both steps.
+// FR: the recompute sweep stores Ez frames of the domain (1: FP32, 2: BF16).
 //
 // Column handling.  Step-1 columns jb-1 .. je are computed (step 2 runs one
 // column behind; the first two produce only what the strip's first output
@@ -552,9 +571,9 @@ __device__ __forceinline__ CoefOff coef_off(const DevGrid& g) {
 // so every level reads its inputs from registers: the field values of the
 // previous level's current column and of its own previous column.  Only the
 // last level stores the fields; every level records receivers, the source
-// recorder and (FRAMES, recompute sweep) its domain frame, slot l at
+// recorder and (FR, recompute sweep) its domain frame, slot l at
 // frame1 + l * LD * n.
-template <int NS, bool FRAMES>
+template <int NS, int FR>
 __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, int L, TileId tid,
                                           const float* __restrict__ E0,
                                           const float* __restrict__ Hx0,
@@ -606,8 +625,38 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   const __amdgpu_buffer_rsrc_t rJZ =
       rsrc_of(s.jz ? s.jz + (long long)shot * s.nt : nullptr, s.jz ? s.nt : 0);
   // domain frames of the NS steps (slot l at l * FD)
+  // domain frames of the NS steps: slot l at l * FD (FR = 1), or the pairs of
+  // slots (2p, 2p+1) in word plane p (FR = 2)
+  // (FR = 2: the descriptor starts NS * LD words early, so that a level's
+  // column lag, -l * LD, rides in the scalar offset with the plane)
   const __amdgpu_buffer_rsrc_t rFR =
-      rsrc_of(FRAMES ? frame1 + (long long)shot * fstride : nullptr, FRAMES ? NS * FD : 0);
+      FR == 2 ? rsrc_of(frame1 + (long long)shot * fstride - NS * LD, NS / 2 * FD + NS * LD)
+              : rsrc_of(FR ? frame1 + (long long)shot * fstride : nullptr, FR ? NS * FD : 0);
+  // FR = 2, interior tiles: the frame of each even level is held a column
+  // (fh), until the odd level after it reaches that column and stores the
+  // pair in one 32-bit store; the even level's value of this column (fhn)
+  // replaces it after that store.  The other paths, whose registers are
+  // spoken for, store each BF16 half alone (frame_single).
+  float fh[NS / 2 > 0 ? NS / 2 : 1], fhn[NS / 2 > 0 ? NS / 2 : 1];
+#pragma unroll
+  for (int m = 0; m < NS / 2; ++m) fh[m] = fhn[m] = 0.f;
+  // level l's frame at column c - l: v0 = (c - dj0) * LD + row, the lanes'
+  // offset of level 0's column; plane l / 2 and the column lag in the scalar
+  // offset
+  auto fsoff = [&](int l) { return (unsigned)(l / 2 * FD + (NS - l) * LD) * 4u; };
+  auto frame_single = [&](int l, float e, unsigned v0, bool on) {
+    __builtin_amdgcn_raw_buffer_store_b16(frame_bf16(e), rFR, (int)(on ? v0 * 4u : kOOB),
+                                          (int)(fsoff(l) + 2u * (unsigned)(l & 1)), 0);
+  };
+  auto frame_pair = [&](int l, float e, unsigned v0, bool on) {
+    if (l % 2 == 0) {
+      fhn[l / 2] = e;
+    } else {
+      __builtin_amdgcn_raw_buffer_store_b32(frame_pair_bits(fh[l / 2], e), rFR,
+                                            (int)(on ? v0 * 4u : kOOB), (int)fsoff(l), 0);
+      fh[l / 2] = fhn[l / 2];
+    }
+  };
 
   const __amdgpu_buffer_rsrc_t rE0 = rsrc_of(const_cast<float*>(E0), g.FS),
                                rHx0 = rsrc_of(const_cast<float*>(Hx0), g.FS),
@@ -684,8 +733,10 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       }
     }
     bstore(e, rJZ, (unsigned)(it + l), ol && i == si && cl == sj);
-    if (FRAMES)
+    if (FR == 1)
       bstore(e, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + ic), ol && dom_row && dom_col(cl));
+    if (FR == 2)
+      frame_single(l, e, (unsigned)((cl + l - g.dj0) * LD + ic), ol && dom_row && dom_col(cl));
   };
   // Levels whose column c-l is one of the strip's own columns [jb, je), as
   // bit l of a wave-uniform mask (one range per column instead of two
@@ -709,9 +760,10 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       }
     }
     bstore(e, rJZ, (unsigned)(it + l), ol && i == si && cl == sj);
-    if (FRAMES)
+    if (FR == 1)
       bstorek(e, rFR, (unsigned)(l * (FD - LD)) * 4u, (unsigned)((cl + l - g.dj0) * LD + ic),
               own_dom && on);
+    if (FR == 2) frame_single(l, e, (unsigned)((cl + l - g.dj0) * LD + ic), own_dom && on);
   };
 
   // ---- state.  Level 0 steps column c from the loaded fields; level l >= 1
@@ -877,7 +929,8 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   // ---- interior chunk, all rows inner: scalar H coefficients, (Ceze, Cezhy) loads.
   // SRC: the wave's rows hold the source row (injection and source recorder);
   // the other waves carry neither.
-  auto inner_compute = [&](int jj, const InnerBuf& b, auto SRC) {
+  // PAIRS: the interior tiles' own loop (BF16 frames stored in pairs)
+  auto inner_compute = [&](int jj, const InnerBuf& b, auto SRC, auto PAIRS) {
     constexpr bool src = decltype(SRC)::value;
 #pragma unroll
     for (int u = 0; u < UI; ++u) {
@@ -885,7 +938,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       float eo[NS], hxo[NS], hyo[NS], zo[NS];
       if (src) rid_l[0] = rec_lookup(c);
       // per-level stores (frames, receivers, source recorder) test the level mask
-      const unsigned lv = (FRAMES || src) ? level_mask(c) : 0u;
+      const unsigned lv = (FR || src) ? level_mask(c) : 0u;
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
         const int cl = c - l;
@@ -910,9 +963,15 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
         // interior columns are columns of the stored domain (see level_out_in)
         // level l's frame: plane l, column c-l; the per-level part l (FD - LD)
         // rides in the scalar offset, the lanes share one offset per column
-        if (FRAMES)
+        if (FR == 1)
           bstorek(e1, rFR, (unsigned)(l * (FD - LD)) * 4u, (unsigned)((c - g.dj0) * LD + i),
                   own_dom && on);
+        if (FR == 2) {
+          if (decltype(PAIRS)::value)
+            frame_pair(l, e1, (unsigned)((c - g.dj0) * LD + i), own_dom && on);
+          else
+            frame_single(l, e1, (unsigned)((c - g.dj0) * LD + i), own_dom && on);
+        }
         if (l == NS - 1) {
           const bool ol = own && cl >= jb && cl < je;
           const unsigned k2 = (unsigned)(cl * LD + i);
@@ -1027,11 +1086,11 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     inner_load(j, A, true);
     while (j <= jlast) {
       inner_load(j + UI, B, j + UI <= jlast);
-      inner_compute(j, A, std::false_type{});
+      inner_compute(j, A, std::false_type{}, std::true_type{});
       j += UI;
       if (j > jlast) break;
       inner_load(j + UI, A, j + UI <= jlast);
-      inner_compute(j, B, std::false_type{});
+      inner_compute(j, B, std::false_type{}, std::true_type{});
       j += UI;
     }
     return;
@@ -1046,11 +1105,11 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       inner_load(j, A, j < jz);
       while (j < jz) {
         inner_load(j + UI, B, j + UI < jz);
-        inner_compute(j, A, SRC);
+        inner_compute(j, A, SRC, std::false_type{});
         j += UI;
         if (j >= jz) break;
         inner_load(j + UI, A, j + UI < jz);
-        inner_compute(j, B, SRC);
+        inner_compute(j, B, SRC, std::false_type{});
         j += UI;
       }
     };
@@ -1067,7 +1126,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   }
 }
 
-template <bool FRAMES>
+template <int FR>
 __global__ void __launch_bounds__(256, GJW_FWD_WAVES)
 k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
             const float* __restrict__ Hx0, const float* __restrict__ Hy0,
@@ -1075,11 +1134,11 @@ k_fwd_step2(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
             float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
             float* __restrict__ Y2, int it, float* __restrict__ tr, float* __restrict__ frame1,
             long long fstride) {
-  fwdN_body<2, FRAMES>(g, s, L, vtile_of_block(g.NX, g.NY, L, 60, true), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
+  fwdN_body<2, FR>(g, s, L, vtile_of_block(g.NX, g.NY, L, 60, true), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
                        Hy2, X2, Y2, it, tr, frame1, fstride);
 }
 
-template <bool FRAMES>
+template <int FR>
 __global__ void __launch_bounds__(256, GJW_FWD_WAVES)
 k_fwd_step4(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
             const float* __restrict__ Hx0, const float* __restrict__ Hy0,
@@ -1087,11 +1146,11 @@ k_fwd_step4(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
             float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
             float* __restrict__ Y2, int it, float* __restrict__ tr, float* __restrict__ frame1,
             long long fstride) {
-  fwdN_body<4, FRAMES>(g, s, L, vtile_of_block(g.NX, g.NY, L, 56, true), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
+  fwdN_body<4, FR>(g, s, L, vtile_of_block(g.NX, g.NY, L, 56, true), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
                        Hy2, X2, Y2, it, tr, frame1, fstride);
 }
 
-template <bool FRAMES>
+template <int FR>
 __global__ void __launch_bounds__(256, GJW_FWD8_WAVES)
 k_fwd_step8(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
             const float* __restrict__ Hx0, const float* __restrict__ Hy0,
@@ -1099,7 +1158,7 @@ k_fwd_step8(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
             float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
             float* __restrict__ Y2, int it, float* __restrict__ tr, float* __restrict__ frame1,
             long long fstride) {
-  fwdN_body<8, FRAMES>(g, s, L, vtile_of_block(g.NX, g.NY, L, 48, true), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
+  fwdN_body<8, FR>(g, s, L, vtile_of_block(g.NX, g.NY, L, 48, true), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
                        Hy2, X2, Y2, it, tr, frame1, fstride);
 }
 
@@ -1113,7 +1172,7 @@ __device__ __forceinline__ void fwd_multi_body(const MultiTable& tab, const Mult
   float* f = S.fld;
   const long long z = S.fsz;
   float* fr1 = FRAMES ? S.frames + (long long)a.fslot[k] * S.FD : nullptr;
-  fwdN_body<NS, FRAMES>(S.g, S.s, a.L[k], tid, f + 3 * c * z, f + (3 * c + 1) * z,
+  fwdN_body<NS, FRAMES ? 1 : 0>(S.g, S.s, a.L[k], tid, f + 3 * c * z, f + (3 * c + 1) * z,
                         f + (3 * c + 2) * z, f + (6 + 2 * x) * z, f + (7 + 2 * x) * z,
                         f + 3 * (c ^ 1) * z, f + (3 * (c ^ 1) + 1) * z, f + (3 * (c ^ 1) + 2) * z,
                         f + (6 + 2 * (x ^ 1)) * z, f + (7 + 2 * (x ^ 1)) * z, a.it[k],
@@ -1137,12 +1196,15 @@ template __global__ void k_fwd_step8_multi<true>(const MultiTable, MultiArgs);
 
 #define GJW_FWDN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \
     const float*, const float*, float*, float*, float*, float*, float*, int, float*, float*, long long
-template __global__ void k_fwd_step2<false>(GJW_FWDN_ARGS);
-template __global__ void k_fwd_step2<true>(GJW_FWDN_ARGS);
-template __global__ void k_fwd_step4<false>(GJW_FWDN_ARGS);
-template __global__ void k_fwd_step4<true>(GJW_FWDN_ARGS);
-template __global__ void k_fwd_step8<false>(GJW_FWDN_ARGS);
-template __global__ void k_fwd_step8<true>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step2<0>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step2<1>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step2<2>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step4<0>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step4<1>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step4<2>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step8<0>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step8<1>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step8<2>(GJW_FWDN_ARGS);
 #undef GJW_FWDN_ARGS
 template __global__ void k_fwd_stepN_multi<2, false>(const MultiTable, MultiArgs);
 template __global__ void k_fwd_stepN_multi<2, true>(const MultiTable, MultiArgs);
@@ -1433,7 +1495,14 @@ k_adj_step_multi(const MultiTable tab, MultiArgs a) {
 // accumulated once per pass, at the last level's column, from each level's
 // input M (kept for the levels' column lag) and the NS+1 frames:
 //   acc.x += sum_l M_l (F_l - F_{l+1}),  acc.y += sum_l M_l (F_l + F_{l+1}).
-template <int NS>
+// FR = 2: BF16 frames.  Then acc.x is summed by parts,
+//   acc.x += M_0 F_0 + sum_{l>=1} (M_l - M_{l-1}) F_l - M_{NS-1} F_NS,
+// the same sum in exact arithmetic: the difference is taken of M (FP32, in
+// registers) instead of the rounded frames, whose consecutive values differ
+// by a few percent (their difference would carry the frames' 2^-9 rounding
+// amplified by that factor).  A pass's last frame is the next pass's first,
+// the same stored value, so the boundary terms cancel across passes.
+template <int NS, int FR>
 __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, int L, TileId tid,
                                           const float* __restrict__ M0,
                                           const float* __restrict__ Nx0,
@@ -1491,8 +1560,12 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
                                rY2 = rsrc_of(Y2, g.FS),
                                rK = rsrc_of(const_cast<float*>(g.coef), 19 * g.FS),
                                // frames of S_{it-NS+1} .. S_{it+1}: slot k at k * FD
-                               rFR = rsrc_of(const_cast<float*>(Fa - NS * FD + (long long)shot * fstride),
-                                             (NS + 1) * FD),
+                               // FR = 1: frames of S_{it-NS+1} .. S_{it+1}, slot k at k * FD;
+                               // FR = 2: the NS/2 + 1 word planes that hold them
+                               rFR = FR == 2 ? rsrc_of(const_cast<float*>(Fa - NS / 2 * FD + (long long)shot * fstride),
+                                                       (NS / 2 + 1) * FD)
+                                             : rsrc_of(const_cast<float*>(Fa - NS * FD + (long long)shot * fstride),
+                                                       (NS + 1) * FD),
                                rAC = rsrc_of(ac, ac ? 2 * FD : 0),
                                rGW = rsrc_of(gw ? gw + (long long)shot * s.nt : nullptr,
                                              gw ? s.nt : 0);
@@ -1623,10 +1696,19 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
 #pragma unroll
     for (int l = 0; l < NS; ++l) mz[l] = l == NS - 1 ? me_in[l] : mh[l][NS - 2 - l];
     const bool inner_z = e_inner && cz >= P + 1 && cz <= NY - P - 1;
+    if constexpr (FR == 2) {
+      t.x = t.x + mz[0] * f[0];
 #pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      t.x = t.x + mz[l] * (f[l] - f[l + 1]);
-      t.y = t.y + mz[l] * (f[l] + f[l + 1]);
+      for (int l = 1; l < NS; ++l) t.x = t.x + (mz[l] - mz[l - 1]) * f[l];
+      t.x = t.x - mz[NS - 1] * f[NS];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) t.y = t.y + mz[l] * (f[l] + f[l + 1]);
+    } else {
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        t.x = t.x + mz[l] * (f[l] - f[l + 1]);
+        t.y = t.y + mz[l] * (f[l] + f[l + 1]);
+      }
     }
     bstore2(t, rAC, 2 * (unsigned)((cz - g.dj0) * LD + ic), ac && inner_z && corr_on(cz));
     if (decltype(SRC)::value && src_rows) {  // wave-uniform: a scalar branch
@@ -1660,8 +1742,20 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   // so the NS+1 loads share one per-lane offset)
   auto load_corr = [&](int cz, float* f, float2& t, bool on) {
     const unsigned fk = (unsigned)((cz - g.dj0) * LD + ic);
+    if constexpr (FR == 2) {
+      // it - k0 is odd in every pass: S_{it+1-2m} is the high half of plane
+      // m back from Fa, S_{it-2m} the low half
 #pragma unroll
-    for (int k = 0; k <= NS; ++k) f[k] = bloadk(rFR, (unsigned)((NS - k) * FD) * 4u, fk, on);
+      for (int m = 0; m <= NS / 2; ++m) {
+        const unsigned w =
+            __float_as_uint(bloadk(rFR, (unsigned)((NS / 2 - m) * FD) * 4u, fk, on));
+        f[2 * m] = __uint_as_float(w & 0xFFFF0000u);
+        if (2 * m + 1 <= NS) f[2 * m + 1] = __uint_as_float(w << 16);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k <= NS; ++k) f[k] = bloadk(rFR, (unsigned)((NS - k) * FD) * 4u, fk, on);
+    }
     t = bload2(rAC, 2 * fk, on);
   };
 
@@ -1954,6 +2048,7 @@ __device__ __forceinline__ void adjN_body(const DevGrid& g, const DevShots& s, i
   for (; j <= jlast; j += UA) univ_chunk(j);
 }
 
+template <int FR>
 __global__ void __launch_bounds__(256, GJW_ADJ_WAVES)
 k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             const float* __restrict__ Nx0, const float* __restrict__ Ny0,
@@ -1962,10 +2057,11 @@ k_adj_step2(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             float* __restrict__ Y2, int it, const float* __restrict__ adj,
             const float* __restrict__ Fa, long long fstride, float* __restrict__ acc,
             long long astride, float* __restrict__ gw) {
-  adjN_body<2>(g, s, L, vtile_of_block(g.NX, g.NY, L, 60, false), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
+  adjN_body<2, FR>(g, s, L, vtile_of_block(g.NX, g.NY, L, 60, false), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
                Y2, it, adj, Fa, fstride, acc, astride, gw);
 }
 
+template <int FR>
 __global__ void __launch_bounds__(256, GJW_ADJ_WAVES)
 k_adj_step4(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             const float* __restrict__ Nx0, const float* __restrict__ Ny0,
@@ -1974,10 +2070,11 @@ k_adj_step4(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             float* __restrict__ Y2, int it, const float* __restrict__ adj,
             const float* __restrict__ Fa, long long fstride, float* __restrict__ acc,
             long long astride, float* __restrict__ gw) {
-  adjN_body<4>(g, s, L, vtile_of_block(g.NX, g.NY, L, 56, false), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
+  adjN_body<4, FR>(g, s, L, vtile_of_block(g.NX, g.NY, L, 56, false), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
                Y2, it, adj, Fa, fstride, acc, astride, gw);
 }
 
+template <int FR>
 __global__ void __launch_bounds__(256, GJW_ADJ6_WAVES)
 k_adj_step6(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             const float* __restrict__ Nx0, const float* __restrict__ Ny0,
@@ -1986,10 +2083,11 @@ k_adj_step6(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             float* __restrict__ Y2, int it, const float* __restrict__ adj,
             const float* __restrict__ Fa, long long fstride, float* __restrict__ acc,
             long long astride, float* __restrict__ gw) {
-  adjN_body<6>(g, s, L, vtile_of_block(g.NX, g.NY, L, 52, false), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
+  adjN_body<6, FR>(g, s, L, vtile_of_block(g.NX, g.NY, L, 52, false), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
                Y2, it, adj, Fa, fstride, acc, astride, gw);
 }
 
+template <int FR>
 __global__ void __launch_bounds__(256, GJW_ADJ8_WAVES)
 k_adj_step8(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             const float* __restrict__ Nx0, const float* __restrict__ Ny0,
@@ -1998,7 +2096,7 @@ k_adj_step8(DevGrid g, DevShots s, int L, const float* __restrict__ M0,
             float* __restrict__ Y2, int it, const float* __restrict__ adj,
             const float* __restrict__ Fa, long long fstride, float* __restrict__ acc,
             long long astride, float* __restrict__ gw) {
-  adjN_body<8>(g, s, L, vtile_of_block(g.NX, g.NY, L, 48, false), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
+  adjN_body<8, FR>(g, s, L, vtile_of_block(g.NX, g.NY, L, 48, false), M0, Nx0, Ny0, X0, Y0, M2, Nx2, Ny2, X2,
                Y2, it, adj, Fa, fstride, acc, astride, gw);
 }
 
@@ -2013,11 +2111,23 @@ k_adj_stepN_multi(const MultiTable tab, MultiArgs a) {
   float* m = S.adjf;
   const long long z = S.fsz;
   const float* Fa = S.frames + (long long)(a.fslot[k] + 1) * S.FD;
-  adjN_body<NS>(S.g, S.s, a.L[k], tid, m + 5 * c * z, m + (5 * c + 1) * z, m + (5 * c + 2) * z,
+  adjN_body<NS, 1>(S.g, S.s, a.L[k], tid, m + 5 * c * z, m + (5 * c + 1) * z, m + (5 * c + 2) * z,
                 m + (5 * c + 3) * z, m + (5 * c + 4) * z, m + 5 * d * z, m + (5 * d + 1) * z,
                 m + (5 * d + 2) * z, m + (5 * d + 3) * z, m + (5 * d + 4) * z, a.it[k], S.adj, Fa,
                 S.fstride, S.acc, S.FD, nullptr);
 }
+#define GJW_ADJN_ARGS DevGrid, DevShots, int, const float*, const float*, const float*, \
+    const float*, const float*, float*, float*, float*, float*, float*, int, const float*, \
+    const float*, long long, float*, long long, float*
+template __global__ void k_adj_step2<1>(GJW_ADJN_ARGS);
+template __global__ void k_adj_step2<2>(GJW_ADJN_ARGS);
+template __global__ void k_adj_step4<1>(GJW_ADJN_ARGS);
+template __global__ void k_adj_step4<2>(GJW_ADJN_ARGS);
+template __global__ void k_adj_step6<1>(GJW_ADJN_ARGS);
+template __global__ void k_adj_step6<2>(GJW_ADJN_ARGS);
+template __global__ void k_adj_step8<1>(GJW_ADJN_ARGS);
+template __global__ void k_adj_step8<2>(GJW_ADJN_ARGS);
+#undef GJW_ADJN_ARGS
 template __global__ void k_adj_stepN_multi<2>(const MultiTable, MultiArgs);
 template __global__ void k_adj_stepN_multi<4>(const MultiTable, MultiArgs);
 template __global__ void k_adj_stepN_multi<6>(const MultiTable, MultiArgs);
@@ -2251,16 +2361,21 @@ __global__ void k_misfit_stage2(const float* __restrict__ P, const float2* __res
 }
 
 // Ez(S) on the domain -> frame (used for the frame of a restored checkpoint)
+// (bf16: frames of kind 2, rounded as the recompute sweep's frame stores)
 __global__ void k_domain_frame(DevGrid g, int ns, const float* __restrict__ E,
-                               float* __restrict__ frame, long long fstride) {
+                               float* __restrict__ frame, long long fstride, int bf16) {
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const long long per = (long long)g.m * g.n;
   if (t >= per * ns) return;
   const int sh = (int)(t / per);
   const long long r = t % per;
   const int b = (int)(r / g.m), a = (int)(r % g.m);
-  frame[(long long)sh * fstride + (long long)b * g.LD + (a + g.di0)] =
-      E[(long long)sh * g.FS + (long long)(b + g.dj0) * g.LD + (a + g.di0)];
+  const long long o = (long long)sh * fstride + (long long)b * g.LD + (a + g.di0);
+  const float v = E[(long long)sh * g.FS + (long long)(b + g.dj0) * g.LD + (a + g.di0)];
+  if (bf16)  // slot 0: the high half of word plane 0
+    reinterpret_cast<unsigned short*>(frame)[2 * o + 1] = frame_bf16(v);
+  else
+    frame[o] = v;
 }
 
 // acc (float2 per domain cell per shot) -> FP64 gradient, summed over shots

@@ -111,6 +111,16 @@ int gjw_set_shots(gjw_survey *s, int32_t ns, const int32_t *src, const float *wa
                                 batched path; 0 (default): tuned in the first gradient sweep
                                 (the widest of the candidates within 1.5% of the fastest).
                                 A fixed width makes gradients repeat bit for bit across runs */
+#define GJW_OPT_FRAMES 12    /* element type of the Ez frames the recompute sweep hands to the
+                                adjoint on the batched path: GJW_FRAMES_F32 (default) or
+                                GJW_FRAMES_BF16 (half the frame traffic; the adjoint then sums
+                                its frame correlation by parts so that no difference of rounded
+                                frames enters the gradient).  BF16 frames apply to single-survey
+                                gradients with FP32 snapshots, an even nt and checkpoint
+                                interval and at least two fused forward steps; other gradients
+                                keep FP32 frames (gjw_query GJW_Q_FRAMES tells which was used) */
+#define GJW_FRAMES_F32 0
+#define GJW_FRAMES_BF16 1
 #define GJW_PATH_BATCHED 0   /* all shots per launch, checkpoint + recompute        */
 #define GJW_PATH_PLAIN 1     /* one shot at a time, full wavefield stored (as w_solve.m) */
 #define GJW_SNAP_F32 0
@@ -328,11 +338,14 @@ int gjw_tile(gjw_survey *s, int32_t which, int32_t *L);
  * the batched path uses (the option, else about 2 sqrt(nt) in whole launches,
  * else the least-memory interval; 0 on the plain path); GJW_Q_FWD_DEPTH and
  * GJW_Q_ADJ_DEPTH, the forward and adjoint steps per launch (0 before the
- * first forward has tuned them).  Has no counterpart in the reference (the
- * MATLAB code stores the whole wavefield and steps one at a time). */
+ * first forward has tuned them); GJW_Q_FRAMES, the element type (GJW_FRAMES_*)
+ * of the frames of the last batched gradient.  Has no counterpart in the
+ * reference (the MATLAB code stores the whole wavefield and steps one at a
+ * time). */
 #define GJW_Q_CKPT_INTERVAL 1
 #define GJW_Q_FWD_DEPTH 2
 #define GJW_Q_ADJ_DEPTH 3
+#define GJW_Q_FRAMES 4 /* GJW_FRAMES_* of the last batched gradient's frames */
 int gjw_query(gjw_survey *s, int32_t what, int64_t *value);
 
 #ifdef __cplusplus

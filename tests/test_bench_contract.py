@@ -28,6 +28,8 @@ def _profile(name):
                 vals["sha"] = line.split(":")[1].strip()
             elif "kernel-trace average at this grid:" in line:
                 vals["ms"] = float(line.split(":")[1].split("ms")[0])
+            elif line.startswith("frames:"):
+                vals["frames"] = line.split(":")[1].strip()
     return vals
 
 
@@ -52,7 +54,7 @@ def test_compulsory_bytes_bound_the_measured_traffic():
         if p.get("sha") != sha:
             continue  # a summary of other kernel sources says nothing about these
         S, kind = kinds[name]
-        comp = bench.compulsory_bytes(kind, S, g, ns, ntr)
+        comp = bench.compulsory_bytes(kind, S, g, ns, ntr, 2 if p.get("frames") == "bf16" else 4)
         cells = (g.nx + 1) * (g.ny + 1) * ns
         assert comp <= p["bytes"], (name, comp, p["bytes"])
         assert bench.BYTES_PER_CELL_UPDATE * S * cells > p["bytes"]  # the single-pass model overcounts
