@@ -632,11 +632,11 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   const __amdgpu_buffer_rsrc_t rFR =
       FR == 2 ? rsrc_of(frame1 + (long long)shot * fstride - NS * LD, NS / 2 * FD + NS * LD)
               : rsrc_of(FR ? frame1 + (long long)shot * fstride : nullptr, FR ? NS * FD : 0);
-  // FR = 2, interior tiles: the frame of each even level is held a column
-  // (fh), until the odd level after it reaches that column and stores the
-  // pair in one 32-bit store; the even level's value of this column (fhn)
-  // replaces it after that store.  The other paths, whose registers are
-  // spoken for, store each BF16 half alone (frame_single).
+  // FR = 2: the frame of each even level is held a column (fh), until the
+  // odd level after it reaches that column and stores the pair in one 32-bit
+  // store; the even level's value of this column (fhn) replaces it after
+  // that store.  Every chunk kind runs every level of every column, so the
+  // held values carry across the kinds.
   float fh[NS / 2 > 0 ? NS / 2 : 1], fhn[NS / 2 > 0 ? NS / 2 : 1];
 #pragma unroll
   for (int m = 0; m < NS / 2; ++m) fh[m] = fhn[m] = 0.f;
@@ -644,10 +644,6 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   // offset of level 0's column; plane l / 2 and the column lag in the scalar
   // offset
   auto fsoff = [&](int l) { return (unsigned)(l / 2 * FD + (NS - l) * LD) * 4u; };
-  auto frame_single = [&](int l, float e, unsigned v0, bool on) {
-    __builtin_amdgcn_raw_buffer_store_b16(frame_bf16(e), rFR, (int)(on ? v0 * 4u : kOOB),
-                                          (int)(fsoff(l) + 2u * (unsigned)(l & 1)), 0);
-  };
   auto frame_pair = [&](int l, float e, unsigned v0, bool on) {
     if (l % 2 == 0) {
       fhn[l / 2] = e;
@@ -736,7 +732,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     if (FR == 1)
       bstore(e, rFR, (unsigned)(l * FD + (cl - g.dj0) * LD + ic), ol && dom_row && dom_col(cl));
     if (FR == 2)
-      frame_single(l, e, (unsigned)((cl + l - g.dj0) * LD + ic), ol && dom_row && dom_col(cl));
+      frame_pair(l, e, (unsigned)((cl + l - g.dj0) * LD + ic), ol && dom_row && dom_col(cl));
   };
   // Levels whose column c-l is one of the strip's own columns [jb, je), as
   // bit l of a wave-uniform mask (one range per column instead of two
@@ -763,7 +759,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     if (FR == 1)
       bstorek(e, rFR, (unsigned)(l * (FD - LD)) * 4u, (unsigned)((cl + l - g.dj0) * LD + ic),
               own_dom && on);
-    if (FR == 2) frame_single(l, e, (unsigned)((cl + l - g.dj0) * LD + ic), own_dom && on);
+    if (FR == 2) frame_pair(l, e, (unsigned)((cl + l - g.dj0) * LD + ic), own_dom && on);
   };
 
   // ---- state.  Level 0 steps column c from the loaded fields; level l >= 1
@@ -929,8 +925,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
   // ---- interior chunk, all rows inner: scalar H coefficients, (Ceze, Cezhy) loads.
   // SRC: the wave's rows hold the source row (injection and source recorder);
   // the other waves carry neither.
-  // PAIRS: the interior tiles' own loop (BF16 frames stored in pairs)
-  auto inner_compute = [&](int jj, const InnerBuf& b, auto SRC, auto PAIRS) {
+  auto inner_compute = [&](int jj, const InnerBuf& b, auto SRC) {
     constexpr bool src = decltype(SRC)::value;
 #pragma unroll
     for (int u = 0; u < UI; ++u) {
@@ -966,12 +961,7 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
         if (FR == 1)
           bstorek(e1, rFR, (unsigned)(l * (FD - LD)) * 4u, (unsigned)((c - g.dj0) * LD + i),
                   own_dom && on);
-        if (FR == 2) {
-          if (decltype(PAIRS)::value)
-            frame_pair(l, e1, (unsigned)((c - g.dj0) * LD + i), own_dom && on);
-          else
-            frame_single(l, e1, (unsigned)((c - g.dj0) * LD + i), own_dom && on);
-        }
+        if (FR == 2) frame_pair(l, e1, (unsigned)((c - g.dj0) * LD + i), own_dom && on);
         if (l == NS - 1) {
           const bool ol = own && cl >= jb && cl < je;
           const unsigned k2 = (unsigned)(cl * LD + i);
@@ -1086,11 +1076,11 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
     inner_load(j, A, true);
     while (j <= jlast) {
       inner_load(j + UI, B, j + UI <= jlast);
-      inner_compute(j, A, std::false_type{}, std::true_type{});
+      inner_compute(j, A, std::false_type{});
       j += UI;
       if (j > jlast) break;
       inner_load(j + UI, A, j + UI <= jlast);
-      inner_compute(j, B, std::false_type{}, std::true_type{});
+      inner_compute(j, B, std::false_type{});
       j += UI;
     }
     return;
@@ -1105,11 +1095,11 @@ __device__ __forceinline__ void fwdN_body(const DevGrid& g, const DevShots& s, i
       inner_load(j, A, j < jz);
       while (j < jz) {
         inner_load(j + UI, B, j + UI < jz);
-        inner_compute(j, A, SRC, std::false_type{});
+        inner_compute(j, A, SRC);
         j += UI;
         if (j >= jz) break;
         inner_load(j + UI, A, j + UI < jz);
-        inner_compute(j, B, SRC, std::false_type{});
+        inner_compute(j, B, SRC);
         j += UI;
       }
     };

@@ -15,6 +15,11 @@ carries the frames' 2^-9 rounding undifferenced:
   noise, its correlation with the smooth forward field is weak, and the
   frames' rounding is a large part of it (1.3e-2 in the eps gradient, as
   with BF16 checkpoint snapshots on such a source); the bar is 3e-2;
+* narrow strips (16 and 24 columns) on a grid with PML rows, receiver rows
+  and a source, so that many strips start, end and change chunk kind: each
+  column of the BF16-frame gradient stays within 1e-2 of the FP32-frame one
+  (a frame pair written with a wrong half would put that column's error
+  near 1);
 * an odd number of steps keeps FP32 frames (the one-step kernels store FP32
   frames only), reported by gjw_query, and then equals the FP32-frame
   gradient bit for bit.
@@ -30,11 +35,11 @@ from test_gpu_scale import cosine, gpu_l2_gradient, observed, oracle_grad, rel_l
 pytestmark = pytest.mark.gpu
 
 
-def frames_gradient(case, ctx, dobs, frames, adj=None):
+def frames_gradient(case, ctx, dobs, frames, adj=None, **kw):
     """Exact gradient with the given frame type; the L2 residual of dobs or
     the adjoint source adj.  Returns (g_eps, g_sig, residual, frames used,
     rec_ptr)."""
-    sv = survey(case, ctx)
+    sv = survey(case, ctx, **kw)
     sv.set_option(N.GJW_OPT_FRAMES, frames)
     sv.forward(keep=True)
     if adj is None:
@@ -103,3 +108,17 @@ def test_bf16_frames_odd_nt_keeps_fp32(gpu_ctx):
     g32 = frames_gradient(case, gpu_ctx, None, N.GJW_FRAMES_F32, adj=adj)
     assert used == N.GJW_FRAMES_F32
     assert np.array_equal(ge, g32[0]) and np.array_equal(gs, g32[1])
+
+
+@pytest.mark.parametrize("L", [16, 24])
+def test_bf16_frames_narrow_strips_per_column(gpu_ctx, L):
+    case = configs.small(nt=400, ns=3)
+    dobs = observed(case, gpu_ctx)
+    ge, gs, e, used, _ = frames_gradient(case, gpu_ctx, dobs, N.GJW_FRAMES_BF16, L=L, fuse=8)
+    g32 = frames_gradient(case, gpu_ctx, dobs, N.GJW_FRAMES_F32, L=L, fuse=8)
+    assert used == N.GJW_FRAMES_BF16
+    for a, b in ((ge, g32[0]), (gs, g32[1])):
+        col = np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(b, axis=0),
+                                                          1e-30 + 1e-6 * np.linalg.norm(b))
+        print(f"L {L}: worst column rel {col.max():.2e}, whole {rel_l2(a, b):.2e}", flush=True)
+        assert col.max() <= 1e-2 and rel_l2(a, b) <= 1e-3
