@@ -96,6 +96,9 @@ def main():
     ap.add_argument("--frames", choices=["fp32", "bf16"], default="bf16",
                     help="element type of the recompute sweep's Ez frames (GJW_OPT_FRAMES); "
                          "with bf16 a gradient with FP32 frames is timed beside (frames_fp32)")
+    ap.add_argument("--row-pairs", type=int, default=1, choices=[0, 1],
+                    help="1: the 8-step forward's interior row tiles hold two rows per lane "
+                         "(GJW_OPT_ROW_PAIRS); 0: one row per lane")
     ap.add_argument("--graph", type=int, default=1, help="1: replay the gradient as captured HIP graphs")
     ap.add_argument("--no-sqrt", action="store_true",
                     help="skip the extra gradient timed at a checkpoint interval of about sqrt(nt)")
@@ -150,6 +153,7 @@ def main():
         sv.set_option(N.GJW_OPT_CKPT_EVERY, args.ckpt)
     if args.frames == "bf16":
         sv.set_option(N.GJW_OPT_FRAMES, N.GJW_FRAMES_BF16)
+    sv.set_option(N.GJW_OPT_ROW_PAIRS, args.row_pairs)
     if args.graph:
         sv.set_option(N.GJW_OPT_GRAPH, 1)
     # observed data: true model, kept in HBM
@@ -334,6 +338,7 @@ def main():
                                       "checkpoint_sqrt_nt)" if not args.ckpt else "set by --ckpt",
                    "steps_per_launch": {"forward": depths[0], "adjoint": depths[1]},
                    "frames": frames_used,
+                   "row_pairs": bool(args.row_pairs),
                    "frames_note": "element type of the Ez frames the recompute sweep hands to the "
                                   "adjoint (GJW_OPT_FRAMES); the BF16-frame gradient of one shot of "
                                   "this workload against the FP64 oracle is checked to <= 1e-3 in "

@@ -23,6 +23,7 @@ GJW_OPT_SHOT_GRADS = 10
 GJW_OPT_ADJ_TILE = 11
 GJW_OPT_FRAMES = 12
 GJW_FRAMES_F32, GJW_FRAMES_BF16 = 0, 1
+GJW_OPT_ROW_PAIRS = 13
 GJW_LOW_PASS, GJW_HI_PASS = 0, 1
 GJW_SNAP_F32, GJW_SNAP_BF16, GJW_SNAP_F16 = 0, 1, 2
 GJW_MISFIT_L2 = 0
@@ -31,6 +32,7 @@ MISFIT_KINDS = {"L2": 0, "FREQstp": 0, "env_L2": 1, "phase_L2": 2, "env_L22": 3,
 GJW_MISFIT_FREQSTP = 5  # line-search objectives: L2 after an extra band filter
 GJW_GRAD_EXACT, GJW_GRAD_REF = 0, 1
 GJW_Q_CKPT_INTERVAL, GJW_Q_FWD_DEPTH, GJW_Q_ADJ_DEPTH, GJW_Q_FRAMES = 1, 2, 3, 4
+GJW_Q_ROW_PAIRS = 5
 
 # every symbol include/gjw.h declares
 EXPORTS = ["gjw_create", "gjw_destroy", "gjw_last_error", "gjw_version", "gjw_survey_create",

@@ -64,16 +64,16 @@ __global__ void k_fwd_step2(GJW_FWDN_ARGS);
 template <int FR>
 __global__ void k_fwd_step4(GJW_FWDN_ARGS);
 template <int FR>
-__global__ void k_fwd_step8(GJW_FWDN_ARGS);
+__global__ void k_fwd_step8(GJW_FWDN_ARGS, RowPairs);
 extern template __global__ void k_fwd_step2<0>(GJW_FWDN_ARGS);
 extern template __global__ void k_fwd_step2<1>(GJW_FWDN_ARGS);
 extern template __global__ void k_fwd_step2<2>(GJW_FWDN_ARGS);
 extern template __global__ void k_fwd_step4<0>(GJW_FWDN_ARGS);
 extern template __global__ void k_fwd_step4<1>(GJW_FWDN_ARGS);
 extern template __global__ void k_fwd_step4<2>(GJW_FWDN_ARGS);
-extern template __global__ void k_fwd_step8<0>(GJW_FWDN_ARGS);
-extern template __global__ void k_fwd_step8<1>(GJW_FWDN_ARGS);
-extern template __global__ void k_fwd_step8<2>(GJW_FWDN_ARGS);
+extern template __global__ void k_fwd_step8<0>(GJW_FWDN_ARGS, RowPairs);
+extern template __global__ void k_fwd_step8<1>(GJW_FWDN_ARGS, RowPairs);
+extern template __global__ void k_fwd_step8<2>(GJW_FWDN_ARGS, RowPairs);
 #undef GJW_FWDN_ARGS
 __global__ void k_fwd_step_multi(const MultiTable, MultiArgs);
 template <int NS, bool FRAMES>
@@ -241,6 +241,8 @@ struct gjw_survey {
   int frames16 = 0;  // GJW_OPT_FRAMES: BF16 recompute frames asked for
   int fr16 = 0;      // the gradient in progress keeps its frames in BF16 (frames_bf16)
   int fr16_used = 0; // ... and the last one did (GJW_Q_FRAMES)
+  int row_pairs = 1; // GJW_OPT_ROW_PAIRS: two-row interior tiles in the 8-step forward
+  int pairs_last = 0; // two-row tiles of the last 8-step forward launch (GJW_Q_ROW_PAIRS)
   // adjoint-source band filter of gjw_fwi_gradient (w_update_e.m:69-70); off
   // while flt_hi <= flt_lo
   double flt_lo = 0, flt_hi = 0;
@@ -493,6 +495,12 @@ extern "C" int gjw_set_option(gjw_survey* s, int32_t key, int64_t v) {
       if (v != GJW_FRAMES_F32 && v != GJW_FRAMES_BF16)
         return fail(GJW_ERR_ARG, "GJW_OPT_FRAMES: GJW_FRAMES_F32 or GJW_FRAMES_BF16");
       s->frames16 = (int)v;
+      break;
+    case GJW_OPT_ROW_PAIRS:
+      if (v != 0 && v != 1) return fail(GJW_ERR_ARG, "GJW_OPT_ROW_PAIRS must be 0 or 1");
+      s->row_pairs = (int)v;
+      s->tuned_L8 = 0;  // the 8-step forward's width is tuned with its tiling
+      s->graph_calls = 0;
       break;
     case GJW_OPT_SHOT_GRADS:
       if (v != 0 && v != 1) return fail(GJW_ERR_ARG, "GJW_OPT_SHOT_GRADS must be 0 or 1");
@@ -995,6 +1003,53 @@ static inline dim3 stepN_grid(const gjw_survey* s, int NS, int L, int nshots, in
                          (4 / wpb)));
 }
 
+// Two-row interior tiles of the 8-step forward (fwd8_pairs) for strips of L
+// columns.  One-row tiles stay where a wave's rows reach a PML row, a
+// receiver or a source (the top and bottom tiles of row_tile's layout); the
+// rows between go to waves of two rows per lane, in the strips whose march
+// runs through interior columns only.  n = 0 (every tile one row per lane)
+// when no such rows or strips exist.
+static RowPairs row_pairs_plan(const gjw_survey* s, int L) {
+  RowPairs rp{};
+  if (!s->row_pairs) return rp;
+  constexpr int NS = 8, R = 64 - 2 * NS, R2 = 128 - 2 * NS;
+  const int NX = s->NX, NY = s->NY, P = s->P;
+  const int T = (int)cdiv(NX + 1, R);
+  if (T < 3) return rp;
+  auto first = [&](int t) { return t == T - 1 ? std::max(NX + 1 - R, 0) : t * R; };
+  int smax = -1;
+  for (int q = 0; q < s->ns; ++q) smax = std::max(smax, s->si[q]);
+  // top: the first tile whose wave rows (from its first owned row - NS) are
+  // inner rows below every receiver and source row
+  int t0 = 1;
+  while (t0 < T && !(first(t0) - NS >= P && first(t0) > s->rrow_max && first(t0) - NS > smax))
+    ++t0;
+  // bottom: the one-row tiles tb .. T-1 hold the rows from where a pair
+  // wave's needed rows (up to its last owned row + NS - 1) would reach the PML
+  int tb = T - 1;
+  while (tb > t0 && first(tb) - 1 + NS - 1 > NX - P - 1) --tb;
+  if (tb <= t0) return rp;
+  const int lo = first(t0), hi = first(tb) - 1;
+  const int n = (int)cdiv(hi - lo + 1, R2);
+  // frames are stored for rows of the stored domain only
+  if (hi < lo || t0 + n > tb || lo < s->di0 || hi > s->di0 + s->gr.m - 1) return rp;
+  // strips: level 0 marches columns jb-NS+1 .. je+NS-2 and level NS-1 starts
+  // NS-1 columns behind; all of them interior
+  int slo = 0;
+  while (slo * L - 2 * (NS - 1) < P + 1) ++slo;
+  int shi = (int)cdiv(NY + 1, L) - 1;
+  while (shi >= slo && std::min(shi * L + L, NY + 1) + NS - 2 > NY - P - 1) --shi;
+  if (shi < slo) return rp;
+  rp.t0 = t0;
+  rp.n = n;
+  rp.tb = tb;
+  rp.lo = lo;
+  rp.hi = hi;
+  rp.slo = slo;
+  rp.shi = shi;
+  return rp;
+}
+
 // NS = 2 or 4 fused forward steps; fr1 (optional) receives the NS domain
 // frames Ez(S_{it+1}) .. Ez(S_{it+NS}) at fr1 + l * LD * n
 static int fwdN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& sh, int nsh, int L,
@@ -1010,14 +1065,21 @@ static int fwdN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& 
   hipLaunchKernelGGL(K, grid, block, 0, st, g, sh, L, b.E[cur], b.Hx[cur], b.Hy[cur], b.X[xc], \
                      b.Y[xc], b.E[cur ^ 1], b.Hx[cur ^ 1], b.Hy[cur ^ 1], b.X[xc ^ 1],        \
                      b.Y[xc ^ 1], it, tr, fr1, fstride)
+#define GJW_FWD8_CALL(K, RP)                                                                 \
+  hipLaunchKernelGGL(K, grid, block, 0, st, g, sh, L, b.E[cur], b.Hx[cur], b.Hy[cur], b.X[xc], \
+                     b.Y[xc], b.E[cur ^ 1], b.Hx[cur ^ 1], b.Hy[cur ^ 1], b.X[xc ^ 1],        \
+                     b.Y[xc ^ 1], it, tr, fr1, fstride, RP)
   const int FR = fr1 ? (s->fr16 ? 2 : 1) : 0;
   if (NS == 8) {
+    // FP32 frames keep one row per lane everywhere
+    const RowPairs rp = FR == 1 ? RowPairs{} : row_pairs_plan(s, L);
+    s->pairs_last = rp.n;
     if (FR == 2)
-      GJW_FWDN_CALL(k_fwd_step8<2>);
+      GJW_FWD8_CALL(k_fwd_step8<2>, rp);
     else if (FR == 1)
-      GJW_FWDN_CALL(k_fwd_step8<1>);
+      GJW_FWD8_CALL(k_fwd_step8<1>, rp);
     else
-      GJW_FWDN_CALL(k_fwd_step8<0>);
+      GJW_FWD8_CALL(k_fwd_step8<0>, rp);
   } else if (NS == 4) {
     if (FR == 2)
       GJW_FWDN_CALL(k_fwd_step4<2>);
@@ -1034,6 +1096,7 @@ static int fwdN_launch(gjw_survey* s, int NS, const DevGrid& g, const DevShots& 
       GJW_FWDN_CALL(k_fwd_step2<0>);
   }
 #undef GJW_FWDN_CALL
+#undef GJW_FWD8_CALL
   HIPCHK(hipGetLastError());
   return kev_end(s, slot);
 }
@@ -2253,7 +2316,7 @@ static std::vector<uintptr_t> fwi_graph_key(const gjw_survey* s) {
           u(s->acc.p),    u(s->adjf.p),  u(s->coef.p),   u(s->fe.p),    u(s->fs.p),
           u(s->d_pml.p),  u(s->d_si.p),  u(s->d_sj.p),   u(s->d_sc.p),  u(s->d_wav.p),
           u(s->d_cptr.p), u(s->d_crow.p), u(s->d_cid.p), u(s->d_trs.p), u(s->d_trr.p),
-          u(s->d_trc.p),  (uintptr_t)s->frames16, (uintptr_t)s->ns, (uintptr_t)s->ntr, (uintptr_t)s->shot_grads,
+          u(s->d_trc.p),  (uintptr_t)s->frames16, (uintptr_t)s->row_pairs, (uintptr_t)s->ns, (uintptr_t)s->ntr, (uintptr_t)s->shot_grads,
           (uintptr_t)s->gr.nt, (uintptr_t)s->tuned_L, (uintptr_t)s->tuned_L1,
           (uintptr_t)s->tuned_L4, (uintptr_t)s->tuned_L8, (uintptr_t)s->fuse_eff,
           (uintptr_t)s->tuned_La2,
@@ -3026,6 +3089,9 @@ extern "C" int gjw_query(gjw_survey* s, int32_t what, int64_t* value) {
       return GJW_OK;
     case GJW_Q_FRAMES:
       *value = s->fr16_used ? GJW_FRAMES_BF16 : GJW_FRAMES_F32;
+      return GJW_OK;
+    case GJW_Q_ROW_PAIRS:
+      *value = s->pairs_last;
       return GJW_OK;
     default:
       return fail(GJW_ERR_ARG, "gjw_query: unknown query %d", what);

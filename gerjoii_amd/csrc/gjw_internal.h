@@ -41,6 +41,17 @@ struct DevGrid {
   int di0, dj0, m, n;
 };
 
+// Row tiles of the 8-step forward that march two rows per lane (k_fwd_step8,
+// fwd8_pairs).  In strips slo..shi, row tiles t0 .. t0+n-1 are such tiles:
+// tile t0+q owns rows lo + 112 q .. min(lo + 112 q + 111, hi); row tiles
+// t0+n .. tb-1 hold nothing; the others are the one-row tiles of the same
+// index.  n = 0: every tile holds one row per lane.
+struct RowPairs {
+  int t0, n, tb;
+  int lo, hi;
+  int slo, shi;
+};
+
 // Per-shot geometry of one survey.
 struct DevShots {
   int nt;

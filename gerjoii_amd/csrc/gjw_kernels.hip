@@ -1140,6 +1140,168 @@ k_fwd_step4(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
                        Hy2, X2, Y2, it, tr, frame1, fstride);
 }
 
+// Interior row tiles of the 8-step forward with two rows per lane (FR = 0,
+// or 2: BF16 frame pairs).  Lane k holds rows r0 + 2k (a) and r0 + 2k + 1
+// (b): one 64-bit load or store moves a field's two rows, and one DPP shift
+// per level serves both rows' vertical neighbours (row a's Hy above is lane
+// k-1's row b, row b's Ez below is lane k+1's row a).  The wave covers 128
+// rows with the same 2 x 8 rows of trapezoid halo as a one-row wave's 64, so
+// it owns 112 rows instead of 48.  The host gives these tiles only inner
+// rows of the stored domain, no receiver or source row within their 128 rows,
+// and strips whose march runs through interior columns only (RowPairs); the
+// update is the interior chunk's of fwdN_body, row by row.  ODD: the tile
+// owns an odd number of rows, so one lane owns row a only and stores it
+// alone.
+template <int FR, bool ODD>
+__device__ __forceinline__ void fwd8_pairs(const DevGrid& g, int shot, int strip, int q,
+                                           const RowPairs& rp, int L,
+                                           const float* __restrict__ E0,
+                                           const float* __restrict__ Hx0,
+                                           const float* __restrict__ Hy0, float* __restrict__ E2,
+                                           float* __restrict__ Hx2, float* __restrict__ Hy2,
+                                           float* __restrict__ frame1, long long fstride) {
+  constexpr int NS = 8, R2 = 128 - 2 * NS;
+  const int lane = threadIdx.x;
+  const long long LD = g.LD;
+  const int own_lo = __builtin_amdgcn_readfirstlane(rp.lo + q * R2);
+  const int own_hi = __builtin_amdgcn_readfirstlane(min(rp.lo + q * R2 + R2 - 1, rp.hi));
+  const int jb = __builtin_amdgcn_readfirstlane(strip * L);
+  const int je = min(jb + L, g.NY + 1);
+  const int ia = own_lo - NS + 2 * lane;  // row a; row b = ia + 1
+  const bool own_a = ia >= own_lo && ia <= own_hi;
+  const bool own_b = ia + 1 <= own_hi && ia + 1 >= own_lo;
+  const bool own2 = own_a && own_b, own1 = ODD && own_a && !own_b;
+  const long long so = (long long)shot * g.FS;
+  const __amdgpu_buffer_rsrc_t rE0 = rsrc_of(const_cast<float*>(E0 + so), g.FS),
+                               rHx0 = rsrc_of(const_cast<float*>(Hx0 + so), g.FS),
+                               rHy0 = rsrc_of(const_cast<float*>(Hy0 + so), g.FS),
+                               rE2 = rsrc_of(E2 + so, g.FS), rHx2 = rsrc_of(Hx2 + so, g.FS),
+                               rHy2 = rsrc_of(Hy2 + so, g.FS),
+                               rK = rsrc_of(const_cast<float*>(g.coef), 19 * g.FS);
+  const CoefOff ko = coef_off(g);
+  const float bx0 = g.bx0, by0 = g.by0;
+  const long long FD = (long long)LD * g.n;
+  // frames as in fwdN_body (FR = 2): the descriptor starts NS * LD words
+  // early, plane l / 2 and the column lag of level l ride in the scalar offset
+  const __amdgpu_buffer_rsrc_t rFR =
+      rsrc_of(FR == 2 ? frame1 + (long long)shot * fstride - NS * LD : nullptr,
+              FR == 2 ? NS / 2 * FD + NS * LD : 0);
+  auto fsoff = [&](int l) { return (int)((unsigned)(l / 2 * FD + (NS - l) * LD) * 4u); };
+  auto level_mask = [&](int c) {
+    const int lo = max(0, c - je + 1), hi = min(NS - 1, c - jb);
+    return hi >= lo ? (2u << hi) - (1u << lo) : 0u;
+  };
+
+  // state of fwdN_body's interior chunks, for rows (a, b): level l >= 1's
+  // inputs of its previous column, hxp[l] its Hx of that column, cc_l[l] the
+  // (Ceze, Cezhy) of rows a and b of its column; fh: even levels' values held
+  // for the frame pairs
+  const int j1 = jb - (NS - 1);
+  float2 e0_c, e_in[NS], hx_in[NS], hy_in[NS], hxp[NS], fh[NS / 2];
+  float4 cc_l[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    e_in[l] = hx_in[l] = hy_in[l] = hxp[l] = make_float2(0.f, 0.f);
+    cc_l[l] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+#pragma unroll
+  for (int m = 0; m < NS / 2; ++m) fh[m] = make_float2(0.f, 0.f);
+  {
+    const unsigned ka = (unsigned)(j1 * LD + ia), kb = ka - (unsigned)LD;
+    e0_c = bload2(rE0, ka, true);
+    const float2 eb = bload2(rE0, kb, true), hxb = bload2(rHx0, kb, true);
+    // (cyh, AX, BX, AY) of column j1-1
+    const float4 ua = bload4k(rK, ko.u1, 4 * kb, true), ub = bload4k(rK, ko.u1, 4 * (kb + 1), true);
+    hxp[0].x = ua.y * hxb.x + ua.z * (e0_c.x - eb.x);
+    hxp[0].y = ub.y * hxb.y + ub.z * (e0_c.y - eb.y);
+  }
+  struct Buf {
+    float2 en, hx, hy;
+    float4 c;  // (Ceze, Cezhy) of row a, then of row b
+  };
+  auto load = [&](int c, Buf& b, bool on) {
+    const unsigned kk = (unsigned)(c * LD + ia);
+    b.en = bload2(rE0, kk + (unsigned)LD, on);
+    b.hx = bload2(rHx0, kk, on);
+    b.hy = bload2(rHy0, kk, on);
+    b.c = bload4k(rK, ko.cc, 2 * kk, on);
+  };
+  auto compute = [&](int c, const Buf& b) {
+    const unsigned lv = FR ? level_mask(c) : 0u;
+    const unsigned v0 = (unsigned)((c - g.dj0) * LD + ia);
+    float2 eo[NS], hxo[NS], hyo[NS];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const int cl = c - l;
+      const float2 ec = l == 0 ? e0_c : e_in[l], en = l == 0 ? b.en : eo[l - 1];
+      const float2 hx = l == 0 ? b.hx : hx_in[l], hy = l == 0 ? b.hy : hy_in[l];
+      const float4 cf = l == 0 ? b.c : cc_l[l];
+      const float e_dn = shfl_dn1(ec.x);  // Ez of row b + 1 (lane k+1's row a)
+      float2 hy1, hx1, e1;
+      hy1.x = hy.x + by0 * (ec.y - ec.x);
+      hy1.y = hy.y + by0 * (e_dn - ec.y);
+      hx1.x = hx.x + bx0 * (en.x - ec.x);
+      hx1.y = hx.y + bx0 * (en.y - ec.y);
+      const float hyu = shfl_up1(hy1.y);  // Hy of row a - 1 (lane k-1's row b)
+      e1.x = cf.x * ec.x + cf.y * ((hy1.x - hyu) - (hx1.x - hxp[l].x));
+      e1.y = cf.z * ec.y + cf.w * ((hy1.y - hy1.x) - (hx1.y - hxp[l].y));
+      eo[l] = e1;
+      hxo[l] = hx1;
+      hyo[l] = hy1;
+      if (FR == 2 && l % 2 == 1) {
+        // the pair (level l-1, level l) of column c - l: level l-1 reached
+        // it one column ago
+        const bool on = (lv >> l) & 1u;
+        u32x2 w;
+        w.x = frame_pair_bits(fh[l / 2].x, e1.x);
+        w.y = frame_pair_bits(fh[l / 2].y, e1.y);
+        __builtin_amdgcn_raw_buffer_store_b64(w, rFR, (int)(own2 && on ? v0 * 4u : kOOB),
+                                              fsoff(l), 0);
+        if (ODD)
+          __builtin_amdgcn_raw_buffer_store_b32(w.x, rFR, (int)(own1 && on ? v0 * 4u : kOOB),
+                                                fsoff(l), 0);
+        fh[l / 2] = eo[l - 1];
+      }
+      if (l == NS - 1) {
+        const bool ol = cl >= jb && cl < je;
+        const unsigned k2 = (unsigned)(cl * LD + ia);
+        bstore2(e1, rE2, k2, own2 && ol);
+        bstore2(hx1, rHx2, k2, own2 && ol);
+        bstore2(hy1, rHy2, k2, own2 && ol);
+        if (ODD) {
+          bstore(e1.x, rE2, k2, own1 && ol);
+          bstore(hx1.x, rHx2, k2, own1 && ol);
+          bstore(hy1.x, rHy2, k2, own1 && ol);
+        }
+      }
+    }
+    e0_c = b.en;
+#pragma unroll
+    for (int l = NS - 1; l >= 1; --l) {
+      e_in[l] = eo[l - 1];
+      hx_in[l] = hxo[l - 1];
+      hy_in[l] = hyo[l - 1];
+      cc_l[l] = l == 1 ? b.c : cc_l[l - 1];
+    }
+#pragma unroll
+    for (int l = 0; l < NS; ++l) hxp[l] = hxo[l];
+  };
+  // level 0 at columns j1 .. je+NS-2, one column per chunk, the next
+  // column's loads in flight while this one is computed
+  const int jlast = je + NS - 2;
+  int j = j1;
+  Buf A, B;
+  load(j, A, true);
+  while (j <= jlast) {
+    load(j + 1, B, j + 1 <= jlast);
+    compute(j, A);
+    if (++j > jlast) break;
+    load(j + 1, A, j + 1 <= jlast);
+    compute(j, B);
+    ++j;
+  }
+}
+
 template <int FR>
 __global__ void __launch_bounds__(256, GJW_FWD8_WAVES)
 k_fwd_step8(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
@@ -1147,9 +1309,26 @@ k_fwd_step8(DevGrid g, DevShots s, int L, const float* __restrict__ E0,
             const float* __restrict__ X0, const float* __restrict__ Y0, float* __restrict__ E2,
             float* __restrict__ Hx2, float* __restrict__ Hy2, float* __restrict__ X2,
             float* __restrict__ Y2, int it, float* __restrict__ tr, float* __restrict__ frame1,
-            long long fstride) {
-  fwdN_body<8, FR>(g, s, L, vtile_of_block(g.NX, g.NY, L, 48, true), E0, Hx0, Hy0, X0, Y0, E2, Hx2,
-                       Hy2, X2, Y2, it, tr, frame1, fstride);
+            long long fstride, RowPairs rp) {
+  const TileId tid = vtile_of_block(g.NX, g.NY, L, 48, true);
+  if (FR != 1 && rp.n > 0 && tid.strip >= rp.slo && tid.strip <= rp.shi) {
+    const int t = tid.rowtile * 4 + tid.wy;
+    if (t >= rp.t0 && t < rp.tb) {
+      const int q = t - rp.t0;
+      if (q < rp.n) {
+        const int lo = rp.lo + q * 112, hi = min(lo + 111, rp.hi);
+        if ((hi - lo + 1) & 1)
+          fwd8_pairs<FR, true>(g, tid.shot, tid.strip, q, rp, L, E0, Hx0, Hy0, E2, Hx2, Hy2,
+                               frame1, fstride);
+        else
+          fwd8_pairs<FR, false>(g, tid.shot, tid.strip, q, rp, L, E0, Hx0, Hy0, E2, Hx2, Hy2,
+                                frame1, fstride);
+      }
+      return;
+    }
+  }
+  fwdN_body<8, FR>(g, s, L, tid, E0, Hx0, Hy0, X0, Y0, E2, Hx2, Hy2, X2, Y2, it, tr, frame1,
+                   fstride);
 }
 
 template <int NS, bool FRAMES>
@@ -1192,9 +1371,9 @@ template __global__ void k_fwd_step2<2>(GJW_FWDN_ARGS);
 template __global__ void k_fwd_step4<0>(GJW_FWDN_ARGS);
 template __global__ void k_fwd_step4<1>(GJW_FWDN_ARGS);
 template __global__ void k_fwd_step4<2>(GJW_FWDN_ARGS);
-template __global__ void k_fwd_step8<0>(GJW_FWDN_ARGS);
-template __global__ void k_fwd_step8<1>(GJW_FWDN_ARGS);
-template __global__ void k_fwd_step8<2>(GJW_FWDN_ARGS);
+template __global__ void k_fwd_step8<0>(GJW_FWDN_ARGS, RowPairs);
+template __global__ void k_fwd_step8<1>(GJW_FWDN_ARGS, RowPairs);
+template __global__ void k_fwd_step8<2>(GJW_FWDN_ARGS, RowPairs);
 #undef GJW_FWDN_ARGS
 template __global__ void k_fwd_stepN_multi<2, false>(const MultiTable, MultiArgs);
 template __global__ void k_fwd_stepN_multi<2, true>(const MultiTable, MultiArgs);

@@ -121,6 +121,11 @@ int gjw_set_shots(gjw_survey *s, int32_t ns, const int32_t *src, const float *wa
                                 keep FP32 frames (gjw_query GJW_Q_FRAMES tells which was used) */
 #define GJW_FRAMES_F32 0
 #define GJW_FRAMES_BF16 1
+#define GJW_OPT_ROW_PAIRS 13 /* 1 (default): the 8-step forward of the batched path marches the
+                                rows between the PML, receiver and source rows with two rows
+                                per wavefront lane (112 owned rows per wave instead of 48), in
+                                strips of interior columns, except with FP32 frames; 0: one
+                                row per lane everywhere.  The update is the same, row by row */
 #define GJW_PATH_BATCHED 0   /* all shots per launch, checkpoint + recompute        */
 #define GJW_PATH_PLAIN 1     /* one shot at a time, full wavefield stored (as w_solve.m) */
 #define GJW_SNAP_F32 0
@@ -346,6 +351,8 @@ int gjw_tile(gjw_survey *s, int32_t which, int32_t *L);
 #define GJW_Q_FWD_DEPTH 2
 #define GJW_Q_ADJ_DEPTH 3
 #define GJW_Q_FRAMES 4 /* GJW_FRAMES_* of the last batched gradient's frames */
+#define GJW_Q_ROW_PAIRS 5 /* two-row tiles per strip of the last 8-step forward launch
+                             (GJW_OPT_ROW_PAIRS; 0: none) */
 int gjw_query(gjw_survey *s, int32_t what, int64_t *value);
 
 #ifdef __cplusplus

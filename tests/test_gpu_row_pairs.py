@@ -1,0 +1,71 @@
+"""Two-row interior tiles of the 8-step forward (GJW_OPT_ROW_PAIRS) against
+the one-row tiles, through the C ABI.
+
+The two-row waves run the interior update of the one-row waves row by row, so
+the wavefields, traces and gradients of the two tilings agree to rounding
+(the same operations in the same order give the same bits unless the
+compiler contracts them differently).  The survey is 300 x 300 with a 20-cell
+PML and 32- or 33-column strips, so that the plan holds three two-row tiles
+per strip (the last one owns an odd number of rows) in strips 2 .. 8, next
+to the one-row PML, receiver and strip-edge tiles.  A wrong row half, lane,
+column lag or frame plane in the two-row path would put the error of that
+row or column near 1; the bars are 1e-5 per trace, per gradient column and
+per gradient row.  The benchmark workload's FP64-oracle checks
+(test_gpu_frames16.py, test_gpu_scale.py) run with two-row tiles on, as the
+library's default.
+"""
+import numpy as np
+import pytest
+
+from gerjoii_amd import configs
+from gerjoii_amd import _native as N
+
+from test_gpu_scale import observed, rel_l2, survey
+
+pytestmark = pytest.mark.gpu
+
+
+def _case():
+    return configs.small(m=300, n=300, P=20, nt=400, ns=3, nr=24)
+
+
+def _worst(a, b, axis):
+    num = np.linalg.norm(a - b, axis=axis)
+    den = np.maximum(np.linalg.norm(b, axis=axis), 1e-6 * np.linalg.norm(b) + 1e-30)
+    return float((num / den).max())
+
+
+def _run(case, ctx, dobs, pairs, L, frames):
+    sv = survey(case, ctx, L=L, fuse=8)
+    sv.set_option(N.GJW_OPT_ROW_PAIRS, pairs)
+    sv.set_option(N.GJW_OPT_FRAMES, frames)
+    d = sv.forward(keep=True)
+    n_pairs = sv.query(N.GJW_Q_ROW_PAIRS)
+    sv.misfit(dobs)
+    ge, gs = sv.gradient()
+    used = sv.query(N.GJW_Q_FRAMES)
+    sv.close()
+    return np.asarray(d), ge, gs, n_pairs, used
+
+
+@pytest.mark.parametrize("L", [32, 33])
+@pytest.mark.parametrize("frames", ["bf16", "fp32"])
+def test_row_pairs_match_one_row_tiles(gpu_ctx, L, frames):
+    case = _case()
+    dobs = observed(case, gpu_ctx)
+    fr = N.GJW_FRAMES_BF16 if frames == "bf16" else N.GJW_FRAMES_F32
+    d1, ge1, gs1, n1, u1 = _run(case, gpu_ctx, dobs, 0, L, fr)
+    d2, ge2, gs2, n2, u2 = _run(case, gpu_ctx, dobs, 1, L, fr)
+    assert n1 == 0
+    # the FP32-frame recompute keeps one-row tiles; the forward sweep's
+    # eight-step launches (the last one queried) use the two-row tiles
+    assert n2 == 3
+    assert u1 == u2 == fr
+    same = (np.array_equal(d1, d2), np.array_equal(ge1, ge2), np.array_equal(gs1, gs2))
+    print(f"L {L} {frames}: bit-equal traces/eps/sigma {same}; traces {rel_l2(d2, d1):.2e}, "
+          f"eps {rel_l2(ge2, ge1):.2e} (worst column {_worst(ge2, ge1, 0):.2e}, "
+          f"row {_worst(ge2, ge1, 1):.2e}), sigma {rel_l2(gs2, gs1):.2e}", flush=True)
+    assert _worst(d2, d1, 1) <= 1e-5  # per trace
+    for a, b in ((ge2, ge1), (gs2, gs1)):
+        assert rel_l2(a, b) <= 1e-5
+        assert _worst(a, b, 0) <= 1e-5 and _worst(a, b, 1) <= 1e-5
