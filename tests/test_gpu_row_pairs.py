@@ -14,11 +14,14 @@ per gradient row.  The benchmark workload's FP64-oracle checks
 (test_gpu_frames16.py, test_gpu_scale.py) run with two-row tiles on, as the
 library's default.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
 from gerjoii_amd import configs
 from gerjoii_amd import _native as N
+from gerjoii_amd.survey import rec_Mw
 
 from test_gpu_scale import observed, rel_l2, survey
 
@@ -66,6 +69,30 @@ def test_row_pairs_match_one_row_tiles(gpu_ctx, L, frames):
           f"eps {rel_l2(ge2, ge1):.2e} (worst column {_worst(ge2, ge1, 0):.2e}, "
           f"row {_worst(ge2, ge1, 1):.2e}), sigma {rel_l2(gs2, gs1):.2e}", flush=True)
     assert _worst(d2, d1, 1) <= 1e-5  # per trace
+    for a, b in ((ge2, ge1), (gs2, gs1)):
+        assert rel_l2(a, b) <= 1e-5
+        assert _worst(a, b, 0) <= 1e-5 and _worst(a, b, 1) <= 1e-5
+
+
+@pytest.mark.parametrize("rows, expect", [((60, 150), 1), ((100, 290), 0)])
+def test_row_pairs_with_receivers_at_depth(gpu_ctx, rows, expect):
+    """A borehole line of receivers (model rows 60..150, or 100..290, at
+    column 150) beside the surface ones.  Two-row tiles hold no receiver row,
+    so the plan starts them below the deepest receiver: one tile of 101 rows
+    (rows 192..292) for the first line, none for the second, whose receivers
+    reach the bottom tile.  Traces and gradients equal the one-row tiles'."""
+    case = _case()
+    g = case.grid
+    rz = np.arange(rows[0], rows[1] + 1, 6)
+    hole = rec_Mw(g, rz, np.full(rz.shape, 150))
+    case = dataclasses.replace(case, receivers=[np.concatenate([r, hole]) for r in case.receivers])
+    dobs = observed(case, gpu_ctx)
+    d1, ge1, gs1, n1, _ = _run(case, gpu_ctx, dobs, 0, 32, N.GJW_FRAMES_BF16)
+    d2, ge2, gs2, n2, _ = _run(case, gpu_ctx, dobs, 1, 32, N.GJW_FRAMES_BF16)
+    assert n1 == 0 and n2 == expect
+    print(f"receivers at model rows {rows}: {n2} two-row tiles; traces {rel_l2(d2, d1):.2e}, "
+          f"eps {rel_l2(ge2, ge1):.2e}, sigma {rel_l2(gs2, gs1):.2e}", flush=True)
+    assert _worst(d2, d1, 1) <= 1e-5
     for a, b in ((ge2, ge1), (gs2, gs1)):
         assert rel_l2(a, b) <= 1e-5
         assert _worst(a, b, 0) <= 1e-5 and _worst(a, b, 1) <= 1e-5
